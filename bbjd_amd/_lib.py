@@ -1,0 +1,116 @@
+"""ctypes binding of libbbjd.so -- the C ABI declared in include/bbjd.h,
+include/bbjd_gpu.h and include/bbjd_ops.h.
+
+The library is built in-tree (``python -c "import __graft_entry__ as g;
+g.build()"`` or ``make -C bbjd_amd/csrc``).  Importing this module never
+falls back to anything: when the shared object is missing, ``load()`` raises
+ImportError.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libbbjd.so")
+
+
+class zcomplex(C.Structure):
+    _fields_ = [("re", C.c_double), ("im", C.c_double)]
+
+
+# int f(bbjd_int n, const bbjd_zomplex *x, bbjd_zomplex *y, void *data)
+CALLBACK = C.CFUNCTYPE(C.c_int, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p)
+
+
+class Params(C.Structure):
+    """bbjd_params (field order of reference include/bbjd.h:114-150)."""
+    _fields_ = [("n", C.c_int64), ("nev", C.c_int64), ("sigma", zcomplex), ("target", C.c_int),
+                ("maxsub", C.c_int64), ("minsub", C.c_int64), ("tol", C.c_double),
+                ("maxiter", C.c_int64), ("inner_maxiter", C.c_int64), ("inner_tol", C.c_double),
+                ("v0", C.c_void_p), ("matvec_A", C.c_void_p), ("matvec_B", C.c_void_p),
+                ("precond", C.c_void_p), ("op_data", C.c_void_p), ("verbose", C.c_int)]
+
+
+class Result(C.Structure):
+    """bbjd_result (reference include/bbjd.h:155-162)."""
+    _fields_ = [("nconv", C.c_int64), ("eigenvalues", C.c_void_p), ("eigenvectors", C.c_void_p),
+                ("residuals", C.c_void_p), ("iters", C.c_int64), ("status", C.c_int)]
+
+
+class GpuOptions(C.Structure):
+    _fields_ = [("callback_space", C.c_int), ("precision", C.c_int), ("stream", C.c_void_p),
+                ("matvec_A_c64", C.c_void_p), ("matvec_B_c64", C.c_void_p),
+                ("precond_c64", C.c_void_p)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("outer_iters", C.c_int64), ("lock_iters", C.c_int64), ("restarts", C.c_int64),
+                ("inner_steps", C.c_int64), ("dgks_second_passes", C.c_int64),
+                ("sum_k", C.c_int64), ("qz_seconds", C.c_double), ("solve_seconds", C.c_double),
+                ("stream_bytes", C.c_double)]
+
+
+class PencilDesc(C.Structure):
+    _fields_ = [("n", C.c_int64), ("bs", C.c_int), ("hw", C.c_int), ("ntau", C.c_int),
+                ("band", C.c_void_p), ("tau", C.c_void_p), ("bdiag", C.c_void_p)]
+
+
+CALLBACK_HOST, CALLBACK_DEVICE = 0, 1
+PREC_FP64, PREC_MIXED = 0, 1
+PC_NONE, PC_EXACT_LU, PC_BLOCK_JACOBI = 0, 1, 2
+
+_lib = None
+
+
+def _sig(lib, name, res, *args):
+    f = getattr(lib, name)
+    f.restype = res
+    f.argtypes = list(args)
+
+
+def load():
+    """Load libbbjd.so (once).  Raises ImportError when it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"bbjd_amd: {LIB_PATH} is not built; run __graft_entry__.build() "
+                          "or make -C bbjd_amd/csrc")
+    lib = C.CDLL(LIB_PATH, mode=C.RTLD_GLOBAL)
+    P, V, I64, D = C.POINTER(Params), C.c_void_p, C.c_int64, C.c_double
+    _sig(lib, "bbjd_params_init", None, P)
+    _sig(lib, "bbjd_workspace_alloc", V, P)
+    _sig(lib, "bbjd_workspace_free", None, V)
+    _sig(lib, "bbjd_solve", C.c_int, P, V, C.POINTER(Result))
+    _sig(lib, "bbjd_result_free", None, C.POINTER(Result))
+    _sig(lib, "bbjd_mgs_orthogonalize", D, I64, I64, V, V)
+    _sig(lib, "bbjd_block_orthogonalize", D, I64, I64, V, V, V)
+    _sig(lib, "bbjd_correction_gmres", C.c_int, I64, V, V, zcomplex, zcomplex, V, V, V, V, V, V,
+         I64, D, V)
+    _sig(lib, "bbjd_gpu_options_init", None, C.POINTER(GpuOptions))
+    _sig(lib, "bbjd_workspace_bytes", C.c_size_t, P, C.POINTER(GpuOptions))
+    _sig(lib, "bbjd_workspace_alloc_gpu", V, P, C.POINTER(GpuOptions))
+    _sig(lib, "bbjd_callback_stream", V)
+    _sig(lib, "bbjd_workspace_stats", C.c_int, V, C.POINTER(Stats))
+    _sig(lib, "bbjd_memcpy_d2h", C.c_int, V, V, C.c_size_t)
+    _sig(lib, "bbjd_memcpy_h2d", C.c_int, V, V, C.c_size_t)
+    _sig(lib, "bbjd_pencil_generate", C.c_int, I64, C.c_int, C.c_int, C.c_int, C.c_uint64, D, V, V, V)
+    _sig(lib, "bbjd_op_create", V, C.POINTER(PencilDesc), zcomplex, C.c_int, C.c_int, V)
+    _sig(lib, "bbjd_op_free", None, V)
+    _sig(lib, "bbjd_op_bytes", C.c_size_t, V)
+    for nm in ("bbjd_op_matvec_A", "bbjd_op_matvec_B", "bbjd_op_precond",
+               "bbjd_op_matvec_A_c64", "bbjd_op_matvec_B_c64", "bbjd_op_precond_c64"):
+        _sig(lib, nm, C.c_int, I64, V, V, V)
+    _lib = lib
+    return lib
+
+
+def fn_addr(name: str) -> int:
+    """Address of an exported C function (for params.matvec_A & co.)."""
+    return C.cast(getattr(load(), name), C.c_void_p).value
+
+
+def zc(z: complex) -> zcomplex:
+    z = complex(z)
+    return zcomplex(z.real, z.imag)

@@ -1,0 +1,131 @@
+/*
+ * engine.h -- internal interface between the C host driver and the HIP
+ * kernel launchers (kernels.hip, ops.hip).  Plain C ABI: device arrays are
+ * passed as void* / typed pointers, complex128 has the layout of double[2],
+ * complex64 that of float[2].
+ */
+#ifndef BBJD_ENGINE_H
+#define BBJD_ENGINE_H
+
+#include <stdint.h>
+#include <stddef.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BBJD_TPB 256              /* threads per block of streaming kernels */
+#define BBJD_PROJ_ROWS 1024       /* rows per block of the projection kernel */
+#define BBJD_UPD_ROWS 1024        /* rows per block of the update kernel    */
+#define BBJD_MAXK 4096            /* largest column count of one projection */
+
+/* Scalar slots in the device scalar array (double). */
+enum {
+    SL_NB = 0,       /* norm before the first CGS pass                  */
+    SL_NA,           /* norm after the last CGS pass                    */
+    SL_U,            /* ||V x|| of the Ritz vector                      */
+    SL_R,            /* ||theta_b A u - theta_a B u||                   */
+    SL_V,            /* norm of the new search column                   */
+    SL_W,            /* norm of the new test column                     */
+    SL_Q,            /* norm of a deflation column                      */
+    SL_G,            /* GMRES vector norms                              */
+    SL_G2,
+    SL_T0, SL_T1,    /* scratch                                         */
+    SL_COUNT = 16
+};
+/* Int flag slots. */
+enum { FL_PASS2 = 0, FL_ONE = 1, FL_DGKS_COUNT = 2, FL_COUNT = 8 };
+
+/* Per-workspace scratch for the two-stage reductions. */
+typedef struct {
+    void   *part;        /* partial sums: tiles x ldp complex (x2 vectors) */
+    double *pnorm;       /* partial norms per tile                          */
+    void   *h;           /* coefficient vector, first pass                 */
+    void   *h2;          /* second pass                                    */
+    void   *hsum;        /* h + h2 (GMRES Hessenberg column)               */
+    double *scal;        /* SL_COUNT doubles                                */
+    int    *flag;        /* FL_COUNT ints                                   */
+    void   *coef;        /* uploaded small vectors                          */
+    void   *coef2;
+    int64_t part_cap;    /* complex entries available in part               */
+} bbjd_red;
+
+/* ---- complex128 streaming kernels (kernels.hip) ---------------------- */
+
+/* CGS2 with the DGKS test (2 passes when ||v_after|| < 0.7071 ||v_before||);
+ * device-side decision, no host sync.  Final ||v|| goes to scal[out_slot];
+ * if hsum is non-NULL the summed coefficients (k) are written there.
+ * k == 0: only the norm. */
+int bbjd_k_cgs2(hipStream_t s, int64_t n, int k, const void *Q, int64_t ldq, void *v,
+                bbjd_red *rd, int out_slot, void *hsum);
+
+/* v -= Q Q^H v twice (no norm test), k columns. */
+int bbjd_k_project_out(hipStream_t s, int64_t n, int k, const void *Q, int64_t ldq, void *v,
+                       bbjd_red *rd);
+
+/* out[j] = Q[:, j]^H x for j < k, and out2[j] = Q[:, j]^H x2 when x2 != NULL
+ * (one pass over Q).  out/out2 are device arrays. */
+int bbjd_k_gemv_c(hipStream_t s, int64_t n, int k, const void *Q, int64_t ldq,
+                  const void *x, const void *x2, void *out, void *out2, bbjd_red *rd);
+
+/* complex64 form of bbjd_k_gemv_c (FP32 accumulation) */
+int bbjd_k_gemv_c_c(hipStream_t s, int64_t n, int k, const void *Q, int64_t ldq,
+                    const void *x, const void *x2, void *out, void *out2, bbjd_red *rd);
+
+/* y = Q[:, :k] c (c device, k entries) and scal[slot] = ||y||. */
+int bbjd_k_gemv_n(hipStream_t s, int64_t n, int k, const void *Q, int64_t ldq,
+                  const void *c, void *y, bbjd_red *rd, int slot);
+
+/* ||x|| -> scal[slot] */
+int bbjd_k_norm(hipStream_t s, int64_t n, const void *x, bbjd_red *rd, int slot);
+
+/* mode 0: x *= 1/scal[slot]; mode 1: only when scal[slot] > 1e-300 */
+int bbjd_k_scale_inv(hipStream_t s, int64_t n, void *x, const double *scal, int slot, int mode);
+
+/* x *= a (host scalar a = (are, aim)) */
+int bbjd_k_scale(hipStream_t s, int64_t n, void *x, double are, double aim);
+
+/* out = a x + b y (host scalars); out may alias x or y */
+int bbjd_k_lincomb(hipStream_t s, int64_t n, double are, double aim, const void *x,
+                   double bre, double bim, const void *y, void *out);
+
+/* r = a x + b y and scal[slot] = ||r|| */
+int bbjd_k_lincomb_norm(hipStream_t s, int64_t n, double are, double aim, const void *x,
+                        double bre, double bim, const void *y, void *r, bbjd_red *rd, int slot);
+
+/* Sleijpen-van der Vorst step: z -= s (u^H z)/(u^H s) when |u^H s| > 1e-300 */
+int bbjd_k_svdv(hipStream_t s, int64_t n, const void *u, void *z, const void *sv, bbjd_red *rd);
+
+/* X[:, :keep] = X[:, :k] Zr where Zr is device, row-major k x keep
+ * (Zr[j keep + c] = Z[j, c]); in place, row-tiled. */
+int bbjd_k_rotate(hipStream_t s, int64_t n, int k, int keep, void *X, int64_t ldx,
+                  const void *Zr);
+
+/* G[i + j ldg] = X[:, i]^H Y[:, j], i, j < m (device G) */
+int bbjd_k_gram(hipStream_t s, int64_t n, int m, const void *X, const void *Y, int64_t ld,
+                void *G, int64_t ldg, bbjd_red *rd);
+
+/* precision conversion */
+int bbjd_k_z2c(hipStream_t s, int64_t n, const void *z, void *c);
+int bbjd_k_c2z(hipStream_t s, int64_t n, const void *c, void *z);
+
+/* ---- complex64 / FP32-accumulation kernels for the inner solve --------- */
+int bbjd_k_cgs2_c(hipStream_t s, int64_t n, int k, const void *Q, int64_t ldq, void *v,
+                  bbjd_red *rd, int out_slot, void *hsum);
+int bbjd_k_project_out_c(hipStream_t s, int64_t n, int k, const void *Q, int64_t ldq, void *v,
+                         bbjd_red *rd);
+int bbjd_k_gemv_n_c(hipStream_t s, int64_t n, int k, const void *Q, int64_t ldq,
+                    const void *c, void *y, bbjd_red *rd, int slot);
+int bbjd_k_norm_c(hipStream_t s, int64_t n, const void *x, bbjd_red *rd, int slot);
+int bbjd_k_scale_c(hipStream_t s, int64_t n, void *x, double are, double aim);
+int bbjd_k_lincomb_c(hipStream_t s, int64_t n, double are, double aim, const void *x,
+                     double bre, double bim, const void *y, void *out);
+
+/* bytes of reduction scratch for the given shape */
+int64_t bbjd_red_part_entries(int64_t n, int maxk);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

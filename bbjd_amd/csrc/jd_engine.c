@@ -1,0 +1,946 @@
+/*
+ * jd_engine.c -- JDQZ outer iteration on one MI355X.
+ *
+ * Host driver in C11 over the HIP kernels of kernels.hip.  It implements
+ * the reference C API (include/bbjd.h) with the algorithm of reference
+ * src/jd_core.c:743-1120 -- same steps, thresholds (DGKS 0.7071, breakdown
+ * 1e-14, guards 1e-300, lock matching |lambda - lambda_c| < 100 tol), same
+ * start vector (glibc srand(42)/rand() sequence, reproduced thread-safely
+ * with random_r) and same host LAPACK for the projected QZ -- but with
+ * V, W, AV, BV, Q_right, Q_left, the GMRES basis and the n-vectors in HBM.
+ *
+ * Per outer iteration the host/device traffic is: the k Ritz coefficients up,
+ * two norms down, the new pencil row/column down (4k+2 complex), i.e. the
+ * host never touches an n-sized array unless callbacks live on the host.
+ */
+#include "../../include/bbjd.h"
+#include "../../include/bbjd_gpu.h"
+#include "engine.h"
+#include "lapack_host.h"
+#include "qz_host.h"
+
+#include <complex.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+typedef double complex zcx;
+typedef struct { float re, im; } c64;
+
+struct bbjd_dev {
+    hipStream_t      stream;
+    int              own_stream;
+    bbjd_gpu_options opt;
+    int              ims;
+    void            *dblock;           /* the single device allocation     */
+    size_t           dbytes;
+    bbjd_red         rd;
+    void            *zr;               /* restart rotation, maxsub^2       */
+    void            *pencil;           /* 4 (maxsub + 1) complex            */
+    void            *gram;             /* 2 maxsub^2 complex                */
+    /* mixed precision */
+    void            *gvc;              /* complex64 basis ld x (ims + 1)   */
+    void            *qrc, *qlc;        /* complex64 projectors             */
+    void            *tc[3];            /* complex64 n-vectors               */
+    /* host side */
+    double          *hscal;            /* pinned: SL_COUNT doubles + flags  */
+    zcx             *hsmall;           /* pinned: small vector staging      */
+    zcx             *hx, *hy;          /* pinned: host-callback staging     */
+    zcx             *al, *be, *yv, *xv, *zrh, *hcol;
+    bbjd_qz          qz;
+    struct random_data rng;
+    int32_t          rng_state[32];
+    bbjd_stats       st;
+};
+
+static __thread hipStream_t tl_cb_stream;
+void *bbjd_callback_stream(void) { return (void *)tl_cb_stream; }
+
+static double now_s(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+#define HIPOK(x) ((x) == hipSuccess)
+
+/* ------------------------------------------------------------------ */
+/* allocation                                                          */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    bbjd_int n, ld, maxsub, nev, ims;
+    int mixed;
+    size_t off[32];
+    size_t total;
+} layout_t;
+
+enum { O_V, O_W, O_AV, O_BV, O_QR, O_QL, O_GV, O_T1, O_T2, O_T3, O_T4, O_HW, O_PART, O_PNORM, O_H, O_H2,
+       O_HSUM, O_SCAL, O_FLAG, O_COEF, O_COEF2, O_ZR, O_PEN, O_GRAM, O_GVC, O_QRC, O_QLC, O_TC0, O_TC1, O_TC2,
+       O_END };
+
+static bbjd_int auto_maxsub(const bbjd_params *p)
+{
+    bbjd_int ms = 3 * p->nev + 10;      /* reference src/jd_core.c:95-98 */
+    if (ms < 40) ms = 40;
+    return p->maxsub > 0 ? p->maxsub : (ms < p->n ? ms : p->n);
+}
+
+static void make_layout(layout_t *L, const bbjd_params *p, const bbjd_gpu_options *o)
+{
+    memset(L, 0, sizeof(*L));
+    L->n = p->n;
+    L->ld = (p->n + 63) / 64 * 64;
+    L->maxsub = auto_maxsub(p);
+    L->nev = p->nev;
+    L->ims = p->inner_maxiter > 0 ? p->inner_maxiter : 1;
+    L->mixed = o && o->precision == BBJD_PREC_MIXED;
+    const size_t Z = 16, C = 8, ld = (size_t)L->ld, ms = (size_t)L->maxsub;
+    size_t sz[O_END];
+    memset(sz, 0, sizeof(sz));
+    sz[O_V] = sz[O_W] = sz[O_AV] = sz[O_BV] = ld * ms * Z;
+    sz[O_QR] = sz[O_QL] = ld * (size_t)(L->nev + 1) * Z;
+    sz[O_GV] = L->mixed ? 0 : ld * (size_t)(L->ims + 1) * Z;
+    sz[O_T1] = sz[O_T2] = sz[O_T3] = sz[O_T4] = ld * Z;
+    sz[O_HW] = (2 * ms + (size_t)L->nev + 4) * Z;
+    int maxk = (int)(ms > (size_t)L->ims + 1 ? ms : (size_t)L->ims + 1);
+    if (maxk < L->nev + 1) maxk = (int)L->nev + 1;
+    sz[O_PART] = (size_t)bbjd_red_part_entries(L->n, maxk + 2) * Z;
+    sz[O_PNORM] = ((size_t)(L->n + BBJD_PROJ_ROWS - 1) / BBJD_PROJ_ROWS + 64) * 8;
+    sz[O_H] = sz[O_H2] = sz[O_HSUM] = ((size_t)maxk + 8) * Z;
+    sz[O_SCAL] = SL_COUNT * 8;
+    sz[O_FLAG] = FL_COUNT * 4;
+    sz[O_COEF] = ((size_t)maxk + 8) * Z;
+    sz[O_COEF2] = 64 * Z;
+    sz[O_ZR] = ms * ms * Z;
+    sz[O_PEN] = 4 * (ms + 1) * Z;
+    sz[O_GRAM] = 2 * ms * ms * Z;
+    if (L->mixed) {
+        sz[O_GVC] = ld * (size_t)(L->ims + 1) * C;
+        sz[O_QRC] = sz[O_QLC] = ld * (size_t)(L->nev + 1) * C;
+        sz[O_TC0] = sz[O_TC1] = sz[O_TC2] = ld * C;
+    }
+    size_t off = 0;
+    for (int i = 0; i < O_END; i++) {
+        L->off[i] = off;
+        off += (sz[i] + 255) / 256 * 256;
+    }
+    L->total = off;
+}
+
+void bbjd_params_init(bbjd_params *p)
+{
+    /* reference src/jd_core.c:73-84 */
+    memset(p, 0, sizeof(*p));
+    p->nev = 6;
+    p->sigma = 0.0;
+    p->target = BBJD_TARGET_NEAREST;
+    p->tol = 1e-10;
+    p->maxiter = 200;
+    p->inner_maxiter = 0;
+    p->inner_tol = 1e-2;
+    p->v0 = NULL;
+}
+
+void bbjd_gpu_options_init(bbjd_gpu_options *o)
+{
+    memset(o, 0, sizeof(*o));
+    o->callback_space = BBJD_CALLBACK_HOST;
+    o->precision = BBJD_PREC_FP64;
+}
+
+size_t bbjd_workspace_bytes(const bbjd_params *p, const bbjd_gpu_options *o)
+{
+    layout_t L;
+    make_layout(&L, p, o);
+    return L.total;
+}
+
+void bbjd_workspace_free(bbjd_workspace *ws)
+{
+    if (!ws) return;
+    struct bbjd_dev *d = ws->dev;
+    if (d) {
+        if (d->dblock) hipFree(d->dblock);
+        if (d->hscal) hipHostFree(d->hscal);
+        if (d->hsmall) hipHostFree(d->hsmall);
+        if (d->hx) hipHostFree(d->hx);
+        if (d->hy) hipHostFree(d->hy);
+        if (d->own_stream && d->stream) hipStreamDestroy(d->stream);
+        free(d->al); free(d->be); free(d->yv); free(d->xv); free(d->zrh); free(d->hcol);
+        bbjd_qz_free(&d->qz);
+        free(d);
+    }
+    free(ws->MA); free(ws->MB); free(ws->SA); free(ws->SB); free(ws->ZA); free(ws->ZB);
+    free(ws->gmres_H); free(ws->gmres_cs); free(ws->gmres_sn); free(ws->gmres_g);
+    free(ws);
+}
+
+bbjd_workspace *bbjd_workspace_alloc_gpu(const bbjd_params *p, const bbjd_gpu_options *o)
+{
+    if (!p || p->n <= 0 || p->nev <= 0) return NULL;
+    bbjd_gpu_options dflt;
+    bbjd_gpu_options_init(&dflt);
+    if (!o) o = &dflt;
+    layout_t L;
+    make_layout(&L, p, o);
+    if (L.maxsub > BBJD_MAXK - 2 || L.ims > BBJD_MAXK - 2) return NULL;
+
+    bbjd_workspace *ws = calloc(1, sizeof(*ws));
+    if (!ws) return NULL;
+    struct bbjd_dev *d = calloc(1, sizeof(*d));
+    if (!d) { free(ws); return NULL; }
+    ws->dev = d;
+    d->opt = *o;
+    d->ims = (int)L.ims;
+    ws->n = L.n; ws->maxsub = L.maxsub; ws->nev = L.nev; ws->ld = L.ld;
+    ws->inner_maxiter = L.ims;
+
+    if (o->stream) d->stream = (hipStream_t)o->stream;
+    else {
+        if (!HIPOK(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking))) goto fail;
+        d->own_stream = 1;
+    }
+    if (!HIPOK(hipMalloc(&d->dblock, L.total))) { d->dblock = NULL; goto fail; }
+    d->dbytes = L.total;
+    char *b = (char *)d->dblock;
+#define AT(i) ((void *)(b + L.off[i]))
+    ws->V = AT(O_V); ws->W = AT(O_W); ws->AV = AT(O_AV); ws->BV = AT(O_BV);
+    ws->Q_right = AT(O_QR); ws->Q_left = AT(O_QL);
+    ws->gmres_V = L.mixed ? AT(O_GVC) : AT(O_GV);
+    ws->tmp1 = AT(O_T1); ws->tmp2 = AT(O_T2); ws->tmp3 = AT(O_T3); ws->tmp4 = AT(O_T4);
+    ws->h_work = AT(O_HW);
+    ws->swap_nm = NULL;
+    d->rd.part = AT(O_PART);
+    d->rd.pnorm = AT(O_PNORM);
+    d->rd.h = AT(O_H); d->rd.h2 = AT(O_H2); d->rd.hsum = AT(O_HSUM);
+    d->rd.scal = AT(O_SCAL); d->rd.flag = AT(O_FLAG);
+    d->rd.coef = AT(O_COEF); d->rd.coef2 = AT(O_COEF2);
+    {
+        int maxk = (int)(L.maxsub > L.ims + 1 ? L.maxsub : L.ims + 1);
+        if (maxk < L.nev + 1) maxk = (int)L.nev + 1;
+        d->rd.part_cap = bbjd_red_part_entries(L.n, maxk + 2);
+    }
+    d->zr = AT(O_ZR); d->pencil = AT(O_PEN); d->gram = AT(O_GRAM);
+    if (L.mixed) {
+        d->gvc = AT(O_GVC); d->qrc = AT(O_QRC); d->qlc = AT(O_QLC);
+        d->tc[0] = AT(O_TC0); d->tc[1] = AT(O_TC1); d->tc[2] = AT(O_TC2);
+    }
+#undef AT
+    if (!HIPOK(hipMemsetAsync(d->rd.scal, 0, SL_COUNT * 8, d->stream))) goto fail;
+    if (!HIPOK(hipMemsetAsync(d->rd.flag, 0, FL_COUNT * 4, d->stream))) goto fail;
+    if (!HIPOK(hipHostMalloc((void **)&d->hscal, 256, 0))) { d->hscal = NULL; goto fail; }
+    const size_t small = (size_t)(6 * L.maxsub + 2 * L.ims + 64);
+    if (!HIPOK(hipHostMalloc((void **)&d->hsmall, small * sizeof(zcx), 0))) { d->hsmall = NULL; goto fail; }
+    if (o->callback_space == BBJD_CALLBACK_HOST) {
+        if (!HIPOK(hipHostMalloc((void **)&d->hx, (size_t)L.n * sizeof(zcx), 0))) { d->hx = NULL; goto fail; }
+        if (!HIPOK(hipHostMalloc((void **)&d->hy, (size_t)L.n * sizeof(zcx), 0))) { d->hy = NULL; goto fail; }
+    }
+    const size_t ss = (size_t)L.maxsub * L.maxsub;
+    ws->MA = calloc(ss, sizeof(zcx)); ws->MB = calloc(ss, sizeof(zcx));
+    ws->SA = calloc(ss, sizeof(zcx)); ws->SB = calloc(ss, sizeof(zcx));
+    ws->ZA = calloc(ss, sizeof(zcx)); ws->ZB = calloc(ss, sizeof(zcx));
+    ws->gmres_H = calloc((size_t)(L.ims + 1) * L.ims, sizeof(zcx));
+    ws->gmres_cs = calloc((size_t)L.ims, sizeof(zcx));
+    ws->gmres_sn = calloc((size_t)L.ims, sizeof(zcx));
+    ws->gmres_g = calloc((size_t)L.ims + 1, sizeof(zcx));
+    d->al = malloc((size_t)L.maxsub * sizeof(zcx)); d->be = malloc((size_t)L.maxsub * sizeof(zcx));
+    d->yv = malloc((size_t)(L.maxsub + L.ims + 2) * sizeof(zcx));
+    d->xv = malloc((size_t)(L.maxsub + 2) * sizeof(zcx));
+    d->zrh = malloc(ss * sizeof(zcx));
+    d->hcol = malloc((size_t)(L.maxsub + L.ims + 8) * 4 * sizeof(zcx));
+    if (!ws->MA || !ws->MB || !ws->SA || !ws->SB || !ws->ZA || !ws->ZB || !ws->gmres_H || !ws->gmres_cs ||
+        !ws->gmres_sn || !ws->gmres_g || !d->al || !d->be || !d->yv || !d->xv || !d->zrh || !d->hcol)
+        goto fail;
+    if (bbjd_qz_init(&d->qz, L.maxsub)) goto fail;
+    if (!HIPOK(hipStreamSynchronize(d->stream))) goto fail;
+    return ws;
+fail:
+    bbjd_workspace_free(ws);
+    return NULL;
+}
+
+bbjd_workspace *bbjd_workspace_alloc(const bbjd_params *p) { return bbjd_workspace_alloc_gpu(p, NULL); }
+
+void bbjd_result_free(bbjd_result *r)
+{
+    if (!r) return;
+    free(r->eigenvalues); free(r->eigenvectors); free(r->residuals);
+    r->eigenvalues = NULL; r->eigenvectors = NULL; r->residuals = NULL;
+}
+
+int bbjd_workspace_stats(const bbjd_workspace *ws, bbjd_stats *st)
+{
+    if (!ws || !ws->dev || !st) return -1;
+    *st = ws->dev->st;
+    return 0;
+}
+
+int bbjd_memcpy_d2h(void *dst, const void *src, size_t bytes)
+{ return HIPOK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost)) ? 0 : -1; }
+int bbjd_memcpy_h2d(void *dst, const void *src, size_t bytes)
+{ return HIPOK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice)) ? 0 : -1; }
+
+/* ------------------------------------------------------------------ */
+/* helpers                                                             */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    bbjd_workspace   *ws;
+    struct bbjd_dev  *d;
+    const bbjd_params *P;
+    hipStream_t       s;
+    bbjd_int          n, ld, ms;
+    int               err;     /* device/launch error seen */
+} ctx_t;
+
+#define COLZ(M, j) ((void *)((zcx *)(M) + (size_t)(j) * c->ld))
+#define COLC(M, j) ((void *)((c64 *)(M) + (size_t)(j) * c->ld))
+#define COLZ_STATIC(M, j, ldv) ((void *)((zcx *)(M) + (size_t)(j) * (ldv)))
+
+/* glibc rand() reproduced per solve: srand(s) == srandom(s) on the default
+ * 128-byte TYPE_3 state; random_r on a private state of that size gives the
+ * same sequence without touching the process-global generator. */
+static void rng_seed(struct bbjd_dev *d, unsigned seed)
+{
+    memset(&d->rng, 0, sizeof(d->rng));
+    initstate_r(seed, (char *)d->rng_state, sizeof(d->rng_state), &d->rng);
+}
+static double rng_unit(struct bbjd_dev *d)
+{
+    int32_t r;
+    random_r(&d->rng, &r);
+    return (double)r / 2147483647.0;   /* RAND_MAX */
+}
+
+static int sync_scal(ctx_t *c)
+{
+    struct bbjd_dev *d = c->d;
+    if (!HIPOK(hipMemcpyAsync(d->hscal, d->rd.scal, SL_COUNT * 8, hipMemcpyDeviceToHost, c->s))) return -1;
+    if (!HIPOK(hipStreamSynchronize(c->s))) return -1;
+    return 0;
+}
+
+/* Apply a user operator in the workspace's callback space. */
+static int apply_op(ctx_t *c, bbjd_matvec_fn f, const void *x, void *y)
+{
+    struct bbjd_dev *d = c->d;
+    if (d->opt.callback_space == BBJD_CALLBACK_DEVICE) {
+        hipStream_t prev = tl_cb_stream;
+        tl_cb_stream = c->s;
+        int rc = f(c->n, (const bbjd_zomplex *)x, (bbjd_zomplex *)y, c->P->op_data);
+        tl_cb_stream = prev;
+        return rc;
+    }
+    const size_t b = (size_t)c->n * sizeof(zcx);
+    if (!HIPOK(hipMemcpyAsync(d->hx, x, b, hipMemcpyDeviceToHost, c->s))) return -1;
+    if (!HIPOK(hipStreamSynchronize(c->s))) return -1;
+    int rc = f(c->n, (const bbjd_zomplex *)d->hx, (bbjd_zomplex *)d->hy, c->P->op_data);
+    if (rc) return rc;
+    if (!HIPOK(hipMemcpyAsync(y, d->hy, b, hipMemcpyHostToDevice, c->s))) return -1;
+    return 0;
+}
+
+static int apply_pc(ctx_t *c, const void *x, void *y)
+{
+    if (c->P->precond) return apply_op(c, (bbjd_matvec_fn)c->P->precond, x, y);
+    return HIPOK(hipMemcpyAsync(y, x, (size_t)c->n * sizeof(zcx), hipMemcpyDeviceToDevice, c->s)) ? 0 : -1;
+}
+
+static int upload_small(ctx_t *c, void *dst, const zcx *src, bbjd_int cnt)
+{
+    memcpy(c->d->hsmall, src, (size_t)cnt * sizeof(zcx));
+    return HIPOK(hipMemcpyAsync(dst, c->d->hsmall, (size_t)cnt * sizeof(zcx), hipMemcpyHostToDevice, c->s)) ? 0
+                                                                                                          : -1;
+}
+
+/* Column j of the deflation projectors (reference src/jd_core.c:516-528):
+ * Q_right[:, j] = normalised CGS2(u), Q_left[:, j] = normalised
+ * CGS2(nu0 Au + mu0 Bu); both normalisations only when the norm > 1e-300. */
+static int deflation_append(ctx_t *c, bbjd_int j, const void *u, const void *Au, const void *Bu, zcx nu0,
+                            zcx mu0)
+{
+    bbjd_workspace *ws = c->ws;
+    struct bbjd_dev *d = c->d;
+    int rc = 0;
+    void *qr = COLZ(ws->Q_right, j), *ql = COLZ(ws->Q_left, j);
+    rc |= !HIPOK(hipMemcpyAsync(qr, u, (size_t)c->n * 16, hipMemcpyDeviceToDevice, c->s));
+    rc |= bbjd_k_cgs2(c->s, c->n, (int)j, ws->Q_right, c->ld, qr, &d->rd, SL_Q, NULL);
+    rc |= bbjd_k_scale_inv(c->s, c->n, qr, d->rd.scal, SL_Q, 1);
+    rc |= bbjd_k_lincomb(c->s, c->n, creal(nu0), cimag(nu0), Au, creal(mu0), cimag(mu0), Bu, ql);
+    rc |= bbjd_k_cgs2(c->s, c->n, (int)j, ws->Q_left, c->ld, ql, &d->rd, SL_Q, NULL);
+    rc |= bbjd_k_scale_inv(c->s, c->n, ql, d->rd.scal, SL_Q, 1);
+    if (d->opt.precision == BBJD_PREC_MIXED) {
+        rc |= bbjd_k_z2c(c->s, c->n, qr, COLC(d->qrc, j));
+        rc |= bbjd_k_z2c(c->s, c->n, ql, COLC(d->qlc, j));
+    }
+    d->st.stream_bytes += 16.0 * c->n * (3 + 4.0 * j + 4 + 4.0 * j);
+    return rc ? -1 : 0;
+}
+
+/* Givens with real cs >= 0 (reference src/jd_core.c:477-511). */
+static void givens(zcx f, zcx g, double *cs, zcx *sn, zcx *r)
+{
+    double af = cabs(f), ag = cabs(g);
+    if (ag < 1e-300) { *cs = 1.0; *sn = 0.0; *r = f; return; }
+    if (af < 1e-300) { *cs = 0.0; *sn = conj(g) / ag; *r = ag; return; }
+    double sc = af + ag, fs = af / sc, gs = ag / sc;
+    double nrm = sc * sqrt(fs * fs + gs * gs);
+    *cs = af / nrm;
+    *sn = (f / af) * conj(g) / nrm;
+    *r = (f / af) * nrm;
+}
+
+/* complex64 operator apply for the mixed inner solve: the c64 callback when
+ * given, else the FP64 one through conversions (z64 scratch, n complex128). */
+static int apply_c64(ctx_t *c, bbjd_op_c64_fn f64, bbjd_matvec_fn f, const void *x, void *y, void *z1, void *z2)
+{
+    if (f64 && c->d->opt.callback_space == BBJD_CALLBACK_DEVICE) {
+        hipStream_t prev = tl_cb_stream;
+        tl_cb_stream = c->s;
+        int rc = f64(c->n, (const bbjd_cfloat *)x, (bbjd_cfloat *)y, c->P->op_data);
+        tl_cb_stream = prev;
+        return rc;
+    }
+    int rc = bbjd_k_c2z(c->s, c->n, x, z1);
+    if (rc) return -1;
+    if (f) rc = apply_op(c, f, z1, z2);
+    else rc = !HIPOK(hipMemcpyAsync(z2, z1, (size_t)c->n * 16, hipMemcpyDeviceToDevice, c->s));
+    if (rc) return rc;
+    return bbjd_k_z2c(c->s, c->n, z2, y);
+}
+
+/* Left-preconditioned GMRES on the projected correction equation
+ * (reference src/jd_core.c:568-727).  r, z: device complex128; QR, QL: the
+ * ndefl leading columns of the deflation projectors.  Arnoldi is CGS2 with
+ * the same DGKS test and Hessenberg column (sum of both passes) as the
+ * reference's MGS2; Givens, least squares and stopping rule are the
+ * reference's, on the host in FP64.  Mixed precision: basis, operator
+ * applies and projections in complex64 with FP32 accumulation. */
+static int gmres(ctx_t *c, bbjd_int ndefl, const void *QR, const void *QL, zcx ta, zcx tb, const void *r, void *z,
+                 bbjd_int ims, double tol)
+{
+    bbjd_workspace *ws = c->ws;
+    struct bbjd_dev *d = c->d;
+    const bbjd_params *P = c->P;
+    const bbjd_int n = c->n, ldh = ws->inner_maxiter + 1;
+    const int mixed = d->opt.precision == BBJD_PREC_MIXED;
+    zcx *H = (zcx *)ws->gmres_H, *sn = (zcx *)ws->gmres_sn, *g = (zcx *)ws->gmres_g;
+    double *cs = (double *)ws->gmres_cs;
+    memset(H, 0, (size_t)ldh * ws->inner_maxiter * sizeof(zcx));
+    int rc = 0;
+    /* FP64 temporaries: t1 = tmp3, t2 = tmp4 as in the reference (tmp4 holds
+     * r, read only by the first step) */
+    void *t1 = ws->tmp3, *t2 = ws->tmp4;
+    void *V = mixed ? d->gvc : ws->gmres_V;
+    const size_t esz = mixed ? 8 : 16;
+#define GCOL(j) ((void *)((char *)V + (size_t)(j) * c->ld * esz))
+    const void *qr = mixed ? d->qrc : QR, *ql = mixed ? d->qlc : QL;
+
+    if (!mixed) {
+        rc |= bbjd_k_lincomb(c->s, n, -1.0, 0.0, r, 0.0, 0.0, r, t1);
+        rc |= bbjd_k_project_out(c->s, n, (int)ndefl, QL, c->ld, t1, &d->rd);
+        if (apply_pc(c, t1, GCOL(0))) return -1;
+        rc |= bbjd_k_norm(c->s, n, GCOL(0), &d->rd, SL_G);
+    } else {
+        void *tc0 = d->tc[0];
+        rc |= bbjd_k_z2c(c->s, n, r, tc0);
+        rc |= bbjd_k_scale_c(c->s, n, tc0, -1.0, 0.0);
+        rc |= bbjd_k_project_out_c(c->s, n, (int)ndefl, ql, c->ld, tc0, &d->rd);
+        if (P->precond) {
+            if (apply_c64(c, d->opt.precond_c64, (bbjd_matvec_fn)P->precond, tc0, GCOL(0), t1, t2)) return -1;
+        } else
+            rc |= !HIPOK(hipMemcpyAsync(GCOL(0), tc0, (size_t)n * 8, hipMemcpyDeviceToDevice, c->s));
+        rc |= bbjd_k_norm_c(c->s, n, GCOL(0), &d->rd, SL_G);
+    }
+    if (rc || sync_scal(c)) return -2;
+    const double beta0 = d->hscal[SL_G];
+    if (beta0 < 1e-300) {
+        return HIPOK(hipMemsetAsync(z, 0, (size_t)n * 16, c->s)) ? 0 : -2;
+    }
+    if (!mixed) rc |= bbjd_k_scale(c->s, n, GCOL(0), 1.0 / beta0, 0.0);
+    else rc |= bbjd_k_scale_c(c->s, n, GCOL(0), 1.0 / beta0, 0.0);
+    memset(g, 0, (size_t)(ims + 1) * sizeof(zcx));
+    g[0] = beta0;
+
+    bbjd_int j;
+    for (j = 0; j < ims; j++) {
+        void *vj = GCOL(j), *vj1 = GCOL(j + 1);
+        if (!mixed) {
+            rc |= !HIPOK(hipMemcpyAsync(t1, vj, (size_t)n * 16, hipMemcpyDeviceToDevice, c->s));
+            rc |= bbjd_k_project_out(c->s, n, (int)ndefl, QR, c->ld, t1, &d->rd);
+            if (apply_op(c, P->matvec_A, t1, t2)) return -1;
+            if (apply_op(c, P->matvec_B, t1, vj1)) return -1;
+            rc |= bbjd_k_lincomb(c->s, n, creal(tb), cimag(tb), t2, -creal(ta), -cimag(ta), vj1, t2);
+            rc |= bbjd_k_project_out(c->s, n, (int)ndefl, QL, c->ld, t2, &d->rd);
+            if (apply_pc(c, t2, vj1)) return -1;
+            rc |= bbjd_k_cgs2(c->s, n, (int)(j + 1), V, c->ld, vj1, &d->rd, SL_G, d->rd.hsum);
+        } else {
+            void *a = d->tc[0], *bq = d->tc[1], *e = d->tc[2];
+            rc |= !HIPOK(hipMemcpyAsync(a, vj, (size_t)n * 8, hipMemcpyDeviceToDevice, c->s));
+            rc |= bbjd_k_project_out_c(c->s, n, (int)ndefl, qr, c->ld, a, &d->rd);
+            if (apply_c64(c, d->opt.matvec_A_c64, P->matvec_A, a, bq, t1, t2)) return -1;
+            if (apply_c64(c, d->opt.matvec_B_c64, P->matvec_B, a, e, t1, t2)) return -1;
+            rc |= bbjd_k_lincomb_c(c->s, n, creal(tb), cimag(tb), bq, -creal(ta), -cimag(ta), e, bq);
+            rc |= bbjd_k_project_out_c(c->s, n, (int)ndefl, ql, c->ld, bq, &d->rd);
+            if (P->precond) {
+                if (apply_c64(c, d->opt.precond_c64, (bbjd_matvec_fn)P->precond, bq, vj1, t1, t2)) return -1;
+            } else
+                rc |= !HIPOK(hipMemcpyAsync(vj1, bq, (size_t)n * 8, hipMemcpyDeviceToDevice, c->s));
+            rc |= bbjd_k_cgs2_c(c->s, n, (int)(j + 1), V, c->ld, vj1, &d->rd, SL_G, d->rd.hsum);
+        }
+        /* Hessenberg column + both norms down */
+        if (!mixed)
+            rc |= !HIPOK(hipMemcpyAsync(d->hsmall, d->rd.hsum, (size_t)(j + 1) * 16, hipMemcpyDeviceToHost, c->s));
+        else
+            rc |= !HIPOK(hipMemcpyAsync(d->hsmall, d->rd.hsum, (size_t)(j + 1) * 8, hipMemcpyDeviceToHost, c->s));
+        if (rc || sync_scal(c)) return -2;
+        d->st.inner_steps++;
+        d->st.stream_bytes += (double)esz * n * (8.0 * ndefl + 4.0 * (j + 1) + 8);
+        if (!mixed)
+            for (bbjd_int i = 0; i <= j; i++) H[i + j * ldh] = d->hsmall[i];
+        else {
+            const c64 *hc = (const c64 *)d->hsmall;
+            for (bbjd_int i = 0; i <= j; i++) H[i + j * ldh] = hc[i].re + hc[i].im * I;
+        }
+        const double wb = d->hscal[SL_NB], wa = d->hscal[SL_G];
+        H[(j + 1) + j * ldh] = wa;
+        if (wa < 1e-14 * wb) { j++; break; }                      /* :674 */
+        if (!mixed) rc |= bbjd_k_scale(c->s, n, vj1, 1.0 / wa, 0.0);
+        else rc |= bbjd_k_scale_c(c->s, n, vj1, 1.0 / wa, 0.0);
+        for (bbjd_int i = 0; i < j; i++) {                       /* :680-685 */
+            zcx h1 = H[i + j * ldh], h2 = H[i + 1 + j * ldh];
+            H[i + j * ldh] = cs[i] * h1 + sn[i] * h2;
+            H[i + 1 + j * ldh] = -conj(sn[i]) * h1 + cs[i] * h2;
+        }
+        double cj; zcx sj, rj;
+        givens(H[j + j * ldh], H[j + 1 + j * ldh], &cj, &sj, &rj);
+        cs[j] = cj; sn[j] = sj;
+        H[j + j * ldh] = rj;
+        H[j + 1 + j * ldh] = 0.0;
+        zcx g1 = g[j], g2 = g[j + 1];
+        g[j] = cs[j] * g1 + sn[j] * g2;
+        g[j + 1] = -conj(sn[j]) * g1 + cs[j] * g2;
+        if (cabs(g[j + 1]) < tol * beta0) { j++; break; }
+    }
+    const bbjd_int m = j;
+    zcx *y = g;                                                    /* :709-716 */
+    for (bbjd_int i = m - 1; i >= 0; i--) {
+        for (bbjd_int k2 = i + 1; k2 < m; k2++) y[i] -= H[i + k2 * ldh] * y[k2];
+        if (cabs(H[i + i * ldh]) > 1e-300) y[i] /= H[i + i * ldh];
+    }
+    if (m == 0) {
+        rc |= !HIPOK(hipMemsetAsync(z, 0, (size_t)n * 16, c->s));
+    } else if (!mixed) {
+        rc |= upload_small(c, d->rd.coef, y, m);
+        rc |= bbjd_k_gemv_n(c->s, n, (int)m, V, c->ld, d->rd.coef, z, &d->rd, SL_T1);
+    } else {
+        c64 *yc = (c64 *)d->hsmall;
+        for (bbjd_int i = 0; i < m; i++) { yc[i].re = (float)creal(y[i]); yc[i].im = (float)cimag(y[i]); }
+        rc |= !HIPOK(hipMemcpyAsync(d->rd.coef, yc, (size_t)m * 8, hipMemcpyHostToDevice, c->s));
+        rc |= bbjd_k_gemv_n_c(c->s, n, (int)m, V, c->ld, d->rd.coef, d->tc[0], &d->rd, SL_T1);
+        rc |= bbjd_k_c2z(c->s, n, d->tc[0], z);
+    }
+    rc |= bbjd_k_project_out(c->s, n, (int)ndefl, QR, c->ld, z, &d->rd);  /* :724, FP64 */
+#undef GCOL
+    return rc ? -2 : 0;
+}
+
+/* Pencil column/row k after expansion (reference src/jd_core.c:1085-1104):
+ * MA[0:k+1, k] = W^H AV[:, k], MB likewise, MA[k, 0:k] = W[:, k]^H AV[:, :k].
+ * Rows come as conj(AV^H w).  One host sync. */
+static int pencil_update(ctx_t *c, bbjd_int k)
+{
+    bbjd_workspace *ws = c->ws;
+    struct bbjd_dev *d = c->d;
+    const bbjd_int ms = c->ms;
+    zcx *pen = (zcx *)d->pencil;
+    int rc = 0;
+    rc |= bbjd_k_gemv_c(c->s, c->n, (int)(k + 1), ws->W, c->ld, COLZ(ws->AV, k), COLZ(ws->BV, k), pen,
+                        pen + (ms + 1), &d->rd);
+    if (k > 0) {
+        rc |= bbjd_k_gemv_c(c->s, c->n, (int)k, ws->AV, c->ld, COLZ(ws->W, k), NULL, pen + 2 * (ms + 1), NULL,
+                            &d->rd);
+        rc |= bbjd_k_gemv_c(c->s, c->n, (int)k, ws->BV, c->ld, COLZ(ws->W, k), NULL, pen + 3 * (ms + 1), NULL,
+                            &d->rd);
+    }
+    rc |= !HIPOK(hipMemcpyAsync(d->hcol, pen, (size_t)4 * (ms + 1) * 16, hipMemcpyDeviceToHost, c->s));
+    rc |= !HIPOK(hipStreamSynchronize(c->s));
+    if (rc) return -1;
+    zcx *MA = (zcx *)ws->MA, *MB = (zcx *)ws->MB;
+    const zcx *ca = d->hcol, *cb = d->hcol + (ms + 1), *ra = d->hcol + 2 * (ms + 1), *rb = d->hcol + 3 * (ms + 1);
+    for (bbjd_int i = 0; i <= k; i++) { MA[i + k * ms] = ca[i]; MB[i + k * ms] = cb[i]; }
+    for (bbjd_int i = 0; i < k; i++) { MA[k + i * ms] = conj(ra[i]); MB[k + i * ms] = conj(rb[i]); }
+    d->st.stream_bytes += 16.0 * c->n * ((k + 1) + 2 + 2.0 * (k + 1));
+    return 0;
+}
+
+/* Thick restart (reference src/jd_core.c:975-1030): V, AV, BV <- (.) ZB[:, :keep]
+ * in place on the device, W rebuilt column by column (CGS2), MA/MB = W^H AV,
+ * W^H BV. */
+static int thick_restart(ctx_t *c, bbjd_int k, bbjd_int keep, zcx nu0, zcx mu0)
+{
+    bbjd_workspace *ws = c->ws;
+    struct bbjd_dev *d = c->d;
+    const bbjd_int ms = c->ms, n = c->n;
+    const zcx *ZB = (const zcx *)ws->ZB;
+    int rc = 0;
+    for (bbjd_int j = 0; j < k; j++)
+        for (bbjd_int cc = 0; cc < keep; cc++) d->zrh[(size_t)j * keep + cc] = ZB[j + cc * ms];
+    rc |= !HIPOK(hipMemcpyAsync(d->zr, d->zrh, (size_t)k * keep * 16, hipMemcpyHostToDevice, c->s));
+    rc |= bbjd_k_rotate(c->s, n, (int)k, (int)keep, ws->V, c->ld, d->zr);
+    rc |= bbjd_k_rotate(c->s, n, (int)k, (int)keep, ws->AV, c->ld, d->zr);
+    rc |= bbjd_k_rotate(c->s, n, (int)k, (int)keep, ws->BV, c->ld, d->zr);
+    for (bbjd_int j = 0; j < keep; j++) {
+        void *wj = COLZ(ws->W, j);
+        rc |= bbjd_k_lincomb(c->s, n, creal(nu0), cimag(nu0), COLZ(ws->AV, j), creal(mu0), cimag(mu0),
+                             COLZ(ws->BV, j), wj);
+        rc |= bbjd_k_cgs2(c->s, n, (int)j, ws->W, c->ld, wj, &d->rd, SL_T0, NULL);
+        rc |= bbjd_k_scale_inv(c->s, n, wj, d->rd.scal, SL_T0, 1);
+    }
+    zcx *G = (zcx *)d->gram;   /* [keep x keep] A then B, ld keep */
+    for (bbjd_int j = 0; j < keep; j++)
+        rc |= bbjd_k_gemv_c(c->s, n, (int)keep, ws->W, c->ld, COLZ(ws->AV, j), COLZ(ws->BV, j), G + j * keep,
+                            G + keep * keep + j * keep, &d->rd);
+    zcx *hg = malloc((size_t)2 * keep * keep * sizeof(zcx));
+    if (!hg) return -1;
+    rc |= !HIPOK(hipMemcpyAsync(hg, G, (size_t)2 * keep * keep * 16, hipMemcpyDeviceToHost, c->s));
+    rc |= !HIPOK(hipStreamSynchronize(c->s));
+    zcx *MA = (zcx *)ws->MA, *MB = (zcx *)ws->MB;
+    for (bbjd_int j = 0; j < keep; j++)
+        for (bbjd_int i = 0; i < keep; i++) {
+            MA[i + j * ms] = hg[i + j * keep];
+            MB[i + j * ms] = hg[keep * keep + i + j * keep];
+        }
+    free(hg);
+    d->st.restarts++;
+    d->st.stream_bytes += 16.0 * n * (3.0 * (k + keep) + 3.0 * keep + (double)keep * keep * 2 + 2.0 * keep * keep);
+    return rc ? -1 : 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* the outer iteration                                                 */
+/* ------------------------------------------------------------------ */
+bbjd_status bbjd_solve(const bbjd_params *P, bbjd_workspace *ws, bbjd_result *R)
+{
+    if (!P || !ws || !R) return BBJD_ERR_INVALID;
+    struct bbjd_dev *d = ws->dev;
+    memset(R, 0, sizeof(*R));
+    if (P->n != ws->n || P->nev > ws->nev || P->nev <= 0 || !P->matvec_A || !P->matvec_B ||
+        (P->inner_maxiter > 0 && P->inner_maxiter > ws->inner_maxiter)) {
+        R->status = BBJD_ERR_INVALID;
+        return BBJD_ERR_INVALID;
+    }
+    const double t_start = now_s();
+    memset(&d->st, 0, sizeof(d->st));
+    ctx_t cx = {ws, d, P, d->stream, P->n, ws->ld, ws->maxsub, 0};
+    ctx_t *c = &cx;
+    const bbjd_int n = P->n, nev = P->nev, ms = ws->maxsub;
+    bbjd_int minsub = P->minsub > 0 ? P->minsub : (2 * nev + 5 < 20 ? 20 : 2 * nev + 5);
+    const zcx sigma = P->sigma;
+    const double tol = P->tol;
+    zcx *MA = (zcx *)ws->MA, *MB = (zcx *)ws->MB;
+    zcx *al = d->al, *be = d->be;
+    bbjd_status st = BBJD_SUCCESS;
+    int rc = 0;
+
+    R->eigenvalues = calloc((size_t)nev, sizeof(zcx));
+    R->eigenvectors = calloc((size_t)n * nev, sizeof(zcx));
+    R->residuals = calloc((size_t)nev, sizeof(double));
+    if (!R->eigenvalues || !R->eigenvectors || !R->residuals) {
+        bbjd_result_free(R);
+        R->status = BBJD_ERR_ALLOC;
+        return BBJD_ERR_ALLOC;
+    }
+    zcx *eval = (zcx *)R->eigenvalues;
+
+    zcx nu0 = 0.0, mu0 = 1.0;                                      /* :783-789 */
+    if (P->target == BBJD_TARGET_NEAREST) {
+        double sc = 1.0 / sqrt(1.0 + cabs(sigma) * cabs(sigma));
+        nu0 = sc; mu0 = -sigma * sc;
+    } else if (P->target == BBJD_TARGET_SMALLEST_MAG) {
+        nu0 = 1.0; mu0 = 0.0;
+    }
+    ws->kdim = 0; ws->nlock = 0;
+    bbjd_qz_invalidate(&d->qz);
+    rng_seed(d, 42);
+    hipMemsetAsync(d->rd.flag, 0, FL_COUNT * 4, c->s);
+
+    /* ---- start vector (:790-820) ---- */
+    {
+        zcx *hv = d->hx ? d->hx : malloc((size_t)n * sizeof(zcx));
+        if (!hv) { st = BBJD_ERR_ALLOC; goto done; }
+        if (P->v0) memcpy(hv, P->v0, (size_t)n * sizeof(zcx));
+        else
+            for (bbjd_int i = 0; i < n; i++) {
+                double re = rng_unit(d) - 0.5;
+                double im = rng_unit(d) - 0.5;
+                hv[i] = re + im * I;
+            }
+        rc |= !HIPOK(hipMemcpyAsync(ws->V, hv, (size_t)n * 16, hipMemcpyHostToDevice, c->s));
+        rc |= bbjd_k_norm(c->s, n, ws->V, &d->rd, SL_T0);
+        rc |= sync_scal(c);
+        if (!d->hx) free(hv);
+    }
+    if (rc) { st = BBJD_ERR_ALLOC; goto done; }
+    if (d->hscal[SL_T0] < 1e-300) { st = BBJD_ERR_BREAKDOWN; goto done; }
+    rc |= bbjd_k_scale(c->s, n, ws->V, 1.0 / d->hscal[SL_T0], 0.0);
+    if (apply_op(c, P->matvec_A, ws->V, ws->AV) || apply_op(c, P->matvec_B, ws->V, ws->BV)) goto cb_err;
+    rc |= bbjd_k_lincomb_norm(c->s, n, creal(nu0), cimag(nu0), ws->AV, creal(mu0), cimag(mu0), ws->BV, ws->W,
+                              &d->rd, SL_W);
+    rc |= bbjd_k_scale_inv(c->s, n, ws->W, d->rd.scal, SL_W, 0);
+    if (rc || pencil_update(c, 0)) { st = BBJD_ERR_ALLOC; goto done; }
+    ws->kdim = 1;
+
+    for (bbjd_int iter = 0; iter < P->maxiter; iter++) {
+        bbjd_int k = ws->kdim;
+        const bbjd_int locked = ws->nlock;
+        bbjd_int p = locked;
+        d->st.outer_iters++;
+        d->st.sum_k += k;
+        /* ---- step 1: projected QZ on the host ---- */
+        {
+            bbjd_int nkeep = minsub - locked;
+            if (nkeep < 1) nkeep = 1;
+            const double tq = now_s();
+            int qrc = bbjd_qz_solve(&d->qz, k, MA, MB, (zcx *)ws->SA, (zcx *)ws->SB, (zcx *)ws->ZA,
+                                    (zcx *)ws->ZB, al, be, sigma, P->target, nkeep, eval, R->nconv, tol, &p);
+            d->st.qz_seconds += now_s() - tq;
+            if (qrc) { st = BBJD_ERR_LAPACK; goto done; }
+        }
+        const zcx ta = al[p], tb = be[p];
+        /* ---- step 2: Ritz vector u = V ZB y on the device ---- */
+        void *u = ws->tmp1, *Au = ws->tmp2, *Bu = ws->tmp3, *r = ws->tmp4;
+        bbjd_ritz_coefficients(k, ms, p, ta, tb, (const zcx *)ws->SA, (const zcx *)ws->SB, (const zcx *)ws->ZB,
+                               d->yv, d->xv);
+        rc |= upload_small(c, d->rd.coef, d->xv, k);
+        rc |= bbjd_k_gemv_n(c->s, n, (int)k, ws->V, c->ld, d->rd.coef, u, &d->rd, SL_U);
+        rc |= bbjd_k_scale_inv(c->s, n, u, d->rd.scal, SL_U, 1);
+        /* ---- step 3: residual ---- */
+        if (apply_op(c, P->matvec_A, u, Au) || apply_op(c, P->matvec_B, u, Bu)) goto cb_err;
+        rc |= bbjd_k_lincomb_norm(c->s, n, creal(tb), cimag(tb), Au, -creal(ta), -cimag(ta), Bu, r, &d->rd, SL_R);
+        if (rc || sync_scal(c)) { st = BBJD_ERR_ALLOC; goto done; }
+        d->st.stream_bytes += 16.0 * n * (k + 1 + 1 + 3);
+        if (d->hscal[SL_U] < 1e-300) { st = BBJD_ERR_BREAKDOWN; goto done; }
+        const double den = cabs(ta) + cabs(tb);
+        const double rel = d->hscal[SL_R] / (den > 1e-300 ? den : 1.0);
+        const zcx lam = cabs(tb) > 1e-300 ? ta / tb : ta;
+        if (P->verbose >= 2)
+            fprintf(stderr, "  iter %3lld  kdim %3lld  locked %2lld  eig (%+.6e, %+.6e)  res %.3e\n",
+                    (long long)iter, (long long)k, (long long)locked, creal(lam), cimag(lam), rel);
+
+        /* ---- step 4: convergence / locking (:905-932) ---- */
+        if (rel < tol) {
+            eval[R->nconv] = lam;
+            rc |= !HIPOK(hipMemcpyAsync((zcx *)R->eigenvectors + (size_t)R->nconv * n, u, (size_t)n * 16,
+                                        hipMemcpyDeviceToHost, c->s));
+            rc |= !HIPOK(hipStreamSynchronize(c->s));
+            R->residuals[R->nconv] = rel;
+            R->nconv++;
+            d->st.lock_iters++;
+            if (P->verbose >= 1)
+                fprintf(stderr, "  *** converged #%lld: (%+.10e, %+.10e)  res %.3e  iter %lld\n",
+                        (long long)R->nconv, creal(lam), cimag(lam), rel, (long long)iter);
+            if (R->nconv >= nev) { R->iters = iter + 1; goto done; }
+            if (deflation_append(c, ws->nlock, u, Au, Bu, nu0, mu0)) { st = BBJD_ERR_ALLOC; goto done; }
+            ws->nlock++;
+            continue;          /* (MA, MB) unchanged: the QZ cache stays valid */
+        }
+
+        /* ---- step 6: correction (:944-972) ---- */
+        void *zc = ws->tmp2;
+        if (P->inner_maxiter > 0) {
+            if (deflation_append(c, locked, u, Au, Bu, nu0, mu0)) { st = BBJD_ERR_ALLOC; goto done; }
+            int g = gmres(c, ws->nlock + 1, ws->Q_right, ws->Q_left, ta, tb, r, zc, P->inner_maxiter,
+                          P->inner_tol);
+            if (g == -1) { st = BBJD_ERR_CALLBACK; goto done; }
+            if (g) { st = BBJD_ERR_ALLOC; goto done; }
+        } else {
+            void *sv = ws->tmp4;
+            if (P->precond) {
+                if (apply_op(c, (bbjd_matvec_fn)P->precond, r, zc)) { st = BBJD_ERR_CALLBACK; goto done; }
+                if (apply_op(c, (bbjd_matvec_fn)P->precond, u, sv)) { st = BBJD_ERR_CALLBACK; goto done; }
+            } else {
+                rc |= !HIPOK(hipMemcpyAsync(zc, r, (size_t)n * 16, hipMemcpyDeviceToDevice, c->s));
+                rc |= !HIPOK(hipMemcpyAsync(sv, u, (size_t)n * 16, hipMemcpyDeviceToDevice, c->s));
+            }
+            rc |= bbjd_k_svdv(c->s, n, u, zc, sv, &d->rd);
+            d->st.stream_bytes += 16.0 * n * 6;
+        }
+
+        /* ---- step 7: thick restart (:975-1030) ---- */
+        if (k >= ms) {
+            bbjd_int keep = minsub < k ? minsub : k - 1;
+            if (keep < locked + 1) keep = locked + 1;
+            if (thick_restart(c, k, keep, nu0, mu0)) { st = BBJD_ERR_ALLOC; goto done; }
+            ws->kdim = keep;
+            k = keep;
+        }
+
+        /* ---- step 8: expansion (:1032-1068) ---- */
+        void *vn = COLZ(ws->V, k);
+        rc |= !HIPOK(hipMemcpyAsync(vn, zc, (size_t)n * 16, hipMemcpyDeviceToDevice, c->s));
+        rc |= bbjd_k_cgs2(c->s, n, (int)k, ws->V, c->ld, vn, &d->rd, SL_V, NULL);
+        if (rc || sync_scal(c)) { st = BBJD_ERR_ALLOC; goto done; }
+        double nrm = d->hscal[SL_V];
+        if (nrm < 1e-14) {
+            zcx *tmpv = d->hx ? d->hx : malloc((size_t)n * sizeof(zcx));
+            if (!tmpv) { st = BBJD_ERR_ALLOC; goto done; }
+            for (bbjd_int i = 0; i < n; i++) {
+                double re = rng_unit(d) - 0.5;
+                double im = rng_unit(d) - 0.5;
+                tmpv[i] = re + im * I;
+            }
+            rc |= !HIPOK(hipMemcpyAsync(vn, tmpv, (size_t)n * 16, hipMemcpyHostToDevice, c->s));
+            rc |= bbjd_k_cgs2(c->s, n, (int)k, ws->V, c->ld, vn, &d->rd, SL_V, NULL);
+            rc |= sync_scal(c);
+            if (!d->hx) free(tmpv);
+            nrm = d->hscal[SL_V];
+            if (nrm < 1e-14) { R->iters = iter + 1; st = BBJD_ERR_BREAKDOWN; goto done; }
+        }
+        rc |= bbjd_k_scale(c->s, n, vn, 1.0 / nrm, 0.0);
+        if (apply_op(c, P->matvec_A, vn, COLZ(ws->AV, k)) || apply_op(c, P->matvec_B, vn, COLZ(ws->BV, k)))
+            goto cb_err;
+        void *wn = COLZ(ws->W, k);
+        rc |= bbjd_k_lincomb(c->s, n, creal(nu0), cimag(nu0), COLZ(ws->AV, k), creal(mu0), cimag(mu0),
+                             COLZ(ws->BV, k), wn);
+        rc |= bbjd_k_cgs2(c->s, n, (int)k, ws->W, c->ld, wn, &d->rd, SL_W, NULL);
+        if (rc || sync_scal(c)) { st = BBJD_ERR_ALLOC; goto done; }
+        if (d->hscal[SL_W] < 1e-14) {
+            rc |= !HIPOK(hipMemcpyAsync(wn, COLZ(ws->AV, k), (size_t)n * 16, hipMemcpyDeviceToDevice, c->s));
+            rc |= bbjd_k_cgs2(c->s, n, (int)k, ws->W, c->ld, wn, &d->rd, SL_W, NULL);
+        }
+        rc |= bbjd_k_scale_inv(c->s, n, wn, d->rd.scal, SL_W, 1);
+        d->st.stream_bytes += 16.0 * n * (4.0 * k + 4 + 3 + 4.0 * k + 2);
+        /* ---- projected pencil row/column (:1085-1104) ---- */
+        if (rc || pencil_update(c, k)) { st = BBJD_ERR_ALLOC; goto done; }
+        bbjd_qz_invalidate(&d->qz);
+        ws->kdim = k + 1;
+        R->iters = iter + 1;
+    }
+    if (R->nconv < nev) st = BBJD_MAX_ITER;
+done:
+    {
+        int flags[FL_COUNT];
+        if (HIPOK(hipMemcpyAsync(flags, d->rd.flag, sizeof(flags), hipMemcpyDeviceToHost, c->s)) &&
+            HIPOK(hipStreamSynchronize(c->s)))
+            d->st.dgks_second_passes = flags[FL_DGKS_COUNT];
+    }
+    d->st.solve_seconds = now_s() - t_start;
+    R->status = st;
+    return st;
+cb_err:
+    hipStreamSynchronize(c->s);
+    d->st.solve_seconds = now_s() - t_start;
+    R->status = BBJD_ERR_CALLBACK;
+    return BBJD_ERR_CALLBACK;
+}
+
+/* ------------------------------------------------------------------ */
+/* reference test helpers on host arrays                               */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    hipStream_t s;
+    void *blk;
+    bbjd_red rd;
+    double *hscal;
+} helper_t;
+
+static int helper_open(helper_t *h, bbjd_int n, bbjd_int k, size_t extra, void **extra_ptr)
+{
+    memset(h, 0, sizeof(*h));
+    if (!HIPOK(hipStreamCreateWithFlags(&h->s, hipStreamNonBlocking))) return -1;
+    const int64_t pe = bbjd_red_part_entries(n, (int)(k + 2));
+    size_t off_part = 0, off_pn = off_part + (size_t)pe * 16;
+    size_t off_h = off_pn + ((size_t)(n + BBJD_PROJ_ROWS - 1) / BBJD_PROJ_ROWS + 64) * 8;
+    size_t kk = (size_t)(k + 8) * 16;
+    size_t off_h2 = off_h + kk, off_hs = off_h2 + kk, off_sc = off_hs + kk, off_fl = off_sc + 256,
+           off_co = off_fl + 64, off_c2 = off_co + kk, off_ex = (off_c2 + 64 * 16 + 255) / 256 * 256;
+    if (!HIPOK(hipMalloc(&h->blk, off_ex + extra))) { h->blk = NULL; return -1; }
+    char *b = (char *)h->blk;
+    h->rd.part = b + off_part; h->rd.pnorm = (double *)(b + off_pn); h->rd.h = b + off_h; h->rd.h2 = b + off_h2;
+    h->rd.hsum = b + off_hs; h->rd.scal = (double *)(b + off_sc); h->rd.flag = (int *)(b + off_fl);
+    h->rd.coef = b + off_co; h->rd.coef2 = b + off_c2; h->rd.part_cap = pe;
+    *extra_ptr = b + off_ex;
+    hipMemsetAsync(h->rd.flag, 0, 64, h->s);
+    if (!HIPOK(hipHostMalloc((void **)&h->hscal, 256, 0))) { h->hscal = NULL; return -1; }
+    return 0;
+}
+
+static void helper_close(helper_t *h)
+{
+    if (h->blk) hipFree(h->blk);
+    if (h->hscal) hipHostFree(h->hscal);
+    if (h->s) hipStreamDestroy(h->s);
+}
+
+double bbjd_block_orthogonalize(bbjd_int n, bbjd_int k, const bbjd_zomplex *Q, bbjd_zomplex *v,
+                                bbjd_zomplex *h_work)
+{
+    helper_t h;
+    void *dq = NULL;
+    const size_t qb = (size_t)n * (k > 0 ? k : 0) * 16, vb = (size_t)n * 16;
+    double res = NAN;
+    if (helper_open(&h, n, k > 0 ? k : 1, qb + vb + 256, &dq) == 0) {
+        void *dv = (char *)dq + (qb + 255) / 256 * 256;
+        int rc = 0;
+        if (k > 0) rc |= !HIPOK(hipMemcpyAsync(dq, Q, qb, hipMemcpyHostToDevice, h.s));
+        rc |= !HIPOK(hipMemcpyAsync(dv, v, vb, hipMemcpyHostToDevice, h.s));
+        rc |= bbjd_k_cgs2(h.s, n, (int)k, dq, n, dv, &h.rd, SL_T0, h.rd.hsum);
+        rc |= !HIPOK(hipMemcpyAsync(v, dv, vb, hipMemcpyDeviceToHost, h.s));
+        rc |= !HIPOK(hipMemcpyAsync(h.hscal, h.rd.scal, SL_COUNT * 8, hipMemcpyDeviceToHost, h.s));
+        if (h_work && k > 0) rc |= !HIPOK(hipMemcpyAsync(h_work, h.rd.hsum, (size_t)k * 16, hipMemcpyDeviceToHost, h.s));
+        rc |= !HIPOK(hipStreamSynchronize(h.s));
+        if (!rc) res = h.hscal[SL_T0];
+    }
+    helper_close(&h);
+    return res;
+}
+
+double bbjd_mgs_orthogonalize(bbjd_int n, bbjd_int k, const bbjd_zomplex *Q, bbjd_zomplex *v)
+{
+    return bbjd_block_orthogonalize(n, k, Q, v, NULL);
+}
+
+int bbjd_correction_gmres(bbjd_int n, const bbjd_zomplex *Q_right, const bbjd_zomplex *Q_left,
+                          bbjd_zomplex theta_a, bbjd_zomplex theta_b, const bbjd_zomplex *r, bbjd_zomplex *z,
+                          bbjd_matvec_fn matvec_A, bbjd_matvec_fn matvec_B, bbjd_precond_fn precond,
+                          void *op_data, bbjd_int maxiter, double tol, bbjd_workspace *ws)
+{
+    if (!ws || n != ws->n || maxiter <= 0 || maxiter > ws->inner_maxiter) return -1;
+    struct bbjd_dev *d = ws->dev;
+    bbjd_params P;
+    bbjd_params_init(&P);
+    P.n = n; P.nev = ws->nev; P.matvec_A = matvec_A; P.matvec_B = matvec_B; P.precond = precond;
+    P.op_data = op_data; P.inner_maxiter = maxiter; P.inner_tol = tol;
+    ctx_t cx = {ws, d, &P, d->stream, n, ws->ld, ws->maxsub, 0};
+    const bbjd_int ndefl = ws->nlock + 1;
+    const int devspace = d->opt.callback_space == BBJD_CALLBACK_DEVICE;
+    int rc = 0;
+    const void *qr = Q_right, *ql = Q_left, *dr = r;
+    void *dz = z;
+    if (!devspace) {
+        for (bbjd_int j = 0; j < ndefl; j++) {
+            rc |= !HIPOK(hipMemcpyAsync(COLZ_STATIC(ws->Q_right, j, ws->ld), (const zcx *)Q_right + j * n,
+                                        (size_t)n * 16, hipMemcpyHostToDevice, cx.s));
+            rc |= !HIPOK(hipMemcpyAsync(COLZ_STATIC(ws->Q_left, j, ws->ld), (const zcx *)Q_left + j * n,
+                                        (size_t)n * 16, hipMemcpyHostToDevice, cx.s));
+        }
+        rc |= !HIPOK(hipMemcpyAsync(ws->tmp4, r, (size_t)n * 16, hipMemcpyHostToDevice, cx.s));
+        qr = ws->Q_right; ql = ws->Q_left; dr = ws->tmp4; dz = ws->tmp2;
+    }   /* device space: Q_right/Q_left have leading dimension ws->ld */
+    if (d->opt.precision == BBJD_PREC_MIXED)
+        for (bbjd_int j = 0; j < ndefl; j++) {
+            rc |= bbjd_k_z2c(cx.s, n, (const zcx *)qr + j * ws->ld, (c64 *)d->qrc + j * ws->ld);
+            rc |= bbjd_k_z2c(cx.s, n, (const zcx *)ql + j * ws->ld, (c64 *)d->qlc + j * ws->ld);
+        }
+    if (rc) return -1;
+    int g = gmres(&cx, ndefl, qr, ql, theta_a, theta_b, dr, dz, maxiter, tol);
+    if (g) return -1;
+    if (!devspace) {
+        rc |= !HIPOK(hipMemcpyAsync(z, dz, (size_t)n * 16, hipMemcpyDeviceToHost, cx.s));
+        rc |= !HIPOK(hipStreamSynchronize(cx.s));
+    }
+    return rc ? -1 : 0;
+}

@@ -1,0 +1,672 @@
+// kernels.hip -- streaming kernels of the JDQZ outer loop for gfx950.
+//
+// Every n-sized step of one outer iteration (reference src/jd_core.c:
+// CGS2 207-241, project_out 269-309, Ritz vector 870-877, residual 887-892,
+// Sleijpen-van der Vorst step 952-971, pencil update 1085-1104, thick
+// restart 975-1030) is HBM-bound tall-skinny work on column-major tiles.
+//
+// Design for CDNA (64-wide wavefronts):
+//  * projections Q^H x: a block owns 1024 rows x C columns; each thread
+//    keeps 4 rows of x in registers and accumulates 2*NV*C = 32 real
+//    partial sums, reduced across the wavefront by a transposing register
+//    butterfly (31 lane exchanges for 32 values) instead of 32 separate
+//    6-step reductions, then across the 4 waves through 512 B of LDS;
+//  * second stage sums tile partials in a fixed order -> results do not
+//    depend on scheduling or on what else runs on the GPU;
+//  * updates x += alpha Q h stage h in LDS (broadcast reads) and fuse the
+//    squared norm of the result;
+//  * the DGKS decision (second pass iff ||v_after|| < 0.7071 ||v_before||)
+//    is taken on the device: pass-2 kernels read a flag and exit early, so
+//    an orthogonalisation needs no host round trip.
+//  * the same templates give the complex64 kernels (FP32 accumulation) of
+//    the mixed-precision inner solve.
+#include <hip/hip_runtime.h>
+#include "engine.h"
+
+typedef long long i64;
+
+template <typename R> struct alignas(2 * sizeof(R)) Cx { R re, im; };
+
+template <typename R> __device__ __forceinline__ Cx<R> cx(R a, R b) { Cx<R> c; c.re = a; c.im = b; return c; }
+template <typename R> __device__ __forceinline__ Cx<R> operator+(Cx<R> a, Cx<R> b) { return cx(a.re + b.re, a.im + b.im); }
+template <typename R> __device__ __forceinline__ Cx<R> operator-(Cx<R> a, Cx<R> b) { return cx(a.re - b.re, a.im - b.im); }
+template <typename R> __device__ __forceinline__ Cx<R> operator*(Cx<R> a, Cx<R> b)
+{ return cx(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re); }
+// a += conj(q) * x
+template <typename R> __device__ __forceinline__ void cfma_conj(R &are, R &aim, Cx<R> q, Cx<R> x)
+{ are += q.re * x.re + q.im * x.im; aim += q.re * x.im - q.im * x.re; }
+// a += q * h
+template <typename R> __device__ __forceinline__ void cfma(Cx<R> &a, Cx<R> q, Cx<R> h)
+{ a.re += q.re * h.re - q.im * h.im; a.im += q.re * h.im + q.im * h.re; }
+
+static constexpr int TPB = BBJD_TPB;
+static constexpr int PROJ_E = BBJD_PROJ_ROWS / TPB;   // rows per thread
+static constexpr int UPD_E = BBJD_UPD_ROWS / TPB;
+
+// Reduce 2^LS values per lane over the 64-lane wavefront.  Afterwards lane l
+// holds in v[0] the wave sum of value index (l >> (6 - LS)).
+template <typename R, int LS>
+__device__ __forceinline__ void wave_transpose_reduce(R (&v)[1 << LS])
+{
+    constexpr int S = 1 << LS;
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int st = 0; st < LS; st++) {
+        const int h = S >> (st + 1);
+        const int m = 32 >> st;
+        const bool up = (lane & m) != 0;
+#pragma unroll
+        for (int i = 0; i < h; i++) {
+            R send = up ? v[i] : v[i + h];
+            R keep = up ? v[i + h] : v[i];
+            v[i] = keep + __shfl_xor(send, m, 64);
+        }
+    }
+#pragma unroll
+    for (int m = (32 >> LS); m >= 1; m >>= 1) v[0] += __shfl_xor(v[0], m, 64);
+}
+
+template <typename R> __device__ __forceinline__ R wave_sum(R v)
+{
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+// Block-wide sum of one value per thread (TPB = 256 -> 4 waves); result valid
+// in thread 0.  Fixed order.
+template <typename R> __device__ __forceinline__ R block_sum(R v, R *lds4)
+{
+    v = wave_sum(v);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) lds4[w] = v;
+    __syncthreads();
+    R s = 0;
+    if (threadIdx.x == 0) s = (lds4[0] + lds4[1]) + (lds4[2] + lds4[3]);
+    return s;
+}
+
+// ---------------------------------------------------------------------------
+// Projection partials: part[(tile*NV + vi)*ldp + col] = sum over the tile's
+// rows of conj(Q[r, col]) x_vi[r]; pnorm[tile] = sum |x_0[r]|^2 when
+// want_norm (column-tile 0 blocks only).  grid = (col tiles, row tiles).
+template <typename R, int NV, int C>
+__global__ __launch_bounds__(TPB) void k_proj(i64 n, int k, const Cx<R> *__restrict__ Q, i64 ldq,
+                                              const Cx<R> *__restrict__ x0, const Cx<R> *__restrict__ x1,
+                                              Cx<R> *__restrict__ part, int ldp, R *__restrict__ pnorm,
+                                              int want_norm, const int *__restrict__ cond)
+{
+    if (cond && cond[0] == 0) return;
+    constexpr int NVAL = 2 * NV * C;
+    static_assert(NVAL == 32, "butterfly width");
+    constexpr int LS = 5;
+    __shared__ R lds[4][NVAL];
+    __shared__ R lds4[4];
+    const int t = threadIdx.x;
+    const int tile = blockIdx.y;
+    const i64 rb = (i64)tile * (TPB * PROJ_E);
+    const int c0 = blockIdx.x * C;
+
+    Cx<R> xv[NV][PROJ_E];
+#pragma unroll
+    for (int e = 0; e < PROJ_E; e++) {
+        const i64 r = rb + t + (i64)e * TPB;
+        xv[0][e] = r < n ? x0[r] : cx<R>(0, 0);
+        if (NV == 2) xv[NV - 1][e] = r < n ? x1[r] : cx<R>(0, 0);
+    }
+    R acc[NVAL];
+#pragma unroll
+    for (int i = 0; i < NVAL; i++) acc[i] = 0;
+#pragma unroll
+    for (int c = 0; c < C; c++) {
+        const int col = c0 + c;
+        if (col < k) {
+            const Cx<R> *q = Q + (i64)col * ldq;
+#pragma unroll
+            for (int e = 0; e < PROJ_E; e++) {
+                const i64 r = rb + t + (i64)e * TPB;
+                if (r < n) {
+                    const Cx<R> qq = q[r];
+#pragma unroll
+                    for (int vi = 0; vi < NV; vi++)
+                        cfma_conj(acc[2 * (vi * C + c)], acc[2 * (vi * C + c) + 1], qq, xv[vi][e]);
+                }
+            }
+        }
+    }
+    wave_transpose_reduce<R, LS>(acc);
+    const int lane = t & 63, w = t >> 6;
+    if ((lane & 1) == 0) lds[w][lane >> 1] = acc[0];
+    R nv = 0;
+    if (want_norm && blockIdx.x == 0) {
+#pragma unroll
+        for (int e = 0; e < PROJ_E; e++) nv += xv[0][e].re * xv[0][e].re + xv[0][e].im * xv[0][e].im;
+    }
+    __syncthreads();
+    if (t < NVAL) {
+        const R s = (lds[0][t] + lds[1][t]) + (lds[2][t] + lds[3][t]);
+        const int vi = t / (2 * C), c = (t / 2) % C, part_im = t & 1;
+        const int col = c0 + c;
+        if (col < k) {
+            R *dst = reinterpret_cast<R *>(part + ((i64)tile * NV + vi) * ldp + col);
+            dst[part_im] = s;
+        }
+    }
+    if (want_norm && blockIdx.x == 0) {
+        const R s = block_sum(nv, lds4);
+        if (t == 0) pnorm[tile] = s;
+    }
+}
+
+// out_v[c] = sum_t part[(t*nv + v)*ldp + c] for c < ncols (grid.x = ncols,
+// grid.y = nv); an extra block (grid.x == ncols + 1 with norm_slot >= 0)
+// turns the norm partials into scal[norm_slot] = sqrt(sum).  64 threads.
+template <typename R>
+__global__ __launch_bounds__(64) void k_reduce_part(const Cx<R> *__restrict__ part, int ntiles, int nv, int ldp,
+                                                    int ncols, Cx<R> *out0, Cx<R> *out1,
+                                                    const R *__restrict__ pnorm, double *scal, int norm_slot,
+                                                    const int *__restrict__ cond)
+{
+    if (cond && cond[0] == 0) return;
+    const int lane = threadIdx.x;
+    const int c = blockIdx.x, v = blockIdx.y;
+    if (c == ncols) {
+        if (v != 0 || norm_slot < 0) return;
+        R s = 0;
+        for (int t = lane; t < ntiles; t += 64) s += pnorm[t];
+        s = wave_sum(s);
+        if (lane == 0) scal[norm_slot] = sqrt((double)s);
+        return;
+    }
+    R sr = 0, si = 0;
+    for (int t = lane; t < ntiles; t += 64) {
+        const Cx<R> p = part[((i64)t * nv + v) * ldp + c];
+        sr += p.re; si += p.im;
+    }
+    sr = wave_sum(sr);
+    si = wave_sum(si);
+    if (lane == 0) (v == 0 ? out0 : out1)[c] = cx(sr, si);
+}
+
+// x = beta*x + alpha * Q[:, :k] h, squared norm partial per block.
+// Dynamic LDS: k complex.
+template <typename R>
+__global__ __launch_bounds__(TPB) void k_update(i64 n, int k, const Cx<R> *__restrict__ Q, i64 ldq,
+                                                const Cx<R> *__restrict__ h, R alpha, int beta_one,
+                                                Cx<R> *__restrict__ x, R *__restrict__ pnorm,
+                                                const int *__restrict__ cond)
+{
+    if (cond && cond[0] == 0) return;
+    extern __shared__ char smem_raw[];
+    Cx<R> *sh = reinterpret_cast<Cx<R> *>(smem_raw);
+    __shared__ R lds4[4];
+    const int t = threadIdx.x;
+    for (int j = t; j < k; j += TPB) { const Cx<R> hj = h[j]; sh[j] = cx(alpha * hj.re, alpha * hj.im); }
+    __syncthreads();
+    const i64 rb = (i64)blockIdx.x * (TPB * UPD_E);
+    Cx<R> a[UPD_E];
+#pragma unroll
+    for (int e = 0; e < UPD_E; e++) {
+        const i64 r = rb + t + (i64)e * TPB;
+        a[e] = (beta_one && r < n) ? x[r] : cx<R>(0, 0);
+    }
+    if (rb + TPB * UPD_E <= n) {
+#pragma unroll 4
+        for (int j = 0; j < k; j++) {
+            const Cx<R> hj = sh[j];
+            const Cx<R> *q = Q + (i64)j * ldq + rb + t;
+#pragma unroll
+            for (int e = 0; e < UPD_E; e++) cfma(a[e], q[(i64)e * TPB], hj);
+        }
+    } else {
+        for (int j = 0; j < k; j++) {
+            const Cx<R> hj = sh[j];
+            const Cx<R> *q = Q + (i64)j * ldq;
+#pragma unroll
+            for (int e = 0; e < UPD_E; e++) {
+                const i64 r = rb + t + (i64)e * TPB;
+                if (r < n) cfma(a[e], q[r], hj);
+            }
+        }
+    }
+    R s = 0;
+#pragma unroll
+    for (int e = 0; e < UPD_E; e++) {
+        const i64 r = rb + t + (i64)e * TPB;
+        if (r < n) { x[r] = a[e]; s += a[e].re * a[e].re + a[e].im * a[e].im; }
+    }
+    if (pnorm) {
+        s = block_sum(s, lds4);
+        if (t == 0) pnorm[blockIdx.x] = s;
+    }
+}
+
+// Squared-norm partials of x (grid-stride over UPD tiles).
+template <typename R>
+__global__ __launch_bounds__(TPB) void k_norm_part(i64 n, const Cx<R> *__restrict__ x, R *__restrict__ pnorm)
+{
+    __shared__ R lds4[4];
+    const i64 rb = (i64)blockIdx.x * (TPB * UPD_E);
+    R s = 0;
+#pragma unroll
+    for (int e = 0; e < UPD_E; e++) {
+        const i64 r = rb + threadIdx.x + (i64)e * TPB;
+        if (r < n) { const Cx<R> v = x[r]; s += v.re * v.re + v.im * v.im; }
+    }
+    s = block_sum(s, lds4);
+    if (threadIdx.x == 0) pnorm[blockIdx.x] = s;
+}
+
+// Final norm of an orthogonalisation pass / a plain norm.
+//  mode 0: scal[slot] = nrm
+//  mode 1: after CGS pass 1: scal[SL_NA] = scal[slot] = nrm;
+//          flag[FL_PASS2] = nrm < 0.7071 scal[SL_NB]
+//  mode 2: after CGS pass 2 (runs only when flagged): scal[slot] = nrm
+template <typename R>
+__global__ __launch_bounds__(64) void k_norm_finish(const R *__restrict__ pnorm, int ntiles, double *scal,
+                                                    int *flag, int mode, int slot, const int *__restrict__ cond)
+{
+    if (cond && cond[0] == 0) return;
+    R s = 0;
+    for (int t = threadIdx.x; t < ntiles; t += 64) s += pnorm[t];
+    s = wave_sum(s);
+    if (threadIdx.x == 0) {
+        const double nrm = sqrt((double)s);
+        scal[slot] = nrm;
+        if (mode == 1) {
+            scal[SL_NA] = nrm;
+            const int p2 = nrm < 0.7071 * scal[SL_NB];
+            flag[FL_PASS2] = p2;
+            if (p2) flag[FL_DGKS_COUNT] += 1;
+        }
+    }
+}
+
+template <typename R>
+__global__ void k_hsum(int k, const Cx<R> *h, const Cx<R> *h2, Cx<R> *hs, const int *flag)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < k) hs[j] = flag[FL_PASS2] ? h[j] + h2[j] : h[j];
+}
+
+template <typename R>
+__global__ __launch_bounds__(TPB) void k_scale_inv(i64 n, Cx<R> *x, const double *scal, int slot, int mode)
+{
+    const double nrm = scal[slot];
+    if (mode == 1 && !(nrm > 1e-300)) return;
+    const R s = (R)(1.0 / nrm);
+    for (i64 i = (i64)blockIdx.x * TPB + threadIdx.x; i < n; i += (i64)gridDim.x * TPB) {
+        Cx<R> v = x[i];
+        v.re *= s; v.im *= s;
+        x[i] = v;
+    }
+}
+
+template <typename R>
+__global__ __launch_bounds__(TPB) void k_scale(i64 n, Cx<R> *x, R are, R aim)
+{
+    const Cx<R> a = cx(are, aim);
+    for (i64 i = (i64)blockIdx.x * TPB + threadIdx.x; i < n; i += (i64)gridDim.x * TPB) x[i] = a * x[i];
+}
+
+template <typename R>
+__global__ __launch_bounds__(TPB) void k_lincomb(i64 n, R are, R aim, const Cx<R> *x, R bre, R bim,
+                                                 const Cx<R> *y, Cx<R> *out, R *pnorm)
+{
+    __shared__ R lds4[4];
+    const Cx<R> a = cx(are, aim), b = cx(bre, bim);
+    const i64 rb = (i64)blockIdx.x * (TPB * UPD_E);
+    R s = 0;
+#pragma unroll
+    for (int e = 0; e < UPD_E; e++) {
+        const i64 r = rb + threadIdx.x + (i64)e * TPB;
+        if (r < n) {
+            const Cx<R> v = a * x[r] + b * y[r];
+            out[r] = v;
+            s += v.re * v.re + v.im * v.im;
+        }
+    }
+    if (pnorm) {
+        s = block_sum(s, lds4);
+        if (threadIdx.x == 0) pnorm[blockIdx.x] = s;
+    }
+}
+
+// SvdV coefficient from the two dots d = (u^H z, u^H s): coef = -d0/d1 when
+// |d1| > 1e-300 (reference src/jd_core.c:966-971), else 0.
+__global__ void k_svdv_coef(Cx<double> *d)
+{
+    const Cx<double> uz = d[0], us = d[1];
+    const double a = hypot(us.re, us.im);
+    Cx<double> c = cx(0.0, 0.0);
+    if (a > 1e-300) {
+        const double den = us.re * us.re + us.im * us.im;
+        c = cx(-(uz.re * us.re + uz.im * us.im) / den, -(uz.im * us.re - uz.re * us.im) / den);
+    }
+    d[2] = c;
+}
+
+__global__ __launch_bounds__(TPB) void k_axpy_dev(i64 n, const Cx<double> *coef, const Cx<double> *s,
+                                                  Cx<double> *z)
+{
+    const Cx<double> c = coef[0];
+    for (i64 i = (i64)blockIdx.x * TPB + threadIdx.x; i < n; i += (i64)gridDim.x * TPB) {
+        Cx<double> v = z[i];
+        cfma(v, s[i], c);
+        z[i] = v;
+    }
+}
+
+// In-place basis rotation X[:, :keep] = X[:, :k] Z (Zr row-major k x keep).
+// A block stages RT rows x k columns of X in LDS, so every input of its rows
+// is read before any output is written; outputs go back through LDS for
+// column-contiguous stores.
+template <int RT>
+__global__ __launch_bounds__(TPB) void k_rotate(i64 n, int k, int keep, Cx<double> *X, i64 ldx,
+                                                const Cx<double> *__restrict__ Zr)
+{
+    extern __shared__ char smem_raw[];
+    Cx<double> *tin = reinterpret_cast<Cx<double> *>(smem_raw);     // k x RT
+    Cx<double> *tout = tin + (i64)k * RT;                            // keep x RT
+    const int t = threadIdx.x;
+    const i64 rb = (i64)blockIdx.x * RT;
+    for (int idx = t; idx < RT * k; idx += TPB) {
+        const int j = idx / RT, rr = idx % RT;
+        const i64 r = rb + rr;
+        tin[idx] = r < n ? X[(i64)j * ldx + r] : cx(0.0, 0.0);
+    }
+    __syncthreads();
+    constexpr int RPT = RT / 4;            // rows per thread
+    const int cl = t & 63, rg = t >> 6;    // column lane, row group
+    for (int cb = 0; cb < keep; cb += 64) {
+        const int c = cb + cl;
+        Cx<double> a[RPT];
+#pragma unroll
+        for (int q = 0; q < RPT; q++) a[q] = cx(0.0, 0.0);
+        if (c < keep) {
+            for (int j = 0; j < k; j++) {
+                const Cx<double> z = Zr[(i64)j * keep + c];
+#pragma unroll
+                for (int q = 0; q < RPT; q++) cfma(a[q], tin[j * RT + rg * RPT + q], z);
+            }
+#pragma unroll
+            for (int q = 0; q < RPT; q++) tout[c * RT + rg * RPT + q] = a[q];
+        }
+    }
+    __syncthreads();
+    for (int idx = t; idx < RT * keep; idx += TPB) {
+        const int c = idx / RT, rr = idx % RT;
+        const i64 r = rb + rr;
+        if (r < n) X[(i64)c * ldx + r] = tout[idx];
+    }
+}
+
+__global__ __launch_bounds__(TPB) void k_z2c(i64 n, const Cx<double> *z, Cx<float> *c)
+{
+    for (i64 i = (i64)blockIdx.x * TPB + threadIdx.x; i < n; i += (i64)gridDim.x * TPB)
+        c[i] = cx((float)z[i].re, (float)z[i].im);
+}
+__global__ __launch_bounds__(TPB) void k_c2z(i64 n, const Cx<float> *c, Cx<double> *z)
+{
+    for (i64 i = (i64)blockIdx.x * TPB + threadIdx.x; i < n; i += (i64)gridDim.x * TPB)
+        z[i] = cx((double)c[i].re, (double)c[i].im);
+}
+
+// ---------------------------------------------------------------------------
+// host launchers
+static inline int ntiles_proj(i64 n) { return (int)((n + BBJD_PROJ_ROWS - 1) / BBJD_PROJ_ROWS); }
+static inline int ntiles_upd(i64 n) { return (int)((n + BBJD_UPD_ROWS - 1) / BBJD_UPD_ROWS); }
+static inline int grid_stride_blocks(i64 n)
+{
+    i64 b = (n + TPB - 1) / TPB;
+    if (b > 8192) b = 8192;
+    if (b < 1) b = 1;
+    return (int)b;
+}
+static inline int chk(void) { return hipGetLastError() == hipSuccess ? 0 : -1; }
+
+extern "C" int64_t bbjd_red_part_entries(int64_t n, int maxk)
+{
+    return (int64_t)ntiles_proj(n) * 2 * (maxk + 1) + 64;
+}
+
+// one projection pass: part <- tiles, then h (and h2) <- reduced; optional
+// norm of x0 into scal[norm_slot].
+template <typename R>
+static int proj_pass(hipStream_t s, i64 n, int k, const void *Q, i64 ldq, const void *x0, const void *x1,
+                     void *out0, void *out1, bbjd_red *rd, int norm_slot, const int *cond)
+{
+    const int nt = ntiles_proj(n);
+    const int nv = x1 ? 2 : 1;
+    const int ldp = k > 0 ? k : 1;
+    if ((i64)nt * nv * ldp > rd->part_cap || k > BBJD_MAXK) return -2;
+    const int want_norm = norm_slot >= 0;
+    if (k > 0 || want_norm) {
+        if (nv == 1) {
+            constexpr int C = 16;
+            dim3 g(k > 0 ? (k + C - 1) / C : 1, nt);
+            hipLaunchKernelGGL((k_proj<R, 1, C>), g, dim3(TPB), 0, s, (i64)n, k, (const Cx<R> *)Q, (i64)ldq,
+                               (const Cx<R> *)x0, (const Cx<R> *)nullptr, (Cx<R> *)rd->part, ldp, (R *)rd->pnorm,
+                               want_norm, cond);
+        } else {
+            constexpr int C = 8;
+            dim3 g(k > 0 ? (k + C - 1) / C : 1, nt);
+            hipLaunchKernelGGL((k_proj<R, 2, C>), g, dim3(TPB), 0, s, (i64)n, k, (const Cx<R> *)Q, (i64)ldq,
+                               (const Cx<R> *)x0, (const Cx<R> *)x1, (Cx<R> *)rd->part, ldp, (R *)rd->pnorm,
+                               want_norm, cond);
+        }
+    }
+    dim3 gr(k + (want_norm ? 1 : 0), nv);
+    if (gr.x > 0)
+        hipLaunchKernelGGL((k_reduce_part<R>), gr, dim3(64), 0, s, (const Cx<R> *)rd->part, nt, nv, ldp, k,
+                           (Cx<R> *)out0, (Cx<R> *)out1, (const R *)rd->pnorm, rd->scal, norm_slot, cond);
+    return chk();
+}
+
+template <typename R>
+static int update_pass(hipStream_t s, i64 n, int k, const void *Q, i64 ldq, const void *h, R alpha, int beta_one,
+                       void *x, R *pnorm, const int *cond)
+{
+    if (k > BBJD_MAXK) return -2;
+    const int nt = ntiles_upd(n);
+    const size_t lds = (size_t)(k > 0 ? k : 1) * sizeof(Cx<R>);
+    hipLaunchKernelGGL((k_update<R>), dim3(nt), dim3(TPB), lds, s, (i64)n, k, (const Cx<R> *)Q, (i64)ldq,
+                       (const Cx<R> *)h, alpha, beta_one, (Cx<R> *)x, pnorm, cond);
+    return chk();
+}
+
+template <typename R>
+static int norm_into(hipStream_t s, i64 n, const void *x, bbjd_red *rd, int slot)
+{
+    const int nt = ntiles_upd(n);
+    hipLaunchKernelGGL((k_norm_part<R>), dim3(nt), dim3(TPB), 0, s, (i64)n, (const Cx<R> *)x, (R *)rd->pnorm);
+    hipLaunchKernelGGL((k_norm_finish<R>), dim3(1), dim3(64), 0, s, (const R *)rd->pnorm, nt, rd->scal, rd->flag,
+                       0, slot, (const int *)nullptr);
+    return chk();
+}
+
+template <typename R>
+static int cgs2(hipStream_t s, i64 n, int k, const void *Q, i64 ldq, void *v, bbjd_red *rd, int out_slot,
+                void *hsum)
+{
+    if (k == 0) return norm_into<R>(s, n, v, rd, out_slot);
+    const int ntu = ntiles_upd(n);
+    int rc = 0;
+    // pass 1: h = Q^H v with ||v|| -> SL_NB; v -= Q h; ||v|| -> SL_NA, out_slot, flag
+    rc |= proj_pass<R>(s, n, k, Q, ldq, v, nullptr, rd->h, nullptr, rd, SL_NB, nullptr);
+    rc |= update_pass<R>(s, n, k, Q, ldq, rd->h, (R)-1, 1, v, (R *)rd->pnorm, nullptr);
+    hipLaunchKernelGGL((k_norm_finish<R>), dim3(1), dim3(64), 0, s, (const R *)rd->pnorm, ntu, rd->scal, rd->flag,
+                       1, out_slot, (const int *)nullptr);
+    // pass 2, only when flagged on the device
+    const int *cond = rd->flag + FL_PASS2;
+    rc |= proj_pass<R>(s, n, k, Q, ldq, v, nullptr, rd->h2, nullptr, rd, -1, cond);
+    rc |= update_pass<R>(s, n, k, Q, ldq, rd->h2, (R)-1, 1, v, (R *)rd->pnorm, cond);
+    hipLaunchKernelGGL((k_norm_finish<R>), dim3(1), dim3(64), 0, s, (const R *)rd->pnorm, ntu, rd->scal, rd->flag,
+                       2, out_slot, cond);
+    if (hsum)
+        hipLaunchKernelGGL((k_hsum<R>), dim3((k + 63) / 64), dim3(64), 0, s, k, (const Cx<R> *)rd->h,
+                           (const Cx<R> *)rd->h2, (Cx<R> *)hsum, (const int *)rd->flag);
+    return rc | chk();
+}
+
+template <typename R>
+static int project_out(hipStream_t s, i64 n, int k, const void *Q, i64 ldq, void *v, bbjd_red *rd)
+{
+    if (k == 0) return 0;
+    int rc = 0;
+    for (int pass = 0; pass < 2; pass++) {
+        rc |= proj_pass<R>(s, n, k, Q, ldq, v, nullptr, rd->h, nullptr, rd, -1, nullptr);
+        rc |= update_pass<R>(s, n, k, Q, ldq, rd->h, (R)-1, 1, v, (R *)nullptr, nullptr);
+    }
+    return rc;
+}
+
+template <typename R>
+static int gemv_n(hipStream_t s, i64 n, int k, const void *Q, i64 ldq, const void *c, void *y, bbjd_red *rd,
+                  int slot)
+{
+    int rc = update_pass<R>(s, n, k, Q, ldq, c, (R)1, 0, y, (R *)rd->pnorm, nullptr);
+    hipLaunchKernelGGL((k_norm_finish<R>), dim3(1), dim3(64), 0, s, (const R *)rd->pnorm, ntiles_upd(n), rd->scal,
+                       rd->flag, 0, slot, (const int *)nullptr);
+    return rc | chk();
+}
+
+template <typename R>
+static int lincomb(hipStream_t s, i64 n, double are, double aim, const void *x, double bre, double bim,
+                   const void *y, void *out, bbjd_red *rd, int slot)
+{
+    const int nt = ntiles_upd(n);
+    hipLaunchKernelGGL((k_lincomb<R>), dim3(nt), dim3(TPB), 0, s, (i64)n, (R)are, (R)aim, (const Cx<R> *)x,
+                       (R)bre, (R)bim, (const Cx<R> *)y, (Cx<R> *)out, rd ? (R *)rd->pnorm : (R *)nullptr);
+    if (rd)
+        hipLaunchKernelGGL((k_norm_finish<R>), dim3(1), dim3(64), 0, s, (const R *)rd->pnorm, nt, rd->scal,
+                           rd->flag, 0, slot, (const int *)nullptr);
+    return chk();
+}
+
+extern "C" {
+
+int bbjd_k_cgs2(hipStream_t s, int64_t n, int k, const void *Q, int64_t ldq, void *v, bbjd_red *rd,
+                int out_slot, void *hsum)
+{ return cgs2<double>(s, n, k, Q, ldq, v, rd, out_slot, hsum); }
+
+int bbjd_k_cgs2_c(hipStream_t s, int64_t n, int k, const void *Q, int64_t ldq, void *v, bbjd_red *rd,
+                  int out_slot, void *hsum)
+{ return cgs2<float>(s, n, k, Q, ldq, v, rd, out_slot, hsum); }
+
+int bbjd_k_project_out(hipStream_t s, int64_t n, int k, const void *Q, int64_t ldq, void *v, bbjd_red *rd)
+{ return project_out<double>(s, n, k, Q, ldq, v, rd); }
+
+int bbjd_k_project_out_c(hipStream_t s, int64_t n, int k, const void *Q, int64_t ldq, void *v, bbjd_red *rd)
+{ return project_out<float>(s, n, k, Q, ldq, v, rd); }
+
+int bbjd_k_gemv_c(hipStream_t s, int64_t n, int k, const void *Q, int64_t ldq, const void *x, const void *x2,
+                  void *out, void *out2, bbjd_red *rd)
+{ return proj_pass<double>(s, n, k, Q, ldq, x, x2, out, out2, rd, -1, nullptr); }
+
+int bbjd_k_gemv_c_c(hipStream_t s, int64_t n, int k, const void *Q, int64_t ldq, const void *x, const void *x2,
+                    void *out, void *out2, bbjd_red *rd)
+{ return proj_pass<float>(s, n, k, Q, ldq, x, x2, out, out2, rd, -1, nullptr); }
+
+int bbjd_k_gemv_n(hipStream_t s, int64_t n, int k, const void *Q, int64_t ldq, const void *c, void *y,
+                  bbjd_red *rd, int slot)
+{ return gemv_n<double>(s, n, k, Q, ldq, c, y, rd, slot); }
+
+int bbjd_k_gemv_n_c(hipStream_t s, int64_t n, int k, const void *Q, int64_t ldq, const void *c, void *y,
+                    bbjd_red *rd, int slot)
+{ return gemv_n<float>(s, n, k, Q, ldq, c, y, rd, slot); }
+
+int bbjd_k_norm(hipStream_t s, int64_t n, const void *x, bbjd_red *rd, int slot)
+{ return norm_into<double>(s, n, x, rd, slot); }
+
+int bbjd_k_norm_c(hipStream_t s, int64_t n, const void *x, bbjd_red *rd, int slot)
+{ return norm_into<float>(s, n, x, rd, slot); }
+
+int bbjd_k_scale_inv(hipStream_t s, int64_t n, void *x, const double *scal, int slot, int mode)
+{
+    hipLaunchKernelGGL((k_scale_inv<double>), dim3(grid_stride_blocks(n)), dim3(TPB), 0, s, (i64)n,
+                       (Cx<double> *)x, scal, slot, mode);
+    return chk();
+}
+
+int bbjd_k_scale(hipStream_t s, int64_t n, void *x, double are, double aim)
+{
+    hipLaunchKernelGGL((k_scale<double>), dim3(grid_stride_blocks(n)), dim3(TPB), 0, s, (i64)n, (Cx<double> *)x,
+                       are, aim);
+    return chk();
+}
+
+int bbjd_k_scale_c(hipStream_t s, int64_t n, void *x, double are, double aim)
+{
+    hipLaunchKernelGGL((k_scale<float>), dim3(grid_stride_blocks(n)), dim3(TPB), 0, s, (i64)n, (Cx<float> *)x,
+                       (float)are, (float)aim);
+    return chk();
+}
+
+int bbjd_k_lincomb(hipStream_t s, int64_t n, double are, double aim, const void *x, double bre, double bim,
+                   const void *y, void *out)
+{ return lincomb<double>(s, n, are, aim, x, bre, bim, y, out, nullptr, 0); }
+
+int bbjd_k_lincomb_c(hipStream_t s, int64_t n, double are, double aim, const void *x, double bre, double bim,
+                     const void *y, void *out)
+{ return lincomb<float>(s, n, are, aim, x, bre, bim, y, out, nullptr, 0); }
+
+int bbjd_k_lincomb_norm(hipStream_t s, int64_t n, double are, double aim, const void *x, double bre, double bim,
+                        const void *y, void *r, bbjd_red *rd, int slot)
+{ return lincomb<double>(s, n, are, aim, x, bre, bim, y, r, rd, slot); }
+
+int bbjd_k_svdv(hipStream_t s, int64_t n, const void *u, void *z, const void *sv, bbjd_red *rd)
+{
+    Cx<double> *d = (Cx<double> *)rd->coef2;
+    int rc = proj_pass<double>(s, n, 1, u, n, z, sv, d, d + 1, rd, -1, nullptr);
+    hipLaunchKernelGGL(k_svdv_coef, dim3(1), dim3(1), 0, s, d);
+    hipLaunchKernelGGL(k_axpy_dev, dim3(grid_stride_blocks(n)), dim3(TPB), 0, s, (i64)n, (const Cx<double> *)(d + 2),
+                       (const Cx<double> *)sv, (Cx<double> *)z);
+    return rc | chk();
+}
+
+int bbjd_k_rotate(hipStream_t s, int64_t n, int k, int keep, void *X, int64_t ldx, const void *Zr)
+{
+    if (keep > k || keep <= 0) return -2;
+    int rt = 16;
+    while (rt > 4 && (size_t)rt * (k + keep) * 16 > 64 * 1024) rt /= 2;
+    if ((size_t)rt * (k + keep) * 16 > 64 * 1024) return -2;
+    const size_t lds = (size_t)rt * (k + keep) * 16;
+    const int nb = (int)((n + rt - 1) / rt);
+    if (rt == 16)
+        hipLaunchKernelGGL((k_rotate<16>), dim3(nb), dim3(TPB), lds, s, (i64)n, k, keep, (Cx<double> *)X, (i64)ldx,
+                           (const Cx<double> *)Zr);
+    else if (rt == 8)
+        hipLaunchKernelGGL((k_rotate<8>), dim3(nb), dim3(TPB), lds, s, (i64)n, k, keep, (Cx<double> *)X, (i64)ldx,
+                           (const Cx<double> *)Zr);
+    else
+        hipLaunchKernelGGL((k_rotate<4>), dim3(nb), dim3(TPB), lds, s, (i64)n, k, keep, (Cx<double> *)X, (i64)ldx,
+                           (const Cx<double> *)Zr);
+    return chk();
+}
+
+int bbjd_k_gram(hipStream_t s, int64_t n, int m, const void *X, const void *Y, int64_t ld, void *G, int64_t ldg,
+                bbjd_red *rd)
+{
+    int rc = 0;
+    for (int j = 0; j < m; j++)
+        rc |= proj_pass<double>(s, n, m, X, ld, (const Cx<double> *)Y + (i64)j * ld, nullptr,
+                                (Cx<double> *)G + (i64)j * ldg, nullptr, rd, -1, nullptr);
+    return rc;
+}
+
+int bbjd_k_z2c(hipStream_t s, int64_t n, const void *z, void *c)
+{
+    hipLaunchKernelGGL(k_z2c, dim3(grid_stride_blocks(n)), dim3(TPB), 0, s, (i64)n, (const Cx<double> *)z,
+                       (Cx<float> *)c);
+    return chk();
+}
+
+int bbjd_k_c2z(hipStream_t s, int64_t n, const void *c, void *z)
+{
+    hipLaunchKernelGGL(k_c2z, dim3(grid_stride_blocks(n)), dim3(TPB), 0, s, (i64)n, (const Cx<float> *)c,
+                       (Cx<double> *)z);
+    return chk();
+}
+
+}  // extern "C"

@@ -1,0 +1,23 @@
+/* ops_dev.h -- launchers of ops.hip (device operators), plain C ABI. */
+#ifndef BBJD_OPS_DEV_H
+#define BBJD_OPS_DEV_H
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+int bbjd_o_band_mv(hipStream_t s, int64_t n1, int64_t nb, int bs, int hw, const void *band, const void *x, void *y,
+                   int single);
+int bbjd_o_diag_mv(hipStream_t s, int64_t n, const void *dg, const void *x, void *y, int single);
+int bbjd_o_bj_apply(hipStream_t s, int64_t n, int bs, const void *inv, const void *x, void *y, int single);
+int bbjd_o_bcr_fwd(hipStream_t st, int s, int64_t M, int64_t h, int64_t ne, const void *AG, void *b);
+int bbjd_o_bcr_bwd(hipStream_t st, int s, int64_t M, int64_t h, int64_t no, const void *DPQ, const void *b, void *x);
+int bbjd_o_bcr_final(hipStream_t st, int s, const void *D, const void *b, void *x);
+int bbjd_o_border_small(hipStream_t st, int ntau, const void *b2, const void *dots, const void *Sinv, void *y2,
+                        void *ytail);
+int bbjd_o_border_update(hipStream_t st, int64_t n1, int ntau, const void *xb, const void *Y, const void *y2,
+                         void *y);
+#ifdef __cplusplus
+}
+#endif
+#endif

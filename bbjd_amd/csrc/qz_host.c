@@ -1,0 +1,182 @@
+/*
+ * qz_host.c -- the projected generalized Schur step on the host, FP64.
+ *
+ * Follows reference src/jd_core.c:319-462 (solve_projected): ZGGES of the
+ * k x k projected pencil (MA, MB), selection of the eigenvalues matching
+ * converged ones (absolute |lambda - lambda_c| < 100 tol) plus the nkeep best
+ * by the target key, ZTGSEN(ijob = 0) to move the selection to the leading
+ * block, and the target position as the best unconverged value over ALL
+ * positions.  Same LAPACK routines, so the Schur data equal the reference's
+ * for equal input.
+ *
+ * Engine-side differences (results unchanged):
+ *  - LAPACK workspaces are sized once per workspace, not malloc'd per call;
+ *  - the ZGGES output is cached: on a lock iteration the reference leaves
+ *    (MA, MB) unchanged (`continue` at src/jd_core.c:931) so the next ZGGES
+ *    would recompute the same factors; the cache replays them and only the
+ *    selection and ZTGSEN run again.
+ */
+#include "qz_host.h"
+#include "lapack_host.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+int bbjd_qz_init(bbjd_qz *q, bbjd_int maxsub)
+{
+    memset(q, 0, sizeof(*q));
+    q->maxsub = maxsub;
+    size_t ss = (size_t)maxsub * maxsub;
+    q->cSA = malloc(ss * sizeof(zcplx)); q->cSB = malloc(ss * sizeof(zcplx));
+    q->cZA = malloc(ss * sizeof(zcplx)); q->cZB = malloc(ss * sizeof(zcplx));
+    q->cal = malloc((size_t)maxsub * sizeof(zcplx)); q->cbe = malloc((size_t)maxsub * sizeof(zcplx));
+    q->rwork = malloc((size_t)8 * maxsub * sizeof(double));
+    q->bwork = malloc((size_t)maxsub * sizeof(blas_int));
+    q->sel = malloc((size_t)maxsub * sizeof(blas_int));
+    /* workspace queries at the largest size */
+    blas_int bk = (blas_int)maxsub, lda = bk, info = 0, sdim = 0, lw = -1;
+    zcplx wq = 0;
+    zcplx *A = calloc(ss, sizeof(zcplx)), *B = calloc(ss, sizeof(zcplx));
+    scipy_zgges_("V", "V", "N", NULL, &bk, A, &lda, B, &lda, &sdim, q->cal, q->cbe, q->cZA, &lda, q->cZB,
+                 &lda, &wq, &lw, q->rwork, q->bwork, &info);
+    q->lwork = (blas_int)creal(wq) + 1;
+    blas_int ijob = 0, one = 1, m = 0, tli = -1, tiq = 0;
+    double pl, pr, dif[2];
+    lw = -1;
+    for (bbjd_int i = 0; i < maxsub; i++) q->sel[i] = 1;
+    scipy_ztgsen_(&ijob, &one, &one, q->sel, &bk, A, &lda, B, &lda, q->cal, q->cbe, q->cZA, &lda, q->cZB, &lda,
+                  &m, &pl, &pr, dif, &wq, &lw, &tiq, &tli, &info);
+    blas_int tl = (blas_int)creal(wq) + 1;
+    if (tl > q->lwork) q->lwork = tl;
+    q->liwork = tiq + 1 > 1 ? tiq + 1 : 1;
+    if (q->lwork < 2 * maxsub + 64) q->lwork = 2 * maxsub + 64;
+    q->work = malloc((size_t)q->lwork * sizeof(zcplx));
+    q->iwork = malloc((size_t)q->liwork * sizeof(blas_int));
+    free(A); free(B);
+    if (!q->cSA || !q->cSB || !q->cZA || !q->cZB || !q->cal || !q->cbe || !q->rwork || !q->bwork ||
+        !q->sel || !q->work || !q->iwork) {
+        bbjd_qz_free(q);
+        return -1;
+    }
+    return 0;
+}
+
+void bbjd_qz_free(bbjd_qz *q)
+{
+    free(q->cSA); free(q->cSB); free(q->cZA); free(q->cZB); free(q->cal); free(q->cbe);
+    free(q->rwork); free(q->bwork); free(q->sel); free(q->work); free(q->iwork);
+    memset(q, 0, sizeof(*q));
+}
+
+/* target key (reference src/jd_core.c:387-396) */
+static double key_of(bbjd_target t, zcplx lam, zcplx sigma)
+{
+    switch (t) {
+    case BBJD_TARGET_LARGEST_MAG:   return -cabs(lam);
+    case BBJD_TARGET_SMALLEST_MAG:  return cabs(lam);
+    case BBJD_TARGET_LARGEST_REAL:  return -creal(lam);
+    case BBJD_TARGET_SMALLEST_REAL: return creal(lam);
+    case BBJD_TARGET_LARGEST_IMAG:  return -cimag(lam);
+    case BBJD_TARGET_SMALLEST_IMAG: return cimag(lam);
+    default:                        return cabs(lam - sigma);
+    }
+}
+
+static int is_converged(zcplx lam, const zcplx *conv, bbjd_int nconv, double tol)
+{
+    for (bbjd_int c = 0; c < nconv; c++)
+        if (cabs(lam - conv[c]) < tol * 100) return 1;
+    return 0;
+}
+
+int bbjd_qz_solve(bbjd_qz *q, bbjd_int k, const zcplx *MA, const zcplx *MB, zcplx *SA, zcplx *SB, zcplx *ZA,
+                  zcplx *ZB, zcplx *al, zcplx *be, zcplx sigma, bbjd_target tgt, bbjd_int nkeep,
+                  const zcplx *conv, bbjd_int nconv, double tol, bbjd_int *tpos)
+{
+    const bbjd_int ld = q->maxsub;
+    blas_int bk = (blas_int)k, lda = (blas_int)ld, info = 0;
+    const size_t colb = (size_t)k * sizeof(zcplx);
+    if (q->cache_valid && q->cache_k == k) {
+        q->cache_hits++;
+        for (bbjd_int j = 0; j < k; j++) {
+            memcpy(SA + j * ld, q->cSA + j * ld, colb); memcpy(SB + j * ld, q->cSB + j * ld, colb);
+            memcpy(ZA + j * ld, q->cZA + j * ld, colb); memcpy(ZB + j * ld, q->cZB + j * ld, colb);
+        }
+        memcpy(al, q->cal, colb); memcpy(be, q->cbe, colb);
+    } else {
+        for (bbjd_int j = 0; j < k; j++) {
+            memcpy(SA + j * ld, MA + j * ld, colb);
+            memcpy(SB + j * ld, MB + j * ld, colb);
+        }
+        blas_int sdim = 0, lw = q->lwork;
+        scipy_zgges_("V", "V", "N", NULL, &bk, SA, &lda, SB, &lda, &sdim, al, be, ZA, &lda, ZB, &lda, q->work,
+                     &lw, q->rwork, q->bwork, &info);
+        if (info) return -1;
+        q->gges_calls++;
+        for (bbjd_int j = 0; j < k; j++) {
+            memcpy(q->cSA + j * ld, SA + j * ld, colb); memcpy(q->cSB + j * ld, SB + j * ld, colb);
+            memcpy(q->cZA + j * ld, ZA + j * ld, colb); memcpy(q->cZB + j * ld, ZB + j * ld, colb);
+        }
+        memcpy(q->cal, al, colb); memcpy(q->cbe, be, colb);
+        q->cache_valid = 1;
+        q->cache_k = k;
+    }
+
+    blas_int *sel = q->sel;
+    for (bbjd_int i = 0; i < k; i++) {
+        sel[i] = 0;
+        if (cabs(be[i]) < 1e-300) continue;
+        if (is_converged(al[i] / be[i], conv, nconv, tol)) sel[i] = 1;
+    }
+    for (bbjd_int pick = 0; pick < nkeep; pick++) {
+        bbjd_int best = -1;
+        double bv = DBL_MAX;
+        for (bbjd_int i = 0; i < k; i++) {
+            if (sel[i] || cabs(be[i]) < 1e-300) continue;
+            double v = key_of(tgt, al[i] / be[i], sigma);
+            if (v < bv) { bv = v; best = i; }
+        }
+        if (best < 0) break;
+        sel[best] = 1;
+    }
+    blas_int nsel = 0;
+    for (bbjd_int i = 0; i < k; i++) nsel += sel[i];
+    if (nsel > 0) {
+        blas_int ijob = 0, wq = 1, wz = 1, m = 0, lw = q->lwork, li = q->liwork;
+        double pl, pr, dif[2];
+        scipy_ztgsen_(&ijob, &wq, &wz, sel, &bk, SA, &lda, SB, &lda, al, be, ZA, &lda, ZB, &lda, &m, &pl, &pr,
+                      dif, q->work, &lw, q->iwork, &li, &info);
+        if (info) return -1;
+    }
+    bbjd_int p = 0;
+    double bv = DBL_MAX;
+    for (bbjd_int i = 0; i < k; i++) {
+        if (cabs(be[i]) < 1e-300) continue;
+        zcplx lam = al[i] / be[i];
+        if (is_converged(lam, conv, nconv, tol)) continue;
+        double v = key_of(tgt, lam, sigma);
+        if (v < bv) { bv = v; p = i; }
+    }
+    *tpos = p;
+    return 0;
+}
+
+/* Ritz coefficients (reference src/jd_core.c:854-870): back-substitution
+ * (tb SA - ta SB)[0:p,0:p] y = -(tb SA - ta SB)[0:p,p], y[p] = 1, then
+ * x = ZB[0:k, 0:p+1] y by ZGEMV.  y, x need maxsub entries each. */
+void bbjd_ritz_coefficients(bbjd_int k, bbjd_int ld, bbjd_int p, zcplx ta, zcplx tb, const zcplx *SA,
+                            const zcplx *SB, const zcplx *ZB, zcplx *y, zcplx *x)
+{
+    y[p] = 1.0;
+    for (bbjd_int i = p - 1; i >= 0; i--) {
+        zcplx acc = 0.0;
+        for (bbjd_int j = i + 1; j <= p; j++) acc += (tb * SA[i + j * ld] - ta * SB[i + j * ld]) * y[j];
+        zcplx d = tb * SA[i + i * ld] - ta * SB[i + i * ld];
+        y[i] = (cabs(d) > 1e-300) ? -acc / d : 0.0;
+    }
+    blas_int bk = (blas_int)k, bp1 = (blas_int)(p + 1), ldz = (blas_int)ld, one = 1;
+    const zcplx a1 = 1.0, b0 = 0.0;
+    scipy_zgemv_("N", &bk, &bp1, &a1, ZB, &ldz, y, &one, &b0, x, &one);
+}

@@ -1,0 +1,148 @@
+"""Block-banded pencils with dense tau rows, and their device operators.
+
+``BandedTauPencil`` holds the host arrays in the layout of
+include/bbjd_ops.h; ``generate`` draws the seeded benchmark pencils (the C
+generator of the library, so Python, C and the benchmarks see identical
+matrices).  ``DeviceOperator`` uploads a pencil and builds the device
+matvec/preconditioner for a shift; its C callbacks are what the engine calls
+in DEVICE callback space.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+
+
+class BandedTauPencil:
+    """A = block-banded (block size bs, block half-width hw) with the last
+    ntau rows dense; B = diag(bdiag)."""
+
+    def __init__(self, n, bs, hw, ntau, band, tau, bdiag):
+        self.n, self.bs, self.hw, self.ntau = int(n), int(bs), int(hw), int(ntau)
+        if self.n % self.bs:
+            raise ValueError("n must be a multiple of the block size")
+        nb, nd = self.n // self.bs, 2 * self.hw + 1
+        self.band = np.ascontiguousarray(band, dtype=np.complex128).reshape(nb * nd * bs * bs)
+        self.tau = np.ascontiguousarray(tau, dtype=np.complex128).reshape(max(self.ntau, 0) * self.n)
+        self.bdiag = np.ascontiguousarray(bdiag, dtype=np.complex128).reshape(self.n)
+
+    @classmethod
+    def generate(cls, n, bs, hw, ntau, seed, diag_shift=4.0):
+        lib = _lib.load()
+        n, bs, hw, ntau = int(n), int(bs), int(hw), int(ntau)
+        nb, nd = n // bs, 2 * hw + 1
+        band = np.empty(nb * nd * bs * bs, dtype=np.complex128)
+        tau = np.empty(max(ntau, 1) * n, dtype=np.complex128)
+        bdiag = np.empty(n, dtype=np.complex128)
+        rc = lib.bbjd_pencil_generate(n, bs, hw, ntau, C.c_uint64(seed), float(diag_shift),
+                                      band.ctypes.data, tau.ctypes.data, bdiag.ctypes.data)
+        if rc:
+            raise ValueError("invalid pencil shape")
+        return cls(n, bs, hw, ntau, band, tau[: ntau * n], bdiag)
+
+    @property
+    def n1(self):
+        return self.n - self.ntau
+
+    def desc(self):
+        d = _lib.PencilDesc()
+        d.n, d.bs, d.hw, d.ntau = self.n, self.bs, self.hw, self.ntau
+        d.band = self.band.ctypes.data
+        d.tau = self.tau.ctypes.data if self.ntau else None
+        d.bdiag = self.bdiag.ctypes.data
+        return d
+
+    def to_scipy(self):
+        """(A, B) as scipy CSR matrices (for host-side checks)."""
+        import scipy.sparse as sp
+        n, bs, hw = self.n, self.bs, self.hw
+        nb, nd = n // bs, 2 * hw + 1
+        blk = self.band.reshape(nb, nd, bs, bs)         # [bi, d, c, r]
+        rows, cols, vals = [], [], []
+        bi, d, c, r = np.meshgrid(np.arange(nb), np.arange(nd), np.arange(bs), np.arange(bs),
+                                  indexing="ij")
+        bj = bi + d - hw
+        i = bi * bs + r
+        j = bj * bs + c
+        keep = (bj >= 0) & (bj < nb) & (i < self.n1)
+        rows.append(i[keep]); cols.append(j[keep]); vals.append(blk[keep])
+        if self.ntau:
+            t = self.tau.reshape(self.ntau, n)
+            ti, tj = np.meshgrid(np.arange(self.ntau), np.arange(n), indexing="ij")
+            rows.append((self.n1 + ti).ravel()); cols.append(tj.ravel()); vals.append(t.ravel())
+        A = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                          shape=(n, n))
+        B = sp.diags(self.bdiag, format="csr")
+        return A, B
+
+
+_PC = {"none": _lib.PC_NONE, None: _lib.PC_NONE, "exact": _lib.PC_EXACT_LU,
+       "exact_lu": _lib.PC_EXACT_LU, "block_jacobi": _lib.PC_BLOCK_JACOBI}
+
+
+class DeviceOperator:
+    """The pencil on the GPU with the preconditioner of A - sigma B.
+
+    pc: 'exact' (block cyclic reduction + Schur complement of the tau rows),
+    'block_jacobi' (bs x bs diagonal-block inverses) or 'none'.
+    c64: also keep complex64 copies for the mixed-precision inner solve.
+    """
+
+    def __init__(self, pencil: BandedTauPencil, sigma=0.0, pc="exact", c64=False, stream=None):
+        lib = _lib.load()
+        if pc not in _PC:
+            raise ValueError(f"pc must be one of {list(k for k in _PC if k)}")
+        self.pencil = pencil
+        self.n = pencil.n
+        self.sigma = complex(sigma)
+        self.pc = _PC[pc]
+        self.c64 = bool(c64)
+        self._desc = pencil.desc()
+        h = lib.bbjd_op_create(C.byref(self._desc), _lib.zc(sigma), self.pc, int(self.c64), stream)
+        if not h:
+            raise RuntimeError("bbjd_op_create failed (device memory or a singular pivot block)")
+        self.handle = h
+
+    @property
+    def shape(self):
+        return (self.n, self.n)
+
+    @property
+    def device_bytes(self):
+        return _lib.load().bbjd_op_bytes(self.handle)
+
+    def free(self):
+        if getattr(self, "handle", None):
+            _lib.load().bbjd_op_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    # host-array convenience applies (copy in, apply on the device, copy out)
+    def _apply(self, name, x):
+        import torch
+        x = np.ascontiguousarray(x, dtype=np.complex128)
+        dx = torch.from_numpy(x).to("cuda")
+        dy = torch.empty_like(dx)
+        torch.cuda.synchronize()
+        rc = getattr(_lib.load(), name)(self.n, dx.data_ptr(), dy.data_ptr(), self.handle)
+        torch.cuda.synchronize()
+        if rc:
+            raise RuntimeError(f"{name} failed")
+        return dy.cpu().numpy()
+
+    def matvec(self, x):
+        return self._apply("bbjd_op_matvec_A", x)
+
+    def matvec_B(self, x):
+        return self._apply("bbjd_op_matvec_B", x)
+
+    def solve(self, x):
+        return self._apply("bbjd_op_precond", x)

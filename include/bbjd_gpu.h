@@ -1,0 +1,86 @@
+/*
+ * bbjd_gpu.h -- engine extensions beyond the reference C surface.
+ *
+ * None of these has a counterpart in the reference (kyle-augustson/BBJD);
+ * they select how the MI355X engine runs a solve.  bbjd_workspace_alloc()
+ * in bbjd.h is bbjd_workspace_alloc_gpu(p, NULL): host callbacks, FP64.
+ */
+#ifndef BBJD_GPU_H
+#define BBJD_GPU_H
+
+#include "bbjd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Where the callback vectors x, y live. */
+typedef enum {
+    BBJD_CALLBACK_HOST   = 0,  /* host memory; the engine stages D2H/H2D    */
+    BBJD_CALLBACK_DEVICE = 1   /* device pointers, ordered on the engine's
+                                  stream (bbjd_callback_stream())            */
+} bbjd_callback_space;
+
+/* Arithmetic of the correction solve.  The outer loop (orthogonalisation,
+ * projected pencil, residuals, QZ) is FP64 in both. */
+typedef enum {
+    BBJD_PREC_FP64  = 0,       /* everything complex128                     */
+    BBJD_PREC_MIXED = 1        /* inner GMRES + preconditioner in complex64,
+                                  FP32 accumulation                          */
+} bbjd_precision;
+
+/* complex64 element, layout (re, im) of float */
+typedef struct { float re, im; } bbjd_cfloat;
+
+/* complex64 device-pointer operator, same contract as bbjd_matvec_fn */
+typedef int (*bbjd_op_c64_fn)(bbjd_int n, const bbjd_cfloat *x,
+                              bbjd_cfloat *y, void *data);
+
+typedef struct {
+    bbjd_callback_space callback_space;
+    bbjd_precision      precision;
+    void               *stream;      /* hipStream_t; NULL: workspace owns one */
+    /* complex64 operators for BBJD_PREC_MIXED (device pointers).  NULL
+     * entries fall back to the FP64 callbacks through conversion kernels. */
+    bbjd_op_c64_fn      matvec_A_c64;
+    bbjd_op_c64_fn      matvec_B_c64;
+    bbjd_op_c64_fn      precond_c64;
+} bbjd_gpu_options;
+
+void bbjd_gpu_options_init(bbjd_gpu_options *o);
+
+/* Exact device bytes bbjd_workspace_alloc_gpu() will take for (p, o);
+ * the serving loop admits requests against this figure. */
+size_t bbjd_workspace_bytes(const bbjd_params *p, const bbjd_gpu_options *o);
+
+bbjd_workspace *bbjd_workspace_alloc_gpu(const bbjd_params *p,
+                                         const bbjd_gpu_options *o);
+
+/* The stream on which the engine is running the current callback of the
+ * calling thread (hipStream_t), for DEVICE-space callbacks. */
+void *bbjd_callback_stream(void);
+
+/* Per-solve counters of the last bbjd_solve on a workspace. */
+typedef struct {
+    bbjd_int outer_iters;       /* iterations of the main loop            */
+    bbjd_int lock_iters;        /* of which ended in a lock               */
+    bbjd_int restarts;
+    bbjd_int inner_steps;       /* GMRES Arnoldi steps, all solves        */
+    bbjd_int dgks_second_passes;/* CGS2 calls that took the second pass   */
+    bbjd_int sum_k;             /* sum of subspace sizes over iterations  */
+    double   qz_seconds;        /* host ZGGES/ZTGSEN + selection          */
+    double   solve_seconds;     /* wall time of bbjd_solve                */
+    double   stream_bytes;      /* algorithmic HBM bytes of the n-sized
+                                   engine kernels (operators excluded)    */
+} bbjd_stats;
+
+int bbjd_workspace_stats(const bbjd_workspace *ws, bbjd_stats *st);
+
+/* Device pointer -> host vector download helper for tests/tools. */
+int bbjd_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int bbjd_memcpy_h2d(void *dst, const void *src, size_t bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BBJD_GPU_H */

@@ -1,0 +1,83 @@
+/*
+ * bbjd_ops.h -- device operators for block-banded pencils with dense tau
+ * rows (the pencils the reference is built for: block-banded interior plus
+ * dense boundary-condition rows, reference include/bbjd.h:8-9).
+ *
+ * The reference leaves the operators to the caller (scipy CSC matvec,
+ * bbjd.py:183-205; SuperLU/FPLUS preconditioner, tests/test_bbjd.py:59-73).
+ * These are the engine's own DEVICE-space callbacks for that structure:
+ * pass bbjd_op_matvec_A / _B / _precond as params.matvec_A / matvec_B /
+ * precond and the bbjd_op* as params.op_data, with callback_space
+ * BBJD_CALLBACK_DEVICE.
+ *
+ * Pencil layout (host arrays handed to bbjd_op_create):
+ *   n rows, block size bs (n % bs == 0), nb = n / bs block rows,
+ *   block half-width hw (1 block-tridiagonal, 2 block-pentadiagonal).
+ *   band: nb x (2 hw + 1) blocks of bs x bs, block (i, d) at offset
+ *         (i (2 hw + 1) + d) bs^2 holds A[i bs + r, (i + d - hw) bs + c] at
+ *         [c bs + r] (column-major block); out-of-range blocks are ignored.
+ *   tau:  ntau x n row-major, the last ntau rows of A (dense); the band
+ *         entries of those rows are ignored.
+ *   bdiag: n, B = diag(bdiag).
+ */
+#ifndef BBJD_OPS_H
+#define BBJD_OPS_H
+
+#include "bbjd_gpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    bbjd_int            n;
+    int                 bs, hw, ntau;
+    const bbjd_zomplex *band;
+    const bbjd_zomplex *tau;
+    const bbjd_zomplex *bdiag;
+} bbjd_pencil_desc;
+
+/* Seeded generator of the benchmark pencils: every band block and tau entry
+ * iid CN(0,1) (real and imaginary parts N(0, 1/2), Box-Muller over a
+ * splitmix64 counter hash, so the value of an entry depends only on
+ * (seed, entry) and not on threading), plus diag_shift on the diagonal of
+ * the diagonal blocks; bdiag = 1 except 0 on the ntau tau rows.
+ * Arrays are caller-allocated with the sizes above.  Returns 0. */
+int bbjd_pencil_generate(bbjd_int n, int bs, int hw, int ntau, uint64_t seed,
+                         double diag_shift, bbjd_zomplex *band,
+                         bbjd_zomplex *tau, bbjd_zomplex *bdiag);
+
+typedef enum {
+    BBJD_PC_NONE         = 0,
+    BBJD_PC_EXACT_LU     = 1,  /* block cyclic reduction of the banded part,
+                                  tau rows by Schur complement              */
+    BBJD_PC_BLOCK_JACOBI = 2   /* inverses of the bs x bs diagonal blocks   */
+} bbjd_pc_kind;
+
+typedef struct bbjd_op bbjd_op;
+
+/* Uploads the pencil, builds M = A - sigma B's preconditioner of kind pc.
+ * want_c64 also keeps complex64 copies for the mixed-precision inner solve.
+ * stream: hipStream_t used for set-up and by default for applies (NULL:
+ * the null stream).  Returns NULL on failure. */
+bbjd_op *bbjd_op_create(const bbjd_pencil_desc *d, bbjd_zomplex sigma,
+                        bbjd_pc_kind pc, int want_c64, void *stream);
+void     bbjd_op_free(bbjd_op *op);
+size_t   bbjd_op_bytes(const bbjd_op *op);   /* device bytes held */
+
+/* DEVICE-space callbacks (bbjd_matvec_fn / bbjd_precond_fn); data is the
+ * bbjd_op*.  They run on bbjd_callback_stream() when called by the engine,
+ * else on the op's stream. */
+int bbjd_op_matvec_A(bbjd_int n, const bbjd_zomplex *x, bbjd_zomplex *y, void *op);
+int bbjd_op_matvec_B(bbjd_int n, const bbjd_zomplex *x, bbjd_zomplex *y, void *op);
+int bbjd_op_precond (bbjd_int n, const bbjd_zomplex *x, bbjd_zomplex *y, void *op);
+
+/* complex64 forms (bbjd_op_c64_fn), available when want_c64 was set. */
+int bbjd_op_matvec_A_c64(bbjd_int n, const bbjd_cfloat *x, bbjd_cfloat *y, void *op);
+int bbjd_op_matvec_B_c64(bbjd_int n, const bbjd_cfloat *x, bbjd_cfloat *y, void *op);
+int bbjd_op_precond_c64 (bbjd_int n, const bbjd_cfloat *x, bbjd_cfloat *y, void *op);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BBJD_OPS_H */

@@ -1,0 +1,151 @@
+"""GPU parity: the engine's kernels, operators and whole solves through the
+C-ABI (libbbjd.so) against the oracle (CPU restatement of the reference,
+oracle/) on the same seeded inputs.
+
+Floating-point bar: orthogonalisation / operator outputs agree to a few ulp
+relative to the operand norms (1e-12); converged eigenvalues agree with the
+oracle's to 1e-8 relative (the parity bar of the reference tests, e.g.
+tests/test_bbjd.py:276) and every residual is below tol.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from bbjd_amd import BandedTauPencil, DeviceOperator, _lib, jdqz
+from bbjd_amd._bbjd_c import jdqz_c
+from oracle import orc
+from tests.helpers import make_diagonal, match_error, rel_match, residual
+
+pytestmark = pytest.mark.gpu
+
+
+def _orthonormal(n, k, seed):
+    rng = np.random.default_rng(seed)
+    M = rng.standard_normal((n, k)) + 1j * rng.standard_normal((n, k))
+    q, _ = np.linalg.qr(M)
+    return np.asfortranarray(q)
+
+
+@pytest.mark.parametrize("n,k", [(3000, 17), (2049, 1), (70000, 40), (5, 3)])
+def test_cgs2_matches_oracle(n, k):
+    lib = _lib.load()
+    olib = orc.load_oracle()
+    rng = np.random.default_rng(n + k)
+    Q = _orthonormal(n, k, n)
+    v = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    for case in ("generic", "near_parallel"):
+        x = v.copy() if case == "generic" else Q @ rng.standard_normal(k) + 1e-6 * v
+        vg, vo = x.copy(), x.copy()
+        hg = np.zeros(k, np.complex128)
+        ho = np.zeros(k, np.complex128)
+        ng = lib.bbjd_block_orthogonalize(n, k, Q.ctypes.data, vg.ctypes.data, hg.ctypes.data)
+        no = olib.orc_block_orthogonalize(n, k, Q.ctypes.data, vo.ctypes.data, ho.ctypes.data)
+        scale = np.linalg.norm(x)
+        assert np.linalg.norm(vg - vo) <= 1e-12 * scale, case
+        assert abs(ng - no) <= 1e-12 * scale, case
+        assert abs(ng - np.linalg.norm(vg)) <= 1e-12 * scale
+        # the output is orthogonal to Q to rounding
+        assert np.linalg.norm(Q.conj().T @ vg) <= 1e-12 * scale
+
+
+SHAPES = [(2000, 4, 1, 2), (4096, 8, 2, 16), (1000, 8, 1, 4), (640, 4, 2, 0)]
+
+
+@pytest.mark.parametrize("n,bs,hw,ntau", SHAPES)
+def test_device_operators_match_oracle(n, bs, hw, ntau):
+    p = BandedTauPencil.generate(n, bs, hw, ntau, seed=11)
+    sigma = 0.5 + 0.25j
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    for pc, opc in (("exact", 1), ("block_jacobi", 2)):
+        dev = DeviceOperator(p, sigma, pc=pc)
+        cpu = orc.CpuPencilOps(p, sigma, opc)
+        ya, yb, yp = dev.matvec(x), dev.matvec_B(x), dev.solve(x)
+        ra, rb, rp = cpu.apply("orc_matvec_A", x), cpu.apply("orc_matvec_B", x), cpu.apply("orc_precond", x)
+        assert np.linalg.norm(ya - ra) <= 1e-13 * np.linalg.norm(ra)
+        assert np.linalg.norm(yb - rb) <= 1e-15 * np.linalg.norm(rb) + 1e-300
+        # the preconditioner solves M y = x: both are exact solves, compare the
+        # solutions and the residual of the device one
+        assert np.linalg.norm(yp - rp) <= 1e-9 * np.linalg.norm(rp), pc
+        dev.free()
+
+
+def _cfg1(n=2000, k=8, seed=1):
+    return BandedTauPencil.generate(n, 4, 1, 2, seed=seed), 0.5 + 0.5j
+
+
+def test_config1_device_operators_match_oracle():
+    p, sigma = _cfg1()
+    ops = orc.CpuPencilOps(p, sigma, 1)
+    ref = orc.solve("oracle", p.n, 8, sigma, tol=1e-10, maxsub=40, minsub=20, maxiter=500, ops=ops)
+    dev = DeviceOperator(p, sigma, pc="exact")
+    ev, X, info = jdqz(dev, k=8, sigma=sigma, tol=1e-10, maxsub=40, minsub=20, maxiter=500,
+                       preconditioner=dev, return_info=True)
+    assert info["nconv"] == 8 == ref["nconv"]
+    assert rel_match(ev, ref["evals"]) <= 1e-8
+    assert rel_match(ref["evals"], ev) <= 1e-8
+    A, B = p.to_scipy()
+    for j in range(8):
+        th = ev[j]
+        r = np.linalg.norm(A @ X[:, j] - th * (B @ X[:, j]))
+        assert r <= 1e-8
+    assert np.all(info["residuals"] < 1e-10)
+    # trajectory: same algorithm, different rounding -> iteration counts close
+    assert abs(info["iters"] - ref["iters"]) <= max(5, ref["iters"] // 10)
+
+
+def test_host_callbacks_diagonal():
+    """reference tests/test_bbjd.py:162-174 through the host-callback path"""
+    from scipy.sparse.linalg import splu
+    A, B = make_diagonal(50)
+    lu = splu(A.tocsc())
+    ev, _, info = jdqz(A, B, k=5, sigma=0.0, tol=1e-10, preconditioner=lu, return_info=True)
+    assert info["nconv"] == 5
+    assert np.allclose(np.sort(ev.real), [0.1, 0.2, 0.3, 0.4, 0.5], atol=1e-10)
+
+
+def test_host_callbacks_match_oracle_trajectory():
+    """same operators in host space: the engine follows the oracle's steps"""
+    p, sigma = _cfg1(n=400)
+    ops = orc.CpuPencilOps(p, sigma, 1)
+    ref = orc.solve("oracle", p.n, 4, sigma, tol=1e-10, maxsub=30, minsub=12, maxiter=400, ops=ops)
+    out = jdqz_c(p.n, 4, sigma, 0, lambda x: ops.apply("orc_matvec_A", x),
+                 lambda x: ops.apply("orc_matvec_B", x), lambda x: ops.apply("orc_precond", x),
+                 tol=1e-10, maxsub=30, minsub=12, maxiter=400)
+    ev, _, res, nconv, iters, status = out
+    assert nconv == ref["nconv"] == 4
+    assert rel_match(ev, ref["evals"]) <= 1e-8
+    assert abs(iters - ref["iters"]) <= 3
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_inner_gmres_block_jacobi(precision):
+    """inner GMRES with an approximate preconditioner (block Jacobi), FP64
+    or complex64: same eigenvalues as the oracle's FP64 GMRES"""
+    p = BandedTauPencil.generate(3200, 8, 2, 8, seed=5)
+    sigma = 1 + 2j
+    ops = orc.CpuPencilOps(p, sigma, 2)
+    ref = orc.solve("oracle", p.n, 4, sigma, tol=1e-10, maxsub=40, minsub=16, maxiter=400,
+                    inner_maxiter=20, inner_tol=1e-3, ops=ops)
+    dev = DeviceOperator(p, sigma, pc="block_jacobi", c64=True)
+    ev, X, info = jdqz(dev, k=4, sigma=sigma, tol=1e-10, maxsub=40, minsub=16, maxiter=400,
+                       inner_maxiter=20, inner_tol=1e-3, preconditioner=dev, return_info=True,
+                       precision=precision)
+    assert info["nconv"] == 4 == ref["nconv"]
+    assert rel_match(ev, ref["evals"]) <= 1e-8
+    A, B = p.to_scipy()
+    for j in range(4):
+        assert np.linalg.norm(A @ X[:, j] - ev[j] * (B @ X[:, j])) <= 1e-8
+
+
+def test_thick_restart_small_maxsub():
+    """forces several restarts (maxsub 16): parity with the oracle"""
+    p, sigma = _cfg1(n=800, seed=3)
+    ops = orc.CpuPencilOps(p, sigma, 1)
+    ref = orc.solve("oracle", p.n, 6, sigma, tol=1e-10, maxsub=16, minsub=8, maxiter=600, ops=ops)
+    dev = DeviceOperator(p, sigma, pc="exact")
+    ev, _, info = jdqz(dev, k=6, sigma=sigma, tol=1e-10, maxsub=16, minsub=8, maxiter=600,
+                       preconditioner=dev, return_info=True)
+    assert info["nconv"] == ref["nconv"] == 6
+    assert rel_match(ev, ref["evals"]) <= 1e-8
