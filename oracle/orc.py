@@ -151,6 +151,38 @@ class CpuPencilOps:
             pass
 
 
+def gmres(n, QR, QL, ta, tb, r, ops, maxiter, tol):
+    """orc_correction_gmres on host arrays: QR, QL are n x ndefl."""
+    lib = load_oracle()
+    QR = np.asfortranarray(QR, dtype=np.complex128)
+    QL = np.asfortranarray(QL, dtype=np.complex128)
+    ndefl = QR.shape[1]
+    r = np.ascontiguousarray(r, dtype=np.complex128)
+    z = np.zeros(n, np.complex128)
+    V = np.zeros((maxiter + 1) * n, np.complex128)
+    H = np.zeros((maxiter + 1) * maxiter, np.complex128)
+    cs = np.zeros(maxiter, np.float64)
+    sn = np.zeros(maxiter, np.complex128)
+    g = np.zeros(maxiter + 1, np.complex128)
+    t1 = np.zeros(n, np.complex128)
+    t2 = np.zeros(n, np.complex128)
+    steps = C.c_int64(0)
+    f = lib.orc_correction_gmres
+    f.restype = C.c_int
+    f.argtypes = [C.c_int64, C.c_int64, C.c_void_p, C.c_void_p, zcomplex, zcomplex, C.c_void_p,
+                  C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_double,
+                  C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                  C.POINTER(C.c_int64)]
+    rc = f(n, ndefl, QR.ctypes.data, QL.ctypes.data, zcomplex(ta.real, ta.imag),
+           zcomplex(tb.real, tb.imag), r.ctypes.data, z.ctypes.data, ops.addr("orc_matvec_A"),
+           ops.addr("orc_matvec_B"), ops.addr("orc_precond") if ops.pc else None, ops.handle,
+           maxiter, tol, V.ctypes.data, H.ctypes.data, cs.ctypes.data, sn.ctypes.data,
+           g.ctypes.data, t1.ctypes.data, t2.ctypes.data, C.byref(steps))
+    if rc:
+        raise RuntimeError("orc_correction_gmres")
+    return z
+
+
 def _fill_params(p, n, nev, sigma, target, tol, maxiter, maxsub, minsub, inner_maxiter,
                  inner_tol, verbose, v0):
     p.n, p.nev = n, nev

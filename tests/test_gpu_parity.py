@@ -120,23 +120,71 @@ def test_host_callbacks_match_oracle_trajectory():
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_inner_gmres_block_jacobi(precision):
-    """inner GMRES with an approximate preconditioner (block Jacobi), FP64
-    or complex64: same eigenvalues as the oracle's FP64 GMRES"""
+def test_inner_gmres_approximate_preconditioner(precision):
+    """inner GMRES(20) with an approximate preconditioner -- the exact LU at a
+    shift 0.5 away, as in reference tests/test_bbjd.py:459-479 -- in FP64 or
+    complex64: same eigenvalues as the oracle's FP64 GMRES, in no more outer
+    iterations than the single-step correction needs."""
     p = BandedTauPencil.generate(3200, 8, 2, 8, seed=5)
     sigma = 1 + 2j
-    ops = orc.CpuPencilOps(p, sigma, 2)
+    ops = orc.CpuPencilOps(p, sigma + 0.5, 1)
     ref = orc.solve("oracle", p.n, 4, sigma, tol=1e-10, maxsub=40, minsub=16, maxiter=400,
                     inner_maxiter=20, inner_tol=1e-3, ops=ops)
-    dev = DeviceOperator(p, sigma, pc="block_jacobi", c64=True)
+    assert ref["nconv"] == 4
+    dev = DeviceOperator(p, sigma + 0.5, pc="exact", c64=True)
     ev, X, info = jdqz(dev, k=4, sigma=sigma, tol=1e-10, maxsub=40, minsub=16, maxiter=400,
                        inner_maxiter=20, inner_tol=1e-3, preconditioner=dev, return_info=True,
                        precision=precision)
-    assert info["nconv"] == 4 == ref["nconv"]
+    assert info["nconv"] == 4
     assert rel_match(ev, ref["evals"]) <= 1e-8
+    assert rel_match(ref["evals"], ev) <= 1e-8
     A, B = p.to_scipy()
     for j in range(4):
         assert np.linalg.norm(A @ X[:, j] - ev[j] * (B @ X[:, j])) <= 1e-8
+    ev0, _, info0 = jdqz(dev, k=4, sigma=sigma, tol=1e-10, maxsub=40, minsub=16, maxiter=400,
+                         preconditioner=dev, return_info=True)
+    assert info["iters"] <= info0["iters"]
+
+
+def test_correction_gmres_entry_matches_oracle():
+    """bbjd_correction_gmres (reference include/bbjd.h:268-282 with the
+    argument order of src/jd_core.c:568-582) on host arrays vs the oracle's
+    GMRES on the same projected correction equation"""
+    lib = _lib.load()
+    p = BandedTauPencil.generate(1200, 4, 1, 2, seed=9)
+    sigma = 0.3 + 0.1j
+    ops = orc.CpuPencilOps(p, sigma + 0.3, 1)
+    n = p.n
+    rng = np.random.default_rng(1)
+    u = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    u /= np.linalg.norm(u)
+    w = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    w /= np.linalg.norm(w)
+    r = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    ta, tb = 0.7 - 0.2j, 0.9 + 0.1j
+    z_ref = orc.gmres(n, u[:, None], w[:, None], ta, tb, r, ops, maxiter=15, tol=1e-6)
+    prm = _lib.Params()
+    lib.bbjd_params_init(C.byref(prm))
+    prm.n, prm.nev, prm.inner_maxiter, prm.maxsub = n, 2, 15, 20
+    ws = lib.bbjd_workspace_alloc(C.byref(prm))
+    assert ws
+    z = np.zeros(n, np.complex128)
+    cbs = [_lib.CALLBACK(lambda nn, x, y, d, f=f: _apply_into(ops, f, nn, x, y))
+           for f in ("orc_matvec_A", "orc_matvec_B", "orc_precond")]
+    rc = lib.bbjd_correction_gmres(n, u.ctypes.data, w.ctypes.data, _lib.zc(ta), _lib.zc(tb),
+                                   r.ctypes.data, z.ctypes.data,
+                                   *[C.cast(c, C.c_void_p).value for c in cbs], None, 15, 1e-6, ws)
+    lib.bbjd_workspace_free(ws)
+    assert rc == 0
+    assert np.linalg.norm(z - z_ref) <= 1e-9 * np.linalg.norm(z_ref)
+    assert abs(np.vdot(u, z)) <= 1e-12 * np.linalg.norm(z)
+
+
+def _apply_into(ops, name, n, x, y):
+    xa = np.frombuffer((C.c_double * (2 * n)).from_address(x), dtype=np.complex128)
+    ya = np.frombuffer((C.c_double * (2 * n)).from_address(y), dtype=np.complex128)
+    ya[:] = ops.apply(name, xa.copy())
+    return 0
 
 
 def test_thick_restart_small_maxsub():
