@@ -77,7 +77,7 @@ def load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"bbjd_amd: {LIB_PATH} is not built; run __graft_entry__.build() "
                           "or make -C bbjd_amd/csrc")
-    lib = C.CDLL(LIB_PATH, mode=C.RTLD_GLOBAL)
+    lib = C.CDLL(LIB_PATH)
     P, V, I64, D = C.POINTER(Params), C.c_void_p, C.c_int64, C.c_double
     _sig(lib, "bbjd_params_init", None, P)
     _sig(lib, "bbjd_workspace_alloc", V, P)

@@ -53,6 +53,8 @@ struct bbjd_dev {
     struct random_data rng;
     int32_t          rng_state[32];
     bbjd_stats       st;
+    double          *pair_res;         /* per converged pair, last solve    */
+    bbjd_int         nconv_last;
 };
 
 static __thread hipStream_t tl_cb_stream;
@@ -169,7 +171,7 @@ void bbjd_workspace_free(bbjd_workspace *ws)
         if (d->hx) hipHostFree(d->hx);
         if (d->hy) hipHostFree(d->hy);
         if (d->own_stream && d->stream) hipStreamDestroy(d->stream);
-        free(d->al); free(d->be); free(d->yv); free(d->xv); free(d->zrh); free(d->hcol);
+        free(d->al); free(d->be); free(d->yv); free(d->xv); free(d->zrh); free(d->hcol); free(d->pair_res);
         bbjd_qz_free(&d->qz);
         free(d);
     }
@@ -251,8 +253,9 @@ bbjd_workspace *bbjd_workspace_alloc_gpu(const bbjd_params *p, const bbjd_gpu_op
     d->xv = malloc((size_t)(L.maxsub + 2) * sizeof(zcx));
     d->zrh = malloc(ss * sizeof(zcx));
     d->hcol = malloc((size_t)(L.maxsub + L.ims + 8) * 4 * sizeof(zcx));
+    d->pair_res = calloc((size_t)L.nev + 1, sizeof(double));
     if (!ws->MA || !ws->MB || !ws->SA || !ws->SB || !ws->ZA || !ws->ZB || !ws->gmres_H || !ws->gmres_cs ||
-        !ws->gmres_sn || !ws->gmres_g || !d->al || !d->be || !d->yv || !d->xv || !d->zrh || !d->hcol)
+        !ws->gmres_sn || !ws->gmres_g || !d->al || !d->be || !d->yv || !d->xv || !d->zrh || !d->hcol || !d->pair_res)
         goto fail;
     if (bbjd_qz_init(&d->qz, L.maxsub)) goto fail;
     if (!HIPOK(hipStreamSynchronize(d->stream))) goto fail;
@@ -270,6 +273,8 @@ void bbjd_result_free(bbjd_result *r)
     free(r->eigenvalues); free(r->eigenvectors); free(r->residuals);
     r->eigenvalues = NULL; r->eigenvectors = NULL; r->residuals = NULL;
 }
+
+void *bbjd_workspace_stream(const bbjd_workspace *ws) { return ws && ws->dev ? (void *)ws->dev->stream : NULL; }
 
 int bbjd_workspace_stats(const bbjd_workspace *ws, bbjd_stats *st)
 {
@@ -291,8 +296,9 @@ typedef struct {
     struct bbjd_dev  *d;
     const bbjd_params *P;
     hipStream_t       s;
-    bbjd_int          n, ld, ms;
-    int               err;     /* device/launch error seen */
+    bbjd_int          n, ld;
+    bbjd_int          ms;      /* maxsub of this solve (restart threshold)   */
+    bbjd_int          ldm;     /* leading dimension of the host k x k data   */
 } ctx_t;
 
 #define COLZ(M, j) ((void *)((zcx *)(M) + (size_t)(j) * c->ld))
@@ -554,7 +560,7 @@ static int pencil_update(ctx_t *c, bbjd_int k)
 {
     bbjd_workspace *ws = c->ws;
     struct bbjd_dev *d = c->d;
-    const bbjd_int ms = c->ms;
+    const bbjd_int ms = c->ldm;
     zcx *pen = (zcx *)d->pencil;
     int rc = 0;
     rc |= bbjd_k_gemv_c(c->s, c->n, (int)(k + 1), ws->W, c->ld, COLZ(ws->AV, k), COLZ(ws->BV, k), pen,
@@ -583,7 +589,7 @@ static int thick_restart(ctx_t *c, bbjd_int k, bbjd_int keep, zcx nu0, zcx mu0)
 {
     bbjd_workspace *ws = c->ws;
     struct bbjd_dev *d = c->d;
-    const bbjd_int ms = c->ms, n = c->n;
+    const bbjd_int ms = c->ldm, n = c->n;
     const zcx *ZB = (const zcx *)ws->ZB;
     int rc = 0;
     for (bbjd_int j = 0; j < k; j++)
@@ -622,21 +628,15 @@ static int thick_restart(ctx_t *c, bbjd_int k, bbjd_int keep, zcx nu0, zcx mu0)
 /* ------------------------------------------------------------------ */
 /* the outer iteration                                                 */
 /* ------------------------------------------------------------------ */
-bbjd_status bbjd_solve(const bbjd_params *P, bbjd_workspace *ws, bbjd_result *R)
+static bbjd_status solve_impl(const bbjd_params *P, bbjd_workspace *ws, bbjd_result *R, bbjd_int ms_solve)
 {
-    if (!P || !ws || !R) return BBJD_ERR_INVALID;
     struct bbjd_dev *d = ws->dev;
-    memset(R, 0, sizeof(*R));
-    if (P->n != ws->n || P->nev > ws->nev || P->nev <= 0 || !P->matvec_A || !P->matvec_B ||
-        (P->inner_maxiter > 0 && P->inner_maxiter > ws->inner_maxiter)) {
-        R->status = BBJD_ERR_INVALID;
-        return BBJD_ERR_INVALID;
-    }
     const double t_start = now_s();
     memset(&d->st, 0, sizeof(d->st));
-    ctx_t cx = {ws, d, P, d->stream, P->n, ws->ld, ws->maxsub, 0};
+    ctx_t cx = {ws, d, P, d->stream, P->n, ws->ld, ms_solve, ws->maxsub};
     ctx_t *c = &cx;
-    const bbjd_int n = P->n, nev = P->nev, ms = ws->maxsub;
+    const bbjd_int n = P->n, nev = P->nev, ms = ms_solve, ldm = ws->maxsub;
+    d->nconv_last = 0;
     bbjd_int minsub = P->minsub > 0 ? P->minsub : (2 * nev + 5 < 20 ? 20 : 2 * nev + 5);
     const zcx sigma = P->sigma;
     const double tol = P->tol;
@@ -712,7 +712,7 @@ bbjd_status bbjd_solve(const bbjd_params *P, bbjd_workspace *ws, bbjd_result *R)
         const zcx ta = al[p], tb = be[p];
         /* ---- step 2: Ritz vector u = V ZB y on the device ---- */
         void *u = ws->tmp1, *Au = ws->tmp2, *Bu = ws->tmp3, *r = ws->tmp4;
-        bbjd_ritz_coefficients(k, ms, p, ta, tb, (const zcx *)ws->SA, (const zcx *)ws->SB, (const zcx *)ws->ZB,
+        bbjd_ritz_coefficients(k, ldm, p, ta, tb, (const zcx *)ws->SA, (const zcx *)ws->SB, (const zcx *)ws->ZB,
                                d->yv, d->xv);
         rc |= upload_small(c, d->rd.coef, d->xv, k);
         rc |= bbjd_k_gemv_n(c->s, n, (int)k, ws->V, c->ld, d->rd.coef, u, &d->rd, SL_U);
@@ -737,7 +737,11 @@ bbjd_status bbjd_solve(const bbjd_params *P, bbjd_workspace *ws, bbjd_result *R)
                                         hipMemcpyDeviceToHost, c->s));
             rc |= !HIPOK(hipStreamSynchronize(c->s));
             R->residuals[R->nconv] = rel;
+            /* ||A u - lambda B u|| with ||u|| = 1: the FP64 residual of the
+             * returned pair from the operator applies, r / theta_b */
+            d->pair_res[R->nconv] = cabs(tb) > 1e-300 ? d->hscal[SL_R] / cabs(tb) : d->hscal[SL_R];
             R->nconv++;
+            d->nconv_last = R->nconv;
             d->st.lock_iters++;
             if (P->verbose >= 1)
                 fprintf(stderr, "  *** converged #%lld: (%+.10e, %+.10e)  res %.3e  iter %lld\n",
@@ -837,6 +841,40 @@ cb_err:
     return BBJD_ERR_CALLBACK;
 }
 
+bbjd_status bbjd_solve(const bbjd_params *P, bbjd_workspace *ws, bbjd_result *R)
+{
+    if (!P || !ws || !R) return BBJD_ERR_INVALID;
+    memset(R, 0, sizeof(*R));
+    /* reference semantics: the workspace was allocated for this problem and
+     * its maxsub is the solve's (reference src/jd_core.c:749) */
+    if (P->n != ws->n || P->nev > ws->nev || P->nev <= 0 || !P->matvec_A || !P->matvec_B ||
+        (P->inner_maxiter > 0 && P->inner_maxiter > ws->inner_maxiter)) {
+        R->status = BBJD_ERR_INVALID;
+        return BBJD_ERR_INVALID;
+    }
+    return solve_impl(P, ws, R, ws->maxsub);
+}
+
+bbjd_status bbjd_solve_in(const bbjd_params *P, bbjd_workspace *ws, bbjd_result *R)
+{
+    if (!P || !ws || !R) return BBJD_ERR_INVALID;
+    memset(R, 0, sizeof(*R));
+    const bbjd_int ms = auto_maxsub(P);
+    if (P->n <= 0 || P->n > ws->n || P->nev > ws->nev || P->nev <= 0 || ms > ws->maxsub || ms < 2 ||
+        !P->matvec_A || !P->matvec_B || (P->inner_maxiter > 0 && P->inner_maxiter > ws->inner_maxiter)) {
+        R->status = BBJD_ERR_INVALID;
+        return BBJD_ERR_INVALID;
+    }
+    return solve_impl(P, ws, R, ms);
+}
+
+int bbjd_workspace_pair_residuals(const bbjd_workspace *ws, double *out)
+{
+    if (!ws || !ws->dev || !out) return -1;
+    memcpy(out, ws->dev->pair_res, (size_t)ws->dev->nconv_last * sizeof(double));
+    return (int)ws->dev->nconv_last;
+}
+
 /* ------------------------------------------------------------------ */
 /* reference test helpers on host arrays                               */
 /* ------------------------------------------------------------------ */
@@ -914,7 +952,7 @@ int bbjd_correction_gmres(bbjd_int n, const bbjd_zomplex *Q_right, const bbjd_zo
     bbjd_params_init(&P);
     P.n = n; P.nev = ws->nev; P.matvec_A = matvec_A; P.matvec_B = matvec_B; P.precond = precond;
     P.op_data = op_data; P.inner_maxiter = maxiter; P.inner_tol = tol;
-    ctx_t cx = {ws, d, &P, d->stream, n, ws->ld, ws->maxsub, 0};
+    ctx_t cx = {ws, d, &P, d->stream, n, ws->ld, ws->maxsub, ws->maxsub};
     const bbjd_int ndefl = ws->nlock + 1;
     const int devspace = d->opt.callback_space == BBJD_CALLBACK_DEVICE;
     int rc = 0;

@@ -21,9 +21,31 @@
 //  * the same templates give the complex64 kernels (FP32 accumulation) of
 //    the mixed-precision inner solve.
 #include <hip/hip_runtime.h>
+#include <mutex>
+#include <vector>
 #include "engine.h"
 
 typedef long long i64;
+
+// ---------------------------------------------------------------------------
+// Launch timing of the projection kernel (k_proj, one vector): when enabled,
+// every launch is bracketed by two HIP events on its own stream and its
+// algorithmic bytes (Q tile read once, x read once per column tile) are
+// recorded; bbjd_ktime_harvest() turns the events into durations.
+namespace {
+struct KRec { hipEvent_t a, b; double bytes; };
+std::mutex g_kt_mu;
+std::vector<KRec> g_kt;
+std::vector<hipEvent_t> g_kt_pool;
+bool g_kt_on = false;
+hipEvent_t kt_event()
+{
+    if (!g_kt_pool.empty()) { hipEvent_t e = g_kt_pool.back(); g_kt_pool.pop_back(); return e; }
+    hipEvent_t e;
+    hipEventCreate(&e);
+    return e;
+}
+}  // namespace
 
 template <typename R> struct alignas(2 * sizeof(R)) Cx { R re, im; };
 
@@ -445,9 +467,22 @@ static int proj_pass(hipStream_t s, i64 n, int k, const void *Q, i64 ldq, const 
         if (nv == 1) {
             constexpr int C = 16;
             dim3 g(k > 0 ? (k + C - 1) / C : 1, nt);
+            const bool timed = g_kt_on && sizeof(R) == 8 && cond == nullptr;
+            hipEvent_t ea = nullptr, eb = nullptr;
+            if (timed) {
+                std::lock_guard<std::mutex> lk(g_kt_mu);
+                ea = kt_event(); eb = kt_event();
+            }
+            if (timed) hipEventRecord(ea, s);
             hipLaunchKernelGGL((k_proj<R, 1, C>), g, dim3(TPB), 0, s, (i64)n, k, (const Cx<R> *)Q, (i64)ldq,
                                (const Cx<R> *)x0, (const Cx<R> *)nullptr, (Cx<R> *)rd->part, ldp, (R *)rd->pnorm,
                                want_norm, cond);
+            if (timed) {
+                hipEventRecord(eb, s);
+                const double bytes = (double)sizeof(Cx<R>) * (double)n * ((double)k + (double)g.x);
+                std::lock_guard<std::mutex> lk(g_kt_mu);
+                g_kt.push_back({ea, eb, bytes});
+            }
         } else {
             constexpr int C = 8;
             dim3 g(k > 0 ? (k + C - 1) / C : 1, nt);
@@ -545,6 +580,38 @@ static int lincomb(hipStream_t s, i64 n, double are, double aim, const void *x, 
 }
 
 extern "C" {
+
+void bbjd_ktime_enable(int on)
+{
+    std::lock_guard<std::mutex> lk(g_kt_mu);
+    g_kt_on = on != 0;
+}
+
+// Waits for the recorded launches; returns their count and fills the sums of
+// durations (ms) and algorithmic bytes, then clears the record.
+int bbjd_ktime_harvest(double *total_ms, double *total_bytes)
+{
+    std::vector<KRec> recs;
+    {
+        std::lock_guard<std::mutex> lk(g_kt_mu);
+        recs.swap(g_kt);
+    }
+    double ms = 0, by = 0;
+    int cnt = 0;
+    for (auto &r : recs) {
+        float t = 0;
+        if (hipEventSynchronize(r.b) == hipSuccess && hipEventElapsedTime(&t, r.a, r.b) == hipSuccess) {
+            ms += t; by += r.bytes; cnt++;
+        }
+    }
+    {
+        std::lock_guard<std::mutex> lk(g_kt_mu);
+        for (auto &r : recs) { g_kt_pool.push_back(r.a); g_kt_pool.push_back(r.b); }
+    }
+    if (total_ms) *total_ms = ms;
+    if (total_bytes) *total_bytes = by;
+    return cnt;
+}
 
 int bbjd_k_cgs2(hipStream_t s, int64_t n, int k, const void *Q, int64_t ldq, void *v, bbjd_red *rd,
                 int out_slot, void *hsum)

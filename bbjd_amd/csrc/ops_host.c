@@ -271,6 +271,63 @@ static int exact_apply(struct bbjd_op *op, hipStream_t st, const void *x, void *
     return rc;
 }
 
+/* ---- device layout --------------------------------------------------- */
+enum { D_BAND, D_TAUC, D_BDIAG, D_BANDC, D_TAUCC, D_BDIAGC, D_BJ, D_BJC, D_BCR, D_BV, D_XV, D_Y, D_SINV, D_SMALL,
+       D_PART, D_PNORM, D_RSC, D_ZT1, D_ZT2, D_END };
+
+static size_t bcr_block_count(bbjd_int N)
+{
+    int L = 0;
+    while (((bbjd_int)1 << L) < N) L++;
+    size_t nbk = 0;
+    for (int l = 0; l < L; l++) {
+        const bbjd_int M = (N + ((bbjd_int)1 << l) - 1) >> l;
+        nbk += (size_t)2 * ((M + 1) / 2) + (size_t)3 * (M / 2);
+    }
+    return nbk + 1;
+}
+
+static size_t op_layout(const bbjd_pencil_desc *d, int pc, int want_c64, bbjd_int N, int s, size_t bcr_blocks,
+                        size_t *sz, size_t *off)
+{
+    const bbjd_int n = d->n, n1 = d->n - d->ntau, nb = d->n / d->bs;
+    const size_t nbandel = (size_t)nb * (2 * d->hw + 1) * d->bs * d->bs;
+    memset(sz, 0, sizeof(size_t) * D_END);
+    sz[D_BAND] = nbandel * 16;
+    sz[D_TAUC] = (size_t)n * d->ntau * 16;
+    sz[D_BDIAG] = (size_t)n * 16;
+    if (want_c64) { sz[D_BANDC] = nbandel * 8; sz[D_TAUCC] = (size_t)n * d->ntau * 8; sz[D_BDIAGC] = (size_t)n * 8; }
+    if (pc == BBJD_PC_BLOCK_JACOBI) {
+        sz[D_BJ] = (size_t)nb * d->bs * d->bs * 16;
+        if (want_c64) sz[D_BJC] = (size_t)nb * d->bs * d->bs * 8;
+    }
+    if (pc == BBJD_PC_EXACT_LU) {
+        sz[D_BCR] = bcr_blocks * s * s * 16;
+        sz[D_BV] = sz[D_XV] = (size_t)N * s * 16;
+        sz[D_Y] = (size_t)n1 * d->ntau * 16;
+        sz[D_SINV] = (size_t)d->ntau * d->ntau * 16 + 16;
+    }
+    sz[D_SMALL] = (size_t)(2 * d->ntau + 16) * 16;
+    sz[D_PART] = (size_t)bbjd_red_part_entries(n, d->ntau > 0 ? d->ntau + 2 : 2) * 16;
+    sz[D_PNORM] = ((size_t)(n + BBJD_PROJ_ROWS - 1) / BBJD_PROJ_ROWS + 64) * 8;
+    sz[D_RSC] = 4096;
+    if (want_c64 && pc != BBJD_PC_NONE) sz[D_ZT1] = sz[D_ZT2] = (size_t)n * 16;
+    size_t tot = 0;
+    for (int i = 0; i < D_END; i++) { off[i] = tot; tot += (sz[i] + 255) / 256 * 256; }
+    return tot;
+}
+
+size_t bbjd_op_bytes_for(const bbjd_pencil_desc *d, int pc, int want_c64)
+{
+    if (!d || d->n <= 0 || d->bs <= 0) return 0;
+    size_t sz[D_END], off[D_END];
+    int s = (d->hw > 0 ? d->hw : 1) * d->bs;
+    bbjd_int N = (d->n - d->ntau + s - 1) / s;
+    return op_layout(d, pc, want_c64, N, s, pc == BBJD_PC_EXACT_LU ? bcr_block_count(N) : 0, sz, off);
+}
+
+bbjd_int bbjd_op_size(const bbjd_op *op) { return op ? op->n : 0; }
+
 /* ---- public ---------------------------------------------------------- */
 void bbjd_op_free(bbjd_op *op)
 {
@@ -294,7 +351,6 @@ bbjd_op *bbjd_op_create(const bbjd_pencil_desc *d, bbjd_zomplex sigma, bbjd_pc_k
     op->stream = (hipStream_t)stream;
     op->sigma = sigma;
     const bbjd_int n = op->n, n1 = op->n1;
-    const size_t nbandel = (size_t)op->nb * op->nd * op->bs * op->bs;
     zcx *bcr_host = NULL;
     size_t bcr_blocks = 0;
     if (pc == BBJD_PC_EXACT_LU) {
@@ -304,32 +360,10 @@ bbjd_op *bbjd_op_create(const bbjd_pencil_desc *d, bbjd_zomplex sigma, bbjd_pc_k
         if (bcr_factor(op, d, &bcr_host, &bcr_blocks)) { bbjd_op_free(op); return NULL; }
     }
     /* device layout */
-    size_t off[24], sz[24];
-    memset(sz, 0, sizeof(sz));
-    enum { D_BAND, D_TAUC, D_BDIAG, D_BANDC, D_TAUCC, D_BDIAGC, D_BJ, D_BJC, D_BCR, D_BV, D_XV, D_Y, D_SINV, D_SMALL,
-           D_PART, D_PNORM, D_RSC, D_ZT1, D_ZT2, D_END };
-    sz[D_BAND] = nbandel * 16;
-    sz[D_TAUC] = (size_t)n * op->ntau * 16;
-    sz[D_BDIAG] = (size_t)n * 16;
-    if (want_c64) { sz[D_BANDC] = nbandel * 8; sz[D_TAUCC] = (size_t)n * op->ntau * 8; sz[D_BDIAGC] = (size_t)n * 8; }
-    if (pc == BBJD_PC_BLOCK_JACOBI) {
-        sz[D_BJ] = (size_t)op->nb * op->bs * op->bs * 16;
-        if (want_c64) sz[D_BJC] = (size_t)op->nb * op->bs * op->bs * 8;
-    }
-    if (pc == BBJD_PC_EXACT_LU) {
-        sz[D_BCR] = bcr_blocks * op->s * op->s * 16;
-        sz[D_BV] = sz[D_XV] = (size_t)op->N * op->s * 16;
-        sz[D_Y] = (size_t)n1 * op->ntau * 16;
-        sz[D_SINV] = (size_t)op->ntau * op->ntau * 16 + 16;
-    }
-    sz[D_SMALL] = (size_t)(2 * op->ntau + 16) * 16;
+    size_t off[D_END], sz[D_END];
+    const size_t tot = op_layout(d, pc, want_c64, op->N, op->s, bcr_blocks, sz, off);
+    const size_t nbandel = (size_t)op->nb * op->nd * op->bs * op->bs;
     const int64_t pe = bbjd_red_part_entries(n, op->ntau > 0 ? op->ntau + 2 : 2);
-    sz[D_PART] = (size_t)pe * 16;
-    sz[D_PNORM] = ((size_t)(n + BBJD_PROJ_ROWS - 1) / BBJD_PROJ_ROWS + 64) * 8;
-    sz[D_RSC] = 4096;
-    if (want_c64 && pc != BBJD_PC_NONE) sz[D_ZT1] = sz[D_ZT2] = (size_t)n * 16;
-    size_t tot = 0;
-    for (int i = 0; i < D_END; i++) { off[i] = tot; tot += (sz[i] + 255) / 256 * 256; }
     if (!HIPOK(hipMalloc(&op->dblock, tot))) { op->dblock = NULL; free(bcr_host); bbjd_op_free(op); return NULL; }
     op->dbytes = tot;
     char *b = (char *)op->dblock;

@@ -1,1 +1,292 @@
-/* serve.c placeholder */
+/*
+ * serve.c -- continuous-batching serving loop (include/bbjd_serve.h).
+ *
+ * The reference runs the independent solves of an eigenspectrum sweep on
+ * separate MPI ranks (reference README.md:59-83).  On one MI355X the solve is
+ * bound by the host QZ of its small projected pencil while the GPU idles, so
+ * many solves are served at once: W workers, each a host thread with its own
+ * HIP stream and a resident workspace sized for the largest request.  The
+ * caller's thread admits requests in arrival order whenever a worker is free
+ * and the HBM budget holds the request's operator (when the server builds
+ * it); a finished request frees its slot immediately and the next one is
+ * admitted while the others are still iterating.  Requests never share
+ * device memory, so each result is the same alone and in the mix: the
+ * engine's reductions depend only on the request's shape.
+ */
+#include "../../include/bbjd_serve.h"
+#include "engine.h"
+
+#include <complex.h>
+#include <math.h>
+#include <omp.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+int openblas_set_num_threads_local(int n);
+
+typedef double complex zcx;
+
+struct server;
+typedef struct {
+    struct server  *S;
+    int             id;
+    pthread_t       th;
+    bbjd_workspace *ws;
+    int             req;        /* assigned request or -1 */
+    size_t          held;       /* bytes charged for its on-demand operator */
+    int             ok;
+} worker_t;
+
+typedef struct server {
+    pthread_mutex_t   mu;
+    pthread_cond_t    cv_work, cv_free;
+    bbjd_request     *reqs;
+    int               nreq, stop, active, peak_active;
+    size_t            budget, used, peak;
+    double            t0;
+    worker_t         *w;
+    int               nw;
+    bbjd_params       cap;
+    bbjd_gpu_options  opt;
+    int               init_failed;
+} server_t;
+
+static double now_s(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+int bbjd_device_init(int sync_yield)
+{
+    if (sync_yield) hipSetDeviceFlags(hipDeviceScheduleYield);
+    hipFree(0);
+    return 0;
+}
+
+void bbjd_serve_config_init(bbjd_serve_config *c)
+{
+    memset(c, 0, sizeof(*c));
+    c->workers = 16;
+    c->precision = BBJD_PREC_FP64;
+    c->realtime = 1;
+    c->sync_yield = 1;
+}
+
+static bbjd_int req_n(const bbjd_request *q) { return q->op ? bbjd_op_size(q->op) : (q->pencil ? q->pencil->n : 0); }
+
+static bbjd_int req_maxsub(const bbjd_request *q)
+{
+    bbjd_int ms = 3 * q->nev + 10;
+    if (ms < 40) ms = 40;
+    const bbjd_int n = req_n(q);
+    return q->maxsub > 0 ? q->maxsub : (ms < n ? ms : n);
+}
+
+static void run_request(server_t *S, worker_t *w, bbjd_request *q)
+{
+    q->t_start = now_s() - S->t0;
+    q->worker = w->id;
+    bbjd_op *op = q->op;
+    int own = 0;
+    hipStream_t st = (hipStream_t)bbjd_workspace_stream(w->ws);
+    if (!op) {
+        op = bbjd_op_create(q->pencil, q->sigma, (bbjd_pc_kind)q->pc, S->opt.precision == BBJD_PREC_MIXED, st);
+        own = 1;
+        if (!op) { q->status = BBJD_ERR_ALLOC; q->t_done = now_s() - S->t0; return; }
+    }
+    bbjd_params p;
+    bbjd_params_init(&p);
+    p.n = bbjd_op_size(op);
+    p.nev = q->nev;
+    p.sigma = q->sigma;
+    p.maxsub = q->maxsub;
+    p.minsub = q->minsub;
+    p.maxiter = q->maxiter > 0 ? q->maxiter : 200;
+    p.tol = q->tol > 0 ? q->tol : 1e-10;
+    p.inner_maxiter = q->inner_maxiter;
+    p.inner_tol = q->inner_tol > 0 ? q->inner_tol : 1e-2;
+    p.matvec_A = bbjd_op_matvec_A;
+    p.matvec_B = bbjd_op_matvec_B;
+    p.precond = bbjd_op_precond;
+    p.op_data = op;
+    bbjd_result R;
+    memset(&R, 0, sizeof(R));
+    q->status = bbjd_solve_in(&p, w->ws, &R);
+    q->nconv = R.nconv;
+    q->iters = R.iters;
+    if (R.nconv > 0) {
+        if (q->eigenvalues) memcpy(q->eigenvalues, R.eigenvalues, (size_t)R.nconv * sizeof(zcx));
+        if (q->residuals) memcpy(q->residuals, R.residuals, (size_t)R.nconv * sizeof(double));
+        if (q->eigenvectors) memcpy(q->eigenvectors, R.eigenvectors, (size_t)R.nconv * p.n * sizeof(zcx));
+        if (q->pair_residuals) bbjd_workspace_pair_residuals(w->ws, q->pair_residuals);
+    }
+    bbjd_stats stt;
+    if (bbjd_workspace_stats(w->ws, &stt) == 0) q->qz_seconds = stt.qz_seconds;
+    bbjd_result_free(&R);
+    if (own) bbjd_op_free(op);
+    q->t_done = now_s() - S->t0;
+}
+
+static void *worker_main(void *arg)
+{
+    worker_t *w = (worker_t *)arg;
+    server_t *S = w->S;
+    /* one LAPACK thread and one OpenMP thread per worker: the workers are
+     * the parallelism */
+    openblas_set_num_threads_local(1);
+    omp_set_num_threads(1);
+    w->ws = bbjd_workspace_alloc_gpu(&S->cap, &S->opt);
+    pthread_mutex_lock(&S->mu);
+    w->ok = w->ws != NULL;
+    if (!w->ok) S->init_failed++;
+    pthread_cond_broadcast(&S->cv_free);
+    for (;;) {
+        while (w->req < 0 && !S->stop) pthread_cond_wait(&S->cv_work, &S->mu);
+        if (w->req < 0 && S->stop) break;
+        const int r = w->req;
+        pthread_mutex_unlock(&S->mu);
+        run_request(S, w, &S->reqs[r]);
+        pthread_mutex_lock(&S->mu);
+        S->used -= w->held;
+        w->held = 0;
+        w->req = -1;
+        S->active--;
+        pthread_cond_broadcast(&S->cv_free);
+    }
+    pthread_mutex_unlock(&S->mu);
+    if (w->ws) bbjd_workspace_free(w->ws);
+    return NULL;
+}
+
+static int cmp_arrival(const void *a, const void *b, void *ctx)
+{
+    const bbjd_request *R = (const bbjd_request *)ctx;
+    const double x = R[*(const int *)a].arrival, y = R[*(const int *)b].arrival;
+    return x < y ? -1 : x > y ? 1 : (*(const int *)a - *(const int *)b);
+}
+
+int bbjd_serve(bbjd_request *reqs, int nreq, const bbjd_serve_config *cfg, bbjd_serve_report *rep)
+{
+    if (!reqs || nreq <= 0 || !cfg || !rep) return -1;
+    memset(rep, 0, sizeof(*rep));
+    if (cfg->sync_yield) hipSetDeviceFlags(hipDeviceScheduleYield);   /* ignored once the device is up */
+    server_t S;
+    memset(&S, 0, sizeof(S));
+    pthread_mutex_init(&S.mu, NULL);
+    pthread_cond_init(&S.cv_work, NULL);
+    pthread_cond_init(&S.cv_free, NULL);
+    S.reqs = reqs;
+    S.nreq = nreq;
+    /* capacity of the resident workspaces: the largest request */
+    bbjd_params_init(&S.cap);
+    S.cap.n = 1; S.cap.nev = 1; S.cap.maxsub = 2; S.cap.inner_maxiter = 0;
+    for (int i = 0; i < nreq; i++) {
+        bbjd_request *q = &reqs[i];
+        const bbjd_int n = req_n(q);
+        if (n <= 0 || q->nev <= 0) return -1;
+        if (n > S.cap.n) S.cap.n = n;
+        if (q->nev > S.cap.nev) S.cap.nev = q->nev;
+        const bbjd_int ms = req_maxsub(q);
+        if (ms > S.cap.maxsub) S.cap.maxsub = ms;
+        if (q->inner_maxiter > S.cap.inner_maxiter) S.cap.inner_maxiter = q->inner_maxiter;
+        q->status = 0; q->nconv = 0; q->iters = 0; q->worker = -1;
+    }
+    bbjd_gpu_options_init(&S.opt);
+    S.opt.callback_space = BBJD_CALLBACK_DEVICE;
+    S.opt.precision = cfg->precision;
+    if (cfg->precision == BBJD_PREC_MIXED) {
+        S.opt.matvec_A_c64 = bbjd_op_matvec_A_c64;
+        S.opt.matvec_B_c64 = bbjd_op_matvec_B_c64;
+        S.opt.precond_c64 = bbjd_op_precond_c64;
+    }
+    const size_t wsb = bbjd_workspace_bytes(&S.cap, &S.opt);
+    size_t freeb = 0, totb = 0;
+    if (hipMemGetInfo(&freeb, &totb) != hipSuccess) return -1;
+    S.budget = cfg->hbm_budget ? cfg->hbm_budget : (size_t)(0.9 * (double)freeb);
+    int nw = cfg->workers > 0 ? cfg->workers : 16;
+    if (nw > nreq) nw = nreq;
+    while (nw > 1 && (size_t)nw * wsb > S.budget) nw--;
+    if ((size_t)nw * wsb > S.budget) return -1;
+    S.used = (size_t)nw * wsb;
+    S.peak = S.used;
+    rep->workers = nw;
+    rep->workspace_bytes = wsb;
+    S.nw = nw;
+    S.w = calloc((size_t)nw, sizeof(worker_t));
+    int *order = malloc((size_t)nreq * sizeof(int));
+    if (!S.w || !order) { free(S.w); free(order); return -1; }
+    for (int i = 0; i < nreq; i++) order[i] = i;
+    qsort_r(order, (size_t)nreq, sizeof(int), cmp_arrival, reqs);
+
+    S.t0 = now_s();
+    for (int i = 0; i < nw; i++) {
+        S.w[i].S = &S; S.w[i].id = i; S.w[i].req = -1;
+        pthread_create(&S.w[i].th, NULL, worker_main, &S.w[i]);
+    }
+    /* wait for the workspaces */
+    pthread_mutex_lock(&S.mu);
+    for (;;) {
+        int up = 0;
+        for (int i = 0; i < nw; i++) up += S.w[i].ok;
+        if (up + S.init_failed >= nw) break;
+        pthread_cond_wait(&S.cv_free, &S.mu);
+    }
+    const int failed_init = S.init_failed;
+    pthread_mutex_unlock(&S.mu);
+    int rc = 0;
+    if (failed_init == nw) rc = -1;
+    S.t0 = now_s();     /* the run starts with every workspace resident */
+    for (int oi = 0; oi < nreq && rc == 0; oi++) {
+        const int r = order[oi];
+        bbjd_request *q = &reqs[r];
+        if (cfg->realtime) {
+            double wait = q->arrival - (now_s() - S.t0);
+            if (wait > 0) {
+                struct timespec ts = {(time_t)wait, (long)((wait - (double)(time_t)wait) * 1e9)};
+                nanosleep(&ts, NULL);
+            }
+        }
+        const size_t need = q->op ? 0 : bbjd_op_bytes_for(q->pencil, q->pc, cfg->precision == BBJD_PREC_MIXED);
+        pthread_mutex_lock(&S.mu);
+        int slot = -1;
+        for (;;) {
+            for (int i = 0; i < nw; i++)
+                if (S.w[i].ok && S.w[i].req < 0) { slot = i; break; }
+            if (slot >= 0 && (S.used + need <= S.budget || S.active == 0)) break;
+            slot = -1;
+            pthread_cond_wait(&S.cv_free, &S.mu);
+        }
+        S.used += need;
+        if (S.used > S.peak) S.peak = S.used;
+        S.w[slot].held = need;
+        S.w[slot].req = r;
+        S.active++;
+        if (S.active > S.peak_active) S.peak_active = S.active;
+        q->t_admit = now_s() - S.t0;
+        pthread_cond_broadcast(&S.cv_work);
+        pthread_mutex_unlock(&S.mu);
+    }
+    pthread_mutex_lock(&S.mu);
+    while (S.active > 0) pthread_cond_wait(&S.cv_free, &S.mu);
+    S.stop = 1;
+    pthread_cond_broadcast(&S.cv_work);
+    pthread_mutex_unlock(&S.mu);
+    for (int i = 0; i < nw; i++) pthread_join(S.w[i].th, NULL);
+    rep->wall_seconds = now_s() - S.t0;
+    rep->peak_bytes = S.peak;
+    rep->peak_concurrency = S.peak_active;
+    for (int i = 0; i < nreq; i++) {
+        rep->pairs += reqs[i].nconv;
+        if (reqs[i].status < 0) rep->failed++;
+    }
+    free(order);
+    free(S.w);
+    pthread_mutex_destroy(&S.mu);
+    pthread_cond_destroy(&S.cv_work);
+    pthread_cond_destroy(&S.cv_free);
+    return rc;
+}

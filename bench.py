@@ -1,0 +1,260 @@
+#!/usr/bin/env python3
+"""bench.py -- converged eigenpairs/s at tol 1e-10, served vs plain, one MI355X,
+with p50/p99 solve latency.
+
+Workload (one step): the serving-loop stream of 256 JDQZ requests with
+Poisson arrivals at a fixed rate; each request draws n from {2e4, 5e4, 1e5,
+2e5} (block size 8, block-tridiagonal, 4 dense tau rows, CN(0,1) entries +4
+on the diagonal blocks), k from {8, 16, 32}, sigma uniform on [-4,4]+[-4,4]i,
+tol 1e-10, maxsub 4k, minsub 2k, exact banded LU of A - sigma B as the
+preconditioner.  The operators (pencil + factorisation) are built and resident
+in HBM before the timed region; the timed region is the serving loop.
+
+  value        served eigenpairs/s over the timed steps (all pairs converged
+               at tol, each pair's FP64 residual checked)
+  plain        the same solves one at a time (FP64 engine, no overlap), on a
+               seeded sample of the requests -> eigenpairs/s
+  latency      p50/p99 of completion - arrival over the timed requests
+  roofline     the projection kernel Q^H x (the dominant streaming kernel),
+               HIP-event durations of every launch in the timed region vs
+               its algorithmic bytes 16 n (k + column tiles), HBM peak 8 TB/s
+  cpu_baseline the reference solver compiled from its sources (oracle/_ref;
+               else the oracle restatement) on the host cores, on a bounded
+               sample of the workload's requests, CPU exact banded LU
+
+Usage: python bench.py [--gpus 1] [--steps 1] [--warmup 1] [--nreq 256]
+One JSON line on stdout; progress on stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_requests(nreq, rate, seed):
+    rng = np.random.default_rng(seed)
+    ns = rng.choice([20000, 50000, 100000, 200000], size=nreq)
+    ks = rng.choice([8, 16, 32], size=nreq)
+    sig = rng.uniform(-4, 4, size=nreq) + 1j * rng.uniform(-4, 4, size=nreq)
+    arr = np.cumsum(rng.exponential(1.0 / rate, size=nreq))
+    pseeds = rng.integers(1, 2**62, size=nreq)
+    return [dict(n=int(ns[i]), k=int(ks[i]), sigma=complex(sig[i]), arrival=float(arr[i]),
+                 pseed=int(pseeds[i])) for i in range(nreq)]
+
+
+def norm_lower_bound(p):
+    """max row 2-norm of A: a lower bound of ||A||_2 (stricter residual test)"""
+    nb, nd, bs = p.n // p.bs, 2 * p.hw + 1, p.bs
+    blk = p.band.reshape(nb, nd, bs, bs)           # [bi, d, c, r]
+    rows = np.sqrt((np.abs(blk) ** 2).sum(axis=(1, 2))).reshape(-1)[: p.n1]
+    m = rows.max() if rows.size else 0.0
+    if p.ntau:
+        m = max(m, np.sqrt((np.abs(p.tau.reshape(p.ntau, p.n)) ** 2).sum(axis=1)).max())
+    return float(m)
+
+
+def pct(a, q):
+    return float(np.percentile(np.asarray(a), q)) if len(a) else None
+
+
+def cpu_baseline(reqs, budget_s, threads):
+    """Reference solver on the host (oracle/_ref if built, else the oracle)."""
+    from bbjd_amd import BandedTauPencil
+    from oracle import orc
+    which = "ref" if orc.have_ref() else "oracle"
+    lib = orc.load_oracle()
+    lib.orc_set_threads(int(threads))
+    pairs, secs, done = 0, 0.0, []
+    # smallest requests first, so the bounded sample finishes a few solves
+    for r in sorted(reqs, key=lambda r: (r["n"], r["k"])):
+        if secs >= budget_s:
+            break
+        p = BandedTauPencil.generate(r["n"], 8, 1, 4, seed=r["pseed"])
+        ops = orc.CpuPencilOps(p, r["sigma"], 1)
+        t = time.perf_counter()
+        out = orc.solve(which, p.n, r["k"], r["sigma"], tol=1e-10, maxsub=4 * r["k"], minsub=2 * r["k"],
+                        maxiter=r.get("maxiter", 1000), ops=ops)
+        dt = time.perf_counter() - t
+        pairs += out["nconv"]
+        secs += dt
+        done.append(f"n={r['n']} k={r['k']}: {out['nconv']} pairs {dt:.1f}s")
+    return {"value": pairs / secs if secs else None, "unit": "eigenpairs/s", "cores": int(threads),
+            "kind": "reference" if which == "ref" else "port",
+            "sample": f"{len(done)} of the workload's requests, smallest first, solved to tol 1e-10 "
+                      f"with a CPU exact banded LU (LAPACK zgbtrf + Schur complement of the tau rows): "
+                      + "; ".join(done)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--nreq", type=int, default=256)
+    ap.add_argument("--rate", type=float, default=32.0, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--workers", type=int, default=16)
+    ap.add_argument("--maxiter", type=int, default=1000)
+    ap.add_argument("--plain-sample", type=int, default=16)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--seed", type=int, default=20261203)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the solves of this workload are independent and "
+              "would be placed per GPU without communication; run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from bbjd_amd import BandedTauPencil, DeviceOperator
+    from bbjd_amd import serve as srv
+
+    srv.device_init(sync_yield=True)
+    reqs = make_requests(args.nreq, args.rate, args.seed)
+    for r in reqs:
+        r["maxiter"] = args.maxiter
+
+    # ---- resident operators (not timed) ----
+    t0 = time.perf_counter()
+    ops, norms = [], []
+    for i, r in enumerate(reqs):
+        p = BandedTauPencil.generate(r["n"], 8, 1, 4, seed=r["pseed"])
+        norms.append(norm_lower_bound(p))
+        op = DeviceOperator(p, r["sigma"], pc="exact")
+        op.pencil = None          # host copy no longer needed
+        ops.append(op)
+    t_setup = time.perf_counter() - t0
+    op_bytes = sum(o.device_bytes for o in ops)
+    log(f"[bench] {len(ops)} operators resident ({op_bytes / 1e9:.1f} GB) in {t_setup:.1f}s")
+
+    def jobs_for(idx, with_arrival=True):
+        return [dict(op=ops[i], sigma=reqs[i]["sigma"], nev=reqs[i]["k"], maxsub=4 * reqs[i]["k"],
+                     minsub=2 * reqs[i]["k"], maxiter=reqs[i]["maxiter"], tol=1e-10,
+                     arrival=reqs[i]["arrival"] if with_arrival else 0.0) for i in idx]
+
+    # ---- warmup (untimed): short streams ----
+    for w in range(args.warmup):
+        idx = list(range(min(16, len(reqs))))
+        srv.serve(jobs_for(idx, False), workers=args.workers, realtime=False)
+        log(f"[bench] warmup step {w} done")
+
+    # ---- timed: served ----
+    all_idx = list(range(len(reqs)))
+    served, walls = [], []
+    srv.ktime_enable(True)
+    for s in range(args.steps):
+        t = time.perf_counter()
+        res, rep = srv.serve(jobs_for(all_idx), workers=args.workers, realtime=True)
+        walls.append(time.perf_counter() - t)
+        served.append((res, rep))
+        log(f"[bench] step {s}: {rep['pairs']} pairs in {walls[-1]:.1f}s, workers {rep['workers']}, "
+            f"peak concurrency {rep['peak_concurrency']}")
+    srv.ktime_enable(False)
+    kcnt, kms, kbytes = srv.ktime_harvest()
+
+    pairs = sum(rep["pairs"] for _, rep in served)
+    wall = sum(walls)
+    lat = [r["t_done"] - r["arrival"] for res, _ in served for r in res]
+    # FP64 residual of every pair: ||A x - l B x|| / (||A|| + |l| ||B||) <= tol
+    worst = 0.0
+    npairs_checked = 0
+    bad = 0
+    for res, _ in served:
+        for i, r in enumerate(res):
+            for lam, pr in zip(r["eigenvalues"], r["pair_residuals"]):
+                v = pr / (norms[i] + abs(lam) * 1.0)
+                worst = max(worst, v)
+                npairs_checked += 1
+                bad += v > 1e-10
+            bad += sum(r["residuals"] >= 1e-10)
+    statuses = [r["status"] for res, _ in served for r in res]
+    log(f"[bench] served: {pairs} pairs / {wall:.1f}s; worst normalised residual {worst:.2e}; "
+        f"bad {bad}; statuses {sorted(set(statuses))}")
+
+    # ---- plain: one solve at a time, FP64, seeded sample ----
+    rng = np.random.default_rng(args.seed + 1)
+    sample = sorted(rng.choice(len(reqs), size=min(args.plain_sample, len(reqs)), replace=False).tolist())
+    t = time.perf_counter()
+    pres, prep = srv.serve(jobs_for(sample, False), workers=1, realtime=False)
+    t_plain = time.perf_counter() - t
+    plain_pairs = prep["pairs"]
+    plain_rate = plain_pairs / t_plain if t_plain else None
+    # same eigenvalues alone and in the mix
+    mix = served[0][0]
+    diff = 0.0
+    for j, i in enumerate(sample):
+        a, b = np.sort_complex(pres[j]["eigenvalues"]), np.sort_complex(mix[i]["eigenvalues"])
+        if a.size and a.size == b.size:
+            diff = max(diff, float(np.max(np.abs(a - b) / np.abs(a))))
+    log(f"[bench] plain sample: {plain_pairs} pairs / {t_plain:.1f}s; "
+        f"max rel eig diff plain vs served {diff:.1e}")
+
+    served_rate = pairs / wall if wall else None
+    roof = None
+    if kcnt:
+        achieved = kbytes / (kms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_proj<double,1,16> (Q^H x, complex128)", "launches": kcnt,
+                "avg_us": round(kms * 1e3 / kcnt, 2), "avg_bytes": round(kbytes / kcnt)}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        thr = args.cpu_threads or min(16, os.cpu_count() or 1)
+        try:
+            cpu = cpu_baseline(reqs, args.cpu_seconds, thr)
+            log(f"[bench] cpu baseline: {cpu['value']} eigenpairs/s ({cpu['kind']}, {cpu['cores']} threads)")
+        except Exception as e:  # noqa: BLE001
+            log(f"[bench] cpu baseline unavailable: {e}")
+
+    out = {
+        "metric": "eigenpairs/s @tol1e-10 (served vs plain, 1 GPU); p50/p99 solve latency",
+        "value": round(served_rate, 3) if served_rate else None,
+        "unit": "eigenpairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * wall / args.steps, 1),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c128",
+        "data": "synthetic",
+        "config": {"workload": f"serving loop: {args.nreq} JDQZ requests, Poisson arrivals "
+                               f"{args.rate}/s, n in {{2e4,5e4,1e5,2e5}} (bs 8 block-tridiagonal, "
+                               f"4 tau rows), k in {{8,16,32}}, sigma in [-4,4]+[-4,4]i, tol 1e-10, "
+                               f"maxsub 4k, minsub 2k, maxiter {args.maxiter}, exact banded LU",
+                   "workers": args.workers, "requests": args.nreq},
+        "served_eigenpairs_per_s": round(served_rate, 3) if served_rate else None,
+        "plain_eigenpairs_per_s": round(plain_rate, 3) if plain_rate else None,
+        "served_over_plain": round(served_rate / plain_rate, 2) if served_rate and plain_rate else None,
+        "plain_sample_requests": len(sample),
+        "latency_p50_s": round(pct(lat, 50), 3) if lat else None,
+        "latency_p99_s": round(pct(lat, 99), 3) if lat else None,
+        "pairs": pairs,
+        "max_normalised_residual": worst,
+        "pairs_over_tol": int(bad),
+        "max_rel_eig_diff_alone_vs_mix": diff,
+        "setup_seconds": round(t_setup, 1),
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

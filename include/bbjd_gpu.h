@@ -56,9 +56,24 @@ size_t bbjd_workspace_bytes(const bbjd_params *p, const bbjd_gpu_options *o);
 bbjd_workspace *bbjd_workspace_alloc_gpu(const bbjd_params *p,
                                          const bbjd_gpu_options *o);
 
+/* Solve on a resident workspace allocated for a larger shape: p->n,
+ * p->nev, the solve's maxsub (p->maxsub, or the reference's auto rule
+ * min(max(3 nev + 10, 40), n)) and p->inner_maxiter may each be at most the
+ * workspace's.  bbjd_solve keeps the reference rule that the workspace's
+ * maxsub is the solve's (reference src/jd_core.c:749). */
+bbjd_status bbjd_solve_in(const bbjd_params *p, bbjd_workspace *ws, bbjd_result *r);
+
+/* ||A x_j - lambda_j B x_j|| (||x_j|| = 1) of each pair converged in the last
+ * solve on ws, from the FP64 operator applies of the convergence test.
+ * Writes nconv values, returns nconv (or -1). */
+int bbjd_workspace_pair_residuals(const bbjd_workspace *ws, double *out);
+
 /* The stream on which the engine is running the current callback of the
  * calling thread (hipStream_t), for DEVICE-space callbacks. */
 void *bbjd_callback_stream(void);
+
+/* hipStream_t the workspace runs on */
+void *bbjd_workspace_stream(const bbjd_workspace *ws);
 
 /* Per-solve counters of the last bbjd_solve on a workspace. */
 typedef struct {
@@ -75,6 +90,19 @@ typedef struct {
 } bbjd_stats;
 
 int bbjd_workspace_stats(const bbjd_workspace *ws, bbjd_stats *st);
+
+/* First GPU call of a process that serves many solves from many host
+ * threads: sync_yield = 1 makes threads waiting on the GPU yield their core
+ * (the host QZ of other solves needs it).  Returns 0 (also when the device
+ * was already initialised and the flag could not change). */
+int bbjd_device_init(int sync_yield);
+
+/* Launch timing of the engine's dominant streaming kernel (the one-vector
+ * projection Q^H x): enable, run, then harvest the number of timed launches,
+ * the sum of their durations (ms, HIP events on the launching stream) and of
+ * their algorithmic bytes (16 n (k + column tiles)). */
+void bbjd_ktime_enable(int on);
+int  bbjd_ktime_harvest(double *total_ms, double *total_bytes);
 
 /* Device pointer -> host vector download helper for tests/tools. */
 int bbjd_memcpy_d2h(void *dst, const void *src, size_t bytes);

@@ -64,6 +64,9 @@ bbjd_op *bbjd_op_create(const bbjd_pencil_desc *d, bbjd_zomplex sigma,
                         bbjd_pc_kind pc, int want_c64, void *stream);
 void     bbjd_op_free(bbjd_op *op);
 size_t   bbjd_op_bytes(const bbjd_op *op);   /* device bytes held */
+bbjd_int bbjd_op_size(const bbjd_op *op);    /* n */
+/* device bytes bbjd_op_create(d, ., pc, want_c64, .) will take */
+size_t   bbjd_op_bytes_for(const bbjd_pencil_desc *d, int pc, int want_c64);
 
 /* DEVICE-space callbacks (bbjd_matvec_fn / bbjd_precond_fn); data is the
  * bbjd_op*.  They run on bbjd_callback_stream() when called by the engine,

@@ -1,0 +1,72 @@
+/*
+ * bbjd_serve.h -- continuous-batching serving loop for independent solves.
+ *
+ * No counterpart in the reference: there, each shift sigma of an
+ * eigenspectrum sweep is an independent solve on its own MPI rank
+ * (reference README.md:59-83, include/bbjd.h:14-37).  Here the solves are
+ * requests served on one GPU: each worker owns a host thread, a HIP stream
+ * and a resident workspace sized for the largest request; requests are
+ * admitted in arrival order as workers and HBM free up, so the host QZ of
+ * one request overlaps the device work of the others.
+ */
+#ifndef BBJD_SERVE_H
+#define BBJD_SERVE_H
+
+#include "bbjd_ops.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    /* problem: a resident operator, or a host pencil the server uploads and
+     * factors at admission (and frees at completion) */
+    bbjd_op                *op;
+    const bbjd_pencil_desc *pencil;
+    int                     pc;           /* bbjd_pc_kind for an on-demand op  */
+    bbjd_zomplex            sigma;
+    bbjd_int                nev, maxsub, minsub, maxiter, inner_maxiter;
+    double                  tol, inner_tol;
+    double                  arrival;      /* seconds after the start of the run */
+    /* outputs (caller-allocated arrays of nev entries; eigenvectors n x nev
+     * column-major or NULL) */
+    bbjd_zomplex           *eigenvalues;
+    double                 *residuals;       /* reference criterion           */
+    double                 *pair_residuals;  /* ||A x - lambda B x||, ||x|| = 1 */
+    bbjd_zomplex           *eigenvectors;
+    bbjd_int                nconv, iters;
+    int                     status;
+    int                     worker;
+    double                  t_admit, t_start, t_done;   /* seconds after start */
+    double                  qz_seconds;
+} bbjd_request;
+
+typedef struct {
+    int    workers;       /* concurrent solves (0: 16)                       */
+    size_t hbm_budget;    /* bytes for workspaces + on-demand operators
+                             (0: 90% of the free device memory at start)     */
+    int    precision;     /* bbjd_precision of every request                 */
+    int    realtime;      /* 1: admit a request no earlier than its arrival  */
+    int    sync_yield;    /* 1: host threads yield while waiting on the GPU  */
+} bbjd_serve_config;
+
+typedef struct {
+    double   wall_seconds;
+    bbjd_int pairs;         /* converged pairs, all requests                 */
+    bbjd_int failed;        /* requests with a negative status               */
+    int      workers;       /* workers actually run (after the HBM budget)   */
+    size_t   workspace_bytes;
+    size_t   peak_bytes;    /* workspaces + on-demand operators at the peak  */
+    int      peak_concurrency;
+} bbjd_serve_report;
+
+void bbjd_serve_config_init(bbjd_serve_config *c);
+
+/* Serves nreq requests; returns 0 when the run itself worked (individual
+ * request failures are in their status), -1 on a set-up error. */
+int bbjd_serve(bbjd_request *reqs, int nreq, const bbjd_serve_config *cfg, bbjd_serve_report *rep);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

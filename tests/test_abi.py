@@ -1,0 +1,114 @@
+"""The drop-in boundary on the CPU: libbbjd.so loads without a GPU and
+exports every function include/*.h declares; the public structs and enums
+have the reference header's layout and values (compiled probe against both
+headers); the Python wrapper validates input like the reference's
+(bbjd.py:160-224) before touching the device."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from bbjd_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+INC = os.path.join(ROOT, "include")
+REF_INC = "/root/reference/include"
+
+
+def declared_functions():
+    names = set()
+    for h in sorted(os.listdir(INC)):
+        txt = open(os.path.join(INC, h)).read()
+        txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+        for m in re.finditer(r"^\s*(?:[A-Za-z_][\w\s\*]*?)\b(bbjd_\w+)\s*\(", txt, flags=re.M):
+            if "typedef" in txt[max(0, m.start() - 40):m.start() + 1].split("\n")[-1]:
+                continue
+            names.add(m.group(1))
+    return sorted(names)
+
+
+def test_library_loads_and_exports_every_declared_function():
+    lib = _lib.load()
+    names = declared_functions()
+    assert "bbjd_solve" in names and "bbjd_correction_gmres" in names and "bbjd_serve" in names
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+PROBE = r"""
+#include <stdio.h>
+#include <stddef.h>
+#include "bbjd.h"
+#define O(T, f) printf(#T "." #f " %zu\n", offsetof(T, f))
+int main(void) {
+  printf("sizeof.params %zu\nsizeof.result %zu\n", sizeof(bbjd_params), sizeof(bbjd_result));
+  O(bbjd_params, n); O(bbjd_params, nev); O(bbjd_params, sigma); O(bbjd_params, target);
+  O(bbjd_params, maxsub); O(bbjd_params, minsub); O(bbjd_params, tol); O(bbjd_params, maxiter);
+  O(bbjd_params, inner_maxiter); O(bbjd_params, inner_tol); O(bbjd_params, v0);
+  O(bbjd_params, matvec_A); O(bbjd_params, matvec_B); O(bbjd_params, precond);
+  O(bbjd_params, op_data); O(bbjd_params, verbose);
+  O(bbjd_result, nconv); O(bbjd_result, eigenvalues); O(bbjd_result, eigenvectors);
+  O(bbjd_result, residuals); O(bbjd_result, iters); O(bbjd_result, status);
+  O(bbjd_workspace, n); O(bbjd_workspace, V); O(bbjd_workspace, MA); O(bbjd_workspace, nlock);
+  O(bbjd_workspace, kdim); O(bbjd_workspace, gmres_V); O(bbjd_workspace, inner_maxiter);
+  O(bbjd_workspace, tmp4); O(bbjd_workspace, swap_nm); O(bbjd_workspace, Q_left);
+  O(bbjd_workspace, h_work);
+  printf("enum %d %d %d %d %d %d %d\n", BBJD_SUCCESS, BBJD_MAX_ITER, BBJD_ERR_ALLOC, BBJD_ERR_INVALID,
+         BBJD_ERR_CALLBACK, BBJD_ERR_LAPACK, BBJD_ERR_BREAKDOWN);
+  printf("target %d %d %d %d %d %d %d\n", BBJD_TARGET_NEAREST, BBJD_TARGET_LARGEST_MAG,
+         BBJD_TARGET_SMALLEST_MAG, BBJD_TARGET_LARGEST_REAL, BBJD_TARGET_SMALLEST_REAL,
+         BBJD_TARGET_LARGEST_IMAG, BBJD_TARGET_SMALLEST_IMAG);
+  return 0;
+}
+"""
+
+
+def _probe(incdir, tmp_path, tag):
+    src = tmp_path / f"probe_{tag}.c"
+    exe = tmp_path / f"probe_{tag}"
+    src.write_text(PROBE)
+    subprocess.run(["gcc", "-std=c11", f"-I{incdir}", str(src), "-o", str(exe)], check=True)
+    return subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+
+
+def test_struct_layout_matches_ctypes(tmp_path):
+    out = dict(line.rsplit(" ", 1) for line in _probe(INC, tmp_path, "mine").splitlines()
+               if not line.startswith(("enum", "target")))
+    assert int(out["sizeof.params"]) == C.sizeof(_lib.Params)
+    assert int(out["sizeof.result"]) == C.sizeof(_lib.Result)
+    for f, _ in _lib.Params._fields_:
+        assert int(out[f"bbjd_params.{f}"]) == getattr(_lib.Params, f).offset, f
+    for f, _ in _lib.Result._fields_:
+        assert int(out[f"bbjd_result.{f}"]) == getattr(_lib.Result, f).offset, f
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(REF_INC, "bbjd.h")), reason="reference header absent")
+def test_layout_and_enums_identical_to_reference_header(tmp_path):
+    mine = _probe(INC, tmp_path, "mine").splitlines()
+    ref = _probe(REF_INC, tmp_path, "ref").splitlines()
+    assert mine == ref
+
+
+def test_wrapper_validates_like_the_reference():
+    from bbjd_amd import jdqz
+    A = sp.eye(10, dtype=np.complex128, format="csr")
+    with pytest.raises(ValueError):
+        jdqz(sp.random(10, 12, density=0.5, format="csr"), k=2)
+    with pytest.raises(ValueError):
+        jdqz(A, k=2, which="XX")
+    with pytest.raises(ValueError):
+        jdqz(A, k=2, v0=np.ones(9))
+    with pytest.raises(ValueError):
+        jdqz(A, k=2, preconditioner=3)
+    with pytest.raises(ValueError):
+        jdqz(A, B=object(), k=2)
+
+
+def test_target_and_status_tables():
+    from bbjd_amd import bbjd
+    assert bbjd._TARGET_MAP == {'nearest': 0, 'LM': 1, 'SM': 2, 'LR': 3, 'SR': 4, 'LI': 5, 'SI': 6}
+    assert sorted(bbjd._STATUS_MSG) == [-5, -4, -3, -2, -1]
