@@ -93,6 +93,11 @@ def load():
     _sig(lib, "bbjd_workspace_alloc_gpu", V, P, C.POINTER(GpuOptions))
     _sig(lib, "bbjd_callback_stream", V)
     _sig(lib, "bbjd_workspace_stats", C.c_int, V, C.POINTER(Stats))
+    _sig(lib, "bbjd_dev_alloc", V, C.c_size_t)
+    _sig(lib, "bbjd_dev_free", None, V)
+    _sig(lib, "bbjd_dev_sync", C.c_int)
+    _sig(lib, "bbjd_time_op", C.c_int, V, I64, V, V, V, C.c_int, C.POINTER(D))
+    _sig(lib, "bbjd_kbench", C.c_int, C.c_int, I64, C.c_int, C.c_int, C.POINTER(D), C.POINTER(D))
     _sig(lib, "bbjd_memcpy_d2h", C.c_int, V, V, C.c_size_t)
     _sig(lib, "bbjd_memcpy_h2d", C.c_int, V, V, C.c_size_t)
     _sig(lib, "bbjd_pencil_generate", C.c_int, I64, C.c_int, C.c_int, C.c_int, C.c_uint64, D, V, V, V)
@@ -104,6 +109,43 @@ def load():
         _sig(lib, nm, C.c_int, I64, V, V, V)
     _lib = lib
     return lib
+
+
+class DeviceBuffer:
+    """A device allocation made by the library (no other GPU runtime)."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        self.ptr = load().bbjd_dev_alloc(self.nbytes)
+        if not self.ptr:
+            raise MemoryError(f"device allocation of {nbytes} bytes failed")
+
+    @classmethod
+    def from_array(cls, a):
+        import numpy as np
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        if load().bbjd_memcpy_h2d(b.ptr, a.ctypes.data, a.nbytes):
+            raise RuntimeError("host-to-device copy failed")
+        return b
+
+    def to_array(self, dtype, count):
+        import numpy as np
+        a = np.empty(count, dtype=dtype)
+        if load().bbjd_memcpy_d2h(a.ctypes.data, self.ptr, a.nbytes):
+            raise RuntimeError("device-to-host copy failed")
+        return a
+
+    def free(self):
+        if self.ptr:
+            load().bbjd_dev_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 def fn_addr(name: str) -> int:

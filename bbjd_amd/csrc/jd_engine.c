@@ -283,6 +283,39 @@ int bbjd_workspace_stats(const bbjd_workspace *ws, bbjd_stats *st)
     return 0;
 }
 
+void *bbjd_dev_alloc(size_t bytes)
+{
+    void *p = NULL;
+    return hipMalloc(&p, bytes ? bytes : 1) == hipSuccess ? p : NULL;
+}
+
+void bbjd_dev_free(void *p)
+{
+    if (p) hipFree(p);
+}
+
+int bbjd_dev_sync(void) { return HIPOK(hipDeviceSynchronize()) ? 0 : -1; }
+
+int bbjd_time_op(bbjd_matvec_fn f, bbjd_int n, const void *x, void *y, void *data, int reps, double *avg_ms)
+{
+    hipEvent_t e0, e1;
+    if (!f || reps <= 0 || !HIPOK(hipEventCreate(&e0))) return -1;
+    if (!HIPOK(hipEventCreate(&e1))) { hipEventDestroy(e0); return -1; }
+    int rc = 0;
+    rc |= f(n, (const bbjd_zomplex *)x, (bbjd_zomplex *)y, data);       /* warm */
+    rc |= !HIPOK(hipDeviceSynchronize());
+    rc |= !HIPOK(hipEventRecord(e0, NULL));
+    for (int r = 0; r < reps && !rc; r++) rc |= f(n, (const bbjd_zomplex *)x, (bbjd_zomplex *)y, data);
+    rc |= !HIPOK(hipEventRecord(e1, NULL));
+    float ms = 0;
+    rc |= !HIPOK(hipEventSynchronize(e1));
+    rc |= !HIPOK(hipEventElapsedTime(&ms, e0, e1));
+    if (avg_ms) *avg_ms = ms / reps;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    return rc ? -1 : 0;
+}
+
 int bbjd_memcpy_d2h(void *dst, const void *src, size_t bytes)
 { return HIPOK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost)) ? 0 : -1; }
 int bbjd_memcpy_h2d(void *dst, const void *src, size_t bytes)

@@ -42,7 +42,7 @@ hipEvent_t kt_event()
 {
     if (!g_kt_pool.empty()) { hipEvent_t e = g_kt_pool.back(); g_kt_pool.pop_back(); return e; }
     hipEvent_t e;
-    hipEventCreate(&e);
+    (void)hipEventCreate(&e);
     return e;
 }
 }  // namespace
@@ -473,12 +473,12 @@ static int proj_pass(hipStream_t s, i64 n, int k, const void *Q, i64 ldq, const 
                 std::lock_guard<std::mutex> lk(g_kt_mu);
                 ea = kt_event(); eb = kt_event();
             }
-            if (timed) hipEventRecord(ea, s);
+            if (timed) (void)hipEventRecord(ea, s);
             hipLaunchKernelGGL((k_proj<R, 1, C>), g, dim3(TPB), 0, s, (i64)n, k, (const Cx<R> *)Q, (i64)ldq,
                                (const Cx<R> *)x0, (const Cx<R> *)nullptr, (Cx<R> *)rd->part, ldp, (R *)rd->pnorm,
                                want_norm, cond);
             if (timed) {
-                hipEventRecord(eb, s);
+                (void)hipEventRecord(eb, s);
                 const double bytes = (double)sizeof(Cx<R>) * (double)n * ((double)k + (double)g.x);
                 std::lock_guard<std::mutex> lk(g_kt_mu);
                 g_kt.push_back({ea, eb, bytes});
@@ -734,6 +734,106 @@ int bbjd_k_c2z(hipStream_t s, int64_t n, const void *c, void *z)
     hipLaunchKernelGGL(k_c2z, dim3(grid_stride_blocks(n)), dim3(TPB), 0, s, (i64)n, (const Cx<float> *)c,
                        (Cx<double> *)z);
     return chk();
+}
+
+// ---------------------------------------------------------------------------
+// Kernel benchmark: times one kernel in isolation with HIP events on a private
+// stream.  which: 0 k_proj one vector, 1 k_proj two vectors, 2 k_update,
+// 3 k_rotate (k -> k/2 columns), 4 full CGS2 (one pass + skipped pass 2).
+// Returns the average ms per launch and the algorithmic bytes per launch.
+__global__ void k_fill(i64 cnt, Cx<double> *a, double scale)
+{
+    for (i64 i = (i64)blockIdx.x * TPB + threadIdx.x; i < cnt; i += (i64)gridDim.x * TPB)
+        a[i] = cx(scale * (double)((i * 2654435761LL) % 1000 - 500) * 1e-3, scale * (double)((i * 40503LL) % 997 - 498) * 1e-3);
+}
+
+int bbjd_kbench(int which, int64_t n, int k, int reps, double *avg_ms, double *bytes)
+{
+    if (n <= 0 || k <= 0 || k > BBJD_MAXK || reps <= 0) return -2;
+    hipStream_t s;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return -1;
+    const i64 ld = (n + 63) / 64 * 64;
+    void *Q = nullptr, *x = nullptr, *x2 = nullptr, *blk = nullptr, *Zr = nullptr;
+    const i64 pe = bbjd_red_part_entries(n, k + 2);
+    int rc = 0;
+    rc |= hipMalloc(&Q, (size_t)ld * k * 16) != hipSuccess;
+    rc |= hipMalloc(&x, (size_t)ld * 16) != hipSuccess;
+    rc |= hipMalloc(&x2, (size_t)ld * 16) != hipSuccess;
+    rc |= hipMalloc(&blk, (size_t)pe * 16 + (size_t)(ntiles_proj(n) + 64) * 8 + (size_t)(k + 8) * 16 * 5 + 4096) != hipSuccess;
+    rc |= hipMalloc(&Zr, (size_t)k * k * 16) != hipSuccess;
+    if (rc) { hipFree(Q); hipFree(x); hipFree(x2); hipFree(blk); hipFree(Zr); hipStreamDestroy(s); return -1; }
+    bbjd_red rd;
+    char *b = (char *)blk;
+    rd.part = b; b += (size_t)pe * 16;
+    rd.pnorm = (double *)b; b += (size_t)(ntiles_proj(n) + 64) * 8;
+    rd.h = b; b += (size_t)(k + 8) * 16;
+    rd.h2 = b; b += (size_t)(k + 8) * 16;
+    rd.hsum = b; b += (size_t)(k + 8) * 16;
+    rd.coef = b; b += (size_t)(k + 8) * 16;
+    rd.coef2 = b; b += (size_t)(k + 8) * 16;
+    rd.scal = (double *)b; rd.flag = (int *)(b + 256);
+    rd.part_cap = pe;
+    hipLaunchKernelGGL(k_fill, dim3(4096), dim3(TPB), 0, s, (i64)ld * k, (Cx<double> *)Q, 1.0);
+    hipLaunchKernelGGL(k_fill, dim3(1024), dim3(TPB), 0, s, (i64)ld, (Cx<double> *)x, 0.7);
+    hipLaunchKernelGGL(k_fill, dim3(1024), dim3(TPB), 0, s, (i64)ld, (Cx<double> *)x2, 0.3);
+    hipLaunchKernelGGL(k_fill, dim3(64), dim3(TPB), 0, s, (i64)k * k, (Cx<double> *)Zr, 0.1);
+    hipLaunchKernelGGL(k_fill, dim3(1), dim3(TPB), 0, s, (i64)(k + 8), (Cx<double> *)rd.h, 0.01);
+    (void)hipMemsetAsync(rd.flag, 0, 64, s);
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    const int nt = ntiles_proj(n);
+    double by = 0;
+    auto launch = [&]() {
+        switch (which) {
+        case 0: {
+            dim3 g((k + 15) / 16, nt);
+            hipLaunchKernelGGL((k_proj<double, 1, 16>), g, dim3(TPB), 0, s, (i64)n, k, (const Cx<double> *)Q, ld,
+                               (const Cx<double> *)x, (const Cx<double> *)nullptr, (Cx<double> *)rd.part, k,
+                               rd.pnorm, 0, (const int *)nullptr);
+            by = 16.0 * n * (k + g.x);
+            break;
+        }
+        case 1: {
+            dim3 g((k + 7) / 8, nt);
+            hipLaunchKernelGGL((k_proj<double, 2, 8>), g, dim3(TPB), 0, s, (i64)n, k, (const Cx<double> *)Q, ld,
+                               (const Cx<double> *)x, (const Cx<double> *)x2, (Cx<double> *)rd.part, k, rd.pnorm,
+                               0, (const int *)nullptr);
+            by = 16.0 * n * (k + 2.0 * g.x);
+            break;
+        }
+        case 2:
+            hipLaunchKernelGGL((k_update<double>), dim3(ntiles_upd(n)), dim3(TPB), (size_t)k * 16, s, (i64)n, k,
+                               (const Cx<double> *)Q, ld, (const Cx<double> *)rd.h, -1.0, 1, (Cx<double> *)x,
+                               rd.pnorm, (const int *)nullptr);
+            by = 16.0 * n * (k + 2);
+            break;
+        case 3:
+            bbjd_k_rotate(s, n, k, k / 2 > 0 ? k / 2 : 1, Q, ld, Zr);
+            by = 16.0 * n * (k + (k / 2 > 0 ? k / 2 : 1));
+            break;
+        default:
+            cgs2<double>(s, n, k, Q, ld, x, &rd, SL_T0, nullptr);
+            by = 16.0 * n * (2.0 * k + 3);
+            break;
+        }
+    };
+    launch();
+    launch();
+    (void)hipEventRecord(e0, s);
+    for (int r = 0; r < reps; r++) launch();
+    (void)hipEventRecord(e1, s);
+    float ms = 0;
+    rc |= hipEventSynchronize(e1) != hipSuccess;
+    rc |= hipEventElapsedTime(&ms, e0, e1) != hipSuccess;
+    rc |= hipGetLastError() != hipSuccess;
+    if (avg_ms) *avg_ms = ms / reps;
+    if (bytes) *bytes = by;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    hipFree(Q); hipFree(x); hipFree(x2); hipFree(blk); hipFree(Zr);
+    (void)hipStreamDestroy(s);
+    return rc ? -1 : 0;
 }
 
 }  // extern "C"

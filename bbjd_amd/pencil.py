@@ -127,16 +127,28 @@ class DeviceOperator:
 
     # host-array convenience applies (copy in, apply on the device, copy out)
     def _apply(self, name, x):
-        import torch
+        lib = _lib.load()
         x = np.ascontiguousarray(x, dtype=np.complex128)
-        dx = torch.from_numpy(x).to("cuda")
-        dy = torch.empty_like(dx)
-        torch.cuda.synchronize()
-        rc = getattr(_lib.load(), name)(self.n, dx.data_ptr(), dy.data_ptr(), self.handle)
-        torch.cuda.synchronize()
+        dx = _lib.DeviceBuffer.from_array(x)
+        dy = _lib.DeviceBuffer(x.nbytes)
+        rc = getattr(lib, name)(self.n, dx.ptr, dy.ptr, self.handle)
+        rc |= lib.bbjd_dev_sync()
         if rc:
             raise RuntimeError(f"{name} failed")
-        return dy.cpu().numpy()
+        return dy.to_array(np.complex128, self.n)
+
+    def time_apply(self, name, reps=20):
+        """average ms of a device apply (HIP events, null stream)"""
+        import ctypes as C
+        lib = _lib.load()
+        x = np.random.default_rng(0).standard_normal(self.n) + 0j
+        dx = _lib.DeviceBuffer.from_array(x)
+        dy = _lib.DeviceBuffer(x.nbytes)
+        ms = C.c_double()
+        rc = lib.bbjd_time_op(_lib.fn_addr(name), self.n, dx.ptr, dy.ptr, self.handle, reps, C.byref(ms))
+        if rc:
+            raise RuntimeError(f"timing {name} failed")
+        return ms.value
 
     def matvec(self, x):
         return self._apply("bbjd_op_matvec_A", x)

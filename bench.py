@@ -192,6 +192,10 @@ def main():
     pres, prep = srv.serve(jobs_for(sample, False), workers=1, realtime=False)
     t_plain = time.perf_counter() - t
     plain_pairs = prep["pairs"]
+    for j, i in enumerate(sample):
+        r = pres[j]
+        log(f"[bench]   plain n={reqs[i]['n']} k={reqs[i]['k']}: {r['nconv']} pairs, {r['iters']} iters, "
+            f"{r['t_done'] - r['t_start']:.2f}s (qz {r['qz_seconds']:.2f}s), status {r['status']}")
     plain_rate = plain_pairs / t_plain if t_plain else None
     # same eigenvalues alone and in the mix
     mix = served[0][0]

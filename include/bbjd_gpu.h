@@ -104,6 +104,23 @@ int bbjd_device_init(int sync_yield);
 void bbjd_ktime_enable(int on);
 int  bbjd_ktime_harvest(double *total_ms, double *total_bytes);
 
+/* Kernel benchmark (tile autotuning and roofline checks): times one engine
+ * kernel in isolation at shape (n, k) on a private stream, reps launches.
+ * which: 0 projection Q^H x, 1 projection of two vectors, 2 update
+ * x -= Q h, 3 in-place basis rotation k -> k/2, 4 full CGS2.  Writes the
+ * average ms per launch and the algorithmic bytes per launch. */
+int bbjd_kbench(int which, int64_t n, int k, int reps, double *avg_ms, double *bytes);
+
+/* Device memory helpers for callers without another GPU runtime (tests,
+ * tools): allocate / free / synchronise the device. */
+void *bbjd_dev_alloc(size_t bytes);
+void  bbjd_dev_free(void *p);
+int   bbjd_dev_sync(void);
+
+/* Average ms of reps applies of a DEVICE-space operator f(n, x, y, data) on
+ * the null stream (HIP events). */
+int bbjd_time_op(bbjd_matvec_fn f, bbjd_int n, const void *x, void *y, void *data, int reps, double *avg_ms);
+
 /* Device pointer -> host vector download helper for tests/tools. */
 int bbjd_memcpy_d2h(void *dst, const void *src, size_t bytes);
 int bbjd_memcpy_h2d(void *dst, const void *src, size_t bytes);
