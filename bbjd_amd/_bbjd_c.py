@@ -77,6 +77,8 @@ def jdqz_c(n, nev, sigma, target, matvec_A, matvec_B, precond_solve, tol=1e-10, 
         p.matvec_A = _lib.fn_addr("bbjd_op_matvec_A")
         p.matvec_B = _lib.fn_addr("bbjd_op_matvec_B")
         p.precond = _lib.fn_addr("bbjd_op_precond") if precond_solve is not None else None
+        if precond_solve is not None:
+            opts.precond2 = _lib.fn_addr("bbjd_op_precond2")
         p.op_data = device_op.handle
         if precision == "mixed" and device_op.c64:
             opts.matvec_A_c64 = _lib.fn_addr("bbjd_op_matvec_A_c64")

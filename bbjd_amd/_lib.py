@@ -41,7 +41,7 @@ class Result(C.Structure):
 class GpuOptions(C.Structure):
     _fields_ = [("callback_space", C.c_int), ("precision", C.c_int), ("stream", C.c_void_p),
                 ("matvec_A_c64", C.c_void_p), ("matvec_B_c64", C.c_void_p),
-                ("precond_c64", C.c_void_p)]
+                ("precond_c64", C.c_void_p), ("precond2", C.c_void_p)]
 
 
 class Stats(C.Structure):
@@ -104,6 +104,7 @@ def load():
     _sig(lib, "bbjd_op_create", V, C.POINTER(PencilDesc), zcomplex, C.c_int, C.c_int, V)
     _sig(lib, "bbjd_op_free", None, V)
     _sig(lib, "bbjd_op_bytes", C.c_size_t, V)
+    _sig(lib, "bbjd_op_precond2", C.c_int, I64, V, V, V, V, V)
     for nm in ("bbjd_op_matvec_A", "bbjd_op_matvec_B", "bbjd_op_precond",
                "bbjd_op_matvec_A_c64", "bbjd_op_matvec_B_c64", "bbjd_op_precond_c64"):
         _sig(lib, nm, C.c_int, I64, V, V, V)

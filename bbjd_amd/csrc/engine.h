@@ -37,6 +37,12 @@ enum {
 /* Int flag slots. */
 enum { FL_PASS2 = 0, FL_ONE = 1, FL_DGKS_COUNT = 2, FL_COUNT = 8 };
 
+/* Counters of the last-block-done reductions: [0, BBJD_CNT_UPD) one per
+ * column tile of a projection, then one for updates, one for norms. */
+#define BBJD_CNT_UPD 1024
+#define BBJD_CNT_VEC 1025
+#define BBJD_CNT_SLOTS 1032
+
 /* Per-workspace scratch for the two-stage reductions. */
 typedef struct {
     void   *part;        /* partial sums: tiles x ldp complex (x2 vectors) */
@@ -48,6 +54,7 @@ typedef struct {
     int    *flag;        /* FL_COUNT ints                                   */
     void   *coef;        /* uploaded small vectors                          */
     void   *coef2;
+    int    *cnt;         /* BBJD_CNT_SLOTS zero-initialised counters        */
     int64_t part_cap;    /* complex entries available in part               */
 } bbjd_red;
 

@@ -80,8 +80,8 @@ typedef struct {
 } layout_t;
 
 enum { O_V, O_W, O_AV, O_BV, O_QR, O_QL, O_GV, O_T1, O_T2, O_T3, O_T4, O_HW, O_PART, O_PNORM, O_H, O_H2,
-       O_HSUM, O_SCAL, O_FLAG, O_COEF, O_COEF2, O_ZR, O_PEN, O_GRAM, O_GVC, O_QRC, O_QLC, O_TC0, O_TC1, O_TC2,
-       O_END };
+       O_HSUM, O_SCAL, O_FLAG, O_CNT, O_COEF, O_COEF2, O_ZR, O_PEN, O_GRAM, O_GVC, O_QRC, O_QLC, O_TC0, O_TC1,
+       O_TC2, O_END };
 
 static bbjd_int auto_maxsub(const bbjd_params *p)
 {
@@ -114,6 +114,7 @@ static void make_layout(layout_t *L, const bbjd_params *p, const bbjd_gpu_option
     sz[O_H] = sz[O_H2] = sz[O_HSUM] = ((size_t)maxk + 8) * Z;
     sz[O_SCAL] = SL_COUNT * 8;
     sz[O_FLAG] = FL_COUNT * 4;
+    sz[O_CNT] = BBJD_CNT_SLOTS * 4;
     sz[O_COEF] = ((size_t)maxk + 8) * Z;
     sz[O_COEF2] = 64 * Z;
     sz[O_ZR] = ms * ms * Z;
@@ -218,7 +219,7 @@ bbjd_workspace *bbjd_workspace_alloc_gpu(const bbjd_params *p, const bbjd_gpu_op
     d->rd.part = AT(O_PART);
     d->rd.pnorm = AT(O_PNORM);
     d->rd.h = AT(O_H); d->rd.h2 = AT(O_H2); d->rd.hsum = AT(O_HSUM);
-    d->rd.scal = AT(O_SCAL); d->rd.flag = AT(O_FLAG);
+    d->rd.scal = AT(O_SCAL); d->rd.flag = AT(O_FLAG); d->rd.cnt = AT(O_CNT);
     d->rd.coef = AT(O_COEF); d->rd.coef2 = AT(O_COEF2);
     {
         int maxk = (int)(L.maxsub > L.ims + 1 ? L.maxsub : L.ims + 1);
@@ -233,6 +234,7 @@ bbjd_workspace *bbjd_workspace_alloc_gpu(const bbjd_params *p, const bbjd_gpu_op
 #undef AT
     if (!HIPOK(hipMemsetAsync(d->rd.scal, 0, SL_COUNT * 8, d->stream))) goto fail;
     if (!HIPOK(hipMemsetAsync(d->rd.flag, 0, FL_COUNT * 4, d->stream))) goto fail;
+    if (!HIPOK(hipMemsetAsync(d->rd.cnt, 0, BBJD_CNT_SLOTS * 4, d->stream))) goto fail;
     if (!HIPOK(hipHostMalloc((void **)&d->hscal, 256, 0))) { d->hscal = NULL; goto fail; }
     const size_t small = (size_t)(6 * L.maxsub + 2 * L.ims + 64);
     if (!HIPOK(hipHostMalloc((void **)&d->hsmall, small * sizeof(zcx), 0))) { d->hsmall = NULL; goto fail; }
@@ -795,7 +797,17 @@ static bbjd_status solve_impl(const bbjd_params *P, bbjd_workspace *ws, bbjd_res
             if (g) { st = BBJD_ERR_ALLOC; goto done; }
         } else {
             void *sv = ws->tmp4;
-            if (P->precond) {
+            if (P->precond && d->opt.precond2 && d->opt.callback_space == BBJD_CALLBACK_DEVICE) {
+                /* z = M^-1 r and s = M^-1 u in one pass; s overwrites r, so
+                 * r goes through tmp3 (Bu is not needed any more) */
+                rc |= !HIPOK(hipMemcpyAsync(ws->tmp3, r, (size_t)n * 16, hipMemcpyDeviceToDevice, c->s));
+                hipStream_t prev = tl_cb_stream;
+                tl_cb_stream = c->s;
+                int e = d->opt.precond2(n, (const bbjd_zomplex *)ws->tmp3, (bbjd_zomplex *)zc,
+                                        (const bbjd_zomplex *)u, (bbjd_zomplex *)sv, P->op_data);
+                tl_cb_stream = prev;
+                if (e) { st = BBJD_ERR_CALLBACK; goto done; }
+            } else if (P->precond) {
                 if (apply_op(c, (bbjd_matvec_fn)P->precond, r, zc)) { st = BBJD_ERR_CALLBACK; goto done; }
                 if (apply_op(c, (bbjd_matvec_fn)P->precond, u, sv)) { st = BBJD_ERR_CALLBACK; goto done; }
             } else {
@@ -927,14 +939,16 @@ static int helper_open(helper_t *h, bbjd_int n, bbjd_int k, size_t extra, void *
     size_t off_h = off_pn + ((size_t)(n + BBJD_PROJ_ROWS - 1) / BBJD_PROJ_ROWS + 64) * 8;
     size_t kk = (size_t)(k + 8) * 16;
     size_t off_h2 = off_h + kk, off_hs = off_h2 + kk, off_sc = off_hs + kk, off_fl = off_sc + 256,
-           off_co = off_fl + 64, off_c2 = off_co + kk, off_ex = (off_c2 + 64 * 16 + 255) / 256 * 256;
+           off_cn = off_fl + 64, off_co = off_cn + BBJD_CNT_SLOTS * 4, off_c2 = off_co + kk,
+           off_ex = (off_c2 + 64 * 16 + 255) / 256 * 256;
     if (!HIPOK(hipMalloc(&h->blk, off_ex + extra))) { h->blk = NULL; return -1; }
     char *b = (char *)h->blk;
     h->rd.part = b + off_part; h->rd.pnorm = (double *)(b + off_pn); h->rd.h = b + off_h; h->rd.h2 = b + off_h2;
     h->rd.hsum = b + off_hs; h->rd.scal = (double *)(b + off_sc); h->rd.flag = (int *)(b + off_fl);
     h->rd.coef = b + off_co; h->rd.coef2 = b + off_c2; h->rd.part_cap = pe;
+    h->rd.cnt = (int *)(b + off_cn);
     *extra_ptr = b + off_ex;
-    hipMemsetAsync(h->rd.flag, 0, 64, h->s);
+    if (!HIPOK(hipMemsetAsync(h->rd.flag, 0, 64 + BBJD_CNT_SLOTS * 4, h->s))) return -1;
     if (!HIPOK(hipHostMalloc((void **)&h->hscal, 256, 0))) { h->hscal = NULL; return -1; }
     return 0;
 }

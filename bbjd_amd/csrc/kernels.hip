@@ -109,14 +109,111 @@ template <typename R> __device__ __forceinline__ R block_sum(R v, R *lds4)
 }
 
 // ---------------------------------------------------------------------------
-// Projection partials: part[(tile*NV + vi)*ldp + col] = sum over the tile's
-// rows of conj(Q[r, col]) x_vi[r]; pnorm[tile] = sum |x_0[r]|^2 when
-// want_norm (column-tile 0 blocks only).  grid = (col tiles, row tiles).
+// Last-block-done finishing.  Every reduction kernel writes per-block
+// partials; the last block of a group to arrive (counter in bbjd_red.cnt)
+// sums them in a fixed order -- tile 0, 1, 2, ... -- so the result does not
+// depend on scheduling, and resets the counter for the next launch on the
+// stream.  Only thread 0 of a block fences: an agent-scope release (L2
+// write-back, after the workgroup barrier has ordered the block's stores)
+// before the counter increment, and, in the last block only, an agent-scope
+// acquire (L2 invalidate) before the partials of blocks on other XCDs are
+// read.  A fence in every thread would write back and invalidate the L2 once
+// per thread group and throttle the streaming kernels.
+__device__ __forceinline__ bool last_block_of(int *cnt, int group_size, int *lds_flag)
+{
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        const int last = atomicAdd(cnt, 1) == group_size - 1;
+        if (last) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        *lds_flag = last;
+    }
+    __syncthreads();
+    return *lds_flag != 0;
+}
+
+// What the last block of a norm-producing kernel does with the norm.
+//  mode 0: scal[slot] = ||x||
+//  mode 1: after CGS pass 1: also scal[SL_NA] = ||x||, flag[FL_PASS2] =
+//          ||x|| < 0.7071 scal[SL_NB]; hsum[:k] = h when hsum
+//  mode 2: after CGS pass 2 (runs only when flagged): scal[slot] = ||x||,
+//          hsum[:k] = h + h2 when hsum
+struct NormFin {
+    double *scal;
+    int *flag;
+    int mode, slot, k;
+    void *hsum;
+    const void *h, *h2;
+};
+
+template <typename R>
+__device__ __forceinline__ void finish_norm(const R *pnorm, int nblk, const NormFin &f, R *lds4)
+{
+    R s = 0;
+    for (int t = threadIdx.x; t < nblk; t += TPB) s += pnorm[t];
+    s = block_sum(s, lds4);
+    if (threadIdx.x == 0) {
+        const double nrm = sqrt((double)s);
+        f.scal[f.slot] = nrm;
+        if (f.mode == 1) {
+            f.scal[SL_NA] = nrm;
+            const int p2 = nrm < 0.7071 * f.scal[SL_NB];
+            f.flag[FL_PASS2] = p2;
+            if (p2) f.flag[FL_DGKS_COUNT] += 1;
+        }
+    }
+    if (f.hsum) {
+        Cx<R> *hs = reinterpret_cast<Cx<R> *>(f.hsum);
+        const Cx<R> *h = reinterpret_cast<const Cx<R> *>(f.h), *h2 = reinterpret_cast<const Cx<R> *>(f.h2);
+        for (int j = threadIdx.x; j < f.k; j += TPB) hs[j] = f.mode == 2 ? h[j] + h2[j] : h[j];
+    }
+}
+
+// Projection Q^H x (and Q^H x1): a block owns 1024 rows x C columns and
+// writes its partial sums part[(tile*NV + vi)*ldp + col]; the last row tile
+// of each column tile to finish reduces that column tile's partials into
+// out0/out1.  With want_norm the column-tile-0 blocks also produce ||x0||^2
+// partials and the last of them writes scal[norm_slot] = ||x0||.
+// grid = (column tiles, row tiles).
+// Finish of a projection for column tile c0: the partials of ntiles row
+// tiles in a fixed order (16 lanes per (vector, column) pair), plus the norm
+// when norm_here.  Run by the last block of the column tile (fused) or by
+// k_proj_finish.
+template <typename R, int NV, int C>
+__device__ __forceinline__ void proj_finish(int c0, int ntiles, int k, const Cx<R> *part, int ldp, const R *pnorm,
+                                            bool norm_here, Cx<R> *out0, Cx<R> *out1, double *scal, int norm_slot,
+                                            R *lds4)
+{
+    const int t = threadIdx.x;
+    const int g = t >> 4, l = t & 15;
+    const int vi = g / C, c = g % C, col = c0 + c;
+    R sr = 0, si = 0;
+    if (col < k)
+        for (int tt = l; tt < ntiles; tt += 16) {
+            const Cx<R> p = part[((i64)tt * NV + vi) * ldp + col];
+            sr += p.re;
+            si += p.im;
+        }
+#pragma unroll
+    for (int m = 8; m >= 1; m >>= 1) {
+        sr += __shfl_xor(sr, m, 64);
+        si += __shfl_xor(si, m, 64);
+    }
+    if (l == 0 && col < k) (vi == 0 ? out0 : out1)[col] = cx(sr, si);
+    if (norm_here) {
+        R s = 0;
+        for (int tt = t; tt < ntiles; tt += TPB) s += pnorm[tt];
+        s = block_sum(s, lds4);
+        if (t == 0) scal[norm_slot] = sqrt((double)s);
+    }
+}
+
 template <typename R, int NV, int C>
 __global__ __launch_bounds__(TPB) void k_proj(i64 n, int k, const Cx<R> *__restrict__ Q, i64 ldq,
                                               const Cx<R> *__restrict__ x0, const Cx<R> *__restrict__ x1,
-                                              Cx<R> *__restrict__ part, int ldp, R *__restrict__ pnorm,
-                                              int want_norm, const int *__restrict__ cond)
+                                              Cx<R> *part, int ldp, R *pnorm, int want_norm,
+                                              const int *__restrict__ cond, int *cnt, Cx<R> *out0, Cx<R> *out1,
+                                              double *scal, int norm_slot, int fused)
 {
     if (cond && cond[0] == 0) return;
     constexpr int NVAL = 2 * NV * C;
@@ -124,6 +221,7 @@ __global__ __launch_bounds__(TPB) void k_proj(i64 n, int k, const Cx<R> *__restr
     constexpr int LS = 5;
     __shared__ R lds[4][NVAL];
     __shared__ R lds4[4];
+    __shared__ int lflag;
     const int t = threadIdx.x;
     const int tile = blockIdx.y;
     const i64 rb = (i64)tile * (TPB * PROJ_E);
@@ -178,50 +276,38 @@ __global__ __launch_bounds__(TPB) void k_proj(i64 n, int k, const Cx<R> *__restr
         const R s = block_sum(nv, lds4);
         if (t == 0) pnorm[tile] = s;
     }
+    if (!fused) return;
+    if (!last_block_of(cnt + blockIdx.x, gridDim.y, &lflag)) return;
+    proj_finish<R, NV, C>(c0, gridDim.y, k, part, ldp, pnorm, want_norm && blockIdx.x == 0, out0, out1, scal,
+                          norm_slot, lds4);
+    if (t == 0) cnt[blockIdx.x] = 0;
 }
 
-// out_v[c] = sum_t part[(t*nv + v)*ldp + c] for c < ncols (grid.x = ncols,
-// grid.y = nv); an extra block (grid.x == ncols + 1 with norm_slot >= 0)
-// turns the norm partials into scal[norm_slot] = sqrt(sum).  64 threads.
-template <typename R>
-__global__ __launch_bounds__(64) void k_reduce_part(const Cx<R> *__restrict__ part, int ntiles, int nv, int ldp,
-                                                    int ncols, Cx<R> *out0, Cx<R> *out1,
-                                                    const R *__restrict__ pnorm, double *scal, int norm_slot,
-                                                    const int *__restrict__ cond)
+template <typename R, int NV, int C>
+__global__ __launch_bounds__(TPB) void k_proj_finish(int ntiles, int k, const Cx<R> *part, int ldp, const R *pnorm,
+                                                     int want_norm, const int *__restrict__ cond, Cx<R> *out0,
+                                                     Cx<R> *out1, double *scal, int norm_slot)
 {
     if (cond && cond[0] == 0) return;
-    const int lane = threadIdx.x;
-    const int c = blockIdx.x, v = blockIdx.y;
-    if (c == ncols) {
-        if (v != 0 || norm_slot < 0) return;
-        R s = 0;
-        for (int t = lane; t < ntiles; t += 64) s += pnorm[t];
-        s = wave_sum(s);
-        if (lane == 0) scal[norm_slot] = sqrt((double)s);
-        return;
-    }
-    R sr = 0, si = 0;
-    for (int t = lane; t < ntiles; t += 64) {
-        const Cx<R> p = part[((i64)t * nv + v) * ldp + c];
-        sr += p.re; si += p.im;
-    }
-    sr = wave_sum(sr);
-    si = wave_sum(si);
-    if (lane == 0) (v == 0 ? out0 : out1)[c] = cx(sr, si);
+    __shared__ R lds4[4];
+    proj_finish<R, NV, C>(blockIdx.x * C, ntiles, k, part, ldp, pnorm, want_norm && blockIdx.x == 0, out0, out1,
+                          scal, norm_slot, lds4);
 }
 
-// x = beta*x + alpha * Q[:, :k] h, squared norm partial per block.
+// x = beta*x + alpha * Q[:, :k] h; with fin.mode >= 0 the squared norm of
+// the result is reduced (last block) and finished per NormFin.
 // Dynamic LDS: k complex.
 template <typename R>
 __global__ __launch_bounds__(TPB) void k_update(i64 n, int k, const Cx<R> *__restrict__ Q, i64 ldq,
                                                 const Cx<R> *__restrict__ h, R alpha, int beta_one,
-                                                Cx<R> *__restrict__ x, R *__restrict__ pnorm,
-                                                const int *__restrict__ cond)
+                                                Cx<R> *__restrict__ x, R *pnorm, const int *__restrict__ cond,
+                                                int *cnt, NormFin fin, int fused)
 {
     if (cond && cond[0] == 0) return;
     extern __shared__ char smem_raw[];
     Cx<R> *sh = reinterpret_cast<Cx<R> *>(smem_raw);
     __shared__ R lds4[4];
+    __shared__ int lflag;
     const int t = threadIdx.x;
     for (int j = t; j < k; j += TPB) { const Cx<R> hj = h[j]; sh[j] = cx(alpha * hj.re, alpha * hj.im); }
     __syncthreads();
@@ -257,17 +343,30 @@ __global__ __launch_bounds__(TPB) void k_update(i64 n, int k, const Cx<R> *__res
         const i64 r = rb + t + (i64)e * TPB;
         if (r < n) { x[r] = a[e]; s += a[e].re * a[e].re + a[e].im * a[e].im; }
     }
-    if (pnorm) {
-        s = block_sum(s, lds4);
-        if (t == 0) pnorm[blockIdx.x] = s;
-    }
+    if (fin.mode < 0) return;
+    s = block_sum(s, lds4);
+    if (t == 0) pnorm[blockIdx.x] = s;
+    if (!fused || !last_block_of(cnt, gridDim.x, &lflag)) return;
+    finish_norm<R>(pnorm, gridDim.x, fin, lds4);
+    if (t == 0) *cnt = 0;
 }
 
-// Squared-norm partials of x (grid-stride over UPD tiles).
+// separate finish of a norm (one workgroup)
 template <typename R>
-__global__ __launch_bounds__(TPB) void k_norm_part(i64 n, const Cx<R> *__restrict__ x, R *__restrict__ pnorm)
+__global__ __launch_bounds__(TPB) void k_norm_fin(const R *pnorm, int nblk, NormFin fin, const int *__restrict__ cond)
+{
+    if (cond && cond[0] == 0) return;
+    __shared__ R lds4[4];
+    finish_norm<R>(pnorm, nblk, fin, lds4);
+}
+
+// ||x|| -> scal[slot] in one launch.
+template <typename R>
+__global__ __launch_bounds__(TPB) void k_norm(i64 n, const Cx<R> *__restrict__ x, R *pnorm, int *cnt, NormFin fin,
+                                              int fused)
 {
     __shared__ R lds4[4];
+    __shared__ int lflag;
     const i64 rb = (i64)blockIdx.x * (TPB * UPD_E);
     R s = 0;
 #pragma unroll
@@ -277,38 +376,9 @@ __global__ __launch_bounds__(TPB) void k_norm_part(i64 n, const Cx<R> *__restric
     }
     s = block_sum(s, lds4);
     if (threadIdx.x == 0) pnorm[blockIdx.x] = s;
-}
-
-// Final norm of an orthogonalisation pass / a plain norm.
-//  mode 0: scal[slot] = nrm
-//  mode 1: after CGS pass 1: scal[SL_NA] = scal[slot] = nrm;
-//          flag[FL_PASS2] = nrm < 0.7071 scal[SL_NB]
-//  mode 2: after CGS pass 2 (runs only when flagged): scal[slot] = nrm
-template <typename R>
-__global__ __launch_bounds__(64) void k_norm_finish(const R *__restrict__ pnorm, int ntiles, double *scal,
-                                                    int *flag, int mode, int slot, const int *__restrict__ cond)
-{
-    if (cond && cond[0] == 0) return;
-    R s = 0;
-    for (int t = threadIdx.x; t < ntiles; t += 64) s += pnorm[t];
-    s = wave_sum(s);
-    if (threadIdx.x == 0) {
-        const double nrm = sqrt((double)s);
-        scal[slot] = nrm;
-        if (mode == 1) {
-            scal[SL_NA] = nrm;
-            const int p2 = nrm < 0.7071 * scal[SL_NB];
-            flag[FL_PASS2] = p2;
-            if (p2) flag[FL_DGKS_COUNT] += 1;
-        }
-    }
-}
-
-template <typename R>
-__global__ void k_hsum(int k, const Cx<R> *h, const Cx<R> *h2, Cx<R> *hs, const int *flag)
-{
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j < k) hs[j] = flag[FL_PASS2] ? h[j] + h2[j] : h[j];
+    if (!fused || !last_block_of(cnt, gridDim.x, &lflag)) return;
+    finish_norm<R>(pnorm, gridDim.x, fin, lds4);
+    if (threadIdx.x == 0) *cnt = 0;
 }
 
 template <typename R>
@@ -333,9 +403,11 @@ __global__ __launch_bounds__(TPB) void k_scale(i64 n, Cx<R> *x, R are, R aim)
 
 template <typename R>
 __global__ __launch_bounds__(TPB) void k_lincomb(i64 n, R are, R aim, const Cx<R> *x, R bre, R bim,
-                                                 const Cx<R> *y, Cx<R> *out, R *pnorm)
+                                                 const Cx<R> *y, Cx<R> *out, R *pnorm, int *cnt, NormFin fin,
+                                                 int fused)
 {
     __shared__ R lds4[4];
+    __shared__ int lflag;
     const Cx<R> a = cx(are, aim), b = cx(bre, bim);
     const i64 rb = (i64)blockIdx.x * (TPB * UPD_E);
     R s = 0;
@@ -348,10 +420,12 @@ __global__ __launch_bounds__(TPB) void k_lincomb(i64 n, R are, R aim, const Cx<R
             s += v.re * v.re + v.im * v.im;
         }
     }
-    if (pnorm) {
-        s = block_sum(s, lds4);
-        if (threadIdx.x == 0) pnorm[blockIdx.x] = s;
-    }
+    if (fin.mode < 0) return;
+    s = block_sum(s, lds4);
+    if (threadIdx.x == 0) pnorm[blockIdx.x] = s;
+    if (!fused || !last_block_of(cnt, gridDim.x, &lflag)) return;
+    finish_norm<R>(pnorm, gridDim.x, fin, lds4);
+    if (threadIdx.x == 0) *cnt = 0;
 }
 
 // SvdV coefficient from the two dots d = (u^H z, u^H s): coef = -d0/d1 when
@@ -452,8 +526,27 @@ extern "C" int64_t bbjd_red_part_entries(int64_t n, int maxk)
     return (int64_t)ntiles_proj(n) * 2 * (maxk + 1) + 64;
 }
 
-// one projection pass: part <- tiles, then h (and h2) <- reduced; optional
-// norm of x0 into scal[norm_slot].
+// Blocks up to which the finishing step runs in the producing kernel's last
+// block; larger grids finish in a second small kernel instead (one L2
+// write-back per block costs more than a launch there).  Tunable.
+static int g_fuse_max_blocks = 48;
+
+static inline NormFin fin_none()
+{
+    NormFin f;
+    f.scal = nullptr; f.flag = nullptr; f.mode = -1; f.slot = 0; f.k = 0;
+    f.hsum = nullptr; f.h = nullptr; f.h2 = nullptr;
+    return f;
+}
+static inline NormFin fin_norm(bbjd_red *rd, int mode, int slot)
+{
+    NormFin f = fin_none();
+    f.scal = rd->scal; f.flag = rd->flag; f.mode = mode; f.slot = slot;
+    return f;
+}
+
+// One projection pass in one launch: out0 (and out1) <- Q^H x0 (x1); with
+// norm_slot >= 0 also scal[norm_slot] <- ||x0||.
 template <typename R>
 static int proj_pass(hipStream_t s, i64 n, int k, const void *Q, i64 ldq, const void *x0, const void *x1,
                      void *out0, void *out1, bbjd_red *rd, int norm_slot, const int *cond)
@@ -461,52 +554,60 @@ static int proj_pass(hipStream_t s, i64 n, int k, const void *Q, i64 ldq, const 
     const int nt = ntiles_proj(n);
     const int nv = x1 ? 2 : 1;
     const int ldp = k > 0 ? k : 1;
-    if ((i64)nt * nv * ldp > rd->part_cap || k > BBJD_MAXK) return -2;
     const int want_norm = norm_slot >= 0;
-    if (k > 0 || want_norm) {
-        if (nv == 1) {
-            constexpr int C = 16;
-            dim3 g(k > 0 ? (k + C - 1) / C : 1, nt);
-            const bool timed = g_kt_on && sizeof(R) == 8 && cond == nullptr;
-            hipEvent_t ea = nullptr, eb = nullptr;
-            if (timed) {
-                std::lock_guard<std::mutex> lk(g_kt_mu);
-                ea = kt_event(); eb = kt_event();
-            }
-            if (timed) (void)hipEventRecord(ea, s);
-            hipLaunchKernelGGL((k_proj<R, 1, C>), g, dim3(TPB), 0, s, (i64)n, k, (const Cx<R> *)Q, (i64)ldq,
-                               (const Cx<R> *)x0, (const Cx<R> *)nullptr, (Cx<R> *)rd->part, ldp, (R *)rd->pnorm,
-                               want_norm, cond);
-            if (timed) {
-                (void)hipEventRecord(eb, s);
-                const double bytes = (double)sizeof(Cx<R>) * (double)n * ((double)k + (double)g.x);
-                std::lock_guard<std::mutex> lk(g_kt_mu);
-                g_kt.push_back({ea, eb, bytes});
-            }
-        } else {
-            constexpr int C = 8;
-            dim3 g(k > 0 ? (k + C - 1) / C : 1, nt);
-            hipLaunchKernelGGL((k_proj<R, 2, C>), g, dim3(TPB), 0, s, (i64)n, k, (const Cx<R> *)Q, (i64)ldq,
-                               (const Cx<R> *)x0, (const Cx<R> *)x1, (Cx<R> *)rd->part, ldp, (R *)rd->pnorm,
-                               want_norm, cond);
+    if ((i64)nt * nv * ldp > rd->part_cap || k > BBJD_MAXK) return -2;
+    if (k == 0 && !want_norm) return 0;
+    const int fused = nt <= g_fuse_max_blocks;
+    if (nv == 1) {
+        constexpr int C = 16;
+        dim3 g(k > 0 ? (k + C - 1) / C : 1, nt);
+        const bool timed = g_kt_on && sizeof(R) == 8 && cond == nullptr;
+        hipEvent_t ea = nullptr, eb = nullptr;
+        if (timed) {
+            std::lock_guard<std::mutex> lk(g_kt_mu);
+            ea = kt_event(); eb = kt_event();
         }
+        if (timed) (void)hipEventRecord(ea, s);
+        hipLaunchKernelGGL((k_proj<R, 1, C>), g, dim3(TPB), 0, s, (i64)n, k, (const Cx<R> *)Q, (i64)ldq,
+                           (const Cx<R> *)x0, (const Cx<R> *)nullptr, (Cx<R> *)rd->part, ldp, (R *)rd->pnorm,
+                           want_norm, cond, rd->cnt, (Cx<R> *)out0, (Cx<R> *)out1, rd->scal, norm_slot, fused);
+        if (timed) {
+            (void)hipEventRecord(eb, s);
+            const double bytes = (double)sizeof(Cx<R>) * (double)n * ((double)k + (double)g.x);
+            std::lock_guard<std::mutex> lk(g_kt_mu);
+            g_kt.push_back({ea, eb, bytes});
+        }
+        if (!fused)
+            hipLaunchKernelGGL((k_proj_finish<R, 1, C>), dim3(g.x), dim3(TPB), 0, s, nt, k,
+                               (const Cx<R> *)rd->part, ldp, (const R *)rd->pnorm, want_norm, cond, (Cx<R> *)out0,
+                               (Cx<R> *)out1, rd->scal, norm_slot);
+    } else {
+        constexpr int C = 8;
+        dim3 g(k > 0 ? (k + C - 1) / C : 1, nt);
+        hipLaunchKernelGGL((k_proj<R, 2, C>), g, dim3(TPB), 0, s, (i64)n, k, (const Cx<R> *)Q, (i64)ldq,
+                           (const Cx<R> *)x0, (const Cx<R> *)x1, (Cx<R> *)rd->part, ldp, (R *)rd->pnorm,
+                           want_norm, cond, rd->cnt, (Cx<R> *)out0, (Cx<R> *)out1, rd->scal, norm_slot, fused);
+        if (!fused)
+            hipLaunchKernelGGL((k_proj_finish<R, 2, C>), dim3(g.x), dim3(TPB), 0, s, nt, k,
+                               (const Cx<R> *)rd->part, ldp, (const R *)rd->pnorm, want_norm, cond, (Cx<R> *)out0,
+                               (Cx<R> *)out1, rd->scal, norm_slot);
     }
-    dim3 gr(k + (want_norm ? 1 : 0), nv);
-    if (gr.x > 0)
-        hipLaunchKernelGGL((k_reduce_part<R>), gr, dim3(64), 0, s, (const Cx<R> *)rd->part, nt, nv, ldp, k,
-                           (Cx<R> *)out0, (Cx<R> *)out1, (const R *)rd->pnorm, rd->scal, norm_slot, cond);
     return chk();
 }
 
 template <typename R>
 static int update_pass(hipStream_t s, i64 n, int k, const void *Q, i64 ldq, const void *h, R alpha, int beta_one,
-                       void *x, R *pnorm, const int *cond)
+                       void *x, bbjd_red *rd, NormFin fin, const int *cond)
 {
     if (k > BBJD_MAXK) return -2;
     const int nt = ntiles_upd(n);
     const size_t lds = (size_t)(k > 0 ? k : 1) * sizeof(Cx<R>);
+    const int fused = nt <= g_fuse_max_blocks;
     hipLaunchKernelGGL((k_update<R>), dim3(nt), dim3(TPB), lds, s, (i64)n, k, (const Cx<R> *)Q, (i64)ldq,
-                       (const Cx<R> *)h, alpha, beta_one, (Cx<R> *)x, pnorm, cond);
+                       (const Cx<R> *)h, alpha, beta_one, (Cx<R> *)x, (R *)rd->pnorm, cond,
+                       rd->cnt + BBJD_CNT_UPD, fin, fused);
+    if (!fused && fin.mode >= 0)
+        hipLaunchKernelGGL((k_norm_fin<R>), dim3(1), dim3(TPB), 0, s, (const R *)rd->pnorm, nt, fin, cond);
     return chk();
 }
 
@@ -514,34 +615,34 @@ template <typename R>
 static int norm_into(hipStream_t s, i64 n, const void *x, bbjd_red *rd, int slot)
 {
     const int nt = ntiles_upd(n);
-    hipLaunchKernelGGL((k_norm_part<R>), dim3(nt), dim3(TPB), 0, s, (i64)n, (const Cx<R> *)x, (R *)rd->pnorm);
-    hipLaunchKernelGGL((k_norm_finish<R>), dim3(1), dim3(64), 0, s, (const R *)rd->pnorm, nt, rd->scal, rd->flag,
-                       0, slot, (const int *)nullptr);
+    const int fused = nt <= g_fuse_max_blocks;
+    hipLaunchKernelGGL((k_norm<R>), dim3(nt), dim3(TPB), 0, s, (i64)n, (const Cx<R> *)x, (R *)rd->pnorm,
+                       rd->cnt + BBJD_CNT_VEC, fin_norm(rd, 0, slot), fused);
+    if (!fused)
+        hipLaunchKernelGGL((k_norm_fin<R>), dim3(1), dim3(TPB), 0, s, (const R *)rd->pnorm, nt,
+                           fin_norm(rd, 0, slot), (const int *)nullptr);
     return chk();
 }
 
+// CGS2 with the DGKS test in four launches: projection (+ ||v|| before),
+// update (+ ||v|| after, decision), and the second pass, which reads the
+// decision flag on the device and returns at once when it is not needed.
 template <typename R>
 static int cgs2(hipStream_t s, i64 n, int k, const void *Q, i64 ldq, void *v, bbjd_red *rd, int out_slot,
                 void *hsum)
 {
     if (k == 0) return norm_into<R>(s, n, v, rd, out_slot);
-    const int ntu = ntiles_upd(n);
     int rc = 0;
-    // pass 1: h = Q^H v with ||v|| -> SL_NB; v -= Q h; ||v|| -> SL_NA, out_slot, flag
     rc |= proj_pass<R>(s, n, k, Q, ldq, v, nullptr, rd->h, nullptr, rd, SL_NB, nullptr);
-    rc |= update_pass<R>(s, n, k, Q, ldq, rd->h, (R)-1, 1, v, (R *)rd->pnorm, nullptr);
-    hipLaunchKernelGGL((k_norm_finish<R>), dim3(1), dim3(64), 0, s, (const R *)rd->pnorm, ntu, rd->scal, rd->flag,
-                       1, out_slot, (const int *)nullptr);
-    // pass 2, only when flagged on the device
+    NormFin f1 = fin_norm(rd, 1, out_slot);
+    if (hsum) { f1.hsum = hsum; f1.h = rd->h; f1.k = k; }
+    rc |= update_pass<R>(s, n, k, Q, ldq, rd->h, (R)-1, 1, v, rd, f1, nullptr);
     const int *cond = rd->flag + FL_PASS2;
     rc |= proj_pass<R>(s, n, k, Q, ldq, v, nullptr, rd->h2, nullptr, rd, -1, cond);
-    rc |= update_pass<R>(s, n, k, Q, ldq, rd->h2, (R)-1, 1, v, (R *)rd->pnorm, cond);
-    hipLaunchKernelGGL((k_norm_finish<R>), dim3(1), dim3(64), 0, s, (const R *)rd->pnorm, ntu, rd->scal, rd->flag,
-                       2, out_slot, cond);
-    if (hsum)
-        hipLaunchKernelGGL((k_hsum<R>), dim3((k + 63) / 64), dim3(64), 0, s, k, (const Cx<R> *)rd->h,
-                           (const Cx<R> *)rd->h2, (Cx<R> *)hsum, (const int *)rd->flag);
-    return rc | chk();
+    NormFin f2 = fin_norm(rd, 2, out_slot);
+    if (hsum) { f2.hsum = hsum; f2.h = rd->h; f2.h2 = rd->h2; f2.k = k; }
+    rc |= update_pass<R>(s, n, k, Q, ldq, rd->h2, (R)-1, 1, v, rd, f2, cond);
+    return rc;
 }
 
 template <typename R>
@@ -551,7 +652,7 @@ static int project_out(hipStream_t s, i64 n, int k, const void *Q, i64 ldq, void
     int rc = 0;
     for (int pass = 0; pass < 2; pass++) {
         rc |= proj_pass<R>(s, n, k, Q, ldq, v, nullptr, rd->h, nullptr, rd, -1, nullptr);
-        rc |= update_pass<R>(s, n, k, Q, ldq, rd->h, (R)-1, 1, v, (R *)nullptr, nullptr);
+        rc |= update_pass<R>(s, n, k, Q, ldq, rd->h, (R)-1, 1, v, rd, fin_none(), nullptr);
     }
     return rc;
 }
@@ -560,10 +661,7 @@ template <typename R>
 static int gemv_n(hipStream_t s, i64 n, int k, const void *Q, i64 ldq, const void *c, void *y, bbjd_red *rd,
                   int slot)
 {
-    int rc = update_pass<R>(s, n, k, Q, ldq, c, (R)1, 0, y, (R *)rd->pnorm, nullptr);
-    hipLaunchKernelGGL((k_norm_finish<R>), dim3(1), dim3(64), 0, s, (const R *)rd->pnorm, ntiles_upd(n), rd->scal,
-                       rd->flag, 0, slot, (const int *)nullptr);
-    return rc | chk();
+    return update_pass<R>(s, n, k, Q, ldq, c, (R)1, 0, y, rd, fin_norm(rd, 0, slot), nullptr);
 }
 
 template <typename R>
@@ -571,15 +669,19 @@ static int lincomb(hipStream_t s, i64 n, double are, double aim, const void *x, 
                    const void *y, void *out, bbjd_red *rd, int slot)
 {
     const int nt = ntiles_upd(n);
+    const int fused = nt <= g_fuse_max_blocks;
     hipLaunchKernelGGL((k_lincomb<R>), dim3(nt), dim3(TPB), 0, s, (i64)n, (R)are, (R)aim, (const Cx<R> *)x,
-                       (R)bre, (R)bim, (const Cx<R> *)y, (Cx<R> *)out, rd ? (R *)rd->pnorm : (R *)nullptr);
-    if (rd)
-        hipLaunchKernelGGL((k_norm_finish<R>), dim3(1), dim3(64), 0, s, (const R *)rd->pnorm, nt, rd->scal,
-                           rd->flag, 0, slot, (const int *)nullptr);
+                       (R)bre, (R)bim, (const Cx<R> *)y, (Cx<R> *)out, rd ? (R *)rd->pnorm : (R *)nullptr,
+                       rd ? rd->cnt + BBJD_CNT_VEC : (int *)nullptr, rd ? fin_norm(rd, 0, slot) : fin_none(), fused);
+    if (rd && !fused)
+        hipLaunchKernelGGL((k_norm_fin<R>), dim3(1), dim3(TPB), 0, s, (const R *)rd->pnorm, nt,
+                           fin_norm(rd, 0, slot), (const int *)nullptr);
     return chk();
 }
 
 extern "C" {
+
+void bbjd_set_fuse_blocks(int max_blocks) { g_fuse_max_blocks = max_blocks; }
 
 void bbjd_ktime_enable(int on)
 {
@@ -623,6 +725,7 @@ int bbjd_k_cgs2_c(hipStream_t s, int64_t n, int k, const void *Q, int64_t ldq, v
 
 int bbjd_k_project_out(hipStream_t s, int64_t n, int k, const void *Q, int64_t ldq, void *v, bbjd_red *rd)
 { return project_out<double>(s, n, k, Q, ldq, v, rd); }
+
 
 int bbjd_k_project_out_c(hipStream_t s, int64_t n, int k, const void *Q, int64_t ldq, void *v, bbjd_red *rd)
 { return project_out<float>(s, n, k, Q, ldq, v, rd); }
@@ -759,7 +862,7 @@ int bbjd_kbench(int which, int64_t n, int k, int reps, double *avg_ms, double *b
     rc |= hipMalloc(&Q, (size_t)ld * k * 16) != hipSuccess;
     rc |= hipMalloc(&x, (size_t)ld * 16) != hipSuccess;
     rc |= hipMalloc(&x2, (size_t)ld * 16) != hipSuccess;
-    rc |= hipMalloc(&blk, (size_t)pe * 16 + (size_t)(ntiles_proj(n) + 64) * 8 + (size_t)(k + 8) * 16 * 5 + 4096) != hipSuccess;
+    rc |= hipMalloc(&blk, (size_t)pe * 16 + (size_t)(ntiles_proj(n) + 64) * 8 + (size_t)(k + 8) * 16 * 5 + 8192) != hipSuccess;
     rc |= hipMalloc(&Zr, (size_t)k * k * 16) != hipSuccess;
     if (rc) { hipFree(Q); hipFree(x); hipFree(x2); hipFree(blk); hipFree(Zr); hipStreamDestroy(s); return -1; }
     bbjd_red rd;
@@ -771,14 +874,14 @@ int bbjd_kbench(int which, int64_t n, int k, int reps, double *avg_ms, double *b
     rd.hsum = b; b += (size_t)(k + 8) * 16;
     rd.coef = b; b += (size_t)(k + 8) * 16;
     rd.coef2 = b; b += (size_t)(k + 8) * 16;
-    rd.scal = (double *)b; rd.flag = (int *)(b + 256);
+    rd.scal = (double *)b; rd.flag = (int *)(b + 256); rd.cnt = (int *)(b + 512);
     rd.part_cap = pe;
     hipLaunchKernelGGL(k_fill, dim3(4096), dim3(TPB), 0, s, (i64)ld * k, (Cx<double> *)Q, 1.0);
     hipLaunchKernelGGL(k_fill, dim3(1024), dim3(TPB), 0, s, (i64)ld, (Cx<double> *)x, 0.7);
     hipLaunchKernelGGL(k_fill, dim3(1024), dim3(TPB), 0, s, (i64)ld, (Cx<double> *)x2, 0.3);
     hipLaunchKernelGGL(k_fill, dim3(64), dim3(TPB), 0, s, (i64)k * k, (Cx<double> *)Zr, 0.1);
     hipLaunchKernelGGL(k_fill, dim3(1), dim3(TPB), 0, s, (i64)(k + 8), (Cx<double> *)rd.h, 0.01);
-    (void)hipMemsetAsync(rd.flag, 0, 64, s);
+    (void)hipMemsetAsync(rd.flag, 0, 256 + BBJD_CNT_SLOTS * 4, s);
     hipEvent_t e0, e1;
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
@@ -790,7 +893,8 @@ int bbjd_kbench(int which, int64_t n, int k, int reps, double *avg_ms, double *b
             dim3 g((k + 15) / 16, nt);
             hipLaunchKernelGGL((k_proj<double, 1, 16>), g, dim3(TPB), 0, s, (i64)n, k, (const Cx<double> *)Q, ld,
                                (const Cx<double> *)x, (const Cx<double> *)nullptr, (Cx<double> *)rd.part, k,
-                               rd.pnorm, 0, (const int *)nullptr);
+                               rd.pnorm, 0, (const int *)nullptr, rd.cnt, (Cx<double> *)rd.h2,
+                               (Cx<double> *)nullptr, rd.scal, -1, 0);
             by = 16.0 * n * (k + g.x);
             break;
         }
@@ -798,14 +902,15 @@ int bbjd_kbench(int which, int64_t n, int k, int reps, double *avg_ms, double *b
             dim3 g((k + 7) / 8, nt);
             hipLaunchKernelGGL((k_proj<double, 2, 8>), g, dim3(TPB), 0, s, (i64)n, k, (const Cx<double> *)Q, ld,
                                (const Cx<double> *)x, (const Cx<double> *)x2, (Cx<double> *)rd.part, k, rd.pnorm,
-                               0, (const int *)nullptr);
+                               0, (const int *)nullptr, rd.cnt, (Cx<double> *)rd.h2, (Cx<double> *)rd.hsum,
+                               rd.scal, -1, 0);
             by = 16.0 * n * (k + 2.0 * g.x);
             break;
         }
         case 2:
             hipLaunchKernelGGL((k_update<double>), dim3(ntiles_upd(n)), dim3(TPB), (size_t)k * 16, s, (i64)n, k,
                                (const Cx<double> *)Q, ld, (const Cx<double> *)rd.h, -1.0, 1, (Cx<double> *)x,
-                               rd.pnorm, (const int *)nullptr);
+                               rd.pnorm, (const int *)nullptr, rd.cnt + BBJD_CNT_UPD, fin_norm(&rd, 0, SL_T1), 0);
             by = 16.0 * n * (k + 2);
             break;
         case 3:

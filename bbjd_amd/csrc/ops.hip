@@ -19,6 +19,8 @@ typedef long long i64;
 template <typename R> struct alignas(2 * sizeof(R)) Cx { R re, im; };
 template <typename R> __device__ __forceinline__ void cfma(Cx<R> &a, Cx<R> q, Cx<R> h)
 { a.re += q.re * h.re - q.im * h.im; a.im += q.re * h.im + q.im * h.re; }
+__device__ __forceinline__ Cx<double> cx(double a, double b) { Cx<double> c; c.re = a; c.im = b; return c; }
+__device__ __forceinline__ Cx<double> cx0() { return cx(0.0, 0.0); }
 template <typename R> __device__ __forceinline__ void cfms(Cx<R> &a, Cx<R> q, Cx<R> h)
 { a.re -= q.re * h.re - q.im * h.im; a.im -= q.re * h.im + q.im * h.re; }
 
@@ -69,87 +71,169 @@ __global__ __launch_bounds__(TPB) void k_bj_apply(i64 n, int bs, const Cx<R> *__
     y[i] = a;
 }
 
-// forward level: evens m = 2j at stride h, j < ne.
-__global__ __launch_bounds__(TPB) void k_bcr_fwd(int s, i64 M, i64 h, i64 ne, const Cx<double> *__restrict__ AG,
-                                                 Cx<double> *b)
+// BCR on NR right-hand sides at once (vector q at b + q vs): the factor
+// blocks are read once for all of them.
+//
+// forward level: evens m = 2j at stride h, j < ne:
+//   b_i -= alpha_j b_{i-h} + gamma_j b_{i+h}
+template <int NR>
+__device__ __forceinline__ void bcr_fwd_row(int s, i64 M, i64 h, i64 j, int r, const Cx<double> *__restrict__ AG,
+                                            Cx<double> *b, i64 vs)
 {
-    const i64 gid = (i64)blockIdx.x * TPB + threadIdx.x;
-    const i64 j = gid / s;
-    const int r = (int)(gid % s);
-    if (j >= ne) return;
     const i64 m = 2 * j, i = m * h, ss = (i64)s * s;
-    Cx<double> acc = b[i * s + r];
+    Cx<double> acc[NR];
+#pragma unroll
+    for (int q = 0; q < NR; q++) acc[q] = b[q * vs + i * s + r];
     if (m >= 1) {
         const Cx<double> *al = AG + j * 2 * ss + r;
-        const Cx<double> *bl = b + (i - h) * s;
-        for (int c = 0; c < s; c++) cfms(acc, al[(i64)c * s], bl[c]);
+        const i64 o = (i - h) * s;
+        for (int c = 0; c < s; c++) {
+            const Cx<double> a = al[(i64)c * s];
+#pragma unroll
+            for (int q = 0; q < NR; q++) cfms(acc[q], a, b[q * vs + o + c]);
+        }
     }
     if (m + 1 < M) {
         const Cx<double> *ga = AG + j * 2 * ss + ss + r;
-        const Cx<double> *br = b + (i + h) * s;
-        for (int c = 0; c < s; c++) cfms(acc, ga[(i64)c * s], br[c]);
+        const i64 o = (i + h) * s;
+        for (int c = 0; c < s; c++) {
+            const Cx<double> g = ga[(i64)c * s];
+#pragma unroll
+            for (int q = 0; q < NR; q++) cfms(acc[q], g, b[q * vs + o + c]);
+        }
     }
-    b[i * s + r] = acc;
+#pragma unroll
+    for (int q = 0; q < NR; q++) b[q * vs + i * s + r] = acc[q];
 }
 
-// backward level: odds m = 2j + 1 at stride h, j < no.  DPQ = (Dinv, Dinv L,
-// Dinv U) per odd.  Reads b and the already-solved even x, writes x.
-__global__ __launch_bounds__(TPB) void k_bcr_bwd(int s, i64 M, i64 h, i64 no, const Cx<double> *__restrict__ DPQ,
-                                                 const Cx<double> *__restrict__ b, Cx<double> *x)
+// backward level: odds m = 2j + 1, DPQ = (Dinv, Dinv L, Dinv U):
+//   x_i = Dinv b_i - P x_{i-h} - Q x_{i+h}
+template <int NR>
+__device__ __forceinline__ void bcr_bwd_row(int s, i64 M, i64 h, i64 j, int r, const Cx<double> *__restrict__ DPQ,
+                                            const Cx<double> *b, Cx<double> *x, i64 vs)
+{
+    const i64 m = 2 * j + 1, i = m * h, ss = (i64)s * s;
+    const Cx<double> *D = DPQ + j * 3 * ss + r, *P = D + ss, *Qm = P + ss;
+    Cx<double> acc[NR];
+#pragma unroll
+    for (int q = 0; q < NR; q++) acc[q] = cx0();
+    const i64 ob = i * s, ol = (i - h) * s;
+    for (int c = 0; c < s; c++) {
+        const Cx<double> dd = D[(i64)c * s], pp = P[(i64)c * s];
+#pragma unroll
+        for (int q = 0; q < NR; q++) { cfma(acc[q], dd, b[q * vs + ob + c]); cfms(acc[q], pp, x[q * vs + ol + c]); }
+    }
+    if (m + 1 < M) {
+        const i64 orr = (i + h) * s;
+        for (int c = 0; c < s; c++) {
+            const Cx<double> qq = Qm[(i64)c * s];
+#pragma unroll
+            for (int q = 0; q < NR; q++) cfms(acc[q], qq, x[q * vs + orr + c]);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NR; q++) x[q * vs + i * s + r] = acc[q];
+}
+
+template <int NR>
+__global__ __launch_bounds__(TPB) void k_bcr_fwd(int s, i64 M, i64 h, i64 ne, const Cx<double> *__restrict__ AG,
+                                                 Cx<double> *b, i64 vs)
 {
     const i64 gid = (i64)blockIdx.x * TPB + threadIdx.x;
     const i64 j = gid / s;
-    const int r = (int)(gid % s);
-    if (j >= no) return;
-    const i64 m = 2 * j + 1, i = m * h, ss = (i64)s * s;
-    const Cx<double> *D = DPQ + j * 3 * ss + r, *P = D + ss, *Q = P + ss;
-    const Cx<double> *bi = b + i * s, *xl = x + (i - h) * s;
-    Cx<double> acc = {0, 0};
-    for (int c = 0; c < s; c++) { cfma(acc, D[(i64)c * s], bi[c]); cfms(acc, P[(i64)c * s], xl[c]); }
-    if (m + 1 < M) {
-        const Cx<double> *xr = x + (i + h) * s;
-        for (int c = 0; c < s; c++) cfms(acc, Q[(i64)c * s], xr[c]);
-    }
-    x[i * s + r] = acc;
+    if (j >= ne) return;
+    bcr_fwd_row<NR>(s, M, h, j, (int)(gid % s), AG, b, vs);
 }
 
-// final block: x_0 = Dinv b_0
-__global__ void k_bcr_final(int s, const Cx<double> *__restrict__ D, const Cx<double> *__restrict__ b,
-                            Cx<double> *x)
+template <int NR>
+__global__ __launch_bounds__(TPB) void k_bcr_bwd(int s, i64 M, i64 h, i64 no, const Cx<double> *__restrict__ DPQ,
+                                                 const Cx<double> *b, Cx<double> *x, i64 vs)
 {
-    const int r = threadIdx.x;
-    if (r >= s) return;
-    Cx<double> acc = {0, 0};
-    for (int c = 0; c < s; c++) cfma(acc, D[(i64)c * s + r], b[c]);
-    x[r] = acc;
+    const i64 gid = (i64)blockIdx.x * TPB + threadIdx.x;
+    const i64 j = gid / s;
+    if (j >= no) return;
+    bcr_bwd_row<NR>(s, M, h, j, (int)(gid % s), DPQ, b, x, vs);
 }
 
-// border: t = b2 - dots; y2 = Sinv t -> y2out (ntau) and yout[n1 + a]
-__global__ void k_border_small(int ntau, const Cx<double> *__restrict__ b2, const Cx<double> *__restrict__ dots,
-                               const Cx<double> *__restrict__ Sinv, Cx<double> *y2out, Cx<double> *ytail)
+// The small levels l0..L-1, the final s x s solve and the backward levels
+// L-1..l0 in one workgroup of 1024 threads (every level here has at most
+// 1024 rows of work), separated by workgroup barriers.  off[2l] / off[2l+1]:
+// block offsets of level l's even / odd factor blocks, off[2L]: final block.
+template <int NR>
+__global__ __launch_bounds__(1024) void k_bcr_tail(int s, i64 N, int l0, int L, const i64 *__restrict__ off,
+                                                   const Cx<double> *__restrict__ F, Cx<double> *b, Cx<double> *x,
+                                                   i64 vs)
+{
+    const int t = threadIdx.x;
+    const i64 ss = (i64)s * s;
+    for (int l = l0; l < L; l++) {
+        const i64 h = (i64)1 << l, M = (N + h - 1) >> l, ne = (M + 1) / 2;
+        for (i64 g = t; g < ne * s; g += 1024) bcr_fwd_row<NR>(s, M, h, g / s, (int)(g % s), F + off[2 * l] * ss, b, vs);
+        __syncthreads();
+    }
+    if (t < s) {
+        const Cx<double> *D = F + off[2 * L] * ss + t;
+        Cx<double> acc[NR];
+#pragma unroll
+        for (int q = 0; q < NR; q++) acc[q] = cx0();
+        for (int c = 0; c < s; c++) {
+            const Cx<double> dd = D[(i64)c * s];
+#pragma unroll
+            for (int q = 0; q < NR; q++) cfma(acc[q], dd, b[q * vs + c]);
+        }
+#pragma unroll
+        for (int q = 0; q < NR; q++) x[q * vs + t] = acc[q];
+    }
+    __syncthreads();
+    for (int l = L - 1; l >= l0; l--) {
+        const i64 h = (i64)1 << l, M = (N + h - 1) >> l, no = M / 2;
+        for (i64 g = t; g < no * s; g += 1024) bcr_bwd_row<NR>(s, M, h, g / s, (int)(g % s), F + off[2 * l + 1] * ss, b, x, vs);
+        __syncthreads();
+    }
+}
+
+// border, vector q: t = b2_q - dots_q; y2_q = Sinv t -> y2out + q ntau and the
+// tail of y_q.  One workgroup, NR * ntau threads used.
+template <int NR>
+__global__ void k_border_small(int ntau, const Cx<double> *b2_0, const Cx<double> *b2_1, const Cx<double> *dots0,
+                               const Cx<double> *dots1, const Cx<double> *__restrict__ Sinv, Cx<double> *y2out,
+                               Cx<double> *ytail0, Cx<double> *ytail1)
 {
     extern __shared__ char sm[];
-    Cx<double> *t = reinterpret_cast<Cx<double> *>(sm);
-    const int a = threadIdx.x;
-    if (a < ntau) { t[a].re = b2[a].re - dots[a].re; t[a].im = b2[a].im - dots[a].im; }
+    Cx<double> *tt = reinterpret_cast<Cx<double> *>(sm);
+    const int a = threadIdx.x % ntau, q = threadIdx.x / ntau;
+    const bool act = q < NR;
+    if (act) {
+        const Cx<double> b2 = (q == 0 ? b2_0 : b2_1)[a], d = (q == 0 ? dots0 : dots1)[a];
+        tt[q * ntau + a] = cx(b2.re - d.re, b2.im - d.im);
+    }
     __syncthreads();
-    if (a < ntau) {
-        Cx<double> acc = {0, 0};
-        for (int c = 0; c < ntau; c++) cfma(acc, Sinv[(i64)c * ntau + a], t[c]);
-        y2out[a] = acc;
-        ytail[a] = acc;
+    if (act) {
+        Cx<double> acc = cx0();
+        for (int c = 0; c < ntau; c++) cfma(acc, Sinv[(i64)c * ntau + a], tt[q * ntau + c]);
+        y2out[q * ntau + a] = acc;
+        (q == 0 ? ytail0 : ytail1)[a] = acc;
     }
 }
 
-// y[r] = xb[r] - sum_a Y[a n1 + r] y2[a], r < n1
-__global__ __launch_bounds__(TPB) void k_border_update(i64 n1, int ntau, const Cx<double> *__restrict__ xb,
+// y_q[r] = xb_q[r] - sum_a Y[a n1 + r] y2_q[a], r < n1
+template <int NR>
+__global__ __launch_bounds__(TPB) void k_border_update(i64 n1, int ntau, const Cx<double> *__restrict__ xb, i64 vs,
                                                        const Cx<double> *__restrict__ Y,
-                                                       const Cx<double> *__restrict__ y2, Cx<double> *y)
+                                                       const Cx<double> *__restrict__ y2, Cx<double> *y0,
+                                                       Cx<double> *y1)
 {
     for (i64 i = (i64)blockIdx.x * TPB + threadIdx.x; i < n1; i += (i64)gridDim.x * TPB) {
-        Cx<double> acc = xb[i];
-        for (int a = 0; a < ntau; a++) cfms(acc, Y[(i64)a * n1 + i], y2[a]);
-        y[i] = acc;
+        Cx<double> acc[NR];
+#pragma unroll
+        for (int q = 0; q < NR; q++) acc[q] = xb[q * vs + i];
+        for (int a = 0; a < ntau; a++) {
+            const Cx<double> yv = Y[(i64)a * n1 + i];
+#pragma unroll
+            for (int q = 0; q < NR; q++) cfms(acc[q], yv, y2[q * ntau + a]);
+        }
+        y0[i] = acc[0];
+        if (NR == 2) y1[i] = acc[NR - 1];
     }
 }
 
@@ -194,43 +278,74 @@ int bbjd_o_bj_apply(hipStream_t s, int64_t n, int bs, const void *inv, const voi
     return chk();
 }
 
-int bbjd_o_bcr_fwd(hipStream_t st, int s, int64_t M, int64_t h, int64_t ne, const void *AG, void *b)
+int bbjd_o_bcr_fwd(hipStream_t st, int s, int64_t M, int64_t h, int64_t ne, const void *AG, void *b, int nrhs,
+                   int64_t vs)
 {
     if (ne <= 0) return 0;
-    hipLaunchKernelGGL(k_bcr_fwd, dim3(blocks_for(ne * s)), dim3(TPB), 0, st, s, (i64)M, (i64)h, (i64)ne,
-                       (const Cx<double> *)AG, (Cx<double> *)b);
+    if (nrhs == 2)
+        hipLaunchKernelGGL(k_bcr_fwd<2>, dim3(blocks_for(ne * s)), dim3(TPB), 0, st, s, (i64)M, (i64)h, (i64)ne,
+                           (const Cx<double> *)AG, (Cx<double> *)b, (i64)vs);
+    else
+        hipLaunchKernelGGL(k_bcr_fwd<1>, dim3(blocks_for(ne * s)), dim3(TPB), 0, st, s, (i64)M, (i64)h, (i64)ne,
+                           (const Cx<double> *)AG, (Cx<double> *)b, (i64)vs);
     return chk();
 }
 
-int bbjd_o_bcr_bwd(hipStream_t st, int s, int64_t M, int64_t h, int64_t no, const void *DPQ, const void *b, void *x)
+int bbjd_o_bcr_bwd(hipStream_t st, int s, int64_t M, int64_t h, int64_t no, const void *DPQ, const void *b, void *x,
+                   int nrhs, int64_t vs)
 {
     if (no <= 0) return 0;
-    hipLaunchKernelGGL(k_bcr_bwd, dim3(blocks_for(no * s)), dim3(TPB), 0, st, s, (i64)M, (i64)h, (i64)no,
-                       (const Cx<double> *)DPQ, (const Cx<double> *)b, (Cx<double> *)x);
+    if (nrhs == 2)
+        hipLaunchKernelGGL(k_bcr_bwd<2>, dim3(blocks_for(no * s)), dim3(TPB), 0, st, s, (i64)M, (i64)h, (i64)no,
+                           (const Cx<double> *)DPQ, (const Cx<double> *)b, (Cx<double> *)x, (i64)vs);
+    else
+        hipLaunchKernelGGL(k_bcr_bwd<1>, dim3(blocks_for(no * s)), dim3(TPB), 0, st, s, (i64)M, (i64)h, (i64)no,
+                           (const Cx<double> *)DPQ, (const Cx<double> *)b, (Cx<double> *)x, (i64)vs);
     return chk();
 }
 
-int bbjd_o_bcr_final(hipStream_t st, int s, const void *D, const void *b, void *x)
+int bbjd_o_bcr_tail(hipStream_t st, int s, int64_t N, int l0, int L, const int64_t *off, const void *F, void *b,
+                    void *x, int nrhs, int64_t vs)
 {
-    hipLaunchKernelGGL(k_bcr_final, dim3(1), dim3(s <= 64 ? 64 : 256), 0, st, s, (const Cx<double> *)D,
-                       (const Cx<double> *)b, (Cx<double> *)x);
+    if (s > 1024) return -2;
+    if (nrhs == 2)
+        hipLaunchKernelGGL(k_bcr_tail<2>, dim3(1), dim3(1024), 0, st, s, (i64)N, l0, L, (const i64 *)off,
+                           (const Cx<double> *)F, (Cx<double> *)b, (Cx<double> *)x, (i64)vs);
+    else
+        hipLaunchKernelGGL(k_bcr_tail<1>, dim3(1), dim3(1024), 0, st, s, (i64)N, l0, L, (const i64 *)off,
+                           (const Cx<double> *)F, (Cx<double> *)b, (Cx<double> *)x, (i64)vs);
     return chk();
 }
 
-int bbjd_o_border_small(hipStream_t st, int ntau, const void *b2, const void *dots, const void *Sinv, void *y2,
-                        void *ytail)
+int bbjd_o_border_small(hipStream_t st, int ntau, const void *b2_0, const void *b2_1, const void *dots0,
+                        const void *dots1, const void *Sinv, void *y2, void *ytail0, void *ytail1, int nrhs)
 {
-    hipLaunchKernelGGL(k_border_small, dim3(1), dim3(ntau <= 64 ? 64 : 256), (size_t)ntau * 16, st, ntau,
-                       (const Cx<double> *)b2, (const Cx<double> *)dots, (const Cx<double> *)Sinv,
-                       (Cx<double> *)y2, (Cx<double> *)ytail);
+    if (ntau <= 0 || ntau * nrhs > 1024) return -2;
+    const int thr = ntau * nrhs <= 64 ? 64 : (ntau * nrhs <= 256 ? 256 : 1024);
+    if (nrhs == 2)
+        hipLaunchKernelGGL(k_border_small<2>, dim3(1), dim3(thr), (size_t)2 * ntau * 16, st, ntau,
+                           (const Cx<double> *)b2_0, (const Cx<double> *)b2_1, (const Cx<double> *)dots0,
+                           (const Cx<double> *)dots1, (const Cx<double> *)Sinv, (Cx<double> *)y2,
+                           (Cx<double> *)ytail0, (Cx<double> *)ytail1);
+    else
+        hipLaunchKernelGGL(k_border_small<1>, dim3(1), dim3(thr), (size_t)ntau * 16, st, ntau,
+                           (const Cx<double> *)b2_0, (const Cx<double> *)b2_0, (const Cx<double> *)dots0,
+                           (const Cx<double> *)dots0, (const Cx<double> *)Sinv, (Cx<double> *)y2,
+                           (Cx<double> *)ytail0, (Cx<double> *)ytail0);
     return chk();
 }
 
-int bbjd_o_border_update(hipStream_t st, int64_t n1, int ntau, const void *xb, const void *Y, const void *y2,
-                         void *y)
+int bbjd_o_border_update(hipStream_t st, int64_t n1, int ntau, const void *xb, int64_t vs, const void *Y,
+                         const void *y2, void *y0, void *y1, int nrhs)
 {
-    hipLaunchKernelGGL(k_border_update, dim3(gs_blocks(n1)), dim3(TPB), 0, st, (i64)n1, ntau,
-                       (const Cx<double> *)xb, (const Cx<double> *)Y, (const Cx<double> *)y2, (Cx<double> *)y);
+    if (nrhs == 2)
+        hipLaunchKernelGGL(k_border_update<2>, dim3(gs_blocks(n1)), dim3(TPB), 0, st, (i64)n1, ntau,
+                           (const Cx<double> *)xb, (i64)vs, (const Cx<double> *)Y, (const Cx<double> *)y2,
+                           (Cx<double> *)y0, (Cx<double> *)y1);
+    else
+        hipLaunchKernelGGL(k_border_update<1>, dim3(gs_blocks(n1)), dim3(TPB), 0, st, (i64)n1, ntau,
+                           (const Cx<double> *)xb, (i64)vs, (const Cx<double> *)Y, (const Cx<double> *)y2,
+                           (Cx<double> *)y0, (Cx<double> *)y0);
     return chk();
 }
 

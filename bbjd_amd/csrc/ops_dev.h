@@ -10,13 +10,16 @@ int bbjd_o_band_mv(hipStream_t s, int64_t n1, int64_t nb, int bs, int hw, const 
                    int single);
 int bbjd_o_diag_mv(hipStream_t s, int64_t n, const void *dg, const void *x, void *y, int single);
 int bbjd_o_bj_apply(hipStream_t s, int64_t n, int bs, const void *inv, const void *x, void *y, int single);
-int bbjd_o_bcr_fwd(hipStream_t st, int s, int64_t M, int64_t h, int64_t ne, const void *AG, void *b);
-int bbjd_o_bcr_bwd(hipStream_t st, int s, int64_t M, int64_t h, int64_t no, const void *DPQ, const void *b, void *x);
-int bbjd_o_bcr_final(hipStream_t st, int s, const void *D, const void *b, void *x);
-int bbjd_o_border_small(hipStream_t st, int ntau, const void *b2, const void *dots, const void *Sinv, void *y2,
-                        void *ytail);
-int bbjd_o_border_update(hipStream_t st, int64_t n1, int ntau, const void *xb, const void *Y, const void *y2,
-                         void *y);
+int bbjd_o_bcr_fwd(hipStream_t st, int s, int64_t M, int64_t h, int64_t ne, const void *AG, void *b, int nrhs,
+                   int64_t vs);
+int bbjd_o_bcr_bwd(hipStream_t st, int s, int64_t M, int64_t h, int64_t no, const void *DPQ, const void *b, void *x,
+                   int nrhs, int64_t vs);
+int bbjd_o_bcr_tail(hipStream_t st, int s, int64_t N, int l0, int L, const int64_t *off, const void *F, void *b,
+                    void *x, int nrhs, int64_t vs);
+int bbjd_o_border_small(hipStream_t st, int ntau, const void *b2_0, const void *b2_1, const void *dots0,
+                        const void *dots1, const void *Sinv, void *y2, void *ytail0, void *ytail1, int nrhs);
+int bbjd_o_border_update(hipStream_t st, int64_t n1, int ntau, const void *xb, int64_t vs, const void *Y,
+                         const void *y2, void *y0, void *y1, int nrhs);
 #ifdef __cplusplus
 }
 #endif

@@ -46,6 +46,8 @@ struct bbjd_op {
     void *bcr;                          /* factor storage                    */
     bbjd_int *even_off, *odd_off;       /* per level, in s x s blocks        */
     bbjd_int final_off;
+    int l0;                             /* first level run by the tail kernel */
+    void *doff;                         /* device copy of the level offsets  */
     void *bvec, *xvec;                  /* padded work vectors (N s)         */
     void *Y, *Sinv, *small;             /* border                            */
     bbjd_red rd;                        /* tau-row dot scratch               */
@@ -163,6 +165,11 @@ static int bcr_factor(struct bbjd_op *op, const bbjd_pencil_desc *d, zcx **host_
     }
     op->final_off = (bbjd_int)nbk;
     nbk += 1;
+    op->l0 = L;
+    for (int l = 0; l < L; l++) {
+        const bbjd_int M = (N + ((bbjd_int)1 << l) - 1) >> l;
+        if (((M + 1) / 2) * s <= 1024) { op->l0 = l; break; }
+    }
     zcx *H = malloc(nbk * ss * sizeof(zcx));
     zcx *Lc = malloc((size_t)N * ss * sizeof(zcx)), *Dc = malloc((size_t)N * ss * sizeof(zcx)),
         *Uc = malloc((size_t)N * ss * sizeof(zcx));
@@ -234,46 +241,60 @@ static hipStream_t op_stream(const struct bbjd_op *op)
     return s ? s : op->stream;
 }
 
-/* x (padded N s, b already loaded) -> BCR solve into op->xvec */
-static int bcr_solve(struct bbjd_op *op, hipStream_t st)
+/* BCR solve of nrhs right-hand sides already in op->bvec (vector q at
+ * q N s) into op->xvec: the large levels one launch each, the levels from
+ * op->l0 on, the final solve and their backward levels in one workgroup. */
+static int bcr_solve(struct bbjd_op *op, hipStream_t st, int nrhs)
 {
     const int s = op->s;
     const size_t ss = (size_t)s * s;
+    const int64_t vs = (int64_t)op->N * s;
     int rc = 0;
-    for (int l = 0; l < op->L; l++) {
+    for (int l = 0; l < op->l0; l++) {
         const bbjd_int h = (bbjd_int)1 << l, M = (op->N + h - 1) >> l;
         rc |= bbjd_o_bcr_fwd(st, s, M, h, (M + 1) / 2, (const zcx *)op->bcr + (size_t)op->even_off[l] * ss,
-                             op->bvec);
+                             op->bvec, nrhs, vs);
     }
-    rc |= bbjd_o_bcr_final(st, s, (const zcx *)op->bcr + (size_t)op->final_off * ss, op->bvec, op->xvec);
-    for (int l = op->L - 1; l >= 0; l--) {
+    rc |= bbjd_o_bcr_tail(st, s, op->N, op->l0, op->L, (const int64_t *)op->doff, op->bcr, op->bvec, op->xvec,
+                          nrhs, vs);
+    for (int l = op->l0 - 1; l >= 0; l--) {
         const bbjd_int h = (bbjd_int)1 << l, M = (op->N + h - 1) >> l;
         rc |= bbjd_o_bcr_bwd(st, s, M, h, M / 2, (const zcx *)op->bcr + (size_t)op->odd_off[l] * ss, op->bvec,
-                             op->xvec);
+                             op->xvec, nrhs, vs);
     }
     return rc;
 }
 
-static int exact_apply(struct bbjd_op *op, hipStream_t st, const void *x, void *y)
+/* y_q = M^-1 x_q for nrhs (1 or 2) vectors */
+static int exact_apply(struct bbjd_op *op, hipStream_t st, int nrhs, const void *x0, void *y0, const void *x1,
+                       void *y1)
 {
     int rc = 0;
     const bbjd_int n1 = op->n1;
-    rc |= !HIPOK(hipMemcpyAsync(op->bvec, x, (size_t)n1 * 16, hipMemcpyDeviceToDevice, st));
-    rc |= bcr_solve(op, st);
+    const int64_t vs = (int64_t)op->N * op->s;
+    rc |= !HIPOK(hipMemcpyAsync(op->bvec, x0, (size_t)n1 * 16, hipMemcpyDeviceToDevice, st));
+    if (nrhs == 2)
+        rc |= !HIPOK(hipMemcpyAsync((zcx *)op->bvec + vs, x1, (size_t)n1 * 16, hipMemcpyDeviceToDevice, st));
+    rc |= bcr_solve(op, st, nrhs);
     if (op->ntau == 0) {
-        rc |= !HIPOK(hipMemcpyAsync(y, op->xvec, (size_t)n1 * 16, hipMemcpyDeviceToDevice, st));
+        rc |= !HIPOK(hipMemcpyAsync(y0, op->xvec, (size_t)n1 * 16, hipMemcpyDeviceToDevice, st));
+        if (nrhs == 2)
+            rc |= !HIPOK(hipMemcpyAsync(y1, (zcx *)op->xvec + vs, (size_t)n1 * 16, hipMemcpyDeviceToDevice, st));
         return rc;
     }
-    zcx *sm = (zcx *)op->small;   /* [0, ntau): dots; [ntau, 2 ntau): y2 */
-    rc |= bbjd_k_gemv_c(st, n1, op->ntau, op->tauc, op->n, op->xvec, NULL, sm, NULL, &op->rd);
-    rc |= bbjd_o_border_small(st, op->ntau, (const zcx *)x + n1, sm, op->Sinv, sm + op->ntau, (zcx *)y + n1);
-    rc |= bbjd_o_border_update(st, n1, op->ntau, op->xvec, op->Y, sm + op->ntau, y);
+    const int nt = op->ntau;
+    zcx *sm = (zcx *)op->small;   /* dots 2 nt | y2 2 nt */
+    rc |= bbjd_k_gemv_c(st, n1, nt, op->tauc, op->n, op->xvec, nrhs == 2 ? (zcx *)op->xvec + vs : NULL, sm,
+                        sm + nt, &op->rd);
+    rc |= bbjd_o_border_small(st, nt, (const zcx *)x0 + n1, nrhs == 2 ? (const zcx *)x1 + n1 : NULL, sm, sm + nt,
+                              op->Sinv, sm + 2 * nt, (zcx *)y0 + n1, nrhs == 2 ? (zcx *)y1 + n1 : NULL, nrhs);
+    rc |= bbjd_o_border_update(st, n1, nt, op->xvec, vs, op->Y, sm + 2 * nt, y0, y1, nrhs);
     return rc;
 }
 
 /* ---- device layout --------------------------------------------------- */
 enum { D_BAND, D_TAUC, D_BDIAG, D_BANDC, D_TAUCC, D_BDIAGC, D_BJ, D_BJC, D_BCR, D_BV, D_XV, D_Y, D_SINV, D_SMALL,
-       D_PART, D_PNORM, D_RSC, D_ZT1, D_ZT2, D_END };
+       D_PART, D_PNORM, D_RSC, D_ZT1, D_ZT2, D_OFF, D_END };
 
 static size_t bcr_block_count(bbjd_int N)
 {
@@ -303,14 +324,15 @@ static size_t op_layout(const bbjd_pencil_desc *d, int pc, int want_c64, bbjd_in
     }
     if (pc == BBJD_PC_EXACT_LU) {
         sz[D_BCR] = bcr_blocks * s * s * 16;
-        sz[D_BV] = sz[D_XV] = (size_t)N * s * 16;
+        sz[D_BV] = sz[D_XV] = (size_t)2 * N * s * 16;      /* two right-hand sides */
+        sz[D_OFF] = (size_t)(2 * 64 + 2) * 8;
         sz[D_Y] = (size_t)n1 * d->ntau * 16;
         sz[D_SINV] = (size_t)d->ntau * d->ntau * 16 + 16;
     }
-    sz[D_SMALL] = (size_t)(2 * d->ntau + 16) * 16;
+    sz[D_SMALL] = (size_t)(4 * d->ntau + 16) * 16;
     sz[D_PART] = (size_t)bbjd_red_part_entries(n, d->ntau > 0 ? d->ntau + 2 : 2) * 16;
     sz[D_PNORM] = ((size_t)(n + BBJD_PROJ_ROWS - 1) / BBJD_PROJ_ROWS + 64) * 8;
-    sz[D_RSC] = 4096;
+    sz[D_RSC] = 512 + BBJD_CNT_SLOTS * 4 + 1024;
     if (want_c64 && pc != BBJD_PC_NONE) sz[D_ZT1] = sz[D_ZT2] = (size_t)n * 16;
     size_t tot = 0;
     for (int i = 0; i < D_END; i++) { off[i] = tot; tot += (sz[i] + 255) / 256 * 256; }
@@ -375,10 +397,11 @@ bbjd_op *bbjd_op_create(const bbjd_pencil_desc *d, bbjd_zomplex sigma, bbjd_pc_k
     op->rd.part = AT(D_PART); op->rd.pnorm = AT(D_PNORM); op->rd.part_cap = pe;
     {
         char *r = b + off[D_RSC];
-        op->rd.scal = (double *)r; op->rd.flag = (int *)(r + 256); op->rd.h = r + 512; op->rd.h2 = r + 512;
-        op->rd.hsum = r + 512; op->rd.coef = r + 512; op->rd.coef2 = r + 512;
+        op->rd.scal = (double *)r; op->rd.flag = (int *)(r + 256); op->rd.cnt = (int *)(r + 512);
+        op->rd.h = r + 512 + BBJD_CNT_SLOTS * 4; op->rd.h2 = op->rd.h; op->rd.hsum = op->rd.h;
+        op->rd.coef = op->rd.h; op->rd.coef2 = op->rd.h;
     }
-    op->zt1 = AT(D_ZT1); op->zt2 = AT(D_ZT2);
+    op->zt1 = AT(D_ZT1); op->zt2 = AT(D_ZT2); op->doff = AT(D_OFF);
 #undef AT
     hipStream_t st = op->stream;
     int rc = 0;
@@ -447,6 +470,13 @@ bbjd_op *bbjd_op_create(const bbjd_pencil_desc *d, bbjd_zomplex sigma, bbjd_pc_k
     }
     if (!rc && pc == BBJD_PC_EXACT_LU) {
         rc |= !HIPOK(hipMemcpy(op->bcr, bcr_host, bcr_blocks * op->s * op->s * 16, hipMemcpyHostToDevice));
+        {
+            int64_t offs[2 * 64 + 2];
+            memset(offs, 0, sizeof(offs));
+            for (int l = 0; l < op->L && l < 64; l++) { offs[2 * l] = op->even_off[l]; offs[2 * l + 1] = op->odd_off[l]; }
+            offs[2 * op->L] = op->final_off;
+            rc |= !HIPOK(hipMemcpy(op->doff, offs, (size_t)(2 * op->L + 1) * 8, hipMemcpyHostToDevice));
+        }
         if (!rc && op->ntau > 0) {
             /* Y = M11^-1 M12 column by column on the device */
             const int nt = op->ntau;
@@ -459,7 +489,7 @@ bbjd_op *bbjd_op_create(const bbjd_pencil_desc *d, bbjd_zomplex sigma, bbjd_pc_k
                 memset(col, 0, (size_t)n1 * sizeof(zcx));
                 for (bbjd_int i = lo; i < n1; i++) col[i] = m_at(d, sigma, i, n1 + a);
                 rc |= !HIPOK(hipMemcpy(op->bvec, col, (size_t)n1 * 16, hipMemcpyHostToDevice));
-                rc |= bcr_solve(op, st);
+                rc |= bcr_solve(op, st, 1);
                 rc |= !HIPOK(hipMemcpyAsync((zcx *)op->Y + (size_t)a * n1, op->xvec, (size_t)n1 * 16,
                                             hipMemcpyDeviceToDevice, st));
                 /* M21 Y[:, a] */
@@ -477,7 +507,7 @@ bbjd_op *bbjd_op_create(const bbjd_pencil_desc *d, bbjd_zomplex sigma, bbjd_pc_k
                 free(work); free(piv);
             }
             /* the padded RHS entries of bvec must be zero for every apply */
-            rc |= !HIPOK(hipMemsetAsync(op->bvec, 0, (size_t)op->N * op->s * 16, st));
+            rc |= !HIPOK(hipMemsetAsync(op->bvec, 0, (size_t)2 * op->N * op->s * 16, st));
             free(col); free(Sh); free(dots);
         }
     }
@@ -512,10 +542,20 @@ int bbjd_op_precond(bbjd_int n, const bbjd_zomplex *x, bbjd_zomplex *y, void *da
     if (!op || n != op->n) return -1;
     hipStream_t st = op_stream(op);
     switch (op->pc) {
-    case BBJD_PC_EXACT_LU: return exact_apply(op, st, x, y) ? -1 : 0;
+    case BBJD_PC_EXACT_LU: return exact_apply(op, st, 1, x, y, NULL, NULL) ? -1 : 0;
     case BBJD_PC_BLOCK_JACOBI: return bbjd_o_bj_apply(st, n, op->bs, op->bj, x, y, 0) ? -1 : 0;
     default: return HIPOK(hipMemcpyAsync(y, x, (size_t)n * 16, hipMemcpyDeviceToDevice, st)) ? 0 : -1;
     }
+}
+
+int bbjd_op_precond2(bbjd_int n, const bbjd_zomplex *x0, bbjd_zomplex *y0, const bbjd_zomplex *x1,
+                     bbjd_zomplex *y1, void *data)
+{
+    struct bbjd_op *op = (struct bbjd_op *)data;
+    if (!op || n != op->n) return -1;
+    if (op->pc == BBJD_PC_EXACT_LU) return exact_apply(op, op_stream(op), 2, x0, y0, x1, y1) ? -1 : 0;
+    if (bbjd_op_precond(n, x0, y0, data)) return -1;
+    return bbjd_op_precond(n, x1, y1, data);
 }
 
 int bbjd_op_matvec_A_c64(bbjd_int n, const bbjd_cfloat *x, bbjd_cfloat *y, void *data)
@@ -545,7 +585,7 @@ int bbjd_op_precond_c64(bbjd_int n, const bbjd_cfloat *x, bbjd_cfloat *y, void *
     case BBJD_PC_BLOCK_JACOBI: return bbjd_o_bj_apply(st, n, op->bs, op->bj_c, x, y, 1) ? -1 : 0;
     case BBJD_PC_EXACT_LU: {
         int rc = bbjd_k_c2z(st, n, x, op->zt1);
-        rc |= exact_apply(op, st, op->zt1, op->zt2);
+        rc |= exact_apply(op, st, 1, op->zt1, op->zt2, NULL, NULL);
         rc |= bbjd_k_z2c(st, n, op->zt2, y);
         return rc ? -1 : 0;
     }

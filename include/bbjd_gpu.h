@@ -32,6 +32,11 @@ typedef enum {
 /* complex64 element, layout (re, im) of float */
 typedef struct { float re, im; } bbjd_cfloat;
 
+/* Two preconditioner applies at once, y0 = M^-1 x0 and y1 = M^-1 x1, in the
+ * workspace's callback space; same return contract as bbjd_precond_fn. */
+typedef int (*bbjd_precond2_fn)(bbjd_int n, const bbjd_zomplex *x0, bbjd_zomplex *y0,
+                                const bbjd_zomplex *x1, bbjd_zomplex *y1, void *data);
+
 /* complex64 device-pointer operator, same contract as bbjd_matvec_fn */
 typedef int (*bbjd_op_c64_fn)(bbjd_int n, const bbjd_cfloat *x,
                               bbjd_cfloat *y, void *data);
@@ -45,6 +50,10 @@ typedef struct {
     bbjd_op_c64_fn      matvec_A_c64;
     bbjd_op_c64_fn      matvec_B_c64;
     bbjd_op_c64_fn      precond_c64;
+    /* optional: the Sleijpen-van der Vorst step's two preconditioner applies
+     * (M^-1 r, M^-1 u) as one call; NULL: two precond calls.  Must apply the
+     * same M as params.precond. */
+    bbjd_precond2_fn    precond2;
 } bbjd_gpu_options;
 
 void bbjd_gpu_options_init(bbjd_gpu_options *o);
@@ -110,6 +119,10 @@ int  bbjd_ktime_harvest(double *total_ms, double *total_bytes);
  * x -= Q h, 3 in-place basis rotation k -> k/2, 4 full CGS2.  Writes the
  * average ms per launch and the algorithmic bytes per launch. */
 int bbjd_kbench(int which, int64_t n, int k, int reps, double *avg_ms, double *bytes);
+
+/* Grid size (blocks) up to which a reduction's finishing step runs in the
+ * last block of the producing kernel instead of a second launch. */
+void bbjd_set_fuse_blocks(int max_blocks);
 
 /* Device memory helpers for callers without another GPU runtime (tests,
  * tools): allocate / free / synchronise the device. */

@@ -75,6 +75,11 @@ int bbjd_op_matvec_A(bbjd_int n, const bbjd_zomplex *x, bbjd_zomplex *y, void *o
 int bbjd_op_matvec_B(bbjd_int n, const bbjd_zomplex *x, bbjd_zomplex *y, void *op);
 int bbjd_op_precond (bbjd_int n, const bbjd_zomplex *x, bbjd_zomplex *y, void *op);
 
+/* Two preconditioner applies in one pass over the factor (bbjd_precond2_fn):
+ * y0 = M^-1 x0, y1 = M^-1 x1. */
+int bbjd_op_precond2(bbjd_int n, const bbjd_zomplex *x0, bbjd_zomplex *y0, const bbjd_zomplex *x1,
+                     bbjd_zomplex *y1, void *op);
+
 /* complex64 forms (bbjd_op_c64_fn), available when want_c64 was set. */
 int bbjd_op_matvec_A_c64(bbjd_int n, const bbjd_cfloat *x, bbjd_cfloat *y, void *op);
 int bbjd_op_matvec_B_c64(bbjd_int n, const bbjd_cfloat *x, bbjd_cfloat *y, void *op);
