@@ -48,7 +48,7 @@ class Stats(C.Structure):
     _fields_ = [("outer_iters", C.c_int64), ("lock_iters", C.c_int64), ("restarts", C.c_int64),
                 ("inner_steps", C.c_int64), ("dgks_second_passes", C.c_int64),
                 ("sum_k", C.c_int64), ("qz_seconds", C.c_double), ("solve_seconds", C.c_double),
-                ("stream_bytes", C.c_double)]
+                ("stream_bytes", C.c_double), ("restart_fallbacks", C.c_int64)]
 
 
 class PencilDesc(C.Structure):
@@ -98,6 +98,9 @@ def load():
     _sig(lib, "bbjd_dev_sync", C.c_int)
     _sig(lib, "bbjd_time_op", C.c_int, V, I64, V, V, V, C.c_int, C.POINTER(D))
     _sig(lib, "bbjd_kbench", C.c_int, C.c_int, I64, C.c_int, C.c_int, C.POINTER(D), C.POINTER(D))
+    _sig(lib, "bbjd_set_tiles", C.c_int, C.c_int, C.c_int, C.c_int)
+    _sig(lib, "bbjd_block_gram", C.c_int, I64, I64, C.c_void_p, I64, C.c_void_p, C.c_void_p)
+    _sig(lib, "bbjd_set_fuse_blocks", None, C.c_int)
     _sig(lib, "bbjd_memcpy_d2h", C.c_int, V, V, C.c_size_t)
     _sig(lib, "bbjd_memcpy_h2d", C.c_int, V, V, C.c_size_t)
     _sig(lib, "bbjd_pencil_generate", C.c_int, I64, C.c_int, C.c_int, C.c_int, C.c_uint64, D, V, V, V)

@@ -16,8 +16,10 @@ extern "C" {
 #endif
 
 #define BBJD_TPB 256              /* threads per block of streaming kernels */
-#define BBJD_PROJ_ROWS 1024       /* rows per block of the projection kernel */
-#define BBJD_UPD_ROWS 1024        /* rows per block of the update kernel    */
+#define BBJD_TILE_MIN_ROWS 64     /* smallest row tile of any streaming kernel:
+                                     per-block norm partials are sized for it */
+#define BBJD_PROJ_MIN_ROWS 256    /* smallest row chunk of the projection:
+                                     projection partials are sized for it */
 #define BBJD_MAXK 4096            /* largest column count of one projection */
 
 /* Scalar slots in the device scalar array (double). */
@@ -109,8 +111,9 @@ int bbjd_k_svdv(hipStream_t s, int64_t n, const void *u, void *z, const void *sv
 int bbjd_k_rotate(hipStream_t s, int64_t n, int k, int keep, void *X, int64_t ldx,
                   const void *Zr);
 
-/* G[i + j ldg] = X[:, i]^H Y[:, j], i, j < m (device G) */
-int bbjd_k_gram(hipStream_t s, int64_t n, int m, const void *X, const void *Y, int64_t ld,
+/* G (p x q, column-major, leading dimension ldg) = X^H Y for tall-skinny
+ * X (n x p) and Y (n x q); deterministic (fixed-order chunk sums). */
+int bbjd_k_gram(hipStream_t s, int64_t n, int p, const void *X, int64_t ldx, int q, const void *Y, int64_t ldy,
                 void *G, int64_t ldg, bbjd_red *rd);
 
 /* precision conversion */

@@ -48,7 +48,8 @@ struct bbjd_dev {
     double          *hscal;            /* pinned: SL_COUNT doubles + flags  */
     zcx             *hsmall;           /* pinned: small vector staging      */
     zcx             *hx, *hy;          /* pinned: host-callback staging     */
-    zcx             *al, *be, *yv, *xv, *zrh, *hcol;
+    zcx             *al, *be, *yv, *xv;
+    zcx             *zrh, *hcol, *hgram;  /* pinned: rotation, pencil, Gram */
     bbjd_qz          qz;
     struct random_data rng;
     int32_t          rng_state[32];
@@ -110,7 +111,7 @@ static void make_layout(layout_t *L, const bbjd_params *p, const bbjd_gpu_option
     int maxk = (int)(ms > (size_t)L->ims + 1 ? ms : (size_t)L->ims + 1);
     if (maxk < L->nev + 1) maxk = (int)L->nev + 1;
     sz[O_PART] = (size_t)bbjd_red_part_entries(L->n, maxk + 2) * Z;
-    sz[O_PNORM] = ((size_t)(L->n + BBJD_PROJ_ROWS - 1) / BBJD_PROJ_ROWS + 64) * 8;
+    sz[O_PNORM] = ((size_t)(L->n + BBJD_TILE_MIN_ROWS - 1) / BBJD_TILE_MIN_ROWS + 64) * 8;
     sz[O_H] = sz[O_H2] = sz[O_HSUM] = ((size_t)maxk + 8) * Z;
     sz[O_SCAL] = SL_COUNT * 8;
     sz[O_FLAG] = FL_COUNT * 4;
@@ -166,13 +167,17 @@ void bbjd_workspace_free(bbjd_workspace *ws)
     if (!ws) return;
     struct bbjd_dev *d = ws->dev;
     if (d) {
+        if (d->stream) (void)hipStreamSynchronize(d->stream);   /* copies into pinned buffers */
         if (d->dblock) hipFree(d->dblock);
         if (d->hscal) hipHostFree(d->hscal);
         if (d->hsmall) hipHostFree(d->hsmall);
         if (d->hx) hipHostFree(d->hx);
         if (d->hy) hipHostFree(d->hy);
         if (d->own_stream && d->stream) hipStreamDestroy(d->stream);
-        free(d->al); free(d->be); free(d->yv); free(d->xv); free(d->zrh); free(d->hcol); free(d->pair_res);
+        if (d->zrh) hipHostFree(d->zrh);
+        if (d->hcol) hipHostFree(d->hcol);
+        if (d->hgram) hipHostFree(d->hgram);
+        free(d->al); free(d->be); free(d->yv); free(d->xv); free(d->pair_res);
         bbjd_qz_free(&d->qz);
         free(d);
     }
@@ -253,11 +258,15 @@ bbjd_workspace *bbjd_workspace_alloc_gpu(const bbjd_params *p, const bbjd_gpu_op
     d->al = malloc((size_t)L.maxsub * sizeof(zcx)); d->be = malloc((size_t)L.maxsub * sizeof(zcx));
     d->yv = malloc((size_t)(L.maxsub + L.ims + 2) * sizeof(zcx));
     d->xv = malloc((size_t)(L.maxsub + 2) * sizeof(zcx));
-    d->zrh = malloc(ss * sizeof(zcx));
-    d->hcol = malloc((size_t)(L.maxsub + L.ims + 8) * 4 * sizeof(zcx));
+    if (!HIPOK(hipHostMalloc((void **)&d->zrh, ss * sizeof(zcx), 0))) { d->zrh = NULL; goto fail; }
+    if (!HIPOK(hipHostMalloc((void **)&d->hcol, (size_t)(L.maxsub + L.ims + 8) * 4 * sizeof(zcx), 0))) {
+        d->hcol = NULL;
+        goto fail;
+    }
+    if (!HIPOK(hipHostMalloc((void **)&d->hgram, 2 * ss * sizeof(zcx), 0))) { d->hgram = NULL; goto fail; }
     d->pair_res = calloc((size_t)L.nev + 1, sizeof(double));
     if (!ws->MA || !ws->MB || !ws->SA || !ws->SB || !ws->ZA || !ws->ZB || !ws->gmres_H || !ws->gmres_cs ||
-        !ws->gmres_sn || !ws->gmres_g || !d->al || !d->be || !d->yv || !d->xv || !d->zrh || !d->hcol || !d->pair_res)
+        !ws->gmres_sn || !ws->gmres_g || !d->al || !d->be || !d->yv || !d->xv || !d->pair_res)
         goto fail;
     if (bbjd_qz_init(&d->qz, L.maxsub)) goto fail;
     if (!HIPOK(hipStreamSynchronize(d->stream))) goto fail;
@@ -607,6 +616,8 @@ static int pencil_update(ctx_t *c, bbjd_int k)
                             &d->rd);
     }
     rc |= !HIPOK(hipMemcpyAsync(d->hcol, pen, (size_t)4 * (ms + 1) * 16, hipMemcpyDeviceToHost, c->s));
+    /* the norms of the expansion (breakdown checks) come back with it */
+    rc |= !HIPOK(hipMemcpyAsync(d->hscal, d->rd.scal, SL_COUNT * 8, hipMemcpyDeviceToHost, c->s));
     rc |= !HIPOK(hipStreamSynchronize(c->s));
     if (rc) return -1;
     zcx *MA = (zcx *)ws->MA, *MB = (zcx *)ws->MB;
@@ -617,9 +628,58 @@ static int pencil_update(ctx_t *c, bbjd_int k)
     return 0;
 }
 
+/* One Cholesky-QR pass on W[:, :keep] in place: G = W^H W on the device,
+ * G = R^H R on the host, W <- W R^-1 on the device.  Returns 1 (W unchanged)
+ * when the pass cannot be trusted -- G not numerically positive definite, R
+ * too ill-conditioned (diagonal ratio below 1e-6, where the loss of
+ * orthogonality eps cond^2 would no longer be repaired by a second pass), or,
+ * with check, W already far from orthonormal (max |G - I| > 0.1) -- so the
+ * caller falls back to column-by-column CGS2. */
+static int cholqr_pass(ctx_t *c, bbjd_int keep, int check)
+{
+    bbjd_workspace *ws = c->ws;
+    struct bbjd_dev *d = c->d;
+    const bbjd_int n = c->n;
+    zcx *G = d->hgram;
+    int rc = bbjd_k_gram(c->s, n, (int)keep, ws->W, c->ld, (int)keep, ws->W, c->ld, d->gram, keep, &d->rd);
+    rc |= !HIPOK(hipMemcpyAsync(G, d->gram, (size_t)keep * keep * 16, hipMemcpyDeviceToHost, c->s));
+    rc |= !HIPOK(hipStreamSynchronize(c->s));
+    if (rc) return -1;
+    if (check)
+        for (bbjd_int j = 0; j < keep; j++)
+            for (bbjd_int i = 0; i <= j; i++)
+                if (cabs(G[i + j * keep] - (i == j ? 1.0 : 0.0)) > 0.1) return 1;
+    blas_int bk = (blas_int)keep, info = 0;
+    scipy_zpotrf_("U", &bk, G, &bk, &info);
+    if (info != 0) return 1;
+    double dmin = INFINITY, dmax = 0;
+    for (bbjd_int j = 0; j < keep; j++) {
+        const double r = creal(G[j + j * keep]);
+        if (r < dmin) dmin = r;
+        if (r > dmax) dmax = r;
+    }
+    if (!(dmin > 1e-6 * dmax)) return 1;
+    scipy_ztrtri_("U", "N", &bk, G, &bk, &info);
+    if (info != 0) return 1;
+    /* row-major R^-1 for the rotation: Zr[j keep + c] = (R^-1)[j, c] */
+    for (bbjd_int j = 0; j < keep; j++)
+        for (bbjd_int cc = 0; cc < keep; cc++) d->zrh[(size_t)j * keep + cc] = j <= cc ? G[j + cc * keep] : 0.0;
+    rc |= !HIPOK(hipMemcpyAsync(d->zr, d->zrh, (size_t)keep * keep * 16, hipMemcpyHostToDevice, c->s));
+    rc |= bbjd_k_rotate(c->s, n, (int)keep, (int)keep, ws->W, c->ld, d->zr);
+    return rc ? -1 : 0;
+}
+
 /* Thick restart (reference src/jd_core.c:975-1030): V, AV, BV <- (.) ZB[:, :keep]
- * in place on the device, W rebuilt column by column (CGS2), MA/MB = W^H AV,
- * W^H BV. */
+ * in place on the device; W[:, :keep] = orth(nu0 AV + mu0 BV); MA, MB =
+ * W^H AV, W^H BV.
+ *
+ * The reference orthonormalises the new W column by column (CGS2 against the
+ * previous columns, scaled when the norm exceeds 1e-300): the Q factor, with
+ * positive real R diagonal, of X = nu0 AV + mu0 BV.  Here the same Q factor
+ * comes from Cholesky QR twice -- two Gram passes and two in-place
+ * triangular rotations instead of keep^2 / 2 vector passes -- and the
+ * column-by-column CGS2 is kept as the fallback whenever a pass cannot be
+ * trusted (near-dependent columns).  The pencil is two Gram passes. */
 static int thick_restart(ctx_t *c, bbjd_int k, bbjd_int keep, zcx nu0, zcx mu0)
 {
     bbjd_workspace *ws = c->ws;
@@ -633,19 +693,27 @@ static int thick_restart(ctx_t *c, bbjd_int k, bbjd_int keep, zcx nu0, zcx mu0)
     rc |= bbjd_k_rotate(c->s, n, (int)k, (int)keep, ws->V, c->ld, d->zr);
     rc |= bbjd_k_rotate(c->s, n, (int)k, (int)keep, ws->AV, c->ld, d->zr);
     rc |= bbjd_k_rotate(c->s, n, (int)k, (int)keep, ws->BV, c->ld, d->zr);
-    for (bbjd_int j = 0; j < keep; j++) {
-        void *wj = COLZ(ws->W, j);
-        rc |= bbjd_k_lincomb(c->s, n, creal(nu0), cimag(nu0), COLZ(ws->AV, j), creal(mu0), cimag(mu0),
-                             COLZ(ws->BV, j), wj);
-        rc |= bbjd_k_cgs2(c->s, n, (int)j, ws->W, c->ld, wj, &d->rd, SL_T0, NULL);
-        rc |= bbjd_k_scale_inv(c->s, n, wj, d->rd.scal, SL_T0, 1);
+    /* X = nu0 AV + mu0 BV over the keep columns at once (padding rows
+     * between columns are never read) */
+    rc |= bbjd_k_lincomb(c->s, (keep - 1) * c->ld + n, creal(nu0), cimag(nu0), ws->AV, creal(mu0), cimag(mu0),
+                         ws->BV, ws->W);
+    if (rc) return -1;
+    int q = cholqr_pass(c, keep, 0);
+    if (q == 0) q = cholqr_pass(c, keep, 1);
+    if (q < 0) return -1;
+    if (q > 0) {
+        /* column-by-column CGS2 on the current W (X, or X R1^-1 after a first
+         * pass: the same column space, hence the same Q factor) */
+        d->st.restart_fallbacks++;
+        for (bbjd_int j = 0; j < keep; j++) {
+            void *wj = COLZ(ws->W, j);
+            rc |= bbjd_k_cgs2(c->s, n, (int)j, ws->W, c->ld, wj, &d->rd, SL_T0, NULL);
+            rc |= bbjd_k_scale_inv(c->s, n, wj, d->rd.scal, SL_T0, 1);
+        }
     }
-    zcx *G = (zcx *)d->gram;   /* [keep x keep] A then B, ld keep */
-    for (bbjd_int j = 0; j < keep; j++)
-        rc |= bbjd_k_gemv_c(c->s, n, (int)keep, ws->W, c->ld, COLZ(ws->AV, j), COLZ(ws->BV, j), G + j * keep,
-                            G + keep * keep + j * keep, &d->rd);
-    zcx *hg = malloc((size_t)2 * keep * keep * sizeof(zcx));
-    if (!hg) return -1;
+    zcx *G = (zcx *)d->gram, *hg = d->hgram;
+    rc |= bbjd_k_gram(c->s, n, (int)keep, ws->W, c->ld, (int)keep, ws->AV, c->ld, G, keep, &d->rd);
+    rc |= bbjd_k_gram(c->s, n, (int)keep, ws->W, c->ld, (int)keep, ws->BV, c->ld, G + keep * keep, keep, &d->rd);
     rc |= !HIPOK(hipMemcpyAsync(hg, G, (size_t)2 * keep * keep * 16, hipMemcpyDeviceToHost, c->s));
     rc |= !HIPOK(hipStreamSynchronize(c->s));
     zcx *MA = (zcx *)ws->MA, *MB = (zcx *)ws->MB;
@@ -654,15 +722,111 @@ static int thick_restart(ctx_t *c, bbjd_int k, bbjd_int keep, zcx nu0, zcx mu0)
             MA[i + j * ms] = hg[i + j * keep];
             MB[i + j * ms] = hg[keep * keep + i + j * keep];
         }
-    free(hg);
     d->st.restarts++;
-    d->st.stream_bytes += 16.0 * n * (3.0 * (k + keep) + 3.0 * keep + (double)keep * keep * 2 + 2.0 * keep * keep);
+    d->st.stream_bytes += 16.0 * n * (3.0 * (k + keep) + 3.0 * keep + 4.0 * keep + 3.0 * keep);
     return rc ? -1 : 0;
 }
 
 /* ------------------------------------------------------------------ */
 /* the outer iteration                                                 */
 /* ------------------------------------------------------------------ */
+/* Random start vector for a breakdown (reference src/jd_core.c:1043-1052):
+ * entries rand()/RAND_MAX - 0.5 in both parts, uploaded to v. */
+static int random_vector(ctx_t *c, void *v)
+{
+    struct bbjd_dev *d = c->d;
+    const bbjd_int n = c->n;
+    zcx *tmpv = d->hx ? d->hx : malloc((size_t)n * sizeof(zcx));
+    if (!tmpv) return -2;
+    for (bbjd_int i = 0; i < n; i++) {
+        double re = rng_unit(d) - 0.5;
+        double im = rng_unit(d) - 0.5;
+        tmpv[i] = re + im * I;
+    }
+    int rc = !HIPOK(hipMemcpyAsync(v, tmpv, (size_t)n * 16, hipMemcpyHostToDevice, c->s));
+    if (!d->hx) {
+        rc |= !HIPOK(hipStreamSynchronize(c->s));
+        free(tmpv);
+    }
+    return rc ? -2 : 0;
+}
+
+/* Expansion of the search and test spaces by column k (reference
+ * src/jd_core.c:1032-1104): v = CGS2(V, z) / ||.||, with a random vector
+ * when ||v|| < 1e-14; AV, BV columns; w = CGS2(W, nu0 Av + mu0 Bv) / ||.||,
+ * with Av in place of the combination when ||w|| < 1e-14; then the new row
+ * and column of the projected pencil.
+ *
+ * With device-space callbacks the breakdown tests are deferred: the
+ * expansion runs through with the norms applied on the device, and the norms
+ * come back to the host with the pencil column (one synchronisation instead
+ * of three).  In the rare breakdown case the column is recomputed along the
+ * reference's branch; every result of the first attempt lives in column k
+ * and is overwritten.  Returns 0, -1 callback error, -2 device error, -3
+ * breakdown. */
+static int expand(ctx_t *c, bbjd_int k, const void *z, zcx nu0, zcx mu0)
+{
+    bbjd_workspace *ws = c->ws;
+    struct bbjd_dev *d = c->d;
+    const bbjd_params *P = c->P;
+    const bbjd_int n = c->n;
+    void *vn = COLZ(ws->V, k), *wn = COLZ(ws->W, k), *an = COLZ(ws->AV, k), *bn = COLZ(ws->BV, k);
+    const int optimistic = d->opt.callback_space == BBJD_CALLBACK_DEVICE;
+    int rc = 0;
+    /* z may already be column k (formed in place); a breakdown of v replaces
+     * it with a random vector, so z is not needed after CGS2 */
+    const int in_place = z == vn;
+    if (!in_place) rc |= !HIPOK(hipMemcpyAsync(vn, z, (size_t)n * 16, hipMemcpyDeviceToDevice, c->s));
+    rc |= bbjd_k_cgs2(c->s, n, (int)k, ws->V, c->ld, vn, &d->rd, SL_V, NULL);
+    double nrm;
+    if (optimistic) {
+        rc |= bbjd_k_scale_inv(c->s, n, vn, d->rd.scal, SL_V, 1);
+        if (rc) return -2;
+        if (apply_op(c, P->matvec_A, vn, an) || apply_op(c, P->matvec_B, vn, bn)) return -1;
+        rc |= bbjd_k_lincomb(c->s, n, creal(nu0), cimag(nu0), an, creal(mu0), cimag(mu0), bn, wn);
+        rc |= bbjd_k_cgs2(c->s, n, (int)k, ws->W, c->ld, wn, &d->rd, SL_W, NULL);
+        rc |= bbjd_k_scale_inv(c->s, n, wn, d->rd.scal, SL_W, 1);
+        if (rc || pencil_update(c, k)) return -2;
+        d->st.stream_bytes += 16.0 * n * (4.0 * k + 4 + 3 + 4.0 * k + 2);
+        const double nv = d->hscal[SL_V], nw = d->hscal[SL_W];
+        if (nv >= 1e-14 && nw >= 1e-14) return 0;
+        if (nv >= 1e-14) {
+            /* only w broke down: v, Av, Bv stand; w from Av (the reference's
+             * fallback), then the pencil again */
+            rc |= !HIPOK(hipMemcpyAsync(wn, an, (size_t)n * 16, hipMemcpyDeviceToDevice, c->s));
+            rc |= bbjd_k_cgs2(c->s, n, (int)k, ws->W, c->ld, wn, &d->rd, SL_W, NULL);
+            rc |= bbjd_k_scale_inv(c->s, n, wn, d->rd.scal, SL_W, 1);
+            if (rc || pencil_update(c, k)) return -2;
+            return 0;
+        }
+        nrm = nv;           /* v broke down: the random-vector branch below */
+    } else {
+        if (rc || sync_scal(c)) return -2;
+        nrm = d->hscal[SL_V];
+    }
+    if (nrm < 1e-14) {
+        if (random_vector(c, vn)) return -2;
+        rc |= bbjd_k_cgs2(c->s, n, (int)k, ws->V, c->ld, vn, &d->rd, SL_V, NULL);
+        if (rc || sync_scal(c)) return -2;
+        nrm = d->hscal[SL_V];
+        if (nrm < 1e-14) return -3;
+    }
+    rc |= bbjd_k_scale(c->s, n, vn, 1.0 / nrm, 0.0);
+    if (rc) return -2;
+    if (apply_op(c, P->matvec_A, vn, an) || apply_op(c, P->matvec_B, vn, bn)) return -1;
+    rc |= bbjd_k_lincomb(c->s, n, creal(nu0), cimag(nu0), an, creal(mu0), cimag(mu0), bn, wn);
+    rc |= bbjd_k_cgs2(c->s, n, (int)k, ws->W, c->ld, wn, &d->rd, SL_W, NULL);
+    if (rc || sync_scal(c)) return -2;
+    if (d->hscal[SL_W] < 1e-14) {
+        rc |= !HIPOK(hipMemcpyAsync(wn, an, (size_t)n * 16, hipMemcpyDeviceToDevice, c->s));
+        rc |= bbjd_k_cgs2(c->s, n, (int)k, ws->W, c->ld, wn, &d->rd, SL_W, NULL);
+    }
+    rc |= bbjd_k_scale_inv(c->s, n, wn, d->rd.scal, SL_W, 1);
+    d->st.stream_bytes += 16.0 * n * (4.0 * k + 4 + 3 + 4.0 * k + 2);
+    if (rc || pencil_update(c, k)) return -2;
+    return 0;
+}
+
 static bbjd_status solve_impl(const bbjd_params *P, bbjd_workspace *ws, bbjd_result *R, bbjd_int ms_solve)
 {
     struct bbjd_dev *d = ws->dev;
@@ -788,7 +952,10 @@ static bbjd_status solve_impl(const bbjd_params *P, bbjd_workspace *ws, bbjd_res
         }
 
         /* ---- step 6: correction (:944-972) ---- */
+        /* Without a restart this iteration the correction is formed in place
+         * in the next basis column (the expansion's copy is then skipped). */
         void *zc = ws->tmp2;
+        if (P->inner_maxiter <= 0 && k < ms) zc = COLZ(ws->V, k);
         if (P->inner_maxiter > 0) {
             if (deflation_append(c, locked, u, Au, Bu, nu0, mu0)) { st = BBJD_ERR_ALLOC; goto done; }
             int g = gmres(c, ws->nlock + 1, ws->Q_right, ws->Q_left, ta, tb, r, zc, P->inner_maxiter,
@@ -796,14 +963,12 @@ static bbjd_status solve_impl(const bbjd_params *P, bbjd_workspace *ws, bbjd_res
             if (g == -1) { st = BBJD_ERR_CALLBACK; goto done; }
             if (g) { st = BBJD_ERR_ALLOC; goto done; }
         } else {
-            void *sv = ws->tmp4;
+            void *sv = ws->tmp3;       /* Bu is not needed any more */
             if (P->precond && d->opt.precond2 && d->opt.callback_space == BBJD_CALLBACK_DEVICE) {
-                /* z = M^-1 r and s = M^-1 u in one pass; s overwrites r, so
-                 * r goes through tmp3 (Bu is not needed any more) */
-                rc |= !HIPOK(hipMemcpyAsync(ws->tmp3, r, (size_t)n * 16, hipMemcpyDeviceToDevice, c->s));
+                /* z = M^-1 r and s = M^-1 u in one pass */
                 hipStream_t prev = tl_cb_stream;
                 tl_cb_stream = c->s;
-                int e = d->opt.precond2(n, (const bbjd_zomplex *)ws->tmp3, (bbjd_zomplex *)zc,
+                int e = d->opt.precond2(n, (const bbjd_zomplex *)r, (bbjd_zomplex *)zc,
                                         (const bbjd_zomplex *)u, (bbjd_zomplex *)sv, P->op_data);
                 tl_cb_stream = prev;
                 if (e) { st = BBJD_ERR_CALLBACK; goto done; }
@@ -828,42 +993,12 @@ static bbjd_status solve_impl(const bbjd_params *P, bbjd_workspace *ws, bbjd_res
         }
 
         /* ---- step 8: expansion (:1032-1068) ---- */
-        void *vn = COLZ(ws->V, k);
-        rc |= !HIPOK(hipMemcpyAsync(vn, zc, (size_t)n * 16, hipMemcpyDeviceToDevice, c->s));
-        rc |= bbjd_k_cgs2(c->s, n, (int)k, ws->V, c->ld, vn, &d->rd, SL_V, NULL);
-        if (rc || sync_scal(c)) { st = BBJD_ERR_ALLOC; goto done; }
-        double nrm = d->hscal[SL_V];
-        if (nrm < 1e-14) {
-            zcx *tmpv = d->hx ? d->hx : malloc((size_t)n * sizeof(zcx));
-            if (!tmpv) { st = BBJD_ERR_ALLOC; goto done; }
-            for (bbjd_int i = 0; i < n; i++) {
-                double re = rng_unit(d) - 0.5;
-                double im = rng_unit(d) - 0.5;
-                tmpv[i] = re + im * I;
-            }
-            rc |= !HIPOK(hipMemcpyAsync(vn, tmpv, (size_t)n * 16, hipMemcpyHostToDevice, c->s));
-            rc |= bbjd_k_cgs2(c->s, n, (int)k, ws->V, c->ld, vn, &d->rd, SL_V, NULL);
-            rc |= sync_scal(c);
-            if (!d->hx) free(tmpv);
-            nrm = d->hscal[SL_V];
-            if (nrm < 1e-14) { R->iters = iter + 1; st = BBJD_ERR_BREAKDOWN; goto done; }
+        {
+            int e = expand(c, k, zc, nu0, mu0);
+            if (e == -1) goto cb_err;
+            if (e == -3) { R->iters = iter + 1; st = BBJD_ERR_BREAKDOWN; goto done; }
+            if (e) { st = BBJD_ERR_ALLOC; goto done; }
         }
-        rc |= bbjd_k_scale(c->s, n, vn, 1.0 / nrm, 0.0);
-        if (apply_op(c, P->matvec_A, vn, COLZ(ws->AV, k)) || apply_op(c, P->matvec_B, vn, COLZ(ws->BV, k)))
-            goto cb_err;
-        void *wn = COLZ(ws->W, k);
-        rc |= bbjd_k_lincomb(c->s, n, creal(nu0), cimag(nu0), COLZ(ws->AV, k), creal(mu0), cimag(mu0),
-                             COLZ(ws->BV, k), wn);
-        rc |= bbjd_k_cgs2(c->s, n, (int)k, ws->W, c->ld, wn, &d->rd, SL_W, NULL);
-        if (rc || sync_scal(c)) { st = BBJD_ERR_ALLOC; goto done; }
-        if (d->hscal[SL_W] < 1e-14) {
-            rc |= !HIPOK(hipMemcpyAsync(wn, COLZ(ws->AV, k), (size_t)n * 16, hipMemcpyDeviceToDevice, c->s));
-            rc |= bbjd_k_cgs2(c->s, n, (int)k, ws->W, c->ld, wn, &d->rd, SL_W, NULL);
-        }
-        rc |= bbjd_k_scale_inv(c->s, n, wn, d->rd.scal, SL_W, 1);
-        d->st.stream_bytes += 16.0 * n * (4.0 * k + 4 + 3 + 4.0 * k + 2);
-        /* ---- projected pencil row/column (:1085-1104) ---- */
-        if (rc || pencil_update(c, k)) { st = BBJD_ERR_ALLOC; goto done; }
         bbjd_qz_invalidate(&d->qz);
         ws->kdim = k + 1;
         R->iters = iter + 1;
@@ -936,7 +1071,7 @@ static int helper_open(helper_t *h, bbjd_int n, bbjd_int k, size_t extra, void *
     if (!HIPOK(hipStreamCreateWithFlags(&h->s, hipStreamNonBlocking))) return -1;
     const int64_t pe = bbjd_red_part_entries(n, (int)(k + 2));
     size_t off_part = 0, off_pn = off_part + (size_t)pe * 16;
-    size_t off_h = off_pn + ((size_t)(n + BBJD_PROJ_ROWS - 1) / BBJD_PROJ_ROWS + 64) * 8;
+    size_t off_h = off_pn + ((size_t)(n + BBJD_TILE_MIN_ROWS - 1) / BBJD_TILE_MIN_ROWS + 64) * 8;
     size_t kk = (size_t)(k + 8) * 16;
     size_t off_h2 = off_h + kk, off_hs = off_h2 + kk, off_sc = off_hs + kk, off_fl = off_sc + 256,
            off_cn = off_fl + 64, off_co = off_cn + BBJD_CNT_SLOTS * 4, off_c2 = off_co + kk,
@@ -981,6 +1116,34 @@ double bbjd_block_orthogonalize(bbjd_int n, bbjd_int k, const bbjd_zomplex *Q, b
     }
     helper_close(&h);
     return res;
+}
+
+int bbjd_block_gram(bbjd_int n, bbjd_int p, const bbjd_zomplex *X, bbjd_int q, const bbjd_zomplex *Y,
+                    bbjd_zomplex *G)
+{
+    if (n <= 0 || p <= 0 || q <= 0 || p > BBJD_MAXK || q > BBJD_MAXK) return -1;
+    /* partial-sum room for at least one chunk of the p x q result */
+    const bbjd_int nt = (n + BBJD_PROJ_MIN_ROWS - 1) / BBJD_PROJ_MIN_ROWS;
+    bbjd_int kk = p > q ? p : q;
+    const bbjd_int need = (p * q + 2 * nt - 1) / (2 * nt);
+    if (need > kk) kk = need;
+    helper_t h;
+    void *dx = NULL;
+    const size_t xb = (size_t)n * p * 16, yb = (size_t)n * q * 16, gb = (size_t)p * q * 16;
+    const size_t oy = (xb + 255) / 256 * 256, og = oy + (yb + 255) / 256 * 256;
+    int rc = -1;
+    if (helper_open(&h, n, kk, og + gb, &dx) == 0) {
+        void *dy = (char *)dx + oy, *dg = (char *)dx + og;
+        rc = 0;
+        rc |= !HIPOK(hipMemcpyAsync(dx, X, xb, hipMemcpyHostToDevice, h.s));
+        rc |= !HIPOK(hipMemcpyAsync(dy, Y, yb, hipMemcpyHostToDevice, h.s));
+        rc |= bbjd_k_gram(h.s, n, (int)p, dx, n, (int)q, dy, n, dg, p, &h.rd);
+        rc |= !HIPOK(hipMemcpyAsync(G, dg, gb, hipMemcpyDeviceToHost, h.s));
+        rc |= !HIPOK(hipStreamSynchronize(h.s));
+        rc = rc ? -1 : 0;
+    }
+    helper_close(&h);
+    return rc;
 }
 
 double bbjd_mgs_orthogonalize(bbjd_int n, bbjd_int k, const bbjd_zomplex *Q, bbjd_zomplex *v)

@@ -62,8 +62,6 @@ template <typename R> __device__ __forceinline__ void cfma(Cx<R> &a, Cx<R> q, Cx
 { a.re += q.re * h.re - q.im * h.im; a.im += q.re * h.im + q.im * h.re; }
 
 static constexpr int TPB = BBJD_TPB;
-static constexpr int PROJ_E = BBJD_PROJ_ROWS / TPB;   // rows per thread
-static constexpr int UPD_E = BBJD_UPD_ROWS / TPB;
 
 // Reduce 2^LS values per lane over the 64-lane wavefront.  Afterwards lane l
 // holds in v[0] the wave sum of value index (l >> (6 - LS)).
@@ -169,12 +167,13 @@ __device__ __forceinline__ void finish_norm(const R *pnorm, int nblk, const Norm
     }
 }
 
-// Projection Q^H x (and Q^H x1): a block owns 1024 rows x C columns and
-// writes its partial sums part[(tile*NV + vi)*ldp + col]; the last row tile
-// of each column tile to finish reduces that column tile's partials into
+// Projection Q^H x (and Q^H x1): a block owns a chunk of rows x C columns
+// and writes its partial sums part[(tile*NV + vi)*ldp + col]; the last row
+// tile of each column tile to finish reduces that column tile's partials into
 // out0/out1.  With want_norm the column-tile-0 blocks also produce ||x0||^2
 // partials and the last of them writes scal[norm_slot] = ||x0||.
-// grid = (column tiles, row tiles).
+// grid = (column tiles, row chunks).
+//
 // Finish of a projection for column tile c0: the partials of ntiles row
 // tiles in a fixed order (16 lanes per (vector, column) pair), plus the norm
 // when norm_here.  Run by the last block of the column tile (fused) or by
@@ -209,7 +208,7 @@ __device__ __forceinline__ void proj_finish(int c0, int ntiles, int k, const Cx<
 }
 
 template <typename R, int NV, int C>
-__global__ __launch_bounds__(TPB) void k_proj(i64 n, int k, const Cx<R> *__restrict__ Q, i64 ldq,
+__global__ __launch_bounds__(TPB) void k_proj(i64 n, i64 chunk, int k, const Cx<R> *__restrict__ Q, i64 ldq,
                                               const Cx<R> *__restrict__ x0, const Cx<R> *__restrict__ x1,
                                               Cx<R> *part, int ldp, R *pnorm, int want_norm,
                                               const int *__restrict__ cond, int *cnt, Cx<R> *out0, Cx<R> *out1,
@@ -217,51 +216,62 @@ __global__ __launch_bounds__(TPB) void k_proj(i64 n, int k, const Cx<R> *__restr
 {
     if (cond && cond[0] == 0) return;
     constexpr int NVAL = 2 * NV * C;
-    static_assert(NVAL == 32, "butterfly width");
-    constexpr int LS = 5;
+    static_assert(NVAL == 32 || NVAL == 16, "butterfly width");
+    constexpr int LS = NVAL == 32 ? 5 : 4;
     __shared__ R lds[4][NVAL];
     __shared__ R lds4[4];
     __shared__ int lflag;
     const int t = threadIdx.x;
     const int tile = blockIdx.y;
-    const i64 rb = (i64)tile * (TPB * PROJ_E);
     const int c0 = blockIdx.x * C;
+    const i64 r1 = (i64)(tile + 1) * chunk < n ? (i64)(tile + 1) * chunk : n;
+    const bool do_norm = want_norm && blockIdx.x == 0;
+    const Cx<R> *q0 = Q + (i64)c0 * ldq;
 
-    Cx<R> xv[NV][PROJ_E];
-#pragma unroll
-    for (int e = 0; e < PROJ_E; e++) {
-        const i64 r = rb + t + (i64)e * TPB;
-        xv[0][e] = r < n ? x0[r] : cx<R>(0, 0);
-        if (NV == 2) xv[NV - 1][e] = r < n ? x1[r] : cx<R>(0, 0);
-    }
+    // A block streams its chunk of rows; a thread accumulates C columns
+    // (NV vectors) over rows t, t + TPB, ... of the chunk.
     R acc[NVAL];
 #pragma unroll
     for (int i = 0; i < NVAL; i++) acc[i] = 0;
+    R nv = 0;
+    if (c0 + C <= k) {
+#pragma unroll 2
+        for (i64 r = (i64)tile * chunk + t; r < r1; r += TPB) {
+            Cx<R> xv[NV];
+            xv[0] = x0[r];
+            if (NV == 2) xv[NV - 1] = x1[r];
+            Cx<R> qv[C];
 #pragma unroll
-    for (int c = 0; c < C; c++) {
-        const int col = c0 + c;
-        if (col < k) {
-            const Cx<R> *q = Q + (i64)col * ldq;
+            for (int c = 0; c < C; c++) qv[c] = q0[(i64)c * ldq + r];
+            if (do_norm) nv += xv[0].re * xv[0].re + xv[0].im * xv[0].im;
 #pragma unroll
-            for (int e = 0; e < PROJ_E; e++) {
-                const i64 r = rb + t + (i64)e * TPB;
-                if (r < n) {
-                    const Cx<R> qq = q[r];
+            for (int c = 0; c < C; c++)
+#pragma unroll
+                for (int vi = 0; vi < NV; vi++)
+                    cfma_conj(acc[2 * (vi * C + c)], acc[2 * (vi * C + c) + 1], qv[c], xv[vi]);
+        }
+    } else {
+        const int ncol = k - c0;
+        for (i64 r = (i64)tile * chunk + t; r < r1; r += TPB) {
+            Cx<R> xv[NV];
+            xv[0] = x0[r];
+            if (NV == 2) xv[NV - 1] = x1[r];
+            if (do_norm) nv += xv[0].re * xv[0].re + xv[0].im * xv[0].im;
+#pragma unroll
+            for (int c = 0; c < C; c++) {
+                if (c < ncol) {
+                    const Cx<R> qq = q0[(i64)c * ldq + r];
 #pragma unroll
                     for (int vi = 0; vi < NV; vi++)
-                        cfma_conj(acc[2 * (vi * C + c)], acc[2 * (vi * C + c) + 1], qq, xv[vi][e]);
+                        cfma_conj(acc[2 * (vi * C + c)], acc[2 * (vi * C + c) + 1], qq, xv[vi]);
                 }
             }
         }
     }
     wave_transpose_reduce<R, LS>(acc);
     const int lane = t & 63, w = t >> 6;
-    if ((lane & 1) == 0) lds[w][lane >> 1] = acc[0];
-    R nv = 0;
-    if (want_norm && blockIdx.x == 0) {
-#pragma unroll
-        for (int e = 0; e < PROJ_E; e++) nv += xv[0][e].re * xv[0][e].re + xv[0][e].im * xv[0][e].im;
-    }
+    // after the butterfly, lane l holds value index l >> (6 - LS)
+    if ((lane & ((64 >> LS) - 1)) == 0) lds[w][lane >> (6 - LS)] = acc[0];
     __syncthreads();
     if (t < NVAL) {
         const R s = (lds[0][t] + lds[1][t]) + (lds[2][t] + lds[3][t]);
@@ -272,7 +282,7 @@ __global__ __launch_bounds__(TPB) void k_proj(i64 n, int k, const Cx<R> *__restr
             dst[part_im] = s;
         }
     }
-    if (want_norm && blockIdx.x == 0) {
+    if (do_norm) {
         const R s = block_sum(nv, lds4);
         if (t == 0) pnorm[tile] = s;
     }
@@ -297,7 +307,7 @@ __global__ __launch_bounds__(TPB) void k_proj_finish(int ntiles, int k, const Cx
 // x = beta*x + alpha * Q[:, :k] h; with fin.mode >= 0 the squared norm of
 // the result is reduced (last block) and finished per NormFin.
 // Dynamic LDS: k complex.
-template <typename R>
+template <typename R, int E>
 __global__ __launch_bounds__(TPB) void k_update(i64 n, int k, const Cx<R> *__restrict__ Q, i64 ldq,
                                                 const Cx<R> *__restrict__ h, R alpha, int beta_one,
                                                 Cx<R> *__restrict__ x, R *pnorm, const int *__restrict__ cond,
@@ -311,27 +321,28 @@ __global__ __launch_bounds__(TPB) void k_update(i64 n, int k, const Cx<R> *__res
     const int t = threadIdx.x;
     for (int j = t; j < k; j += TPB) { const Cx<R> hj = h[j]; sh[j] = cx(alpha * hj.re, alpha * hj.im); }
     __syncthreads();
-    const i64 rb = (i64)blockIdx.x * (TPB * UPD_E);
-    Cx<R> a[UPD_E];
+    const i64 rb = (i64)blockIdx.x * (TPB * E);
+    Cx<R> a[E];
 #pragma unroll
-    for (int e = 0; e < UPD_E; e++) {
+    for (int e = 0; e < E; e++) {
         const i64 r = rb + t + (i64)e * TPB;
         a[e] = (beta_one && r < n) ? x[r] : cx<R>(0, 0);
     }
-    if (rb + TPB * UPD_E <= n) {
-#pragma unroll 4
+    if (rb + TPB * E <= n) {
+        constexpr int UNR = E == 4 ? 4 : 8;     // loads in flight per thread
+#pragma unroll UNR
         for (int j = 0; j < k; j++) {
             const Cx<R> hj = sh[j];
             const Cx<R> *q = Q + (i64)j * ldq + rb + t;
 #pragma unroll
-            for (int e = 0; e < UPD_E; e++) cfma(a[e], q[(i64)e * TPB], hj);
+            for (int e = 0; e < E; e++) cfma(a[e], q[(i64)e * TPB], hj);
         }
     } else {
         for (int j = 0; j < k; j++) {
             const Cx<R> hj = sh[j];
             const Cx<R> *q = Q + (i64)j * ldq;
 #pragma unroll
-            for (int e = 0; e < UPD_E; e++) {
+            for (int e = 0; e < E; e++) {
                 const i64 r = rb + t + (i64)e * TPB;
                 if (r < n) cfma(a[e], q[r], hj);
             }
@@ -339,9 +350,74 @@ __global__ __launch_bounds__(TPB) void k_update(i64 n, int k, const Cx<R> *__res
     }
     R s = 0;
 #pragma unroll
-    for (int e = 0; e < UPD_E; e++) {
+    for (int e = 0; e < E; e++) {
         const i64 r = rb + t + (i64)e * TPB;
         if (r < n) { x[r] = a[e]; s += a[e].re * a[e].re + a[e].im * a[e].im; }
+    }
+    if (fin.mode < 0) return;
+    s = block_sum(s, lds4);
+    if (t == 0) pnorm[blockIdx.x] = s;
+    if (!fused || !last_block_of(cnt, gridDim.x, &lflag)) return;
+    finish_norm<R>(pnorm, gridDim.x, fin, lds4);
+    if (t == 0) *cnt = 0;
+}
+
+// The same update with the columns split over the block's four waves: a
+// block owns 64 * RW rows, wave w accumulates columns w, w + 4, ... of its
+// rows, and the four partial sums are added in a fixed order through LDS.
+// Four times the blocks of k_update for the same rows, so more loads are in
+// flight on shapes whose rows alone do not fill the GPU.
+template <typename R, int RW>
+__global__ __launch_bounds__(TPB) void k_update_split(i64 n, int k, const Cx<R> *__restrict__ Q, i64 ldq,
+                                                      const Cx<R> *__restrict__ h, R alpha, int beta_one,
+                                                      Cx<R> *__restrict__ x, R *pnorm,
+                                                      const int *__restrict__ cond, int *cnt, NormFin fin,
+                                                      int fused)
+{
+    if (cond && cond[0] == 0) return;
+    extern __shared__ char smem_raw[];
+    Cx<R> *sh = reinterpret_cast<Cx<R> *>(smem_raw);
+    __shared__ Cx<R> red[4][64 * RW];
+    __shared__ R lds4[4];
+    __shared__ int lflag;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    for (int j = t; j < k; j += TPB) { const Cx<R> hj = h[j]; sh[j] = cx(alpha * hj.re, alpha * hj.im); }
+    __syncthreads();
+    const i64 rb = (i64)blockIdx.x * (64 * RW);
+    Cx<R> a[RW];
+#pragma unroll
+    for (int e = 0; e < RW; e++) a[e] = cx<R>(0, 0);
+    if (rb + 64 * RW <= n) {
+#pragma unroll 8
+        for (int j = w; j < k; j += 4) {
+            const Cx<R> hj = sh[j];
+            const Cx<R> *q = Q + (i64)j * ldq + rb + lane;
+#pragma unroll
+            for (int e = 0; e < RW; e++) cfma(a[e], q[e * 64], hj);
+        }
+    } else {
+        for (int j = w; j < k; j += 4) {
+            const Cx<R> hj = sh[j];
+            const Cx<R> *q = Q + (i64)j * ldq;
+#pragma unroll
+            for (int e = 0; e < RW; e++) {
+                const i64 r = rb + lane + e * 64;
+                if (r < n) cfma(a[e], q[r], hj);
+            }
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < RW; e++) red[w][lane + e * 64] = a[e];
+    __syncthreads();
+    R s = 0;
+    if (t < 64 * RW) {
+        const i64 r = rb + t;
+        if (r < n) {
+            Cx<R> v = (red[0][t] + red[1][t]) + (red[2][t] + red[3][t]);
+            if (beta_one) v = x[r] + v;
+            x[r] = v;
+            s = v.re * v.re + v.im * v.im;
+        }
     }
     if (fin.mode < 0) return;
     s = block_sum(s, lds4);
@@ -361,16 +437,16 @@ __global__ __launch_bounds__(TPB) void k_norm_fin(const R *pnorm, int nblk, Norm
 }
 
 // ||x|| -> scal[slot] in one launch.
-template <typename R>
+template <typename R, int E>
 __global__ __launch_bounds__(TPB) void k_norm(i64 n, const Cx<R> *__restrict__ x, R *pnorm, int *cnt, NormFin fin,
                                               int fused)
 {
     __shared__ R lds4[4];
     __shared__ int lflag;
-    const i64 rb = (i64)blockIdx.x * (TPB * UPD_E);
+    const i64 rb = (i64)blockIdx.x * (TPB * E);
     R s = 0;
 #pragma unroll
-    for (int e = 0; e < UPD_E; e++) {
+    for (int e = 0; e < E; e++) {
         const i64 r = rb + threadIdx.x + (i64)e * TPB;
         if (r < n) { const Cx<R> v = x[r]; s += v.re * v.re + v.im * v.im; }
     }
@@ -401,7 +477,7 @@ __global__ __launch_bounds__(TPB) void k_scale(i64 n, Cx<R> *x, R are, R aim)
     for (i64 i = (i64)blockIdx.x * TPB + threadIdx.x; i < n; i += (i64)gridDim.x * TPB) x[i] = a * x[i];
 }
 
-template <typename R>
+template <typename R, int E>
 __global__ __launch_bounds__(TPB) void k_lincomb(i64 n, R are, R aim, const Cx<R> *x, R bre, R bim,
                                                  const Cx<R> *y, Cx<R> *out, R *pnorm, int *cnt, NormFin fin,
                                                  int fused)
@@ -409,10 +485,10 @@ __global__ __launch_bounds__(TPB) void k_lincomb(i64 n, R are, R aim, const Cx<R
     __shared__ R lds4[4];
     __shared__ int lflag;
     const Cx<R> a = cx(are, aim), b = cx(bre, bim);
-    const i64 rb = (i64)blockIdx.x * (TPB * UPD_E);
+    const i64 rb = (i64)blockIdx.x * (TPB * E);
     R s = 0;
 #pragma unroll
-    for (int e = 0; e < UPD_E; e++) {
+    for (int e = 0; e < E; e++) {
         const i64 r = rb + threadIdx.x + (i64)e * TPB;
         if (r < n) {
             const Cx<R> v = a * x[r] + b * y[r];
@@ -497,6 +573,78 @@ __global__ __launch_bounds__(TPB) void k_rotate(i64 n, int k, int keep, Cx<doubl
     }
 }
 
+// Gram block G = X^H Y of two tall-skinny complex matrices (X n x p, Y n x q):
+// a block owns a 64 x 64 tile of G and a chunk of rows; the chunk is staged
+// through LDS in slabs of GR_SR rows and each thread accumulates a 4 x 4
+// sub-tile (rows ty + 16a, columns tx + 16b).  Partials per chunk go to
+// part[chunk][i + j p]; k_gram_finish adds the chunks in a fixed order.
+static constexpr int GR_T = 64, GR_SR = 32, GR_LD = GR_SR + 1;   // +1: LDS banks
+
+__global__ __launch_bounds__(TPB) void k_gram(i64 n, i64 chunk, int p, int q, const Cx<double> *__restrict__ X,
+                                              i64 ldx, const Cx<double> *__restrict__ Y, i64 ldy,
+                                              Cx<double> *__restrict__ part)
+{
+    __shared__ Cx<double> xs[GR_T * GR_LD], ys[GR_T * GR_LD];
+    const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
+    const int tiles_q = (q + GR_T - 1) / GR_T;
+    const int i0 = (blockIdx.x / tiles_q) * GR_T, j0 = (blockIdx.x % tiles_q) * GR_T;
+    const i64 rb = (i64)blockIdx.y * chunk;
+    const i64 re = rb + chunk < n ? rb + chunk : n;
+    double ar[4][4], ai[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) ar[a][b] = ai[a][b] = 0;
+    for (i64 r0 = rb; r0 < re; r0 += GR_SR) {
+        // stage GR_SR rows x 64 columns of X and of Y (zeros outside)
+#pragma unroll
+        for (int e = 0; e < GR_T * GR_SR / TPB; e++) {
+            const int idx = t + e * TPB, rr = idx % GR_SR, cc = idx / GR_SR;
+            const i64 r = r0 + rr;
+            const bool rok = r < re;
+            xs[cc * GR_LD + rr] = (rok && i0 + cc < p) ? X[(i64)(i0 + cc) * ldx + r] : cx(0.0, 0.0);
+            ys[cc * GR_LD + rr] = (rok && j0 + cc < q) ? Y[(i64)(j0 + cc) * ldy + r] : cx(0.0, 0.0);
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int rr = 0; rr < GR_SR; rr++) {
+            Cx<double> xa[4], yb[4];
+#pragma unroll
+            for (int a = 0; a < 4; a++) xa[a] = xs[(ty + 16 * a) * GR_LD + rr];
+#pragma unroll
+            for (int b = 0; b < 4; b++) yb[b] = ys[(tx + 16 * b) * GR_LD + rr];
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int b = 0; b < 4; b++) cfma_conj(ar[a][b], ai[a][b], xa[a], yb[b]);
+        }
+        __syncthreads();
+    }
+    Cx<double> *pc = part + (i64)blockIdx.y * p * q;
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            const int i = i0 + ty + 16 * a, j = j0 + tx + 16 * b;
+            if (i < p && j < q) pc[(i64)j * p + i] = cx(ar[a][b], ai[a][b]);
+        }
+}
+
+__global__ __launch_bounds__(TPB) void k_gram_finish(int nch, int p, int q, const Cx<double> *__restrict__ part,
+                                                     Cx<double> *G, i64 ldg)
+{
+    const i64 e = (i64)blockIdx.x * TPB + threadIdx.x, pq = (i64)p * q;
+    if (e >= pq) return;
+    double sr = 0, si = 0;
+    for (int c = 0; c < nch; c++) {
+        const Cx<double> v = part[(i64)c * pq + e];
+        sr += v.re;
+        si += v.im;
+    }
+    const int i = (int)(e % p), j = (int)(e / p);
+    G[(i64)j * ldg + i] = cx(sr, si);
+}
+
 __global__ __launch_bounds__(TPB) void k_z2c(i64 n, const Cx<double> *z, Cx<float> *c)
 {
     for (i64 i = (i64)blockIdx.x * TPB + threadIdx.x; i < n; i += (i64)gridDim.x * TPB)
@@ -510,8 +658,43 @@ __global__ __launch_bounds__(TPB) void k_c2z(i64 n, const Cx<float> *c, Cx<doubl
 
 // ---------------------------------------------------------------------------
 // host launchers
-static inline int ntiles_proj(i64 n) { return (int)((n + BBJD_PROJ_ROWS - 1) / BBJD_PROJ_ROWS); }
-static inline int ntiles_upd(i64 n) { return (int)((n + BBJD_UPD_ROWS - 1) / BBJD_UPD_ROWS); }
+// Row tiles.  A block of the norm / lincomb kernels, and of the update for
+// k < 16, owns TPB * E rows, E in {1, 2, 4}, chosen per launch from the
+// shape: the largest tile that still gives g_blocks_target blocks.  From
+// k = 16 the update splits the columns over the block's waves (128 rows per
+// block).  A block of the projection kernel streams a chunk of rows (a
+// multiple of BBJD_PROJ_MIN_ROWS) sized so that the launch has about
+// g_proj_blocks blocks: long chunks amortise the block's cross-lane
+// reduction, enough blocks keep every CU streaming.  The defaults come from
+// sweeps of n in [2e4, 1e6], k in [8, 128] on MI355X; all can be forced for
+// tuning.
+// Partial-sum buffers are sized for the smallest tiles, BBJD_TILE_MIN_ROWS
+// (norm partials, one per block) and BBJD_PROJ_MIN_ROWS (projection chunks).
+static int g_upd_rows = 0;                      // 0: choose from the shape
+static int g_blocks_target = 2048;
+static int g_proj_blocks = 192;
+
+static inline int ntiles(i64 n, int rows) { return (int)((n + rows - 1) / rows); }
+static inline int pick_rows(i64 n, int col_tiles)
+{
+    for (int rows = 4 * TPB; rows > TPB; rows >>= 1)
+        if ((i64)ntiles(n, rows) * col_tiles >= g_blocks_target) return rows;
+    return TPB;
+}
+static inline int upd_rows(i64 n, int k)
+{
+    if (g_upd_rows) return g_upd_rows;
+    return k >= 16 ? 128 : pick_rows(n, 1);     // measured: column split wins from k = 16
+}
+static inline int vec_rows(i64 n) { return g_upd_rows >= TPB ? g_upd_rows : pick_rows(n, 1); }
+static inline i64 proj_chunk(i64 n, int col_tiles)
+{
+    i64 chunks = (g_proj_blocks + col_tiles - 1) / col_tiles;
+    if (chunks < 1) chunks = 1;
+    i64 chunk = (n + chunks - 1) / chunks;
+    chunk = (chunk + BBJD_PROJ_MIN_ROWS - 1) / BBJD_PROJ_MIN_ROWS * BBJD_PROJ_MIN_ROWS;
+    return chunk > 0 ? chunk : BBJD_PROJ_MIN_ROWS;
+}
 static inline int grid_stride_blocks(i64 n)
 {
     i64 b = (n + TPB - 1) / TPB;
@@ -523,7 +706,7 @@ static inline int chk(void) { return hipGetLastError() == hipSuccess ? 0 : -1; }
 
 extern "C" int64_t bbjd_red_part_entries(int64_t n, int maxk)
 {
-    return (int64_t)ntiles_proj(n) * 2 * (maxk + 1) + 64;
+    return (int64_t)ntiles(n, BBJD_PROJ_MIN_ROWS) * 2 * (maxk + 1) + 64;
 }
 
 // Blocks up to which the finishing step runs in the producing kernel's last
@@ -545,54 +728,59 @@ static inline NormFin fin_norm(bbjd_red *rd, int mode, int slot)
     return f;
 }
 
-// One projection pass in one launch: out0 (and out1) <- Q^H x0 (x1); with
-// norm_slot >= 0 also scal[norm_slot] <- ||x0||.
+#define BBJD_E_SWITCH(rows, CALL) \
+    switch (rows) {                  \
+    case 4 * TPB: CALL(4); break;    \
+    case 2 * TPB: CALL(2); break;    \
+    default: CALL(1); break;         \
+    }
+
+// One projection pass in one launch (+ a finishing launch for large grids):
+// out0 (and out1) <- Q^H x0 (x1); with norm_slot >= 0 also scal[norm_slot]
+// <- ||x0||.
+template <typename R, int NV>
+static int proj_pass_nv(hipStream_t s, i64 n, int k, const void *Q, i64 ldq, const void *x0, const void *x1,
+                        void *out0, void *out1, bbjd_red *rd, int norm_slot, const int *cond)
+{
+    constexpr int C = NV == 1 ? 16 : 8;
+    const int ctiles = k > 0 ? (k + C - 1) / C : 1;
+    const i64 chunk = proj_chunk(n, ctiles);
+    const int nt = (int)((n + chunk - 1) / chunk);
+    const int ldp = k > 0 ? k : 1;
+    const int want_norm = norm_slot >= 0;
+    if ((i64)nt * NV * ldp > rd->part_cap || k > BBJD_MAXK) return -2;
+    if (k == 0 && !want_norm) return 0;
+    const int fused = nt <= g_fuse_max_blocks;
+    dim3 g(ctiles, nt);
+    const bool timed = NV == 1 && g_kt_on && sizeof(R) == 8 && cond == nullptr;
+    hipEvent_t ea = nullptr, eb = nullptr;
+    if (timed) {
+        std::lock_guard<std::mutex> lk(g_kt_mu);
+        ea = kt_event(); eb = kt_event();
+        (void)hipEventRecord(ea, s);
+    }
+    hipLaunchKernelGGL((k_proj<R, NV, C>), g, dim3(TPB), 0, s, (i64)n, chunk, k, (const Cx<R> *)Q, (i64)ldq,
+                       (const Cx<R> *)x0, (const Cx<R> *)x1, (Cx<R> *)rd->part, ldp, (R *)rd->pnorm, want_norm, cond,
+                       rd->cnt, (Cx<R> *)out0, (Cx<R> *)out1, rd->scal, norm_slot, fused);
+    if (timed) {
+        (void)hipEventRecord(eb, s);
+        const double bytes = (double)sizeof(Cx<R>) * (double)n * ((double)k + 1.0);
+        std::lock_guard<std::mutex> lk(g_kt_mu);
+        g_kt.push_back({ea, eb, bytes});
+    }
+    if (!fused)
+        hipLaunchKernelGGL((k_proj_finish<R, NV, C>), dim3(g.x), dim3(TPB), 0, s, nt, k, (const Cx<R> *)rd->part,
+                           ldp, (const R *)rd->pnorm, want_norm, cond, (Cx<R> *)out0, (Cx<R> *)out1, rd->scal,
+                           norm_slot);
+    return chk();
+}
+
 template <typename R>
 static int proj_pass(hipStream_t s, i64 n, int k, const void *Q, i64 ldq, const void *x0, const void *x1,
                      void *out0, void *out1, bbjd_red *rd, int norm_slot, const int *cond)
 {
-    const int nt = ntiles_proj(n);
-    const int nv = x1 ? 2 : 1;
-    const int ldp = k > 0 ? k : 1;
-    const int want_norm = norm_slot >= 0;
-    if ((i64)nt * nv * ldp > rd->part_cap || k > BBJD_MAXK) return -2;
-    if (k == 0 && !want_norm) return 0;
-    const int fused = nt <= g_fuse_max_blocks;
-    if (nv == 1) {
-        constexpr int C = 16;
-        dim3 g(k > 0 ? (k + C - 1) / C : 1, nt);
-        const bool timed = g_kt_on && sizeof(R) == 8 && cond == nullptr;
-        hipEvent_t ea = nullptr, eb = nullptr;
-        if (timed) {
-            std::lock_guard<std::mutex> lk(g_kt_mu);
-            ea = kt_event(); eb = kt_event();
-        }
-        if (timed) (void)hipEventRecord(ea, s);
-        hipLaunchKernelGGL((k_proj<R, 1, C>), g, dim3(TPB), 0, s, (i64)n, k, (const Cx<R> *)Q, (i64)ldq,
-                           (const Cx<R> *)x0, (const Cx<R> *)nullptr, (Cx<R> *)rd->part, ldp, (R *)rd->pnorm,
-                           want_norm, cond, rd->cnt, (Cx<R> *)out0, (Cx<R> *)out1, rd->scal, norm_slot, fused);
-        if (timed) {
-            (void)hipEventRecord(eb, s);
-            const double bytes = (double)sizeof(Cx<R>) * (double)n * ((double)k + (double)g.x);
-            std::lock_guard<std::mutex> lk(g_kt_mu);
-            g_kt.push_back({ea, eb, bytes});
-        }
-        if (!fused)
-            hipLaunchKernelGGL((k_proj_finish<R, 1, C>), dim3(g.x), dim3(TPB), 0, s, nt, k,
-                               (const Cx<R> *)rd->part, ldp, (const R *)rd->pnorm, want_norm, cond, (Cx<R> *)out0,
-                               (Cx<R> *)out1, rd->scal, norm_slot);
-    } else {
-        constexpr int C = 8;
-        dim3 g(k > 0 ? (k + C - 1) / C : 1, nt);
-        hipLaunchKernelGGL((k_proj<R, 2, C>), g, dim3(TPB), 0, s, (i64)n, k, (const Cx<R> *)Q, (i64)ldq,
-                           (const Cx<R> *)x0, (const Cx<R> *)x1, (Cx<R> *)rd->part, ldp, (R *)rd->pnorm,
-                           want_norm, cond, rd->cnt, (Cx<R> *)out0, (Cx<R> *)out1, rd->scal, norm_slot, fused);
-        if (!fused)
-            hipLaunchKernelGGL((k_proj_finish<R, 2, C>), dim3(g.x), dim3(TPB), 0, s, nt, k,
-                               (const Cx<R> *)rd->part, ldp, (const R *)rd->pnorm, want_norm, cond, (Cx<R> *)out0,
-                               (Cx<R> *)out1, rd->scal, norm_slot);
-    }
-    return chk();
+    return x1 ? proj_pass_nv<R, 2>(s, n, k, Q, ldq, x0, x1, out0, out1, rd, norm_slot, cond)
+              : proj_pass_nv<R, 1>(s, n, k, Q, ldq, x0, nullptr, out0, nullptr, rd, norm_slot, cond);
 }
 
 template <typename R>
@@ -600,12 +788,29 @@ static int update_pass(hipStream_t s, i64 n, int k, const void *Q, i64 ldq, cons
                        void *x, bbjd_red *rd, NormFin fin, const int *cond)
 {
     if (k > BBJD_MAXK) return -2;
-    const int nt = ntiles_upd(n);
+    const int rows = upd_rows(n, k);
+    const int nt = ntiles(n, rows);
     const size_t lds = (size_t)(k > 0 ? k : 1) * sizeof(Cx<R>);
     const int fused = nt <= g_fuse_max_blocks;
-    hipLaunchKernelGGL((k_update<R>), dim3(nt), dim3(TPB), lds, s, (i64)n, k, (const Cx<R> *)Q, (i64)ldq,
-                       (const Cx<R> *)h, alpha, beta_one, (Cx<R> *)x, (R *)rd->pnorm, cond,
-                       rd->cnt + BBJD_CNT_UPD, fin, fused);
+    if (rows < TPB) {
+        if (rows == 128)
+            hipLaunchKernelGGL((k_update_split<R, 2>), dim3(nt), dim3(TPB), lds, s, (i64)n, k, (const Cx<R> *)Q,
+                               (i64)ldq, (const Cx<R> *)h, alpha, beta_one, (Cx<R> *)x, (R *)rd->pnorm, cond,
+                               rd->cnt + BBJD_CNT_UPD, fin, fused);
+        else
+            hipLaunchKernelGGL((k_update_split<R, 1>), dim3(nt), dim3(TPB), lds, s, (i64)n, k, (const Cx<R> *)Q,
+                               (i64)ldq, (const Cx<R> *)h, alpha, beta_one, (Cx<R> *)x, (R *)rd->pnorm, cond,
+                               rd->cnt + BBJD_CNT_UPD, fin, fused);
+        if (!fused && fin.mode >= 0)
+            hipLaunchKernelGGL((k_norm_fin<R>), dim3(1), dim3(TPB), 0, s, (const R *)rd->pnorm, nt, fin, cond);
+        return chk();
+    }
+#define BBJD_LAUNCH_UPD(E)                                                                                         \
+    hipLaunchKernelGGL((k_update<R, E>), dim3(nt), dim3(TPB), lds, s, (i64)n, k, (const Cx<R> *)Q, (i64)ldq,         \
+                       (const Cx<R> *)h, alpha, beta_one, (Cx<R> *)x, (R *)rd->pnorm, cond, rd->cnt + BBJD_CNT_UPD, \
+                       fin, fused)
+    BBJD_E_SWITCH(rows, BBJD_LAUNCH_UPD)
+#undef BBJD_LAUNCH_UPD
     if (!fused && fin.mode >= 0)
         hipLaunchKernelGGL((k_norm_fin<R>), dim3(1), dim3(TPB), 0, s, (const R *)rd->pnorm, nt, fin, cond);
     return chk();
@@ -614,10 +819,14 @@ static int update_pass(hipStream_t s, i64 n, int k, const void *Q, i64 ldq, cons
 template <typename R>
 static int norm_into(hipStream_t s, i64 n, const void *x, bbjd_red *rd, int slot)
 {
-    const int nt = ntiles_upd(n);
+    const int rows = vec_rows(n);
+    const int nt = ntiles(n, rows);
     const int fused = nt <= g_fuse_max_blocks;
-    hipLaunchKernelGGL((k_norm<R>), dim3(nt), dim3(TPB), 0, s, (i64)n, (const Cx<R> *)x, (R *)rd->pnorm,
-                       rd->cnt + BBJD_CNT_VEC, fin_norm(rd, 0, slot), fused);
+#define BBJD_LAUNCH_NORM(E)                                                                                        \
+    hipLaunchKernelGGL((k_norm<R, E>), dim3(nt), dim3(TPB), 0, s, (i64)n, (const Cx<R> *)x, (R *)rd->pnorm,         \
+                       rd->cnt + BBJD_CNT_VEC, fin_norm(rd, 0, slot), fused)
+    BBJD_E_SWITCH(rows, BBJD_LAUNCH_NORM)
+#undef BBJD_LAUNCH_NORM
     if (!fused)
         hipLaunchKernelGGL((k_norm_fin<R>), dim3(1), dim3(TPB), 0, s, (const R *)rd->pnorm, nt,
                            fin_norm(rd, 0, slot), (const int *)nullptr);
@@ -668,11 +877,15 @@ template <typename R>
 static int lincomb(hipStream_t s, i64 n, double are, double aim, const void *x, double bre, double bim,
                    const void *y, void *out, bbjd_red *rd, int slot)
 {
-    const int nt = ntiles_upd(n);
+    const int rows = vec_rows(n);
+    const int nt = ntiles(n, rows);
     const int fused = nt <= g_fuse_max_blocks;
-    hipLaunchKernelGGL((k_lincomb<R>), dim3(nt), dim3(TPB), 0, s, (i64)n, (R)are, (R)aim, (const Cx<R> *)x,
-                       (R)bre, (R)bim, (const Cx<R> *)y, (Cx<R> *)out, rd ? (R *)rd->pnorm : (R *)nullptr,
-                       rd ? rd->cnt + BBJD_CNT_VEC : (int *)nullptr, rd ? fin_norm(rd, 0, slot) : fin_none(), fused);
+#define BBJD_LAUNCH_LC(E)                                                                                          \
+    hipLaunchKernelGGL((k_lincomb<R, E>), dim3(nt), dim3(TPB), 0, s, (i64)n, (R)are, (R)aim, (const Cx<R> *)x,      \
+                       (R)bre, (R)bim, (const Cx<R> *)y, (Cx<R> *)out, rd ? (R *)rd->pnorm : (R *)nullptr,          \
+                       rd ? rd->cnt + BBJD_CNT_VEC : (int *)nullptr, rd ? fin_norm(rd, 0, slot) : fin_none(), fused)
+    BBJD_E_SWITCH(rows, BBJD_LAUNCH_LC)
+#undef BBJD_LAUNCH_LC
     if (rd && !fused)
         hipLaunchKernelGGL((k_norm_fin<R>), dim3(1), dim3(TPB), 0, s, (const R *)rd->pnorm, nt,
                            fin_norm(rd, 0, slot), (const int *)nullptr);
@@ -682,6 +895,16 @@ static int lincomb(hipStream_t s, i64 n, double are, double aim, const void *x, 
 extern "C" {
 
 void bbjd_set_fuse_blocks(int max_blocks) { g_fuse_max_blocks = max_blocks; }
+
+int bbjd_set_tiles(int proj_blocks, int upd_rows, int blocks_target)
+{
+    auto ok = [](int r) { return r == 0 || r == 64 || r == 128 || r == TPB || r == 2 * TPB || r == 4 * TPB; };
+    if (proj_blocks < 0 || !ok(upd_rows) || blocks_target < 0) return -2;
+    if (proj_blocks > 0) g_proj_blocks = proj_blocks;
+    g_upd_rows = upd_rows;
+    if (blocks_target > 0) g_blocks_target = blocks_target;
+    return 0;
+}
 
 void bbjd_ktime_enable(int on)
 {
@@ -795,6 +1018,29 @@ int bbjd_k_svdv(hipStream_t s, int64_t n, const void *u, void *z, const void *sv
     return rc | chk();
 }
 
+// G (p x q, column-major, ld ldg) = X^H Y; partials in rd->part.
+int bbjd_k_gram(hipStream_t s, int64_t n, int p, const void *X, int64_t ldx, int q, const void *Y, int64_t ldy,
+                void *G, int64_t ldg, bbjd_red *rd)
+{
+    if (p <= 0 || q <= 0) return 0;
+    if (n <= 0 || ldg < p) return -2;
+    const i64 pq = (i64)p * q;
+    const int tiles = ((p + GR_T - 1) / GR_T) * ((q + GR_T - 1) / GR_T);
+    i64 nch = (512 + tiles - 1) / tiles;
+    if (nch * pq > rd->part_cap) nch = rd->part_cap / pq;
+    if (nch < 1) return -2;
+    const i64 maxch = (n + GR_SR - 1) / GR_SR;
+    if (nch > maxch) nch = maxch;
+    i64 chunk = (n + nch - 1) / nch;
+    chunk = (chunk + GR_SR - 1) / GR_SR * GR_SR;
+    nch = (n + chunk - 1) / chunk;
+    hipLaunchKernelGGL(k_gram, dim3(tiles, (unsigned)nch), dim3(TPB), 0, s, (i64)n, chunk, p, q,
+                       (const Cx<double> *)X, (i64)ldx, (const Cx<double> *)Y, (i64)ldy, (Cx<double> *)rd->part);
+    hipLaunchKernelGGL(k_gram_finish, dim3((unsigned)((pq + TPB - 1) / TPB)), dim3(TPB), 0, s, (int)nch, p, q,
+                       (const Cx<double> *)rd->part, (Cx<double> *)G, (i64)ldg);
+    return chk();
+}
+
 int bbjd_k_rotate(hipStream_t s, int64_t n, int k, int keep, void *X, int64_t ldx, const void *Zr)
 {
     if (keep > k || keep <= 0) return -2;
@@ -815,16 +1061,6 @@ int bbjd_k_rotate(hipStream_t s, int64_t n, int k, int keep, void *X, int64_t ld
     return chk();
 }
 
-int bbjd_k_gram(hipStream_t s, int64_t n, int m, const void *X, const void *Y, int64_t ld, void *G, int64_t ldg,
-                bbjd_red *rd)
-{
-    int rc = 0;
-    for (int j = 0; j < m; j++)
-        rc |= proj_pass<double>(s, n, m, X, ld, (const Cx<double> *)Y + (i64)j * ld, nullptr,
-                                (Cx<double> *)G + (i64)j * ldg, nullptr, rd, -1, nullptr);
-    return rc;
-}
-
 int bbjd_k_z2c(hipStream_t s, int64_t n, const void *z, void *c)
 {
     hipLaunchKernelGGL(k_z2c, dim3(grid_stride_blocks(n)), dim3(TPB), 0, s, (i64)n, (const Cx<double> *)z,
@@ -842,12 +1078,19 @@ int bbjd_k_c2z(hipStream_t s, int64_t n, const void *c, void *z)
 // ---------------------------------------------------------------------------
 // Kernel benchmark: times one kernel in isolation with HIP events on a private
 // stream.  which: 0 k_proj one vector, 1 k_proj two vectors, 2 k_update,
-// 3 k_rotate (k -> k/2 columns), 4 full CGS2 (one pass + skipped pass 2).
+// 3 k_rotate (k -> k/2 columns), 4 full CGS2 (one pass + skipped pass 2),
+// 5 a plain copy of n complex values.
 // Returns the average ms per launch and the algorithmic bytes per launch.
 __global__ void k_fill(i64 cnt, Cx<double> *a, double scale)
 {
     for (i64 i = (i64)blockIdx.x * TPB + threadIdx.x; i < cnt; i += (i64)gridDim.x * TPB)
         a[i] = cx(scale * (double)((i * 2654435761LL) % 1000 - 500) * 1e-3, scale * (double)((i * 40503LL) % 997 - 498) * 1e-3);
+}
+
+// Plain streaming copy: the reference point for counter-based HBM traffic.
+__global__ void k_copy(i64 cnt, const Cx<double> *__restrict__ x, Cx<double> *__restrict__ y)
+{
+    for (i64 i = (i64)blockIdx.x * TPB + threadIdx.x; i < cnt; i += (i64)gridDim.x * TPB) y[i] = x[i];
 }
 
 int bbjd_kbench(int which, int64_t n, int k, int reps, double *avg_ms, double *bytes)
@@ -862,13 +1105,14 @@ int bbjd_kbench(int which, int64_t n, int k, int reps, double *avg_ms, double *b
     rc |= hipMalloc(&Q, (size_t)ld * k * 16) != hipSuccess;
     rc |= hipMalloc(&x, (size_t)ld * 16) != hipSuccess;
     rc |= hipMalloc(&x2, (size_t)ld * 16) != hipSuccess;
-    rc |= hipMalloc(&blk, (size_t)pe * 16 + (size_t)(ntiles_proj(n) + 64) * 8 + (size_t)(k + 8) * 16 * 5 + 8192) != hipSuccess;
+    const size_t npn = (size_t)ntiles(n, BBJD_TILE_MIN_ROWS) + 64;
+    rc |= hipMalloc(&blk, (size_t)pe * 16 + npn * 8 + (size_t)(k + 8) * 16 * 5 + 8192) != hipSuccess;
     rc |= hipMalloc(&Zr, (size_t)k * k * 16) != hipSuccess;
     if (rc) { hipFree(Q); hipFree(x); hipFree(x2); hipFree(blk); hipFree(Zr); hipStreamDestroy(s); return -1; }
     bbjd_red rd;
     char *b = (char *)blk;
     rd.part = b; b += (size_t)pe * 16;
-    rd.pnorm = (double *)b; b += (size_t)(ntiles_proj(n) + 64) * 8;
+    rd.pnorm = (double *)b; b += npn * 8;
     rd.h = b; b += (size_t)(k + 8) * 16;
     rd.h2 = b; b += (size_t)(k + 8) * 16;
     rd.hsum = b; b += (size_t)(k + 8) * 16;
@@ -885,33 +1129,25 @@ int bbjd_kbench(int which, int64_t n, int k, int reps, double *avg_ms, double *b
     hipEvent_t e0, e1;
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
-    const int nt = ntiles_proj(n);
     double by = 0;
     auto launch = [&]() {
         switch (which) {
-        case 0: {
-            dim3 g((k + 15) / 16, nt);
-            hipLaunchKernelGGL((k_proj<double, 1, 16>), g, dim3(TPB), 0, s, (i64)n, k, (const Cx<double> *)Q, ld,
-                               (const Cx<double> *)x, (const Cx<double> *)nullptr, (Cx<double> *)rd.part, k,
-                               rd.pnorm, 0, (const int *)nullptr, rd.cnt, (Cx<double> *)rd.h2,
-                               (Cx<double> *)nullptr, rd.scal, -1, 0);
-            by = 16.0 * n * (k + g.x);
+        case 0:
+            (void)proj_pass<double>(s, n, k, Q, ld, x, nullptr, rd.h2, nullptr, &rd, -1, nullptr);
+            by = 16.0 * n * (k + 1.0);
             break;
-        }
-        case 1: {
-            dim3 g((k + 7) / 8, nt);
-            hipLaunchKernelGGL((k_proj<double, 2, 8>), g, dim3(TPB), 0, s, (i64)n, k, (const Cx<double> *)Q, ld,
-                               (const Cx<double> *)x, (const Cx<double> *)x2, (Cx<double> *)rd.part, k, rd.pnorm,
-                               0, (const int *)nullptr, rd.cnt, (Cx<double> *)rd.h2, (Cx<double> *)rd.hsum,
-                               rd.scal, -1, 0);
-            by = 16.0 * n * (k + 2.0 * g.x);
+        case 1:
+            (void)proj_pass<double>(s, n, k, Q, ld, x, x2, rd.h2, rd.hsum, &rd, -1, nullptr);
+            by = 16.0 * n * (k + 2.0);
             break;
-        }
         case 2:
-            hipLaunchKernelGGL((k_update<double>), dim3(ntiles_upd(n)), dim3(TPB), (size_t)k * 16, s, (i64)n, k,
-                               (const Cx<double> *)Q, ld, (const Cx<double> *)rd.h, -1.0, 1, (Cx<double> *)x,
-                               rd.pnorm, (const int *)nullptr, rd.cnt + BBJD_CNT_UPD, fin_norm(&rd, 0, SL_T1), 0);
+            (void)update_pass<double>(s, n, k, Q, ld, rd.h, -1.0, 1, x, &rd, fin_norm(&rd, 0, SL_T1), nullptr);
             by = 16.0 * n * (k + 2);
+            break;
+        case 5:
+            hipLaunchKernelGGL(k_copy, dim3(2048), dim3(TPB), 0, s, (i64)n, (const Cx<double> *)x,
+                               (Cx<double> *)x2);
+            by = 32.0 * n;
             break;
         case 3:
             bbjd_k_rotate(s, n, k, k / 2 > 0 ? k / 2 : 1, Q, ld, Zr);

@@ -27,5 +27,8 @@ void scipy_zgetrf_(const blas_int *m, const blas_int *n, zcplx *a, const blas_in
                    blas_int *info);
 void scipy_zgetri_(const blas_int *n, zcplx *a, const blas_int *lda, const blas_int *ipiv, zcplx *work,
                    const blas_int *lwork, blas_int *info);
+void scipy_zpotrf_(const char *uplo, const blas_int *n, zcplx *a, const blas_int *lda, blas_int *info);
+void scipy_ztrtri_(const char *uplo, const char *diag, const blas_int *n, zcplx *a, const blas_int *lda,
+                   blas_int *info);
 void scipy_openblas_set_num_threads(int n);
 #endif

@@ -76,10 +76,11 @@ __global__ __launch_bounds__(TPB) void k_bj_apply(i64 n, int bs, const Cx<R> *__
 //
 // forward level: evens m = 2j at stride h, j < ne:
 //   b_i -= alpha_j b_{i-h} + gamma_j b_{i+h}
-template <int NR>
-__device__ __forceinline__ void bcr_fwd_row(int s, i64 M, i64 h, i64 j, int r, const Cx<double> *__restrict__ AG,
+template <int NR, int S>
+__device__ __forceinline__ void bcr_fwd_row(int s_rt, i64 M, i64 h, i64 j, int r, const Cx<double> *__restrict__ AG,
                                             Cx<double> *b, i64 vs)
 {
+    const int s = S > 0 ? S : s_rt;
     const i64 m = 2 * j, i = m * h, ss = (i64)s * s;
     Cx<double> acc[NR];
 #pragma unroll
@@ -87,6 +88,7 @@ __device__ __forceinline__ void bcr_fwd_row(int s, i64 M, i64 h, i64 j, int r, c
     if (m >= 1) {
         const Cx<double> *al = AG + j * 2 * ss + r;
         const i64 o = (i - h) * s;
+#pragma unroll
         for (int c = 0; c < s; c++) {
             const Cx<double> a = al[(i64)c * s];
 #pragma unroll
@@ -96,6 +98,7 @@ __device__ __forceinline__ void bcr_fwd_row(int s, i64 M, i64 h, i64 j, int r, c
     if (m + 1 < M) {
         const Cx<double> *ga = AG + j * 2 * ss + ss + r;
         const i64 o = (i + h) * s;
+#pragma unroll
         for (int c = 0; c < s; c++) {
             const Cx<double> g = ga[(i64)c * s];
 #pragma unroll
@@ -108,16 +111,18 @@ __device__ __forceinline__ void bcr_fwd_row(int s, i64 M, i64 h, i64 j, int r, c
 
 // backward level: odds m = 2j + 1, DPQ = (Dinv, Dinv L, Dinv U):
 //   x_i = Dinv b_i - P x_{i-h} - Q x_{i+h}
-template <int NR>
-__device__ __forceinline__ void bcr_bwd_row(int s, i64 M, i64 h, i64 j, int r, const Cx<double> *__restrict__ DPQ,
+template <int NR, int S>
+__device__ __forceinline__ void bcr_bwd_row(int s_rt, i64 M, i64 h, i64 j, int r, const Cx<double> *__restrict__ DPQ,
                                             const Cx<double> *b, Cx<double> *x, i64 vs)
 {
+    const int s = S > 0 ? S : s_rt;
     const i64 m = 2 * j + 1, i = m * h, ss = (i64)s * s;
     const Cx<double> *D = DPQ + j * 3 * ss + r, *P = D + ss, *Qm = P + ss;
     Cx<double> acc[NR];
 #pragma unroll
     for (int q = 0; q < NR; q++) acc[q] = cx0();
     const i64 ob = i * s, ol = (i - h) * s;
+#pragma unroll
     for (int c = 0; c < s; c++) {
         const Cx<double> dd = D[(i64)c * s], pp = P[(i64)c * s];
 #pragma unroll
@@ -125,6 +130,7 @@ __device__ __forceinline__ void bcr_bwd_row(int s, i64 M, i64 h, i64 j, int r, c
     }
     if (m + 1 < M) {
         const i64 orr = (i + h) * s;
+#pragma unroll
         for (int c = 0; c < s; c++) {
             const Cx<double> qq = Qm[(i64)c * s];
 #pragma unroll
@@ -135,49 +141,60 @@ __device__ __forceinline__ void bcr_bwd_row(int s, i64 M, i64 h, i64 j, int r, c
     for (int q = 0; q < NR; q++) x[q * vs + i * s + r] = acc[q];
 }
 
-template <int NR>
+// S: the super-block size when known at compile time (the row loops unroll
+// and every factor load of a row is in flight at once), 0 for any s.
+template <int NR, int S>
 __global__ __launch_bounds__(TPB) void k_bcr_fwd(int s, i64 M, i64 h, i64 ne, const Cx<double> *__restrict__ AG,
                                                  Cx<double> *b, i64 vs)
 {
+    const int sv = S > 0 ? S : s;
     const i64 gid = (i64)blockIdx.x * TPB + threadIdx.x;
-    const i64 j = gid / s;
+    const i64 j = gid / sv;
     if (j >= ne) return;
-    bcr_fwd_row<NR>(s, M, h, j, (int)(gid % s), AG, b, vs);
+    bcr_fwd_row<NR, S>(s, M, h, j, (int)(gid % sv), AG, b, vs);
 }
 
-template <int NR>
+template <int NR, int S>
 __global__ __launch_bounds__(TPB) void k_bcr_bwd(int s, i64 M, i64 h, i64 no, const Cx<double> *__restrict__ DPQ,
                                                  const Cx<double> *b, Cx<double> *x, i64 vs)
 {
+    const int sv = S > 0 ? S : s;
     const i64 gid = (i64)blockIdx.x * TPB + threadIdx.x;
-    const i64 j = gid / s;
+    const i64 j = gid / sv;
     if (j >= no) return;
-    bcr_bwd_row<NR>(s, M, h, j, (int)(gid % s), DPQ, b, x, vs);
+    bcr_bwd_row<NR, S>(s, M, h, j, (int)(gid % sv), DPQ, b, x, vs);
 }
 
 // The small levels l0..L-1, the final s x s solve and the backward levels
 // L-1..l0 in one workgroup of 1024 threads (every level here has at most
 // 1024 rows of work), separated by workgroup barriers.  off[2l] / off[2l+1]:
-// block offsets of level l's even / odd factor blocks, off[2L]: final block.
-template <int NR>
+// block offsets of level l's even / odd factor blocks, off[2L]: final block
+// (read once into LDS).
+template <int NR, int S>
 __global__ __launch_bounds__(1024) void k_bcr_tail(int s, i64 N, int l0, int L, const i64 *__restrict__ off,
                                                    const Cx<double> *__restrict__ F, Cx<double> *b, Cx<double> *x,
                                                    i64 vs)
 {
+    __shared__ i64 loff[2 * 64 + 1];
     const int t = threadIdx.x;
-    const i64 ss = (i64)s * s;
+    const int sv = S > 0 ? S : s;
+    const i64 ss = (i64)sv * sv;
+    if (t <= 2 * L) loff[t] = off[t];
+    __syncthreads();
     for (int l = l0; l < L; l++) {
         const i64 h = (i64)1 << l, M = (N + h - 1) >> l, ne = (M + 1) / 2;
-        for (i64 g = t; g < ne * s; g += 1024) bcr_fwd_row<NR>(s, M, h, g / s, (int)(g % s), F + off[2 * l] * ss, b, vs);
+        const Cx<double> *AG = F + loff[2 * l] * ss;
+        for (i64 g = t; g < ne * sv; g += 1024) bcr_fwd_row<NR, S>(s, M, h, g / sv, (int)(g % sv), AG, b, vs);
         __syncthreads();
     }
-    if (t < s) {
-        const Cx<double> *D = F + off[2 * L] * ss + t;
+    if (t < sv) {
+        const Cx<double> *D = F + loff[2 * L] * ss + t;
         Cx<double> acc[NR];
 #pragma unroll
         for (int q = 0; q < NR; q++) acc[q] = cx0();
-        for (int c = 0; c < s; c++) {
-            const Cx<double> dd = D[(i64)c * s];
+#pragma unroll
+        for (int c = 0; c < sv; c++) {
+            const Cx<double> dd = D[(i64)c * sv];
 #pragma unroll
             for (int q = 0; q < NR; q++) cfma(acc[q], dd, b[q * vs + c]);
         }
@@ -187,7 +204,8 @@ __global__ __launch_bounds__(1024) void k_bcr_tail(int s, i64 N, int l0, int L, 
     __syncthreads();
     for (int l = L - 1; l >= l0; l--) {
         const i64 h = (i64)1 << l, M = (N + h - 1) >> l, no = M / 2;
-        for (i64 g = t; g < no * s; g += 1024) bcr_bwd_row<NR>(s, M, h, g / s, (int)(g % s), F + off[2 * l + 1] * ss, b, x, vs);
+        const Cx<double> *DPQ = F + loff[2 * l + 1] * ss;
+        for (i64 g = t; g < no * sv; g += 1024) bcr_bwd_row<NR, S>(s, M, h, g / sv, (int)(g % sv), DPQ, b, x, vs);
         __syncthreads();
     }
 }
@@ -278,16 +296,34 @@ int bbjd_o_bj_apply(hipStream_t s, int64_t n, int bs, const void *inv, const voi
     return chk();
 }
 
+// Instantiation by (right-hand sides, super-block size): s = 4, 8, 16 are
+// compiled with fixed loops, other sizes take the generic S = 0 kernels.
+#define BBJD_BCR_DISPATCH(KERNEL_CALL)                      \
+    if (nrhs == 2) {                                        \
+        switch (s) {                                        \
+        case 4: KERNEL_CALL(2, 4); break;                   \
+        case 8: KERNEL_CALL(2, 8); break;                   \
+        case 16: KERNEL_CALL(2, 16); break;                 \
+        default: KERNEL_CALL(2, 0); break;                  \
+        }                                                   \
+    } else {                                                \
+        switch (s) {                                        \
+        case 4: KERNEL_CALL(1, 4); break;                   \
+        case 8: KERNEL_CALL(1, 8); break;                   \
+        case 16: KERNEL_CALL(1, 16); break;                 \
+        default: KERNEL_CALL(1, 0); break;                  \
+        }                                                   \
+    }
+
 int bbjd_o_bcr_fwd(hipStream_t st, int s, int64_t M, int64_t h, int64_t ne, const void *AG, void *b, int nrhs,
                    int64_t vs)
 {
     if (ne <= 0) return 0;
-    if (nrhs == 2)
-        hipLaunchKernelGGL(k_bcr_fwd<2>, dim3(blocks_for(ne * s)), dim3(TPB), 0, st, s, (i64)M, (i64)h, (i64)ne,
-                           (const Cx<double> *)AG, (Cx<double> *)b, (i64)vs);
-    else
-        hipLaunchKernelGGL(k_bcr_fwd<1>, dim3(blocks_for(ne * s)), dim3(TPB), 0, st, s, (i64)M, (i64)h, (i64)ne,
-                           (const Cx<double> *)AG, (Cx<double> *)b, (i64)vs);
+#define BBJD_FWD(NR, S)                                                                                         \
+    hipLaunchKernelGGL((k_bcr_fwd<NR, S>), dim3(blocks_for(ne * s)), dim3(TPB), 0, st, s, (i64)M, (i64)h, (i64)ne, \
+                       (const Cx<double> *)AG, (Cx<double> *)b, (i64)vs)
+    BBJD_BCR_DISPATCH(BBJD_FWD)
+#undef BBJD_FWD
     return chk();
 }
 
@@ -295,25 +331,23 @@ int bbjd_o_bcr_bwd(hipStream_t st, int s, int64_t M, int64_t h, int64_t no, cons
                    int nrhs, int64_t vs)
 {
     if (no <= 0) return 0;
-    if (nrhs == 2)
-        hipLaunchKernelGGL(k_bcr_bwd<2>, dim3(blocks_for(no * s)), dim3(TPB), 0, st, s, (i64)M, (i64)h, (i64)no,
-                           (const Cx<double> *)DPQ, (const Cx<double> *)b, (Cx<double> *)x, (i64)vs);
-    else
-        hipLaunchKernelGGL(k_bcr_bwd<1>, dim3(blocks_for(no * s)), dim3(TPB), 0, st, s, (i64)M, (i64)h, (i64)no,
-                           (const Cx<double> *)DPQ, (const Cx<double> *)b, (Cx<double> *)x, (i64)vs);
+#define BBJD_BWD(NR, S)                                                                                         \
+    hipLaunchKernelGGL((k_bcr_bwd<NR, S>), dim3(blocks_for(no * s)), dim3(TPB), 0, st, s, (i64)M, (i64)h, (i64)no, \
+                       (const Cx<double> *)DPQ, (const Cx<double> *)b, (Cx<double> *)x, (i64)vs)
+    BBJD_BCR_DISPATCH(BBJD_BWD)
+#undef BBJD_BWD
     return chk();
 }
 
 int bbjd_o_bcr_tail(hipStream_t st, int s, int64_t N, int l0, int L, const int64_t *off, const void *F, void *b,
                     void *x, int nrhs, int64_t vs)
 {
-    if (s > 1024) return -2;
-    if (nrhs == 2)
-        hipLaunchKernelGGL(k_bcr_tail<2>, dim3(1), dim3(1024), 0, st, s, (i64)N, l0, L, (const i64 *)off,
-                           (const Cx<double> *)F, (Cx<double> *)b, (Cx<double> *)x, (i64)vs);
-    else
-        hipLaunchKernelGGL(k_bcr_tail<1>, dim3(1), dim3(1024), 0, st, s, (i64)N, l0, L, (const i64 *)off,
-                           (const Cx<double> *)F, (Cx<double> *)b, (Cx<double> *)x, (i64)vs);
+    if (s > 1024 || L > 64) return -2;
+#define BBJD_TAIL(NR, S)                                                                                        \
+    hipLaunchKernelGGL((k_bcr_tail<NR, S>), dim3(1), dim3(1024), 0, st, s, (i64)N, l0, L, (const i64 *)off,       \
+                       (const Cx<double> *)F, (Cx<double> *)b, (Cx<double> *)x, (i64)vs)
+    BBJD_BCR_DISPATCH(BBJD_TAIL)
+#undef BBJD_TAIL
     return chk();
 }
 

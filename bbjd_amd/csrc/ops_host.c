@@ -331,7 +331,7 @@ static size_t op_layout(const bbjd_pencil_desc *d, int pc, int want_c64, bbjd_in
     }
     sz[D_SMALL] = (size_t)(4 * d->ntau + 16) * 16;
     sz[D_PART] = (size_t)bbjd_red_part_entries(n, d->ntau > 0 ? d->ntau + 2 : 2) * 16;
-    sz[D_PNORM] = ((size_t)(n + BBJD_PROJ_ROWS - 1) / BBJD_PROJ_ROWS + 64) * 8;
+    sz[D_PNORM] = ((size_t)(n + BBJD_TILE_MIN_ROWS - 1) / BBJD_TILE_MIN_ROWS + 64) * 8;
     sz[D_RSC] = 512 + BBJD_CNT_SLOTS * 4 + 1024;
     if (want_c64 && pc != BBJD_PC_NONE) sz[D_ZT1] = sz[D_ZT2] = (size_t)n * 16;
     size_t tot = 0;

@@ -15,14 +15,18 @@ in HBM before the timed region; the timed region is the serving loop.
   plain        the same solves one at a time (FP64 engine, no overlap), on a
                seeded sample of the requests -> eigenpairs/s
   latency      p50/p99 of completion - arrival over the timed requests
-  roofline     the projection kernel Q^H x (the dominant streaming kernel),
-               HIP-event durations of every launch in the timed region vs
-               its algorithmic bytes 16 n (k + column tiles), HBM peak 8 TB/s
+  roofline     the projection kernel Q^H x (the dominant streaming kernel):
+               HIP-event durations of every launch of the plain leg (one
+               stream, so a launch's duration is its own) vs its algorithmic
+               bytes 16 n (k + 1), HBM peak 8 TB/s; the same kernel's rate
+               inside the served region (launches time-share the GPU there)
+               is reported beside it
   cpu_baseline the reference solver compiled from its sources (oracle/_ref;
                else the oracle restatement) on the host cores, on a bounded
                sample of the workload's requests, CPU exact banded LU
 
 Usage: python bench.py [--gpus 1] [--steps 1] [--warmup 1] [--nreq 256]
+       python bench.py --plain-only ...   (the plain leg alone: for profilers)
 One JSON line on stdout; progress on stderr.
 """
 from __future__ import annotations
@@ -112,6 +116,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--plain-only", action="store_true", help="skip the served steps (profiling the plain leg)")
     ap.add_argument("--seed", type=int, default=20261203)
     args = ap.parse_args()
 
@@ -149,14 +154,14 @@ def main():
     # ---- warmup (untimed): short streams ----
     for w in range(args.warmup):
         idx = list(range(min(16, len(reqs))))
-        srv.serve(jobs_for(idx, False), workers=args.workers, realtime=False)
+        srv.serve(jobs_for(idx, False), workers=1 if args.plain_only else args.workers, realtime=False)
         log(f"[bench] warmup step {w} done")
 
     # ---- timed: served ----
     all_idx = list(range(len(reqs)))
     served, walls = [], []
     srv.ktime_enable(True)
-    for s in range(args.steps):
+    for s in range(0 if args.plain_only else args.steps):
         t = time.perf_counter()
         res, rep = srv.serve(jobs_for(all_idx), workers=args.workers, realtime=True)
         walls.append(time.perf_counter() - t)
@@ -164,7 +169,7 @@ def main():
         log(f"[bench] step {s}: {rep['pairs']} pairs in {walls[-1]:.1f}s, workers {rep['workers']}, "
             f"peak concurrency {rep['peak_concurrency']}")
     srv.ktime_enable(False)
-    kcnt, kms, kbytes = srv.ktime_harvest()
+    s_cnt, s_ms, s_bytes = srv.ktime_harvest()
 
     pairs = sum(rep["pairs"] for _, rep in served)
     wall = sum(walls)
@@ -188,24 +193,38 @@ def main():
     # ---- plain: one solve at a time, FP64, seeded sample ----
     rng = np.random.default_rng(args.seed + 1)
     sample = sorted(rng.choice(len(reqs), size=min(args.plain_sample, len(reqs)), replace=False).tolist())
+    srv.ktime_enable(True)
     t = time.perf_counter()
     pres, prep = srv.serve(jobs_for(sample, False), workers=1, realtime=False)
     t_plain = time.perf_counter() - t
+    srv.ktime_enable(False)
+    kcnt, kms, kbytes = srv.ktime_harvest()
     plain_pairs = prep["pairs"]
     for j, i in enumerate(sample):
         r = pres[j]
         log(f"[bench]   plain n={reqs[i]['n']} k={reqs[i]['k']}: {r['nconv']} pairs, {r['iters']} iters, "
             f"{r['t_done'] - r['t_start']:.2f}s (qz {r['qz_seconds']:.2f}s), status {r['status']}")
     plain_rate = plain_pairs / t_plain if t_plain else None
-    # same eigenvalues alone and in the mix
-    mix = served[0][0]
-    diff = 0.0
-    for j, i in enumerate(sample):
-        a, b = np.sort_complex(pres[j]["eigenvalues"]), np.sort_complex(mix[i]["eigenvalues"])
-        if a.size and a.size == b.size:
-            diff = max(diff, float(np.max(np.abs(a - b) / np.abs(a))))
+    diff = None
+    if served:
+        # same eigenvalues alone and in the mix
+        mix = served[0][0]
+        diff = 0.0
+        for j, i in enumerate(sample):
+            a, b = np.sort_complex(pres[j]["eigenvalues"]), np.sort_complex(mix[i]["eigenvalues"])
+            if a.size and a.size == b.size:
+                diff = max(diff, float(np.max(np.abs(a - b) / np.abs(a))))
+        # where served time goes: the sample's solve and QZ seconds alone vs in
+        # the mix, and the mean number of solves in flight
+        s_alone = sum(pres[j]["t_done"] - pres[j]["t_start"] for j in range(len(sample)))
+        s_mix = sum(mix[i]["t_done"] - mix[i]["t_start"] for i in sample)
+        q_alone = sum(pres[j]["qz_seconds"] for j in range(len(sample)))
+        q_mix = sum(mix[i]["qz_seconds"] for i in sample)
+        busy = sum(r["t_done"] - r["t_start"] for r in mix)
+        log(f"[bench] sample solve seconds alone {s_alone:.1f} / in the mix {s_mix:.1f} "
+            f"(qz {q_alone:.1f} / {q_mix:.1f}); mean solves in flight {busy / walls[0]:.1f}")
     log(f"[bench] plain sample: {plain_pairs} pairs / {t_plain:.1f}s; "
-        f"max rel eig diff plain vs served {diff:.1e}")
+        f"max rel eig diff plain vs served {diff}")
 
     served_rate = pairs / wall if wall else None
     roof = None
@@ -213,11 +232,14 @@ def main():
         achieved = kbytes / (kms * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_proj<double,1,16> (Q^H x, complex128)", "launches": kcnt,
-                "avg_us": round(kms * 1e3 / kcnt, 2), "avg_bytes": round(kbytes / kcnt)}
+                "kernel": "k_proj<double,1,16> (Q^H x, complex128)", "measured_on": "plain leg",
+                "launches": kcnt, "avg_us": round(kms * 1e3 / kcnt, 2), "avg_bytes": round(kbytes / kcnt)}
+        if s_cnt:
+            roof["served_region_GBs"] = round(s_bytes / (s_ms * 1e-3) / 1e9, 1)
+            roof["served_region_avg_us"] = round(s_ms * 1e3 / s_cnt, 2)
 
     cpu = None
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and not args.plain_only:
         thr = args.cpu_threads or min(16, os.cpu_count() or 1)
         try:
             cpu = cpu_baseline(reqs, args.cpu_seconds, thr)
@@ -232,7 +254,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(1e3 * wall / args.steps, 1),
+        "ms_per_step": round(1e3 * wall / args.steps, 1) if served else None,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,

@@ -96,6 +96,8 @@ typedef struct {
     double   solve_seconds;     /* wall time of bbjd_solve                */
     double   stream_bytes;      /* algorithmic HBM bytes of the n-sized
                                    engine kernels (operators excluded)    */
+    bbjd_int restart_fallbacks; /* restarts whose Cholesky QR of the test
+                                   space fell back to column-wise CGS2    */
 } bbjd_stats;
 
 int bbjd_workspace_stats(const bbjd_workspace *ws, bbjd_stats *st);
@@ -109,20 +111,37 @@ int bbjd_device_init(int sync_yield);
 /* Launch timing of the engine's dominant streaming kernel (the one-vector
  * projection Q^H x): enable, run, then harvest the number of timed launches,
  * the sum of their durations (ms, HIP events on the launching stream) and of
- * their algorithmic bytes (16 n (k + column tiles)). */
+ * their algorithmic bytes (16 n (k + 1): Q and x read once). */
 void bbjd_ktime_enable(int on);
 int  bbjd_ktime_harvest(double *total_ms, double *total_bytes);
+
+/* G (p x q, column-major) = X^H Y for host arrays X (n x p) and Y (n x q),
+ * column-major with leading dimension n, through the engine's device Gram
+ * kernel (the restart's test-space QR and projected pencil).  No
+ * counterpart in the reference; a utility for host callers and tests.
+ * Returns 0, or -1 on invalid sizes or a device error. */
+int bbjd_block_gram(bbjd_int n, bbjd_int p, const bbjd_zomplex *X, bbjd_int q, const bbjd_zomplex *Y,
+                    bbjd_zomplex *G);
 
 /* Kernel benchmark (tile autotuning and roofline checks): times one engine
  * kernel in isolation at shape (n, k) on a private stream, reps launches.
  * which: 0 projection Q^H x, 1 projection of two vectors, 2 update
- * x -= Q h, 3 in-place basis rotation k -> k/2, 4 full CGS2.  Writes the
- * average ms per launch and the algorithmic bytes per launch. */
+ * x -= Q h, 3 in-place basis rotation k -> k/2, 4 full CGS2, 5 a plain copy
+ * of n complex values (the reference for counter-measured traffic).  Writes
+ * the average ms per launch and the algorithmic bytes per launch. */
 int bbjd_kbench(int which, int64_t n, int k, int reps, double *avg_ms, double *bytes);
 
 /* Grid size (blocks) up to which a reduction's finishing step runs in the
  * last block of the producing kernel instead of a second launch. */
 void bbjd_set_fuse_blocks(int max_blocks);
+
+/* Tiles of the streaming kernels, for tuning.  proj_blocks: blocks per
+ * projection launch (rows are split into chunks to give about that many;
+ * 0 keeps the current value).  upd_rows: rows per block of the update / norm
+ * kernels, 256, 512 or 1024, or 0 to choose per launch from the shape (the
+ * largest tile that still gives blocks_target blocks; blocks_target 0 keeps
+ * the current target).  Returns -2 on an unsupported value. */
+int bbjd_set_tiles(int proj_blocks, int upd_rows, int blocks_target);
 
 /* Device memory helpers for callers without another GPU runtime (tests,
  * tools): allocate / free / synchronise the device. */

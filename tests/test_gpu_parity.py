@@ -49,7 +49,35 @@ def test_cgs2_matches_oracle(n, k):
         assert np.linalg.norm(Q.conj().T @ vg) <= 1e-12 * scale
 
 
-SHAPES = [(2000, 4, 1, 2), (4096, 8, 2, 16), (1000, 8, 1, 4), (640, 4, 2, 0)]
+@pytest.mark.parametrize("proj_blocks,rows", [(8, 64), (256, 128), (1024, 256), (6000, 512), (100, 1024)])
+def test_cgs2_every_row_tile(proj_blocks, rows):
+    """each row tile / chunking of the streaming kernels (forced), fused and
+    two-launch finishing, against the oracle; ragged last tiles"""
+    lib = _lib.load()
+    olib = orc.load_oracle()
+    try:
+        for n, k, fuse in ((40000 + 77, 33, 48), (3000 + 5, 9, 48), (3000 + 5, 9, 0)):
+            assert lib.bbjd_set_tiles(proj_blocks, rows, 0) == 0
+            lib.bbjd_set_fuse_blocks(fuse)
+            rng = np.random.default_rng(n + k + rows)
+            Q = _orthonormal(n, k, n + 1)
+            x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+            vg, vo = x.copy(), x.copy()
+            hg = np.zeros(k, np.complex128)
+            ho = np.zeros(k, np.complex128)
+            ng = lib.bbjd_block_orthogonalize(n, k, Q.ctypes.data, vg.ctypes.data, hg.ctypes.data)
+            no = olib.orc_block_orthogonalize(n, k, Q.ctypes.data, vo.ctypes.data, ho.ctypes.data)
+            scale = np.linalg.norm(x)
+            assert np.linalg.norm(vg - vo) <= 1e-12 * scale, (n, k, fuse)
+            assert np.linalg.norm(hg - ho) <= 1e-12 * scale, (n, k, fuse)
+            assert abs(ng - no) <= 1e-12 * scale, (n, k, fuse)
+    finally:
+        lib.bbjd_set_tiles(192, 0, 2048)
+        lib.bbjd_set_fuse_blocks(48)
+
+
+# super-block sizes hw * bs = 4, 16, 8, 8 (compiled loops) and 12 (generic)
+SHAPES = [(2000, 4, 1, 2), (4096, 8, 2, 16), (1000, 8, 1, 4), (640, 4, 2, 0), (1206, 6, 2, 3)]
 
 
 @pytest.mark.parametrize("n,bs,hw,ntau", SHAPES)
@@ -197,3 +225,47 @@ def test_thick_restart_small_maxsub():
                        preconditioner=dev, return_info=True)
     assert info["nconv"] == ref["nconv"] == 6
     assert rel_match(ev, ref["evals"]) <= 1e-8
+
+
+@pytest.mark.parametrize("space", ["device", "host"])
+def test_search_space_exhaustion_breakdown(space):
+    """tol below rounding: the search space fills C^n, the next expansion and
+    its random restart vector both vanish under CGS2 and the solve ends with
+    BBJD_ERR_BREAKDOWN at the same iteration as the oracle (reference
+    src/jd_core.c:1032-1058).  The device path defers the breakdown test to
+    the pencil update and recomputes the column along this branch."""
+    for n, k, ms, mn, nt in ((16, 6, 30, 14, 2), (8, 3, 12, 4, 0)):
+        p = BandedTauPencil.generate(n, 4, 1, nt, seed=3)
+        sigma = 0.3 + 0.1j
+        ops = orc.CpuPencilOps(p, sigma, 1)
+        ref = orc.solve("oracle", n, k, sigma, tol=1e-17, maxsub=ms, minsub=mn, maxiter=300, ops=ops)
+        assert ref["status"] == -5 and ref["nconv"] == 0
+        if space == "device":
+            dev = DeviceOperator(p, sigma, pc="exact")
+            out = jdqz_c(n, k, sigma, 0, None, None, dev, tol=1e-17, maxsub=ms, minsub=mn, maxiter=300,
+                         device_op=dev)
+        else:
+            out = jdqz_c(n, k, sigma, 0, lambda x: ops.apply("orc_matvec_A", x),
+                         lambda x: ops.apply("orc_matvec_B", x), lambda x: ops.apply("orc_precond", x),
+                         tol=1e-17, maxsub=ms, minsub=mn, maxiter=300)
+        ev, _, res, nconv, iters, status = out
+        assert status == -5 and nconv == 0, (n, status, nconv)
+        assert iters == ref["iters"], (n, iters, ref["iters"])
+
+
+@pytest.mark.parametrize("n,p,q", [(5000, 64, 64), (70001, 17, 40), (300, 130, 7), (33, 1, 1)])
+def test_block_gram_matches_numpy(n, p, q):
+    """the restart's device Gram X^H Y (ragged tiles, row chunks) against
+    numpy; deterministic across calls"""
+    lib = _lib.load()
+    rng = np.random.default_rng(n + p + q)
+    X = np.asfortranarray(rng.standard_normal((n, p)) + 1j * rng.standard_normal((n, p)))
+    Y = np.asfortranarray(rng.standard_normal((n, q)) + 1j * rng.standard_normal((n, q)))
+    G = np.zeros((p, q), np.complex128, order="F")
+    assert lib.bbjd_block_gram(n, p, X.ctypes.data, q, Y.ctypes.data, G.ctypes.data) == 0
+    ref = X.conj().T @ Y
+    scale = np.linalg.norm(X, axis=0)[:, None] * np.linalg.norm(Y, axis=0)[None, :]
+    assert np.max(np.abs(G - ref) / scale) <= 1e-14
+    G2 = np.zeros_like(G)
+    assert lib.bbjd_block_gram(n, p, X.ctypes.data, q, Y.ctypes.data, G2.ctypes.data) == 0
+    assert np.array_equal(G, G2)
