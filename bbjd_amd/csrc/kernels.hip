@@ -77,8 +77,11 @@ __device__ __forceinline__ void wave_transpose_reduce(R (&v)[1 << LS])
         const bool up = (lane & m) != 0;
 #pragma unroll
         for (int i = 0; i < h; i++) {
-            R send = up ? v[i] : v[i + h];
-            R keep = up ? v[i + h] : v[i];
+            // select between values, not addresses: a select of two array
+            // elements can become a dynamically indexed (scratch) array
+            const R lo = v[i], hi = v[i + h];
+            const R send = up ? lo : hi;
+            const R keep = up ? hi : lo;
             v[i] = keep + __shfl_xor(send, m, 64);
         }
     }
@@ -207,14 +210,17 @@ __device__ __forceinline__ void proj_finish(int c0, int ntiles, int k, const Cx<
     }
 }
 
-template <typename R, int NV, int C>
+// COND: the launch is the conditional second CGS pass (a separate
+// instantiation, so profiles tell the passes that ran from those that
+// returned at once).
+template <typename R, int NV, int C, bool COND>
 __global__ __launch_bounds__(TPB) void k_proj(i64 n, i64 chunk, int k, const Cx<R> *__restrict__ Q, i64 ldq,
                                               const Cx<R> *__restrict__ x0, const Cx<R> *__restrict__ x1,
                                               Cx<R> *part, int ldp, R *pnorm, int want_norm,
                                               const int *__restrict__ cond, int *cnt, Cx<R> *out0, Cx<R> *out1,
                                               double *scal, int norm_slot, int fused)
 {
-    if (cond && cond[0] == 0) return;
+    if (COND && cond[0] == 0) return;
     constexpr int NVAL = 2 * NV * C;
     static_assert(NVAL == 32 || NVAL == 16, "butterfly width");
     constexpr int LS = NVAL == 32 ? 5 : 4;
@@ -234,39 +240,27 @@ __global__ __launch_bounds__(TPB) void k_proj(i64 n, i64 chunk, int k, const Cx<
 #pragma unroll
     for (int i = 0; i < NVAL; i++) acc[i] = 0;
     R nv = 0;
-    if (c0 + C <= k) {
+    // A partial last column tile reads its last column again in place of
+    // the missing ones (cache hits; their sums are discarded), so every tile
+    // runs the same unrolled loop.
+    const int ncol = k - c0 < C ? k - c0 : C;
+    const Cx<R> *qp[C];
+#pragma unroll
+    for (int c = 0; c < C; c++) qp[c] = q0 + (i64)(c < ncol ? c : ncol - 1) * ldq;
 #pragma unroll 2
-        for (i64 r = (i64)tile * chunk + t; r < r1; r += TPB) {
-            Cx<R> xv[NV];
-            xv[0] = x0[r];
-            if (NV == 2) xv[NV - 1] = x1[r];
-            Cx<R> qv[C];
+    for (i64 r = (i64)tile * chunk + t; r < r1; r += TPB) {
+        Cx<R> xv[NV];
+        xv[0] = x0[r];
+        if (NV == 2) xv[NV - 1] = x1[r];
+        Cx<R> qv[C];
 #pragma unroll
-            for (int c = 0; c < C; c++) qv[c] = q0[(i64)c * ldq + r];
-            if (do_norm) nv += xv[0].re * xv[0].re + xv[0].im * xv[0].im;
+        for (int c = 0; c < C; c++) qv[c] = qp[c][r];
+        if (do_norm) nv += xv[0].re * xv[0].re + xv[0].im * xv[0].im;
 #pragma unroll
-            for (int c = 0; c < C; c++)
+        for (int c = 0; c < C; c++)
 #pragma unroll
-                for (int vi = 0; vi < NV; vi++)
-                    cfma_conj(acc[2 * (vi * C + c)], acc[2 * (vi * C + c) + 1], qv[c], xv[vi]);
-        }
-    } else {
-        const int ncol = k - c0;
-        for (i64 r = (i64)tile * chunk + t; r < r1; r += TPB) {
-            Cx<R> xv[NV];
-            xv[0] = x0[r];
-            if (NV == 2) xv[NV - 1] = x1[r];
-            if (do_norm) nv += xv[0].re * xv[0].re + xv[0].im * xv[0].im;
-#pragma unroll
-            for (int c = 0; c < C; c++) {
-                if (c < ncol) {
-                    const Cx<R> qq = q0[(i64)c * ldq + r];
-#pragma unroll
-                    for (int vi = 0; vi < NV; vi++)
-                        cfma_conj(acc[2 * (vi * C + c)], acc[2 * (vi * C + c) + 1], qq, xv[vi]);
-                }
-            }
-        }
+            for (int vi = 0; vi < NV; vi++)
+                cfma_conj(acc[2 * (vi * C + c)], acc[2 * (vi * C + c) + 1], qv[c], xv[vi]);
     }
     wave_transpose_reduce<R, LS>(acc);
     const int lane = t & 63, w = t >> 6;
@@ -367,14 +361,14 @@ __global__ __launch_bounds__(TPB) void k_update(i64 n, int k, const Cx<R> *__res
 // rows, and the four partial sums are added in a fixed order through LDS.
 // Four times the blocks of k_update for the same rows, so more loads are in
 // flight on shapes whose rows alone do not fill the GPU.
-template <typename R, int RW>
+template <typename R, int RW, bool COND>
 __global__ __launch_bounds__(TPB) void k_update_split(i64 n, int k, const Cx<R> *__restrict__ Q, i64 ldq,
                                                       const Cx<R> *__restrict__ h, R alpha, int beta_one,
                                                       Cx<R> *__restrict__ x, R *pnorm,
                                                       const int *__restrict__ cond, int *cnt, NormFin fin,
                                                       int fused)
 {
-    if (cond && cond[0] == 0) return;
+    if (COND && cond[0] == 0) return;
     extern __shared__ char smem_raw[];
     Cx<R> *sh = reinterpret_cast<Cx<R> *>(smem_raw);
     __shared__ Cx<R> red[4][64 * RW];
@@ -759,9 +753,14 @@ static int proj_pass_nv(hipStream_t s, i64 n, int k, const void *Q, i64 ldq, con
         ea = kt_event(); eb = kt_event();
         (void)hipEventRecord(ea, s);
     }
-    hipLaunchKernelGGL((k_proj<R, NV, C>), g, dim3(TPB), 0, s, (i64)n, chunk, k, (const Cx<R> *)Q, (i64)ldq,
-                       (const Cx<R> *)x0, (const Cx<R> *)x1, (Cx<R> *)rd->part, ldp, (R *)rd->pnorm, want_norm, cond,
-                       rd->cnt, (Cx<R> *)out0, (Cx<R> *)out1, rd->scal, norm_slot, fused);
+    if (cond)
+        hipLaunchKernelGGL((k_proj<R, NV, C, true>), g, dim3(TPB), 0, s, (i64)n, chunk, k, (const Cx<R> *)Q,
+                           (i64)ldq, (const Cx<R> *)x0, (const Cx<R> *)x1, (Cx<R> *)rd->part, ldp, (R *)rd->pnorm,
+                           want_norm, cond, rd->cnt, (Cx<R> *)out0, (Cx<R> *)out1, rd->scal, norm_slot, fused);
+    else
+        hipLaunchKernelGGL((k_proj<R, NV, C, false>), g, dim3(TPB), 0, s, (i64)n, chunk, k, (const Cx<R> *)Q,
+                           (i64)ldq, (const Cx<R> *)x0, (const Cx<R> *)x1, (Cx<R> *)rd->part, ldp, (R *)rd->pnorm,
+                           want_norm, cond, rd->cnt, (Cx<R> *)out0, (Cx<R> *)out1, rd->scal, norm_slot, fused);
     if (timed) {
         (void)hipEventRecord(eb, s);
         const double bytes = (double)sizeof(Cx<R>) * (double)n * ((double)k + 1.0);
@@ -793,14 +792,18 @@ static int update_pass(hipStream_t s, i64 n, int k, const void *Q, i64 ldq, cons
     const size_t lds = (size_t)(k > 0 ? k : 1) * sizeof(Cx<R>);
     const int fused = nt <= g_fuse_max_blocks;
     if (rows < TPB) {
-        if (rows == 128)
-            hipLaunchKernelGGL((k_update_split<R, 2>), dim3(nt), dim3(TPB), lds, s, (i64)n, k, (const Cx<R> *)Q,
-                               (i64)ldq, (const Cx<R> *)h, alpha, beta_one, (Cx<R> *)x, (R *)rd->pnorm, cond,
-                               rd->cnt + BBJD_CNT_UPD, fin, fused);
-        else
-            hipLaunchKernelGGL((k_update_split<R, 1>), dim3(nt), dim3(TPB), lds, s, (i64)n, k, (const Cx<R> *)Q,
-                               (i64)ldq, (const Cx<R> *)h, alpha, beta_one, (Cx<R> *)x, (R *)rd->pnorm, cond,
-                               rd->cnt + BBJD_CNT_UPD, fin, fused);
+#define BBJD_LAUNCH_SPLIT(RW, CO)                                                                                  \
+    hipLaunchKernelGGL((k_update_split<R, RW, CO>), dim3(nt), dim3(TPB), lds, s, (i64)n, k, (const Cx<R> *)Q,        \
+                       (i64)ldq, (const Cx<R> *)h, alpha, beta_one, (Cx<R> *)x, (R *)rd->pnorm, cond,                \
+                       rd->cnt + BBJD_CNT_UPD, fin, fused)
+        if (rows == 128) {
+            if (cond) BBJD_LAUNCH_SPLIT(2, true);
+            else BBJD_LAUNCH_SPLIT(2, false);
+        } else {
+            if (cond) BBJD_LAUNCH_SPLIT(1, true);
+            else BBJD_LAUNCH_SPLIT(1, false);
+        }
+#undef BBJD_LAUNCH_SPLIT
         if (!fused && fin.mode >= 0)
             hipLaunchKernelGGL((k_norm_fin<R>), dim3(1), dim3(TPB), 0, s, (const R *)rd->pnorm, nt, fin, cond);
         return chk();

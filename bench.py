@@ -232,7 +232,7 @@ def main():
         achieved = kbytes / (kms * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_proj<double,1,16> (Q^H x, complex128)", "measured_on": "plain leg",
+                "kernel": "k_proj<double,1,16,false> (Q^H x, complex128)", "measured_on": "plain leg",
                 "launches": kcnt, "avg_us": round(kms * 1e3 / kcnt, 2), "avg_bytes": round(kbytes / kcnt)}
         if s_cnt:
             roof["served_region_GBs"] = round(s_bytes / (s_ms * 1e-3) / 1e9, 1)
