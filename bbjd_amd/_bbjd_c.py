@@ -13,7 +13,9 @@ copies n elements unchecked, src/_bbjd_c.c:68).
 
 Extension (keyword-only): ``device_op`` -- a ``DeviceOperator`` whose
 device-pointer callbacks replace the three Python callables (no host round
-trip per apply); ``precision`` -- 'fp64' or 'mixed' (complex64 inner GMRES).
+trip per apply); ``precision`` -- 'fp64' or 'mixed' (complex64 inner GMRES);
+``qz_mode`` -- 'ritz' (ordered Schur form only where a restart may follow)
+or 'schur' (every iteration, the reference's sequence).
 """
 from __future__ import annotations
 
@@ -51,7 +53,7 @@ def _trampoline(fn, errbox):
 
 def jdqz_c(n, nev, sigma, target, matvec_A, matvec_B, precond_solve, tol=1e-10, maxiter=200,
            maxsub=0, minsub=0, inner_maxiter=0, inner_tol=1e-2, verbose=0, v0=None, *,
-           device_op=None, precision="fp64", return_stats=False):
+           device_op=None, precision="fp64", return_stats=False, qz_mode="ritz"):
     lib = _lib.load()
     n, nev = int(n), int(nev)
     p = _lib.Params()
@@ -70,6 +72,9 @@ def jdqz_c(n, nev, sigma, target, matvec_A, matvec_B, precond_solve, tol=1e-10, 
     if precision not in ("fp64", "mixed"):
         raise ValueError("precision must be 'fp64' or 'mixed'")
     opts.precision = _lib.PREC_MIXED if precision == "mixed" else _lib.PREC_FP64
+    if qz_mode not in ("ritz", "schur"):
+        raise ValueError("qz_mode must be 'ritz' or 'schur'")
+    opts.qz_mode = _lib.QZ_SCHUR if qz_mode == "schur" else _lib.QZ_RITZ
     errbox: list = []
     keep = []
     if device_op is not None:

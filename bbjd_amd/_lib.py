@@ -38,10 +38,13 @@ class Result(C.Structure):
                 ("residuals", C.c_void_p), ("iters", C.c_int64), ("status", C.c_int)]
 
 
+QZ_RITZ, QZ_SCHUR = 0, 1
+
+
 class GpuOptions(C.Structure):
     _fields_ = [("callback_space", C.c_int), ("precision", C.c_int), ("stream", C.c_void_p),
                 ("matvec_A_c64", C.c_void_p), ("matvec_B_c64", C.c_void_p),
-                ("precond_c64", C.c_void_p), ("precond2", C.c_void_p)]
+                ("precond_c64", C.c_void_p), ("precond2", C.c_void_p), ("qz_mode", C.c_int)]
 
 
 class Stats(C.Structure):

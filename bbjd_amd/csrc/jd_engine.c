@@ -898,21 +898,38 @@ static bbjd_status solve_impl(const bbjd_params *P, bbjd_workspace *ws, bbjd_res
         bbjd_int p = locked;
         d->st.outer_iters++;
         d->st.sum_k += k;
-        /* ---- step 1: projected QZ on the host ---- */
+        /* ---- steps 1-2: projected QZ on the host, Ritz coefficients ---- */
+        /* the ordered Schur form where a thick restart may follow (k >= ms),
+         * else the eigenvalues and the target Ritz vector only */
+        const int schur = d->opt.qz_mode == BBJD_QZ_SCHUR || k >= ms;
+        zcx ta, tb;
         {
-            bbjd_int nkeep = minsub - locked;
-            if (nkeep < 1) nkeep = 1;
             const double tq = now_s();
-            int qrc = bbjd_qz_solve(&d->qz, k, MA, MB, (zcx *)ws->SA, (zcx *)ws->SB, (zcx *)ws->ZA,
+            int qrc;
+            if (schur) {
+                bbjd_int nkeep = minsub - locked;
+                if (nkeep < 1) nkeep = 1;
+                qrc = bbjd_qz_solve(&d->qz, k, MA, MB, (zcx *)ws->SA, (zcx *)ws->SB, (zcx *)ws->ZA,
                                     (zcx *)ws->ZB, al, be, sigma, P->target, nkeep, eval, R->nconv, tol, &p);
+                if (!qrc) {
+                    ta = al[p];
+                    tb = be[p];
+                    bbjd_ritz_coefficients(k, ldm, p, ta, tb, (const zcx *)ws->SA, (const zcx *)ws->SB,
+                                           (const zcx *)ws->ZB, d->yv, d->xv);
+                }
+            } else {
+                qrc = bbjd_qz_target(&d->qz, k, MA, MB, al, be, sigma, P->target, eval, R->nconv, tol, &p);
+                if (!qrc) {
+                    ta = al[p];
+                    tb = be[p];
+                    qrc = bbjd_ritz_vector(&d->qz, k, MA, MB, ta, tb, d->xv);
+                }
+            }
             d->st.qz_seconds += now_s() - tq;
             if (qrc) { st = BBJD_ERR_LAPACK; goto done; }
         }
-        const zcx ta = al[p], tb = be[p];
-        /* ---- step 2: Ritz vector u = V ZB y on the device ---- */
+        /* ---- step 2: Ritz vector u = V x on the device ---- */
         void *u = ws->tmp1, *Au = ws->tmp2, *Bu = ws->tmp3, *r = ws->tmp4;
-        bbjd_ritz_coefficients(k, ldm, p, ta, tb, (const zcx *)ws->SA, (const zcx *)ws->SB, (const zcx *)ws->ZB,
-                               d->yv, d->xv);
         rc |= upload_small(c, d->rd.coef, d->xv, k);
         rc |= bbjd_k_gemv_n(c->s, n, (int)k, ws->V, c->ld, d->rd.coef, u, &d->rd, SL_U);
         rc |= bbjd_k_scale_inv(c->s, n, u, d->rd.scal, SL_U, 1);

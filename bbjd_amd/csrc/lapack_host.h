@@ -27,6 +27,11 @@ void scipy_zgetrf_(const blas_int *m, const blas_int *n, zcplx *a, const blas_in
                    blas_int *info);
 void scipy_zgetri_(const blas_int *n, zcplx *a, const blas_int *lda, const blas_int *ipiv, zcplx *work,
                    const blas_int *lwork, blas_int *info);
+void scipy_zggev_(const char *jl, const char *jr, const blas_int *n, zcplx *A, const blas_int *lda, zcplx *B,
+                  const blas_int *ldb, zcplx *al, zcplx *be, zcplx *vl, const blas_int *ldvl, zcplx *vr,
+                  const blas_int *ldvr, zcplx *work, const blas_int *lwork, double *rwork, blas_int *info);
+void scipy_zgetrs_(const char *t, const blas_int *n, const blas_int *nrhs, const zcplx *a, const blas_int *lda,
+                   const blas_int *ipiv, zcplx *b, const blas_int *ldb, blas_int *info);
 void scipy_zpotrf_(const char *uplo, const blas_int *n, zcplx *a, const blas_int *lda, blas_int *info);
 void scipy_ztrtri_(const char *uplo, const char *diag, const blas_int *n, zcplx *a, const blas_int *lda,
                    blas_int *info);

@@ -19,7 +19,16 @@
  *  - the ZGGES output is cached: on a lock iteration the reference leaves
  *    (MA, MB) unchanged (`continue` at src/jd_core.c:931) so the next ZGGES
  *    would recompute the same factors; the cache replays them and only the
- *    selection and ZTGSEN run again.
+ *    selection and ZTGSEN run again;
+ *  - the ordered Schur form is needed only where a thick restart follows
+ *    (it keeps the leading Schur vectors).  The other iterations use only
+ *    the target Ritz pair: bbjd_qz_target computes the eigenvalues alone
+ *    (ZGGEV without vectors: the same alpha, beta as ZGGES's before the
+ *    reordering) and picks the target by the same rule, and
+ *    bbjd_ritz_vector takes the Ritz vector as the null vector of
+ *    tb MA - ta MB by inverse iteration -- in exact arithmetic the vector
+ *    the Schur back-substitution gives, up to scale.  About 40% of the QZ
+ *    step's cost.
  */
 #include "qz_host.h"
 #include "lapack_host.h"
@@ -57,11 +66,18 @@ int bbjd_qz_init(bbjd_qz *q, bbjd_int maxsub)
     if (tl > q->lwork) q->lwork = tl;
     q->liwork = tiq + 1 > 1 ? tiq + 1 : 1;
     if (q->lwork < 2 * maxsub + 64) q->lwork = 2 * maxsub + 64;
+    lw = -1;
+    scipy_zggev_("N", "N", &bk, A, &lda, B, &lda, q->cal, q->cbe, NULL, &bk, NULL, &bk, &wq, &lw, q->rwork, &info);
+    tl = (blas_int)creal(wq) + 1;
+    if (tl > q->lwork) q->lwork = tl;
     q->work = malloc((size_t)q->lwork * sizeof(zcplx));
+    q->eal = malloc((size_t)maxsub * sizeof(zcplx)); q->ebe = malloc((size_t)maxsub * sizeof(zcplx));
+    q->lu = malloc(ss * sizeof(zcplx)); q->rhs = malloc((size_t)maxsub * sizeof(zcplx));
+    q->ipiv = malloc((size_t)maxsub * sizeof(blas_int));
     q->iwork = malloc((size_t)q->liwork * sizeof(blas_int));
     free(A); free(B);
     if (!q->cSA || !q->cSB || !q->cZA || !q->cZB || !q->cal || !q->cbe || !q->rwork || !q->bwork ||
-        !q->sel || !q->work || !q->iwork) {
+        !q->sel || !q->work || !q->iwork || !q->eal || !q->ebe || !q->lu || !q->rhs || !q->ipiv) {
         bbjd_qz_free(q);
         return -1;
     }
@@ -72,6 +88,7 @@ void bbjd_qz_free(bbjd_qz *q)
 {
     free(q->cSA); free(q->cSB); free(q->cZA); free(q->cZB); free(q->cal); free(q->cbe);
     free(q->rwork); free(q->bwork); free(q->sel); free(q->work); free(q->iwork);
+    free(q->eal); free(q->ebe); free(q->lu); free(q->rhs); free(q->ipiv);
     memset(q, 0, sizeof(*q));
 }
 
@@ -184,4 +201,79 @@ void bbjd_ritz_coefficients(bbjd_int k, bbjd_int ld, bbjd_int p, zcplx ta, zcplx
     blas_int bk = (blas_int)k, bp1 = (blas_int)(p + 1), ldz = (blas_int)ld, one = 1;
     const zcplx a1 = 1.0, b0 = 0.0;
     scipy_zgemv_("N", &bk, &bp1, &a1, ZB, &ldz, y, &one, &b0, x, &one);
+}
+
+/* Eigenvalues of the projected pencil and the target position without the
+ * Schur form: ZGGEV('N', 'N') on copies of (MA, MB), cached while the pencil
+ * is unchanged (lock iterations); the target is the best unconverged value
+ * by key over all positions, as in bbjd_qz_solve. */
+int bbjd_qz_target(bbjd_qz *q, bbjd_int k, const zcplx *MA, const zcplx *MB, zcplx *al, zcplx *be, zcplx sigma,
+                   bbjd_target tgt, const zcplx *conv, bbjd_int nconv, double tol, bbjd_int *tpos)
+{
+    const bbjd_int ld = q->maxsub;
+    const size_t colb = (size_t)k * sizeof(zcplx);
+    if (!(q->ecache_valid && q->ecache_k == k)) {
+        zcplx *A = q->cSA, *B = q->cSB;       /* scratch: the ZGGES cache is invalid here */
+        for (bbjd_int j = 0; j < k; j++) {
+            memcpy(A + j * ld, MA + j * ld, colb);
+            memcpy(B + j * ld, MB + j * ld, colb);
+        }
+        q->cache_valid = 0;
+        blas_int bk = (blas_int)k, lda = (blas_int)ld, one = 1, lw = q->lwork, info = 0;
+        scipy_zggev_("N", "N", &bk, A, &lda, B, &lda, q->eal, q->ebe, NULL, &one, NULL, &one, q->work, &lw,
+                     q->rwork, &info);
+        if (info) return -1;
+        q->ggev_calls++;
+        q->ecache_valid = 1;
+        q->ecache_k = k;
+    }
+    memcpy(al, q->eal, colb);
+    memcpy(be, q->ebe, colb);
+    bbjd_int p = 0;
+    double bv = DBL_MAX;
+    for (bbjd_int i = 0; i < k; i++) {
+        if (cabs(be[i]) < 1e-300) continue;
+        zcplx lam = al[i] / be[i];
+        if (is_converged(lam, conv, nconv, tol)) continue;
+        double v = key_of(tgt, lam, sigma);
+        if (v < bv) { bv = v; p = i; }
+    }
+    *tpos = p;
+    return 0;
+}
+
+/* Ritz vector of (MA, MB) for the eigenvalue ta / tb: two steps of inverse
+ * iteration with M = tb MA - ta MB (LU with partial pivoting; an exactly
+ * zero pivot is replaced by eps ||M||), from a fixed start vector; x is
+ * normalised to unit 2-norm (its scale is irrelevant: u = V x is
+ * normalised on the device). */
+int bbjd_ritz_vector(bbjd_qz *q, bbjd_int k, const zcplx *MA, const zcplx *MB, zcplx ta, zcplx tb, zcplx *x)
+{
+    const bbjd_int ld = q->maxsub;
+    zcplx *M = q->lu;
+    double mnorm = 0;
+    for (bbjd_int j = 0; j < k; j++)
+        for (bbjd_int i = 0; i < k; i++) {
+            const zcplx v = tb * MA[i + j * ld] - ta * MB[i + j * ld];
+            M[i + j * k] = v;
+            const double a = cabs(v);
+            if (a > mnorm) mnorm = a;
+        }
+    blas_int bk = (blas_int)k, one = 1, info = 0;
+    scipy_zgetrf_(&bk, &bk, M, &bk, q->ipiv, &info);
+    if (info < 0) return -1;
+    const double tiny = DBL_EPSILON * (mnorm > 0 ? mnorm : 1.0);
+    for (bbjd_int i = 0; i < k; i++)
+        if (cabs(M[i + i * k]) < tiny) M[i + i * k] = tiny;
+    for (bbjd_int i = 0; i < k; i++) x[i] = 1.0 + 0.5 * I * (double)((i * 7919) % 13) / 13.0;
+    for (int step = 0; step < 2; step++) {
+        scipy_zgetrs_("N", &bk, &one, M, &bk, q->ipiv, x, &bk, &info);
+        if (info) return -1;
+        double nrm = 0;
+        for (bbjd_int i = 0; i < k; i++) nrm += creal(x[i]) * creal(x[i]) + cimag(x[i]) * cimag(x[i]);
+        nrm = sqrt(nrm);
+        if (!(nrm > 0) || !isfinite(nrm)) return -1;
+        for (bbjd_int i = 0; i < k; i++) x[i] /= nrm;
+    }
+    return 0;
 }

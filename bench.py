@@ -12,8 +12,12 @@ in HBM before the timed region; the timed region is the serving loop.
 
   value        served eigenpairs/s over the timed steps (all pairs converged
                at tol, each pair's FP64 residual checked)
-  plain        the same solves one at a time (FP64 engine, no overlap), on a
-               seeded sample of the requests -> eigenpairs/s
+  plain        the same solves one at a time (FP64 engine, no overlap): a
+               stratified sample -- per (n, k) class of the workload, two of
+               its requests (seeded) solved alone -- gives each class's mean
+               pairs and seconds per request; weighted by the class counts of
+               the 256 requests they estimate the plain path's eigenpairs/s
+               on the whole workload
   latency      p50/p99 of completion - arrival over the timed requests
   roofline     the projection kernel Q^H x (the dominant streaming kernel):
                HIP-event durations of every launch of the plain leg (one
@@ -112,7 +116,7 @@ def main():
     ap.add_argument("--rate", type=float, default=32.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--workers", type=int, default=16)
     ap.add_argument("--maxiter", type=int, default=1000)
-    ap.add_argument("--plain-sample", type=int, default=16)
+    ap.add_argument("--plain-per-class", type=int, default=2, help="plain solves per (n, k) class")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -190,21 +194,35 @@ def main():
     log(f"[bench] served: {pairs} pairs / {wall:.1f}s; worst normalised residual {worst:.2e}; "
         f"bad {bad}; statuses {sorted(set(statuses))}")
 
-    # ---- plain: one solve at a time, FP64, seeded sample ----
+    # ---- plain: one solve at a time, FP64, stratified by (n, k) class ----
     rng = np.random.default_rng(args.seed + 1)
-    sample = sorted(rng.choice(len(reqs), size=min(args.plain_sample, len(reqs)), replace=False).tolist())
+    classes = {}
+    for i, r in enumerate(reqs):
+        classes.setdefault((r["n"], r["k"]), []).append(i)
+    sample, cls_of = [], {}
+    for key in sorted(classes):
+        members = classes[key]
+        pick = rng.choice(len(members), size=min(args.plain_per_class, len(members)), replace=False)
+        for j in sorted(pick.tolist()):
+            sample.append(members[j])
+            cls_of[members[j]] = key
     srv.ktime_enable(True)
     t = time.perf_counter()
     pres, prep = srv.serve(jobs_for(sample, False), workers=1, realtime=False)
     t_plain = time.perf_counter() - t
     srv.ktime_enable(False)
     kcnt, kms, kbytes = srv.ktime_harvest()
-    plain_pairs = prep["pairs"]
+    per_cls = {}
     for j, i in enumerate(sample):
         r = pres[j]
+        dt = r["t_done"] - r["t_start"]
+        per_cls.setdefault(cls_of[i], []).append((r["nconv"], dt))
         log(f"[bench]   plain n={reqs[i]['n']} k={reqs[i]['k']}: {r['nconv']} pairs, {r['iters']} iters, "
-            f"{r['t_done'] - r['t_start']:.2f}s (qz {r['qz_seconds']:.2f}s), status {r['status']}")
-    plain_rate = plain_pairs / t_plain if t_plain else None
+            f"{dt:.2f}s (qz {r['qz_seconds']:.2f}s), status {r['status']}")
+    est_pairs = sum(len(classes[c]) * np.mean([p for p, _ in v]) for c, v in per_cls.items())
+    est_secs = sum(len(classes[c]) * np.mean([d for _, d in v]) for c, v in per_cls.items())
+    plain_rate = est_pairs / est_secs if est_secs else None
+    plain_pairs = prep["pairs"]
     diff = None
     if served:
         # same eigenvalues alone and in the mix
@@ -223,7 +241,8 @@ def main():
         busy = sum(r["t_done"] - r["t_start"] for r in mix)
         log(f"[bench] sample solve seconds alone {s_alone:.1f} / in the mix {s_mix:.1f} "
             f"(qz {q_alone:.1f} / {q_mix:.1f}); mean solves in flight {busy / walls[0]:.1f}")
-    log(f"[bench] plain sample: {plain_pairs} pairs / {t_plain:.1f}s; "
+    log(f"[bench] plain: {len(sample)} solves in {len(per_cls)} classes, {plain_pairs} pairs / {t_plain:.1f}s; "
+        f"class-weighted estimate for the workload {plain_rate:.2f} pairs/s; "
         f"max rel eig diff plain vs served {diff}")
 
     served_rate = pairs / wall if wall else None
@@ -269,6 +288,7 @@ def main():
         "plain_eigenpairs_per_s": round(plain_rate, 3) if plain_rate else None,
         "served_over_plain": round(served_rate / plain_rate, 2) if served_rate and plain_rate else None,
         "plain_sample_requests": len(sample),
+        "plain_estimate": "per (n, k) class mean of its sampled solves alone, weighted by class counts",
         "latency_p50_s": round(pct(lat, 50), 3) if lat else None,
         "latency_p99_s": round(pct(lat, 99), 3) if lat else None,
         "pairs": pairs,

@@ -54,7 +54,16 @@ typedef struct {
      * (M^-1 r, M^-1 u) as one call; NULL: two precond calls.  Must apply the
      * same M as params.precond. */
     bbjd_precond2_fn    precond2;
+    /* projected step: BBJD_QZ_RITZ (default) computes the ordered
+     * generalized Schur form only on iterations a thick restart may follow
+     * and otherwise the eigenvalues alone plus the target Ritz vector;
+     * BBJD_QZ_SCHUR computes the ordered Schur form every iteration, the
+     * reference's sequence of LAPACK calls.  Same Ritz pairs in exact
+     * arithmetic. */
+    int                 qz_mode;
 } bbjd_gpu_options;
+
+enum { BBJD_QZ_RITZ = 0, BBJD_QZ_SCHUR = 1 };
 
 void bbjd_gpu_options_init(bbjd_gpu_options *o);
 

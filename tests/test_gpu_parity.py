@@ -269,3 +269,26 @@ def test_block_gram_matches_numpy(n, p, q):
     G2 = np.zeros_like(G)
     assert lib.bbjd_block_gram(n, p, X.ctypes.data, q, Y.ctypes.data, G2.ctypes.data) == 0
     assert np.array_equal(G, G2)
+
+
+def test_qz_modes_same_eigenpairs():
+    """the eigenvalue-only projected step with inverse-iteration Ritz vectors
+    ('ritz') and the ordered Schur form every iteration ('schur', the
+    reference's LAPACK sequence) converge to the same eigenvalues, both equal
+    to the oracle's; restarts included"""
+    p, sigma = _cfg1(n=2000, seed=1)
+    ops = orc.CpuPencilOps(p, sigma, 1)
+    ref = orc.solve("oracle", p.n, 8, sigma, tol=1e-10, maxsub=20, minsub=10, maxiter=600, ops=ops)
+    dev = DeviceOperator(p, sigma, pc="exact")
+    out = {}
+    for mode in ("ritz", "schur"):
+        ev, X, res, nconv, iters, status, st = jdqz_c(p.n, 8, sigma, 0, None, None, dev, tol=1e-10, maxsub=20,
+                                                       minsub=10, maxiter=600, device_op=dev, qz_mode=mode,
+                                                       return_stats=True)
+        assert status == 0 and nconv == 8 == ref["nconv"], mode
+        assert st["restarts"] > 0
+        assert rel_match(ev, ref["evals"]) <= 1e-8 and rel_match(ref["evals"], ev) <= 1e-8, mode
+        assert np.all(res < 1e-10)
+        out[mode] = (ev, iters)
+    assert rel_match(out["ritz"][0], out["schur"][0]) <= 1e-8
+    assert abs(out["ritz"][1] - out["schur"][1]) <= max(5, ref["iters"] // 10)
