@@ -21,6 +21,7 @@
 //  * the same templates give the complex64 kernels (FP32 accumulation) of
 //    the mixed-precision inner solve.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 #include <mutex>
 #include <vector>
 #include "engine.h"
@@ -28,10 +29,11 @@
 typedef long long i64;
 
 // ---------------------------------------------------------------------------
-// Launch timing of the projection kernel (k_proj, one vector): when enabled,
-// every launch is bracketed by two HIP events on its own stream and its
-// algorithmic bytes (Q tile read once, x read once per column tile) are
-// recorded; bbjd_ktime_harvest() turns the events into durations.
+// Launch timing of the projection kernel (k_proj, one vector, first CGS
+// pass): when enabled, every launch carries a start and a stop HIP event in
+// its dispatch (hipExtLaunchKernelGGL), so their interval is the kernel's
+// execution, and its algorithmic bytes 16 n (k + 1) are recorded;
+// bbjd_ktime_harvest() turns the events into durations.
 namespace {
 struct KRec { hipEvent_t a, b; double bytes; };
 std::mutex g_kt_mu;
@@ -747,13 +749,22 @@ static int proj_pass_nv(hipStream_t s, i64 n, int k, const void *Q, i64 ldq, con
     const int fused = nt <= g_fuse_max_blocks;
     dim3 g(ctiles, nt);
     const bool timed = NV == 1 && g_kt_on && sizeof(R) == 8 && cond == nullptr;
-    hipEvent_t ea = nullptr, eb = nullptr;
     if (timed) {
+        // events carried by the dispatch itself: their interval is the
+        // kernel's execution, as a kernel trace reports it
+        hipEvent_t ea, eb;
+        {
+            std::lock_guard<std::mutex> lk(g_kt_mu);
+            ea = kt_event(); eb = kt_event();
+        }
+        hipExtLaunchKernelGGL((k_proj<R, NV, C, false>), g, dim3(TPB), 0, s, ea, eb, 0, (i64)n, chunk, k,
+                              (const Cx<R> *)Q, (i64)ldq, (const Cx<R> *)x0, (const Cx<R> *)x1, (Cx<R> *)rd->part,
+                              ldp, (R *)rd->pnorm, want_norm, cond, rd->cnt, (Cx<R> *)out0, (Cx<R> *)out1,
+                              rd->scal, norm_slot, fused);
+        const double bytes = (double)sizeof(Cx<R>) * (double)n * ((double)k + 1.0);
         std::lock_guard<std::mutex> lk(g_kt_mu);
-        ea = kt_event(); eb = kt_event();
-        (void)hipEventRecord(ea, s);
-    }
-    if (cond)
+        g_kt.push_back({ea, eb, bytes});
+    } else if (cond)
         hipLaunchKernelGGL((k_proj<R, NV, C, true>), g, dim3(TPB), 0, s, (i64)n, chunk, k, (const Cx<R> *)Q,
                            (i64)ldq, (const Cx<R> *)x0, (const Cx<R> *)x1, (Cx<R> *)rd->part, ldp, (R *)rd->pnorm,
                            want_norm, cond, rd->cnt, (Cx<R> *)out0, (Cx<R> *)out1, rd->scal, norm_slot, fused);
@@ -761,12 +772,6 @@ static int proj_pass_nv(hipStream_t s, i64 n, int k, const void *Q, i64 ldq, con
         hipLaunchKernelGGL((k_proj<R, NV, C, false>), g, dim3(TPB), 0, s, (i64)n, chunk, k, (const Cx<R> *)Q,
                            (i64)ldq, (const Cx<R> *)x0, (const Cx<R> *)x1, (Cx<R> *)rd->part, ldp, (R *)rd->pnorm,
                            want_norm, cond, rd->cnt, (Cx<R> *)out0, (Cx<R> *)out1, rd->scal, norm_slot, fused);
-    if (timed) {
-        (void)hipEventRecord(eb, s);
-        const double bytes = (double)sizeof(Cx<R>) * (double)n * ((double)k + 1.0);
-        std::lock_guard<std::mutex> lk(g_kt_mu);
-        g_kt.push_back({ea, eb, bytes});
-    }
     if (!fused)
         hipLaunchKernelGGL((k_proj_finish<R, NV, C>), dim3(g.x), dim3(TPB), 0, s, nt, k, (const Cx<R> *)rd->part,
                            ldp, (const R *)rd->pnorm, want_norm, cond, (Cx<R> *)out0, (Cx<R> *)out1, rd->scal,

@@ -20,11 +20,13 @@ in HBM before the timed region; the timed region is the serving loop.
                on the whole workload
   latency      p50/p99 of completion - arrival over the timed requests
   roofline     the projection kernel Q^H x (the dominant streaming kernel):
-               HIP-event durations of every launch of the plain leg (one
-               stream, so a launch's duration is its own) vs its algorithmic
-               bytes 16 n (k + 1), HBM peak 8 TB/s; the same kernel's rate
-               inside the served region (launches time-share the GPU there)
-               is reported beside it
+               durations of every first-pass launch of the plain leg (one
+               stream) from start/stop HIP events carried by each dispatch,
+               vs its algorithmic bytes 16 n (k + 1), HBM peak 8 TB/s; the
+               same kernel's rate inside the served region (launches
+               time-share the GPU there) is reported beside it; traffic =
+               average algorithmic bytes x the counter-measured ratio in
+               profiles/k_proj_traffic.json
   cpu_baseline the reference solver compiled from its sources (oracle/_ref;
                else the oracle restatement) on the host cores, on a bounded
                sample of the workload's requests, CPU exact banded LU
@@ -249,8 +251,17 @@ def main():
     roof = None
     if kcnt:
         achieved = kbytes / (kms * 1e-3) / 1e9
+        # HBM bytes per launch: the counter-measured traffic/algorithmic ratio
+        # of this kernel (rocprofv3 FETCH_SIZE + WRITE_SIZE at the workload's
+        # average shape, calibrated on a copy kernel) times this run's
+        # average algorithmic bytes
+        traffic = None
+        tj = os.path.join(ROOT, "profiles", "k_proj_traffic.json")
+        if os.path.exists(tj):
+            with open(tj) as f:
+                traffic = round(kbytes / kcnt * json.load(f)["traffic_over_algorithmic"])
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": "k_proj<double,1,16,false> (Q^H x, complex128)", "measured_on": "plain leg",
                 "launches": kcnt, "avg_us": round(kms * 1e3 / kcnt, 2), "avg_bytes": round(kbytes / kcnt)}
         if s_cnt:

@@ -119,8 +119,10 @@ int bbjd_device_init(int sync_yield);
 
 /* Launch timing of the engine's dominant streaming kernel (the one-vector
  * projection Q^H x): enable, run, then harvest the number of timed launches,
- * the sum of their durations (ms, HIP events on the launching stream) and of
- * their algorithmic bytes (16 n (k + 1): Q and x read once). */
+ * the sum of their durations (ms, between start and stop HIP events carried
+ * by each dispatch) and of their algorithmic bytes (16 n (k + 1): Q and x
+ * read once).  First CGS passes and plain projections; the conditional
+ * second passes are not timed. */
 void bbjd_ktime_enable(int on);
 int  bbjd_ktime_harvest(double *total_ms, double *total_bytes);
 
