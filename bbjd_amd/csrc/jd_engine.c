@@ -343,6 +343,9 @@ typedef struct {
     bbjd_int          n, ld;
     bbjd_int          ms;      /* maxsub of this solve (restart threshold)   */
     bbjd_int          ldm;     /* leading dimension of the host k x k data   */
+    zcx               nu0, mu0;  /* the test space's combination             */
+    int               w_mixed;   /* a W column came from the Av fallback since
+                                    the last restart: no triangular relation */
 } ctx_t;
 
 #define COLZ(M, j) ((void *)((zcx *)(M) + (size_t)(j) * c->ld))
@@ -600,20 +603,35 @@ static int gmres(ctx_t *c, bbjd_int ndefl, const void *QR, const void *QL, zcx t
 /* Pencil column/row k after expansion (reference src/jd_core.c:1085-1104):
  * MA[0:k+1, k] = W^H AV[:, k], MB likewise, MA[k, 0:k] = W[:, k]^H AV[:, :k].
  * Rows come as conj(AV^H w).  One host sync. */
-static int pencil_update(ctx_t *c, bbjd_int k)
+/* New column and row of the projected pencil (reference src/jd_core.c:1085-1104):
+ * MA[:k+1, k] = W^H AV[:, k], MB[:k+1, k] = W^H BV[:, k] in one two-vector pass;
+ * the row k from one pass.  W[:, k] is the normalised CGS2 remainder of
+ * nu0 AV[:, k] + mu0 BV[:, k] against W[:, :k], so C = nu0 MA + mu0 MB is
+ * upper triangular (w_k is orthogonal to nu0 AV[:, j] + mu0 BV[:, j] =
+ * W[:, :j+1] c_j for j < k, to the orthogonality of W): with nu0 != 0 the
+ * row of MA is -(mu0 / nu0) times the row of MB, which alone is projected
+ * (BV^H w_k); with nu0 = 0, C = MB and its row is zero, and the row of MA
+ * is projected.  rowrel = 0 (some column of W since the last restart came
+ * from the Av fallback, so the relation does not hold) projects both rows as
+ * the reference does. */
+static int pencil_update(ctx_t *c, bbjd_int k, int rowrel)
 {
     bbjd_workspace *ws = c->ws;
     struct bbjd_dev *d = c->d;
     const bbjd_int ms = c->ldm;
     zcx *pen = (zcx *)d->pencil;
+    const int via_b = rowrel && cabs(c->nu0) > 0;
+    const int via_a = rowrel && !via_b;
     int rc = 0;
     rc |= bbjd_k_gemv_c(c->s, c->n, (int)(k + 1), ws->W, c->ld, COLZ(ws->AV, k), COLZ(ws->BV, k), pen,
                         pen + (ms + 1), &d->rd);
     if (k > 0) {
-        rc |= bbjd_k_gemv_c(c->s, c->n, (int)k, ws->AV, c->ld, COLZ(ws->W, k), NULL, pen + 2 * (ms + 1), NULL,
-                            &d->rd);
-        rc |= bbjd_k_gemv_c(c->s, c->n, (int)k, ws->BV, c->ld, COLZ(ws->W, k), NULL, pen + 3 * (ms + 1), NULL,
-                            &d->rd);
+        if (!via_b)
+            rc |= bbjd_k_gemv_c(c->s, c->n, (int)k, ws->AV, c->ld, COLZ(ws->W, k), NULL, pen + 2 * (ms + 1), NULL,
+                                &d->rd);
+        if (!via_a)
+            rc |= bbjd_k_gemv_c(c->s, c->n, (int)k, ws->BV, c->ld, COLZ(ws->W, k), NULL, pen + 3 * (ms + 1), NULL,
+                                &d->rd);
     }
     rc |= !HIPOK(hipMemcpyAsync(d->hcol, pen, (size_t)4 * (ms + 1) * 16, hipMemcpyDeviceToHost, c->s));
     /* the norms of the expansion (breakdown checks) come back with it */
@@ -623,8 +641,15 @@ static int pencil_update(ctx_t *c, bbjd_int k)
     zcx *MA = (zcx *)ws->MA, *MB = (zcx *)ws->MB;
     const zcx *ca = d->hcol, *cb = d->hcol + (ms + 1), *ra = d->hcol + 2 * (ms + 1), *rb = d->hcol + 3 * (ms + 1);
     for (bbjd_int i = 0; i <= k; i++) { MA[i + k * ms] = ca[i]; MB[i + k * ms] = cb[i]; }
-    for (bbjd_int i = 0; i < k; i++) { MA[k + i * ms] = conj(ra[i]); MB[k + i * ms] = conj(rb[i]); }
-    d->st.stream_bytes += 16.0 * c->n * ((k + 1) + 2 + 2.0 * (k + 1));
+    if (via_b) {
+        const zcx f = -c->mu0 / c->nu0;
+        for (bbjd_int i = 0; i < k; i++) { MB[k + i * ms] = conj(rb[i]); MA[k + i * ms] = f * MB[k + i * ms]; }
+    } else if (via_a) {
+        for (bbjd_int i = 0; i < k; i++) { MA[k + i * ms] = conj(ra[i]); MB[k + i * ms] = 0.0; }
+    } else {
+        for (bbjd_int i = 0; i < k; i++) { MA[k + i * ms] = conj(ra[i]); MB[k + i * ms] = conj(rb[i]); }
+    }
+    d->st.stream_bytes += 16.0 * c->n * ((k + 1) + 2 + (rowrel ? 1.0 : 2.0) * (k + 1));
     return 0;
 }
 
@@ -722,6 +747,7 @@ static int thick_restart(ctx_t *c, bbjd_int k, bbjd_int keep, zcx nu0, zcx mu0)
             MA[i + j * ms] = hg[i + j * keep];
             MB[i + j * ms] = hg[keep * keep + i + j * keep];
         }
+    c->w_mixed = 0;     /* W is the orthonormal factor of nu0 AV + mu0 BV again */
     d->st.restarts++;
     d->st.stream_bytes += 16.0 * n * (3.0 * (k + keep) + 3.0 * keep + 4.0 * keep + 3.0 * keep);
     return rc ? -1 : 0;
@@ -786,7 +812,7 @@ static int expand(ctx_t *c, bbjd_int k, const void *z, zcx nu0, zcx mu0)
         rc |= bbjd_k_lincomb(c->s, n, creal(nu0), cimag(nu0), an, creal(mu0), cimag(mu0), bn, wn);
         rc |= bbjd_k_cgs2(c->s, n, (int)k, ws->W, c->ld, wn, &d->rd, SL_W, NULL);
         rc |= bbjd_k_scale_inv(c->s, n, wn, d->rd.scal, SL_W, 1);
-        if (rc || pencil_update(c, k)) return -2;
+        if (rc || pencil_update(c, k, !c->w_mixed)) return -2;
         d->st.stream_bytes += 16.0 * n * (4.0 * k + 4 + 3 + 4.0 * k + 2);
         const double nv = d->hscal[SL_V], nw = d->hscal[SL_W];
         if (nv >= 1e-14 && nw >= 1e-14) return 0;
@@ -796,7 +822,8 @@ static int expand(ctx_t *c, bbjd_int k, const void *z, zcx nu0, zcx mu0)
             rc |= !HIPOK(hipMemcpyAsync(wn, an, (size_t)n * 16, hipMemcpyDeviceToDevice, c->s));
             rc |= bbjd_k_cgs2(c->s, n, (int)k, ws->W, c->ld, wn, &d->rd, SL_W, NULL);
             rc |= bbjd_k_scale_inv(c->s, n, wn, d->rd.scal, SL_W, 1);
-            if (rc || pencil_update(c, k)) return -2;
+            c->w_mixed = 1;
+            if (rc || pencil_update(c, k, 0)) return -2;
             return 0;
         }
         nrm = nv;           /* v broke down: the random-vector branch below */
@@ -817,13 +844,15 @@ static int expand(ctx_t *c, bbjd_int k, const void *z, zcx nu0, zcx mu0)
     rc |= bbjd_k_lincomb(c->s, n, creal(nu0), cimag(nu0), an, creal(mu0), cimag(mu0), bn, wn);
     rc |= bbjd_k_cgs2(c->s, n, (int)k, ws->W, c->ld, wn, &d->rd, SL_W, NULL);
     if (rc || sync_scal(c)) return -2;
-    if (d->hscal[SL_W] < 1e-14) {
+    const int w_fallback = d->hscal[SL_W] < 1e-14;
+    if (w_fallback) {
         rc |= !HIPOK(hipMemcpyAsync(wn, an, (size_t)n * 16, hipMemcpyDeviceToDevice, c->s));
         rc |= bbjd_k_cgs2(c->s, n, (int)k, ws->W, c->ld, wn, &d->rd, SL_W, NULL);
     }
     rc |= bbjd_k_scale_inv(c->s, n, wn, d->rd.scal, SL_W, 1);
     d->st.stream_bytes += 16.0 * n * (4.0 * k + 4 + 3 + 4.0 * k + 2);
-    if (rc || pencil_update(c, k)) return -2;
+    if (w_fallback) c->w_mixed = 1;
+    if (rc || pencil_update(c, k, !c->w_mixed)) return -2;
     return 0;
 }
 
@@ -861,6 +890,8 @@ static bbjd_status solve_impl(const bbjd_params *P, bbjd_workspace *ws, bbjd_res
     } else if (P->target == BBJD_TARGET_SMALLEST_MAG) {
         nu0 = 1.0; mu0 = 0.0;
     }
+    cx.nu0 = nu0;
+    cx.mu0 = mu0;
     ws->kdim = 0; ws->nlock = 0;
     bbjd_qz_invalidate(&d->qz);
     rng_seed(d, 42);
@@ -889,7 +920,7 @@ static bbjd_status solve_impl(const bbjd_params *P, bbjd_workspace *ws, bbjd_res
     rc |= bbjd_k_lincomb_norm(c->s, n, creal(nu0), cimag(nu0), ws->AV, creal(mu0), cimag(mu0), ws->BV, ws->W,
                               &d->rd, SL_W);
     rc |= bbjd_k_scale_inv(c->s, n, ws->W, d->rd.scal, SL_W, 0);
-    if (rc || pencil_update(c, 0)) { st = BBJD_ERR_ALLOC; goto done; }
+    if (rc || pencil_update(c, 0, 1)) { st = BBJD_ERR_ALLOC; goto done; }
     ws->kdim = 1;
 
     for (bbjd_int iter = 0; iter < P->maxiter; iter++) {
@@ -918,7 +949,8 @@ static bbjd_status solve_impl(const bbjd_params *P, bbjd_workspace *ws, bbjd_res
                                            (const zcx *)ws->ZB, d->yv, d->xv);
                 }
             } else {
-                qrc = bbjd_qz_target(&d->qz, k, MA, MB, al, be, sigma, P->target, eval, R->nconv, tol, &p);
+                qrc = bbjd_qz_target(&d->qz, k, MA, MB, nu0, mu0, al, be, sigma, P->target, eval, R->nconv, tol,
+                                     &p);
                 if (!qrc) {
                     ta = al[p];
                     tb = be[p];

@@ -706,9 +706,12 @@ extern "C" int64_t bbjd_red_part_entries(int64_t n, int maxk)
 }
 
 // Blocks up to which the finishing step runs in the producing kernel's last
-// block; larger grids finish in a second small kernel instead (one L2
-// write-back per block costs more than a launch there).  Tunable.
-static int g_fuse_max_blocks = 48;
+// block; larger grids finish in a second small kernel instead.  Each fused
+// block pays an agent-scope release (an L2 write-back), which costs more
+// than a launch beyond a few blocks -- and far more when other streams' data
+// sits dirty in the L2: on the 16-stream serving workload a threshold of 16
+// served 160 pairs/s, 48 served 153 and fusing every grid 108.  Tunable.
+static int g_fuse_max_blocks = 16;
 
 static inline NormFin fin_none()
 {

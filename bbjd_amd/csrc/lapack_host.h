@@ -32,6 +32,14 @@ void scipy_zggev_(const char *jl, const char *jr, const blas_int *n, zcplx *A, c
                   const blas_int *ldvr, zcplx *work, const blas_int *lwork, double *rwork, blas_int *info);
 void scipy_zgetrs_(const char *t, const blas_int *n, const blas_int *nrhs, const zcplx *a, const blas_int *lda,
                    const blas_int *ipiv, zcplx *b, const blas_int *ldb, blas_int *info);
+void scipy_zgehrd_(const blas_int *n, const blas_int *ilo, const blas_int *ihi, zcplx *a, const blas_int *lda,
+                   zcplx *tau, zcplx *work, const blas_int *lwork, blas_int *info);
+void scipy_zhseqr_(const char *job, const char *compz, const blas_int *n, const blas_int *ilo, const blas_int *ihi,
+                   zcplx *h, const blas_int *ldh, zcplx *w, zcplx *z, const blas_int *ldz, zcplx *work,
+                   const blas_int *lwork, blas_int *info);
+void scipy_ztrsm_(const char *side, const char *uplo, const char *transa, const char *diag, const blas_int *m,
+                  const blas_int *n, const zcplx *alpha, const zcplx *a, const blas_int *lda, zcplx *b,
+                  const blas_int *ldb);
 void scipy_zpotrf_(const char *uplo, const blas_int *n, zcplx *a, const blas_int *lda, blas_int *info);
 void scipy_ztrtri_(const char *uplo, const char *diag, const blas_int *n, zcplx *a, const blas_int *lda,
                    blas_int *info);

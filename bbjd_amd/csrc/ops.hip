@@ -210,6 +210,150 @@ __global__ __launch_bounds__(1024) void k_bcr_tail(int s, i64 N, int l0, int L, 
     }
 }
 
+// The same tail levels with the right-hand sides resident in LDS: the rows
+// the tail touches (block rows at multiples of 2^l0, M0 of them) are gathered
+// once, every level reads and writes LDS, and the solution is scattered back
+// at the end; x overwrites b in place (a backward row reads its own b and
+// its neighbours' x only).  A thread's factor loads for a level are issued
+// before the barrier that closes the previous level, so the global latency
+// overlaps it.  M0 s NR <= 4096 complex values (64 KB): every level here
+// has at most 1024 rows of work, ((M0 + 1) / 2) s <= 1024.
+template <int NR, int S>
+__global__ __launch_bounds__(1024) void k_bcr_tail_lds(i64 N, int l0, int L, const i64 *__restrict__ off,
+                                                       const Cx<double> *__restrict__ F, const Cx<double> *b,
+                                                       Cx<double> *x, i64 vs)
+{
+    extern __shared__ char smem_raw[];
+    Cx<double> *lb = reinterpret_cast<Cx<double> *>(smem_raw);
+    __shared__ i64 loff[2 * 64 + 1];
+    constexpr int s = S;
+    constexpr i64 ss = (i64)S * S;
+    const int t = threadIdx.x;
+    const i64 stride0 = (i64)1 << l0;
+    const i64 M0 = (N + stride0 - 1) >> l0;
+    if (t <= 2 * L) loff[t] = off[t];
+    for (i64 e = t; e < M0 * s * NR; e += 1024) {
+        const int q = (int)(e / (M0 * s));
+        const i64 rem = e % (M0 * s), c = rem / s;
+        const int r = (int)(rem % s);
+        lb[e] = b[q * vs + c * stride0 * s + r];
+    }
+    auto L_ = [&](int q, i64 c, int r) -> Cx<double> & { return lb[((i64)q * M0 + c) * s + r]; };
+    __syncthreads();       /* loff and the gathered rows visible */
+    for (int l = l0; l < L; l++) {
+        const i64 h = (i64)1 << l, M = (N + h - 1) >> l, ne = (M + 1) / 2, hc = (i64)1 << (l - l0);
+        const i64 g = t;
+        const bool act = g < ne * s;
+        const i64 j = act ? g / s : 0;
+        const int r = (int)(g % s);
+        const i64 m = 2 * j;
+        Cx<double> al[S], ga[S];
+        if (act) {
+            const Cx<double> *AG = F + loff[2 * l] * ss + j * 2 * ss + r;
+#pragma unroll
+            for (int c = 0; c < S; c++) { al[c] = AG[(i64)c * s]; ga[c] = AG[ss + (i64)c * s]; }
+        }
+        __syncthreads();   /* the previous level's writes are done */
+        // barriers stay outside the divergent branches: a wave whose lanes
+        // split would otherwise execute each branch's barrier
+        Cx<double> acc[NR];
+        if (act) {
+#pragma unroll
+            for (int q = 0; q < NR; q++) acc[q] = L_(q, m * hc, r);
+            if (m >= 1) {
+#pragma unroll
+                for (int c = 0; c < S; c++)
+#pragma unroll
+                    for (int q = 0; q < NR; q++) cfms(acc[q], al[c], L_(q, (m - 1) * hc, c));
+            }
+            if (m + 1 < M) {
+#pragma unroll
+                for (int c = 0; c < S; c++)
+#pragma unroll
+                    for (int q = 0; q < NR; q++) cfms(acc[q], ga[c], L_(q, (m + 1) * hc, c));
+            }
+        }
+        __syncthreads();
+        if (act) {
+#pragma unroll
+            for (int q = 0; q < NR; q++) L_(q, m * hc, r) = acc[q];
+        }
+    }
+    __syncthreads();
+    {
+        Cx<double> dd[S];
+        if (t < s) {
+            const Cx<double> *D = F + loff[2 * L] * ss + t;
+#pragma unroll
+            for (int c = 0; c < S; c++) dd[c] = D[(i64)c * s];
+        }
+        Cx<double> acc[NR];
+        if (t < s) {
+#pragma unroll
+            for (int q = 0; q < NR; q++) acc[q] = cx0();
+#pragma unroll
+            for (int c = 0; c < S; c++)
+#pragma unroll
+                for (int q = 0; q < NR; q++) cfma(acc[q], dd[c], L_(q, 0, c));
+        }
+        __syncthreads();
+        if (t < s) {
+#pragma unroll
+            for (int q = 0; q < NR; q++) L_(q, 0, t) = acc[q];
+        }
+    }
+    for (int l = L - 1; l >= l0; l--) {
+        const i64 h = (i64)1 << l, M = (N + h - 1) >> l, no = M / 2, hc = (i64)1 << (l - l0);
+        const i64 g = t;
+        const bool act = g < no * s;
+        const i64 j = act ? g / s : 0;
+        const int r = (int)(g % s);
+        const i64 m = 2 * j + 1;
+        // D^-1 and P are loaded ahead of the barrier; Q after it (registers:
+        // three S-blocks of factors across a barrier exceed the 128 VGPRs of
+        // a 1024-thread block)
+        Cx<double> dv[S], pv[S];
+        const Cx<double> *D = F + loff[2 * l + 1] * ss + j * 3 * ss + r;
+        if (act) {
+#pragma unroll
+            for (int c = 0; c < S; c++) { dv[c] = D[(i64)c * s]; pv[c] = D[ss + (i64)c * s]; }
+        }
+        __syncthreads();
+        Cx<double> acc[NR];
+        if (act) {
+#pragma unroll
+            for (int q = 0; q < NR; q++) acc[q] = cx0();
+#pragma unroll
+            for (int c = 0; c < S; c++)
+#pragma unroll
+                for (int q = 0; q < NR; q++) {
+                    cfma(acc[q], dv[c], L_(q, m * hc, c));
+                    cfms(acc[q], pv[c], L_(q, (m - 1) * hc, c));
+                }
+            if (m + 1 < M) {
+#pragma unroll
+                for (int c = 0; c < S; c++) {
+                    const Cx<double> qq = D[2 * ss + (i64)c * s];
+#pragma unroll
+                    for (int q = 0; q < NR; q++) cfms(acc[q], qq, L_(q, (m + 1) * hc, c));
+                }
+            }
+        }
+        __syncthreads();
+        if (act) {
+#pragma unroll
+            for (int q = 0; q < NR; q++) L_(q, m * hc, r) = acc[q];
+        }
+    }
+    __syncthreads();
+    for (i64 e = t; e < M0 * s * NR; e += 1024) {
+        const int q = (int)(e / (M0 * s));
+        const i64 rem = e % (M0 * s), c = rem / s;
+        const int r = (int)(rem % s);
+        x[q * vs + c * stride0 * s + r] = lb[e];
+    }
+}
+
 // border, vector q: t = b2_q - dots_q; y2_q = Sinv t -> y2out + q ntau and the
 // tail of y_q.  One workgroup, NR * ntau threads used.
 template <int NR>
@@ -339,10 +483,45 @@ int bbjd_o_bcr_bwd(hipStream_t st, int s, int64_t M, int64_t h, int64_t no, cons
     return chk();
 }
 
+// The LDS tail holds the rows of its first level in at most 60 KB (with its
+// static LDS the block stays within 64 KB); it serves s = 4, 8 (s = 16 needs
+// more than a 1024-thread block's registers for its prefetched factor
+// blocks and keeps the global tail).
+static const size_t TAIL_LDS_MAX = 60 * 1024;
+static inline bool tail_lds_fits(int s, i64 N, int l, int nrhs)
+{
+    const i64 M = (N + ((i64)1 << l) - 1) >> l;
+    return (s == 4 || s == 8) && (size_t)M * s * nrhs * sizeof(Cx<double>) <= TAIL_LDS_MAX;
+}
+
+int bbjd_o_bcr_tail_level(int s, int64_t N, int l0, int L, int nrhs)
+{
+    if (s != 4 && s != 8) return l0;
+    for (int l = l0; l <= L; l++)
+        if (tail_lds_fits(s, N, l, nrhs)) return l;
+    return l0;
+}
+
 int bbjd_o_bcr_tail(hipStream_t st, int s, int64_t N, int l0, int L, const int64_t *off, const void *F, void *b,
                     void *x, int nrhs, int64_t vs)
 {
     if (s > 1024 || L > 64) return -2;
+    const i64 M0 = (N + ((i64)1 << l0) - 1) >> l0;
+    const size_t lds = (size_t)M0 * s * nrhs * sizeof(Cx<double>);
+    if (tail_lds_fits(s, N, l0, nrhs)) {
+#define BBJD_TAIL_LDS(NR, SS)                                                                                  \
+    hipLaunchKernelGGL((k_bcr_tail_lds<NR, SS>), dim3(1), dim3(1024), lds, st, (i64)N, l0, L, (const i64 *)off, \
+                       (const Cx<double> *)F, (const Cx<double> *)b, (Cx<double> *)x, (i64)vs)
+        if (nrhs == 2) {
+            if (s == 4) BBJD_TAIL_LDS(2, 4);
+            else BBJD_TAIL_LDS(2, 8);
+        } else {
+            if (s == 4) BBJD_TAIL_LDS(1, 4);
+            else BBJD_TAIL_LDS(1, 8);
+        }
+#undef BBJD_TAIL_LDS
+        return chk();
+    }
 #define BBJD_TAIL(NR, S)                                                                                        \
     hipLaunchKernelGGL((k_bcr_tail<NR, S>), dim3(1), dim3(1024), 0, st, s, (i64)N, l0, L, (const i64 *)off,       \
                        (const Cx<double> *)F, (Cx<double> *)b, (Cx<double> *)x, (i64)vs)

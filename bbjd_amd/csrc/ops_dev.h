@@ -14,6 +14,9 @@ int bbjd_o_bcr_fwd(hipStream_t st, int s, int64_t M, int64_t h, int64_t ne, cons
                    int64_t vs);
 int bbjd_o_bcr_bwd(hipStream_t st, int s, int64_t M, int64_t h, int64_t no, const void *DPQ, const void *b, void *x,
                    int nrhs, int64_t vs);
+/* first level of the one-workgroup tail for this shape (>= l0, where the
+ * LDS-resident tail fits; l0 when the global tail is used) */
+int bbjd_o_bcr_tail_level(int s, int64_t N, int l0, int L, int nrhs);
 int bbjd_o_bcr_tail(hipStream_t st, int s, int64_t N, int l0, int L, const int64_t *off, const void *F, void *b,
                     void *x, int nrhs, int64_t vs);
 int bbjd_o_border_small(hipStream_t st, int ntau, const void *b2_0, const void *b2_1, const void *dots0,

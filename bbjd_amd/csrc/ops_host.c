@@ -250,14 +250,15 @@ static int bcr_solve(struct bbjd_op *op, hipStream_t st, int nrhs)
     const size_t ss = (size_t)s * s;
     const int64_t vs = (int64_t)op->N * s;
     int rc = 0;
-    for (int l = 0; l < op->l0; l++) {
+    const int lt = bbjd_o_bcr_tail_level(s, op->N, op->l0, op->L, nrhs);
+    for (int l = 0; l < lt; l++) {
         const bbjd_int h = (bbjd_int)1 << l, M = (op->N + h - 1) >> l;
         rc |= bbjd_o_bcr_fwd(st, s, M, h, (M + 1) / 2, (const zcx *)op->bcr + (size_t)op->even_off[l] * ss,
                              op->bvec, nrhs, vs);
     }
-    rc |= bbjd_o_bcr_tail(st, s, op->N, op->l0, op->L, (const int64_t *)op->doff, op->bcr, op->bvec, op->xvec,
-                          nrhs, vs);
-    for (int l = op->l0 - 1; l >= 0; l--) {
+    rc |= bbjd_o_bcr_tail(st, s, op->N, lt, op->L, (const int64_t *)op->doff, op->bcr, op->bvec, op->xvec, nrhs,
+                          vs);
+    for (int l = lt - 1; l >= 0; l--) {
         const bbjd_int h = (bbjd_int)1 << l, M = (op->N + h - 1) >> l;
         rc |= bbjd_o_bcr_bwd(st, s, M, h, M / 2, (const zcx *)op->bcr + (size_t)op->odd_off[l] * ss, op->bvec,
                              op->xvec, nrhs, vs);

@@ -27,8 +27,17 @@
  *    reordering) and picks the target by the same rule, and
  *    bbjd_ritz_vector takes the Ritz vector as the null vector of
  *    tb MA - ta MB by inverse iteration -- in exact arithmetic the vector
- *    the Schur back-substitution gives, up to scale.  About 40% of the QZ
- *    step's cost.
+ *    the Schur back-substitution gives, up to scale;
+ *  - and those eigenvalues come from a standard eigenproblem where the
+ *    harmonic test space allows: W is the orthonormal factor of
+ *    nu0 AV + mu0 BV (CGS2 column by column, or Cholesky QR at a restart),
+ *    so C = nu0 MA + mu0 MB = W^H (nu0 AV + mu0 BV) is upper triangular up
+ *    to the loss of orthogonality of W, and MA x = theta MB x is
+ *    C^-1 MB x = x / (nu0 theta + mu0) (nu0 != 0) or MB^-1 MA x = theta x
+ *    (nu0 = 0, C = MB).  Hessenberg QR of that k x k matrix (ZGEHRD +
+ *    ZHSEQR) costs about half of the generalized QZ (ZGGEV).  When C is not
+ *    triangular to 1e-12 relative (the w <- Av breakdown fallback) or has a
+ *    zero diagonal entry, ZGGEV is used.
  */
 #include "qz_host.h"
 #include "lapack_host.h"
@@ -70,14 +79,26 @@ int bbjd_qz_init(bbjd_qz *q, bbjd_int maxsub)
     scipy_zggev_("N", "N", &bk, A, &lda, B, &lda, q->cal, q->cbe, NULL, &bk, NULL, &bk, &wq, &lw, q->rwork, &info);
     tl = (blas_int)creal(wq) + 1;
     if (tl > q->lwork) q->lwork = tl;
+    {
+        blas_int ilo = 1, ihi = bk;
+        lw = -1;
+        scipy_zgehrd_(&bk, &ilo, &ihi, A, &lda, q->cal, &wq, &lw, &info);
+        tl = (blas_int)creal(wq) + 1;
+        if (tl > q->lwork) q->lwork = tl;
+        lw = -1;
+        scipy_zhseqr_("E", "N", &bk, &ilo, &ihi, A, &lda, q->cal, NULL, &one, &wq, &lw, &info);
+        tl = (blas_int)creal(wq) + 1;
+        if (tl > q->lwork) q->lwork = tl;
+    }
     q->work = malloc((size_t)q->lwork * sizeof(zcplx));
     q->eal = malloc((size_t)maxsub * sizeof(zcplx)); q->ebe = malloc((size_t)maxsub * sizeof(zcplx));
+    q->tri = malloc(ss * sizeof(zcplx));
     q->lu = malloc(ss * sizeof(zcplx)); q->rhs = malloc((size_t)maxsub * sizeof(zcplx));
     q->ipiv = malloc((size_t)maxsub * sizeof(blas_int));
     q->iwork = malloc((size_t)q->liwork * sizeof(blas_int));
     free(A); free(B);
     if (!q->cSA || !q->cSB || !q->cZA || !q->cZB || !q->cal || !q->cbe || !q->rwork || !q->bwork ||
-        !q->sel || !q->work || !q->iwork || !q->eal || !q->ebe || !q->lu || !q->rhs || !q->ipiv) {
+        !q->sel || !q->work || !q->iwork || !q->eal || !q->ebe || !q->lu || !q->rhs || !q->ipiv || !q->tri) {
         bbjd_qz_free(q);
         return -1;
     }
@@ -88,7 +109,7 @@ void bbjd_qz_free(bbjd_qz *q)
 {
     free(q->cSA); free(q->cSB); free(q->cZA); free(q->cZB); free(q->cal); free(q->cbe);
     free(q->rwork); free(q->bwork); free(q->sel); free(q->work); free(q->iwork);
-    free(q->eal); free(q->ebe); free(q->lu); free(q->rhs); free(q->ipiv);
+    free(q->eal); free(q->ebe); free(q->lu); free(q->rhs); free(q->ipiv); free(q->tri);
     memset(q, 0, sizeof(*q));
 }
 
@@ -203,27 +224,75 @@ void bbjd_ritz_coefficients(bbjd_int k, bbjd_int ld, bbjd_int p, zcplx ta, zcplx
     scipy_zgemv_("N", &bk, &bp1, &a1, ZB, &ldz, y, &one, &b0, x, &one);
 }
 
+/* Eigenvalues of the projected pencil through the standard eigenproblem of
+ * the harmonic form (see the file comment); returns 1 when C is not
+ * triangular enough to use it.  alpha / beta = theta. */
+static int eigs_harmonic(bbjd_qz *q, bbjd_int k, const zcplx *MA, const zcplx *MB, zcplx nu0, zcplx mu0)
+{
+    const bbjd_int ld = q->maxsub;
+    const int inv = cabs(nu0) > 0;               /* C = nu0 MA + mu0 MB; else C = MB */
+    zcplx *C = q->tri, *M = q->lu;
+    double cmax = 0, lmax = 0, dmin = INFINITY;
+    for (bbjd_int j = 0; j < k; j++)
+        for (bbjd_int i = 0; i < k; i++) {
+            const zcplx c = inv ? nu0 * MA[i + j * ld] + mu0 * MB[i + j * ld] : MB[i + j * ld];
+            C[i + j * k] = c;
+            const double a = cabs(c);
+            if (a > cmax) cmax = a;
+            if (i > j && a > lmax) lmax = a;
+            if (i == j && a < dmin) dmin = a;
+        }
+    if (!(cmax > 0) || lmax > 1e-12 * cmax || !(dmin > 1e-300)) return 1;
+    /* M = C^-1 MB (or MB^-1 MA), the strictly lower part of C taken as 0 */
+    for (bbjd_int j = 0; j < k; j++)
+        for (bbjd_int i = 0; i < k; i++) M[i + j * k] = inv ? MB[i + j * ld] : MA[i + j * ld];
+    blas_int bk = (blas_int)k, ilo = 1, ihi = bk, one = 1, lw = q->lwork, info = 0;
+    const zcplx a1 = 1.0;
+    scipy_ztrsm_("L", "U", "N", "N", &bk, &bk, &a1, C, &bk, M, &bk);
+    scipy_zgehrd_(&bk, &ilo, &ihi, M, &bk, q->rhs, q->work, &lw, &info);
+    if (info) return 1;
+    scipy_zhseqr_("E", "N", &bk, &ilo, &ihi, M, &bk, q->eal, NULL, &one, q->work, &lw, &info);
+    if (info) return 1;
+    for (bbjd_int i = 0; i < k; i++) {
+        const zcplx mu = q->eal[i];
+        if (!isfinite(creal(mu)) || !isfinite(cimag(mu))) return 1;
+        if (inv) {
+            /* mu = 1 / (nu0 theta + mu0): theta = (1 - mu0 mu) / (nu0 mu) */
+            q->eal[i] = 1.0 - mu0 * mu;
+            q->ebe[i] = nu0 * mu;
+        } else {
+            q->ebe[i] = 1.0;
+        }
+    }
+    q->std_calls++;
+    return 0;
+}
+
 /* Eigenvalues of the projected pencil and the target position without the
- * Schur form: ZGGEV('N', 'N') on copies of (MA, MB), cached while the pencil
- * is unchanged (lock iterations); the target is the best unconverged value
- * by key over all positions, as in bbjd_qz_solve. */
-int bbjd_qz_target(bbjd_qz *q, bbjd_int k, const zcplx *MA, const zcplx *MB, zcplx *al, zcplx *be, zcplx sigma,
-                   bbjd_target tgt, const zcplx *conv, bbjd_int nconv, double tol, bbjd_int *tpos)
+ * Schur form, cached while the pencil is unchanged (lock iterations): the
+ * harmonic standard eigenproblem, else ZGGEV('N', 'N') on copies of
+ * (MA, MB).  The target is the best unconverged value by key over all
+ * positions, as in bbjd_qz_solve. */
+int bbjd_qz_target(bbjd_qz *q, bbjd_int k, const zcplx *MA, const zcplx *MB, zcplx nu0, zcplx mu0, zcplx *al,
+                   zcplx *be, zcplx sigma, bbjd_target tgt, const zcplx *conv, bbjd_int nconv, double tol,
+                   bbjd_int *tpos)
 {
     const bbjd_int ld = q->maxsub;
     const size_t colb = (size_t)k * sizeof(zcplx);
     if (!(q->ecache_valid && q->ecache_k == k)) {
-        zcplx *A = q->cSA, *B = q->cSB;       /* scratch: the ZGGES cache is invalid here */
-        for (bbjd_int j = 0; j < k; j++) {
-            memcpy(A + j * ld, MA + j * ld, colb);
-            memcpy(B + j * ld, MB + j * ld, colb);
+        if (eigs_harmonic(q, k, MA, MB, nu0, mu0)) {
+            zcplx *A = q->cSA, *B = q->cSB;   /* scratch: the ZGGES cache is invalid here */
+            for (bbjd_int j = 0; j < k; j++) {
+                memcpy(A + j * ld, MA + j * ld, colb);
+                memcpy(B + j * ld, MB + j * ld, colb);
+            }
+            blas_int bk = (blas_int)k, lda = (blas_int)ld, one = 1, lw = q->lwork, info = 0;
+            scipy_zggev_("N", "N", &bk, A, &lda, B, &lda, q->eal, q->ebe, NULL, &one, NULL, &one, q->work, &lw,
+                         q->rwork, &info);
+            if (info) return -1;
+            q->ggev_calls++;
         }
         q->cache_valid = 0;
-        blas_int bk = (blas_int)k, lda = (blas_int)ld, one = 1, lw = q->lwork, info = 0;
-        scipy_zggev_("N", "N", &bk, A, &lda, B, &lda, q->eal, q->ebe, NULL, &one, NULL, &one, q->work, &lw,
-                     q->rwork, &info);
-        if (info) return -1;
-        q->ggev_calls++;
         q->ecache_valid = 1;
         q->ecache_k = k;
     }

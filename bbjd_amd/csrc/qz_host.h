@@ -19,9 +19,9 @@ typedef struct {
     /* eigenvalue-only step: its own cache, LU workspace */
     int       ecache_valid;
     bbjd_int  ecache_k;
-    zcplx    *eal, *ebe, *lu, *rhs;
+    zcplx    *eal, *ebe, *lu, *rhs, *tri;
     blas_int *ipiv;
-    long      ggev_calls;
+    long      ggev_calls, std_calls;
 } bbjd_qz;
 
 int  bbjd_qz_init(bbjd_qz *q, bbjd_int maxsub);
@@ -30,8 +30,9 @@ static inline void bbjd_qz_invalidate(bbjd_qz *q) { q->cache_valid = 0; q->ecach
 int  bbjd_qz_solve(bbjd_qz *q, bbjd_int k, const zcplx *MA, const zcplx *MB, zcplx *SA, zcplx *SB,
                    zcplx *ZA, zcplx *ZB, zcplx *al, zcplx *be, zcplx sigma, bbjd_target tgt,
                    bbjd_int nkeep, const zcplx *conv, bbjd_int nconv, double tol, bbjd_int *tpos);
-int  bbjd_qz_target(bbjd_qz *q, bbjd_int k, const zcplx *MA, const zcplx *MB, zcplx *al, zcplx *be, zcplx sigma,
-                    bbjd_target tgt, const zcplx *conv, bbjd_int nconv, double tol, bbjd_int *tpos);
+int  bbjd_qz_target(bbjd_qz *q, bbjd_int k, const zcplx *MA, const zcplx *MB, zcplx nu0, zcplx mu0, zcplx *al,
+                    zcplx *be, zcplx sigma, bbjd_target tgt, const zcplx *conv, bbjd_int nconv, double tol,
+                    bbjd_int *tpos);
 int  bbjd_ritz_vector(bbjd_qz *q, bbjd_int k, const zcplx *MA, const zcplx *MB, zcplx ta, zcplx tb, zcplx *x);
 void bbjd_ritz_coefficients(bbjd_int k, bbjd_int ld, bbjd_int p, zcplx ta, zcplx tb, const zcplx *SA,
                             const zcplx *SB, const zcplx *ZB, zcplx *y, zcplx *x);

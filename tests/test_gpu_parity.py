@@ -56,7 +56,7 @@ def test_cgs2_every_row_tile(proj_blocks, rows):
     lib = _lib.load()
     olib = orc.load_oracle()
     try:
-        for n, k, fuse in ((40000 + 77, 33, 48), (3000 + 5, 9, 48), (3000 + 5, 9, 0)):
+        for n, k, fuse in ((40000 + 77, 33, 48), (3000 + 5, 9, 48), (3000 + 5, 9, 0), (3000 + 5, 9, 16)):
             assert lib.bbjd_set_tiles(proj_blocks, rows, 0) == 0
             lib.bbjd_set_fuse_blocks(fuse)
             rng = np.random.default_rng(n + k + rows)
@@ -73,7 +73,7 @@ def test_cgs2_every_row_tile(proj_blocks, rows):
             assert abs(ng - no) <= 1e-12 * scale, (n, k, fuse)
     finally:
         lib.bbjd_set_tiles(192, 0, 2048)
-        lib.bbjd_set_fuse_blocks(48)
+        lib.bbjd_set_fuse_blocks(16)
 
 
 # super-block sizes hw * bs = 4, 16, 8, 8 (compiled loops) and 12 (generic)
