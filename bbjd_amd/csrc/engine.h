@@ -82,7 +82,7 @@ int bbjd_k_gemv_c(hipStream_t s, int64_t n, int k, const void *Q, int64_t ldq,
 int bbjd_k_gemv_c_c(hipStream_t s, int64_t n, int k, const void *Q, int64_t ldq,
                     const void *x, const void *x2, void *out, void *out2, bbjd_red *rd);
 
-/* y = Q[:, :k] c (c device, k entries) and scal[slot] = ||y||. */
+/* y = Q[:, :k] c (c device, k entries) and, with slot >= 0, scal[slot] = ||y||. */
 int bbjd_k_gemv_n(hipStream_t s, int64_t n, int k, const void *Q, int64_t ldq,
                   const void *c, void *y, bbjd_red *rd, int slot);
 

@@ -80,8 +80,10 @@ typedef struct {
     size_t total;
 } layout_t;
 
+/* O_PEN directly precedes O_SCAL: the pencil column and the scalar slots
+ * come back to the host in one copy */
 enum { O_V, O_W, O_AV, O_BV, O_QR, O_QL, O_GV, O_T1, O_T2, O_T3, O_T4, O_HW, O_PART, O_PNORM, O_H, O_H2,
-       O_HSUM, O_SCAL, O_FLAG, O_CNT, O_COEF, O_COEF2, O_ZR, O_PEN, O_GRAM, O_GVC, O_QRC, O_QLC, O_TC0, O_TC1,
+       O_HSUM, O_PEN, O_SCAL, O_FLAG, O_CNT, O_COEF, O_COEF2, O_ZR, O_GRAM, O_GVC, O_QRC, O_QLC, O_TC0, O_TC1,
        O_TC2, O_END };
 
 static bbjd_int auto_maxsub(const bbjd_params *p)
@@ -259,7 +261,9 @@ bbjd_workspace *bbjd_workspace_alloc_gpu(const bbjd_params *p, const bbjd_gpu_op
     d->yv = malloc((size_t)(L.maxsub + L.ims + 2) * sizeof(zcx));
     d->xv = malloc((size_t)(L.maxsub + 2) * sizeof(zcx));
     if (!HIPOK(hipHostMalloc((void **)&d->zrh, ss * sizeof(zcx), 0))) { d->zrh = NULL; goto fail; }
-    if (!HIPOK(hipHostMalloc((void **)&d->hcol, (size_t)(L.maxsub + L.ims + 8) * 4 * sizeof(zcx), 0))) {
+    /* pencil column/row staging + the scalar slots after their 256-byte
+     * aligned offset (pencil_update) */
+    if (!HIPOK(hipHostMalloc((void **)&d->hcol, (size_t)(L.maxsub + L.ims + 8) * 4 * sizeof(zcx) + 512, 0))) {
         d->hcol = NULL;
         goto fail;
     }
@@ -633,11 +637,13 @@ static int pencil_update(ctx_t *c, bbjd_int k, int rowrel)
             rc |= bbjd_k_gemv_c(c->s, c->n, (int)k, ws->BV, c->ld, COLZ(ws->W, k), NULL, pen + 3 * (ms + 1), NULL,
                                 &d->rd);
     }
-    rc |= !HIPOK(hipMemcpyAsync(d->hcol, pen, (size_t)4 * (ms + 1) * 16, hipMemcpyDeviceToHost, c->s));
-    /* the norms of the expansion (breakdown checks) come back with it */
-    rc |= !HIPOK(hipMemcpyAsync(d->hscal, d->rd.scal, SL_COUNT * 8, hipMemcpyDeviceToHost, c->s));
+    /* one copy: the pencil column/row buffer and, right after it, the scalar
+     * slots (the norms of the expansion, for the breakdown checks) */
+    const size_t gap = (size_t)((const char *)d->rd.scal - (const char *)pen);
+    rc |= !HIPOK(hipMemcpyAsync(d->hcol, pen, gap + SL_COUNT * 8, hipMemcpyDeviceToHost, c->s));
     rc |= !HIPOK(hipStreamSynchronize(c->s));
     if (rc) return -1;
+    memcpy(d->hscal, (const char *)d->hcol + gap, SL_COUNT * 8);
     zcx *MA = (zcx *)ws->MA, *MB = (zcx *)ws->MB;
     const zcx *ca = d->hcol, *cb = d->hcol + (ms + 1), *ra = d->hcol + 2 * (ms + 1), *rb = d->hcol + 3 * (ms + 1);
     for (bbjd_int i = 0; i <= k; i++) { MA[i + k * ms] = ca[i]; MB[i + k * ms] = cb[i]; }
@@ -813,7 +819,7 @@ static int expand(ctx_t *c, bbjd_int k, const void *z, zcx nu0, zcx mu0)
         rc |= bbjd_k_cgs2(c->s, n, (int)k, ws->W, c->ld, wn, &d->rd, SL_W, NULL);
         rc |= bbjd_k_scale_inv(c->s, n, wn, d->rd.scal, SL_W, 1);
         if (rc || pencil_update(c, k, !c->w_mixed)) return -2;
-        d->st.stream_bytes += 16.0 * n * (4.0 * k + 4 + 3 + 4.0 * k + 2);
+        d->st.stream_bytes += 16.0 * n * (2.0 * k + 4 + 3 + 2.0 * k + 2);
         const double nv = d->hscal[SL_V], nw = d->hscal[SL_W];
         if (nv >= 1e-14 && nw >= 1e-14) return 0;
         if (nv >= 1e-14) {
@@ -850,7 +856,7 @@ static int expand(ctx_t *c, bbjd_int k, const void *z, zcx nu0, zcx mu0)
         rc |= bbjd_k_cgs2(c->s, n, (int)k, ws->W, c->ld, wn, &d->rd, SL_W, NULL);
     }
     rc |= bbjd_k_scale_inv(c->s, n, wn, d->rd.scal, SL_W, 1);
-    d->st.stream_bytes += 16.0 * n * (4.0 * k + 4 + 3 + 4.0 * k + 2);
+    d->st.stream_bytes += 16.0 * n * (2.0 * k + 4 + 3 + 2.0 * k + 2);
     if (w_fallback) c->w_mixed = 1;
     if (rc || pencil_update(c, k, !c->w_mixed)) return -2;
     return 0;
@@ -960,17 +966,22 @@ static bbjd_status solve_impl(const bbjd_params *P, bbjd_workspace *ws, bbjd_res
             d->st.qz_seconds += now_s() - tq;
             if (qrc) { st = BBJD_ERR_LAPACK; goto done; }
         }
-        /* ---- step 2: Ritz vector u = V x on the device ---- */
+        /* ---- step 2: Ritz vector u = V x / ||V x|| on the device ---- */
+        /* V is orthonormal, so ||V x|| = ||x||: x is normalised on the host
+         * and u needs no norm pass and no scaling pass */
         void *u = ws->tmp1, *Au = ws->tmp2, *Bu = ws->tmp3, *r = ws->tmp4;
+        double xnrm = 0;
+        for (bbjd_int i = 0; i < k; i++) xnrm += creal(d->xv[i]) * creal(d->xv[i]) + cimag(d->xv[i]) * cimag(d->xv[i]);
+        xnrm = sqrt(xnrm);
+        if (!(xnrm > 1e-300)) { st = BBJD_ERR_BREAKDOWN; goto done; }
+        for (bbjd_int i = 0; i < k; i++) d->xv[i] /= xnrm;
         rc |= upload_small(c, d->rd.coef, d->xv, k);
-        rc |= bbjd_k_gemv_n(c->s, n, (int)k, ws->V, c->ld, d->rd.coef, u, &d->rd, SL_U);
-        rc |= bbjd_k_scale_inv(c->s, n, u, d->rd.scal, SL_U, 1);
+        rc |= bbjd_k_gemv_n(c->s, n, (int)k, ws->V, c->ld, d->rd.coef, u, &d->rd, -1);
         /* ---- step 3: residual ---- */
         if (apply_op(c, P->matvec_A, u, Au) || apply_op(c, P->matvec_B, u, Bu)) goto cb_err;
         rc |= bbjd_k_lincomb_norm(c->s, n, creal(tb), cimag(tb), Au, -creal(ta), -cimag(ta), Bu, r, &d->rd, SL_R);
         if (rc || sync_scal(c)) { st = BBJD_ERR_ALLOC; goto done; }
-        d->st.stream_bytes += 16.0 * n * (k + 1 + 1 + 3);
-        if (d->hscal[SL_U] < 1e-300) { st = BBJD_ERR_BREAKDOWN; goto done; }
+        d->st.stream_bytes += 16.0 * n * (k + 1 + 3);
         const double den = cabs(ta) + cabs(tb);
         const double rel = d->hscal[SL_R] / (den > 1e-300 ? den : 1.0);
         const zcx lam = cabs(tb) > 1e-300 ? ta / tb : ta;
@@ -1059,6 +1070,10 @@ done:
         if (HIPOK(hipMemcpyAsync(flags, d->rd.flag, sizeof(flags), hipMemcpyDeviceToHost, c->s)) &&
             HIPOK(hipStreamSynchronize(c->s)))
             d->st.dgks_second_passes = flags[FL_DGKS_COUNT];
+        /* second CGS passes: 2 k columns each, at the mean subspace size */
+        if (d->st.outer_iters > 0)
+            d->st.stream_bytes += 16.0 * n * 2.0 * d->st.dgks_second_passes *
+                                  ((double)d->st.sum_k / (double)d->st.outer_iters);
     }
     d->st.solve_seconds = now_s() - t_start;
     R->status = st;

@@ -500,9 +500,10 @@ __global__ __launch_bounds__(TPB) void k_lincomb(i64 n, R are, R aim, const Cx<R
     if (threadIdx.x == 0) *cnt = 0;
 }
 
-// SvdV coefficient from the two dots d = (u^H z, u^H s): coef = -d0/d1 when
-// |d1| > 1e-300 (reference src/jd_core.c:966-971), else 0.
-__global__ void k_svdv_coef(Cx<double> *d)
+// Sleijpen-van der Vorst step z += c s, with c = -(u^H z)/(u^H s) when
+// |u^H s| > 1e-300, else 0 (reference src/jd_core.c:966-971), computed by
+// every thread from the two dots d = (u^H z, u^H s).
+__global__ __launch_bounds__(TPB) void k_axpy_svdv(i64 n, const Cx<double> *d, const Cx<double> *s, Cx<double> *z)
 {
     const Cx<double> uz = d[0], us = d[1];
     const double a = hypot(us.re, us.im);
@@ -511,13 +512,6 @@ __global__ void k_svdv_coef(Cx<double> *d)
         const double den = us.re * us.re + us.im * us.im;
         c = cx(-(uz.re * us.re + uz.im * us.im) / den, -(uz.im * us.re - uz.re * us.im) / den);
     }
-    d[2] = c;
-}
-
-__global__ __launch_bounds__(TPB) void k_axpy_dev(i64 n, const Cx<double> *coef, const Cx<double> *s,
-                                                  Cx<double> *z)
-{
-    const Cx<double> c = coef[0];
     for (i64 i = (i64)blockIdx.x * TPB + threadIdx.x; i < n; i += (i64)gridDim.x * TPB) {
         Cx<double> v = z[i];
         cfma(v, s[i], c);
@@ -881,7 +875,8 @@ template <typename R>
 static int gemv_n(hipStream_t s, i64 n, int k, const void *Q, i64 ldq, const void *c, void *y, bbjd_red *rd,
                   int slot)
 {
-    return update_pass<R>(s, n, k, Q, ldq, c, (R)1, 0, y, rd, fin_norm(rd, 0, slot), nullptr);
+    return update_pass<R>(s, n, k, Q, ldq, c, (R)1, 0, y, rd, slot >= 0 ? fin_norm(rd, 0, slot) : fin_none(),
+                          nullptr);
 }
 
 template <typename R>
@@ -1023,8 +1018,7 @@ int bbjd_k_svdv(hipStream_t s, int64_t n, const void *u, void *z, const void *sv
 {
     Cx<double> *d = (Cx<double> *)rd->coef2;
     int rc = proj_pass<double>(s, n, 1, u, n, z, sv, d, d + 1, rd, -1, nullptr);
-    hipLaunchKernelGGL(k_svdv_coef, dim3(1), dim3(1), 0, s, d);
-    hipLaunchKernelGGL(k_axpy_dev, dim3(grid_stride_blocks(n)), dim3(TPB), 0, s, (i64)n, (const Cx<double> *)(d + 2),
+    hipLaunchKernelGGL(k_axpy_svdv, dim3(grid_stride_blocks(n)), dim3(TPB), 0, s, (i64)n, (const Cx<double> *)d,
                        (const Cx<double> *)sv, (Cx<double> *)z);
     return rc | chk();
 }

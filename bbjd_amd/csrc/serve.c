@@ -125,7 +125,10 @@ static void run_request(server_t *S, worker_t *w, bbjd_request *q)
         if (q->pair_residuals) bbjd_workspace_pair_residuals(w->ws, q->pair_residuals);
     }
     bbjd_stats stt;
-    if (bbjd_workspace_stats(w->ws, &stt) == 0) q->qz_seconds = stt.qz_seconds;
+    if (bbjd_workspace_stats(w->ws, &stt) == 0) {
+        q->qz_seconds = stt.qz_seconds;
+        q->stream_bytes = stt.stream_bytes;
+    }
     bbjd_result_free(&R);
     if (own) bbjd_op_free(op);
     q->t_done = now_s() - S->t0;

@@ -26,7 +26,8 @@ class Request(C.Structure):
                 ("residuals", C.c_void_p), ("pair_residuals", C.c_void_p),
                 ("eigenvectors", C.c_void_p), ("nconv", C.c_int64), ("iters", C.c_int64),
                 ("status", C.c_int), ("worker", C.c_int), ("t_admit", C.c_double),
-                ("t_start", C.c_double), ("t_done", C.c_double), ("qz_seconds", C.c_double)]
+                ("t_start", C.c_double), ("t_done", C.c_double), ("qz_seconds", C.c_double),
+                ("stream_bytes", C.c_double)]
 
 
 class ServeConfig(C.Structure):
@@ -120,7 +121,8 @@ def serve(jobs, workers=16, precision="fp64", realtime=True, hbm_budget=0, keep_
         o = outs[i]
         r = {"nconv": nc, "iters": int(q.iters), "status": int(q.status), "worker": int(q.worker),
              "arrival": q.arrival, "t_admit": q.t_admit, "t_start": q.t_start, "t_done": q.t_done,
-             "qz_seconds": q.qz_seconds, "eigenvalues": o["eigenvalues"][:nc].copy(),
+             "qz_seconds": q.qz_seconds, "stream_bytes": q.stream_bytes,
+             "eigenvalues": o["eigenvalues"][:nc].copy(),
              "residuals": o["residuals"][:nc].copy(),
              "pair_residuals": o["pair_residuals"][:nc].copy()}
         if keep_vectors:

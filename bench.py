@@ -242,7 +242,8 @@ def main():
         q_mix = sum(mix[i]["qz_seconds"] for i in sample)
         busy = sum(r["t_done"] - r["t_start"] for r in mix)
         log(f"[bench] sample solve seconds alone {s_alone:.1f} / in the mix {s_mix:.1f} "
-            f"(qz {q_alone:.1f} / {q_mix:.1f}); mean solves in flight {busy / walls[0]:.1f}")
+            f"(qz {q_alone:.1f} / {q_mix:.1f}); mean solves in flight {busy / walls[0]:.1f}; "
+            f"engine stream {sum(r['stream_bytes'] for r in mix) / walls[0] / 1e9:.0f} GB/s over the step")
     log(f"[bench] plain: {len(sample)} solves in {len(per_cls)} classes, {plain_pairs} pairs / {t_plain:.1f}s; "
         f"class-weighted estimate for the workload {plain_rate:.2f} pairs/s; "
         f"max rel eig diff plain vs served {diff}")

@@ -39,6 +39,8 @@ typedef struct {
     int                     worker;
     double                  t_admit, t_start, t_done;   /* seconds after start */
     double                  qz_seconds;
+    double                  stream_bytes;  /* algorithmic HBM bytes of the engine's
+                                              n-sized kernels (bbjd_stats) */
 } bbjd_request;
 
 typedef struct {
