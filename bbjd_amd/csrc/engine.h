@@ -36,13 +36,16 @@ enum {
     SL_T0, SL_T1,    /* scratch                                         */
     SL_COUNT = 16
 };
-/* Int flag slots. */
-enum { FL_PASS2 = 0, FL_ONE = 1, FL_DGKS_COUNT = 2, FL_COUNT = 8 };
+/* Int flag slots.  FL_RESCALE: the fused CGS2 could not normalise its result
+ * in the second pass and left it to a conditional scaling. */
+enum { FL_PASS2 = 0, FL_ONE = 1, FL_DGKS_COUNT = 2, FL_RESCALE = 3, FL_COUNT = 8 };
 
 /* Counters of the last-block-done reductions: [0, BBJD_CNT_UPD) one per
- * column tile of a projection, then one for updates, one for norms. */
+ * column tile of a projection, then one for updates, one for norms, one for
+ * the middle pass of the fused CGS2. */
 #define BBJD_CNT_UPD 1024
 #define BBJD_CNT_VEC 1025
+#define BBJD_CNT_MID 1026
 #define BBJD_CNT_SLOTS 1032
 
 /* Per-workspace scratch for the two-stage reductions. */
@@ -68,6 +71,11 @@ typedef struct {
  * k == 0: only the norm. */
 int bbjd_k_cgs2(hipStream_t s, int64_t n, int k, const void *Q, int64_t ldq, void *v,
                 bbjd_red *rd, int out_slot, void *hsum);
+
+/* bbjd_k_cgs2, then v /= ||v|| when ||v|| > 1e-300 (scal[out_slot] keeps the
+ * norm before the scaling). */
+int bbjd_k_cgs2_unit(hipStream_t s, int64_t n, int k, const void *Q, int64_t ldq, void *v,
+                     bbjd_red *rd, int out_slot);
 
 /* v -= Q Q^H v twice (no norm test), k columns. */
 int bbjd_k_project_out(hipStream_t s, int64_t n, int k, const void *Q, int64_t ldq, void *v,

@@ -423,11 +423,9 @@ static int deflation_append(ctx_t *c, bbjd_int j, const void *u, const void *Au,
     int rc = 0;
     void *qr = COLZ(ws->Q_right, j), *ql = COLZ(ws->Q_left, j);
     rc |= !HIPOK(hipMemcpyAsync(qr, u, (size_t)c->n * 16, hipMemcpyDeviceToDevice, c->s));
-    rc |= bbjd_k_cgs2(c->s, c->n, (int)j, ws->Q_right, c->ld, qr, &d->rd, SL_Q, NULL);
-    rc |= bbjd_k_scale_inv(c->s, c->n, qr, d->rd.scal, SL_Q, 1);
+    rc |= bbjd_k_cgs2_unit(c->s, c->n, (int)j, ws->Q_right, c->ld, qr, &d->rd, SL_Q);
     rc |= bbjd_k_lincomb(c->s, c->n, creal(nu0), cimag(nu0), Au, creal(mu0), cimag(mu0), Bu, ql);
-    rc |= bbjd_k_cgs2(c->s, c->n, (int)j, ws->Q_left, c->ld, ql, &d->rd, SL_Q, NULL);
-    rc |= bbjd_k_scale_inv(c->s, c->n, ql, d->rd.scal, SL_Q, 1);
+    rc |= bbjd_k_cgs2_unit(c->s, c->n, (int)j, ws->Q_left, c->ld, ql, &d->rd, SL_Q);
     if (d->opt.precision == BBJD_PREC_MIXED) {
         rc |= bbjd_k_z2c(c->s, c->n, qr, COLC(d->qrc, j));
         rc |= bbjd_k_z2c(c->s, c->n, ql, COLC(d->qlc, j));
@@ -738,8 +736,7 @@ static int thick_restart(ctx_t *c, bbjd_int k, bbjd_int keep, zcx nu0, zcx mu0)
         d->st.restart_fallbacks++;
         for (bbjd_int j = 0; j < keep; j++) {
             void *wj = COLZ(ws->W, j);
-            rc |= bbjd_k_cgs2(c->s, n, (int)j, ws->W, c->ld, wj, &d->rd, SL_T0, NULL);
-            rc |= bbjd_k_scale_inv(c->s, n, wj, d->rd.scal, SL_T0, 1);
+            rc |= bbjd_k_cgs2_unit(c->s, n, (int)j, ws->W, c->ld, wj, &d->rd, SL_T0);
         }
     }
     zcx *G = (zcx *)d->gram, *hg = d->hgram;
@@ -809,15 +806,13 @@ static int expand(ctx_t *c, bbjd_int k, const void *z, zcx nu0, zcx mu0)
      * it with a random vector, so z is not needed after CGS2 */
     const int in_place = z == vn;
     if (!in_place) rc |= !HIPOK(hipMemcpyAsync(vn, z, (size_t)n * 16, hipMemcpyDeviceToDevice, c->s));
-    rc |= bbjd_k_cgs2(c->s, n, (int)k, ws->V, c->ld, vn, &d->rd, SL_V, NULL);
     double nrm;
     if (optimistic) {
-        rc |= bbjd_k_scale_inv(c->s, n, vn, d->rd.scal, SL_V, 1);
+        rc |= bbjd_k_cgs2_unit(c->s, n, (int)k, ws->V, c->ld, vn, &d->rd, SL_V);
         if (rc) return -2;
         if (apply_op(c, P->matvec_A, vn, an) || apply_op(c, P->matvec_B, vn, bn)) return -1;
         rc |= bbjd_k_lincomb(c->s, n, creal(nu0), cimag(nu0), an, creal(mu0), cimag(mu0), bn, wn);
-        rc |= bbjd_k_cgs2(c->s, n, (int)k, ws->W, c->ld, wn, &d->rd, SL_W, NULL);
-        rc |= bbjd_k_scale_inv(c->s, n, wn, d->rd.scal, SL_W, 1);
+        rc |= bbjd_k_cgs2_unit(c->s, n, (int)k, ws->W, c->ld, wn, &d->rd, SL_W);
         if (rc || pencil_update(c, k, !c->w_mixed)) return -2;
         d->st.stream_bytes += 16.0 * n * (2.0 * k + 4 + 3 + 2.0 * k + 2);
         const double nv = d->hscal[SL_V], nw = d->hscal[SL_W];
@@ -826,14 +821,14 @@ static int expand(ctx_t *c, bbjd_int k, const void *z, zcx nu0, zcx mu0)
             /* only w broke down: v, Av, Bv stand; w from Av (the reference's
              * fallback), then the pencil again */
             rc |= !HIPOK(hipMemcpyAsync(wn, an, (size_t)n * 16, hipMemcpyDeviceToDevice, c->s));
-            rc |= bbjd_k_cgs2(c->s, n, (int)k, ws->W, c->ld, wn, &d->rd, SL_W, NULL);
-            rc |= bbjd_k_scale_inv(c->s, n, wn, d->rd.scal, SL_W, 1);
+            rc |= bbjd_k_cgs2_unit(c->s, n, (int)k, ws->W, c->ld, wn, &d->rd, SL_W);
             c->w_mixed = 1;
             if (rc || pencil_update(c, k, 0)) return -2;
             return 0;
         }
         nrm = nv;           /* v broke down: the random-vector branch below */
     } else {
+        rc |= bbjd_k_cgs2(c->s, n, (int)k, ws->V, c->ld, vn, &d->rd, SL_V, NULL);
         if (rc || sync_scal(c)) return -2;
         nrm = d->hscal[SL_V];
     }
@@ -1070,10 +1065,12 @@ done:
         if (HIPOK(hipMemcpyAsync(flags, d->rd.flag, sizeof(flags), hipMemcpyDeviceToHost, c->s)) &&
             HIPOK(hipStreamSynchronize(c->s)))
             d->st.dgks_second_passes = flags[FL_DGKS_COUNT];
-        /* second CGS passes: 2 k columns each, at the mean subspace size */
-        if (d->st.outer_iters > 0)
-            d->st.stream_bytes += 16.0 * n * 2.0 * d->st.dgks_second_passes *
-                                  ((double)d->st.sum_k / (double)d->st.outer_iters);
+        /* second CGS passes at the mean subspace size: one more read of the
+         * basis in the fused form (k <= 128), two otherwise */
+        if (d->st.outer_iters > 0) {
+            const double kbar = (double)d->st.sum_k / (double)d->st.outer_iters;
+            d->st.stream_bytes += 16.0 * n * d->st.dgks_second_passes * (kbar <= 128 ? kbar + 2 : 2 * kbar + 3);
+        }
     }
     d->st.solve_seconds = now_s() - t_start;
     R->status = st;
@@ -1180,6 +1177,41 @@ double bbjd_block_orthogonalize(bbjd_int n, bbjd_int k, const bbjd_zomplex *Q, b
     }
     helper_close(&h);
     return res;
+}
+
+/* CGS2 + normalisation (unit = 1) or the two-pass projection (unit = 0) of
+ * a host vector through the device kernels. */
+static double host_orth_step(bbjd_int n, bbjd_int k, const bbjd_zomplex *Q, bbjd_zomplex *v, int unit, int *flags)
+{
+    helper_t h;
+    void *dq = NULL;
+    const size_t qb = (size_t)n * (k > 0 ? k : 0) * 16, vb = (size_t)n * 16;
+    double res = NAN;
+    if (helper_open(&h, n, k > 0 ? k : 1, qb + vb + 256, &dq) == 0) {
+        void *dv = (char *)dq + (qb + 255) / 256 * 256;
+        int rc = 0;
+        if (k > 0) rc |= !HIPOK(hipMemcpyAsync(dq, Q, qb, hipMemcpyHostToDevice, h.s));
+        rc |= !HIPOK(hipMemcpyAsync(dv, v, vb, hipMemcpyHostToDevice, h.s));
+        if (unit) rc |= bbjd_k_cgs2_unit(h.s, n, (int)k, dq, n, dv, &h.rd, SL_T0);
+        else rc |= bbjd_k_project_out(h.s, n, (int)k, dq, n, dv, &h.rd);
+        rc |= !HIPOK(hipMemcpyAsync(v, dv, vb, hipMemcpyDeviceToHost, h.s));
+        rc |= !HIPOK(hipMemcpyAsync(h.hscal, h.rd.scal, SL_COUNT * 8, hipMemcpyDeviceToHost, h.s));
+        if (flags) rc |= !HIPOK(hipMemcpyAsync(flags, h.rd.flag, FL_COUNT * 4, hipMemcpyDeviceToHost, h.s));
+        rc |= !HIPOK(hipStreamSynchronize(h.s));
+        if (!rc) res = unit ? h.hscal[SL_T0] : 0.0;
+    }
+    helper_close(&h);
+    return res;
+}
+
+double bbjd_block_orthonormalize(bbjd_int n, bbjd_int k, const bbjd_zomplex *Q, bbjd_zomplex *v, int *flags)
+{
+    return host_orth_step(n, k, Q, v, 1, flags);
+}
+
+int bbjd_project_out_host(bbjd_int n, bbjd_int k, const bbjd_zomplex *Q, bbjd_zomplex *v)
+{
+    return host_orth_step(n, k, Q, v, 0, NULL) == 0.0 ? 0 : -1;
 }
 
 int bbjd_block_gram(bbjd_int n, bbjd_int p, const bbjd_zomplex *X, bbjd_int q, const bbjd_zomplex *Y,

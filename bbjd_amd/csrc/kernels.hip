@@ -423,6 +423,271 @@ __global__ __launch_bounds__(TPB) void k_update_split(i64 n, int k, const Cx<R> 
     if (t == 0) *cnt = 0;
 }
 
+// ---------------------------------------------------------------------------
+// Fused CGS2.  When the DGKS test asks for the second pass, CGS2 reads Q four
+// times (projection and update of each pass).  The fused form reads it three
+// times:
+//   k_proj        h1 = Q^H v0 and ||v0||                            read 1
+//   k_cgs_mid     v1 = v0 - Q h1, partials of Q^H v1 and ||v1||^2   read 2
+//   (finish)      h2 = Q^H v1, ||v1||, the DGKS decision
+//   k_cgs_final   v2 = v1 - Q h2 and ||v2|| when decided            read 3
+// A block of k_cgs_mid owns 64-row tiles, one row per lane; wave w holds the
+// tile's entries of columns w, w + 4, ... in registers, so the update and the
+// projection of a row use the same loaded values.  The projection is computed
+// for every call, also when the test then passes (it costs no extra reads).
+// Unit-norm output folds the normalisation into the last launch: without the
+// second pass v1 / ||v1||; with it v2 / sqrt(||v1||^2 - ||h2||^2), which is
+// v2 / ||v2|| to rounding while ||h2||^2 <= 3/4 ||v1||^2 (Q orthonormal);
+// otherwise v2 stays unscaled, FL_RESCALE is set, and a conditional scaling
+// by the computed ||v2|| follows.
+struct MidFin {
+    double *scal;
+    int *flag;
+    int out_slot;
+    int decide;          // 1: CGS2 (norm, DGKS decision, flags); 0: projection only
+    const void *h1;      // first-pass coefficients (for hsum)
+    void *h2, *hsum;
+};
+
+// h2 from the k_cgs_mid partials (columns col0, col0 + col_step, ... one wave
+// each, fixed order), and with decide the norm, the decision and the flags
+// (written by the writer block only; every finishing block computes them).
+template <typename R>
+__device__ __forceinline__ void cgs_mid_finish(int col0, int col_step, bool writer, int nblk, int k,
+                                               const Cx<R> *part, const R *pnorm, const MidFin &f, R *lds4,
+                                               int *lds_p2)
+{
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    int p2 = 0;
+    if (f.decide) {
+        R s = 0;
+        for (int b = t; b < nblk; b += TPB) s += pnorm[b];
+        s = block_sum(s, lds4);
+        if (t == 0) {
+            const double nrm = sqrt((double)s);
+            const int pass2 = nrm < 0.7071 * f.scal[SL_NB];
+            *lds_p2 = pass2;
+            if (writer) {
+                f.scal[SL_NA] = nrm;
+                f.scal[f.out_slot] = nrm;
+                f.flag[FL_PASS2] = pass2;
+                f.flag[FL_RESCALE] = 0;
+                if (pass2) f.flag[FL_DGKS_COUNT] += 1;
+            }
+        }
+        __syncthreads();
+        p2 = *lds_p2;
+    }
+    Cx<R> *h2 = reinterpret_cast<Cx<R> *>(f.h2);
+    Cx<R> *hs = reinterpret_cast<Cx<R> *>(f.hsum);
+    const Cx<R> *h1 = reinterpret_cast<const Cx<R> *>(f.h1);
+    for (int j = col0 + w; j < k; j += col_step) {
+        R sr = 0, si = 0;
+        for (int b = lane; b < nblk; b += 64) {
+            const Cx<R> p = part[(i64)b * k + j];
+            sr += p.re;
+            si += p.im;
+        }
+        sr = wave_sum(sr);
+        si = wave_sum(si);
+        if (lane == 0) {
+            h2[j] = cx(sr, si);
+            if (hs) hs[j] = p2 ? h1[j] + cx(sr, si) : h1[j];
+        }
+    }
+}
+
+// v1 = v0 - Q h1 (in place) and the block's partials of Q^H v1 (part[block
+// k + j]) and, with fin.decide, of ||v1||^2.  KW >= ceil(k / 4) columns per
+// wave; a block walks `tiles` consecutive 64-row tiles.
+template <typename R, int KW>
+__global__ __launch_bounds__(TPB) void k_cgs_mid(i64 n, int k, const Cx<R> *__restrict__ Q, i64 ldq,
+                                                 const Cx<R> *__restrict__ h, Cx<R> *x, int tiles, Cx<R> *part,
+                                                 R *pnorm, int *cnt, MidFin fin, int fused)
+{
+    constexpr int NVAL = 2 * KW;                  // projection values per lane
+    constexpr int G = NVAL < 16 ? NVAL : 16;      // values per butterfly
+    constexpr int NG = NVAL / G;
+    constexpr int LS = G == 16 ? 4 : (G == 8 ? 3 : 2);
+    static_assert((1 << LS) == G, "butterfly width");
+    __shared__ Cx<R> sh[4 * KW];
+    __shared__ Cx<R> red[4][64];
+    __shared__ Cx<R> xin[64];
+    __shared__ R lds4[4];
+    __shared__ int lflag, lp2;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    for (int j = t; j < 4 * KW; j += TPB) sh[j] = j < k ? h[j] : cx<R>(0, 0);
+    __syncthreads();
+    R acc[NG];
+#pragma unroll
+    for (int g = 0; g < NG; g++) acc[g] = 0;
+    R nrm = 0;
+    const i64 tb0 = (i64)blockIdx.x * tiles;
+    for (int tt = 0; tt < tiles; tt++) {
+        const i64 rb = (tb0 + tt) * 64;
+        if (rb >= n) break;                       // uniform over the block
+        const i64 r = rb + lane;
+        const bool in = r < n;
+        Cx<R> q[KW];
+#pragma unroll
+        for (int i = 0; i < KW; i++) {
+            const int j = w + 4 * i;
+            q[i] = (in && j < k) ? Q[(i64)j * ldq + r] : cx<R>(0, 0);
+        }
+        if (w == 0) xin[lane] = in ? x[r] : cx<R>(0, 0);
+        Cx<R> a = cx<R>(0, 0);
+#pragma unroll
+        for (int i = 0; i < KW; i++) cfma(a, q[i], sh[w + 4 * i]);
+        red[w][lane] = a;
+        __syncthreads();
+        const Cx<R> v = xin[lane] - ((red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]));
+        if (w == 0 && in) {
+            x[r] = v;
+            nrm += v.re * v.re + v.im * v.im;
+        }
+#pragma unroll
+        for (int g = 0; g < NG; g++) {
+            R vals[G];
+#pragma unroll
+            for (int c = 0; c < G / 2; c++) {
+                const Cx<R> qc = q[g * (G / 2) + c];
+                vals[2 * c] = qc.re * v.re + qc.im * v.im;          // conj(q) v
+                vals[2 * c + 1] = qc.re * v.im - qc.im * v.re;
+            }
+            wave_transpose_reduce<R, LS>(vals);
+            acc[g] += vals[0];
+        }
+        __syncthreads();                          // red, xin are rewritten next tile
+    }
+    // lane l holds value index l >> (6 - LS) of each butterfly group
+    constexpr int SH = 6 - LS;
+    if ((lane & ((1 << SH) - 1)) == 0) {
+        const int vi = lane >> SH;
+#pragma unroll
+        for (int g = 0; g < NG; g++) {
+            const int j = w + 4 * (g * (G / 2) + (vi >> 1));
+            if (j < k) reinterpret_cast<R *>(part + (i64)blockIdx.x * k + j)[vi & 1] = acc[g];
+        }
+    }
+    if (fin.decide && w == 0) {
+        const R s = wave_sum(nrm);
+        if (lane == 0) pnorm[blockIdx.x] = s;
+    }
+    if (!fused || !last_block_of(cnt, gridDim.x, &lflag)) return;
+    cgs_mid_finish<R>(0, 4, true, gridDim.x, k, part, pnorm, fin, lds4, &lp2);
+    if (t == 0) *cnt = 0;
+}
+
+template <typename R>
+__global__ __launch_bounds__(TPB) void k_cgs_mid_fin(int nblk, int k, const Cx<R> *part, const R *pnorm,
+                                                     MidFin fin)
+{
+    __shared__ R lds4[4];
+    __shared__ int lp2;
+    cgs_mid_finish<R>(blockIdx.x * 4, gridDim.x * 4, blockIdx.x == 0, nblk, k, part, pnorm, fin, lds4, &lp2);
+}
+
+// Last launch of the fused CGS2 (k <= 128).  With the second pass (decided,
+// or forced for project_out): x = (x - Q h2) * sc with ||x - Q h2|| reduced
+// per fin; sc = 1 / sqrt(||v1||^2 - ||h2||^2) for unit output when that is
+// at least trust ||v1||^2 (trust 1/4: the estimate is then good to a few
+// ulp), else 1 and FL_RESCALE.  Without it: x *= 1 / ||v1|| for unit
+// output (when ||v1|| > 1e-300), else nothing.  Columns split over the four
+// waves as in k_update_split.
+template <typename R, int RW>
+__global__ __launch_bounds__(TPB) void k_cgs_final(i64 n, int k, const Cx<R> *__restrict__ Q, i64 ldq,
+                                                   const Cx<R> *__restrict__ h2, Cx<R> *__restrict__ x, int scale,
+                                                   int force, double trust, const double *scal, int *flag,
+                                                   R *pnorm, int *cnt, NormFin fin, int fused)
+{
+    __shared__ Cx<R> sh[128];
+    __shared__ Cx<R> red[4][64 * RW];
+    __shared__ R lds4[4];
+    __shared__ int lflag;
+    __shared__ double lsc;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const i64 rb = (i64)blockIdx.x * (64 * RW);
+    if (!force && flag[FL_PASS2] == 0) {
+        if (!scale) return;
+        const double nrm = scal[SL_NA];
+        if (!(nrm > 1e-300)) return;
+        const R sc = (R)(1.0 / nrm);
+        if (t < 64 * RW) {
+            const i64 r = rb + t;
+            if (r < n) {
+                Cx<R> v = x[r];
+                v.re *= sc;
+                v.im *= sc;
+                x[r] = v;
+            }
+        }
+        return;
+    }
+    R hh = 0;
+    for (int j = t; j < k; j += TPB) {
+        const Cx<R> hj = h2[j];
+        sh[j] = hj;
+        hh += hj.re * hj.re + hj.im * hj.im;
+    }
+    hh = block_sum(hh, lds4);
+    if (t == 0) {
+        double sc = 1.0;
+        int resc = 0;
+        if (scale) {
+            const double na = scal[SL_NA];
+            const double e = na * na - (double)hh;
+            if (na > 1e-300 && e >= trust * na * na) sc = 1.0 / sqrt(e);
+            else resc = 1;
+        }
+        lsc = sc;
+        if (blockIdx.x == 0) flag[FL_RESCALE] = resc;
+    }
+    __syncthreads();
+    const R sc = (R)lsc;
+    Cx<R> a[RW];
+#pragma unroll
+    for (int e = 0; e < RW; e++) a[e] = cx<R>(0, 0);
+    if (rb + 64 * RW <= n) {
+#pragma unroll 8
+        for (int j = w; j < k; j += 4) {
+            const Cx<R> hj = sh[j];
+            const Cx<R> *q = Q + (i64)j * ldq + rb + lane;
+#pragma unroll
+            for (int e = 0; e < RW; e++) cfma(a[e], q[e * 64], hj);
+        }
+    } else {
+        for (int j = w; j < k; j += 4) {
+            const Cx<R> hj = sh[j];
+            const Cx<R> *q = Q + (i64)j * ldq;
+#pragma unroll
+            for (int e = 0; e < RW; e++) {
+                const i64 r = rb + lane + e * 64;
+                if (r < n) cfma(a[e], q[r], hj);
+            }
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < RW; e++) red[w][lane + e * 64] = a[e];
+    __syncthreads();
+    R s = 0;
+    if (t < 64 * RW) {
+        const i64 r = rb + t;
+        if (r < n) {
+            Cx<R> v = x[r] - ((red[0][t] + red[1][t]) + (red[2][t] + red[3][t]));
+            s = v.re * v.re + v.im * v.im;
+            v.re *= sc;
+            v.im *= sc;
+            x[r] = v;
+        }
+    }
+    if (fin.mode < 0) return;
+    s = block_sum(s, lds4);
+    if (t == 0) pnorm[blockIdx.x] = s;
+    if (!fused || !last_block_of(cnt, gridDim.x, &lflag)) return;
+    finish_norm<R>(pnorm, gridDim.x, fin, lds4);
+    if (t == 0) *cnt = 0;
+}
+
 // separate finish of a norm (one workgroup)
 template <typename R>
 __global__ __launch_bounds__(TPB) void k_norm_fin(const R *pnorm, int nblk, NormFin fin, const int *__restrict__ cond)
@@ -454,8 +719,10 @@ __global__ __launch_bounds__(TPB) void k_norm(i64 n, const Cx<R> *__restrict__ x
 }
 
 template <typename R>
-__global__ __launch_bounds__(TPB) void k_scale_inv(i64 n, Cx<R> *x, const double *scal, int slot, int mode)
+__global__ __launch_bounds__(TPB) void k_scale_inv(i64 n, Cx<R> *x, const double *scal, int slot, int mode,
+                                                   const int *__restrict__ cond)
 {
+    if (cond && cond[0] == 0) return;
     const double nrm = scal[slot];
     if (mode == 1 && !(nrm > 1e-300)) return;
     const R s = (R)(1.0 / nrm);
@@ -838,15 +1105,110 @@ static int norm_into(hipStream_t s, i64 n, const void *x, bbjd_red *rd, int slot
     return chk();
 }
 
-// CGS2 with the DGKS test in four launches: projection (+ ||v|| before),
-// update (+ ||v|| after, decision), and the second pass, which reads the
-// decision flag on the device and returns at once when it is not needed.
+// Fused CGS2 (k <= 128): on by default, switchable for tests and tuning
+// (2: on, with the second pass's normalisation always by the computed norm).
+static int g_cgs_fused = 1;
+
+// columns per wave of k_cgs_mid for k columns; 0 beyond its register budget
+static inline int cgs_kw(int k)
+{
+    return k <= 8 ? 2 : k <= 16 ? 4 : k <= 32 ? 8 : k <= 64 ? 16 : k <= 128 ? 32 : 0;
+}
+
+template <typename R>
+static int scale_inv(hipStream_t s, i64 n, void *x, const double *scal, int slot, const int *cond)
+{
+    hipLaunchKernelGGL((k_scale_inv<R>), dim3(grid_stride_blocks(n)), dim3(TPB), 0, s, (i64)n, (Cx<R> *)x, scal,
+                       slot, 1, cond);
+    return chk();
+}
+
+// Middle launch of the fused CGS2 (+ its finish when the grid is large).
+// Blocks walk enough 64-row tiles for about 1024 blocks, and more when the
+// partials (blocks x k) would not fit the scratch.
+template <typename R>
+static int cgs_mid(hipStream_t s, i64 n, int k, const void *Q, i64 ldq, void *v, bbjd_red *rd, MidFin f)
+{
+    const i64 tiles64 = (n + 63) / 64;
+    i64 per = (tiles64 + 1023) / 1024;
+    if (per < 1) per = 1;
+    i64 nb = (tiles64 + per - 1) / per;
+    while (nb * k > rd->part_cap && nb > 1) {
+        per *= 2;
+        nb = (tiles64 + per - 1) / per;
+    }
+    if (nb * k > rd->part_cap) return -2;
+    const int fused = nb <= g_fuse_max_blocks;
+    int *cnt = rd->cnt + BBJD_CNT_MID;
+#define BBJD_LAUNCH_MID(KW)                                                                                         \
+    hipLaunchKernelGGL((k_cgs_mid<R, KW>), dim3((unsigned)nb), dim3(TPB), 0, s, (i64)n, k, (const Cx<R> *)Q,         \
+                       (i64)ldq, (const Cx<R> *)f.h1, (Cx<R> *)v, (int)per, (Cx<R> *)rd->part, (R *)rd->pnorm, cnt,  \
+                       f, fused)
+    switch (cgs_kw(k)) {
+    case 2: BBJD_LAUNCH_MID(2); break;
+    case 4: BBJD_LAUNCH_MID(4); break;
+    case 8: BBJD_LAUNCH_MID(8); break;
+    case 16: BBJD_LAUNCH_MID(16); break;
+    case 32: BBJD_LAUNCH_MID(32); break;
+    default: return -2;
+    }
+#undef BBJD_LAUNCH_MID
+    if (!fused)
+        hipLaunchKernelGGL((k_cgs_mid_fin<R>), dim3((unsigned)((k + 3) / 4)), dim3(TPB), 0, s, (int)nb, k,
+                           (const Cx<R> *)rd->part, (const R *)rd->pnorm, f);
+    return chk();
+}
+
+// Last launch of the fused CGS2 (+ the norm finish when the grid is large).
+template <typename R>
+static int cgs_final(hipStream_t s, i64 n, int k, const void *Q, i64 ldq, void *v, bbjd_red *rd, NormFin fin,
+                     int scale, int force)
+{
+    constexpr int RW = 2;
+    const int nt = ntiles(n, 64 * RW);
+    const int fused = nt <= g_fuse_max_blocks;
+    // mode 2 never trusts the estimate (e <= ||v1||^2 < 2 ||v1||^2)
+    const double trust = g_cgs_fused == 2 ? 2.0 : 0.25;
+    hipLaunchKernelGGL((k_cgs_final<R, RW>), dim3(nt), dim3(TPB), 0, s, (i64)n, k, (const Cx<R> *)Q, (i64)ldq,
+                       (const Cx<R> *)rd->h2, (Cx<R> *)v, scale, force, trust, (const double *)rd->scal,
+                       rd->flag, (R *)rd->pnorm, rd->cnt + BBJD_CNT_UPD, fin, fused);
+    if (!fused && fin.mode >= 0)
+        hipLaunchKernelGGL((k_norm_fin<R>), dim3(1), dim3(TPB), 0, s, (const R *)rd->pnorm, nt, fin,
+                           force ? (const int *)nullptr : (const int *)(rd->flag + FL_PASS2));
+    return chk();
+}
+
+static inline bool cgs_fusable(int k, const bbjd_red *rd)
+{
+    return g_cgs_fused && cgs_kw(k) > 0 && (i64)k <= rd->part_cap;
+}
+
+// CGS2 with the DGKS test.  k <= 128: the fused form (three reads of Q when
+// the second pass runs).  Otherwise four launches: projection (+ ||v||
+// before), update (+ ||v|| after, decision), and the second pass, which
+// reads the decision flag on the device and returns at once when it is not
+// needed.  No host round trip either way.  scale: normalise the result when
+// its norm exceeds 1e-300 (scal[out_slot] keeps the norm before scaling).
 template <typename R>
 static int cgs2(hipStream_t s, i64 n, int k, const void *Q, i64 ldq, void *v, bbjd_red *rd, int out_slot,
-                void *hsum)
+                void *hsum, int scale = 0)
 {
-    if (k == 0) return norm_into<R>(s, n, v, rd, out_slot);
     int rc = 0;
+    if (k == 0) {
+        rc |= norm_into<R>(s, n, v, rd, out_slot);
+        if (scale) rc |= scale_inv<R>(s, n, v, rd->scal, out_slot, nullptr);
+        return rc;
+    }
+    if (cgs_fusable(k, rd)) {
+        rc |= proj_pass<R>(s, n, k, Q, ldq, v, nullptr, rd->h, nullptr, rd, SL_NB, nullptr);
+        MidFin f;
+        f.scal = rd->scal; f.flag = rd->flag; f.out_slot = out_slot; f.decide = 1;
+        f.h1 = rd->h; f.h2 = rd->h2; f.hsum = hsum;
+        rc |= cgs_mid<R>(s, n, k, Q, ldq, v, rd, f);
+        rc |= cgs_final<R>(s, n, k, Q, ldq, v, rd, fin_norm(rd, 0, out_slot), scale, 0);
+        if (scale) rc |= scale_inv<R>(s, n, v, rd->scal, out_slot, rd->flag + FL_RESCALE);
+        return rc;
+    }
     rc |= proj_pass<R>(s, n, k, Q, ldq, v, nullptr, rd->h, nullptr, rd, SL_NB, nullptr);
     NormFin f1 = fin_norm(rd, 1, out_slot);
     if (hsum) { f1.hsum = hsum; f1.h = rd->h; f1.k = k; }
@@ -856,14 +1218,25 @@ static int cgs2(hipStream_t s, i64 n, int k, const void *Q, i64 ldq, void *v, bb
     NormFin f2 = fin_norm(rd, 2, out_slot);
     if (hsum) { f2.hsum = hsum; f2.h = rd->h; f2.h2 = rd->h2; f2.k = k; }
     rc |= update_pass<R>(s, n, k, Q, ldq, rd->h2, (R)-1, 1, v, rd, f2, cond);
+    if (scale) rc |= scale_inv<R>(s, n, v, rd->scal, out_slot, nullptr);
     return rc;
 }
 
+// v -= Q Q^H v twice (no norm test); fused as in cgs2 for k <= 128.
 template <typename R>
 static int project_out(hipStream_t s, i64 n, int k, const void *Q, i64 ldq, void *v, bbjd_red *rd)
 {
     if (k == 0) return 0;
     int rc = 0;
+    if (cgs_fusable(k, rd)) {
+        rc |= proj_pass<R>(s, n, k, Q, ldq, v, nullptr, rd->h, nullptr, rd, -1, nullptr);
+        MidFin f;
+        f.scal = rd->scal; f.flag = rd->flag; f.out_slot = 0; f.decide = 0;
+        f.h1 = rd->h; f.h2 = rd->h2; f.hsum = nullptr;
+        rc |= cgs_mid<R>(s, n, k, Q, ldq, v, rd, f);
+        rc |= cgs_final<R>(s, n, k, Q, ldq, v, rd, fin_none(), 0, 1);
+        return rc;
+    }
     for (int pass = 0; pass < 2; pass++) {
         rc |= proj_pass<R>(s, n, k, Q, ldq, v, nullptr, rd->h, nullptr, rd, -1, nullptr);
         rc |= update_pass<R>(s, n, k, Q, ldq, rd->h, (R)-1, 1, v, rd, fin_none(), nullptr);
@@ -948,6 +1321,12 @@ int bbjd_k_cgs2(hipStream_t s, int64_t n, int k, const void *Q, int64_t ldq, voi
                 int out_slot, void *hsum)
 { return cgs2<double>(s, n, k, Q, ldq, v, rd, out_slot, hsum); }
 
+int bbjd_k_cgs2_unit(hipStream_t s, int64_t n, int k, const void *Q, int64_t ldq, void *v, bbjd_red *rd,
+                     int out_slot)
+{ return cgs2<double>(s, n, k, Q, ldq, v, rd, out_slot, nullptr, 1); }
+
+void bbjd_set_cgs_fused(int mode) { g_cgs_fused = mode == 2 ? 2 : mode != 0; }
+
 int bbjd_k_cgs2_c(hipStream_t s, int64_t n, int k, const void *Q, int64_t ldq, void *v, bbjd_red *rd,
                   int out_slot, void *hsum)
 { return cgs2<float>(s, n, k, Q, ldq, v, rd, out_slot, hsum); }
@@ -984,7 +1363,7 @@ int bbjd_k_norm_c(hipStream_t s, int64_t n, const void *x, bbjd_red *rd, int slo
 int bbjd_k_scale_inv(hipStream_t s, int64_t n, void *x, const double *scal, int slot, int mode)
 {
     hipLaunchKernelGGL((k_scale_inv<double>), dim3(grid_stride_blocks(n)), dim3(TPB), 0, s, (i64)n,
-                       (Cx<double> *)x, scal, slot, mode);
+                       (Cx<double> *)x, scal, slot, mode, (const int *)nullptr);
     return chk();
 }
 

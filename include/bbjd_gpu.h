@@ -134,6 +134,26 @@ int  bbjd_ktime_harvest(double *total_ms, double *total_bytes);
 int bbjd_block_gram(bbjd_int n, bbjd_int p, const bbjd_zomplex *X, bbjd_int q, const bbjd_zomplex *Y,
                     bbjd_zomplex *G);
 
+/* Test entries on host arrays (no counterpart in the reference; the
+ * engine's own steps, for parity tests against the oracle).
+ * bbjd_block_orthonormalize: the expansion's CGS2 of v against Q (n x k,
+ * leading dimension n) followed by v /= ||v|| when ||v|| > 1e-300, as the
+ * reference's expansion does after bbjd_block_orthogonalize
+ * (src/jd_core.c:1038-1058); returns ||v|| before the scaling, NAN on a
+ * device error; flags (8 ints, may be NULL) receives the device decision
+ * flags, flags[0] = 1 when the DGKS second pass ran.
+ * bbjd_project_out_host: v -= Q (Q^H v), two fixed passes (reference
+ * src/jd_core.c:269-309); returns 0 or -1. */
+double bbjd_block_orthonormalize(bbjd_int n, bbjd_int k, const bbjd_zomplex *Q, bbjd_zomplex *v, int *flags);
+int    bbjd_project_out_host(bbjd_int n, bbjd_int k, const bbjd_zomplex *Q, bbjd_zomplex *v);
+
+/* 1 (default): CGS2 and the two-pass projection take the three-read form
+ * (first update and second projection in one pass over Q) for k <= 128;
+ * 0: the four-launch form everywhere; 2: the three-read form with the
+ * normalisation after a second pass always by the computed norm (the rare
+ * route of 1, forced).  For tests and tuning. */
+void bbjd_set_cgs_fused(int mode);
+
 /* Kernel benchmark (tile autotuning and roofline checks): times one engine
  * kernel in isolation at shape (n, k) on a private stream, reps launches.
  * which: 0 projection Q^H x, 1 projection of two vectors, 2 update

@@ -76,6 +76,61 @@ def test_cgs2_every_row_tile(proj_blocks, rows):
         lib.bbjd_set_fuse_blocks(16)
 
 
+FUSED_K = [1, 2, 7, 8, 9, 16, 17, 31, 32, 33, 64, 65, 100, 128, 129]
+
+
+@pytest.mark.parametrize("n", [1000 + 3, 70000 + 5])
+def test_fused_cgs2_matches_oracle(n):
+    """the three-read CGS2 (first update and second projection in one pass
+    over Q, k <= 128) and the four-launch form against the oracle: every
+    columns-per-wave width, the k = 129 fallback, both DGKS branches, the
+    Hessenberg sum h1 + h2, unit-norm output (mode 2 forces the route where
+    the second pass leaves its result to a scaling by the computed norm) and
+    the two-pass projection"""
+    lib = _lib.load()
+    olib = orc.load_oracle()
+    rng = np.random.default_rng(n)
+    Qall = _orthonormal(n, max(FUSED_K), n + 7)
+    flags = np.zeros(8, np.int32)
+    try:
+        for k in FUSED_K:
+            Q = Qall[:, :k]
+            assert Q.flags.f_contiguous
+            v = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+            r = rng.standard_normal(k) + 1j * rng.standard_normal(k)
+            cases = {"generic": v, "near_parallel": Q @ r + 1e-6 * v, "nearer": Q @ r + 1e-11 * v}
+            for case, x in cases.items():
+                scale = np.linalg.norm(x)
+                vo, ho = x.copy(), np.zeros(k, np.complex128)
+                no = olib.orc_block_orthogonalize(n, k, Q.ctypes.data, vo.ctypes.data, ho.ctypes.data)
+                tol_unit = 1e-12 * scale / no
+                for mode in (1, 2, 0):
+                    lib.bbjd_set_cgs_fused(mode)
+                    tag = (k, case, mode)
+                    vg, hg = x.copy(), np.zeros(k, np.complex128)
+                    ng = lib.bbjd_block_orthogonalize(n, k, Q.ctypes.data, vg.ctypes.data, hg.ctypes.data)
+                    assert np.linalg.norm(vg - vo) <= 1e-12 * scale, tag
+                    # h_work: the reference leaves the last pass's coefficients
+                    # there, the engine the sum of both passes (the Hessenberg
+                    # column of its GMRES), which is Q^H x to rounding
+                    href = ho if case == "generic" else Q.conj().T @ x
+                    assert np.linalg.norm(hg - href) <= 1e-12 * scale, tag
+                    assert abs(ng - no) <= 1e-12 * scale, tag
+                    vu = x.copy()
+                    nu = lib.bbjd_block_orthonormalize(n, k, Q.ctypes.data, vu.ctypes.data, flags.ctypes.data)
+                    assert flags[0] == (case != "generic"), tag      # the DGKS branch taken
+                    assert nu == ng, tag            # the same reductions: the norm before scaling
+                    assert abs(np.linalg.norm(vu) - 1.0) <= 1e-13, tag
+                    assert np.linalg.norm(vu - vo / no) <= tol_unit, tag
+                    assert np.linalg.norm(Q.conj().T @ vu) <= tol_unit, tag
+                    vp, vq = x.copy(), x.copy()
+                    assert lib.bbjd_project_out_host(n, k, Q.ctypes.data, vp.ctypes.data) == 0
+                    olib.orc_project_out(n, k, Q.ctypes.data, vq.ctypes.data)
+                    assert np.linalg.norm(vp - vq) <= 1e-12 * scale, tag
+    finally:
+        lib.bbjd_set_cgs_fused(1)
+
+
 # super-block sizes hw * bs = 4, 16, 8, 8 (compiled loops) and 12 (generic)
 SHAPES = [(2000, 4, 1, 2), (4096, 8, 2, 16), (1000, 8, 1, 4), (640, 4, 2, 0), (1206, 6, 2, 3)]
 
