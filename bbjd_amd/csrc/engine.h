@@ -50,7 +50,7 @@ enum { FL_PASS2 = 0, FL_ONE = 1, FL_DGKS_COUNT = 2, FL_RESCALE = 3, FL_COUNT = 8
 
 /* Per-workspace scratch for the two-stage reductions. */
 typedef struct {
-    void   *part;        /* partial sums: tiles x ldp complex (x2 vectors) */
+    void   *part;        /* partial sums: tiles x ldp complex (<= 3 vectors) */
     double *pnorm;       /* partial norms per tile                          */
     void   *h;           /* coefficient vector, first pass                 */
     void   *h2;          /* second pass                                    */
@@ -76,6 +76,13 @@ int bbjd_k_cgs2(hipStream_t s, int64_t n, int k, const void *Q, int64_t ldq, voi
  * norm before the scaling). */
 int bbjd_k_cgs2_unit(hipStream_t s, int64_t n, int k, const void *Q, int64_t ldq, void *v,
                      bbjd_red *rd, int out_slot);
+
+/* Test-space expansion: bbjd_k_cgs2_unit of w against W, which also yields
+ * the projected pencil column colA[0..k] = [W w]^H a, colB[0..k] = [W w]^H b
+ * (device arrays; a = A v, b = B v).  Returns 1, having done nothing, when
+ * the shape takes the unfused path. */
+int bbjd_k_cgs2_unit_pencil(hipStream_t s, int64_t n, int k, const void *W, int64_t ldw, void *w, const void *a,
+                            const void *b, void *colA, void *colB, bbjd_red *rd, int out_slot);
 
 /* v -= Q Q^H v twice (no norm test), k columns. */
 int bbjd_k_project_out(hipStream_t s, int64_t n, int k, const void *Q, int64_t ldq, void *v,

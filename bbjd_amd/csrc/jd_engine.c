@@ -616,7 +616,7 @@ static int gmres(ctx_t *c, bbjd_int ndefl, const void *QR, const void *QL, zcx t
  * is projected.  rowrel = 0 (some column of W since the last restart came
  * from the Av fallback, so the relation does not hold) projects both rows as
  * the reference does. */
-static int pencil_update(ctx_t *c, bbjd_int k, int rowrel)
+static int pencil_update(ctx_t *c, bbjd_int k, int rowrel, int col_done)
 {
     bbjd_workspace *ws = c->ws;
     struct bbjd_dev *d = c->d;
@@ -625,8 +625,9 @@ static int pencil_update(ctx_t *c, bbjd_int k, int rowrel)
     const int via_b = rowrel && cabs(c->nu0) > 0;
     const int via_a = rowrel && !via_b;
     int rc = 0;
-    rc |= bbjd_k_gemv_c(c->s, c->n, (int)(k + 1), ws->W, c->ld, COLZ(ws->AV, k), COLZ(ws->BV, k), pen,
-                        pen + (ms + 1), &d->rd);
+    if (!col_done)
+        rc |= bbjd_k_gemv_c(c->s, c->n, (int)(k + 1), ws->W, c->ld, COLZ(ws->AV, k), COLZ(ws->BV, k), pen,
+                            pen + (ms + 1), &d->rd);
     if (k > 0) {
         if (!via_b)
             rc |= bbjd_k_gemv_c(c->s, c->n, (int)k, ws->AV, c->ld, COLZ(ws->W, k), NULL, pen + 2 * (ms + 1), NULL,
@@ -653,7 +654,7 @@ static int pencil_update(ctx_t *c, bbjd_int k, int rowrel)
     } else {
         for (bbjd_int i = 0; i < k; i++) { MA[k + i * ms] = conj(ra[i]); MB[k + i * ms] = conj(rb[i]); }
     }
-    d->st.stream_bytes += 16.0 * c->n * ((k + 1) + 2 + (rowrel ? 1.0 : 2.0) * (k + 1));
+    d->st.stream_bytes += 16.0 * c->n * ((col_done ? 3.0 : (k + 3.0)) + (rowrel ? 1.0 : 2.0) * (k + 1));
     return 0;
 }
 
@@ -812,9 +813,14 @@ static int expand(ctx_t *c, bbjd_int k, const void *z, zcx nu0, zcx mu0)
         if (rc) return -2;
         if (apply_op(c, P->matvec_A, vn, an) || apply_op(c, P->matvec_B, vn, bn)) return -1;
         rc |= bbjd_k_lincomb(c->s, n, creal(nu0), cimag(nu0), an, creal(mu0), cimag(mu0), bn, wn);
-        rc |= bbjd_k_cgs2_unit(c->s, n, (int)k, ws->W, c->ld, wn, &d->rd, SL_W);
-        if (rc || pencil_update(c, k, !c->w_mixed)) return -2;
-        d->st.stream_bytes += 16.0 * n * (2.0 * k + 4 + 3 + 2.0 * k + 2);
+        /* the pencil column comes with the first pass over W when fused */
+        zcx *pen = (zcx *)d->pencil;
+        const int pc = bbjd_k_cgs2_unit_pencil(c->s, n, (int)k, ws->W, c->ld, wn, an, bn, pen, pen + (c->ldm + 1),
+                                               &d->rd, SL_W);
+        if (pc < 0) return -2;
+        if (pc > 0) rc |= bbjd_k_cgs2_unit(c->s, n, (int)k, ws->W, c->ld, wn, &d->rd, SL_W);
+        if (rc || pencil_update(c, k, !c->w_mixed, pc == 0)) return -2;
+        d->st.stream_bytes += 16.0 * n * (2.0 * k + 4 + 3 + 2.0 * k + 2 + (pc == 0 ? 2 : 0));
         const double nv = d->hscal[SL_V], nw = d->hscal[SL_W];
         if (nv >= 1e-14 && nw >= 1e-14) return 0;
         if (nv >= 1e-14) {
@@ -823,7 +829,7 @@ static int expand(ctx_t *c, bbjd_int k, const void *z, zcx nu0, zcx mu0)
             rc |= !HIPOK(hipMemcpyAsync(wn, an, (size_t)n * 16, hipMemcpyDeviceToDevice, c->s));
             rc |= bbjd_k_cgs2_unit(c->s, n, (int)k, ws->W, c->ld, wn, &d->rd, SL_W);
             c->w_mixed = 1;
-            if (rc || pencil_update(c, k, 0)) return -2;
+            if (rc || pencil_update(c, k, 0, 0)) return -2;
             return 0;
         }
         nrm = nv;           /* v broke down: the random-vector branch below */
@@ -853,7 +859,7 @@ static int expand(ctx_t *c, bbjd_int k, const void *z, zcx nu0, zcx mu0)
     rc |= bbjd_k_scale_inv(c->s, n, wn, d->rd.scal, SL_W, 1);
     d->st.stream_bytes += 16.0 * n * (2.0 * k + 4 + 3 + 2.0 * k + 2);
     if (w_fallback) c->w_mixed = 1;
-    if (rc || pencil_update(c, k, !c->w_mixed)) return -2;
+    if (rc || pencil_update(c, k, !c->w_mixed, 0)) return -2;
     return 0;
 }
 
@@ -921,7 +927,7 @@ static bbjd_status solve_impl(const bbjd_params *P, bbjd_workspace *ws, bbjd_res
     rc |= bbjd_k_lincomb_norm(c->s, n, creal(nu0), cimag(nu0), ws->AV, creal(mu0), cimag(mu0), ws->BV, ws->W,
                               &d->rd, SL_W);
     rc |= bbjd_k_scale_inv(c->s, n, ws->W, d->rd.scal, SL_W, 0);
-    if (rc || pencil_update(c, 0, 1)) { st = BBJD_ERR_ALLOC; goto done; }
+    if (rc || pencil_update(c, 0, 1, 0)) { st = BBJD_ERR_ALLOC; goto done; }
     ws->kdim = 1;
 
     for (bbjd_int iter = 0; iter < P->maxiter; iter++) {
@@ -1212,6 +1218,42 @@ double bbjd_block_orthonormalize(bbjd_int n, bbjd_int k, const bbjd_zomplex *Q, 
 int bbjd_project_out_host(bbjd_int n, bbjd_int k, const bbjd_zomplex *Q, bbjd_zomplex *v)
 {
     return host_orth_step(n, k, Q, v, 0, NULL) == 0.0 ? 0 : -1;
+}
+
+double bbjd_expand_test_space(bbjd_int n, bbjd_int k, const bbjd_zomplex *W, bbjd_zomplex *w, const bbjd_zomplex *a,
+                              const bbjd_zomplex *b, bbjd_zomplex *colA, bbjd_zomplex *colB)
+{
+    helper_t h;
+    void *dW = NULL;
+    const size_t wb = (size_t)n * (k > 0 ? k : 0) * 16, vb = ((size_t)n * 16 + 255) / 256 * 256,
+                 cb = ((size_t)(k + 1) * 16 + 255) / 256 * 256;
+    double res = NAN;
+    if (k < 1) return NAN;
+    if (helper_open(&h, n, k, (wb + 255) / 256 * 256 + 3 * vb + 2 * cb, &dW) == 0) {
+        char *p = (char *)dW + (wb + 255) / 256 * 256;
+        void *dw = p, *da = p + vb, *db = p + 2 * vb, *dA = p + 3 * vb, *dB = p + 3 * vb + cb;
+        int rc = 0;
+        rc |= !HIPOK(hipMemcpyAsync(dW, W, wb, hipMemcpyHostToDevice, h.s));
+        rc |= !HIPOK(hipMemcpyAsync(dw, w, (size_t)n * 16, hipMemcpyHostToDevice, h.s));
+        rc |= !HIPOK(hipMemcpyAsync(da, a, (size_t)n * 16, hipMemcpyHostToDevice, h.s));
+        rc |= !HIPOK(hipMemcpyAsync(db, b, (size_t)n * 16, hipMemcpyHostToDevice, h.s));
+        int pc = rc ? -1 : bbjd_k_cgs2_unit_pencil(h.s, n, (int)k, dW, n, dw, da, db, dA, dB, &h.rd, SL_T0);
+        if (pc > 0) {
+            /* unfused shape: the expansion's other route */
+            rc |= bbjd_k_cgs2_unit(h.s, n, (int)k, dW, n, dw, &h.rd, SL_T0);
+            rc |= bbjd_k_gemv_c(h.s, n, (int)k, dW, n, da, db, dA, dB, &h.rd);
+            rc |= bbjd_k_gemv_c(h.s, n, 1, dw, n, da, db, (char *)dA + (size_t)k * 16, (char *)dB + (size_t)k * 16,
+                                &h.rd);
+        } else if (pc < 0) rc = 1;
+        rc |= !HIPOK(hipMemcpyAsync(w, dw, (size_t)n * 16, hipMemcpyDeviceToHost, h.s));
+        rc |= !HIPOK(hipMemcpyAsync(colA, dA, (size_t)(k + 1) * 16, hipMemcpyDeviceToHost, h.s));
+        rc |= !HIPOK(hipMemcpyAsync(colB, dB, (size_t)(k + 1) * 16, hipMemcpyDeviceToHost, h.s));
+        rc |= !HIPOK(hipMemcpyAsync(h.hscal, h.rd.scal, SL_COUNT * 8, hipMemcpyDeviceToHost, h.s));
+        rc |= !HIPOK(hipStreamSynchronize(h.s));
+        if (!rc) res = h.hscal[SL_T0];
+    }
+    helper_close(&h);
+    return res;
 }
 
 int bbjd_block_gram(bbjd_int n, bbjd_int p, const bbjd_zomplex *X, bbjd_int q, const bbjd_zomplex *Y,

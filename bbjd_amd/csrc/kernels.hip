@@ -185,25 +185,29 @@ __device__ __forceinline__ void finish_norm(const R *pnorm, int nblk, const Norm
 // k_proj_finish.
 template <typename R, int NV, int C>
 __device__ __forceinline__ void proj_finish(int c0, int ntiles, int k, const Cx<R> *part, int ldp, const R *pnorm,
-                                            bool norm_here, Cx<R> *out0, Cx<R> *out1, double *scal, int norm_slot,
-                                            R *lds4)
+                                            bool norm_here, Cx<R> *out0, Cx<R> *out1, Cx<R> *out2, double *scal,
+                                            int norm_slot, R *lds4)
 {
+    constexpr int PAIRS = NV * C;                 // (vector, column) pairs
+    constexpr int LP = PAIRS <= 16 ? 16 : 8;      // lanes per pair
+    static_assert(PAIRS * LP <= TPB, "finish lanes");
     const int t = threadIdx.x;
-    const int g = t >> 4, l = t & 15;
+    const int g = t / LP, l = t % LP;
     const int vi = g / C, c = g % C, col = c0 + c;
+    const bool mine = g < PAIRS && col < k;
     R sr = 0, si = 0;
-    if (col < k)
-        for (int tt = l; tt < ntiles; tt += 16) {
+    if (mine)
+        for (int tt = l; tt < ntiles; tt += LP) {
             const Cx<R> p = part[((i64)tt * NV + vi) * ldp + col];
             sr += p.re;
             si += p.im;
         }
 #pragma unroll
-    for (int m = 8; m >= 1; m >>= 1) {
+    for (int m = LP / 2; m >= 1; m >>= 1) {
         sr += __shfl_xor(sr, m, 64);
         si += __shfl_xor(si, m, 64);
     }
-    if (l == 0 && col < k) (vi == 0 ? out0 : out1)[col] = cx(sr, si);
+    if (l == 0 && mine) (vi == 0 ? out0 : vi == 1 ? out1 : out2)[col] = cx(sr, si);
     if (norm_here) {
         R s = 0;
         for (int tt = t; tt < ntiles; tt += TPB) s += pnorm[tt];
@@ -218,14 +222,14 @@ __device__ __forceinline__ void proj_finish(int c0, int ntiles, int k, const Cx<
 template <typename R, int NV, int C, bool COND>
 __global__ __launch_bounds__(TPB) void k_proj(i64 n, i64 chunk, int k, const Cx<R> *__restrict__ Q, i64 ldq,
                                               const Cx<R> *__restrict__ x0, const Cx<R> *__restrict__ x1,
-                                              Cx<R> *part, int ldp, R *pnorm, int want_norm,
-                                              const int *__restrict__ cond, int *cnt, Cx<R> *out0, Cx<R> *out1,
-                                              double *scal, int norm_slot, int fused)
+                                              const Cx<R> *__restrict__ x2, Cx<R> *part, int ldp, R *pnorm,
+                                              int want_norm, const int *__restrict__ cond, int *cnt, Cx<R> *out0,
+                                              Cx<R> *out1, Cx<R> *out2, double *scal, int norm_slot, int fused)
 {
     if (COND && cond[0] == 0) return;
     constexpr int NVAL = 2 * NV * C;
-    static_assert(NVAL == 32 || NVAL == 16, "butterfly width");
-    constexpr int LS = NVAL == 32 ? 5 : 4;
+    static_assert(NVAL == 32 || NVAL == 16 || NVAL == 48, "butterfly width");
+    constexpr int LS = NVAL == 16 ? 4 : 5;        // 48 values: a 32 and a 16 butterfly
     __shared__ R lds[4][NVAL];
     __shared__ R lds4[4];
     __shared__ int lflag;
@@ -253,7 +257,8 @@ __global__ __launch_bounds__(TPB) void k_proj(i64 n, i64 chunk, int k, const Cx<
     for (i64 r = (i64)tile * chunk + t; r < r1; r += TPB) {
         Cx<R> xv[NV];
         xv[0] = x0[r];
-        if (NV == 2) xv[NV - 1] = x1[r];
+        if (NV >= 2) xv[1] = x1[r];
+        if (NV == 3) xv[NV - 1] = x2[r];
         Cx<R> qv[C];
 #pragma unroll
         for (int c = 0; c < C; c++) qv[c] = qp[c][r];
@@ -264,10 +269,22 @@ __global__ __launch_bounds__(TPB) void k_proj(i64 n, i64 chunk, int k, const Cx<
             for (int vi = 0; vi < NV; vi++)
                 cfma_conj(acc[2 * (vi * C + c)], acc[2 * (vi * C + c) + 1], qv[c], xv[vi]);
     }
-    wave_transpose_reduce<R, LS>(acc);
     const int lane = t & 63, w = t >> 6;
-    // after the butterfly, lane l holds value index l >> (6 - LS)
-    if ((lane & ((64 >> LS) - 1)) == 0) lds[w][lane >> (6 - LS)] = acc[0];
+    if constexpr (NVAL == 48) {
+        // values 0..31 and 32..47 reduced separately; lane l then holds
+        // value l >> 1 of the first group and 32 + (l >> 2) of the second
+        R(&a32)[32] = *reinterpret_cast<R(*)[32]>(&acc[0]);
+        R(&a16)[16] = *reinterpret_cast<R(*)[16]>(&acc[NVAL - 16]);
+        wave_transpose_reduce<R, 5>(a32);
+        wave_transpose_reduce<R, 4>(a16);
+        if ((lane & 1) == 0) lds[w][lane >> 1] = a32[0];
+        if ((lane & 3) == 0) lds[w][32 + (lane >> 2)] = a16[0];
+    } else {
+        R(&av)[1 << LS] = *reinterpret_cast<R(*)[1 << LS]>(&acc[0]);
+        wave_transpose_reduce<R, LS>(av);
+        // after the butterfly, lane l holds value index l >> (6 - LS)
+        if ((lane & ((64 >> LS) - 1)) == 0) lds[w][lane >> (6 - LS)] = av[0];
+    }
     __syncthreads();
     if (t < NVAL) {
         const R s = (lds[0][t] + lds[1][t]) + (lds[2][t] + lds[3][t]);
@@ -284,20 +301,20 @@ __global__ __launch_bounds__(TPB) void k_proj(i64 n, i64 chunk, int k, const Cx<
     }
     if (!fused) return;
     if (!last_block_of(cnt + blockIdx.x, gridDim.y, &lflag)) return;
-    proj_finish<R, NV, C>(c0, gridDim.y, k, part, ldp, pnorm, want_norm && blockIdx.x == 0, out0, out1, scal,
-                          norm_slot, lds4);
+    proj_finish<R, NV, C>(c0, gridDim.y, k, part, ldp, pnorm, want_norm && blockIdx.x == 0, out0, out1, out2,
+                          scal, norm_slot, lds4);
     if (t == 0) cnt[blockIdx.x] = 0;
 }
 
 template <typename R, int NV, int C>
 __global__ __launch_bounds__(TPB) void k_proj_finish(int ntiles, int k, const Cx<R> *part, int ldp, const R *pnorm,
                                                      int want_norm, const int *__restrict__ cond, Cx<R> *out0,
-                                                     Cx<R> *out1, double *scal, int norm_slot)
+                                                     Cx<R> *out1, Cx<R> *out2, double *scal, int norm_slot)
 {
     if (cond && cond[0] == 0) return;
     __shared__ R lds4[4];
     proj_finish<R, NV, C>(blockIdx.x * C, ntiles, k, part, ldp, pnorm, want_norm && blockIdx.x == 0, out0, out1,
-                          scal, norm_slot, lds4);
+                          out2, scal, norm_slot, lds4);
 }
 
 // x = beta*x + alpha * Q[:, :k] h; with fin.mode >= 0 the squared norm of
@@ -963,7 +980,8 @@ static inline int chk(void) { return hipGetLastError() == hipSuccess ? 0 : -1; }
 
 extern "C" int64_t bbjd_red_part_entries(int64_t n, int maxk)
 {
-    return (int64_t)ntiles(n, BBJD_PROJ_MIN_ROWS) * 2 * (maxk + 1) + 64;
+    // up to three vectors per projection (the test-space expansion)
+    return (int64_t)ntiles(n, BBJD_PROJ_MIN_ROWS) * 3 * (maxk + 1) + 64;
 }
 
 // Blocks up to which the finishing step runs in the producing kernel's last
@@ -1000,7 +1018,8 @@ static inline NormFin fin_norm(bbjd_red *rd, int mode, int slot)
 // <- ||x0||.
 template <typename R, int NV>
 static int proj_pass_nv(hipStream_t s, i64 n, int k, const void *Q, i64 ldq, const void *x0, const void *x1,
-                        void *out0, void *out1, bbjd_red *rd, int norm_slot, const int *cond)
+                        const void *x2, void *out0, void *out1, void *out2, bbjd_red *rd, int norm_slot,
+                        const int *cond)
 {
     constexpr int C = NV == 1 ? 16 : 8;
     const int ctiles = k > 0 ? (k + C - 1) / C : 1;
@@ -1022,24 +1041,26 @@ static int proj_pass_nv(hipStream_t s, i64 n, int k, const void *Q, i64 ldq, con
             ea = kt_event(); eb = kt_event();
         }
         hipExtLaunchKernelGGL((k_proj<R, NV, C, false>), g, dim3(TPB), 0, s, ea, eb, 0, (i64)n, chunk, k,
-                              (const Cx<R> *)Q, (i64)ldq, (const Cx<R> *)x0, (const Cx<R> *)x1, (Cx<R> *)rd->part,
-                              ldp, (R *)rd->pnorm, want_norm, cond, rd->cnt, (Cx<R> *)out0, (Cx<R> *)out1,
-                              rd->scal, norm_slot, fused);
+                              (const Cx<R> *)Q, (i64)ldq, (const Cx<R> *)x0, (const Cx<R> *)x1, (const Cx<R> *)x2,
+                              (Cx<R> *)rd->part, ldp, (R *)rd->pnorm, want_norm, cond, rd->cnt, (Cx<R> *)out0,
+                              (Cx<R> *)out1, (Cx<R> *)out2, rd->scal, norm_slot, fused);
         const double bytes = (double)sizeof(Cx<R>) * (double)n * ((double)k + 1.0);
         std::lock_guard<std::mutex> lk(g_kt_mu);
         g_kt.push_back({ea, eb, bytes});
     } else if (cond)
         hipLaunchKernelGGL((k_proj<R, NV, C, true>), g, dim3(TPB), 0, s, (i64)n, chunk, k, (const Cx<R> *)Q,
-                           (i64)ldq, (const Cx<R> *)x0, (const Cx<R> *)x1, (Cx<R> *)rd->part, ldp, (R *)rd->pnorm,
-                           want_norm, cond, rd->cnt, (Cx<R> *)out0, (Cx<R> *)out1, rd->scal, norm_slot, fused);
+                           (i64)ldq, (const Cx<R> *)x0, (const Cx<R> *)x1, (const Cx<R> *)x2, (Cx<R> *)rd->part, ldp,
+                           (R *)rd->pnorm, want_norm, cond, rd->cnt, (Cx<R> *)out0, (Cx<R> *)out1, (Cx<R> *)out2,
+                           rd->scal, norm_slot, fused);
     else
         hipLaunchKernelGGL((k_proj<R, NV, C, false>), g, dim3(TPB), 0, s, (i64)n, chunk, k, (const Cx<R> *)Q,
-                           (i64)ldq, (const Cx<R> *)x0, (const Cx<R> *)x1, (Cx<R> *)rd->part, ldp, (R *)rd->pnorm,
-                           want_norm, cond, rd->cnt, (Cx<R> *)out0, (Cx<R> *)out1, rd->scal, norm_slot, fused);
+                           (i64)ldq, (const Cx<R> *)x0, (const Cx<R> *)x1, (const Cx<R> *)x2, (Cx<R> *)rd->part, ldp,
+                           (R *)rd->pnorm, want_norm, cond, rd->cnt, (Cx<R> *)out0, (Cx<R> *)out1, (Cx<R> *)out2,
+                           rd->scal, norm_slot, fused);
     if (!fused)
         hipLaunchKernelGGL((k_proj_finish<R, NV, C>), dim3(g.x), dim3(TPB), 0, s, nt, k, (const Cx<R> *)rd->part,
-                           ldp, (const R *)rd->pnorm, want_norm, cond, (Cx<R> *)out0, (Cx<R> *)out1, rd->scal,
-                           norm_slot);
+                           ldp, (const R *)rd->pnorm, want_norm, cond, (Cx<R> *)out0, (Cx<R> *)out1,
+                           (Cx<R> *)out2, rd->scal, norm_slot);
     return chk();
 }
 
@@ -1047,8 +1068,9 @@ template <typename R>
 static int proj_pass(hipStream_t s, i64 n, int k, const void *Q, i64 ldq, const void *x0, const void *x1,
                      void *out0, void *out1, bbjd_red *rd, int norm_slot, const int *cond)
 {
-    return x1 ? proj_pass_nv<R, 2>(s, n, k, Q, ldq, x0, x1, out0, out1, rd, norm_slot, cond)
-              : proj_pass_nv<R, 1>(s, n, k, Q, ldq, x0, nullptr, out0, nullptr, rd, norm_slot, cond);
+    return x1 ? proj_pass_nv<R, 2>(s, n, k, Q, ldq, x0, x1, nullptr, out0, out1, nullptr, rd, norm_slot, cond)
+              : proj_pass_nv<R, 1>(s, n, k, Q, ldq, x0, nullptr, nullptr, out0, nullptr, nullptr, rd, norm_slot,
+                                   cond);
 }
 
 template <typename R>
@@ -1222,6 +1244,32 @@ static int cgs2(hipStream_t s, i64 n, int k, const void *Q, i64 ldq, void *v, bb
     return rc;
 }
 
+// The test-space expansion (k <= 128): w = unit CGS2 of w against W, and in
+// the same first pass over W the pencil column W^H a, W^H b (a = A v,
+// b = B v), then the new diagonal entries w^H a, w^H b after the
+// normalisation: colA[0..k], colB[0..k].  Returns 1 without doing anything
+// when the shape takes the unfused path (the caller projects separately).
+template <typename R>
+static int cgs2_unit_pencil(hipStream_t s, i64 n, int k, const void *W, i64 ldw, void *w, const void *a,
+                            const void *b, void *colA, void *colB, bbjd_red *rd, int out_slot)
+{
+    if (k == 0 || !cgs_fusable(k, rd)) return 1;
+    const int ctiles = (k + 7) / 8;
+    const i64 chunk = proj_chunk(n, ctiles);
+    if ((n + chunk - 1) / chunk * 3 * (i64)k > rd->part_cap) return 1;
+    int rc = 0;
+    rc |= proj_pass_nv<R, 3>(s, n, k, W, ldw, w, a, b, rd->h, colA, colB, rd, SL_NB, nullptr);
+    MidFin f;
+    f.scal = rd->scal; f.flag = rd->flag; f.out_slot = out_slot; f.decide = 1;
+    f.h1 = rd->h; f.h2 = rd->h2; f.hsum = nullptr;
+    rc |= cgs_mid<R>(s, n, k, W, ldw, w, rd, f);
+    rc |= cgs_final<R>(s, n, k, W, ldw, w, rd, fin_norm(rd, 0, out_slot), 1, 0);
+    rc |= scale_inv<R>(s, n, w, rd->scal, out_slot, rd->flag + FL_RESCALE);
+    rc |= proj_pass_nv<R, 2>(s, n, 1, w, n, a, b, nullptr, (Cx<R> *)colA + k, (Cx<R> *)colB + k, nullptr, rd, -1,
+                             nullptr);
+    return rc ? -1 : 0;
+}
+
 // v -= Q Q^H v twice (no norm test); fused as in cgs2 for k <= 128.
 template <typename R>
 static int project_out(hipStream_t s, i64 n, int k, const void *Q, i64 ldq, void *v, bbjd_red *rd)
@@ -1324,6 +1372,10 @@ int bbjd_k_cgs2(hipStream_t s, int64_t n, int k, const void *Q, int64_t ldq, voi
 int bbjd_k_cgs2_unit(hipStream_t s, int64_t n, int k, const void *Q, int64_t ldq, void *v, bbjd_red *rd,
                      int out_slot)
 { return cgs2<double>(s, n, k, Q, ldq, v, rd, out_slot, nullptr, 1); }
+
+int bbjd_k_cgs2_unit_pencil(hipStream_t s, int64_t n, int k, const void *W, int64_t ldw, void *w, const void *a,
+                            const void *b, void *colA, void *colB, bbjd_red *rd, int out_slot)
+{ return cgs2_unit_pencil<double>(s, n, k, W, ldw, w, a, b, colA, colB, rd, out_slot); }
 
 void bbjd_set_cgs_fused(int mode) { g_cgs_fused = mode == 2 ? 2 : mode != 0; }
 

@@ -144,6 +144,8 @@ static void m11_block(const bbjd_pencil_desc *d, zcx sigma, int s, bbjd_int n1, 
         }
 }
 
+#define BCR_TAIL_ROWS 256
+
 /* Host BCR factorisation; fills op->bcr (host image) and the offsets. */
 static int bcr_factor(struct bbjd_op *op, const bbjd_pencil_desc *d, zcx **host_out, size_t *nblocks_out)
 {
@@ -165,10 +167,14 @@ static int bcr_factor(struct bbjd_op *op, const bbjd_pencil_desc *d, zcx **host_
     }
     op->final_off = (bbjd_int)nbk;
     nbk += 1;
+    /* The one-workgroup tail starts at the first level with at most
+     * BCR_TAIL_ROWS rows (M s): a level costs it about 0.17 us per block row
+     * (MI355X, s = 8, two right-hand sides), a launched level about 3.5 us
+     * whatever its size, so wider levels run faster as launches. */
     op->l0 = L;
     for (int l = 0; l < L; l++) {
         const bbjd_int M = (N + ((bbjd_int)1 << l) - 1) >> l;
-        if (((M + 1) / 2) * s <= 1024) { op->l0 = l; break; }
+        if (M * s <= BCR_TAIL_ROWS) { op->l0 = l; break; }
     }
     zcx *H = malloc(nbk * ss * sizeof(zcx));
     zcx *Lc = malloc((size_t)N * ss * sizeof(zcx)), *Dc = malloc((size_t)N * ss * sizeof(zcx)),

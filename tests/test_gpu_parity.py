@@ -131,6 +131,42 @@ def test_fused_cgs2_matches_oracle(n):
         lib.bbjd_set_cgs_fused(1)
 
 
+@pytest.mark.parametrize("n,k", [(1003, 1), (1003, 9), (70005, 33), (70005, 100), (20001, 128), (5000, 130)])
+def test_test_space_expansion_pencil_column(n, k):
+    """the expansion's test-space step: the pencil column [W w]^H [Av Bv]
+    taken in the first pass over W (k <= 128) or by separate projections
+    (k = 130), against numpy; w against the oracle's CGS2, normalised; both
+    DGKS branches"""
+    lib = _lib.load()
+    olib = orc.load_oracle()
+    rng = np.random.default_rng(3 * n + k)
+    W = _orthonormal(n, k, n + k + 5)
+    sigma = 0.7 - 1.3j
+    nu0 = 1 / np.sqrt(1 + abs(sigma) ** 2)
+    mu0 = -sigma * nu0
+    for case in ("generic", "near_parallel"):
+        a = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+        b = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+        if case == "near_parallel":
+            a = W @ (rng.standard_normal(k) + 1j * rng.standard_normal(k)) + 1e-7 * a
+            b = W @ (rng.standard_normal(k) + 1j * rng.standard_normal(k)) + 1e-7 * b
+        w0 = nu0 * a + mu0 * b
+        wo, ho = w0.copy(), np.zeros(k, np.complex128)
+        no = olib.orc_block_orthogonalize(n, k, W.ctypes.data, wo.ctypes.data, ho.ctypes.data)
+        wg = w0.copy()
+        cA = np.zeros(k + 1, np.complex128)
+        cB = np.zeros(k + 1, np.complex128)
+        ng = lib.bbjd_expand_test_space(n, k, W.ctypes.data, wg.ctypes.data, a.ctypes.data, b.ctypes.data,
+                                        cA.ctypes.data, cB.ctypes.data)
+        scale = np.linalg.norm(w0)
+        assert abs(ng - no) <= 1e-12 * scale, case
+        assert abs(np.linalg.norm(wg) - 1.0) <= 1e-13, case
+        assert np.linalg.norm(wg - wo / no) <= 1e-12 * scale / no, case
+        Wx = np.column_stack([W, wg])
+        assert np.linalg.norm(cA - Wx.conj().T @ a) <= 1e-12 * np.linalg.norm(a), case
+        assert np.linalg.norm(cB - Wx.conj().T @ b) <= 1e-12 * np.linalg.norm(b), case
+
+
 # super-block sizes hw * bs = 4, 16, 8, 8 (compiled loops) and 12 (generic)
 SHAPES = [(2000, 4, 1, 2), (4096, 8, 2, 16), (1000, 8, 1, 4), (640, 4, 2, 0), (1206, 6, 2, 3)]
 
