@@ -101,6 +101,10 @@ int bbjd_k_gemv_c_c(hipStream_t s, int64_t n, int k, const void *Q, int64_t ldq,
 int bbjd_k_gemv_n(hipStream_t s, int64_t n, int k, const void *Q, int64_t ldq,
                   const void *c, void *y, bbjd_red *rd, int slot);
 
+/* y = Q[:, :k] c for host coefficients c (k <= 128), passed in the kernel
+ * arguments: no upload. */
+int bbjd_k_gemv_n_host(hipStream_t s, int64_t n, int k, const void *Q, int64_t ldq, const void *c, void *y);
+
 /* ||x|| -> scal[slot] */
 int bbjd_k_norm(hipStream_t s, int64_t n, const void *x, bbjd_red *rd, int slot);
 

@@ -976,8 +976,11 @@ static bbjd_status solve_impl(const bbjd_params *P, bbjd_workspace *ws, bbjd_res
         xnrm = sqrt(xnrm);
         if (!(xnrm > 1e-300)) { st = BBJD_ERR_BREAKDOWN; goto done; }
         for (bbjd_int i = 0; i < k; i++) d->xv[i] /= xnrm;
-        rc |= upload_small(c, d->rd.coef, d->xv, k);
-        rc |= bbjd_k_gemv_n(c->s, n, (int)k, ws->V, c->ld, d->rd.coef, u, &d->rd, -1);
+        if (k <= 128) rc |= bbjd_k_gemv_n_host(c->s, n, (int)k, ws->V, c->ld, d->xv, u);
+        else {
+            rc |= upload_small(c, d->rd.coef, d->xv, k);
+            rc |= bbjd_k_gemv_n(c->s, n, (int)k, ws->V, c->ld, d->rd.coef, u, &d->rd, -1);
+        }
         /* ---- step 3: residual ---- */
         if (apply_op(c, P->matvec_A, u, Au) || apply_op(c, P->matvec_B, u, Bu)) goto cb_err;
         rc |= bbjd_k_lincomb_norm(c->s, n, creal(tb), cimag(tb), Au, -creal(ta), -cimag(ta), Bu, r, &d->rd, SL_R);

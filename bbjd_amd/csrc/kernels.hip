@@ -22,6 +22,7 @@
 //    the mixed-precision inner solve.
 #include <hip/hip_runtime.h>
 #include <hip/hip_ext.h>
+#include <cstring>
 #include <mutex>
 #include <vector>
 #include "engine.h"
@@ -703,6 +704,49 @@ __global__ __launch_bounds__(TPB) void k_cgs_final(i64 n, int k, const Cx<R> *__
     if (!fused || !last_block_of(cnt, gridDim.x, &lflag)) return;
     finish_norm<R>(pnorm, gridDim.x, fin, lds4);
     if (t == 0) *cnt = 0;
+}
+
+// y = Q[:, :k] c with the k <= 128 coefficients passed by value in the
+// kernel arguments (no upload before the launch); columns split over the
+// four waves as in k_update_split.
+struct Coef128 { Cx<double> v[128]; };
+
+template <int RW>
+__global__ __launch_bounds__(TPB) void k_gemv_n_karg(i64 n, int k, const Cx<double> *__restrict__ Q, i64 ldq,
+                                                     Coef128 c, Cx<double> *__restrict__ y)
+{
+    __shared__ Cx<double> red[4][64 * RW];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const i64 rb = (i64)blockIdx.x * (64 * RW);
+    Cx<double> a[RW];
+#pragma unroll
+    for (int e = 0; e < RW; e++) a[e] = cx(0.0, 0.0);
+    if (rb + 64 * RW <= n) {
+#pragma unroll 8
+        for (int j = w; j < k; j += 4) {
+            const Cx<double> hj = c.v[j];
+            const Cx<double> *q = Q + (i64)j * ldq + rb + lane;
+#pragma unroll
+            for (int e = 0; e < RW; e++) cfma(a[e], q[e * 64], hj);
+        }
+    } else {
+        for (int j = w; j < k; j += 4) {
+            const Cx<double> hj = c.v[j];
+            const Cx<double> *q = Q + (i64)j * ldq;
+#pragma unroll
+            for (int e = 0; e < RW; e++) {
+                const i64 r = rb + lane + e * 64;
+                if (r < n) cfma(a[e], q[r], hj);
+            }
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < RW; e++) red[w][lane + e * 64] = a[e];
+    __syncthreads();
+    if (t < 64 * RW) {
+        const i64 r = rb + t;
+        if (r < n) y[r] = (red[0][t] + red[1][t]) + (red[2][t] + red[3][t]);
+    }
 }
 
 // separate finish of a norm (one workgroup)
@@ -1401,6 +1445,16 @@ int bbjd_k_gemv_c_c(hipStream_t s, int64_t n, int k, const void *Q, int64_t ldq,
 int bbjd_k_gemv_n(hipStream_t s, int64_t n, int k, const void *Q, int64_t ldq, const void *c, void *y,
                   bbjd_red *rd, int slot)
 { return gemv_n<double>(s, n, k, Q, ldq, c, y, rd, slot); }
+
+int bbjd_k_gemv_n_host(hipStream_t s, int64_t n, int k, const void *Q, int64_t ldq, const void *hc, void *y)
+{
+    if (k < 1 || k > 128) return -2;
+    Coef128 c;
+    memcpy(c.v, hc, (size_t)k * sizeof(Cx<double>));
+    hipLaunchKernelGGL(k_gemv_n_karg<2>, dim3(ntiles(n, 128)), dim3(TPB), 0, s, (i64)n, k, (const Cx<double> *)Q,
+                       (i64)ldq, c, (Cx<double> *)y);
+    return chk();
+}
 
 int bbjd_k_gemv_n_c(hipStream_t s, int64_t n, int k, const void *Q, int64_t ldq, const void *c, void *y,
                     bbjd_red *rd, int slot)

@@ -141,6 +141,55 @@ __device__ __forceinline__ void bcr_bwd_row(int s_rt, i64 M, i64 h, i64 j, int r
     for (int q = 0; q < NR; q++) x[q * vs + i * s + r] = acc[q];
 }
 
+// Level 0 of the forward reduction reading the right-hand sides from the
+// caller's vectors (n1 rows; the padding rows of the last super-block read
+// as zero): every row of b is written, the even super-rows reduced
+// (b_i = x_i - alpha_j x_{i-1} - gamma_j x_{i+1}), the odd ones copied --
+// the copy of the input into b and the first level in one pass.
+template <int NR, int S>
+__global__ __launch_bounds__(TPB) void k_bcr_fwd0(int s_rt, i64 N, i64 n1, const Cx<double> *__restrict__ AG,
+                                                  const Cx<double> *__restrict__ x0,
+                                                  const Cx<double> *__restrict__ x1, Cx<double> *b, i64 vs)
+{
+    const int s = S > 0 ? S : s_rt;
+    const i64 gid = (i64)blockIdx.x * TPB + threadIdx.x;
+    const i64 i = gid / s, ss = (i64)s * s;
+    const int r = (int)(gid % s);
+    if (i >= N) return;
+    const Cx<double> *xs[2] = {x0, x1};
+    Cx<double> acc[NR];
+#pragma unroll
+    for (int q = 0; q < NR; q++) {
+        const i64 e = i * s + r;
+        acc[q] = e < n1 ? xs[q][e] : cx0();
+    }
+    if ((i & 1) == 0) {
+        const i64 j = i / 2;
+        // neighbour super-row (i -/+ 1) times its coupling block; the left
+        // neighbour is always whole, the right one may hold padding rows
+        auto sub = [&](const Cx<double> *blk, i64 o) {
+            if (o + s <= n1) {
+#pragma unroll
+                for (int c = 0; c < s; c++) {
+                    const Cx<double> a = blk[(i64)c * s];
+#pragma unroll
+                    for (int q = 0; q < NR; q++) cfms(acc[q], a, xs[q][o + c]);
+                }
+            } else {
+                for (int c = 0; c < s && o + c < n1; c++) {
+                    const Cx<double> a = blk[(i64)c * s];
+#pragma unroll
+                    for (int q = 0; q < NR; q++) cfms(acc[q], a, xs[q][o + c]);
+                }
+            }
+        };
+        if (i >= 1) sub(AG + j * 2 * ss + r, (i - 1) * s);
+        if (i + 1 < N) sub(AG + j * 2 * ss + ss + r, (i + 1) * s);
+    }
+#pragma unroll
+    for (int q = 0; q < NR; q++) b[q * vs + i * s + r] = acc[q];
+}
+
 // S: the super-block size when known at compile time (the row loops unroll
 // and every factor load of a row is in flight at once), 0 for any s.
 template <int NR, int S>
@@ -468,6 +517,20 @@ int bbjd_o_bcr_fwd(hipStream_t st, int s, int64_t M, int64_t h, int64_t ne, cons
                        (const Cx<double> *)AG, (Cx<double> *)b, (i64)vs)
     BBJD_BCR_DISPATCH(BBJD_FWD)
 #undef BBJD_FWD
+    return chk();
+}
+
+int bbjd_o_bcr_fwd0(hipStream_t st, int s, int64_t N, int64_t n1, const void *AG, const void *x0, const void *x1,
+                    void *b, int nrhs, int64_t vs)
+{
+    if (N <= 0) return 0;
+    if (s > 64 || (nrhs == 2 && !x1)) return -2;
+#define BBJD_FWD0(NR, S)                                                                                        \
+    hipLaunchKernelGGL((k_bcr_fwd0<NR, S>), dim3(blocks_for(N * s)), dim3(TPB), 0, st, s, (i64)N, (i64)n1,        \
+                       (const Cx<double> *)AG, (const Cx<double> *)x0, (const Cx<double> *)(x1 ? x1 : x0),       \
+                       (Cx<double> *)b, (i64)vs)
+    BBJD_BCR_DISPATCH(BBJD_FWD0)
+#undef BBJD_FWD0
     return chk();
 }
 

@@ -12,6 +12,10 @@ int bbjd_o_diag_mv(hipStream_t s, int64_t n, const void *dg, const void *x, void
 int bbjd_o_bj_apply(hipStream_t s, int64_t n, int bs, const void *inv, const void *x, void *y, int single);
 int bbjd_o_bcr_fwd(hipStream_t st, int s, int64_t M, int64_t h, int64_t ne, const void *AG, void *b, int nrhs,
                    int64_t vs);
+/* level 0 of the forward reduction with the right-hand sides read from x0
+ * (x1) (n1 rows each): writes all N s rows of b */
+int bbjd_o_bcr_fwd0(hipStream_t st, int s, int64_t N, int64_t n1, const void *AG, const void *x0, const void *x1,
+                    void *b, int nrhs, int64_t vs);
 int bbjd_o_bcr_bwd(hipStream_t st, int s, int64_t M, int64_t h, int64_t no, const void *DPQ, const void *b, void *x,
                    int nrhs, int64_t vs);
 /* first level of the one-workgroup tail for this shape (>= l0, where the

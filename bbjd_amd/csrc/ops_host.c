@@ -247,17 +247,30 @@ static hipStream_t op_stream(const struct bbjd_op *op)
     return s ? s : op->stream;
 }
 
-/* BCR solve of nrhs right-hand sides already in op->bvec (vector q at
- * q N s) into op->xvec: the large levels one launch each, the levels from
- * op->l0 on, the final solve and their backward levels in one workgroup. */
-static int bcr_solve(struct bbjd_op *op, hipStream_t st, int nrhs)
+/* BCR solve of nrhs right-hand sides into op->xvec: the large levels one
+ * launch each, the levels from op->l0 on, the final solve and their backward
+ * levels in one workgroup.  With x0 (x1) the right-hand sides are the
+ * caller's n1-row vectors, read by the first level (or copied into op->bvec
+ * when every level is in the tail); without, they are already in op->bvec
+ * (vector q at q N s). */
+static int bcr_solve(struct bbjd_op *op, hipStream_t st, int nrhs, const void *x0, const void *x1)
 {
     const int s = op->s;
     const size_t ss = (size_t)s * s;
     const int64_t vs = (int64_t)op->N * s;
     int rc = 0;
     const int lt = bbjd_o_bcr_tail_level(s, op->N, op->l0, op->L, nrhs);
-    for (int l = 0; l < lt; l++) {
+    int l = 0;
+    if (x0 && lt >= 1 && s <= 64) {
+        rc |= bbjd_o_bcr_fwd0(st, s, op->N, op->n1, (const zcx *)op->bcr + (size_t)op->even_off[0] * ss, x0, x1,
+                              op->bvec, nrhs, vs);
+        l = 1;
+    } else if (x0) {
+        rc |= !HIPOK(hipMemcpyAsync(op->bvec, x0, (size_t)op->n1 * 16, hipMemcpyDeviceToDevice, st));
+        if (nrhs == 2)
+            rc |= !HIPOK(hipMemcpyAsync((zcx *)op->bvec + vs, x1, (size_t)op->n1 * 16, hipMemcpyDeviceToDevice, st));
+    }
+    for (; l < lt; l++) {
         const bbjd_int h = (bbjd_int)1 << l, M = (op->N + h - 1) >> l;
         rc |= bbjd_o_bcr_fwd(st, s, M, h, (M + 1) / 2, (const zcx *)op->bcr + (size_t)op->even_off[l] * ss,
                              op->bvec, nrhs, vs);
@@ -279,10 +292,7 @@ static int exact_apply(struct bbjd_op *op, hipStream_t st, int nrhs, const void 
     int rc = 0;
     const bbjd_int n1 = op->n1;
     const int64_t vs = (int64_t)op->N * op->s;
-    rc |= !HIPOK(hipMemcpyAsync(op->bvec, x0, (size_t)n1 * 16, hipMemcpyDeviceToDevice, st));
-    if (nrhs == 2)
-        rc |= !HIPOK(hipMemcpyAsync((zcx *)op->bvec + vs, x1, (size_t)n1 * 16, hipMemcpyDeviceToDevice, st));
-    rc |= bcr_solve(op, st, nrhs);
+    rc |= bcr_solve(op, st, nrhs, x0, x1);
     if (op->ntau == 0) {
         rc |= !HIPOK(hipMemcpyAsync(y0, op->xvec, (size_t)n1 * 16, hipMemcpyDeviceToDevice, st));
         if (nrhs == 2)
@@ -496,7 +506,7 @@ bbjd_op *bbjd_op_create(const bbjd_pencil_desc *d, bbjd_zomplex sigma, bbjd_pc_k
                 memset(col, 0, (size_t)n1 * sizeof(zcx));
                 for (bbjd_int i = lo; i < n1; i++) col[i] = m_at(d, sigma, i, n1 + a);
                 rc |= !HIPOK(hipMemcpy(op->bvec, col, (size_t)n1 * 16, hipMemcpyHostToDevice));
-                rc |= bcr_solve(op, st, 1);
+                rc |= bcr_solve(op, st, 1, NULL, NULL);
                 rc |= !HIPOK(hipMemcpyAsync((zcx *)op->Y + (size_t)a * n1, op->xvec, (size_t)n1 * 16,
                                             hipMemcpyDeviceToDevice, st));
                 /* M21 Y[:, a] */

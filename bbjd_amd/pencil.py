@@ -158,3 +158,17 @@ class DeviceOperator:
 
     def solve(self, x):
         return self._apply("bbjd_op_precond", x)
+
+    def solve2(self, x0, x1):
+        """both preconditioner solves of one correction step in one pass
+        (bbjd_op_precond2)"""
+        lib = _lib.load()
+        x0 = np.ascontiguousarray(x0, dtype=np.complex128)
+        x1 = np.ascontiguousarray(x1, dtype=np.complex128)
+        d0, d1 = _lib.DeviceBuffer.from_array(x0), _lib.DeviceBuffer.from_array(x1)
+        e0, e1 = _lib.DeviceBuffer(x0.nbytes), _lib.DeviceBuffer(x1.nbytes)
+        rc = lib.bbjd_op_precond2(self.n, d0.ptr, e0.ptr, d1.ptr, e1.ptr, self.handle)
+        rc |= lib.bbjd_dev_sync()
+        if rc:
+            raise RuntimeError("bbjd_op_precond2 failed")
+        return e0.to_array(np.complex128, self.n), e1.to_array(np.complex128, self.n)

@@ -187,6 +187,13 @@ def test_device_operators_match_oracle(n, bs, hw, ntau):
         # the preconditioner solves M y = x: both are exact solves, compare the
         # solutions and the residual of the device one
         assert np.linalg.norm(yp - rp) <= 1e-9 * np.linalg.norm(rp), pc
+        if pc == "exact":
+            # two right-hand sides in one pass: each as the one-vector solve
+            x2 = rng.standard_normal(n) - 0.5j * rng.standard_normal(n)
+            z0, z1 = dev.solve2(x, x2)
+            assert np.linalg.norm(z0 - yp) <= 1e-14 * np.linalg.norm(yp)
+            r2 = cpu.apply("orc_precond", x2)
+            assert np.linalg.norm(z1 - r2) <= 1e-9 * np.linalg.norm(r2)
         dev.free()
 
 
