@@ -1569,7 +1569,8 @@ int bbjd_k_c2z(hipStream_t s, int64_t n, const void *c, void *z)
 // Kernel benchmark: times one kernel in isolation with HIP events on a private
 // stream.  which: 0 k_proj one vector, 1 k_proj two vectors, 2 k_update,
 // 3 k_rotate (k -> k/2 columns), 4 full CGS2 (one pass + skipped pass 2),
-// 5 a plain copy of n complex values.
+// 5 a plain copy of n complex values, 6 the two-pass projection (both
+// passes always run: the fused form's three reads of Q for k <= 128).
 // Returns the average ms per launch and the algorithmic bytes per launch.
 __global__ void k_fill(i64 cnt, Cx<double> *a, double scale)
 {
@@ -1638,6 +1639,12 @@ int bbjd_kbench(int which, int64_t n, int k, int reps, double *avg_ms, double *b
             hipLaunchKernelGGL(k_copy, dim3(2048), dim3(TPB), 0, s, (i64)n, (const Cx<double> *)x,
                                (Cx<double> *)x2);
             by = 32.0 * n;
+            break;
+        case 6:
+            (void)project_out<double>(s, n, k, Q, ld, x, &rd);
+            // fused (k <= 128): three reads of Q, x read three times and
+            // written twice; otherwise four passes of k + 2
+            by = cgs_fusable(k, &rd) ? 16.0 * n * (3.0 * k + 5) : 16.0 * n * (4.0 * k + 6);
             break;
         case 3:
             bbjd_k_rotate(s, n, k, k / 2 > 0 ? k / 2 : 1, Q, ld, Zr);

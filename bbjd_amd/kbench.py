@@ -5,7 +5,9 @@
 KERNEL: proj (Q^H x), proj2 (Q^H [x y]), update (x -= Q h), rotate (in-place
 basis rotation k -> k/2), cgs2 (one CGS2 with the second pass skipped on the
 device), copy (plain copy of N complex values, the reference for
-counter-measured HBM traffic).  Prints one line: kernel, shape, average us per
+counter-measured HBM traffic), projout (the two-pass projection v -= Q Q^H v,
+both passes always: for k <= 128 the fused form, first update and second
+projection in one pass over Q).  Prints one line: kernel, shape, average us per
 launch, algorithmic bytes per launch and GB/s.
 """
 from __future__ import annotations
@@ -15,7 +17,7 @@ import ctypes as C
 
 from . import _lib
 
-KERNELS = {"proj": 0, "proj2": 1, "update": 2, "rotate": 3, "cgs2": 4, "copy": 5}
+KERNELS = {"proj": 0, "proj2": 1, "update": 2, "rotate": 3, "cgs2": 4, "copy": 5, "projout": 6}
 
 
 def run(kernel, n, k, reps=50, proj_blocks=0, upd_rows=0, blocks_target=0):

@@ -165,7 +165,8 @@ void bbjd_set_cgs_fused(int mode);
  * kernel in isolation at shape (n, k) on a private stream, reps launches.
  * which: 0 projection Q^H x, 1 projection of two vectors, 2 update
  * x -= Q h, 3 in-place basis rotation k -> k/2, 4 full CGS2, 5 a plain copy
- * of n complex values (the reference for counter-measured traffic).  Writes
+ * of n complex values (the reference for counter-measured traffic), 6 the
+ * two-pass projection with both passes run.  Writes
  * the average ms per launch and the algorithmic bytes per launch. */
 int bbjd_kbench(int which, int64_t n, int k, int reps, double *avg_ms, double *bytes);
 
