@@ -105,10 +105,11 @@ def load():
     _sig(lib, "bbjd_block_gram", C.c_int, I64, I64, C.c_void_p, I64, C.c_void_p, C.c_void_p)
     _sig(lib, "bbjd_set_fuse_blocks", None, C.c_int)
     _sig(lib, "bbjd_set_cgs_fused", None, C.c_int)
+    _sig(lib, "bbjd_set_gram_blocks", None, C.c_int)
     _sig(lib, "bbjd_block_orthonormalize", D, I64, I64, C.c_void_p, C.c_void_p, C.c_void_p)
     _sig(lib, "bbjd_project_out_host", C.c_int, I64, I64, C.c_void_p, C.c_void_p)
-    _sig(lib, "bbjd_expand_test_space", D, I64, I64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
-         C.c_void_p, C.c_void_p)
+    _sig(lib, "bbjd_expand_test_space", D, I64, I64, C.c_void_p, zcomplex, zcomplex, C.c_void_p, C.c_void_p,
+         C.c_void_p, C.c_void_p, C.c_void_p)
     _sig(lib, "bbjd_memcpy_d2h", C.c_int, V, V, C.c_size_t)
     _sig(lib, "bbjd_memcpy_h2d", C.c_int, V, V, C.c_size_t)
     _sig(lib, "bbjd_pencil_generate", C.c_int, I64, C.c_int, C.c_int, C.c_int, C.c_uint64, D, V, V, V)

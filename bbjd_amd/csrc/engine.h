@@ -37,7 +37,7 @@ enum {
     SL_COUNT = 16
 };
 /* Int flag slots.  FL_RESCALE: the fused CGS2 could not normalise its result
- * in the second pass and left it to a conditional scaling. */
+ * in the second pass and left it to the block that finishes its norm. */
 enum { FL_PASS2 = 0, FL_ONE = 1, FL_DGKS_COUNT = 2, FL_RESCALE = 3, FL_COUNT = 8 };
 
 /* Counters of the last-block-done reductions: [0, BBJD_CNT_UPD) one per
@@ -77,12 +77,14 @@ int bbjd_k_cgs2(hipStream_t s, int64_t n, int k, const void *Q, int64_t ldq, voi
 int bbjd_k_cgs2_unit(hipStream_t s, int64_t n, int k, const void *Q, int64_t ldq, void *v,
                      bbjd_red *rd, int out_slot);
 
-/* Test-space expansion: bbjd_k_cgs2_unit of w against W, which also yields
- * the projected pencil column colA[0..k] = [W w]^H a, colB[0..k] = [W w]^H b
- * (device arrays; a = A v, b = B v).  Returns 1, having done nothing, when
- * the shape takes the unfused path. */
+/* Test-space expansion: w = bbjd_k_cgs2_unit of (ar + i ai) a + (br + i bi) b
+ * against W (a = A v, b = B v; w is written only as the result), which also
+ * yields the projected pencil column colA[0..k] = [W w]^H a,
+ * colB[0..k] = [W w]^H b (device arrays).  Returns 1, having done nothing,
+ * when the shape takes the unfused path. */
 int bbjd_k_cgs2_unit_pencil(hipStream_t s, int64_t n, int k, const void *W, int64_t ldw, void *w, const void *a,
-                            const void *b, void *colA, void *colB, bbjd_red *rd, int out_slot);
+                            const void *b, double are, double aim, double bre, double bim, void *colA, void *colB,
+                            bbjd_red *rd, int out_slot);
 
 /* v -= Q Q^H v twice (no norm test), k columns. */
 int bbjd_k_project_out(hipStream_t s, int64_t n, int k, const void *Q, int64_t ldq, void *v,

@@ -812,15 +812,18 @@ static int expand(ctx_t *c, bbjd_int k, const void *z, zcx nu0, zcx mu0)
         rc |= bbjd_k_cgs2_unit(c->s, n, (int)k, ws->V, c->ld, vn, &d->rd, SL_V);
         if (rc) return -2;
         if (apply_op(c, P->matvec_A, vn, an) || apply_op(c, P->matvec_B, vn, bn)) return -1;
-        rc |= bbjd_k_lincomb(c->s, n, creal(nu0), cimag(nu0), an, creal(mu0), cimag(mu0), bn, wn);
-        /* the pencil column comes with the first pass over W when fused */
+        /* fused: w = nu0 Av + mu0 Bv is formed inside the first passes over W,
+         * which also give the pencil column */
         zcx *pen = (zcx *)d->pencil;
-        const int pc = bbjd_k_cgs2_unit_pencil(c->s, n, (int)k, ws->W, c->ld, wn, an, bn, pen, pen + (c->ldm + 1),
-                                               &d->rd, SL_W);
+        const int pc = bbjd_k_cgs2_unit_pencil(c->s, n, (int)k, ws->W, c->ld, wn, an, bn, creal(nu0), cimag(nu0),
+                                               creal(mu0), cimag(mu0), pen, pen + (c->ldm + 1), &d->rd, SL_W);
         if (pc < 0) return -2;
-        if (pc > 0) rc |= bbjd_k_cgs2_unit(c->s, n, (int)k, ws->W, c->ld, wn, &d->rd, SL_W);
+        if (pc > 0) {
+            rc |= bbjd_k_lincomb(c->s, n, creal(nu0), cimag(nu0), an, creal(mu0), cimag(mu0), bn, wn);
+            rc |= bbjd_k_cgs2_unit(c->s, n, (int)k, ws->W, c->ld, wn, &d->rd, SL_W);
+        }
         if (rc || pencil_update(c, k, !c->w_mixed, pc == 0)) return -2;
-        d->st.stream_bytes += 16.0 * n * (2.0 * k + 4 + 3 + 2.0 * k + 2 + (pc == 0 ? 2 : 0));
+        d->st.stream_bytes += 16.0 * n * (2.0 * k + 4 + 2.0 * k + 2 + (pc == 0 ? 2 : 3));
         const double nv = d->hscal[SL_V], nw = d->hscal[SL_W];
         if (nv >= 1e-14 && nw >= 1e-14) return 0;
         if (nv >= 1e-14) {
@@ -868,7 +871,11 @@ static bbjd_status solve_impl(const bbjd_params *P, bbjd_workspace *ws, bbjd_res
     struct bbjd_dev *d = ws->dev;
     const double t_start = now_s();
     memset(&d->st, 0, sizeof(d->st));
-    ctx_t cx = {ws, d, P, d->stream, P->n, ws->ld, ms_solve, ws->maxsub};
+    /* n-sized arrays are laid out for this problem (leading dimension
+     * roundup(n, 64)) inside the workspace's buffers, which may be sized for a
+     * larger one: a resident workspace serving a small request then touches
+     * only the start of each buffer instead of one stretch per column. */
+    ctx_t cx = {ws, d, P, d->stream, P->n, (P->n + 63) / 64 * 64, ms_solve, ws->maxsub};
     ctx_t *c = &cx;
     const bbjd_int n = P->n, nev = P->nev, ms = ms_solve, ldm = ws->maxsub;
     d->nconv_last = 0;
@@ -1223,26 +1230,29 @@ int bbjd_project_out_host(bbjd_int n, bbjd_int k, const bbjd_zomplex *Q, bbjd_zo
     return host_orth_step(n, k, Q, v, 0, NULL) == 0.0 ? 0 : -1;
 }
 
-double bbjd_expand_test_space(bbjd_int n, bbjd_int k, const bbjd_zomplex *W, bbjd_zomplex *w, const bbjd_zomplex *a,
-                              const bbjd_zomplex *b, bbjd_zomplex *colA, bbjd_zomplex *colB)
+double bbjd_expand_test_space(bbjd_int n, bbjd_int k, const bbjd_zomplex *W, bbjd_zomplex ca, bbjd_zomplex cb,
+                              const bbjd_zomplex *a, const bbjd_zomplex *b, bbjd_zomplex *w, bbjd_zomplex *colA,
+                              bbjd_zomplex *colB)
 {
     helper_t h;
     void *dW = NULL;
     const size_t wb = (size_t)n * (k > 0 ? k : 0) * 16, vb = ((size_t)n * 16 + 255) / 256 * 256,
-                 cb = ((size_t)(k + 1) * 16 + 255) / 256 * 256;
+                 colb = ((size_t)(k + 1) * 16 + 255) / 256 * 256;
     double res = NAN;
     if (k < 1) return NAN;
-    if (helper_open(&h, n, k, (wb + 255) / 256 * 256 + 3 * vb + 2 * cb, &dW) == 0) {
+    if (helper_open(&h, n, k, (wb + 255) / 256 * 256 + 3 * vb + 2 * colb, &dW) == 0) {
         char *p = (char *)dW + (wb + 255) / 256 * 256;
-        void *dw = p, *da = p + vb, *db = p + 2 * vb, *dA = p + 3 * vb, *dB = p + 3 * vb + cb;
+        void *dw = p, *da = p + vb, *db = p + 2 * vb, *dA = p + 3 * vb, *dB = p + 3 * vb + colb;
         int rc = 0;
         rc |= !HIPOK(hipMemcpyAsync(dW, W, wb, hipMemcpyHostToDevice, h.s));
-        rc |= !HIPOK(hipMemcpyAsync(dw, w, (size_t)n * 16, hipMemcpyHostToDevice, h.s));
         rc |= !HIPOK(hipMemcpyAsync(da, a, (size_t)n * 16, hipMemcpyHostToDevice, h.s));
         rc |= !HIPOK(hipMemcpyAsync(db, b, (size_t)n * 16, hipMemcpyHostToDevice, h.s));
-        int pc = rc ? -1 : bbjd_k_cgs2_unit_pencil(h.s, n, (int)k, dW, n, dw, da, db, dA, dB, &h.rd, SL_T0);
+        int pc = rc ? -1
+                    : bbjd_k_cgs2_unit_pencil(h.s, n, (int)k, dW, n, dw, da, db, creal(ca), cimag(ca), creal(cb),
+                                              cimag(cb), dA, dB, &h.rd, SL_T0);
         if (pc > 0) {
             /* unfused shape: the expansion's other route */
+            rc |= bbjd_k_lincomb(h.s, n, creal(ca), cimag(ca), da, creal(cb), cimag(cb), db, dw);
             rc |= bbjd_k_cgs2_unit(h.s, n, (int)k, dW, n, dw, &h.rd, SL_T0);
             rc |= bbjd_k_gemv_c(h.s, n, (int)k, dW, n, da, db, dA, dB, &h.rd);
             rc |= bbjd_k_gemv_c(h.s, n, 1, dw, n, da, db, (char *)dA + (size_t)k * 16, (char *)dB + (size_t)k * 16,

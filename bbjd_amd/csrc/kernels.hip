@@ -142,17 +142,23 @@ __device__ __forceinline__ bool last_block_of(int *cnt, int group_size, int *lds
 //          ||x|| < 0.7071 scal[SL_NB]; hsum[:k] = h when hsum
 //  mode 2: after CGS pass 2 (runs only when flagged): scal[slot] = ||x||,
 //          hsum[:k] = h + h2 when hsum
+//  mode 3: after the second pass of the fused CGS2 with unit output:
+//          scal[slot] = ||x||, and x /= ||x|| by this block when the pass
+//          left the scaling to it (flag[FL_RESCALE]; rare)
 struct NormFin {
     double *scal;
     int *flag;
     int mode, slot, k;
     void *hsum;
     const void *h, *h2;
+    void *x;             // mode 3: the vector to rescale (n entries)
+    i64 n;
 };
 
 template <typename R>
 __device__ __forceinline__ void finish_norm(const R *pnorm, int nblk, const NormFin &f, R *lds4)
 {
+    __shared__ double lsc;
     R s = 0;
     for (int t = threadIdx.x; t < nblk; t += TPB) s += pnorm[t];
     s = block_sum(s, lds4);
@@ -165,6 +171,20 @@ __device__ __forceinline__ void finish_norm(const R *pnorm, int nblk, const Norm
             f.flag[FL_PASS2] = p2;
             if (p2) f.flag[FL_DGKS_COUNT] += 1;
         }
+        if (f.mode == 3) lsc = (f.flag[FL_RESCALE] && nrm > 1e-300) ? 1.0 / nrm : 0.0;
+    }
+    if (f.mode == 3) {
+        __syncthreads();
+        const R sc = (R)lsc;
+        if (lsc != 0.0) {
+            Cx<R> *x = reinterpret_cast<Cx<R> *>(f.x);
+            for (i64 i = threadIdx.x; i < f.n; i += TPB) {
+                Cx<R> v = x[i];
+                v.re *= sc;
+                v.im *= sc;
+                x[i] = v;
+            }
+        }
     }
     if (f.hsum) {
         Cx<R> *hs = reinterpret_cast<Cx<R> *>(f.hsum);
@@ -173,7 +193,8 @@ __device__ __forceinline__ void finish_norm(const R *pnorm, int nblk, const Norm
     }
 }
 
-// Projection Q^H x (and Q^H x1): a block owns a chunk of rows x C columns
+// Projection Q^H x (and Q^H x1; with three vectors Q^H [x, x1, x2] for
+// x = ca x1 + cb x2, formed in registers): a block owns a chunk of rows x C columns
 // and writes its partial sums part[(tile*NV + vi)*ldp + col]; the last row
 // tile of each column tile to finish reduces that column tile's partials into
 // out0/out1.  With want_norm the column-tile-0 blocks also produce ||x0||^2
@@ -223,9 +244,10 @@ __device__ __forceinline__ void proj_finish(int c0, int ntiles, int k, const Cx<
 template <typename R, int NV, int C, bool COND>
 __global__ __launch_bounds__(TPB) void k_proj(i64 n, i64 chunk, int k, const Cx<R> *__restrict__ Q, i64 ldq,
                                               const Cx<R> *__restrict__ x0, const Cx<R> *__restrict__ x1,
-                                              const Cx<R> *__restrict__ x2, Cx<R> *part, int ldp, R *pnorm,
-                                              int want_norm, const int *__restrict__ cond, int *cnt, Cx<R> *out0,
-                                              Cx<R> *out1, Cx<R> *out2, double *scal, int norm_slot, int fused)
+                                              const Cx<R> *__restrict__ x2, Cx<R> ca, Cx<R> cb, Cx<R> *part,
+                                              int ldp, R *pnorm, int want_norm, const int *__restrict__ cond,
+                                              int *cnt, Cx<R> *out0, Cx<R> *out1, Cx<R> *out2, double *scal,
+                                              int norm_slot, int fused)
 {
     if (COND && cond[0] == 0) return;
     constexpr int NVAL = 2 * NV * C;
@@ -257,9 +279,15 @@ __global__ __launch_bounds__(TPB) void k_proj(i64 n, i64 chunk, int k, const Cx<
 #pragma unroll 2
     for (i64 r = (i64)tile * chunk + t; r < r1; r += TPB) {
         Cx<R> xv[NV];
-        xv[0] = x0[r];
-        if (NV >= 2) xv[1] = x1[r];
-        if (NV == 3) xv[NV - 1] = x2[r];
+        if (NV == 3) {
+            // three vectors: x1, x2 and their combination ca x1 + cb x2
+            xv[1] = x1[r];
+            xv[NV - 1] = x2[r];
+            xv[0] = ca * xv[1] + cb * xv[NV - 1];
+        } else {
+            xv[0] = x0[r];
+            if (NV == 2) xv[NV - 1] = x1[r];
+        }
         Cx<R> qv[C];
 #pragma unroll
         for (int c = 0; c < C; c++) qv[c] = qp[c][r];
@@ -456,8 +484,8 @@ __global__ __launch_bounds__(TPB) void k_update_split(i64 n, int k, const Cx<R> 
 // Unit-norm output folds the normalisation into the last launch: without the
 // second pass v1 / ||v1||; with it v2 / sqrt(||v1||^2 - ||h2||^2), which is
 // v2 / ||v2|| to rounding while ||h2||^2 <= 3/4 ||v1||^2 (Q orthonormal);
-// otherwise v2 stays unscaled, FL_RESCALE is set, and a conditional scaling
-// by the computed ||v2|| follows.
+// otherwise v2 stays unscaled, FL_RESCALE is set, and the block that finishes
+// ||v2|| scales it (rare: v0 in span(Q) to a few ulp).
 struct MidFin {
     double *scal;
     int *flag;
@@ -515,13 +543,15 @@ __device__ __forceinline__ void cgs_mid_finish(int col0, int col_step, bool writ
     }
 }
 
-// v1 = v0 - Q h1 (in place) and the block's partials of Q^H v1 (part[block
-// k + j]) and, with fin.decide, of ||v1||^2.  KW >= ceil(k / 4) columns per
-// wave; a block walks `tiles` consecutive 64-row tiles.
+// v1 = v0 - Q h1 (in place; with xa, v0 = ca xa + cb xb and v1 goes to x)
+// and the block's partials of Q^H v1 (part[block k + j]) and, with
+// fin.decide, of ||v1||^2.  KW >= ceil(k / 4) columns per wave; a block walks
+// `tiles` consecutive 64-row tiles.
 template <typename R, int KW>
 __global__ __launch_bounds__(TPB) void k_cgs_mid(i64 n, int k, const Cx<R> *__restrict__ Q, i64 ldq,
-                                                 const Cx<R> *__restrict__ h, Cx<R> *x, int tiles, Cx<R> *part,
-                                                 R *pnorm, int *cnt, MidFin fin, int fused)
+                                                 const Cx<R> *__restrict__ h, Cx<R> *x, const Cx<R> *__restrict__ xa,
+                                                 const Cx<R> *__restrict__ xb, Cx<R> ca, Cx<R> cb, int tiles,
+                                                 Cx<R> *part, R *pnorm, int *cnt, MidFin fin, int fused)
 {
     constexpr int NVAL = 2 * KW;                  // projection values per lane
     constexpr int G = NVAL < 16 ? NVAL : 16;      // values per butterfly
@@ -552,7 +582,7 @@ __global__ __launch_bounds__(TPB) void k_cgs_mid(i64 n, int k, const Cx<R> *__re
             const int j = w + 4 * i;
             q[i] = (in && j < k) ? Q[(i64)j * ldq + r] : cx<R>(0, 0);
         }
-        if (w == 0) xin[lane] = in ? x[r] : cx<R>(0, 0);
+        if (w == 0) xin[lane] = !in ? cx<R>(0, 0) : xa ? ca * xa[r] + cb * xb[r] : x[r];
         Cx<R> a = cx<R>(0, 0);
 #pragma unroll
         for (int i = 0; i < KW; i++) cfma(a, q[i], sh[w + 4 * i]);
@@ -891,76 +921,112 @@ __global__ __launch_bounds__(TPB) void k_rotate(i64 n, int k, int keep, Cx<doubl
     }
 }
 
-// Gram block G = X^H Y of two tall-skinny complex matrices (X n x p, Y n x q):
-// a block owns a 64 x 64 tile of G and a chunk of rows; the chunk is staged
-// through LDS in slabs of GR_SR rows and each thread accumulates a 4 x 4
-// sub-tile (rows ty + 16a, columns tx + 16b).  Partials per chunk go to
-// part[chunk][i + j p]; k_gram_finish adds the chunks in a fixed order.
+// Gram block G = X^H Y of two tall-skinny complex matrices (X n x p, Y n x q)
+// on the FP64 matrix cores.  A block owns a 64 x 64 tile of G and a chunk of
+// rows; the chunk is staged through LDS in slabs of GR_SR rows (coalesced
+// column reads, zeros outside X and Y).  Wave w computes the tile's rows
+// 16w .. 16w+15 against its four 16-column groups with
+// v_mfma_f64_16x16x4f64: per 4-row step, Re += Xr^T Yr + Xi^T Yi and
+// Im += Xr^T Yi - Xi^T Yr (four MFMAs per 16 x 16 complex block).
+// Operand layout of the 16x16x4 f64 MFMA (checked on gfx950): A(m, k) in
+// lane m + 16 k, B(k, n) in lane n + 16 k, and D(m, n) in lane n + 16 (m % 4),
+// element m / 4.  Partials per chunk go to part[chunk][i + j p];
+// k_gram_finish adds the chunks in a fixed order.
 static constexpr int GR_T = 64, GR_SR = 32, GR_LD = GR_SR + 1;   // +1: LDS banks
+typedef double v4d __attribute__((ext_vector_type(4)));
 
 __global__ __launch_bounds__(TPB) void k_gram(i64 n, i64 chunk, int p, int q, const Cx<double> *__restrict__ X,
                                               i64 ldx, const Cx<double> *__restrict__ Y, i64 ldy,
                                               Cx<double> *__restrict__ part)
 {
     __shared__ Cx<double> xs[GR_T * GR_LD], ys[GR_T * GR_LD];
-    const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int tiles_q = (q + GR_T - 1) / GR_T;
     const int i0 = (blockIdx.x / tiles_q) * GR_T, j0 = (blockIdx.x % tiles_q) * GR_T;
     const i64 rb = (i64)blockIdx.y * chunk;
     const i64 re = rb + chunk < n ? rb + chunk : n;
-    double ar[4][4], ai[4][4];
+    const int ml = lane & 15, kq = lane >> 4;
+    v4d accr[4], acci[4];
 #pragma unroll
-    for (int a = 0; a < 4; a++)
+    for (int j = 0; j < 4; j++) {
+        accr[j] = v4d{0, 0, 0, 0};
+        acci[j] = v4d{0, 0, 0, 0};
+    }
+    // The slab's loads are issued into registers one slab ahead, branch-free
+    // (an element outside X or Y loads X[0] / Y[0] and is zeroed), so HBM
+    // latency overlaps the previous slab's MFMAs.
+    constexpr int E = GR_T * GR_SR / TPB;          // elements per thread and matrix
+    Cx<double> xv[E], yv[E];
+    auto load_slab = [&](i64 r0) {
 #pragma unroll
-        for (int b = 0; b < 4; b++) ar[a][b] = ai[a][b] = 0;
-    for (i64 r0 = rb; r0 < re; r0 += GR_SR) {
-        // stage GR_SR rows x 64 columns of X and of Y (zeros outside)
-#pragma unroll
-        for (int e = 0; e < GR_T * GR_SR / TPB; e++) {
+        for (int e = 0; e < E; e++) {
             const int idx = t + e * TPB, rr = idx % GR_SR, cc = idx / GR_SR;
             const i64 r = r0 + rr;
-            const bool rok = r < re;
-            xs[cc * GR_LD + rr] = (rok && i0 + cc < p) ? X[(i64)(i0 + cc) * ldx + r] : cx(0.0, 0.0);
-            ys[cc * GR_LD + rr] = (rok && j0 + cc < q) ? Y[(i64)(j0 + cc) * ldy + r] : cx(0.0, 0.0);
+            const bool okx = r < re && i0 + cc < p, oky = r < re && j0 + cc < q;
+            xv[e] = X[okx ? (i64)(i0 + cc) * ldx + r : 0];
+            yv[e] = Y[oky ? (i64)(j0 + cc) * ldy + r : 0];
+            if (!okx) xv[e] = cx(0.0, 0.0);
+            if (!oky) yv[e] = cx(0.0, 0.0);
+        }
+    };
+    if (rb < re) load_slab(rb);
+    for (i64 r0 = rb; r0 < re; r0 += GR_SR) {
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            const int idx = t + e * TPB, rr = idx % GR_SR, cc = idx / GR_SR;
+            xs[cc * GR_LD + rr] = xv[e];
+            ys[cc * GR_LD + rr] = yv[e];
         }
         __syncthreads();
-#pragma unroll 4
-        for (int rr = 0; rr < GR_SR; rr++) {
-            Cx<double> xa[4], yb[4];
+        if (r0 + GR_SR < re) load_slab(r0 + GR_SR);
+#pragma unroll 2
+        for (int kk = 0; kk < GR_SR; kk += 4) {
+            const Cx<double> a = xs[(16 * w + ml) * GR_LD + kk + kq];
 #pragma unroll
-            for (int a = 0; a < 4; a++) xa[a] = xs[(ty + 16 * a) * GR_LD + rr];
-#pragma unroll
-            for (int b = 0; b < 4; b++) yb[b] = ys[(tx + 16 * b) * GR_LD + rr];
-#pragma unroll
-            for (int a = 0; a < 4; a++)
-#pragma unroll
-                for (int b = 0; b < 4; b++) cfma_conj(ar[a][b], ai[a][b], xa[a], yb[b]);
+            for (int j = 0; j < 4; j++) {
+                const Cx<double> bv = ys[(16 * j + ml) * GR_LD + kk + kq];
+                accr[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.re, bv.re, accr[j], 0, 0, 0);
+                accr[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.im, bv.im, accr[j], 0, 0, 0);
+                acci[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.re, bv.im, acci[j], 0, 0, 0);
+                acci[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.im, bv.re, acci[j], 0, 0, 0);
+            }
         }
         __syncthreads();
     }
     Cx<double> *pc = part + (i64)blockIdx.y * p * q;
 #pragma unroll
-    for (int a = 0; a < 4; a++)
+    for (int j = 0; j < 4; j++)
 #pragma unroll
-        for (int b = 0; b < 4; b++) {
-            const int i = i0 + ty + 16 * a, j = j0 + tx + 16 * b;
-            if (i < p && j < q) pc[(i64)j * p + i] = cx(ar[a][b], ai[a][b]);
+        for (int e = 0; e < 4; e++) {
+            const int i = i0 + 16 * w + 4 * e + kq, jj = j0 + 16 * j + ml;
+            if (i < p && jj < q) pc[(i64)jj * p + i] = cx(accr[j][e], acci[j][e]);
         }
 }
 
+// G[i, j] = sum over chunks of the partials: 16 lanes per entry, lane l adds
+// chunks l, l + 16, ... in order, then a fixed xor tree over the 16 lanes.
 __global__ __launch_bounds__(TPB) void k_gram_finish(int nch, int p, int q, const Cx<double> *__restrict__ part,
                                                      Cx<double> *G, i64 ldg)
 {
-    const i64 e = (i64)blockIdx.x * TPB + threadIdx.x, pq = (i64)p * q;
-    if (e >= pq) return;
+    const i64 pq = (i64)p * q;
+    const i64 e = ((i64)blockIdx.x * TPB + threadIdx.x) >> 4;
+    const int l = threadIdx.x & 15;
     double sr = 0, si = 0;
-    for (int c = 0; c < nch; c++) {
-        const Cx<double> v = part[(i64)c * pq + e];
-        sr += v.re;
-        si += v.im;
+    if (e < pq)
+        for (int c = l; c < nch; c += 16) {
+            const Cx<double> v = part[(i64)c * pq + e];
+            sr += v.re;
+            si += v.im;
+        }
+#pragma unroll
+    for (int m = 8; m >= 1; m >>= 1) {
+        sr += __shfl_xor(sr, m, 64);
+        si += __shfl_xor(si, m, 64);
     }
-    const int i = (int)(e % p), j = (int)(e / p);
-    G[(i64)j * ldg + i] = cx(sr, si);
+    if (l == 0 && e < pq) {
+        const int i = (int)(e % p), j = (int)(e / p);
+        G[(i64)j * ldg + i] = cx(sr, si);
+    }
 }
 
 __global__ __launch_bounds__(TPB) void k_z2c(i64 n, const Cx<double> *z, Cx<float> *c)
@@ -1024,8 +1090,11 @@ static inline int chk(void) { return hipGetLastError() == hipSuccess ? 0 : -1; }
 
 extern "C" int64_t bbjd_red_part_entries(int64_t n, int maxk)
 {
-    // up to three vectors per projection (the test-space expansion)
-    return (int64_t)ntiles(n, BBJD_PROJ_MIN_ROWS) * 3 * (maxk + 1) + 64;
+    // up to three vectors per projection (the test-space expansion), and at
+    // least 128 row chunks of a maxk x maxk Gram matrix (the restart)
+    const int64_t proj = (int64_t)ntiles(n, BBJD_PROJ_MIN_ROWS) * 3 * (maxk + 1);
+    const int64_t gram = (int64_t)128 * maxk * maxk;
+    return (proj > gram ? proj : gram) + 64;
 }
 
 // Blocks up to which the finishing step runs in the producing kernel's last
@@ -1040,7 +1109,7 @@ static inline NormFin fin_none()
 {
     NormFin f;
     f.scal = nullptr; f.flag = nullptr; f.mode = -1; f.slot = 0; f.k = 0;
-    f.hsum = nullptr; f.h = nullptr; f.h2 = nullptr;
+    f.hsum = nullptr; f.h = nullptr; f.h2 = nullptr; f.x = nullptr; f.n = 0;
     return f;
 }
 static inline NormFin fin_norm(bbjd_red *rd, int mode, int slot)
@@ -1063,7 +1132,7 @@ static inline NormFin fin_norm(bbjd_red *rd, int mode, int slot)
 template <typename R, int NV>
 static int proj_pass_nv(hipStream_t s, i64 n, int k, const void *Q, i64 ldq, const void *x0, const void *x1,
                         const void *x2, void *out0, void *out1, void *out2, bbjd_red *rd, int norm_slot,
-                        const int *cond)
+                        const int *cond, Cx<R> ca = Cx<R>{0, 0}, Cx<R> cb = Cx<R>{0, 0})
 {
     constexpr int C = NV == 1 ? 16 : 8;
     const int ctiles = k > 0 ? (k + C - 1) / C : 1;
@@ -1086,19 +1155,19 @@ static int proj_pass_nv(hipStream_t s, i64 n, int k, const void *Q, i64 ldq, con
         }
         hipExtLaunchKernelGGL((k_proj<R, NV, C, false>), g, dim3(TPB), 0, s, ea, eb, 0, (i64)n, chunk, k,
                               (const Cx<R> *)Q, (i64)ldq, (const Cx<R> *)x0, (const Cx<R> *)x1, (const Cx<R> *)x2,
-                              (Cx<R> *)rd->part, ldp, (R *)rd->pnorm, want_norm, cond, rd->cnt, (Cx<R> *)out0,
+                              ca, cb, (Cx<R> *)rd->part, ldp, (R *)rd->pnorm, want_norm, cond, rd->cnt, (Cx<R> *)out0,
                               (Cx<R> *)out1, (Cx<R> *)out2, rd->scal, norm_slot, fused);
         const double bytes = (double)sizeof(Cx<R>) * (double)n * ((double)k + 1.0);
         std::lock_guard<std::mutex> lk(g_kt_mu);
         g_kt.push_back({ea, eb, bytes});
     } else if (cond)
         hipLaunchKernelGGL((k_proj<R, NV, C, true>), g, dim3(TPB), 0, s, (i64)n, chunk, k, (const Cx<R> *)Q,
-                           (i64)ldq, (const Cx<R> *)x0, (const Cx<R> *)x1, (const Cx<R> *)x2, (Cx<R> *)rd->part, ldp,
+                           (i64)ldq, (const Cx<R> *)x0, (const Cx<R> *)x1, (const Cx<R> *)x2, ca, cb, (Cx<R> *)rd->part, ldp,
                            (R *)rd->pnorm, want_norm, cond, rd->cnt, (Cx<R> *)out0, (Cx<R> *)out1, (Cx<R> *)out2,
                            rd->scal, norm_slot, fused);
     else
         hipLaunchKernelGGL((k_proj<R, NV, C, false>), g, dim3(TPB), 0, s, (i64)n, chunk, k, (const Cx<R> *)Q,
-                           (i64)ldq, (const Cx<R> *)x0, (const Cx<R> *)x1, (const Cx<R> *)x2, (Cx<R> *)rd->part, ldp,
+                           (i64)ldq, (const Cx<R> *)x0, (const Cx<R> *)x1, (const Cx<R> *)x2, ca, cb, (Cx<R> *)rd->part, ldp,
                            (R *)rd->pnorm, want_norm, cond, rd->cnt, (Cx<R> *)out0, (Cx<R> *)out1, (Cx<R> *)out2,
                            rd->scal, norm_slot, fused);
     if (!fused)
@@ -1171,6 +1240,9 @@ static int norm_into(hipStream_t s, i64 n, const void *x, bbjd_red *rd, int slot
     return chk();
 }
 
+// Blocks per Gram launch (row chunks x 64 x 64 tiles); tunable.
+static int g_gram_blocks = 512;
+
 // Fused CGS2 (k <= 128): on by default, switchable for tests and tuning
 // (2: on, with the second pass's normalisation always by the computed norm).
 static int g_cgs_fused = 1;
@@ -1193,7 +1265,9 @@ static int scale_inv(hipStream_t s, i64 n, void *x, const double *scal, int slot
 // Blocks walk enough 64-row tiles for about 1024 blocks, and more when the
 // partials (blocks x k) would not fit the scratch.
 template <typename R>
-static int cgs_mid(hipStream_t s, i64 n, int k, const void *Q, i64 ldq, void *v, bbjd_red *rd, MidFin f)
+static int cgs_mid(hipStream_t s, i64 n, int k, const void *Q, i64 ldq, void *v, bbjd_red *rd, MidFin f,
+                   const void *xa = nullptr, const void *xb = nullptr, Cx<R> ca = Cx<R>{0, 0},
+                   Cx<R> cb = Cx<R>{0, 0})
 {
     const i64 tiles64 = (n + 63) / 64;
     i64 per = (tiles64 + 1023) / 1024;
@@ -1208,8 +1282,8 @@ static int cgs_mid(hipStream_t s, i64 n, int k, const void *Q, i64 ldq, void *v,
     int *cnt = rd->cnt + BBJD_CNT_MID;
 #define BBJD_LAUNCH_MID(KW)                                                                                         \
     hipLaunchKernelGGL((k_cgs_mid<R, KW>), dim3((unsigned)nb), dim3(TPB), 0, s, (i64)n, k, (const Cx<R> *)Q,         \
-                       (i64)ldq, (const Cx<R> *)f.h1, (Cx<R> *)v, (int)per, (Cx<R> *)rd->part, (R *)rd->pnorm, cnt,  \
-                       f, fused)
+                       (i64)ldq, (const Cx<R> *)f.h1, (Cx<R> *)v, (const Cx<R> *)xa, (const Cx<R> *)xb, ca, cb,      \
+                       (int)per, (Cx<R> *)rd->part, (R *)rd->pnorm, cnt, f, fused)
     switch (cgs_kw(k)) {
     case 2: BBJD_LAUNCH_MID(2); break;
     case 4: BBJD_LAUNCH_MID(4); break;
@@ -1244,6 +1318,16 @@ static int cgs_final(hipStream_t s, i64 n, int k, const void *Q, i64 ldq, void *
     return chk();
 }
 
+// norm finish of the fused CGS2's last pass; with unit output it also does
+// the rare rescale by the computed norm
+static inline NormFin fin_unit(bbjd_red *rd, int slot, void *x, i64 n, int scale)
+{
+    NormFin f = fin_norm(rd, scale ? 3 : 0, slot);
+    f.x = x;
+    f.n = n;
+    return f;
+}
+
 static inline bool cgs_fusable(int k, const bbjd_red *rd)
 {
     return g_cgs_fused && cgs_kw(k) > 0 && (i64)k <= rd->part_cap;
@@ -1271,8 +1355,7 @@ static int cgs2(hipStream_t s, i64 n, int k, const void *Q, i64 ldq, void *v, bb
         f.scal = rd->scal; f.flag = rd->flag; f.out_slot = out_slot; f.decide = 1;
         f.h1 = rd->h; f.h2 = rd->h2; f.hsum = hsum;
         rc |= cgs_mid<R>(s, n, k, Q, ldq, v, rd, f);
-        rc |= cgs_final<R>(s, n, k, Q, ldq, v, rd, fin_norm(rd, 0, out_slot), scale, 0);
-        if (scale) rc |= scale_inv<R>(s, n, v, rd->scal, out_slot, rd->flag + FL_RESCALE);
+        rc |= cgs_final<R>(s, n, k, Q, ldq, v, rd, fin_unit(rd, out_slot, v, n, scale), scale, 0);
         return rc;
     }
     rc |= proj_pass<R>(s, n, k, Q, ldq, v, nullptr, rd->h, nullptr, rd, SL_NB, nullptr);
@@ -1288,27 +1371,28 @@ static int cgs2(hipStream_t s, i64 n, int k, const void *Q, i64 ldq, void *v, bb
     return rc;
 }
 
-// The test-space expansion (k <= 128): w = unit CGS2 of w against W, and in
-// the same first pass over W the pencil column W^H a, W^H b (a = A v,
-// b = B v), then the new diagonal entries w^H a, w^H b after the
-// normalisation: colA[0..k], colB[0..k].  Returns 1 without doing anything
-// when the shape takes the unfused path (the caller projects separately).
+// The test-space expansion (k <= 128): w = unit CGS2 of ca a + cb b against
+// W (a = A v, b = B v; the combination is formed in the first two passes,
+// never stored before the result), and in the same first pass over W the
+// pencil column W^H a, W^H b, then the new diagonal entries w^H a, w^H b
+// after the normalisation: colA[0..k], colB[0..k].  Returns 1 without doing
+// anything when the shape takes the unfused path (the caller forms w and
+// projects separately).
 template <typename R>
 static int cgs2_unit_pencil(hipStream_t s, i64 n, int k, const void *W, i64 ldw, void *w, const void *a,
-                            const void *b, void *colA, void *colB, bbjd_red *rd, int out_slot)
+                            const void *b, Cx<R> ca, Cx<R> cb, void *colA, void *colB, bbjd_red *rd, int out_slot)
 {
     if (k == 0 || !cgs_fusable(k, rd)) return 1;
     const int ctiles = (k + 7) / 8;
     const i64 chunk = proj_chunk(n, ctiles);
     if ((n + chunk - 1) / chunk * 3 * (i64)k > rd->part_cap) return 1;
     int rc = 0;
-    rc |= proj_pass_nv<R, 3>(s, n, k, W, ldw, w, a, b, rd->h, colA, colB, rd, SL_NB, nullptr);
+    rc |= proj_pass_nv<R, 3>(s, n, k, W, ldw, nullptr, a, b, rd->h, colA, colB, rd, SL_NB, nullptr, ca, cb);
     MidFin f;
     f.scal = rd->scal; f.flag = rd->flag; f.out_slot = out_slot; f.decide = 1;
     f.h1 = rd->h; f.h2 = rd->h2; f.hsum = nullptr;
-    rc |= cgs_mid<R>(s, n, k, W, ldw, w, rd, f);
-    rc |= cgs_final<R>(s, n, k, W, ldw, w, rd, fin_norm(rd, 0, out_slot), 1, 0);
-    rc |= scale_inv<R>(s, n, w, rd->scal, out_slot, rd->flag + FL_RESCALE);
+    rc |= cgs_mid<R>(s, n, k, W, ldw, w, rd, f, a, b, ca, cb);
+    rc |= cgs_final<R>(s, n, k, W, ldw, w, rd, fin_unit(rd, out_slot, w, n, 1), 1, 0);
     rc |= proj_pass_nv<R, 2>(s, n, 1, w, n, a, b, nullptr, (Cx<R> *)colA + k, (Cx<R> *)colB + k, nullptr, rd, -1,
                              nullptr);
     return rc ? -1 : 0;
@@ -1418,8 +1502,12 @@ int bbjd_k_cgs2_unit(hipStream_t s, int64_t n, int k, const void *Q, int64_t ldq
 { return cgs2<double>(s, n, k, Q, ldq, v, rd, out_slot, nullptr, 1); }
 
 int bbjd_k_cgs2_unit_pencil(hipStream_t s, int64_t n, int k, const void *W, int64_t ldw, void *w, const void *a,
-                            const void *b, void *colA, void *colB, bbjd_red *rd, int out_slot)
-{ return cgs2_unit_pencil<double>(s, n, k, W, ldw, w, a, b, colA, colB, rd, out_slot); }
+                            const void *b, double are, double aim, double bre, double bim, void *colA, void *colB,
+                            bbjd_red *rd, int out_slot)
+{
+    return cgs2_unit_pencil<double>(s, n, k, W, ldw, w, a, b, Cx<double>{are, aim}, Cx<double>{bre, bim}, colA,
+                                    colB, rd, out_slot);
+}
 
 void bbjd_set_cgs_fused(int mode) { g_cgs_fused = mode == 2 ? 2 : mode != 0; }
 
@@ -1508,6 +1596,8 @@ int bbjd_k_svdv(hipStream_t s, int64_t n, const void *u, void *z, const void *sv
     return rc | chk();
 }
 
+void bbjd_set_gram_blocks(int blocks) { if (blocks > 0) g_gram_blocks = blocks; }
+
 // G (p x q, column-major, ld ldg) = X^H Y; partials in rd->part.
 int bbjd_k_gram(hipStream_t s, int64_t n, int p, const void *X, int64_t ldx, int q, const void *Y, int64_t ldy,
                 void *G, int64_t ldg, bbjd_red *rd)
@@ -1516,17 +1606,18 @@ int bbjd_k_gram(hipStream_t s, int64_t n, int p, const void *X, int64_t ldx, int
     if (n <= 0 || ldg < p) return -2;
     const i64 pq = (i64)p * q;
     const int tiles = ((p + GR_T - 1) / GR_T) * ((q + GR_T - 1) / GR_T);
-    i64 nch = (512 + tiles - 1) / tiles;
+    i64 nch = (g_gram_blocks + tiles - 1) / tiles;
     if (nch * pq > rd->part_cap) nch = rd->part_cap / pq;
     if (nch < 1) return -2;
-    const i64 maxch = (n + GR_SR - 1) / GR_SR;
+    // at least 256 rows per chunk: the partials stay small beside the inputs
+    const i64 maxch = (n + 255) / 256;
     if (nch > maxch) nch = maxch;
     i64 chunk = (n + nch - 1) / nch;
     chunk = (chunk + GR_SR - 1) / GR_SR * GR_SR;
     nch = (n + chunk - 1) / chunk;
     hipLaunchKernelGGL(k_gram, dim3(tiles, (unsigned)nch), dim3(TPB), 0, s, (i64)n, chunk, p, q,
                        (const Cx<double> *)X, (i64)ldx, (const Cx<double> *)Y, (i64)ldy, (Cx<double> *)rd->part);
-    hipLaunchKernelGGL(k_gram_finish, dim3((unsigned)((pq + TPB - 1) / TPB)), dim3(TPB), 0, s, (int)nch, p, q,
+    hipLaunchKernelGGL(k_gram_finish, dim3((unsigned)((pq * 16 + TPB - 1) / TPB)), dim3(TPB), 0, s, (int)nch, p, q,
                        (const Cx<double> *)rd->part, (Cx<double> *)G, (i64)ldg);
     return chk();
 }
@@ -1570,7 +1661,8 @@ int bbjd_k_c2z(hipStream_t s, int64_t n, const void *c, void *z)
 // stream.  which: 0 k_proj one vector, 1 k_proj two vectors, 2 k_update,
 // 3 k_rotate (k -> k/2 columns), 4 full CGS2 (one pass + skipped pass 2),
 // 5 a plain copy of n complex values, 6 the two-pass projection (both
-// passes always run: the fused form's three reads of Q for k <= 128).
+// passes always run: the fused form's three reads of Q for k <= 128), 7 the
+// Gram matrix Q^H Q (k x k, FP64 matrix cores; 8 n k^2 flops).
 // Returns the average ms per launch and the algorithmic bytes per launch.
 __global__ void k_fill(i64 cnt, Cx<double> *a, double scale)
 {
@@ -1645,6 +1737,12 @@ int bbjd_kbench(int which, int64_t n, int k, int reps, double *avg_ms, double *b
             // fused (k <= 128): three reads of Q, x read three times and
             // written twice; otherwise four passes of k + 2
             by = cgs_fusable(k, &rd) ? 16.0 * n * (3.0 * k + 5) : 16.0 * n * (4.0 * k + 6);
+            break;
+        case 7:
+            // G = Q^H Q on the matrix cores; flops 8 n k^2 (reported by the
+            // caller from the shape), bytes: Q staged twice
+            (void)bbjd_k_gram(s, n, k, Q, ld, k, Q, ld, Zr, k, &rd);
+            by = 16.0 * n * 2.0 * k;
             break;
         case 3:
             bbjd_k_rotate(s, n, k, k / 2 > 0 ? k / 2 : 1, Q, ld, Zr);

@@ -7,7 +7,8 @@ basis rotation k -> k/2), cgs2 (one CGS2 with the second pass skipped on the
 device), copy (plain copy of N complex values, the reference for
 counter-measured HBM traffic), projout (the two-pass projection v -= Q Q^H v,
 both passes always: for k <= 128 the fused form, first update and second
-projection in one pass over Q).  Prints one line: kernel, shape, average us per
+projection in one pass over Q), gram (G = Q^H Q, k x k, on the FP64 matrix
+cores: also prints TFLOP/s for its 8 N K^2 flops).  Prints one line: kernel, shape, average us per
 launch, algorithmic bytes per launch and GB/s.
 """
 from __future__ import annotations
@@ -17,7 +18,7 @@ import ctypes as C
 
 from . import _lib
 
-KERNELS = {"proj": 0, "proj2": 1, "update": 2, "rotate": 3, "cgs2": 4, "copy": 5, "projout": 6}
+KERNELS = {"proj": 0, "proj2": 1, "update": 2, "rotate": 3, "cgs2": 4, "copy": 5, "projout": 6, "gram": 7}
 
 
 def run(kernel, n, k, reps=50, proj_blocks=0, upd_rows=0, blocks_target=0):
@@ -41,8 +42,11 @@ def main():
     ap.add_argument("--upd-rows", type=int, default=0, help="rows per block, 0 = chosen from the shape")
     a = ap.parse_args()
     ms, by = run(a.kernel, a.n, a.k, a.reps, a.proj_blocks, a.upd_rows)
+    extra = ""
+    if a.kernel == "gram":
+        extra = f", {8.0 * a.n * a.k * a.k / (ms * 1e-3) / 1e12:.2f} TFLOP/s"
     print(f"{a.kernel} n={a.n} k={a.k}: {ms * 1e3:.2f} us/launch, {by / 1e6:.3f} MB/launch, "
-          f"{by / (ms * 1e-3) / 1e9:.1f} GB/s", flush=True)
+          f"{by / (ms * 1e-3) / 1e9:.1f} GB/s{extra}", flush=True)
 
 
 if __name__ == "__main__":

@@ -146,13 +146,18 @@ int bbjd_block_gram(bbjd_int n, bbjd_int p, const bbjd_zomplex *X, bbjd_int q, c
  * src/jd_core.c:269-309); returns 0 or -1. */
 double bbjd_block_orthonormalize(bbjd_int n, bbjd_int k, const bbjd_zomplex *Q, bbjd_zomplex *v, int *flags);
 /* bbjd_expand_test_space: the expansion's test-space step (reference
- * src/jd_core.c:1060-1104) for w = nu0 A v + mu0 B v given as w, with
- * a = A v, b = B v: w <- normalised CGS2(w) against W (n x k, k >= 1), and
- * the new pencil column colA = [W w]^H a, colB = [W w]^H b (k + 1 entries
- * each).  Returns ||w|| before the scaling, NAN on a device error. */
-double bbjd_expand_test_space(bbjd_int n, bbjd_int k, const bbjd_zomplex *W, bbjd_zomplex *w, const bbjd_zomplex *a,
-                              const bbjd_zomplex *b, bbjd_zomplex *colA, bbjd_zomplex *colB);
+ * src/jd_core.c:1060-1104) with a = A v, b = B v: w = normalised CGS2 of
+ * ca a + cb b against W (n x k, k >= 1), and the new pencil column
+ * colA = [W w]^H a, colB = [W w]^H b (k + 1 entries each).  Returns ||w||
+ * before the scaling, NAN on a device error. */
+double bbjd_expand_test_space(bbjd_int n, bbjd_int k, const bbjd_zomplex *W, bbjd_zomplex ca, bbjd_zomplex cb,
+                              const bbjd_zomplex *a, const bbjd_zomplex *b, bbjd_zomplex *w, bbjd_zomplex *colA,
+                              bbjd_zomplex *colB);
 int    bbjd_project_out_host(bbjd_int n, bbjd_int k, const bbjd_zomplex *Q, bbjd_zomplex *v);
+
+/* Blocks per Gram launch (the restart's test-space QR and projected pencil):
+ * row chunks x 64 x 64 output tiles, default 512.  For tuning. */
+void bbjd_set_gram_blocks(int blocks);
 
 /* 1 (default): CGS2 and the two-pass projection take the three-read form
  * (first update and second projection in one pass over Q) for k <= 128;

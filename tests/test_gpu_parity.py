@@ -133,10 +133,11 @@ def test_fused_cgs2_matches_oracle(n):
 
 @pytest.mark.parametrize("n,k", [(1003, 1), (1003, 9), (70005, 33), (70005, 100), (20001, 128), (5000, 130)])
 def test_test_space_expansion_pencil_column(n, k):
-    """the expansion's test-space step: the pencil column [W w]^H [Av Bv]
-    taken in the first pass over W (k <= 128) or by separate projections
-    (k = 130), against numpy; w against the oracle's CGS2, normalised; both
-    DGKS branches"""
+    """the expansion's test-space step: w = nu0 Av + mu0 Bv formed in the
+    first passes over W and the pencil column [W w]^H [Av Bv] taken in the
+    first pass (k <= 128), or the combination, CGS2 and projections one after
+    another (k = 130); the column against numpy, w against the oracle's CGS2
+    of the same combination, normalised; both DGKS branches"""
     lib = _lib.load()
     olib = orc.load_oracle()
     rng = np.random.default_rng(3 * n + k)
@@ -153,11 +154,11 @@ def test_test_space_expansion_pencil_column(n, k):
         w0 = nu0 * a + mu0 * b
         wo, ho = w0.copy(), np.zeros(k, np.complex128)
         no = olib.orc_block_orthogonalize(n, k, W.ctypes.data, wo.ctypes.data, ho.ctypes.data)
-        wg = w0.copy()
+        wg = np.zeros(n, np.complex128)
         cA = np.zeros(k + 1, np.complex128)
         cB = np.zeros(k + 1, np.complex128)
-        ng = lib.bbjd_expand_test_space(n, k, W.ctypes.data, wg.ctypes.data, a.ctypes.data, b.ctypes.data,
-                                        cA.ctypes.data, cB.ctypes.data)
+        ng = lib.bbjd_expand_test_space(n, k, W.ctypes.data, _lib.zc(nu0), _lib.zc(mu0), a.ctypes.data,
+                                        b.ctypes.data, wg.ctypes.data, cA.ctypes.data, cB.ctypes.data)
         scale = np.linalg.norm(w0)
         assert abs(ng - no) <= 1e-12 * scale, case
         assert abs(np.linalg.norm(wg) - 1.0) <= 1e-13, case
