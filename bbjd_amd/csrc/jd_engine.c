@@ -1269,6 +1269,33 @@ double bbjd_expand_test_space(bbjd_int n, bbjd_int k, const bbjd_zomplex *W, bbj
     return res;
 }
 
+int bbjd_block_rotate(bbjd_int n, bbjd_int k, bbjd_int keep, bbjd_zomplex *X, const bbjd_zomplex *Z)
+{
+    if (n <= 0 || k <= 0 || keep <= 0 || keep > k) return -1;
+    helper_t h;
+    void *dX = NULL;
+    const size_t xb = (size_t)n * k * 16, zb = (size_t)k * keep * 16;
+    int rc = -1;
+    zcx *zr = malloc(zb);
+    if (!zr) return -1;
+    /* row-major Z for the rotation: zr[j keep + c] = Z[j, c] */
+    for (bbjd_int j = 0; j < k; j++)
+        for (bbjd_int c = 0; c < keep; c++) zr[j * keep + c] = ((const zcx *)Z)[j + c * k];
+    if (helper_open(&h, n, k, (xb + 255) / 256 * 256 + zb, &dX) == 0) {
+        void *dZ = (char *)dX + (xb + 255) / 256 * 256;
+        int e = 0;
+        e |= !HIPOK(hipMemcpyAsync(dX, X, xb, hipMemcpyHostToDevice, h.s));
+        e |= !HIPOK(hipMemcpyAsync(dZ, zr, zb, hipMemcpyHostToDevice, h.s));
+        e |= bbjd_k_rotate(h.s, n, (int)k, (int)keep, dX, n, dZ);
+        e |= !HIPOK(hipMemcpyAsync(X, dX, xb, hipMemcpyDeviceToHost, h.s));
+        e |= !HIPOK(hipStreamSynchronize(h.s));
+        rc = e ? -1 : 0;
+    }
+    helper_close(&h);
+    free(zr);
+    return rc;
+}
+
 int bbjd_block_gram(bbjd_int n, bbjd_int p, const bbjd_zomplex *X, bbjd_int q, const bbjd_zomplex *Y,
                     bbjd_zomplex *G)
 {

@@ -65,6 +65,7 @@ template <typename R> __device__ __forceinline__ void cfma(Cx<R> &a, Cx<R> q, Cx
 { a.re += q.re * h.re - q.im * h.im; a.im += q.re * h.im + q.im * h.re; }
 
 static constexpr int TPB = BBJD_TPB;
+typedef double v4d __attribute__((ext_vector_type(4)));   // f64 MFMA accumulator
 
 // Reduce 2^LS values per lane over the 64-lane wavefront.  Afterwards lane l
 // holds in v[0] the wave sum of value index (l >> (6 - LS)).
@@ -921,6 +922,80 @@ __global__ __launch_bounds__(TPB) void k_rotate(i64 n, int k, int keep, Cx<doubl
     }
 }
 
+// The same rotation on the FP64 matrix cores: a block stages RT rows x k
+// columns of X in LDS (every input of its rows is read before any output is
+// written), and the RT x keep output = (RT x k) Z is cut into 16 x 16 complex
+// blocks, wave w taking blocks w, w + 4, ...; per 4-column step of the inner
+// dimension four v_mfma_f64_16x16x4f64 (Re += Xr Zr - Xi Zi, Im += Xr Zi +
+// Xi Zr; operand layout as in k_gram).  Z (Zr row-major, k x keep) is read
+// from global memory (it is small and shared by every block).  The output goes
+// back through the same LDS for column-contiguous stores.  RT * ceil4(k) * 16
+// bytes of LDS; at most 16 output blocks (2 RT / 16 * ceil(keep / 16) <= 16).
+template <int RT>
+__global__ __launch_bounds__(TPB) void k_rotate_mfma(i64 n, int k, int keep, Cx<double> *X, i64 ldx,
+                                                     const Cx<double> *__restrict__ Zr)
+{
+    extern __shared__ char smem_raw[];
+    Cx<double> *tl = reinterpret_cast<Cx<double> *>(smem_raw);     // column j, row r at j RT + r
+    constexpr int MT = RT / 16;                                     // row blocks
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int ml = lane & 15, kq = lane >> 4;
+    const int k4 = (k + 3) & ~3;
+    const i64 rb = (i64)blockIdx.x * RT;
+    for (int idx = t; idx < RT * k4; idx += TPB) {
+        const int j = idx / RT, rr = idx % RT;
+        const i64 r = rb + rr;
+        const bool ok = j < k && r < n;
+        Cx<double> v = X[ok ? (i64)j * ldx + r : 0];
+        if (!ok) v = cx(0.0, 0.0);
+        tl[idx] = v;
+    }
+    __syncthreads();
+    const int nblk = MT * ((keep + 15) / 16);
+    v4d acr[4], aci[4];
+#pragma unroll
+    for (int s4 = 0; s4 < 4; s4++) {
+        acr[s4] = v4d{0, 0, 0, 0};
+        aci[s4] = v4d{0, 0, 0, 0};
+    }
+    for (int kk = 0; kk < k4; kk += 4) {
+        const int kr = kk + kq;
+#pragma unroll
+        for (int s4 = 0; s4 < 4; s4++) {
+            const int b = w + 4 * s4;
+            if (b < nblk) {                                         // uniform over the wave
+                const int mt = b % MT, nt = b / MT;
+                const Cx<double> a = tl[kr * RT + mt * 16 + ml];
+                const int col = nt * 16 + ml;
+                const bool ok = col < keep && kr < k;
+                Cx<double> z = Zr[ok ? (i64)kr * keep + col : 0];
+                if (!ok) z = cx(0.0, 0.0);
+                acr[s4] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.re, z.re, acr[s4], 0, 0, 0);
+                acr[s4] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.im, z.im, acr[s4], 0, 0, 0);
+                aci[s4] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.re, z.im, aci[s4], 0, 0, 0);
+                aci[s4] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.im, z.re, aci[s4], 0, 0, 0);
+            }
+        }
+    }
+    __syncthreads();                                                // every read of the inputs done
+#pragma unroll
+    for (int s4 = 0; s4 < 4; s4++) {
+        const int b = w + 4 * s4;
+        if (b < nblk) {
+            const int mt = b % MT, nt = b / MT, col = nt * 16 + ml;
+#pragma unroll
+            for (int e = 0; e < 4; e++)
+                if (col < keep) tl[col * RT + mt * 16 + 4 * e + kq] = cx(acr[s4][e], aci[s4][e]);
+        }
+    }
+    __syncthreads();
+    for (int idx = t; idx < RT * keep; idx += TPB) {
+        const int c = idx / RT, rr = idx % RT;
+        const i64 r = rb + rr;
+        if (r < n) X[(i64)c * ldx + r] = tl[idx];
+    }
+}
+
 // Gram block G = X^H Y of two tall-skinny complex matrices (X n x p, Y n x q)
 // on the FP64 matrix cores.  A block owns a 64 x 64 tile of G and a chunk of
 // rows; the chunk is staged through LDS in slabs of GR_SR rows (coalesced
@@ -933,7 +1008,6 @@ __global__ __launch_bounds__(TPB) void k_rotate(i64 n, int k, int keep, Cx<doubl
 // element m / 4.  Partials per chunk go to part[chunk][i + j p];
 // k_gram_finish adds the chunks in a fixed order.
 static constexpr int GR_T = 64, GR_SR = 32, GR_LD = GR_SR + 1;   // +1: LDS banks
-typedef double v4d __attribute__((ext_vector_type(4)));
 
 __global__ __launch_bounds__(TPB) void k_gram(i64 n, i64 chunk, int p, int q, const Cx<double> *__restrict__ X,
                                               i64 ldx, const Cx<double> *__restrict__ Y, i64 ldy,
@@ -1242,6 +1316,8 @@ static int norm_into(hipStream_t s, i64 n, const void *x, bbjd_red *rd, int slot
 
 // Blocks per Gram launch (row chunks x 64 x 64 tiles); tunable.
 static int g_gram_blocks = 512;
+// In-place rotation on the matrix cores (1) or on vector FP64 (0).
+static int g_rotate_mfma = 1;
 
 // Fused CGS2 (k <= 128): on by default, switchable for tests and tuning
 // (2: on, with the second pass's normalisation always by the computed norm).
@@ -1598,6 +1674,8 @@ int bbjd_k_svdv(hipStream_t s, int64_t n, const void *u, void *z, const void *sv
 
 void bbjd_set_gram_blocks(int blocks) { if (blocks > 0) g_gram_blocks = blocks; }
 
+void bbjd_set_rotate_mfma(int on) { g_rotate_mfma = on != 0; }
+
 // G (p x q, column-major, ld ldg) = X^H Y; partials in rd->part.
 int bbjd_k_gram(hipStream_t s, int64_t n, int p, const void *X, int64_t ldx, int q, const void *Y, int64_t ldy,
                 void *G, int64_t ldg, bbjd_red *rd)
@@ -1625,6 +1703,19 @@ int bbjd_k_gram(hipStream_t s, int64_t n, int p, const void *X, int64_t ldx, int
 int bbjd_k_rotate(hipStream_t s, int64_t n, int k, int keep, void *X, int64_t ldx, const void *Zr)
 {
     if (keep > k || keep <= 0) return -2;
+    const int k4 = (k + 3) & ~3;
+    // matrix cores: 32 rows per block while the staged rows fit 64 KB of LDS
+    // and the output has at most 16 blocks of 16 x 16, else 16 rows
+    if (g_rotate_mfma && (size_t)32 * k4 * 16 <= 64 * 1024 && 2 * ((keep + 15) / 16) <= 16) {
+        hipLaunchKernelGGL((k_rotate_mfma<32>), dim3((unsigned)((n + 31) / 32)), dim3(TPB), (size_t)32 * k4 * 16, s,
+                           (i64)n, k, keep, (Cx<double> *)X, (i64)ldx, (const Cx<double> *)Zr);
+        return chk();
+    }
+    if (g_rotate_mfma && (size_t)16 * k4 * 16 <= 64 * 1024 && (keep + 15) / 16 <= 16) {
+        hipLaunchKernelGGL((k_rotate_mfma<16>), dim3((unsigned)((n + 15) / 16)), dim3(TPB), (size_t)16 * k4 * 16, s,
+                           (i64)n, k, keep, (Cx<double> *)X, (i64)ldx, (const Cx<double> *)Zr);
+        return chk();
+    }
     int rt = 16;
     while (rt > 4 && (size_t)rt * (k + keep) * 16 > 64 * 1024) rt /= 2;
     if ((size_t)rt * (k + keep) * 16 > 64 * 1024) return -2;

@@ -159,6 +159,15 @@ int    bbjd_project_out_host(bbjd_int n, bbjd_int k, const bbjd_zomplex *Q, bbjd
  * row chunks x 64 x 64 output tiles, default 512.  For tuning. */
 void bbjd_set_gram_blocks(int blocks);
 
+/* 1 (default): the restart's in-place basis rotation runs on the FP64 matrix
+ * cores when its rows fit the LDS tile; 0: on vector FP64.  For tests. */
+void bbjd_set_rotate_mfma(int on);
+
+/* Test entry: X[:, :keep] = X[:, :k] Z in place for host arrays X (n x k,
+ * leading dimension n) and Z (k x keep, column-major) through the engine's
+ * restart rotation (reference src/jd_core.c:983-1001); returns 0 or -1. */
+int bbjd_block_rotate(bbjd_int n, bbjd_int k, bbjd_int keep, bbjd_zomplex *X, const bbjd_zomplex *Z);
+
 /* 1 (default): CGS2 and the two-pass projection take the three-read form
  * (first update and second projection in one pass over Q) for k <= 128;
  * 0: the four-launch form everywhere; 2: the three-read form with the

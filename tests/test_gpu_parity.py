@@ -370,6 +370,29 @@ def test_block_gram_matches_numpy(n, p, q):
     assert np.array_equal(G, G2)
 
 
+@pytest.mark.parametrize("n,k,keep", [(5000, 64, 32), (70001, 128, 64), (3001, 200, 80), (1000, 9, 3),
+                                       (50, 300, 250)])
+def test_block_rotate_matches_numpy(n, k, keep):
+    """the restart's in-place rotation X[:, :keep] = X Z: matrix cores with 32
+    or 16 rows per block (k = 200 exceeds the 32-row tile) and the vector
+    kernel (k = 300, and with the matrix cores switched off), against numpy"""
+    lib = _lib.load()
+    rng = np.random.default_rng(n + k + keep)
+    X = np.asfortranarray(rng.standard_normal((n, k)) + 1j * rng.standard_normal((n, k)))
+    Z = np.asfortranarray(rng.standard_normal((k, keep)) + 1j * rng.standard_normal((k, keep)))
+    ref = X @ Z
+    try:
+        for mfma in (1, 0):
+            lib.bbjd_set_rotate_mfma(mfma)
+            Xg = X.copy(order="F")
+            assert lib.bbjd_block_rotate(n, k, keep, Xg.ctypes.data, Z.ctypes.data) == 0
+            err = np.linalg.norm(Xg[:, :keep] - ref) / np.linalg.norm(ref)
+            assert err <= 1e-14, (mfma, err)
+            assert np.array_equal(Xg[:, keep:], X[:, keep:])        # columns past keep untouched
+    finally:
+        lib.bbjd_set_rotate_mfma(1)
+
+
 def test_qz_modes_same_eigenpairs():
     """the eigenvalue-only projected step with inverse-iteration Ritz vectors
     ('ritz') and the ordered Schur form every iteration ('schur', the
