@@ -1132,6 +1132,24 @@ static int g_upd_rows = 0;                      // 0: choose from the shape
 static int g_blocks_target = 2048;
 static int g_proj_blocks = 192;
 
+// Per-shape tile table (filled by the autotuner, bbjd_amd/autotune.py):
+// class (i, j) holds the shapes with n <= n_upper[i] and k <= k_upper[j]
+// (the last class of each takes everything larger); entries are the
+// projection's blocks per launch and the update's rows per block (0: the
+// shape rule).  Empty: the global defaults above.  Set before solves run.
+static constexpr int TT_MAX = 8;
+static int g_tt_nn = 0, g_tt_nk = 0;
+static i64 g_tt_n[TT_MAX];
+static int g_tt_k[TT_MAX], g_tt_proj[TT_MAX * TT_MAX], g_tt_upd[TT_MAX * TT_MAX];
+
+static inline int tt_class(i64 n, int k)
+{
+    int i = 0, j = 0;
+    while (i < g_tt_nn - 1 && n > g_tt_n[i]) i++;
+    while (j < g_tt_nk - 1 && k > g_tt_k[j]) j++;
+    return i * TT_MAX + j;
+}
+
 static inline int ntiles(i64 n, int rows) { return (int)((n + rows - 1) / rows); }
 static inline int pick_rows(i64 n, int col_tiles)
 {
@@ -1142,12 +1160,17 @@ static inline int pick_rows(i64 n, int col_tiles)
 static inline int upd_rows(i64 n, int k)
 {
     if (g_upd_rows) return g_upd_rows;
+    if (g_tt_nn > 0) {
+        const int r = g_tt_upd[tt_class(n, k)];
+        if (r) return r;
+    }
     return k >= 16 ? 128 : pick_rows(n, 1);     // measured: column split wins from k = 16
 }
 static inline int vec_rows(i64 n) { return g_upd_rows >= TPB ? g_upd_rows : pick_rows(n, 1); }
-static inline i64 proj_chunk(i64 n, int col_tiles)
+static inline i64 proj_chunk(i64 n, int col_tiles, int k)
 {
-    i64 chunks = (g_proj_blocks + col_tiles - 1) / col_tiles;
+    const int blocks = g_tt_nn > 0 ? g_tt_proj[tt_class(n, k)] : g_proj_blocks;
+    i64 chunks = (blocks + col_tiles - 1) / col_tiles;
     if (chunks < 1) chunks = 1;
     i64 chunk = (n + chunks - 1) / chunks;
     chunk = (chunk + BBJD_PROJ_MIN_ROWS - 1) / BBJD_PROJ_MIN_ROWS * BBJD_PROJ_MIN_ROWS;
@@ -1210,7 +1233,7 @@ static int proj_pass_nv(hipStream_t s, i64 n, int k, const void *Q, i64 ldq, con
 {
     constexpr int C = NV == 1 ? 16 : 8;
     const int ctiles = k > 0 ? (k + C - 1) / C : 1;
-    const i64 chunk = proj_chunk(n, ctiles);
+    const i64 chunk = proj_chunk(n, ctiles, k);
     const int nt = (int)((n + chunk - 1) / chunk);
     const int ldp = k > 0 ? k : 1;
     const int want_norm = norm_slot >= 0;
@@ -1460,7 +1483,7 @@ static int cgs2_unit_pencil(hipStream_t s, i64 n, int k, const void *W, i64 ldw,
 {
     if (k == 0 || !cgs_fusable(k, rd)) return 1;
     const int ctiles = (k + 7) / 8;
-    const i64 chunk = proj_chunk(n, ctiles);
+    const i64 chunk = proj_chunk(n, ctiles, k);
     if ((n + chunk - 1) / chunk * 3 * (i64)k > rd->part_cap) return 1;
     int rc = 0;
     rc |= proj_pass_nv<R, 3>(s, n, k, W, ldw, nullptr, a, b, rd->h, colA, colB, rd, SL_NB, nullptr, ca, cb);
@@ -1526,6 +1549,29 @@ static int lincomb(hipStream_t s, i64 n, double are, double aim, const void *x, 
 extern "C" {
 
 void bbjd_set_fuse_blocks(int max_blocks) { g_fuse_max_blocks = max_blocks; }
+
+int bbjd_set_tile_table(int nn, const int64_t *n_upper, int nk, const int *k_upper, const int *proj_blocks,
+                        const int *upd_rows)
+{
+    if (nn == 0 || nk == 0) { g_tt_nn = g_tt_nk = 0; return 0; }
+    if (nn < 0 || nk < 0 || nn > TT_MAX || nk > TT_MAX || !n_upper || !k_upper || !proj_blocks || !upd_rows)
+        return -2;
+    for (int i = 0; i < nn * nk; i++) {
+        const int r = upd_rows[i];
+        if (proj_blocks[i] <= 0 || !(r == 0 || r == 64 || r == 128 || r == TPB || r == 2 * TPB || r == 4 * TPB))
+            return -2;
+    }
+    for (int i = 0; i < nn; i++) g_tt_n[i] = n_upper[i];
+    for (int j = 0; j < nk; j++) g_tt_k[j] = k_upper[j];
+    for (int i = 0; i < nn; i++)
+        for (int j = 0; j < nk; j++) {
+            g_tt_proj[i * TT_MAX + j] = proj_blocks[i * nk + j];
+            g_tt_upd[i * TT_MAX + j] = upd_rows[i * nk + j];
+        }
+    g_tt_nn = nn;
+    g_tt_nk = nk;
+    return 0;
+}
 
 int bbjd_set_tiles(int proj_blocks, int upd_rows, int blocks_target)
 {

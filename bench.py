@@ -123,6 +123,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--plain-only", action="store_true", help="skip the served steps (profiling the plain leg)")
+    ap.add_argument("--no-tile-table", action="store_true", help="engine default tiles instead of the autotuned table")
     ap.add_argument("--seed", type=int, default=20261203)
     args = ap.parse_args()
 
@@ -135,6 +136,10 @@ def main():
     from bbjd_amd import serve as srv
 
     srv.device_init(sync_yield=True)
+    # the committed per-shape tile table of this device, when there is one
+    from bbjd_amd import autotune
+    tt = None if args.no_tile_table else autotune.apply()
+    log(f"[bench] tile table: {'autotuned ' + tt['tuned'] if tt else 'engine defaults'}")
     reqs = make_requests(args.nreq, args.rate, args.seed)
     for r in reqs:
         r["maxiter"] = args.maxiter

@@ -196,6 +196,16 @@ void bbjd_set_fuse_blocks(int max_blocks);
  * the current target).  Returns -2 on an unsupported value. */
 int bbjd_set_tiles(int proj_blocks, int upd_rows, int blocks_target);
 
+/* Per-shape tile table, as written by the autotuner (bbjd_amd/autotune.py):
+ * nn x nk classes, class (i, j) = shapes with n <= n_upper[i] and
+ * k <= k_upper[j] (the last class in each direction takes larger shapes);
+ * proj_blocks[i nk + j] = blocks per projection launch, upd_rows[i nk + j] =
+ * rows per update block (0 for the shape rule; else 64, 128, 256, 512 or
+ * 1024).  nn = 0 clears the table (global defaults).  Not while solves run.
+ * Returns 0 or -2 for an invalid table (at most 8 classes each way). */
+int bbjd_set_tile_table(int nn, const int64_t *n_upper, int nk, const int *k_upper, const int *proj_blocks,
+                        const int *upd_rows);
+
 /* Device memory helpers for callers without another GPU runtime (tests,
  * tools): allocate / free / synchronise the device. */
 void *bbjd_dev_alloc(size_t bytes);

@@ -112,3 +112,24 @@ def test_target_and_status_tables():
     from bbjd_amd import bbjd
     assert bbjd._TARGET_MAP == {'nearest': 0, 'LM': 1, 'SM': 2, 'LR': 3, 'SR': 4, 'LI': 5, 'SI': 6}
     assert sorted(bbjd._STATUS_MSG) == [-5, -4, -3, -2, -1]
+
+
+def test_autotuned_tile_table_loads_and_is_validated(tmp_path):
+    """the committed autotuner table loads into the engine (host-side only);
+    a table with an unsupported update tile is refused"""
+    import json
+    from bbjd_amd import autotune
+    lib = _lib.load()
+    try:
+        t = autotune.apply()
+        assert t is not None and t["device"] == "gfx950"
+        assert len(t["proj_blocks"]) == len(t["n_upper"]) and len(t["proj_blocks"][0]) == len(t["k_upper"])
+        bad = dict(t)
+        bad["upd_rows"] = [[100] * len(t["k_upper"]) for _ in t["n_upper"]]
+        p = tmp_path / "bad.json"
+        p.write_text(json.dumps(bad))
+        with pytest.raises(ValueError):
+            autotune.apply(str(p))
+        assert autotune.apply(str(tmp_path / "missing.json")) is None
+    finally:
+        lib.bbjd_set_tile_table(0, None, 0, None, None, None)
