@@ -1828,7 +1828,10 @@ int bbjd_kbench(int which, int64_t n, int k, int reps, double *avg_ms, double *b
     const size_t npn = (size_t)ntiles(n, BBJD_TILE_MIN_ROWS) + 64;
     rc |= hipMalloc(&blk, (size_t)pe * 16 + npn * 8 + (size_t)(k + 8) * 16 * 5 + 8192) != hipSuccess;
     rc |= hipMalloc(&Zr, (size_t)k * k * 16) != hipSuccess;
-    if (rc) { hipFree(Q); hipFree(x); hipFree(x2); hipFree(blk); hipFree(Zr); hipStreamDestroy(s); return -1; }
+    auto release = [&]() {
+        (void)hipFree(Q); (void)hipFree(x); (void)hipFree(x2); (void)hipFree(blk); (void)hipFree(Zr);
+    };
+    if (rc) { release(); (void)hipStreamDestroy(s); return -1; }
     bbjd_red rd;
     char *b = (char *)blk;
     rd.part = b; b += (size_t)pe * 16;
@@ -1904,7 +1907,7 @@ int bbjd_kbench(int which, int64_t n, int k, int reps, double *avg_ms, double *b
     if (bytes) *bytes = by;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    hipFree(Q); hipFree(x); hipFree(x2); hipFree(blk); hipFree(Zr);
+    release();
     (void)hipStreamDestroy(s);
     return rc ? -1 : 0;
 }
