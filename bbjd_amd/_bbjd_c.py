@@ -15,7 +15,9 @@ Extension (keyword-only): ``device_op`` -- a ``DeviceOperator`` whose
 device-pointer callbacks replace the three Python callables (no host round
 trip per apply); ``precision`` -- 'fp64' or 'mixed' (complex64 inner GMRES);
 ``qz_mode`` -- 'ritz' (ordered Schur form only where a restart may follow)
-or 'schur' (every iteration, the reference's sequence).
+or 'schur' (every iteration, the reference's sequence); ``graphs`` -- with
+``device_op``, capture the inner GMRES steps into HIP graphs and replay
+them (same kernels, same results).
 """
 from __future__ import annotations
 
@@ -53,7 +55,7 @@ def _trampoline(fn, errbox):
 
 def jdqz_c(n, nev, sigma, target, matvec_A, matvec_B, precond_solve, tol=1e-10, maxiter=200,
            maxsub=0, minsub=0, inner_maxiter=0, inner_tol=1e-2, verbose=0, v0=None, *,
-           device_op=None, precision="fp64", return_stats=False, qz_mode="ritz"):
+           device_op=None, precision="fp64", return_stats=False, qz_mode="ritz", graphs=False):
     lib = _lib.load()
     n, nev = int(n), int(nev)
     p = _lib.Params()
@@ -75,6 +77,7 @@ def jdqz_c(n, nev, sigma, target, matvec_A, matvec_B, precond_solve, tol=1e-10, 
     if qz_mode not in ("ritz", "schur"):
         raise ValueError("qz_mode must be 'ritz' or 'schur'")
     opts.qz_mode = _lib.QZ_SCHUR if qz_mode == "schur" else _lib.QZ_RITZ
+    opts.graphs = int(bool(graphs))
     errbox: list = []
     keep = []
     if device_op is not None:

@@ -44,14 +44,16 @@ QZ_RITZ, QZ_SCHUR = 0, 1
 class GpuOptions(C.Structure):
     _fields_ = [("callback_space", C.c_int), ("precision", C.c_int), ("stream", C.c_void_p),
                 ("matvec_A_c64", C.c_void_p), ("matvec_B_c64", C.c_void_p),
-                ("precond_c64", C.c_void_p), ("precond2", C.c_void_p), ("qz_mode", C.c_int)]
+                ("precond_c64", C.c_void_p), ("precond2", C.c_void_p), ("qz_mode", C.c_int),
+                ("graphs", C.c_int)]
 
 
 class Stats(C.Structure):
     _fields_ = [("outer_iters", C.c_int64), ("lock_iters", C.c_int64), ("restarts", C.c_int64),
                 ("inner_steps", C.c_int64), ("dgks_second_passes", C.c_int64),
                 ("sum_k", C.c_int64), ("qz_seconds", C.c_double), ("solve_seconds", C.c_double),
-                ("stream_bytes", C.c_double), ("restart_fallbacks", C.c_int64)]
+                ("stream_bytes", C.c_double), ("restart_fallbacks", C.c_int64),
+                ("graph_captures", C.c_int64), ("graph_launches", C.c_int64)]
 
 
 class PencilDesc(C.Structure):

@@ -15,7 +15,8 @@ Device fast path (extension): pass a ``DeviceOperator`` as A (B None) and
 the same or another DeviceOperator, or True, as ``preconditioner``; then the
 matvecs and the preconditioner run on the GPU as well and no n-vector
 crosses PCIe during the iteration.  ``precision='mixed'`` runs the inner
-GMRES (inner_maxiter > 0) in complex64 with FP32 accumulation.
+GMRES (inner_maxiter > 0) in complex64 with FP32 accumulation, and
+``graphs=True`` replays its steps from captured HIP graphs.
 """
 from __future__ import annotations
 
@@ -48,7 +49,7 @@ _STATUS_MSG = {
 def jdqz(A, B=None, k: int = 6, sigma: complex = 0.0, which: str = 'nearest', tol: float = 1e-10,
          maxiter: int = 200, maxsub: int = 0, minsub: int = 0, inner_maxiter: int = 0,
          inner_tol: float = 1e-2, preconditioner=None, v0=None, verbose: int = 0,
-         return_info: bool = False, precision: str = 'fp64'):
+         return_info: bool = False, precision: str = 'fp64', graphs: bool = False):
     """Jacobi-Davidson QZ for A x = lambda B x, eigenvalues nearest sigma
     (or by `which`).
 
@@ -79,7 +80,7 @@ def jdqz(A, B=None, k: int = 6, sigma: complex = 0.0, which: str = 'nearest', to
                              None if preconditioner is None else True, tol=tol, maxiter=maxiter,
                              maxsub=maxsub, minsub=minsub, inner_maxiter=inner_maxiter,
                              inner_tol=inner_tol, verbose=verbose, v0=v0, device_op=A,
-                             precision=precision)
+                             precision=precision, graphs=graphs)
         return _finish(out, k, verbose, return_info)
 
     if isinstance(A, np.ndarray):

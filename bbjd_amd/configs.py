@@ -72,7 +72,7 @@ def rel_diff(a, b):
 
 
 def plain(cfg, n, bs, hw, ntau, k, sigma, maxsub, minsub, pc, seed, inner=0, inner_tol=1e-2,
-          precision="fp64", maxiter=2000, verbose=1, pc_shift=0.0):
+          precision="fp64", maxiter=2000, verbose=1, pc_shift=0.0, graphs=False):
     t0 = time.perf_counter()
     p = BandedTauPencil.generate(n, bs, hw, ntau, seed=seed)
     op = DeviceOperator(p, sigma + pc_shift, pc=pc, c64=precision == "mixed")
@@ -80,12 +80,12 @@ def plain(cfg, n, bs, hw, ntau, k, sigma, maxsub, minsub, pc, seed, inner=0, inn
     t = time.perf_counter()
     ev, X, info = jdqz(op, k=k, sigma=sigma, tol=1e-10, maxsub=maxsub, minsub=minsub, maxiter=maxiter,
                        inner_maxiter=inner, inner_tol=inner_tol, preconditioner=op, return_info=True,
-                       precision=precision, verbose=verbose)
+                       precision=precision, verbose=verbose, graphs=graphs)
     dt = time.perf_counter() - t
     res = pair_residuals(op, p, ev, X) if len(ev) else np.zeros(0)
     return {"config": cfg, "n": n, "k": k, "sigma": [sigma.real, sigma.imag], "maxsub": maxsub,
             "minsub": minsub, "preconditioner": pc + (f" at sigma + {pc_shift}" if pc_shift else ""),
-            "inner_maxiter": inner, "precision": precision, "_ev": ev,
+            "inner_maxiter": inner, "precision": precision, "graphs": bool(graphs), "_ev": ev,
             "nconv": int(info["nconv"]), "iters": int(info["iters"]), "status": int(info["status"]),
             "solve_seconds": round(dt, 2), "eigenpairs_per_s": round(info["nconv"] / dt, 3),
             "setup_seconds": round(t_setup, 2), "device_bytes": op.device_bytes,
@@ -95,7 +95,7 @@ def plain(cfg, n, bs, hw, ntau, k, sigma, maxsub, minsub, pc, seed, inner=0, inn
             "eigenvalues_head": [[float(z.real), float(z.imag)] for z in ev[:4]]}
 
 
-def batched(requests, seed, workers=16, pc="block_jacobi"):
+def batched(requests, seed, workers=16, pc="block_jacobi", graphs=False):
     from . import serve as srv
     srv.device_init(sync_yield=True)
     rng = np.random.default_rng(seed)
@@ -113,7 +113,7 @@ def batched(requests, seed, workers=16, pc="block_jacobi"):
     for b0 in range(0, requests, 256):
         t = time.perf_counter()
         r, rep = srv.serve(jobs[b0:b0 + 256], workers=workers, precision="mixed", realtime=False,
-                           keep_vectors=True)
+                           keep_vectors=True, graphs=graphs)
         dt = time.perf_counter() - t
         res += r
         wall += dt
@@ -133,7 +133,8 @@ def batched(requests, seed, workers=16, pc="block_jacobi"):
         checked += len(rr)
         bad += int(np.sum(rr > 1e-10))
     statuses = sorted(set(r["status"] for r in res))
-    return {"config": 5, "preconditioner": pc, "requests": requests, "workers": used, "pairs": pairs,
+    return {"config": 5, "preconditioner": pc, "graphs": bool(graphs), "requests": requests, "workers": used,
+            "pairs": pairs,
             "eigenpairs_per_s": round(pairs / wall, 2), "wall_seconds": round(wall, 2),
             "requests_complete": int(sum(r["nconv"] == 8 for r in res)), "statuses": statuses,
             "fp64_residual_pairs_checked": checked, "max_fp64_residual": worst, "pairs_over_tol": bad,
@@ -151,6 +152,8 @@ def main():
     ap.add_argument("--pc", choices=["block_jacobi", "shifted", "exact"], default="block_jacobi",
                     help="preconditioner: config 3 block_jacobi or shifted, config 5 block_jacobi or exact")
     ap.add_argument("--pc-shift", type=float, default=0.02, help="config 3 shifted: delta")
+    ap.add_argument("--graphs", action="store_true",
+                    help="configs 3 and 5: replay the inner GMRES steps from captured HIP graphs")
     a = ap.parse_args()
     _lib.load()
     if a.config == 1:
@@ -163,7 +166,7 @@ def main():
         shift = a.pc_shift if a.pc == "shifted" else 0.0
         pc = "exact" if a.pc == "shifted" else "block_jacobi"
         out = plain(3, 250_000, 8, 2, 8, 32, 1 + 2j, 120, 48, pc, a.seed, inner=50, inner_tol=1e-3,
-                    precision="mixed", verbose=a.verbose, pc_shift=shift)
+                    precision="mixed", verbose=a.verbose, pc_shift=shift, graphs=a.graphs)
         if out["nconv"]:
             ref = plain(3, 250_000, 8, 2, 8, 32, 1 + 2j, 120, 48, pc, a.seed, inner=50, inner_tol=1e-3,
                         precision="fp64", verbose=a.verbose, pc_shift=shift)
@@ -174,7 +177,8 @@ def main():
         else:
             out.pop("_ev")
     else:
-        out = batched(a.requests, a.seed, a.workers, pc="exact" if a.pc == "exact" else "block_jacobi")
+        out = batched(a.requests, a.seed, a.workers, pc="exact" if a.pc == "exact" else "block_jacobi",
+                      graphs=a.graphs)
     print(json.dumps(out), flush=True)
 
 

@@ -113,12 +113,21 @@ int bbjd_k_norm(hipStream_t s, int64_t n, const void *x, bbjd_red *rd, int slot)
 /* mode 0: x *= 1/scal[slot]; mode 1: only when scal[slot] > 1e-300 */
 int bbjd_k_scale_inv(hipStream_t s, int64_t n, void *x, const double *scal, int slot, int mode);
 
+/* complex64 form of bbjd_k_scale_inv */
+int bbjd_k_scale_inv_c(hipStream_t s, int64_t n, void *x, const double *scal, int slot, int mode);
+
 /* x *= a (host scalar a = (are, aim)) */
 int bbjd_k_scale(hipStream_t s, int64_t n, void *x, double are, double aim);
 
 /* out = a x + b y (host scalars); out may alias x or y */
 int bbjd_k_lincomb(hipStream_t s, int64_t n, double are, double aim, const void *x,
                    double bre, double bim, const void *y, void *out);
+
+/* out = a x + b y with the coefficients in device memory:
+ * co = {Re a, Im a, Re b, Im b} (doubles); same arithmetic as bbjd_k_lincomb.
+ * _c: complex64 vectors. */
+int bbjd_k_lincomb_dev(hipStream_t s, int64_t n, const double *co, const void *x, const void *y, void *out);
+int bbjd_k_lincomb_dev_c(hipStream_t s, int64_t n, const double *co, const void *x, const void *y, void *out);
 
 /* r = a x + b y and scal[slot] = ||r|| */
 int bbjd_k_lincomb_norm(hipStream_t s, int64_t n, double are, double aim, const void *x,
@@ -152,6 +161,18 @@ int bbjd_k_norm_c(hipStream_t s, int64_t n, const void *x, bbjd_red *rd, int slo
 int bbjd_k_scale_c(hipStream_t s, int64_t n, void *x, double are, double aim);
 int bbjd_k_lincomb_c(hipStream_t s, int64_t n, double are, double aim, const void *x,
                      double bre, double bim, const void *y, void *out);
+
+/* Process-wide order between HIP graph captures and device-wide calls.
+ * ROCm's allocation and release calls synchronise the whole device, which
+ * invalidates a stream capture in progress on another thread: captures hold
+ * the lock shared (bbjd_capture_*), set-up and release of workspaces and
+ * operators exclusively (bbjd_devwide_*).  bbjd_devwide_lock takes it only
+ * while a workspace with graphs exists (no capture can run otherwise) and
+ * returns whether it did; pass that to bbjd_devwide_unlock. */
+void bbjd_capture_lock(void);
+void bbjd_capture_unlock(void);
+int  bbjd_devwide_lock(void);
+void bbjd_devwide_unlock(int held);
 
 /* bytes of reduction scratch for the given shape */
 int64_t bbjd_red_part_entries(int64_t n, int maxk);

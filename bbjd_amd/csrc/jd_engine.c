@@ -23,6 +23,7 @@
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
+#include <pthread.h>
 #include <string.h>
 #include <time.h>
 
@@ -56,6 +57,13 @@ struct bbjd_dev {
     bbjd_stats       st;
     double          *pair_res;         /* per converged pair, last solve    */
     bbjd_int         nconv_last;
+    /* captured inner-solve segments (bbjd_gpu_options.graphs): [0] the
+     * GMRES prologue, [1 + j] Arnoldi step j, valid for gkey */
+    hipGraphExec_t  *gx;
+    int              gx_cap;
+    struct gkey { bbjd_int ndefl, n, ld; const void *r, *z, *qr, *ql, *op; } gkey;
+    double          *gco;              /* device: theta_b, -theta_a         */
+    double          *hgco;             /* pinned: their staging             */
 };
 
 static __thread hipStream_t tl_cb_stream;
@@ -84,7 +92,7 @@ typedef struct {
  * come back to the host in one copy */
 enum { O_V, O_W, O_AV, O_BV, O_QR, O_QL, O_GV, O_T1, O_T2, O_T3, O_T4, O_HW, O_PART, O_PNORM, O_H, O_H2,
        O_HSUM, O_PEN, O_SCAL, O_FLAG, O_CNT, O_COEF, O_COEF2, O_ZR, O_GRAM, O_GVC, O_QRC, O_QLC, O_TC0, O_TC1,
-       O_TC2, O_END };
+       O_TC2, O_GCO, O_END };
 
 static bbjd_int auto_maxsub(const bbjd_params *p)
 {
@@ -123,6 +131,7 @@ static void make_layout(layout_t *L, const bbjd_params *p, const bbjd_gpu_option
     sz[O_ZR] = ms * ms * Z;
     sz[O_PEN] = 4 * (ms + 1) * Z;
     sz[O_GRAM] = 2 * ms * ms * Z;
+    sz[O_GCO] = 64;
     if (L->mixed) {
         sz[O_GVC] = ld * (size_t)(L->ims + 1) * C;
         sz[O_QRC] = sz[O_QLC] = ld * (size_t)(L->nev + 1) * C;
@@ -164,12 +173,49 @@ size_t bbjd_workspace_bytes(const bbjd_params *p, const bbjd_gpu_options *o)
     return L.total;
 }
 
-void bbjd_workspace_free(bbjd_workspace *ws)
+static pthread_rwlock_t g_capture_rw = PTHREAD_RWLOCK_INITIALIZER;
+static int g_graph_workspaces;      /* workspaces with graphs on; changed
+                                       under the exclusive lock only */
+void bbjd_capture_lock(void) { pthread_rwlock_rdlock(&g_capture_rw); }
+void bbjd_capture_unlock(void) { pthread_rwlock_unlock(&g_capture_rw); }
+int bbjd_devwide_lock(void)
+{
+    if (__atomic_load_n(&g_graph_workspaces, __ATOMIC_ACQUIRE) == 0) return 0;
+    pthread_rwlock_wrlock(&g_capture_rw);
+    return 1;
+}
+void bbjd_devwide_unlock(int held)
+{
+    if (held) pthread_rwlock_unlock(&g_capture_rw);
+}
+
+static void graphs_reset(struct bbjd_dev *d)
+{
+    for (int i = 0; i < d->gx_cap; i++)
+        if (d->gx[i]) { (void)hipGraphExecDestroy(d->gx[i]); d->gx[i] = NULL; }
+    memset(&d->gkey, 0, sizeof(d->gkey));
+}
+
+/* graphs_reset outside workspace release: destroying executable graphs
+ * frees device memory, so it excludes captures on other threads */
+static void graphs_drop(struct bbjd_dev *d)
+{
+    int any = 0;
+    for (int i = 0; i < d->gx_cap && !any; i++) any = d->gx[i] != NULL;
+    if (!any) { memset(&d->gkey, 0, sizeof(d->gkey)); return; }
+    pthread_rwlock_wrlock(&g_capture_rw);
+    graphs_reset(d);
+    pthread_rwlock_unlock(&g_capture_rw);
+}
+
+static void workspace_free_impl(bbjd_workspace *ws)
 {
     if (!ws) return;
     struct bbjd_dev *d = ws->dev;
     if (d) {
         if (d->stream) (void)hipStreamSynchronize(d->stream);   /* copies into pinned buffers */
+        if (d->gx) { graphs_reset(d); free(d->gx); }
+        if (d->hgco) hipHostFree(d->hgco);
         if (d->dblock) hipFree(d->dblock);
         if (d->hscal) hipHostFree(d->hscal);
         if (d->hsmall) hipHostFree(d->hsmall);
@@ -188,7 +234,18 @@ void bbjd_workspace_free(bbjd_workspace *ws)
     free(ws);
 }
 
-bbjd_workspace *bbjd_workspace_alloc_gpu(const bbjd_params *p, const bbjd_gpu_options *o)
+/* Workspaces always take the exclusive lock: they change the count of
+ * graph-enabled workspaces that gates it. */
+void bbjd_workspace_free(bbjd_workspace *ws)
+{
+    if (!ws) return;
+    pthread_rwlock_wrlock(&g_capture_rw);
+    if (ws->dev && ws->dev->opt.graphs) __atomic_sub_fetch(&g_graph_workspaces, 1, __ATOMIC_RELEASE);
+    workspace_free_impl(ws);
+    pthread_rwlock_unlock(&g_capture_rw);
+}
+
+static bbjd_workspace *workspace_alloc_impl(const bbjd_params *p, const bbjd_gpu_options *o)
 {
     if (!p || p->n <= 0 || p->nev <= 0) return NULL;
     bbjd_gpu_options dflt;
@@ -233,7 +290,7 @@ bbjd_workspace *bbjd_workspace_alloc_gpu(const bbjd_params *p, const bbjd_gpu_op
         if (maxk < L.nev + 1) maxk = (int)L.nev + 1;
         d->rd.part_cap = bbjd_red_part_entries(L.n, maxk + 2);
     }
-    d->zr = AT(O_ZR); d->pencil = AT(O_PEN); d->gram = AT(O_GRAM);
+    d->zr = AT(O_ZR); d->pencil = AT(O_PEN); d->gram = AT(O_GRAM); d->gco = AT(O_GCO);
     if (L.mixed) {
         d->gvc = AT(O_GVC); d->qrc = AT(O_QRC); d->qlc = AT(O_QLC);
         d->tc[0] = AT(O_TC0); d->tc[1] = AT(O_TC1); d->tc[2] = AT(O_TC2);
@@ -268,6 +325,9 @@ bbjd_workspace *bbjd_workspace_alloc_gpu(const bbjd_params *p, const bbjd_gpu_op
         goto fail;
     }
     if (!HIPOK(hipHostMalloc((void **)&d->hgram, 2 * ss * sizeof(zcx), 0))) { d->hgram = NULL; goto fail; }
+    if (!HIPOK(hipHostMalloc((void **)&d->hgco, 64, 0))) { d->hgco = NULL; goto fail; }
+    d->gx_cap = (int)L.ims + 1;
+    if (!(d->gx = calloc((size_t)d->gx_cap, sizeof(hipGraphExec_t)))) goto fail;
     d->pair_res = calloc((size_t)L.nev + 1, sizeof(double));
     if (!ws->MA || !ws->MB || !ws->SA || !ws->SB || !ws->ZA || !ws->ZB || !ws->gmres_H || !ws->gmres_cs ||
         !ws->gmres_sn || !ws->gmres_g || !d->al || !d->be || !d->yv || !d->xv || !d->pair_res)
@@ -276,8 +336,17 @@ bbjd_workspace *bbjd_workspace_alloc_gpu(const bbjd_params *p, const bbjd_gpu_op
     if (!HIPOK(hipStreamSynchronize(d->stream))) goto fail;
     return ws;
 fail:
-    bbjd_workspace_free(ws);
+    workspace_free_impl(ws);
     return NULL;
+}
+
+bbjd_workspace *bbjd_workspace_alloc_gpu(const bbjd_params *p, const bbjd_gpu_options *o)
+{
+    pthread_rwlock_wrlock(&g_capture_rw);
+    bbjd_workspace *ws = workspace_alloc_impl(p, o);
+    if (ws && ws->dev->opt.graphs) __atomic_add_fetch(&g_graph_workspaces, 1, __ATOMIC_RELEASE);
+    pthread_rwlock_unlock(&g_capture_rw);
+    return ws;
 }
 
 bbjd_workspace *bbjd_workspace_alloc(const bbjd_params *p) { return bbjd_workspace_alloc_gpu(p, NULL); }
@@ -350,6 +419,7 @@ typedef struct {
     zcx               nu0, mu0;  /* the test space's combination             */
     int               w_mixed;   /* a W column came from the Av fallback since
                                     the last restart: no triangular relation */
+    int               graphs;    /* capture/replay the inner-solve segments  */
 } ctx_t;
 
 #define COLZ(M, j) ((void *)((zcx *)(M) + (size_t)(j) * c->ld))
@@ -466,13 +536,68 @@ static int apply_c64(ctx_t *c, bbjd_op_c64_fn f64, bbjd_matvec_fn f, const void 
     return bbjd_k_z2c(c->s, c->n, z2, y);
 }
 
+/* Captured segments of the inner solve (bbjd_gpu_options.graphs, device
+ * callbacks).  seg_begin returns 1 when segment idx was replayed from its
+ * graph (skip the body), 0 when the body is to run now (under capture when
+ * graphs are on), -1 on error; seg_end closes a capture, instantiates and
+ * launches it.  A body enqueues work only; its error ends the capture too.
+ * A capture holds the process-wide capture lock shared, so that no other
+ * thread allocates or frees device memory meanwhile; instantiating and
+ * destroying graphs, which do, hold it exclusively. */
+static int seg_begin(ctx_t *c, int idx)
+{
+    struct bbjd_dev *d = c->d;
+    if (!c->graphs) return 0;
+    if (d->gx[idx]) {
+        if (!HIPOK(hipGraphLaunch(d->gx[idx], c->s))) return -1;
+        d->st.graph_launches++;
+        return 1;
+    }
+    bbjd_capture_lock();
+    if (HIPOK(hipStreamBeginCapture(c->s, hipStreamCaptureModeRelaxed))) return 0;
+    bbjd_capture_unlock();
+    return -1;
+}
+
+static int seg_end(ctx_t *c, int idx, int rc)
+{
+    struct bbjd_dev *d = c->d;
+    if (!c->graphs) return rc;
+    hipGraph_t g = NULL;
+    const hipError_t e = hipStreamEndCapture(c->s, &g);
+    bbjd_capture_unlock();
+    if (rc || e != hipSuccess) {
+        if (e == hipSuccess && g) (void)hipGraphDestroy(g);
+        return rc ? rc : -2;
+    }
+    /* instantiation allocates device memory: exclusive */
+    hipGraphExec_t x = NULL;
+    pthread_rwlock_wrlock(&g_capture_rw);
+    const hipError_t ei = hipGraphInstantiateWithFlags(&x, g, 0);
+    (void)hipGraphDestroy(g);
+    pthread_rwlock_unlock(&g_capture_rw);
+    if (ei != hipSuccess) return -2;
+    d->gx[idx] = x;
+    d->st.graph_captures++;
+    if (!HIPOK(hipGraphLaunch(x, c->s))) return -2;
+    d->st.graph_launches++;
+    return 0;
+}
+
 /* Left-preconditioned GMRES on the projected correction equation
  * (reference src/jd_core.c:568-727).  r, z: device complex128; QR, QL: the
  * ndefl leading columns of the deflation projectors.  Arnoldi is CGS2 with
  * the same DGKS test and Hessenberg column (sum of both passes) as the
  * reference's MGS2; Givens, least squares and stopping rule are the
  * reference's, on the host in FP64.  Mixed precision: basis, operator
- * applies and projections in complex64 with FP32 accumulation. */
+ * applies and projections in complex64 with FP32 accumulation.
+ *
+ * The device work of the prologue and of each Arnoldi step is one segment
+ * that ends by copying the scalars (and the Hessenberg column) to the host:
+ * theta_b and -theta_a are read from device memory (gco) and the new basis
+ * vector is scaled by the reciprocal of its device-side norm, so with
+ * graphs a segment is captured once per deflation count and replayed on
+ * later outer iterations. */
 static int gmres(ctx_t *c, bbjd_int ndefl, const void *QR, const void *QL, zcx ta, zcx tb, const void *r, void *z,
                  bbjd_int ims, double tol)
 {
@@ -484,7 +609,7 @@ static int gmres(ctx_t *c, bbjd_int ndefl, const void *QR, const void *QL, zcx t
     zcx *H = (zcx *)ws->gmres_H, *sn = (zcx *)ws->gmres_sn, *g = (zcx *)ws->gmres_g;
     double *cs = (double *)ws->gmres_cs;
     memset(H, 0, (size_t)ldh * ws->inner_maxiter * sizeof(zcx));
-    int rc = 0;
+    int rc = 0, seg;
     /* FP64 temporaries: t1 = tmp3, t2 = tmp4 as in the reference (tmp4 holds
      * r, read only by the first step) */
     void *t1 = ws->tmp3, *t2 = ws->tmp4;
@@ -493,64 +618,97 @@ static int gmres(ctx_t *c, bbjd_int ndefl, const void *QR, const void *QL, zcx t
 #define GCOL(j) ((void *)((char *)V + (size_t)(j) * c->ld * esz))
     const void *qr = mixed ? d->qrc : QR, *ql = mixed ? d->qlc : QL;
 
-    if (!mixed) {
-        rc |= bbjd_k_lincomb(c->s, n, -1.0, 0.0, r, 0.0, 0.0, r, t1);
-        rc |= bbjd_k_project_out(c->s, n, (int)ndefl, QL, c->ld, t1, &d->rd);
-        if (apply_pc(c, t1, GCOL(0))) return -1;
-        rc |= bbjd_k_norm(c->s, n, GCOL(0), &d->rd, SL_G);
-    } else {
-        void *tc0 = d->tc[0];
-        rc |= bbjd_k_z2c(c->s, n, r, tc0);
-        rc |= bbjd_k_scale_c(c->s, n, tc0, -1.0, 0.0);
-        rc |= bbjd_k_project_out_c(c->s, n, (int)ndefl, ql, c->ld, tc0, &d->rd);
-        if (P->precond) {
-            if (apply_c64(c, d->opt.precond_c64, (bbjd_matvec_fn)P->precond, tc0, GCOL(0), t1, t2)) return -1;
-        } else
-            rc |= !HIPOK(hipMemcpyAsync(GCOL(0), tc0, (size_t)n * 8, hipMemcpyDeviceToDevice, c->s));
-        rc |= bbjd_k_norm_c(c->s, n, GCOL(0), &d->rd, SL_G);
+    c->graphs = d->opt.graphs && d->opt.callback_space == BBJD_CALLBACK_DEVICE;
+    if (c->graphs) {
+        struct gkey key;
+        memset(&key, 0, sizeof(key));
+        key.ndefl = ndefl; key.n = n; key.ld = c->ld; key.r = r; key.z = z; key.qr = QR; key.ql = QL;
+        key.op = P->op_data;
+        if (memcmp(&key, &d->gkey, sizeof(key))) {
+            rc |= !HIPOK(hipStreamSynchronize(c->s));
+            graphs_drop(d);
+            d->gkey = key;
+        }
     }
-    if (rc || sync_scal(c)) return -2;
+    d->hgco[0] = creal(tb); d->hgco[1] = cimag(tb); d->hgco[2] = -creal(ta); d->hgco[3] = -cimag(ta);
+    rc |= !HIPOK(hipMemcpyAsync(d->gco, d->hgco, 4 * sizeof(double), hipMemcpyHostToDevice, c->s));
+    if (rc) return -2;
+
+    int cb = 0;              /* a callback failed */
+    if ((seg = seg_begin(c, 0)) < 0) return -2;
+    if (seg == 0) {
+        if (!mixed) {
+            rc |= bbjd_k_lincomb(c->s, n, -1.0, 0.0, r, 0.0, 0.0, r, t1);
+            rc |= bbjd_k_project_out(c->s, n, (int)ndefl, QL, c->ld, t1, &d->rd);
+            if (!rc && apply_pc(c, t1, GCOL(0))) cb = 1;
+            rc |= bbjd_k_norm(c->s, n, GCOL(0), &d->rd, SL_G);
+            rc |= bbjd_k_scale_inv(c->s, n, GCOL(0), d->rd.scal, SL_G, 0);
+        } else {
+            void *tc0 = d->tc[0];
+            rc |= bbjd_k_z2c(c->s, n, r, tc0);
+            rc |= bbjd_k_scale_c(c->s, n, tc0, -1.0, 0.0);
+            rc |= bbjd_k_project_out_c(c->s, n, (int)ndefl, ql, c->ld, tc0, &d->rd);
+            if (P->precond) {
+                if (!rc && apply_c64(c, d->opt.precond_c64, (bbjd_matvec_fn)P->precond, tc0, GCOL(0), t1, t2))
+                    cb = 1;
+            } else
+                rc |= !HIPOK(hipMemcpyAsync(GCOL(0), tc0, (size_t)n * 8, hipMemcpyDeviceToDevice, c->s));
+            rc |= bbjd_k_norm_c(c->s, n, GCOL(0), &d->rd, SL_G);
+            rc |= bbjd_k_scale_inv_c(c->s, n, GCOL(0), d->rd.scal, SL_G, 0);
+        }
+        rc |= !HIPOK(hipMemcpyAsync(d->hscal, d->rd.scal, SL_COUNT * 8, hipMemcpyDeviceToHost, c->s));
+        rc = seg_end(c, 0, rc | cb);
+    }
+    if (cb) return -1;
+    if (rc || !HIPOK(hipStreamSynchronize(c->s))) return -2;
     const double beta0 = d->hscal[SL_G];
     if (beta0 < 1e-300) {
         return HIPOK(hipMemsetAsync(z, 0, (size_t)n * 16, c->s)) ? 0 : -2;
     }
-    if (!mixed) rc |= bbjd_k_scale(c->s, n, GCOL(0), 1.0 / beta0, 0.0);
-    else rc |= bbjd_k_scale_c(c->s, n, GCOL(0), 1.0 / beta0, 0.0);
     memset(g, 0, (size_t)(ims + 1) * sizeof(zcx));
     g[0] = beta0;
 
     bbjd_int j;
     for (j = 0; j < ims; j++) {
         void *vj = GCOL(j), *vj1 = GCOL(j + 1);
-        if (!mixed) {
-            rc |= !HIPOK(hipMemcpyAsync(t1, vj, (size_t)n * 16, hipMemcpyDeviceToDevice, c->s));
-            rc |= bbjd_k_project_out(c->s, n, (int)ndefl, QR, c->ld, t1, &d->rd);
-            if (apply_op(c, P->matvec_A, t1, t2)) return -1;
-            if (apply_op(c, P->matvec_B, t1, vj1)) return -1;
-            rc |= bbjd_k_lincomb(c->s, n, creal(tb), cimag(tb), t2, -creal(ta), -cimag(ta), vj1, t2);
-            rc |= bbjd_k_project_out(c->s, n, (int)ndefl, QL, c->ld, t2, &d->rd);
-            if (apply_pc(c, t2, vj1)) return -1;
-            rc |= bbjd_k_cgs2(c->s, n, (int)(j + 1), V, c->ld, vj1, &d->rd, SL_G, d->rd.hsum);
-        } else {
-            void *a = d->tc[0], *bq = d->tc[1], *e = d->tc[2];
-            rc |= !HIPOK(hipMemcpyAsync(a, vj, (size_t)n * 8, hipMemcpyDeviceToDevice, c->s));
-            rc |= bbjd_k_project_out_c(c->s, n, (int)ndefl, qr, c->ld, a, &d->rd);
-            if (apply_c64(c, d->opt.matvec_A_c64, P->matvec_A, a, bq, t1, t2)) return -1;
-            if (apply_c64(c, d->opt.matvec_B_c64, P->matvec_B, a, e, t1, t2)) return -1;
-            rc |= bbjd_k_lincomb_c(c->s, n, creal(tb), cimag(tb), bq, -creal(ta), -cimag(ta), e, bq);
-            rc |= bbjd_k_project_out_c(c->s, n, (int)ndefl, ql, c->ld, bq, &d->rd);
-            if (P->precond) {
-                if (apply_c64(c, d->opt.precond_c64, (bbjd_matvec_fn)P->precond, bq, vj1, t1, t2)) return -1;
-            } else
-                rc |= !HIPOK(hipMemcpyAsync(vj1, bq, (size_t)n * 8, hipMemcpyDeviceToDevice, c->s));
-            rc |= bbjd_k_cgs2_c(c->s, n, (int)(j + 1), V, c->ld, vj1, &d->rd, SL_G, d->rd.hsum);
+        if ((seg = seg_begin(c, (int)j + 1)) < 0) return -2;
+        if (seg == 0) {
+            if (!mixed) {
+                rc |= !HIPOK(hipMemcpyAsync(t1, vj, (size_t)n * 16, hipMemcpyDeviceToDevice, c->s));
+                rc |= bbjd_k_project_out(c->s, n, (int)ndefl, QR, c->ld, t1, &d->rd);
+                if (!rc && apply_op(c, P->matvec_A, t1, t2)) cb = 1;
+                if (!rc && !cb && apply_op(c, P->matvec_B, t1, vj1)) cb = 1;
+                rc |= bbjd_k_lincomb_dev(c->s, n, d->gco, t2, vj1, t2);
+                rc |= bbjd_k_project_out(c->s, n, (int)ndefl, QL, c->ld, t2, &d->rd);
+                if (!rc && !cb && apply_pc(c, t2, vj1)) cb = 1;
+                rc |= bbjd_k_cgs2(c->s, n, (int)(j + 1), V, c->ld, vj1, &d->rd, SL_G, d->rd.hsum);
+                rc |= !HIPOK(hipMemcpyAsync(d->hsmall, d->rd.hsum, (size_t)(j + 1) * 16, hipMemcpyDeviceToHost,
+                                            c->s));
+                rc |= bbjd_k_scale_inv(c->s, n, vj1, d->rd.scal, SL_G, 0);
+            } else {
+                void *a = d->tc[0], *bq = d->tc[1], *e = d->tc[2];
+                rc |= !HIPOK(hipMemcpyAsync(a, vj, (size_t)n * 8, hipMemcpyDeviceToDevice, c->s));
+                rc |= bbjd_k_project_out_c(c->s, n, (int)ndefl, qr, c->ld, a, &d->rd);
+                if (!rc && apply_c64(c, d->opt.matvec_A_c64, P->matvec_A, a, bq, t1, t2)) cb = 1;
+                if (!rc && !cb && apply_c64(c, d->opt.matvec_B_c64, P->matvec_B, a, e, t1, t2)) cb = 1;
+                rc |= bbjd_k_lincomb_dev_c(c->s, n, d->gco, bq, e, bq);
+                rc |= bbjd_k_project_out_c(c->s, n, (int)ndefl, ql, c->ld, bq, &d->rd);
+                if (P->precond) {
+                    if (!rc && !cb && apply_c64(c, d->opt.precond_c64, (bbjd_matvec_fn)P->precond, bq, vj1, t1, t2))
+                        cb = 1;
+                } else
+                    rc |= !HIPOK(hipMemcpyAsync(vj1, bq, (size_t)n * 8, hipMemcpyDeviceToDevice, c->s));
+                rc |= bbjd_k_cgs2_c(c->s, n, (int)(j + 1), V, c->ld, vj1, &d->rd, SL_G, d->rd.hsum);
+                rc |= !HIPOK(hipMemcpyAsync(d->hsmall, d->rd.hsum, (size_t)(j + 1) * 8, hipMemcpyDeviceToHost,
+                                            c->s));
+                rc |= bbjd_k_scale_inv_c(c->s, n, vj1, d->rd.scal, SL_G, 0);
+            }
+            /* both norms down (the Hessenberg column came above) */
+            rc |= !HIPOK(hipMemcpyAsync(d->hscal, d->rd.scal, SL_COUNT * 8, hipMemcpyDeviceToHost, c->s));
+            rc = seg_end(c, (int)j + 1, rc | cb);
         }
-        /* Hessenberg column + both norms down */
-        if (!mixed)
-            rc |= !HIPOK(hipMemcpyAsync(d->hsmall, d->rd.hsum, (size_t)(j + 1) * 16, hipMemcpyDeviceToHost, c->s));
-        else
-            rc |= !HIPOK(hipMemcpyAsync(d->hsmall, d->rd.hsum, (size_t)(j + 1) * 8, hipMemcpyDeviceToHost, c->s));
-        if (rc || sync_scal(c)) return -2;
+        if (cb) return -1;
+        if (rc || !HIPOK(hipStreamSynchronize(c->s))) return -2;
         d->st.inner_steps++;
         d->st.stream_bytes += (double)esz * n * (8.0 * ndefl + 4.0 * (j + 1) + 8);
         if (!mixed)
@@ -562,8 +720,6 @@ static int gmres(ctx_t *c, bbjd_int ndefl, const void *QR, const void *QL, zcx t
         const double wb = d->hscal[SL_NB], wa = d->hscal[SL_G];
         H[(j + 1) + j * ldh] = wa;
         if (wa < 1e-14 * wb) { j++; break; }                      /* :674 */
-        if (!mixed) rc |= bbjd_k_scale(c->s, n, vj1, 1.0 / wa, 0.0);
-        else rc |= bbjd_k_scale_c(c->s, n, vj1, 1.0 / wa, 0.0);
         for (bbjd_int i = 0; i < j; i++) {                       /* :680-685 */
             zcx h1 = H[i + j * ldh], h2 = H[i + 1 + j * ldh];
             H[i + j * ldh] = cs[i] * h1 + sn[i] * h2;
@@ -871,6 +1027,7 @@ static bbjd_status solve_impl(const bbjd_params *P, bbjd_workspace *ws, bbjd_res
     struct bbjd_dev *d = ws->dev;
     const double t_start = now_s();
     memset(&d->st, 0, sizeof(d->st));
+    graphs_drop(d);
     /* n-sized arrays are laid out for this problem (leading dimension
      * roundup(n, 64)) inside the workspace's buffers, which may be sized for a
      * larger one: a resident workspace serving a small request then touches

@@ -832,13 +832,17 @@ __global__ __launch_bounds__(TPB) void k_scale(i64 n, Cx<R> *x, R are, R aim)
     for (i64 i = (i64)blockIdx.x * TPB + threadIdx.x; i < n; i += (i64)gridDim.x * TPB) x[i] = a * x[i];
 }
 
+// out = a x + b y; with co non-NULL the coefficients are read from device
+// memory instead, a = (co[0], co[1]), b = (co[2], co[3]) (doubles, rounded
+// to R as host arguments are), so a captured graph replays with new ones.
 template <typename R, int E>
 __global__ __launch_bounds__(TPB) void k_lincomb(i64 n, R are, R aim, const Cx<R> *x, R bre, R bim,
                                                  const Cx<R> *y, Cx<R> *out, R *pnorm, int *cnt, NormFin fin,
-                                                 int fused)
+                                                 int fused, const double *__restrict__ co)
 {
     __shared__ R lds4[4];
     __shared__ int lflag;
+    if (co) { are = (R)co[0]; aim = (R)co[1]; bre = (R)co[2]; bim = (R)co[3]; }
     const Cx<R> a = cx(are, aim), b = cx(bre, bim);
     const i64 rb = (i64)blockIdx.x * (TPB * E);
     R s = 0;
@@ -1241,7 +1245,12 @@ static int proj_pass_nv(hipStream_t s, i64 n, int k, const void *Q, i64 ldq, con
     if (k == 0 && !want_norm) return 0;
     const int fused = nt <= g_fuse_max_blocks;
     dim3 g(ctiles, nt);
-    const bool timed = NV == 1 && g_kt_on && sizeof(R) == 8 && cond == nullptr;
+    bool timed = NV == 1 && g_kt_on && sizeof(R) == 8 && cond == nullptr;
+    if (timed) {
+        // a stream under graph capture records no event-carrying dispatch
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) timed = false;
+    }
     if (timed) {
         // events carried by the dispatch itself: their interval is the
         // kernel's execution, as a kernel trace reports it
@@ -1529,7 +1538,7 @@ static int gemv_n(hipStream_t s, i64 n, int k, const void *Q, i64 ldq, const voi
 
 template <typename R>
 static int lincomb(hipStream_t s, i64 n, double are, double aim, const void *x, double bre, double bim,
-                   const void *y, void *out, bbjd_red *rd, int slot)
+                   const void *y, void *out, bbjd_red *rd, int slot, const double *co = nullptr)
 {
     const int rows = vec_rows(n);
     const int nt = ntiles(n, rows);
@@ -1537,7 +1546,8 @@ static int lincomb(hipStream_t s, i64 n, double are, double aim, const void *x, 
 #define BBJD_LAUNCH_LC(E)                                                                                          \
     hipLaunchKernelGGL((k_lincomb<R, E>), dim3(nt), dim3(TPB), 0, s, (i64)n, (R)are, (R)aim, (const Cx<R> *)x,      \
                        (R)bre, (R)bim, (const Cx<R> *)y, (Cx<R> *)out, rd ? (R *)rd->pnorm : (R *)nullptr,          \
-                       rd ? rd->cnt + BBJD_CNT_VEC : (int *)nullptr, rd ? fin_norm(rd, 0, slot) : fin_none(), fused)
+                       rd ? rd->cnt + BBJD_CNT_VEC : (int *)nullptr, rd ? fin_norm(rd, 0, slot) : fin_none(), fused, \
+                       co)
     BBJD_E_SWITCH(rows, BBJD_LAUNCH_LC)
 #undef BBJD_LAUNCH_LC
     if (rd && !fused)
@@ -1683,6 +1693,13 @@ int bbjd_k_scale_inv(hipStream_t s, int64_t n, void *x, const double *scal, int 
     return chk();
 }
 
+int bbjd_k_scale_inv_c(hipStream_t s, int64_t n, void *x, const double *scal, int slot, int mode)
+{
+    hipLaunchKernelGGL((k_scale_inv<float>), dim3(grid_stride_blocks(n)), dim3(TPB), 0, s, (i64)n,
+                       (Cx<float> *)x, scal, slot, mode, (const int *)nullptr);
+    return chk();
+}
+
 int bbjd_k_scale(hipStream_t s, int64_t n, void *x, double are, double aim)
 {
     hipLaunchKernelGGL((k_scale<double>), dim3(grid_stride_blocks(n)), dim3(TPB), 0, s, (i64)n, (Cx<double> *)x,
@@ -1704,6 +1721,12 @@ int bbjd_k_lincomb(hipStream_t s, int64_t n, double are, double aim, const void 
 int bbjd_k_lincomb_c(hipStream_t s, int64_t n, double are, double aim, const void *x, double bre, double bim,
                      const void *y, void *out)
 { return lincomb<float>(s, n, are, aim, x, bre, bim, y, out, nullptr, 0); }
+
+int bbjd_k_lincomb_dev(hipStream_t s, int64_t n, const double *co, const void *x, const void *y, void *out)
+{ return lincomb<double>(s, n, 0, 0, x, 0, 0, y, out, nullptr, 0, co); }
+
+int bbjd_k_lincomb_dev_c(hipStream_t s, int64_t n, const double *co, const void *x, const void *y, void *out)
+{ return lincomb<float>(s, n, 0, 0, x, 0, 0, y, out, nullptr, 0, co); }
 
 int bbjd_k_lincomb_norm(hipStream_t s, int64_t n, double are, double aim, const void *x, double bre, double bim,
                         const void *y, void *r, bbjd_red *rd, int slot)

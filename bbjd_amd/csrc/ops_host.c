@@ -368,7 +368,7 @@ size_t bbjd_op_bytes_for(const bbjd_pencil_desc *d, int pc, int want_c64)
 bbjd_int bbjd_op_size(const bbjd_op *op) { return op ? op->n : 0; }
 
 /* ---- public ---------------------------------------------------------- */
-void bbjd_op_free(bbjd_op *op)
+static void op_free_impl(bbjd_op *op)
 {
     if (!op) return;
     if (op->dblock) hipFree(op->dblock);
@@ -376,9 +376,18 @@ void bbjd_op_free(bbjd_op *op)
     free(op);
 }
 
+void bbjd_op_free(bbjd_op *op)
+{
+    if (!op) return;
+    const int held = bbjd_devwide_lock();
+    op_free_impl(op);
+    bbjd_devwide_unlock(held);
+}
+
 size_t bbjd_op_bytes(const bbjd_op *op) { return op ? op->dbytes : 0; }
 
-bbjd_op *bbjd_op_create(const bbjd_pencil_desc *d, bbjd_zomplex sigma, bbjd_pc_kind pc, int want_c64, void *stream)
+static bbjd_op *op_create_impl(const bbjd_pencil_desc *d, bbjd_zomplex sigma, bbjd_pc_kind pc, int want_c64,
+                               void *stream)
 {
     if (!d || d->n <= 0 || d->bs <= 0 || d->n % d->bs || d->hw < 0 || d->ntau < 0 || d->ntau > 256 ||
         d->ntau >= d->n)
@@ -394,16 +403,16 @@ bbjd_op *bbjd_op_create(const bbjd_pencil_desc *d, bbjd_zomplex sigma, bbjd_pc_k
     size_t bcr_blocks = 0;
     if (pc == BBJD_PC_EXACT_LU) {
         op->s = (d->hw > 0 ? d->hw : 1) * d->bs;
-        if (op->s > 64) { bbjd_op_free(op); return NULL; }
+        if (op->s > 64) { op_free_impl(op); return NULL; }
         op->N = (n1 + op->s - 1) / op->s;
-        if (bcr_factor(op, d, &bcr_host, &bcr_blocks)) { bbjd_op_free(op); return NULL; }
+        if (bcr_factor(op, d, &bcr_host, &bcr_blocks)) { op_free_impl(op); return NULL; }
     }
     /* device layout */
     size_t off[D_END], sz[D_END];
     const size_t tot = op_layout(d, pc, want_c64, op->N, op->s, bcr_blocks, sz, off);
     const size_t nbandel = (size_t)op->nb * op->nd * op->bs * op->bs;
     const int64_t pe = bbjd_red_part_entries(n, op->ntau > 0 ? op->ntau + 2 : 2);
-    if (!HIPOK(hipMalloc(&op->dblock, tot))) { op->dblock = NULL; free(bcr_host); bbjd_op_free(op); return NULL; }
+    if (!HIPOK(hipMalloc(&op->dblock, tot))) { op->dblock = NULL; free(bcr_host); op_free_impl(op); return NULL; }
     op->dbytes = tot;
     char *b = (char *)op->dblock;
 #define AT(i) (sz[i] ? (void *)(b + off[i]) : NULL)
@@ -531,7 +540,15 @@ bbjd_op *bbjd_op_create(const bbjd_pencil_desc *d, bbjd_zomplex sigma, bbjd_pc_k
     free(hb);
     free(bcr_host);
     rc |= !HIPOK(hipStreamSynchronize(st));
-    if (rc) { bbjd_op_free(op); return NULL; }
+    if (rc) { op_free_impl(op); return NULL; }
+    return op;
+}
+
+bbjd_op *bbjd_op_create(const bbjd_pencil_desc *d, bbjd_zomplex sigma, bbjd_pc_kind pc, int want_c64, void *stream)
+{
+    const int held = bbjd_devwide_lock();
+    bbjd_op *op = op_create_impl(d, sigma, pc, want_c64, stream);
+    bbjd_devwide_unlock(held);
     return op;
 }
 

@@ -61,6 +61,13 @@ typedef struct {
      * reference's sequence of LAPACK calls.  Same Ritz pairs in exact
      * arithmetic. */
     int                 qz_mode;
+    /* 1: capture the inner GMRES loop's device work into HIP graphs, once
+     * per deflation count (the prologue and each Arnoldi step), and replay
+     * them; the kernels are the same as without, so are the results.  Takes
+     * effect with BBJD_CALLBACK_DEVICE only, and the device callbacks must
+     * then be capturable: kernels and copies enqueued on the callback
+     * stream, no synchronisation, allocation or memset. */
+    int                 graphs;
 } bbjd_gpu_options;
 
 enum { BBJD_QZ_RITZ = 0, BBJD_QZ_SCHUR = 1 };
@@ -107,6 +114,8 @@ typedef struct {
                                    engine kernels (operators excluded)    */
     bbjd_int restart_fallbacks; /* restarts whose Cholesky QR of the test
                                    space fell back to column-wise CGS2    */
+    bbjd_int graph_captures;    /* inner-solve segments captured (graphs)  */
+    bbjd_int graph_launches;    /* and launched, captures included         */
 } bbjd_stats;
 
 int bbjd_workspace_stats(const bbjd_workspace *ws, bbjd_stats *st);

@@ -50,6 +50,7 @@ typedef struct {
     int    precision;     /* bbjd_precision of every request                 */
     int    realtime;      /* 1: admit a request no earlier than its arrival  */
     int    sync_yield;    /* 1: host threads yield while waiting on the GPU  */
+    int    graphs;        /* bbjd_gpu_options.graphs of every request        */
 } bbjd_serve_config;
 
 typedef struct {

@@ -86,6 +86,45 @@ def test_struct_layout_matches_ctypes(tmp_path):
         assert int(out[f"bbjd_result.{f}"]) == getattr(_lib.Result, f).offset, f
 
 
+GPU_PROBE = r"""
+#include <stdio.h>
+#include <stddef.h>
+#include "bbjd_serve.h"
+#define S(T) printf("sizeof." #T " %zu\n", sizeof(T))
+#define O(T, f) printf(#T "." #f " %zu\n", offsetof(T, f))
+int main(void) {
+  S(bbjd_gpu_options); S(bbjd_stats); S(bbjd_serve_config);
+  O(bbjd_gpu_options, callback_space); O(bbjd_gpu_options, precision); O(bbjd_gpu_options, stream);
+  O(bbjd_gpu_options, matvec_A_c64); O(bbjd_gpu_options, matvec_B_c64); O(bbjd_gpu_options, precond_c64);
+  O(bbjd_gpu_options, precond2); O(bbjd_gpu_options, qz_mode); O(bbjd_gpu_options, graphs);
+  O(bbjd_stats, outer_iters); O(bbjd_stats, lock_iters); O(bbjd_stats, restarts); O(bbjd_stats, inner_steps);
+  O(bbjd_stats, dgks_second_passes); O(bbjd_stats, sum_k); O(bbjd_stats, qz_seconds);
+  O(bbjd_stats, solve_seconds); O(bbjd_stats, stream_bytes); O(bbjd_stats, restart_fallbacks);
+  O(bbjd_stats, graph_captures); O(bbjd_stats, graph_launches);
+  O(bbjd_serve_config, workers); O(bbjd_serve_config, hbm_budget); O(bbjd_serve_config, precision);
+  O(bbjd_serve_config, realtime); O(bbjd_serve_config, sync_yield); O(bbjd_serve_config, graphs);
+  return 0;
+}
+"""
+
+
+def test_gpu_struct_layouts_match_ctypes(tmp_path):
+    """the appended GPU option, statistics and serving structs against their
+    ctypes mirrors, field by field"""
+    from bbjd_amd import serve
+    src = tmp_path / "gprobe.c"
+    exe = tmp_path / "gprobe"
+    src.write_text(GPU_PROBE)
+    subprocess.run(["gcc", "-std=c11", f"-I{INC}", str(src), "-o", str(exe)], check=True)
+    out = dict(line.rsplit(" ", 1) for line in
+               subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.splitlines())
+    for cname, cls in (("bbjd_gpu_options", _lib.GpuOptions), ("bbjd_stats", _lib.Stats),
+                       ("bbjd_serve_config", serve.ServeConfig)):
+        assert int(out[f"sizeof.{cname}"]) == C.sizeof(cls), cname
+        for f, _ in cls._fields_:
+            assert int(out[f"{cname}.{f}"]) == getattr(cls, f).offset, (cname, f)
+
+
 @pytest.mark.skipif(not os.path.exists(os.path.join(REF_INC, "bbjd.h")), reason="reference header absent")
 def test_layout_and_enums_identical_to_reference_header(tmp_path):
     mine = _probe(INC, tmp_path, "mine").splitlines()

@@ -273,6 +273,41 @@ def test_inner_gmres_approximate_preconditioner(precision):
     assert info["iters"] <= info0["iters"]
 
 
+@pytest.mark.parametrize("precision,pc", [("mixed", "exact"), ("fp64", "exact"), ("mixed", "block_jacobi")])
+def test_inner_gmres_graphs_replay_identically(precision, pc):
+    """the inner GMRES prologue and Arnoldi steps captured into HIP graphs
+    (bbjd_gpu_options.graphs) and replayed on later outer iterations run the
+    eager path's kernels: identical eigenvalues, vectors, residuals, outer
+    iterations and inner steps; segments are captured once per deflation
+    count and replayed after that"""
+    p = BandedTauPencil.generate(6000, 4, 1, 2, seed=11)
+    sigma = 0.4 + 0.3j
+    dev = DeviceOperator(p, sigma + (0.5 if pc == "exact" else 0.0), pc=pc, c64=True)
+    kw = dict(tol=1e-10, maxsub=32, minsub=16, maxiter=300, inner_maxiter=20, inner_tol=1e-3,
+              device_op=dev, precision=precision, return_stats=True)
+    eager = jdqz_c(p.n, 6, sigma, 0, None, None, True, graphs=False, **kw)
+    graph = jdqz_c(p.n, 6, sigma, 0, None, None, True, graphs=True, **kw)
+    se, sg = eager[6], graph[6]
+    print(f"[graphs] {precision}/{pc}: nconv {eager[3]} iters {eager[4]} inner {se['inner_steps']} "
+          f"captures {sg['graph_captures']} launches {sg['graph_launches']}", flush=True)
+    assert se["graph_captures"] == 0 and se["graph_launches"] == 0
+    assert eager[3] == graph[3] and eager[4] == graph[4] and eager[5] == graph[5]
+    assert se["inner_steps"] == sg["inner_steps"] > 0
+    np.testing.assert_array_equal(eager[0], graph[0])
+    np.testing.assert_array_equal(eager[1], graph[1])
+    np.testing.assert_array_equal(eager[2], graph[2])
+    assert 0 < sg["graph_captures"] < sg["graph_launches"]
+    # one launch per prologue and per Arnoldi step
+    outer_with_gmres = sg["outer_iters"] - sg["lock_iters"]
+    assert sg["graph_launches"] == sg["inner_steps"] + outer_with_gmres
+    if pc == "exact":
+        assert eager[3] == 6
+        A, B = p.to_scipy()
+        X = graph[1]
+        for j in range(6):
+            assert np.linalg.norm(A @ X[:, j] - graph[0][j] * (B @ X[:, j])) <= 1e-8
+
+
 def test_correction_gmres_entry_matches_oracle():
     """bbjd_correction_gmres (reference include/bbjd.h:268-282 with the
     argument order of src/jd_core.c:568-582) on host arrays vs the oracle's
