@@ -214,6 +214,150 @@ __global__ __launch_bounds__(TPB) void k_bcr_bwd(int s, i64 M, i64 h, i64 no, co
     bcr_bwd_row<NR, S>(s, M, h, j, (int)(gid % sv), DPQ, b, x, vs);
 }
 
+// G backward levels l + G - 1 .. l in one launch.  Block rows at multiples
+// of H = 2^(l+G) come from deeper levels; each block owns U units of H
+// block rows, holds its units' rows in LDS (slot = row offset / 2^l from
+// the first unit's start, slot U 2^G the next block's first boundary) and
+// solves them one level per phase, top level first: phase g solves the odd
+// rows of level l + g, whose neighbours at +- 2^(l+g) are boundaries or rows
+// of an earlier phase.  Each row's arithmetic is bcr_bwd_row's, in the same
+// order.  U fills the last phase's TPB threads (2^(G-1) rows per unit).
+struct BcrLevels { const Cx<double> *f[4]; };
+
+template <int NR, int S, int G>
+__global__ __launch_bounds__(TPB) void k_bcr_bwd_multi(i64 N, int l, BcrLevels dpq, const Cx<double> *b,
+                                                       Cx<double> *x, i64 vs)
+{
+    constexpr int U = TPB / ((1 << (G - 1)) * S);
+    constexpr int SLOTS = U * (1 << G) + 1;
+    constexpr i64 ss = (i64)S * S;
+    __shared__ Cx<double> lx[NR][SLOTS][S];
+    const int t = threadIdx.x;
+    const i64 u0 = (i64)blockIdx.x * U;
+    for (int e = t; e < (U + 1) * S * NR; e += TPB) {
+        const int q = e / ((U + 1) * S), rem = e % ((U + 1) * S), u = rem / S, r = rem % S;
+        const i64 i = ((u0 + u) << G) << l;
+        lx[q][u << G][r] = i < N ? x[q * vs + i * S + r] : cx0();
+    }
+    __syncthreads();
+#pragma unroll
+    for (int g = G - 1; g >= 0; g--) {
+        const int R = 1 << (G - 1 - g);
+        const int lv = l + g;
+        const i64 hg = (i64)1 << lv, M = (N + hg - 1) >> lv;
+        if (t < U * R * S) {
+            const int rr = t / S, r = t % S, u = rr / R, k = rr % R;
+            const int slot = (u << G) + ((2 * k + 1) << g);
+            const i64 m = ((u0 + u) << (G - g)) + 2 * k + 1;
+            if (m < M) {
+                const i64 i = m * hg, j = (m - 1) / 2;
+                const Cx<double> *D = dpq.f[g] + j * 3 * ss + r, *P = D + ss, *Qm = P + ss;
+                Cx<double> acc[NR];
+#pragma unroll
+                for (int q = 0; q < NR; q++) acc[q] = cx0();
+                const i64 ob = i * S;
+#pragma unroll
+                for (int c = 0; c < S; c++) {
+                    const Cx<double> dd = D[(i64)c * S], pp = P[(i64)c * S];
+#pragma unroll
+                    for (int q = 0; q < NR; q++) {
+                        cfma(acc[q], dd, b[q * vs + ob + c]);
+                        cfms(acc[q], pp, lx[q][slot - (1 << g)][c]);
+                    }
+                }
+                if (m + 1 < M) {
+#pragma unroll
+                    for (int c = 0; c < S; c++) {
+                        const Cx<double> qq = Qm[(i64)c * S];
+#pragma unroll
+                        for (int q = 0; q < NR; q++) cfms(acc[q], qq, lx[q][slot + (1 << g)][c]);
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < NR; q++) {
+                    x[q * vs + ob + r] = acc[q];
+                    lx[q][slot][r] = acc[q];
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Forward levels 0 .. G-1 in one launch, reading the caller's right-hand
+// sides (n1 rows; padding rows read as zero), which nothing writes: block
+// rows are grouped in units of 2^G; each block reduces its U units plus the
+// unit to their left (rows 1 .. 2^G - 1 of it reach its first unit within
+// G levels) in LDS, in place, level by level, and writes its own units' rows
+// to b -- every row of them final for the levels below G (odd rows of level
+// g keep their value after level g - 1; the unit boundaries carry level
+// G - 1's update).  Each row's arithmetic is k_bcr_fwd0's and bcr_fwd_row's.
+// U + 1 units make 2 TPB / S rows, so level 0's TPB / S - 1 even rows fill
+// the block.
+template <int NR, int S, int G>
+__global__ __launch_bounds__(TPB) void k_bcr_fwd_multi0(i64 N, i64 n1, BcrLevels ag,
+                                                        const Cx<double> *__restrict__ x0,
+                                                        const Cx<double> *__restrict__ x1, Cx<double> *b, i64 vs)
+{
+    constexpr int U = TPB / ((1 << (G - 1)) * S) - 1;
+    constexpr int ROWS = (U + 1) << G;          // halo unit + own units: 2 TPB / S
+    constexpr i64 ss = (i64)S * S;
+    __shared__ Cx<double> lb[NR][ROWS][S];
+    const int t = threadIdx.x;
+    const i64 p0 = ((i64)blockIdx.x * U - 1) << G;   // block row of slot 0 (the halo unit)
+    const Cx<double> *xs[2] = {x0, x1};
+    for (int e = t; e < ROWS * S * NR; e += TPB) {
+        const int q = e / (ROWS * S), rem = e % (ROWS * S), sl = rem / S, r = rem % S;
+        const i64 i = p0 + sl, el = i * S + r;
+        lb[q][sl][r] = (i >= 0 && el < n1) ? xs[q][el] : cx0();
+    }
+    __syncthreads();
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+        const i64 h = (i64)1 << g, M = (N + h - 1) >> g;
+        // even rows of level g: slots that are multiples of 2^(g+1), except
+        // slot 0 (no row of the block needs it); within a level, rows read
+        // only their own value and odd neighbours, which the level leaves
+        const int nrow = (ROWS >> (g + 1)) - 1;
+        for (int e = t; e < nrow * S; e += TPB) {
+            const int sl = (e / S + 1) << (g + 1), r = e % S;
+            const i64 i = p0 + sl;
+            if (i >= 0 && i < N) {
+                const i64 m = i >> g, j = m / 2;
+                const Cx<double> *al = ag.f[g] + j * 2 * ss + r;
+                Cx<double> acc[NR];
+#pragma unroll
+                for (int q = 0; q < NR; q++) acc[q] = lb[q][sl][r];
+                if (m >= 1) {
+#pragma unroll
+                    for (int c = 0; c < S; c++) {
+                        const Cx<double> a = al[(i64)c * S];
+#pragma unroll
+                        for (int q = 0; q < NR; q++) cfms(acc[q], a, lb[q][sl - h][c]);
+                    }
+                }
+                if (m + 1 < M) {
+#pragma unroll
+                    for (int c = 0; c < S; c++) {
+                        const Cx<double> gm = al[ss + (i64)c * S];
+#pragma unroll
+                        for (int q = 0; q < NR; q++) cfms(acc[q], gm, lb[q][sl + h][c]);
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < NR; q++) lb[q][sl][r] = acc[q];
+            }
+        }
+        __syncthreads();
+    }
+    for (int e = t; e < (ROWS - (1 << G)) * S * NR; e += TPB) {
+        const int q = e / ((ROWS - (1 << G)) * S), rem = e % ((ROWS - (1 << G)) * S);
+        const int sl = (1 << G) + rem / S, r = rem % S;
+        const i64 i = p0 + sl;
+        if (i < N) b[q * vs + i * S + r] = lb[q][sl][r];
+    }
+}
+
 // The small levels l0..L-1, the final s x s solve and the backward levels
 // L-1..l0 in one workgroup of 1024 threads (every level here has at most
 // 1024 rows of work), separated by workgroup barriers.  off[2l] / off[2l+1]:
@@ -543,6 +687,61 @@ int bbjd_o_bcr_bwd(hipStream_t st, int s, int64_t M, int64_t h, int64_t no, cons
                        (const Cx<double> *)DPQ, (const Cx<double> *)b, (Cx<double> *)x, (i64)vs)
     BBJD_BCR_DISPATCH(BBJD_BWD)
 #undef BBJD_BWD
+    return chk();
+}
+
+// Fused level groups: s in {4, 8, 16}, G in {2, 3, 4}; -3 for other shapes
+// (the caller then launches level by level).
+#define BBJD_BCR_GROUP_DISPATCH(CALL)                                                    \
+    switch (s * 16 + G * 2 + (nrhs == 2)) {                                              \
+    case 4 * 16 + 4: CALL(1, 4, 2); break;   case 4 * 16 + 5: CALL(2, 4, 2); break;      \
+    case 4 * 16 + 6: CALL(1, 4, 3); break;   case 4 * 16 + 7: CALL(2, 4, 3); break;      \
+    case 4 * 16 + 8: CALL(1, 4, 4); break;   case 4 * 16 + 9: CALL(2, 4, 4); break;      \
+    case 8 * 16 + 4: CALL(1, 8, 2); break;   case 8 * 16 + 5: CALL(2, 8, 2); break;      \
+    case 8 * 16 + 6: CALL(1, 8, 3); break;   case 8 * 16 + 7: CALL(2, 8, 3); break;      \
+    case 8 * 16 + 8: CALL(1, 8, 4); break;   case 8 * 16 + 9: CALL(2, 8, 4); break;      \
+    case 16 * 16 + 4: CALL(1, 16, 2); break; case 16 * 16 + 5: CALL(2, 16, 2); break;    \
+    case 16 * 16 + 6: CALL(1, 16, 3); break; case 16 * 16 + 7: CALL(2, 16, 3); break;    \
+    case 16 * 16 + 8: CALL(1, 16, 4); break; case 16 * 16 + 9: CALL(2, 16, 4); break;    \
+    default: return -3;                                                                  \
+    }
+
+static inline bool bcr_group_ok(int s, int G) { return (s == 4 || s == 8 || s == 16) && G >= 2 && G <= 4; }
+
+int bbjd_o_bcr_bwd_multi(hipStream_t st, int s, int64_t N, int l, int G, const void *const *dpq, const void *b,
+                         void *x, int nrhs, int64_t vs)
+{
+    if (!bcr_group_ok(s, G) || (nrhs != 1 && nrhs != 2)) return -3;
+    BcrLevels f;
+    for (int g = 0; g < 4; g++) f.f[g] = (const Cx<double> *)(g < G ? dpq[g] : nullptr);
+    const i64 H = (i64)1 << (l + G), units = (N + H - 1) / H;
+#define BBJD_BWDM(NR, S, GG)                                                                                   \
+    {                                                                                                          \
+        constexpr int U = TPB / ((1 << (GG - 1)) * S);                                                         \
+        hipLaunchKernelGGL((k_bcr_bwd_multi<NR, S, GG>), dim3((unsigned)((units + U - 1) / U)), dim3(TPB), 0, st, \
+                           (i64)N, l, f, (const Cx<double> *)b, (Cx<double> *)x, (i64)vs);                     \
+    }
+    BBJD_BCR_GROUP_DISPATCH(BBJD_BWDM)
+#undef BBJD_BWDM
+    return chk();
+}
+
+int bbjd_o_bcr_fwd_multi0(hipStream_t st, int s, int64_t N, int64_t n1, int G, const void *const *ag,
+                          const void *x0, const void *x1, void *b, int nrhs, int64_t vs)
+{
+    if (!bcr_group_ok(s, G) || (nrhs != 1 && nrhs != 2) || (nrhs == 2 && !x1)) return -3;
+    BcrLevels f;
+    for (int g = 0; g < 4; g++) f.f[g] = (const Cx<double> *)(g < G ? ag[g] : nullptr);
+    const i64 units = (N + ((i64)1 << G) - 1) >> G;
+#define BBJD_FWDM(NR, S, GG)                                                                                   \
+    {                                                                                                          \
+        constexpr int U = TPB / ((1 << (GG - 1)) * S) - 1;                                                     \
+        hipLaunchKernelGGL((k_bcr_fwd_multi0<NR, S, GG>), dim3((unsigned)((units + U - 1) / U)), dim3(TPB), 0,  \
+                           st, (i64)N, (i64)n1, f, (const Cx<double> *)x0, (const Cx<double> *)(x1 ? x1 : x0),   \
+                           (Cx<double> *)b, (i64)vs);                                                          \
+    }
+    BBJD_BCR_GROUP_DISPATCH(BBJD_FWDM)
+#undef BBJD_FWDM
     return chk();
 }
 

@@ -247,12 +247,18 @@ static hipStream_t op_stream(const struct bbjd_op *op)
     return s ? s : op->stream;
 }
 
-/* BCR solve of nrhs right-hand sides into op->xvec: the large levels one
- * launch each, the levels from op->l0 on, the final solve and their backward
- * levels in one workgroup.  With x0 (x1) the right-hand sides are the
- * caller's n1-row vectors, read by the first level (or copied into op->bvec
- * when every level is in the tail); without, they are already in op->bvec
- * (vector q at q N s). */
+/* Levels per fused BCR launch (1: one launch per level).  The backward
+ * levels go in groups from the top; the first forward levels, which read
+ * the caller's vectors, in one group. */
+static int g_bcr_group = 4;
+void bbjd_set_bcr_group(int levels) { g_bcr_group = levels < 1 ? 1 : (levels > 4 ? 4 : levels); }
+
+/* BCR solve of nrhs right-hand sides into op->xvec: the large levels in
+ * launches of up to g_bcr_group levels, the levels from the tail level on,
+ * the final solve and their backward levels in one workgroup.  With x0 (x1)
+ * the right-hand sides are the caller's n1-row vectors, read by the first
+ * levels (or copied into op->bvec when every level is in the tail);
+ * without, they are already in op->bvec (vector q at q N s). */
 static int bcr_solve(struct bbjd_op *op, hipStream_t st, int nrhs, const void *x0, const void *x1)
 {
     const int s = op->s;
@@ -260,11 +266,22 @@ static int bcr_solve(struct bbjd_op *op, hipStream_t st, int nrhs, const void *x
     const int64_t vs = (int64_t)op->N * s;
     int rc = 0;
     const int lt = bbjd_o_bcr_tail_level(s, op->N, op->l0, op->L, nrhs);
+    const void *lv[4];
     int l = 0;
     if (x0 && lt >= 1 && s <= 64) {
-        rc |= bbjd_o_bcr_fwd0(st, s, op->N, op->n1, (const zcx *)op->bcr + (size_t)op->even_off[0] * ss, x0, x1,
-                              op->bvec, nrhs, vs);
-        l = 1;
+        const int G = lt < g_bcr_group ? lt : g_bcr_group;
+        int e = -3;
+        if (G >= 2) {
+            for (int g = 0; g < G; g++) lv[g] = (const zcx *)op->bcr + (size_t)op->even_off[g] * ss;
+            e = bbjd_o_bcr_fwd_multi0(st, s, op->N, op->n1, G, lv, x0, x1, op->bvec, nrhs, vs);
+            if (e == 0) l = G;
+            else if (e != -3) rc |= e;
+        }
+        if (e == -3) {
+            rc |= bbjd_o_bcr_fwd0(st, s, op->N, op->n1, (const zcx *)op->bcr + (size_t)op->even_off[0] * ss, x0,
+                                  x1, op->bvec, nrhs, vs);
+            l = 1;
+        }
     } else if (x0) {
         rc |= !HIPOK(hipMemcpyAsync(op->bvec, x0, (size_t)op->n1 * 16, hipMemcpyDeviceToDevice, st));
         if (nrhs == 2)
@@ -277,10 +294,19 @@ static int bcr_solve(struct bbjd_op *op, hipStream_t st, int nrhs, const void *x
     }
     rc |= bbjd_o_bcr_tail(st, s, op->N, lt, op->L, (const int64_t *)op->doff, op->bcr, op->bvec, op->xvec, nrhs,
                           vs);
-    for (int l = lt - 1; l >= 0; l--) {
+    for (int l = lt - 1; l >= 0;) {
+        const int G = l + 1 < g_bcr_group ? l + 1 : g_bcr_group;
+        if (G >= 2) {
+            const int lo = l - G + 1;
+            for (int g = 0; g < G; g++) lv[g] = (const zcx *)op->bcr + (size_t)op->odd_off[lo + g] * ss;
+            const int e = bbjd_o_bcr_bwd_multi(st, s, op->N, lo, G, lv, op->bvec, op->xvec, nrhs, vs);
+            if (e == 0) { l = lo - 1; continue; }
+            if (e != -3) { rc |= e; break; }
+        }
         const bbjd_int h = (bbjd_int)1 << l, M = (op->N + h - 1) >> l;
         rc |= bbjd_o_bcr_bwd(st, s, M, h, M / 2, (const zcx *)op->bcr + (size_t)op->odd_off[l] * ss, op->bvec,
                              op->xvec, nrhs, vs);
+        l--;
     }
     return rc;
 }

@@ -85,6 +85,10 @@ int bbjd_op_matvec_A_c64(bbjd_int n, const bbjd_cfloat *x, bbjd_cfloat *y, void 
 int bbjd_op_matvec_B_c64(bbjd_int n, const bbjd_cfloat *x, bbjd_cfloat *y, void *op);
 int bbjd_op_precond_c64 (bbjd_int n, const bbjd_cfloat *x, bbjd_cfloat *y, void *op);
 
+/* Block cyclic reduction levels per launch of the exact solve (1..4, default
+ * 4; 1: one launch per level).  Process-wide; tuning and tests. */
+void bbjd_set_bcr_group(int levels);
+
 #ifdef __cplusplus
 }
 #endif

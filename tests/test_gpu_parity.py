@@ -198,6 +198,39 @@ def test_device_operators_match_oracle(n, bs, hw, ntau):
         dev.free()
 
 
+@pytest.mark.parametrize("n,bs,hw,ntau", [(60000, 8, 1, 4), (40000, 8, 2, 16), (30004, 4, 1, 2)])
+def test_bcr_level_groups_match_level_by_level(n, bs, hw, ntau):
+    """the exact solve with 2, 3 or 4 block-cyclic-reduction levels per
+    launch (bbjd_set_bcr_group; super-blocks of 8, 16 and 4 rows, the last
+    one ragged) equals the one-launch-per-level solve exactly -- the same
+    row arithmetic in the same order -- for one and two right-hand sides,
+    and solves M y = x as the oracle's banded LU does"""
+    lib = _lib.load()
+    p = BandedTauPencil.generate(n, bs, hw, ntau, seed=21)
+    sigma = 0.3 - 0.2j
+    dev = DeviceOperator(p, sigma, pc="exact")
+    rng = np.random.default_rng(8)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    x2 = rng.standard_normal(n) - 1j * rng.standard_normal(n)
+    try:
+        lib.bbjd_set_bcr_group(1)
+        y1 = dev.solve(x)
+        z0, z1 = dev.solve2(x, x2)
+        for g in (2, 3, 4):
+            lib.bbjd_set_bcr_group(g)
+            np.testing.assert_array_equal(dev.solve(x), y1)
+            a, b = dev.solve2(x, x2)
+            np.testing.assert_array_equal(a, z0)
+            np.testing.assert_array_equal(b, z1)
+    finally:
+        lib.bbjd_set_bcr_group(4)
+        dev.free()
+    cpu = orc.CpuPencilOps(p, sigma, 1)
+    r1, r2 = cpu.apply("orc_precond", x), cpu.apply("orc_precond", x2)
+    assert np.linalg.norm(y1 - r1) <= 1e-9 * np.linalg.norm(r1)
+    assert np.linalg.norm(z1 - r2) <= 1e-9 * np.linalg.norm(r2)
+
+
 def _cfg1(n=2000, k=8, seed=1):
     return BandedTauPencil.generate(n, 4, 1, 2, seed=seed), 0.5 + 0.5j
 
