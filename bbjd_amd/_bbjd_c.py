@@ -87,10 +87,12 @@ def jdqz_c(n, nev, sigma, target, matvec_A, matvec_B, precond_solve, tol=1e-10, 
         p.precond = _lib.fn_addr("bbjd_op_precond") if precond_solve is not None else None
         if precond_solve is not None:
             opts.precond2 = _lib.fn_addr("bbjd_op_precond2")
+        opts.matvec_AB = _lib.fn_addr("bbjd_op_matvec_AB")
         p.op_data = device_op.handle
         if precision == "mixed" and device_op.c64:
             opts.matvec_A_c64 = _lib.fn_addr("bbjd_op_matvec_A_c64")
             opts.matvec_B_c64 = _lib.fn_addr("bbjd_op_matvec_B_c64")
+            opts.matvec_AB_c64 = _lib.fn_addr("bbjd_op_matvec_AB_c64")
             if precond_solve is not None:
                 opts.precond_c64 = _lib.fn_addr("bbjd_op_precond_c64")
     else:

@@ -469,6 +469,23 @@ static int apply_op(ctx_t *c, bbjd_matvec_fn f, const void *x, void *y)
     return 0;
 }
 
+/* ya = A x and yb = B x: one call of the fused callback when there is one
+ * (device callbacks), else the two matvecs */
+static int apply_ab(ctx_t *c, const void *x, void *ya, void *yb)
+{
+    struct bbjd_dev *d = c->d;
+    if (d->opt.matvec_AB && d->opt.callback_space == BBJD_CALLBACK_DEVICE) {
+        hipStream_t prev = tl_cb_stream;
+        tl_cb_stream = c->s;
+        int rc = d->opt.matvec_AB(c->n, (const bbjd_zomplex *)x, (bbjd_zomplex *)ya, (bbjd_zomplex *)yb,
+                                  c->P->op_data);
+        tl_cb_stream = prev;
+        return rc;
+    }
+    int rc = apply_op(c, c->P->matvec_A, x, ya);
+    return rc ? rc : apply_op(c, c->P->matvec_B, x, yb);
+}
+
 static int apply_pc(ctx_t *c, const void *x, void *y)
 {
     if (c->P->precond) return apply_op(c, (bbjd_matvec_fn)c->P->precond, x, y);
@@ -676,8 +693,7 @@ static int gmres(ctx_t *c, bbjd_int ndefl, const void *QR, const void *QL, zcx t
             if (!mixed) {
                 rc |= !HIPOK(hipMemcpyAsync(t1, vj, (size_t)n * 16, hipMemcpyDeviceToDevice, c->s));
                 rc |= bbjd_k_project_out(c->s, n, (int)ndefl, QR, c->ld, t1, &d->rd);
-                if (!rc && apply_op(c, P->matvec_A, t1, t2)) cb = 1;
-                if (!rc && !cb && apply_op(c, P->matvec_B, t1, vj1)) cb = 1;
+                if (!rc && apply_ab(c, t1, t2, vj1)) cb = 1;
                 rc |= bbjd_k_lincomb_dev(c->s, n, d->gco, t2, vj1, t2);
                 rc |= bbjd_k_project_out(c->s, n, (int)ndefl, QL, c->ld, t2, &d->rd);
                 if (!rc && !cb && apply_pc(c, t2, vj1)) cb = 1;
@@ -689,8 +705,17 @@ static int gmres(ctx_t *c, bbjd_int ndefl, const void *QR, const void *QL, zcx t
                 void *a = d->tc[0], *bq = d->tc[1], *e = d->tc[2];
                 rc |= !HIPOK(hipMemcpyAsync(a, vj, (size_t)n * 8, hipMemcpyDeviceToDevice, c->s));
                 rc |= bbjd_k_project_out_c(c->s, n, (int)ndefl, qr, c->ld, a, &d->rd);
-                if (!rc && apply_c64(c, d->opt.matvec_A_c64, P->matvec_A, a, bq, t1, t2)) cb = 1;
-                if (!rc && !cb && apply_c64(c, d->opt.matvec_B_c64, P->matvec_B, a, e, t1, t2)) cb = 1;
+                if (d->opt.matvec_AB_c64 && d->opt.callback_space == BBJD_CALLBACK_DEVICE) {
+                    hipStream_t prev = tl_cb_stream;
+                    tl_cb_stream = c->s;
+                    if (!rc && d->opt.matvec_AB_c64(n, (const bbjd_cfloat *)a, (bbjd_cfloat *)bq, (bbjd_cfloat *)e,
+                                                    P->op_data))
+                        cb = 1;
+                    tl_cb_stream = prev;
+                } else {
+                    if (!rc && apply_c64(c, d->opt.matvec_A_c64, P->matvec_A, a, bq, t1, t2)) cb = 1;
+                    if (!rc && !cb && apply_c64(c, d->opt.matvec_B_c64, P->matvec_B, a, e, t1, t2)) cb = 1;
+                }
                 rc |= bbjd_k_lincomb_dev_c(c->s, n, d->gco, bq, e, bq);
                 rc |= bbjd_k_project_out_c(c->s, n, (int)ndefl, ql, c->ld, bq, &d->rd);
                 if (P->precond) {
@@ -967,7 +992,7 @@ static int expand(ctx_t *c, bbjd_int k, const void *z, zcx nu0, zcx mu0)
     if (optimistic) {
         rc |= bbjd_k_cgs2_unit(c->s, n, (int)k, ws->V, c->ld, vn, &d->rd, SL_V);
         if (rc) return -2;
-        if (apply_op(c, P->matvec_A, vn, an) || apply_op(c, P->matvec_B, vn, bn)) return -1;
+        if (apply_ab(c, vn, an, bn)) return -1;
         /* fused: w = nu0 Av + mu0 Bv is formed inside the first passes over W,
          * which also give the pencil column */
         zcx *pen = (zcx *)d->pencil;
@@ -1006,7 +1031,7 @@ static int expand(ctx_t *c, bbjd_int k, const void *z, zcx nu0, zcx mu0)
     }
     rc |= bbjd_k_scale(c->s, n, vn, 1.0 / nrm, 0.0);
     if (rc) return -2;
-    if (apply_op(c, P->matvec_A, vn, an) || apply_op(c, P->matvec_B, vn, bn)) return -1;
+    if (apply_ab(c, vn, an, bn)) return -1;
     rc |= bbjd_k_lincomb(c->s, n, creal(nu0), cimag(nu0), an, creal(mu0), cimag(mu0), bn, wn);
     rc |= bbjd_k_cgs2(c->s, n, (int)k, ws->W, c->ld, wn, &d->rd, SL_W, NULL);
     if (rc || sync_scal(c)) return -2;
@@ -1087,7 +1112,7 @@ static bbjd_status solve_impl(const bbjd_params *P, bbjd_workspace *ws, bbjd_res
     if (rc) { st = BBJD_ERR_ALLOC; goto done; }
     if (d->hscal[SL_T0] < 1e-300) { st = BBJD_ERR_BREAKDOWN; goto done; }
     rc |= bbjd_k_scale(c->s, n, ws->V, 1.0 / d->hscal[SL_T0], 0.0);
-    if (apply_op(c, P->matvec_A, ws->V, ws->AV) || apply_op(c, P->matvec_B, ws->V, ws->BV)) goto cb_err;
+    if (apply_ab(c, ws->V, ws->AV, ws->BV)) goto cb_err;
     rc |= bbjd_k_lincomb_norm(c->s, n, creal(nu0), cimag(nu0), ws->AV, creal(mu0), cimag(mu0), ws->BV, ws->W,
                               &d->rd, SL_W);
     rc |= bbjd_k_scale_inv(c->s, n, ws->W, d->rd.scal, SL_W, 0);
@@ -1146,7 +1171,7 @@ static bbjd_status solve_impl(const bbjd_params *P, bbjd_workspace *ws, bbjd_res
             rc |= bbjd_k_gemv_n(c->s, n, (int)k, ws->V, c->ld, d->rd.coef, u, &d->rd, -1);
         }
         /* ---- step 3: residual ---- */
-        if (apply_op(c, P->matvec_A, u, Au) || apply_op(c, P->matvec_B, u, Bu)) goto cb_err;
+        if (apply_ab(c, u, Au, Bu)) goto cb_err;
         rc |= bbjd_k_lincomb_norm(c->s, n, creal(tb), cimag(tb), Au, -creal(ta), -cimag(ta), Bu, r, &d->rd, SL_R);
         if (rc || sync_scal(c)) { st = BBJD_ERR_ALLOC; goto done; }
         d->st.stream_bytes += 16.0 * n * (k + 1 + 3);

@@ -56,6 +56,90 @@ __global__ __launch_bounds__(TPB) void k_diag_mv(i64 n, const Cx<R> *__restrict_
     }
 }
 
+// y_A = A x and y_B = B x in one pass over x (the block-banded rows i < n1,
+// the diagonal of B on every row, and per-block partial dots of the ntau
+// dense rows, conj(tauc[:, c]) . x, finished by k_tau_finish).  The band
+// rows and the diagonal are k_band_mv's and k_diag_mv's arithmetic.
+template <typename R>
+__device__ __forceinline__ Cx<R> wave_sum(Cx<R> v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        v.re += __shfl_xor(v.re, o, 64);
+        v.im += __shfl_xor(v.im, o, 64);
+    }
+    return v;
+}
+
+static constexpr int AB_MAX_TAU = 16;
+
+template <typename R>
+__global__ __launch_bounds__(TPB) void k_band_ab(i64 n, i64 n1, i64 nb, int bs, int hw,
+                                                 const Cx<R> *__restrict__ band, const Cx<R> *__restrict__ dg,
+                                                 int ntau, const Cx<R> *__restrict__ tauc,
+                                                 const Cx<R> *__restrict__ x, Cx<R> *__restrict__ ya,
+                                                 Cx<R> *__restrict__ yb, Cx<R> *__restrict__ part)
+{
+    __shared__ Cx<R> red[AB_MAX_TAU][TPB / 64];
+    const i64 i = (i64)blockIdx.x * TPB + threadIdx.x;
+    Cx<R> xi = {0, 0};
+    if (i < n) {
+        xi = x[i];
+        if (i < n1) {
+            const i64 bi = i / bs;
+            const int r = (int)(i % bs), nd = 2 * hw + 1;
+            Cx<R> acc = {0, 0};
+            for (int d = 0; d < nd; d++) {
+                const i64 bj = bi + d - hw;
+                if (bj < 0 || bj >= nb) continue;
+                const Cx<R> *blk = band + (bi * nd + d) * bs * bs + r;
+                const Cx<R> *xs = x + bj * bs;
+                for (int c = 0; c < bs; c++) cfma(acc, blk[(i64)c * bs], xs[c]);
+            }
+            ya[i] = acc;
+        }
+        Cx<R> b = {0, 0};
+        cfma(b, dg[i], xi);
+        yb[i] = b;
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int c = 0; c < ntau; c++) {
+        Cx<R> p = {0, 0};
+        if (i < n) {
+            const Cx<R> q = tauc[i + (i64)c * n];
+            // conj(q) x
+            p.re = q.re * xi.re + q.im * xi.im;
+            p.im = q.re * xi.im - q.im * xi.re;
+        }
+        p = wave_sum(p);
+        if (lane == 0) red[c][w] = p;
+    }
+    __syncthreads();
+    if (threadIdx.x < ntau) {
+        Cx<R> s = red[threadIdx.x][0];
+#pragma unroll
+        for (int v = 1; v < TPB / 64; v++) { s.re += red[threadIdx.x][v].re; s.im += red[threadIdx.x][v].im; }
+        part[(i64)blockIdx.x * ntau + threadIdx.x] = s;
+    }
+}
+
+// y_A's tau rows: wave c sums column c of the nblk block partials in a
+// fixed order
+template <typename R>
+__global__ __launch_bounds__(64 * AB_MAX_TAU) void k_tau_finish(i64 nblk, int ntau, const Cx<R> *__restrict__ part,
+                                                                Cx<R> *__restrict__ out)
+{
+    const int lane = threadIdx.x & 63, c = threadIdx.x >> 6;
+    if (c >= ntau) return;
+    Cx<R> s = {0, 0};
+    for (i64 b = lane; b < nblk; b += 64) {
+        const Cx<R> v = part[b * ntau + c];
+        s.re += v.re; s.im += v.im;
+    }
+    s = wave_sum(s);
+    if (lane == 0) out[c] = s;
+}
+
 template <typename R>
 __global__ __launch_bounds__(TPB) void k_bj_apply(i64 n, int bs, const Cx<R> *__restrict__ inv,
                                                   const Cx<R> *__restrict__ x, Cx<R> *__restrict__ y)
@@ -608,6 +692,19 @@ int bbjd_o_band_mv(hipStream_t s, int64_t n1, int64_t nb, int bs, int hw, const 
     else
         hipLaunchKernelGGL((k_band_mv<double>), dim3(blocks_for(n1)), dim3(TPB), 0, s, (i64)n1, (i64)nb, bs, hw,
                            (const Cx<double> *)band, (const Cx<double> *)x, (Cx<double> *)y);
+    return chk();
+}
+
+int bbjd_o_band_ab(hipStream_t s, int64_t n, int64_t n1, int64_t nb, int bs, int hw, const void *band,
+                   const void *dg, int ntau, const void *tauc, const void *x, void *ya, void *yb, void *part,
+                   int64_t part_cap, int single)
+{
+    if (n <= 0) return 0;
+    const i64 nblk = blocks_for(n);
+    if (ntau < 0 || ntau > AB_MAX_TAU || (ntau > 0 && nblk * ntau > part_cap)) return -3;
+#define BBJD_AB(R)                                                                                                  hipLaunchKernelGGL((k_band_ab<R>), dim3(nblk), dim3(TPB), 0, s, (i64)n, (i64)n1, (i64)nb, bs, hw,                                    (const Cx<R> *)band, (const Cx<R> *)dg, ntau, (const Cx<R> *)tauc, (const Cx<R> *)x,                              (Cx<R> *)ya, (Cx<R> *)yb, (Cx<R> *)part);                                                     if (ntau > 0)                                                                                                       hipLaunchKernelGGL((k_tau_finish<R>), dim3(1), dim3(64 * ntau), 0, s, (i64)nblk, ntau,                                              (const Cx<R> *)part, (Cx<R> *)ya + n1);
+    if (single) { BBJD_AB(float) } else { BBJD_AB(double) }
+#undef BBJD_AB
     return chk();
 }
 

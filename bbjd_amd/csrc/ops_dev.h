@@ -8,6 +8,12 @@ extern "C" {
 #endif
 int bbjd_o_band_mv(hipStream_t s, int64_t n1, int64_t nb, int bs, int hw, const void *band, const void *x, void *y,
                    int single);
+/* y_A = A x (band rows + the ntau dense rows from per-block partials in
+ * part, nblk x ntau) and y_B = diag(dg) x in two launches; -3 when ntau > 16
+ * or part is too small */
+int bbjd_o_band_ab(hipStream_t s, int64_t n, int64_t n1, int64_t nb, int bs, int hw, const void *band,
+                   const void *dg, int ntau, const void *tauc, const void *x, void *ya, void *yb, void *part,
+                   int64_t part_cap, int single);
 int bbjd_o_diag_mv(hipStream_t s, int64_t n, const void *dg, const void *x, void *y, int single);
 int bbjd_o_bj_apply(hipStream_t s, int64_t n, int bs, const void *inv, const void *x, void *y, int single);
 int bbjd_o_bcr_fwd(hipStream_t st, int s, int64_t M, int64_t h, int64_t ne, const void *AG, void *b, int nrhs,

@@ -589,6 +589,18 @@ int bbjd_op_matvec_A(bbjd_int n, const bbjd_zomplex *x, bbjd_zomplex *y, void *d
     return rc ? -1 : 0;
 }
 
+int bbjd_op_matvec_AB(bbjd_int n, const bbjd_zomplex *x, bbjd_zomplex *ya, bbjd_zomplex *yb, void *data)
+{
+    struct bbjd_op *op = (struct bbjd_op *)data;
+    if (!op || n != op->n) return -1;
+    hipStream_t st = op_stream(op);
+    const int e = bbjd_o_band_ab(st, n, op->n1, op->nb, op->bs, op->hw, op->band, op->bdiag, op->ntau, op->tauc, x,
+                                 ya, yb, op->rd.part, op->rd.part_cap, 0);
+    if (e != -3) return e ? -1 : 0;
+    if (bbjd_op_matvec_A(n, x, ya, data)) return -1;
+    return bbjd_op_matvec_B(n, x, yb, data);
+}
+
 int bbjd_op_matvec_B(bbjd_int n, const bbjd_zomplex *x, bbjd_zomplex *y, void *data)
 {
     struct bbjd_op *op = (struct bbjd_op *)data;
@@ -627,6 +639,18 @@ int bbjd_op_matvec_A_c64(bbjd_int n, const bbjd_cfloat *x, bbjd_cfloat *y, void 
     if (op->ntau > 0)
         rc |= bbjd_k_gemv_c_c(st, n, op->ntau, op->tauc_c, n, x, NULL, (c64 *)y + op->n1, NULL, &op->rd);
     return rc ? -1 : 0;
+}
+
+int bbjd_op_matvec_AB_c64(bbjd_int n, const bbjd_cfloat *x, bbjd_cfloat *ya, bbjd_cfloat *yb, void *data)
+{
+    struct bbjd_op *op = (struct bbjd_op *)data;
+    if (!op || n != op->n || !op->want_c64) return -1;
+    hipStream_t st = op_stream(op);
+    const int e = bbjd_o_band_ab(st, n, op->n1, op->nb, op->bs, op->hw, op->band_c, op->bdiag_c, op->ntau,
+                                 op->tauc_c, x, ya, yb, op->rd.part, op->rd.part_cap, 1);
+    if (e != -3) return e ? -1 : 0;
+    if (bbjd_op_matvec_A_c64(n, x, ya, data)) return -1;
+    return bbjd_op_matvec_B_c64(n, x, yb, data);
 }
 
 int bbjd_op_matvec_B_c64(bbjd_int n, const bbjd_cfloat *x, bbjd_cfloat *y, void *data)

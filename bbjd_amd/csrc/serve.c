@@ -202,11 +202,13 @@ int bbjd_serve(bbjd_request *reqs, int nreq, const bbjd_serve_config *cfg, bbjd_
     S.opt.callback_space = BBJD_CALLBACK_DEVICE;
     S.opt.precision = cfg->precision;
     S.opt.precond2 = bbjd_op_precond2;
+    S.opt.matvec_AB = bbjd_op_matvec_AB;
     S.opt.graphs = cfg->graphs;
     if (cfg->precision == BBJD_PREC_MIXED) {
         S.opt.matvec_A_c64 = bbjd_op_matvec_A_c64;
         S.opt.matvec_B_c64 = bbjd_op_matvec_B_c64;
         S.opt.precond_c64 = bbjd_op_precond_c64;
+        S.opt.matvec_AB_c64 = bbjd_op_matvec_AB_c64;
     }
     const size_t wsb = bbjd_workspace_bytes(&S.cap, &S.opt);
     size_t freeb = 0, totb = 0;

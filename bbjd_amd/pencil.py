@@ -159,6 +159,21 @@ class DeviceOperator:
     def solve(self, x):
         return self._apply("bbjd_op_precond", x)
 
+    def matvec_AB(self, x, single=False):
+        """(A x, B x) in one pass over x (bbjd_op_matvec_AB; single: the
+        complex64 form, bbjd_op_matvec_AB_c64)"""
+        lib = _lib.load()
+        dt = np.complex64 if single else np.complex128
+        x = np.ascontiguousarray(x, dtype=dt)
+        dx = _lib.DeviceBuffer.from_array(x)
+        da, db = _lib.DeviceBuffer(x.nbytes), _lib.DeviceBuffer(x.nbytes)
+        fn = lib.bbjd_op_matvec_AB_c64 if single else lib.bbjd_op_matvec_AB
+        rc = fn(self.n, dx.ptr, da.ptr, db.ptr, self.handle)
+        rc |= lib.bbjd_dev_sync()
+        if rc:
+            raise RuntimeError("bbjd_op_matvec_AB failed")
+        return da.to_array(dt, self.n), db.to_array(dt, self.n)
+
     def solve2(self, x0, x1):
         """both preconditioner solves of one correction step in one pass
         (bbjd_op_precond2)"""

@@ -41,6 +41,14 @@ typedef int (*bbjd_precond2_fn)(bbjd_int n, const bbjd_zomplex *x0, bbjd_zomplex
 typedef int (*bbjd_op_c64_fn)(bbjd_int n, const bbjd_cfloat *x,
                               bbjd_cfloat *y, void *data);
 
+/* Both matvecs of one vector at once, ya = A x and yb = B x (device
+ * pointers, no aliasing), same return contract as bbjd_matvec_fn; and its
+ * complex64 form. */
+typedef int (*bbjd_matvec2_fn)(bbjd_int n, const bbjd_zomplex *x, bbjd_zomplex *ya,
+                               bbjd_zomplex *yb, void *data);
+typedef int (*bbjd_op2_c64_fn)(bbjd_int n, const bbjd_cfloat *x, bbjd_cfloat *ya,
+                               bbjd_cfloat *yb, void *data);
+
 typedef struct {
     bbjd_callback_space callback_space;
     bbjd_precision      precision;
@@ -68,6 +76,11 @@ typedef struct {
      * then be capturable: kernels and copies enqueued on the callback
      * stream, no synchronisation, allocation or memset. */
     int                 graphs;
+    /* optional, device callbacks only: A x and B x of the same vector in one
+     * call (and its complex64 form for the mixed inner solve); must apply
+     * the same A and B as params.matvec_A / matvec_B.  NULL: two calls. */
+    bbjd_matvec2_fn     matvec_AB;
+    bbjd_op2_c64_fn     matvec_AB_c64;
 } bbjd_gpu_options;
 
 enum { BBJD_QZ_RITZ = 0, BBJD_QZ_SCHUR = 1 };

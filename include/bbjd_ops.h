@@ -80,7 +80,13 @@ int bbjd_op_precond (bbjd_int n, const bbjd_zomplex *x, bbjd_zomplex *y, void *o
 int bbjd_op_precond2(bbjd_int n, const bbjd_zomplex *x0, bbjd_zomplex *y0, const bbjd_zomplex *x1,
                      bbjd_zomplex *y1, void *op);
 
-/* complex64 forms (bbjd_op_c64_fn), available when want_c64 was set. */
+/* A x and B x in one pass over x (bbjd_matvec2_fn): the band rows, the
+ * diagonal of B and the dense rows' partial dots in one kernel. */
+int bbjd_op_matvec_AB(bbjd_int n, const bbjd_zomplex *x, bbjd_zomplex *ya, bbjd_zomplex *yb, void *op);
+
+/* complex64 forms (bbjd_op_c64_fn, bbjd_op2_c64_fn), available when want_c64
+ * was set. */
+int bbjd_op_matvec_AB_c64(bbjd_int n, const bbjd_cfloat *x, bbjd_cfloat *ya, bbjd_cfloat *yb, void *op);
 int bbjd_op_matvec_A_c64(bbjd_int n, const bbjd_cfloat *x, bbjd_cfloat *y, void *op);
 int bbjd_op_matvec_B_c64(bbjd_int n, const bbjd_cfloat *x, bbjd_cfloat *y, void *op);
 int bbjd_op_precond_c64 (bbjd_int n, const bbjd_cfloat *x, bbjd_cfloat *y, void *op);

@@ -198,6 +198,35 @@ def test_device_operators_match_oracle(n, bs, hw, ntau):
         dev.free()
 
 
+@pytest.mark.parametrize("n,bs,hw,ntau", SHAPES + [(20000, 8, 1, 4), (40000, 8, 2, 16), (3000, 4, 1, 20)])
+def test_fused_matvec_ab_matches_separate(n, bs, hw, ntau):
+    """bbjd_op_matvec_AB (A x and B x in one pass over x): the band rows and
+    B x bit-identical to the separate matvecs, A x to 1e-12 of scipy's
+    (the dense rows' dots sum in another order); the complex64 form to FP32
+    rounding; more than 16 dense rows take the separate kernels"""
+    p = BandedTauPencil.generate(n, bs, hw, ntau, seed=4)
+    dev = DeviceOperator(p, 0.2 + 0.1j, pc="none", c64=True)
+    rng = np.random.default_rng(n + ntau)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    A, B = p.to_scipy()
+    ra_ref, rb_ref = A @ x, B @ x
+    try:
+        ya, yb = dev.matvec_AB(x)
+        ra, rb = dev.matvec(x), dev.matvec_B(x)
+        n1 = n - ntau
+        np.testing.assert_array_equal(ya[:n1], ra[:n1])
+        np.testing.assert_array_equal(yb, rb)
+        if ntau > 16:
+            np.testing.assert_array_equal(ya, ra)
+        assert np.linalg.norm(ya - ra_ref) <= 1e-12 * np.linalg.norm(ra_ref)
+        assert np.linalg.norm(yb - rb_ref) <= 1e-15 * np.linalg.norm(rb_ref) + 1e-300
+        sa, sb = dev.matvec_AB(x, single=True)
+        assert np.linalg.norm(sa - ra_ref) <= 1e-5 * np.linalg.norm(ra_ref)
+        assert np.linalg.norm(sb - rb_ref) <= 1e-6 * np.linalg.norm(rb_ref) + 1e-300
+    finally:
+        dev.free()
+
+
 @pytest.mark.parametrize("n,bs,hw,ntau", [(60000, 8, 1, 4), (40000, 8, 2, 16), (30004, 4, 1, 2)])
 def test_bcr_level_groups_match_level_by_level(n, bs, hw, ntau):
     """the exact solve with 2, 3 or 4 block-cyclic-reduction levels per
