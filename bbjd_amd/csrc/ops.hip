@@ -306,11 +306,12 @@ __global__ __launch_bounds__(TPB) void k_bcr_bwd(int s, i64 M, i64 h, i64 no, co
 // rows of level l + g, whose neighbours at +- 2^(l+g) are boundaries or rows
 // of an earlier phase.  Each row's arithmetic is bcr_bwd_row's, in the same
 // order.  U fills the last phase's TPB threads (2^(G-1) rows per unit).
+// bl.f[g]: the buffer holding level l + g's right-hand sides.
 struct BcrLevels { const Cx<double> *f[4]; };
 
 template <int NR, int S, int G>
-__global__ __launch_bounds__(TPB) void k_bcr_bwd_multi(i64 N, int l, BcrLevels dpq, const Cx<double> *b,
-                                                       Cx<double> *x, i64 vs)
+__global__ __launch_bounds__(TPB) void k_bcr_bwd_multi(i64 N, int l, BcrLevels dpq, BcrLevels bl, Cx<double> *x,
+                                                       i64 vs)
 {
     constexpr int U = TPB / ((1 << (G - 1)) * S);
     constexpr int SLOTS = U * (1 << G) + 1;
@@ -336,6 +337,7 @@ __global__ __launch_bounds__(TPB) void k_bcr_bwd_multi(i64 N, int l, BcrLevels d
             if (m < M) {
                 const i64 i = m * hg, j = (m - 1) / 2;
                 const Cx<double> *D = dpq.f[g] + j * 3 * ss + r, *P = D + ss, *Qm = P + ss;
+                const Cx<double> *b = bl.f[g];
                 Cx<double> acc[NR];
 #pragma unroll
                 for (int q = 0; q < NR; q++) acc[q] = cx0();
@@ -368,46 +370,58 @@ __global__ __launch_bounds__(TPB) void k_bcr_bwd_multi(i64 N, int l, BcrLevels d
     }
 }
 
-// Forward levels 0 .. G-1 in one launch, reading the caller's right-hand
-// sides (n1 rows; padding rows read as zero), which nothing writes: block
-// rows are grouped in units of 2^G; each block reduces its U units plus the
-// unit to their left (rows 1 .. 2^G - 1 of it reach its first unit within
-// G levels) in LDS, in place, level by level, and writes its own units' rows
-// to b -- every row of them final for the levels below G (odd rows of level
-// g keep their value after level g - 1; the unit boundaries carry level
-// G - 1's update).  Each row's arithmetic is k_bcr_fwd0's and bcr_fwd_row's.
-// U + 1 units make 2 TPB / S rows, so level 0's TPB / S - 1 even rows fill
-// the block.
+// Forward levels l .. l + G - 1 in one launch.  Rows are those at
+// multiples of 2^l ("level-l rows", index i, block row i 2^l), grouped in
+// units of 2^G; each block reduces its U units plus the unit to their left
+// (rows 1 .. 2^G - 1 of it reach its first unit within G levels) in LDS, in
+// place, level by level, and writes its own units' level-l rows to bout --
+// every one of them final for the levels below l + G (odd rows of level
+// l + g keep their value after level l + g - 1; the unit boundaries carry
+// level l + G - 1's update).  The input is never written by the launch, so
+// the recomputed halo is exact: with x0 (x1), l = 0 and the caller's n1-row
+// right-hand sides (padding rows read as zero); else bin, another buffer
+// than bout.  Each row's arithmetic is k_bcr_fwd0's and bcr_fwd_row's.  U + 1
+// units make 2 TPB / S rows, so level l's TPB / S - 1 even rows fill the block.
 template <int NR, int S, int G>
-__global__ __launch_bounds__(TPB) void k_bcr_fwd_multi0(i64 N, i64 n1, BcrLevels ag,
-                                                        const Cx<double> *__restrict__ x0,
-                                                        const Cx<double> *__restrict__ x1, Cx<double> *b, i64 vs)
+__global__ __launch_bounds__(TPB) void k_bcr_fwd_multi(i64 N, int l, i64 n1, BcrLevels ag,
+                                                       const Cx<double> *__restrict__ x0,
+                                                       const Cx<double> *__restrict__ x1,
+                                                       const Cx<double> *__restrict__ bin, Cx<double> *bout, i64 vs)
 {
     constexpr int U = TPB / ((1 << (G - 1)) * S) - 1;
     constexpr int ROWS = (U + 1) << G;          // halo unit + own units: 2 TPB / S
     constexpr i64 ss = (i64)S * S;
     __shared__ Cx<double> lb[NR][ROWS][S];
     const int t = threadIdx.x;
-    const i64 p0 = ((i64)blockIdx.x * U - 1) << G;   // block row of slot 0 (the halo unit)
+    const i64 p0 = ((i64)blockIdx.x * U - 1) << G;   // level-l row of slot 0 (the halo unit)
     const Cx<double> *xs[2] = {x0, x1};
     for (int e = t; e < ROWS * S * NR; e += TPB) {
         const int q = e / (ROWS * S), rem = e % (ROWS * S), sl = rem / S, r = rem % S;
-        const i64 i = p0 + sl, el = i * S + r;
-        lb[q][sl][r] = (i >= 0 && el < n1) ? xs[q][el] : cx0();
+        const i64 il = p0 + sl, bi = il << l;
+        Cx<double> v = cx0();
+        if (il >= 0 && bi < N) {
+            if (x0) {
+                const i64 el = bi * S + r;
+                if (el < n1) v = xs[q][el];
+            } else
+                v = bin[q * vs + bi * S + r];
+        }
+        lb[q][sl][r] = v;
     }
     __syncthreads();
 #pragma unroll
     for (int g = 0; g < G; g++) {
-        const i64 h = (i64)1 << g, M = (N + h - 1) >> g;
-        // even rows of level g: slots that are multiples of 2^(g+1), except
-        // slot 0 (no row of the block needs it); within a level, rows read
-        // only their own value and odd neighbours, which the level leaves
+        const int lv = l + g;
+        const i64 h = (i64)1 << g, M = (N + ((i64)1 << lv) - 1) >> lv;
+        // even rows of level l + g: slots that are multiples of 2^(g+1),
+        // except slot 0 (no row of the block needs it); within a level, rows
+        // read only their own value and odd neighbours, which the level leaves
         const int nrow = (ROWS >> (g + 1)) - 1;
         for (int e = t; e < nrow * S; e += TPB) {
             const int sl = (e / S + 1) << (g + 1), r = e % S;
-            const i64 i = p0 + sl;
-            if (i >= 0 && i < N) {
-                const i64 m = i >> g, j = m / 2;
+            const i64 il = p0 + sl;
+            if (il >= 0 && (il << l) < N) {
+                const i64 m = il >> g, j = m / 2;
                 const Cx<double> *al = ag.f[g] + j * 2 * ss + r;
                 Cx<double> acc[NR];
 #pragma unroll
@@ -437,8 +451,8 @@ __global__ __launch_bounds__(TPB) void k_bcr_fwd_multi0(i64 N, i64 n1, BcrLevels
     for (int e = t; e < (ROWS - (1 << G)) * S * NR; e += TPB) {
         const int q = e / ((ROWS - (1 << G)) * S), rem = e % ((ROWS - (1 << G)) * S);
         const int sl = (1 << G) + rem / S, r = rem % S;
-        const i64 i = p0 + sl;
-        if (i < N) b[q * vs + i * S + r] = lb[q][sl][r];
+        const i64 bi = (p0 + sl) << l;
+        if (bi < N) bout[q * vs + bi * S + r] = lb[q][sl][r];
     }
 }
 
@@ -805,37 +819,43 @@ int bbjd_o_bcr_bwd(hipStream_t st, int s, int64_t M, int64_t h, int64_t no, cons
 
 static inline bool bcr_group_ok(int s, int G) { return (s == 4 || s == 8 || s == 16) && G >= 2 && G <= 4; }
 
-int bbjd_o_bcr_bwd_multi(hipStream_t st, int s, int64_t N, int l, int G, const void *const *dpq, const void *b,
-                         void *x, int nrhs, int64_t vs)
+int bbjd_o_bcr_bwd_multi(hipStream_t st, int s, int64_t N, int l, int G, const void *const *dpq,
+                         const void *const *bl, void *x, int nrhs, int64_t vs)
 {
     if (!bcr_group_ok(s, G) || (nrhs != 1 && nrhs != 2)) return -3;
-    BcrLevels f;
-    for (int g = 0; g < 4; g++) f.f[g] = (const Cx<double> *)(g < G ? dpq[g] : nullptr);
+    BcrLevels f, bb;
+    for (int g = 0; g < 4; g++) {
+        f.f[g] = (const Cx<double> *)(g < G ? dpq[g] : nullptr);
+        bb.f[g] = (const Cx<double> *)(g < G ? bl[g] : nullptr);
+    }
     const i64 H = (i64)1 << (l + G), units = (N + H - 1) / H;
 #define BBJD_BWDM(NR, S, GG)                                                                                   \
     {                                                                                                          \
         constexpr int U = TPB / ((1 << (GG - 1)) * S);                                                         \
         hipLaunchKernelGGL((k_bcr_bwd_multi<NR, S, GG>), dim3((unsigned)((units + U - 1) / U)), dim3(TPB), 0, st, \
-                           (i64)N, l, f, (const Cx<double> *)b, (Cx<double> *)x, (i64)vs);                     \
+                           (i64)N, l, f, bb, (Cx<double> *)x, (i64)vs);                                        \
     }
     BBJD_BCR_GROUP_DISPATCH(BBJD_BWDM)
 #undef BBJD_BWDM
     return chk();
 }
 
-int bbjd_o_bcr_fwd_multi0(hipStream_t st, int s, int64_t N, int64_t n1, int G, const void *const *ag,
-                          const void *x0, const void *x1, void *b, int nrhs, int64_t vs)
+int bbjd_o_bcr_fwd_multi(hipStream_t st, int s, int64_t N, int64_t n1, int l, int G, const void *const *ag,
+                         const void *x0, const void *x1, const void *bin, void *bout, int nrhs, int64_t vs)
 {
-    if (!bcr_group_ok(s, G) || (nrhs != 1 && nrhs != 2) || (nrhs == 2 && !x1)) return -3;
+    if (!bcr_group_ok(s, G) || (nrhs != 1 && nrhs != 2) || (x0 && (l != 0 || (nrhs == 2 && !x1))) ||
+        (!x0 && (!bin || bin == bout)))
+        return -3;
     BcrLevels f;
     for (int g = 0; g < 4; g++) f.f[g] = (const Cx<double> *)(g < G ? ag[g] : nullptr);
-    const i64 units = (N + ((i64)1 << G) - 1) >> G;
+    const i64 Ml = (N + ((i64)1 << l) - 1) >> l, units = (Ml + ((i64)1 << G) - 1) >> G;
 #define BBJD_FWDM(NR, S, GG)                                                                                   \
     {                                                                                                          \
         constexpr int U = TPB / ((1 << (GG - 1)) * S) - 1;                                                     \
-        hipLaunchKernelGGL((k_bcr_fwd_multi0<NR, S, GG>), dim3((unsigned)((units + U - 1) / U)), dim3(TPB), 0,  \
-                           st, (i64)N, (i64)n1, f, (const Cx<double> *)x0, (const Cx<double> *)(x1 ? x1 : x0),   \
-                           (Cx<double> *)b, (i64)vs);                                                          \
+        hipLaunchKernelGGL((k_bcr_fwd_multi<NR, S, GG>), dim3((unsigned)((units + U - 1) / U)), dim3(TPB), 0,   \
+                           st, (i64)N, l, (i64)n1, f, (const Cx<double> *)x0,                                  \
+                           (const Cx<double> *)(x1 ? x1 : x0), (const Cx<double> *)bin, (Cx<double> *)bout,     \
+                           (i64)vs);                                                                           \
     }
     BBJD_BCR_GROUP_DISPATCH(BBJD_FWDM)
 #undef BBJD_FWDM

@@ -26,12 +26,14 @@ int bbjd_o_bcr_bwd(hipStream_t st, int s, int64_t M, int64_t h, int64_t no, cons
                    int nrhs, int64_t vs);
 /* G (2..4) consecutive levels in one launch, for s in {4, 8, 16}; -3 for
  * other shapes.  Backward levels l + G - 1 .. l (dpq[g]: level l + g's odd
- * factor blocks); forward levels 0 .. G - 1 from the right-hand sides x0
- * (x1) (ag[g]: level g's even factor blocks), writing every row of b. */
-int bbjd_o_bcr_bwd_multi(hipStream_t st, int s, int64_t N, int l, int G, const void *const *dpq, const void *b,
-                         void *x, int nrhs, int64_t vs);
-int bbjd_o_bcr_fwd_multi0(hipStream_t st, int s, int64_t N, int64_t n1, int G, const void *const *ag,
-                          const void *x0, const void *x1, void *b, int nrhs, int64_t vs);
+ * factor blocks, bl[g]: the buffer holding its right-hand sides); forward
+ * levels l .. l + G - 1 (ag[g]: level l + g's even factor blocks) from the
+ * caller's x0 (x1) (l = 0) or from bin into bout (another buffer), writing
+ * every row at a multiple of 2^l. */
+int bbjd_o_bcr_bwd_multi(hipStream_t st, int s, int64_t N, int l, int G, const void *const *dpq,
+                         const void *const *bl, void *x, int nrhs, int64_t vs);
+int bbjd_o_bcr_fwd_multi(hipStream_t st, int s, int64_t N, int64_t n1, int l, int G, const void *const *ag,
+                         const void *x0, const void *x1, const void *bin, void *bout, int nrhs, int64_t vs);
 /* first level of the one-workgroup tail for this shape (>= l0, where the
  * LDS-resident tail fits; l0 when the global tail is used) */
 int bbjd_o_bcr_tail_level(int s, int64_t N, int l0, int L, int nrhs);

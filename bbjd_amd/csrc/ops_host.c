@@ -49,6 +49,8 @@ struct bbjd_op {
     int l0;                             /* first level run by the tail kernel */
     void *doff;                         /* device copy of the level offsets  */
     void *bvec, *xvec;                  /* padded work vectors (N s)         */
+    void *bvec2;                        /* second right-hand-side buffer: fused
+                                           forward levels read one, write the other */
     void *Y, *Sinv, *small;             /* border                            */
     bbjd_red rd;                        /* tau-row dot scratch               */
     void *zt1, *zt2;                    /* complex128 scratch for c64 applies */
@@ -247,9 +249,7 @@ static hipStream_t op_stream(const struct bbjd_op *op)
     return s ? s : op->stream;
 }
 
-/* Levels per fused BCR launch (1: one launch per level).  The backward
- * levels go in groups from the top; the first forward levels, which read
- * the caller's vectors, in one group. */
+/* Levels per fused BCR launch (1: one launch per level). */
 static int g_bcr_group = 4;
 void bbjd_set_bcr_group(int levels) { g_bcr_group = levels < 1 ? 1 : (levels > 4 ? 4 : levels); }
 
@@ -258,7 +258,14 @@ void bbjd_set_bcr_group(int levels) { g_bcr_group = levels < 1 ? 1 : (levels > 4
  * the final solve and their backward levels in one workgroup.  With x0 (x1)
  * the right-hand sides are the caller's n1-row vectors, read by the first
  * levels (or copied into op->bvec when every level is in the tail);
- * without, they are already in op->bvec (vector q at q N s). */
+ * without, they are already in op->bvec (vector q at q N s).
+ *
+ * A fused forward group reads one right-hand-side buffer and writes the
+ * rows at multiples of 2^l into the other (its halo must see the values
+ * from before the group), so the current buffer alternates between bvec
+ * and bvec2; the odd rows of level l are final once its group has run and
+ * stay in the buffer it wrote (later groups write only multiples of higher
+ * powers of two), which the backward level reads (bl[l]). */
 static int bcr_solve(struct bbjd_op *op, hipStream_t st, int nrhs, const void *x0, const void *x1)
 {
     const int s = op->s;
@@ -266,45 +273,67 @@ static int bcr_solve(struct bbjd_op *op, hipStream_t st, int nrhs, const void *x
     const int64_t vs = (int64_t)op->N * s;
     int rc = 0;
     const int lt = bbjd_o_bcr_tail_level(s, op->N, op->l0, op->L, nrhs);
-    const void *lv[4];
+    void *buf[2] = {op->bvec, op->bvec2};
+    int cur = 0;
+    int bl[64];
+    const void *lv[4], *lb[4];
     int l = 0;
     if (x0 && lt >= 1 && s <= 64) {
         const int G = lt < g_bcr_group ? lt : g_bcr_group;
         int e = -3;
         if (G >= 2) {
             for (int g = 0; g < G; g++) lv[g] = (const zcx *)op->bcr + (size_t)op->even_off[g] * ss;
-            e = bbjd_o_bcr_fwd_multi0(st, s, op->N, op->n1, G, lv, x0, x1, op->bvec, nrhs, vs);
+            e = bbjd_o_bcr_fwd_multi(st, s, op->N, op->n1, 0, G, lv, x0, x1, NULL, buf[0], nrhs, vs);
             if (e == 0) l = G;
             else if (e != -3) rc |= e;
         }
         if (e == -3) {
             rc |= bbjd_o_bcr_fwd0(st, s, op->N, op->n1, (const zcx *)op->bcr + (size_t)op->even_off[0] * ss, x0,
-                                  x1, op->bvec, nrhs, vs);
+                                  x1, buf[0], nrhs, vs);
             l = 1;
         }
+        for (int k = 0; k < l; k++) bl[k] = 0;
     } else if (x0) {
         rc |= !HIPOK(hipMemcpyAsync(op->bvec, x0, (size_t)op->n1 * 16, hipMemcpyDeviceToDevice, st));
         if (nrhs == 2)
             rc |= !HIPOK(hipMemcpyAsync((zcx *)op->bvec + vs, x1, (size_t)op->n1 * 16, hipMemcpyDeviceToDevice, st));
     }
-    for (; l < lt; l++) {
+    while (l < lt) {
+        const int G = lt - l < g_bcr_group ? lt - l : g_bcr_group;
+        if (G >= 2) {
+            for (int g = 0; g < G; g++) lv[g] = (const zcx *)op->bcr + (size_t)op->even_off[l + g] * ss;
+            const int e = bbjd_o_bcr_fwd_multi(st, s, op->N, op->n1, l, G, lv, NULL, NULL, buf[cur], buf[1 - cur],
+                                               nrhs, vs);
+            if (e == 0) {
+                cur = 1 - cur;
+                for (int g = 0; g < G; g++) bl[l + g] = cur;
+                l += G;
+                continue;
+            }
+            if (e != -3) { rc |= e; break; }
+        }
         const bbjd_int h = (bbjd_int)1 << l, M = (op->N + h - 1) >> l;
         rc |= bbjd_o_bcr_fwd(st, s, M, h, (M + 1) / 2, (const zcx *)op->bcr + (size_t)op->even_off[l] * ss,
-                             op->bvec, nrhs, vs);
+                             buf[cur], nrhs, vs);
+        bl[l] = cur;
+        l++;
     }
-    rc |= bbjd_o_bcr_tail(st, s, op->N, lt, op->L, (const int64_t *)op->doff, op->bcr, op->bvec, op->xvec, nrhs,
+    rc |= bbjd_o_bcr_tail(st, s, op->N, lt, op->L, (const int64_t *)op->doff, op->bcr, buf[cur], op->xvec, nrhs,
                           vs);
     for (int l = lt - 1; l >= 0;) {
         const int G = l + 1 < g_bcr_group ? l + 1 : g_bcr_group;
         if (G >= 2) {
             const int lo = l - G + 1;
-            for (int g = 0; g < G; g++) lv[g] = (const zcx *)op->bcr + (size_t)op->odd_off[lo + g] * ss;
-            const int e = bbjd_o_bcr_bwd_multi(st, s, op->N, lo, G, lv, op->bvec, op->xvec, nrhs, vs);
+            for (int g = 0; g < G; g++) {
+                lv[g] = (const zcx *)op->bcr + (size_t)op->odd_off[lo + g] * ss;
+                lb[g] = buf[bl[lo + g]];
+            }
+            const int e = bbjd_o_bcr_bwd_multi(st, s, op->N, lo, G, lv, lb, op->xvec, nrhs, vs);
             if (e == 0) { l = lo - 1; continue; }
             if (e != -3) { rc |= e; break; }
         }
         const bbjd_int h = (bbjd_int)1 << l, M = (op->N + h - 1) >> l;
-        rc |= bbjd_o_bcr_bwd(st, s, M, h, M / 2, (const zcx *)op->bcr + (size_t)op->odd_off[l] * ss, op->bvec,
+        rc |= bbjd_o_bcr_bwd(st, s, M, h, M / 2, (const zcx *)op->bcr + (size_t)op->odd_off[l] * ss, buf[bl[l]],
                              op->xvec, nrhs, vs);
         l--;
     }
@@ -337,7 +366,7 @@ static int exact_apply(struct bbjd_op *op, hipStream_t st, int nrhs, const void 
 
 /* ---- device layout --------------------------------------------------- */
 enum { D_BAND, D_TAUC, D_BDIAG, D_BANDC, D_TAUCC, D_BDIAGC, D_BJ, D_BJC, D_BCR, D_BV, D_XV, D_Y, D_SINV, D_SMALL,
-       D_PART, D_PNORM, D_RSC, D_ZT1, D_ZT2, D_OFF, D_END };
+       D_PART, D_PNORM, D_RSC, D_ZT1, D_ZT2, D_OFF, D_BV2, D_END };
 
 static size_t bcr_block_count(bbjd_int N)
 {
@@ -367,7 +396,7 @@ static size_t op_layout(const bbjd_pencil_desc *d, int pc, int want_c64, bbjd_in
     }
     if (pc == BBJD_PC_EXACT_LU) {
         sz[D_BCR] = bcr_blocks * s * s * 16;
-        sz[D_BV] = sz[D_XV] = (size_t)2 * N * s * 16;      /* two right-hand sides */
+        sz[D_BV] = sz[D_XV] = sz[D_BV2] = (size_t)2 * N * s * 16;      /* two right-hand sides */
         sz[D_OFF] = (size_t)(2 * 64 + 2) * 8;
         sz[D_Y] = (size_t)n1 * d->ntau * 16;
         sz[D_SINV] = (size_t)d->ntau * d->ntau * 16 + 16;
@@ -445,6 +474,7 @@ static bbjd_op *op_create_impl(const bbjd_pencil_desc *d, bbjd_zomplex sigma, bb
     op->band = AT(D_BAND); op->tauc = AT(D_TAUC); op->bdiag = AT(D_BDIAG);
     op->band_c = AT(D_BANDC); op->tauc_c = AT(D_TAUCC); op->bdiag_c = AT(D_BDIAGC);
     op->bj = AT(D_BJ); op->bj_c = AT(D_BJC); op->bcr = AT(D_BCR); op->bvec = AT(D_BV); op->xvec = AT(D_XV);
+    op->bvec2 = AT(D_BV2);
     op->Y = AT(D_Y); op->Sinv = AT(D_SINV); op->small = AT(D_SMALL);
     op->rd.part = AT(D_PART); op->rd.pnorm = AT(D_PNORM); op->rd.part_cap = pe;
     {
