@@ -146,7 +146,13 @@ static void m11_block(const bbjd_pencil_desc *d, zcx sigma, int s, bbjd_int n1, 
         }
 }
 
-#define BCR_TAIL_ROWS 256
+/* The one-workgroup tail starts at the first level with at most this many
+ * rows (M s); set before an operator is created.  With the launched levels
+ * fused four per launch a level there costs little, so the serial tail is
+ * kept short: 64 measured best on MI355X (s = 8, one-vector solve: 81 us
+ * at n = 2e5 against 86 with 256, level at n = 2e4).  Tunable. */
+static int g_bcr_tail_rows = 64;
+void bbjd_set_bcr_tail_rows(int rows) { g_bcr_tail_rows = rows < 1 ? 64 : rows; }
 
 /* Host BCR factorisation; fills op->bcr (host image) and the offsets. */
 static int bcr_factor(struct bbjd_op *op, const bbjd_pencil_desc *d, zcx **host_out, size_t *nblocks_out)
@@ -170,13 +176,13 @@ static int bcr_factor(struct bbjd_op *op, const bbjd_pencil_desc *d, zcx **host_
     op->final_off = (bbjd_int)nbk;
     nbk += 1;
     /* The one-workgroup tail starts at the first level with at most
-     * BCR_TAIL_ROWS rows (M s): a level costs it about 0.17 us per block row
-     * (MI355X, s = 8, two right-hand sides), a launched level about 3.5 us
-     * whatever its size, so wider levels run faster as launches. */
+     * g_bcr_tail_rows rows (M s): a level costs it about 0.17 us per block row
+     * (MI355X, s = 8, two right-hand sides), so wider levels run faster in
+     * the launched groups. */
     op->l0 = L;
     for (int l = 0; l < L; l++) {
         const bbjd_int M = (N + ((bbjd_int)1 << l) - 1) >> l;
-        if (M * s <= BCR_TAIL_ROWS) { op->l0 = l; break; }
+        if (M * s <= g_bcr_tail_rows) { op->l0 = l; break; }
     }
     zcx *H = malloc(nbk * ss * sizeof(zcx));
     zcx *Lc = malloc((size_t)N * ss * sizeof(zcx)), *Dc = malloc((size_t)N * ss * sizeof(zcx)),

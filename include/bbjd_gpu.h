@@ -210,6 +210,9 @@ int bbjd_kbench(int which, int64_t n, int k, int reps, double *avg_ms, double *b
  * last block of the producing kernel instead of a second launch. */
 void bbjd_set_fuse_blocks(int max_blocks);
 
+/* Largest grid of the fused CGS2's middle pass (default 512 blocks). */
+void bbjd_set_cgs_mid_blocks(int max_blocks);
+
 /* Tiles of the streaming kernels, for tuning.  proj_blocks: blocks per
  * projection launch (rows are split into chunks to give about that many;
  * 0 keeps the current value).  upd_rows: rows per block of the update / norm

@@ -94,6 +94,10 @@ int bbjd_op_precond_c64 (bbjd_int n, const bbjd_cfloat *x, bbjd_cfloat *y, void 
 /* Block cyclic reduction levels per launch of the exact solve (1..4, default
  * 4; 1: one launch per level).  Process-wide; tuning and tests. */
 void bbjd_set_bcr_group(int levels);
+/* The exact solve's one-workgroup tail starts at the first reduction level
+ * with at most this many rows (default 64); applies to operators created
+ * afterwards.  Tuning. */
+void bbjd_set_bcr_tail_rows(int rows);
 
 #ifdef __cplusplus
 }
