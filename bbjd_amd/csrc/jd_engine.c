@@ -979,7 +979,6 @@ static int expand(ctx_t *c, bbjd_int k, const void *z, zcx nu0, zcx mu0)
 {
     bbjd_workspace *ws = c->ws;
     struct bbjd_dev *d = c->d;
-    const bbjd_params *P = c->P;
     const bbjd_int n = c->n;
     void *vn = COLZ(ws->V, k), *wn = COLZ(ws->W, k), *an = COLZ(ws->AV, k), *bn = COLZ(ws->BV, k);
     const int optimistic = d->opt.callback_space == BBJD_CALLBACK_DEVICE;

@@ -74,6 +74,7 @@ void bbjd_serve_config_init(bbjd_serve_config *c)
     c->precision = BBJD_PREC_FP64;
     c->realtime = 1;
     c->sync_yield = 1;
+    c->order = BBJD_ORDER_ARRIVAL;
 }
 
 static bbjd_int req_n(const bbjd_request *q) { return q->op ? bbjd_op_size(q->op) : (q->pencil ? q->pencil->n : 0); }
@@ -247,16 +248,42 @@ int bbjd_serve(bbjd_request *reqs, int nreq, const bbjd_serve_config *cfg, bbjd_
     int rc = 0;
     if (failed_init == nw) rc = -1;
     S.t0 = now_s();     /* the run starts with every workspace resident */
-    for (int oi = 0; oi < nreq && rc == 0; oi++) {
-        const int r = order[oi];
-        bbjd_request *q = &reqs[r];
-        if (cfg->realtime) {
-            double wait = q->arrival - (now_s() - S.t0);
+    /* requests that have arrived and wait, in arrival order */
+    int *pend = malloc((size_t)nreq * sizeof(int));
+    if (!pend) rc = -1;
+    int npend = 0, next = 0;
+    for (int admitted = 0; admitted < nreq && rc == 0; admitted++) {
+        /* a free worker first: the choice among waiting requests is made
+         * when one can start */
+        pthread_mutex_lock(&S.mu);
+        for (;;) {
+            int free_slot = 0;
+            for (int i = 0; i < nw && !free_slot; i++) free_slot = S.w[i].ok && S.w[i].req < 0;
+            if (free_slot) break;
+            pthread_cond_wait(&S.cv_free, &S.mu);
+        }
+        pthread_mutex_unlock(&S.mu);
+        /* everything that has arrived by now; at least the next arrival */
+        if (npend == 0 && cfg->realtime) {
+            const double wait = reqs[order[next]].arrival - (now_s() - S.t0);
             if (wait > 0) {
                 struct timespec ts = {(time_t)wait, (long)((wait - (double)(time_t)wait) * 1e9)};
                 nanosleep(&ts, NULL);
             }
         }
+        const double now = now_s() - S.t0;
+        while (next < nreq && (!cfg->realtime || reqs[order[next]].arrival <= now || npend == 0))
+            pend[npend++] = order[next++];
+        int pi = 0;
+        if (cfg->order == BBJD_ORDER_LARGEST)
+            for (int i = 1; i < npend; i++) {
+                const bbjd_request *a = &reqs[pend[i]], *b = &reqs[pend[pi]];
+                if (a->nev > b->nev || (a->nev == b->nev && req_n(a) > req_n(b))) pi = i;
+            }
+        const int r = pend[pi];
+        memmove(pend + pi, pend + pi + 1, (size_t)(npend - pi - 1) * sizeof(int));
+        npend--;
+        bbjd_request *q = &reqs[r];
         const size_t need = q->op ? 0 : bbjd_op_bytes_for(q->pencil, q->pc, cfg->precision == BBJD_PREC_MIXED);
         pthread_mutex_lock(&S.mu);
         int slot = -1;
@@ -277,6 +304,7 @@ int bbjd_serve(bbjd_request *reqs, int nreq, const bbjd_serve_config *cfg, bbjd_
         pthread_cond_broadcast(&S.cv_work);
         pthread_mutex_unlock(&S.mu);
     }
+    free(pend);
     pthread_mutex_lock(&S.mu);
     while (S.active > 0) pthread_cond_wait(&S.cv_free, &S.mu);
     S.stop = 1;

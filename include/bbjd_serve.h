@@ -51,7 +51,13 @@ typedef struct {
     int    realtime;      /* 1: admit a request no earlier than its arrival  */
     int    sync_yield;    /* 1: host threads yield while waiting on the GPU  */
     int    graphs;        /* bbjd_gpu_options.graphs of every request        */
+    int    order;         /* which arrived request a free worker takes:
+                             BBJD_ORDER_ARRIVAL (first come, first served)
+                             or BBJD_ORDER_LARGEST (largest estimated work:
+                             nev, then n)                                    */
 } bbjd_serve_config;
+
+enum { BBJD_ORDER_ARRIVAL = 0, BBJD_ORDER_LARGEST = 1 };
 
 typedef struct {
     double   wall_seconds;
