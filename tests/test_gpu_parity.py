@@ -511,3 +511,52 @@ def test_qz_modes_same_eigenpairs():
         out[mode] = (ev, iters)
     assert rel_match(out["ritz"][0], out["schur"][0]) <= 1e-8
     assert abs(out["ritz"][1] - out["schur"][1]) <= max(5, ref["iters"] // 10)
+
+
+def _serve_jobs(seed=7):
+    rng = np.random.default_rng(seed)
+    shapes = [(2000, 4), (5000, 8), (12000, 4), (3000, 8), (8000, 6), (2000, 8), (6000, 4), (4000, 8)]
+    jobs = []
+    for i, (n, k) in enumerate(shapes):
+        p = BandedTauPencil.generate(n, 8, 1, 4, seed=100 + i)
+        sigma = complex(rng.uniform(-3, 3), rng.uniform(-3, 3))
+        jobs.append(dict(pencil=p, pc="exact", sigma=sigma, nev=k, maxsub=4 * k, minsub=2 * k, maxiter=1000,
+                         tol=1e-10, arrival=0.02 * i))
+    return jobs
+
+
+def test_serving_loop_results_alone_equal_mix():
+    """bbjd_serve (continuous batching over resident workspaces, operators
+    uploaded and factored on admission): each request's eigenvalues are
+    bit-identical alone and in a concurrent mix -- arrivals in real time,
+    largest-first admission, and an HBM budget that throttles admission --
+    every pair's FP64 residual ||A x - lambda B x|| is below
+    tol (||A|| + |lambda| ||B||), and two requests match the oracle"""
+    from bbjd_amd import serve as srv
+    from bbjd_amd.configs import row_norm_bound
+    jobs = _serve_jobs()
+    mix, rep = srv.serve(jobs, workers=4, realtime=True)
+    assert rep["failed"] == 0 and rep["peak_concurrency"] >= 2
+    lpt, _ = srv.serve(jobs, workers=4, realtime=False, order="largest")
+    budget = rep["peak_bytes"] // 2
+    tight, rep_t = srv.serve(jobs, workers=4, realtime=False, hbm_budget=budget)
+    assert rep_t["peak_bytes"] <= budget and rep_t["failed"] == 0
+    print(f"[serve] mix peak {rep['peak_bytes']} concurrency {rep['peak_concurrency']}; tight budget {budget}: "
+          f"workers {rep_t['workers']} peak {rep_t['peak_bytes']} concurrency {rep_t['peak_concurrency']}",
+          flush=True)
+    for i, j in enumerate(jobs):
+        alone, _ = srv.serve([j], workers=1, realtime=False)
+        a = alone[0]
+        assert a["status"] == 0 and a["nconv"] == j["nev"], i
+        for tag, other in (("mix", mix[i]), ("largest-first", lpt[i]), ("tight", tight[i])):
+            np.testing.assert_array_equal(other["eigenvalues"], a["eigenvalues"], err_msg=f"{i} {tag}")
+        anorm = row_norm_bound(j["pencil"])           # lower bound of ||A||: a stricter test
+        assert np.all(a["pair_residuals"] <= 1e-10 * (anorm + np.abs(a["eigenvalues"]))), i
+    for i in (0, 5):
+        j = jobs[i]
+        p = j["pencil"]
+        ref = orc.solve("oracle", p.n, j["nev"], j["sigma"], tol=1e-10, maxsub=j["maxsub"], minsub=j["minsub"],
+                        maxiter=1000, ops=orc.CpuPencilOps(p, j["sigma"], 1))
+        assert ref["nconv"] == j["nev"]
+        assert rel_match(mix[i]["eigenvalues"], ref["evals"]) <= 1e-8
+        assert rel_match(ref["evals"], mix[i]["eigenvalues"]) <= 1e-8
