@@ -88,6 +88,7 @@ def jdqz_c(n, nev, sigma, target, matvec_A, matvec_B, precond_solve, tol=1e-10, 
         if precond_solve is not None:
             opts.precond2 = _lib.fn_addr("bbjd_op_precond2")
         opts.matvec_AB = _lib.fn_addr("bbjd_op_matvec_AB")
+        opts.matvec_ABr = _lib.fn_addr("bbjd_op_matvec_ABr")
         p.op_data = device_op.handle
         if precision == "mixed" and device_op.c64:
             opts.matvec_A_c64 = _lib.fn_addr("bbjd_op_matvec_A_c64")

@@ -45,7 +45,8 @@ class GpuOptions(C.Structure):
     _fields_ = [("callback_space", C.c_int), ("precision", C.c_int), ("stream", C.c_void_p),
                 ("matvec_A_c64", C.c_void_p), ("matvec_B_c64", C.c_void_p),
                 ("precond_c64", C.c_void_p), ("precond2", C.c_void_p), ("qz_mode", C.c_int),
-                ("graphs", C.c_int), ("matvec_AB", C.c_void_p), ("matvec_AB_c64", C.c_void_p)]
+                ("graphs", C.c_int), ("matvec_AB", C.c_void_p), ("matvec_AB_c64", C.c_void_p),
+                ("matvec_ABr", C.c_void_p)]
 
 
 class Stats(C.Structure):
@@ -126,6 +127,7 @@ def load():
     _sig(lib, "bbjd_op_bytes", C.c_size_t, V)
     _sig(lib, "bbjd_op_precond2", C.c_int, I64, V, V, V, V, V)
     _sig(lib, "bbjd_op_matvec_AB", C.c_int, I64, V, V, V, V)
+    _sig(lib, "bbjd_op_matvec_ABr", C.c_int, I64, V, V, V, zcomplex, zcomplex, V, V, V)
     _sig(lib, "bbjd_op_matvec_AB_c64", C.c_int, I64, V, V, V, V)
     for nm in ("bbjd_op_matvec_A", "bbjd_op_matvec_B", "bbjd_op_precond",
                "bbjd_op_matvec_A_c64", "bbjd_op_matvec_B_c64", "bbjd_op_precond_c64"):

@@ -1170,8 +1170,18 @@ static bbjd_status solve_impl(const bbjd_params *P, bbjd_workspace *ws, bbjd_res
             rc |= bbjd_k_gemv_n(c->s, n, (int)k, ws->V, c->ld, d->rd.coef, u, &d->rd, -1);
         }
         /* ---- step 3: residual ---- */
-        if (apply_ab(c, u, Au, Bu)) goto cb_err;
-        rc |= bbjd_k_lincomb_norm(c->s, n, creal(tb), cimag(tb), Au, -creal(ta), -cimag(ta), Bu, r, &d->rd, SL_R);
+        if (d->opt.matvec_ABr && d->opt.callback_space == BBJD_CALLBACK_DEVICE) {
+            hipStream_t prev = tl_cb_stream;
+            tl_cb_stream = c->s;
+            const int e = d->opt.matvec_ABr(n, (const bbjd_zomplex *)u, (bbjd_zomplex *)Au, (bbjd_zomplex *)Bu, tb, -ta,
+                                            (bbjd_zomplex *)r, d->rd.scal + SL_R, P->op_data);
+            tl_cb_stream = prev;
+            if (e) goto cb_err;
+        } else {
+            if (apply_ab(c, u, Au, Bu)) goto cb_err;
+            rc |= bbjd_k_lincomb_norm(c->s, n, creal(tb), cimag(tb), Au, -creal(ta), -cimag(ta), Bu, r, &d->rd,
+                                      SL_R);
+        }
         if (rc || sync_scal(c)) { st = BBJD_ERR_ALLOC; goto done; }
         d->st.stream_bytes += 16.0 * n * (k + 1 + 3);
         const double den = cabs(ta) + cabs(tb);

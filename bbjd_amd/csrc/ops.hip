@@ -59,7 +59,10 @@ __global__ __launch_bounds__(TPB) void k_diag_mv(i64 n, const Cx<R> *__restrict_
 // y_A = A x and y_B = B x in one pass over x (the block-banded rows i < n1,
 // the diagonal of B on every row, and per-block partial dots of the ntau
 // dense rows, conj(tauc[:, c]) . x, finished by k_tau_finish).  The band
-// rows and the diagonal are k_band_mv's and k_diag_mv's arithmetic.
+// rows and the diagonal are k_band_mv's and k_diag_mv's arithmetic.  With r
+// non-NULL also r = a y_A + b y_B on the band rows (the engine's k_lincomb
+// arithmetic) and per-block partial sums of |r|^2 in rpart; the dense rows'
+// r and the norm come in k_tau_finish.
 template <typename R>
 __device__ __forceinline__ Cx<R> wave_sum(Cx<R> v)
 {
@@ -71,6 +74,23 @@ __device__ __forceinline__ Cx<R> wave_sum(Cx<R> v)
     return v;
 }
 
+template <typename R>
+__device__ __forceinline__ R wave_sum_r(R v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// a y_A + b y_B as the engine's k_lincomb forms it: (a * ya) + (b * yb)
+template <typename R>
+__device__ __forceinline__ Cx<R> lin2(Cx<R> a, Cx<R> x, Cx<R> b, Cx<R> y)
+{
+    const Cx<R> p = {a.re * x.re - a.im * x.im, a.re * x.im + a.im * x.re};
+    const Cx<R> q = {b.re * y.re - b.im * y.im, b.re * y.im + b.im * y.re};
+    return Cx<R>{p.re + q.re, p.im + q.im};
+}
+
 static constexpr int AB_MAX_TAU = 16;
 
 template <typename R>
@@ -78,29 +98,37 @@ __global__ __launch_bounds__(TPB) void k_band_ab(i64 n, i64 n1, i64 nb, int bs, 
                                                  const Cx<R> *__restrict__ band, const Cx<R> *__restrict__ dg,
                                                  int ntau, const Cx<R> *__restrict__ tauc,
                                                  const Cx<R> *__restrict__ x, Cx<R> *__restrict__ ya,
-                                                 Cx<R> *__restrict__ yb, Cx<R> *__restrict__ part)
+                                                 Cx<R> *__restrict__ yb, Cx<R> *__restrict__ part, Cx<R> ca,
+                                                 Cx<R> cb, Cx<R> *__restrict__ r, R *__restrict__ rpart)
 {
     __shared__ Cx<R> red[AB_MAX_TAU][TPB / 64];
+    __shared__ R rred[TPB / 64];
     const i64 i = (i64)blockIdx.x * TPB + threadIdx.x;
     Cx<R> xi = {0, 0};
+    R rs = 0;
     if (i < n) {
         xi = x[i];
+        Cx<R> b = {0, 0};
+        cfma(b, dg[i], xi);
+        yb[i] = b;
         if (i < n1) {
             const i64 bi = i / bs;
-            const int r = (int)(i % bs), nd = 2 * hw + 1;
+            const int rr = (int)(i % bs), nd = 2 * hw + 1;
             Cx<R> acc = {0, 0};
             for (int d = 0; d < nd; d++) {
                 const i64 bj = bi + d - hw;
                 if (bj < 0 || bj >= nb) continue;
-                const Cx<R> *blk = band + (bi * nd + d) * bs * bs + r;
+                const Cx<R> *blk = band + (bi * nd + d) * bs * bs + rr;
                 const Cx<R> *xs = x + bj * bs;
                 for (int c = 0; c < bs; c++) cfma(acc, blk[(i64)c * bs], xs[c]);
             }
             ya[i] = acc;
+            if (r) {
+                const Cx<R> v = lin2(ca, acc, cb, b);
+                r[i] = v;
+                rs = v.re * v.re + v.im * v.im;
+            }
         }
-        Cx<R> b = {0, 0};
-        cfma(b, dg[i], xi);
-        yb[i] = b;
     }
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     for (int c = 0; c < ntau; c++) {
@@ -114,6 +142,10 @@ __global__ __launch_bounds__(TPB) void k_band_ab(i64 n, i64 n1, i64 nb, int bs, 
         p = wave_sum(p);
         if (lane == 0) red[c][w] = p;
     }
+    if (r) {
+        rs = wave_sum_r(rs);
+        if (lane == 0) rred[w] = rs;
+    }
     __syncthreads();
     if (threadIdx.x < ntau) {
         Cx<R> s = red[threadIdx.x][0];
@@ -121,23 +153,50 @@ __global__ __launch_bounds__(TPB) void k_band_ab(i64 n, i64 n1, i64 nb, int bs, 
         for (int v = 1; v < TPB / 64; v++) { s.re += red[threadIdx.x][v].re; s.im += red[threadIdx.x][v].im; }
         part[(i64)blockIdx.x * ntau + threadIdx.x] = s;
     }
+    if (r && threadIdx.x == 0) {
+        R s = rred[0];
+#pragma unroll
+        for (int v = 1; v < TPB / 64; v++) s += rred[v];
+        rpart[blockIdx.x] = s;
+    }
 }
 
-// y_A's tau rows: wave c sums column c of the nblk block partials in a
-// fixed order
+// y_A's dense rows: wave c sums column c of the nblk block partials in a
+// fixed order.  With r: wave ntau sums the band rows' |r|^2 partials, then
+// forms the dense rows' r = a y_A + b y_B and writes ||r|| to *rnorm.
 template <typename R>
-__global__ __launch_bounds__(64 * AB_MAX_TAU) void k_tau_finish(i64 nblk, int ntau, const Cx<R> *__restrict__ part,
-                                                                Cx<R> *__restrict__ out)
+__global__ __launch_bounds__(64 * AB_MAX_TAU) void k_tau_finish(i64 nblk, int ntau, i64 n1,
+                                                                      const Cx<R> *__restrict__ part,
+                                                                      Cx<R> *__restrict__ ya,
+                                                                      const Cx<R> *__restrict__ yb, Cx<R> ca,
+                                                                      Cx<R> cb, Cx<R> *__restrict__ r,
+                                                                      const R *__restrict__ rpart,
+                                                                      double *__restrict__ rnorm)
 {
+    __shared__ Cx<R> tv[AB_MAX_TAU];
     const int lane = threadIdx.x & 63, c = threadIdx.x >> 6;
-    if (c >= ntau) return;
-    Cx<R> s = {0, 0};
-    for (i64 b = lane; b < nblk; b += 64) {
-        const Cx<R> v = part[b * ntau + c];
-        s.re += v.re; s.im += v.im;
+    R rs = 0;
+    if (c < ntau) {
+        Cx<R> s = {0, 0};
+        for (i64 b = lane; b < nblk; b += 64) {
+            const Cx<R> v = part[b * ntau + c];
+            s.re += v.re; s.im += v.im;
+        }
+        s = wave_sum(s);
+        if (lane == 0) { ya[n1 + c] = s; tv[c] = s; }
+    } else if (c == ntau && r) {
+        for (i64 b = lane; b < nblk; b += 64) rs += rpart[b];
+        rs = wave_sum_r(rs);
     }
-    s = wave_sum(s);
-    if (lane == 0) out[c] = s;
+    __syncthreads();
+    if (r && c == ntau && lane == 0) {
+        for (int t = 0; t < ntau; t++) {
+            const Cx<R> v = lin2(ca, tv[t], cb, yb[n1 + t]);
+            r[n1 + t] = v;
+            rs += v.re * v.re + v.im * v.im;
+        }
+        *rnorm = sqrt((double)rs);
+    }
 }
 
 template <typename R>
@@ -713,10 +772,30 @@ int bbjd_o_band_ab(hipStream_t s, int64_t n, int64_t n1, int64_t nb, int bs, int
                    const void *dg, int ntau, const void *tauc, const void *x, void *ya, void *yb, void *part,
                    int64_t part_cap, int single)
 {
+    return bbjd_o_band_abr(s, n, n1, nb, bs, hw, band, dg, ntau, tauc, x, ya, yb, part, part_cap, single, 0, 0, 0,
+                           0, nullptr, nullptr, 0, nullptr);
+}
+
+int bbjd_o_band_abr(hipStream_t s, int64_t n, int64_t n1, int64_t nb, int bs, int hw, const void *band,
+                    const void *dg, int ntau, const void *tauc, const void *x, void *ya, void *yb, void *part,
+                    int64_t part_cap, int single, double are, double aim, double bre, double bim, void *r,
+                    void *rpart, int64_t rpart_cap, double *rnorm)
+{
     if (n <= 0) return 0;
     const i64 nblk = blocks_for(n);
+    const int res = r != nullptr;
     if (ntau < 0 || ntau > AB_MAX_TAU || (ntau > 0 && nblk * ntau > part_cap)) return -3;
-#define BBJD_AB(R)                                                                                                  hipLaunchKernelGGL((k_band_ab<R>), dim3(nblk), dim3(TPB), 0, s, (i64)n, (i64)n1, (i64)nb, bs, hw,                                    (const Cx<R> *)band, (const Cx<R> *)dg, ntau, (const Cx<R> *)tauc, (const Cx<R> *)x,                              (Cx<R> *)ya, (Cx<R> *)yb, (Cx<R> *)part);                                                     if (ntau > 0)                                                                                                       hipLaunchKernelGGL((k_tau_finish<R>), dim3(1), dim3(64 * ntau), 0, s, (i64)nblk, ntau,                                              (const Cx<R> *)part, (Cx<R> *)ya + n1);
+    if (res && (single || ntau >= AB_MAX_TAU || !rpart || !rnorm || nblk > rpart_cap)) return -3;
+    const int fin_threads = 64 * (ntau + res);
+#define BBJD_AB(R)                                                                                              \
+    hipLaunchKernelGGL((k_band_ab<R>), dim3(nblk), dim3(TPB), 0, s, (i64)n, (i64)n1, (i64)nb, bs, hw,             \
+                       (const Cx<R> *)band, (const Cx<R> *)dg, ntau, (const Cx<R> *)tauc, (const Cx<R> *)x,       \
+                       (Cx<R> *)ya, (Cx<R> *)yb, (Cx<R> *)part, Cx<R>{(R)are, (R)aim}, Cx<R>{(R)bre, (R)bim},     \
+                       (Cx<R> *)r, (R *)rpart);                                                                 \
+    if (fin_threads > 0)                                                                                        \
+        hipLaunchKernelGGL((k_tau_finish<R>), dim3(1), dim3(fin_threads), 0, s, (i64)nblk, ntau, (i64)n1,        \
+                           (const Cx<R> *)part, (Cx<R> *)ya, (const Cx<R> *)yb, Cx<R>{(R)are, (R)aim},           \
+                           Cx<R>{(R)bre, (R)bim}, (Cx<R> *)r, (const R *)rpart, rnorm);
     if (single) { BBJD_AB(float) } else { BBJD_AB(double) }
 #undef BBJD_AB
     return chk();

@@ -14,6 +14,13 @@ int bbjd_o_band_mv(hipStream_t s, int64_t n1, int64_t nb, int bs, int hw, const 
 int bbjd_o_band_ab(hipStream_t s, int64_t n, int64_t n1, int64_t nb, int bs, int hw, const void *band,
                    const void *dg, int ntau, const void *tauc, const void *x, void *ya, void *yb, void *part,
                    int64_t part_cap, int single);
+/* bbjd_o_band_ab plus r = (are + i aim) y_A + (bre + i bim) y_B and ||r|| ->
+ * *rnorm (device double), with per-block |r|^2 partials in rpart
+ * (rpart_cap doubles); FP64 only, -3 when ntau > 15 */
+int bbjd_o_band_abr(hipStream_t s, int64_t n, int64_t n1, int64_t nb, int bs, int hw, const void *band,
+                    const void *dg, int ntau, const void *tauc, const void *x, void *ya, void *yb, void *part,
+                    int64_t part_cap, int single, double are, double aim, double bre, double bim, void *r,
+                    void *rpart, int64_t rpart_cap, double *rnorm);
 int bbjd_o_diag_mv(hipStream_t s, int64_t n, const void *dg, const void *x, void *y, int single);
 int bbjd_o_bj_apply(hipStream_t s, int64_t n, int bs, const void *inv, const void *x, void *y, int single);
 int bbjd_o_bcr_fwd(hipStream_t st, int s, int64_t M, int64_t h, int64_t ne, const void *AG, void *b, int nrhs,

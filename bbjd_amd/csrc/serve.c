@@ -204,6 +204,7 @@ int bbjd_serve(bbjd_request *reqs, int nreq, const bbjd_serve_config *cfg, bbjd_
     S.opt.precision = cfg->precision;
     S.opt.precond2 = bbjd_op_precond2;
     S.opt.matvec_AB = bbjd_op_matvec_AB;
+    S.opt.matvec_ABr = bbjd_op_matvec_ABr;
     S.opt.graphs = cfg->graphs;
     if (cfg->precision == BBJD_PREC_MIXED) {
         S.opt.matvec_A_c64 = bbjd_op_matvec_A_c64;

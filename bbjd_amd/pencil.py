@@ -174,6 +174,22 @@ class DeviceOperator:
             raise RuntimeError("bbjd_op_matvec_AB failed")
         return da.to_array(dt, self.n), db.to_array(dt, self.n)
 
+    def matvec_ABr(self, x, alpha, beta):
+        """(A x, B x, r = alpha A x + beta B x, ||r||) in one pass over x
+        (bbjd_op_matvec_ABr)"""
+        lib = _lib.load()
+        x = np.ascontiguousarray(x, dtype=np.complex128)
+        dx = _lib.DeviceBuffer.from_array(x)
+        da, db, dr = _lib.DeviceBuffer(x.nbytes), _lib.DeviceBuffer(x.nbytes), _lib.DeviceBuffer(x.nbytes)
+        dn = _lib.DeviceBuffer(8)
+        rc = lib.bbjd_op_matvec_ABr(self.n, dx.ptr, da.ptr, db.ptr, _lib.zc(alpha), _lib.zc(beta), dr.ptr, dn.ptr,
+                                    self.handle)
+        rc |= lib.bbjd_dev_sync()
+        if rc:
+            raise RuntimeError("bbjd_op_matvec_ABr failed")
+        return (da.to_array(np.complex128, self.n), db.to_array(np.complex128, self.n),
+                dr.to_array(np.complex128, self.n), float(dn.to_array(np.float64, 1)[0]))
+
     def solve2(self, x0, x1):
         """both preconditioner solves of one correction step in one pass
         (bbjd_op_precond2)"""

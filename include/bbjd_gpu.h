@@ -49,6 +49,13 @@ typedef int (*bbjd_matvec2_fn)(bbjd_int n, const bbjd_zomplex *x, bbjd_zomplex *
 typedef int (*bbjd_op2_c64_fn)(bbjd_int n, const bbjd_cfloat *x, bbjd_cfloat *ya,
                                bbjd_cfloat *yb, void *data);
 
+/* ya = A x, yb = B x and the residual r = alpha ya + beta yb with ||r||_2
+ * written to the device double *rnorm, in one call (device pointers); same
+ * return contract as bbjd_matvec_fn. */
+typedef int (*bbjd_residual_fn)(bbjd_int n, const bbjd_zomplex *x, bbjd_zomplex *ya, bbjd_zomplex *yb,
+                                bbjd_zomplex alpha, bbjd_zomplex beta, bbjd_zomplex *r, double *rnorm,
+                                void *data);
+
 typedef struct {
     bbjd_callback_space callback_space;
     bbjd_precision      precision;
@@ -81,6 +88,9 @@ typedef struct {
      * the same A and B as params.matvec_A / matvec_B.  NULL: two calls. */
     bbjd_matvec2_fn     matvec_AB;
     bbjd_op2_c64_fn     matvec_AB_c64;
+    /* optional, device callbacks only: the residual step's A u, B u and
+     * r = theta_b A u - theta_a B u with ||r|| in one call (same A, B). */
+    bbjd_residual_fn    matvec_ABr;
 } bbjd_gpu_options;
 
 enum { BBJD_QZ_RITZ = 0, BBJD_QZ_SCHUR = 1 };

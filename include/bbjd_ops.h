@@ -84,6 +84,11 @@ int bbjd_op_precond2(bbjd_int n, const bbjd_zomplex *x0, bbjd_zomplex *y0, const
  * diagonal of B and the dense rows' partial dots in one kernel. */
 int bbjd_op_matvec_AB(bbjd_int n, const bbjd_zomplex *x, bbjd_zomplex *ya, bbjd_zomplex *yb, void *op);
 
+/* bbjd_op_matvec_AB plus r = alpha ya + beta yb and ||r|| -> *rnorm (device)
+ * in the same pass and its finish (bbjd_residual_fn). */
+int bbjd_op_matvec_ABr(bbjd_int n, const bbjd_zomplex *x, bbjd_zomplex *ya, bbjd_zomplex *yb, bbjd_zomplex alpha,
+                       bbjd_zomplex beta, bbjd_zomplex *r, double *rnorm, void *op);
+
 /* complex64 forms (bbjd_op_c64_fn, bbjd_op2_c64_fn), available when want_c64
  * was set. */
 int bbjd_op_matvec_AB_c64(bbjd_int n, const bbjd_cfloat *x, bbjd_cfloat *ya, bbjd_cfloat *yb, void *op);

@@ -223,6 +223,15 @@ def test_fused_matvec_ab_matches_separate(n, bs, hw, ntau):
         sa, sb = dev.matvec_AB(x, single=True)
         assert np.linalg.norm(sa - ra_ref) <= 1e-5 * np.linalg.norm(ra_ref)
         assert np.linalg.norm(sb - rb_ref) <= 1e-6 * np.linalg.norm(rb_ref) + 1e-300
+        # with the residual of the convergence test, r = tb A x - ta B x and ||r||
+        tb, ta = 0.8 - 0.3j, 1.7 + 0.4j
+        qa, qb, r, rn = dev.matvec_ABr(x, tb, -ta)
+        np.testing.assert_array_equal(qa[:n1], ya[:n1])
+        np.testing.assert_array_equal(qb, yb)
+        assert np.linalg.norm(qa - ra_ref) <= 1e-12 * np.linalg.norm(ra_ref)
+        r_ref = tb * ra_ref - ta * rb_ref
+        assert np.linalg.norm(r - r_ref) <= 1e-12 * np.linalg.norm(r_ref)
+        assert abs(rn - np.linalg.norm(r_ref)) <= 1e-12 * np.linalg.norm(r_ref)
     finally:
         dev.free()
 
@@ -540,7 +549,14 @@ def test_serving_loop_results_alone_equal_mix():
     lpt, _ = srv.serve(jobs, workers=4, realtime=False, order="largest")
     budget = rep["peak_bytes"] // 2
     tight, rep_t = srv.serve(jobs, workers=4, realtime=False, hbm_budget=budget)
-    assert rep_t["peak_bytes"] <= budget and rep_t["failed"] == 0
+    # admission keeps within the budget, except that a request runs alone
+    # when it does not fit beside the others
+    ops = [DeviceOperator(j["pencil"], j["sigma"], pc="exact") for j in jobs]
+    largest_op = max(o.device_bytes for o in ops)
+    for o in ops:
+        o.free()
+    assert rep_t["failed"] == 0 and rep_t["peak_concurrency"] < rep["peak_concurrency"] + 1
+    assert rep_t["peak_bytes"] <= max(budget, rep_t["workers"] * rep_t["workspace_bytes"] + largest_op)
     print(f"[serve] mix peak {rep['peak_bytes']} concurrency {rep['peak_concurrency']}; tight budget {budget}: "
           f"workers {rep_t['workers']} peak {rep_t['peak_bytes']} concurrency {rep_t['peak_concurrency']}",
           flush=True)
