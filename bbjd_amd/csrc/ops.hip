@@ -370,7 +370,8 @@ struct BcrLevels { const Cx<double> *f[4]; };
 
 template <int NR, int S, int G>
 __global__ __launch_bounds__(TPB) void k_bcr_bwd_multi(i64 N, int l, BcrLevels dpq, BcrLevels bl, Cx<double> *x,
-                                                       i64 vs)
+                                                       i64 vs, const Cx<double> *__restrict__ tauc, int ntau, i64 n1,
+                                                       i64 ldt, Cx<double> *__restrict__ dpart)
 {
     constexpr int U = TPB / ((1 << (G - 1)) * S);
     constexpr int SLOTS = U * (1 << G) + 1;
@@ -426,6 +427,48 @@ __global__ __launch_bounds__(TPB) void k_bcr_bwd_multi(i64 N, int l, BcrLevels d
             }
         }
         __syncthreads();
+    }
+    // with tauc (the group that ends at level 0, where the block's LDS holds
+    // every row of its units): partial dots conj(tauc[:, t]) . x over the
+    // block's rows below n1, per (t, q), for the dense rows' border
+    if (tauc) {
+        __shared__ Cx<double> dred[16][NR][TPB / 64];
+        const int lane = t & 63, w = t >> 6;
+        for (int tt = 0; tt < ntau; tt++) {
+            Cx<double> p[NR];
+#pragma unroll
+            for (int q = 0; q < NR; q++) p[q] = cx0();
+            for (int e = t; e < U * (1 << G) * S; e += TPB) {
+                const int slot = e / S, r = e % S;
+                const i64 el = ((u0 << G) + slot) * S + r;
+                if (el < n1 && ((u0 << G) + slot) < N) {
+                    const Cx<double> tq = tauc[el + (i64)tt * ldt];
+#pragma unroll
+                    for (int q = 0; q < NR; q++) {
+                        const Cx<double> v = lx[q][slot][r];
+                        p[q].re += tq.re * v.re + tq.im * v.im;
+                        p[q].im += tq.re * v.im - tq.im * v.re;
+                    }
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < NR; q++) {
+#pragma unroll
+                for (int o = 32; o >= 1; o >>= 1) {
+                    p[q].re += __shfl_xor(p[q].re, o, 64);
+                    p[q].im += __shfl_xor(p[q].im, o, 64);
+                }
+                if (lane == 0) dred[tt][q][w] = p[q];
+            }
+        }
+        __syncthreads();
+        if (t < ntau * NR) {
+            const int tt = t / NR, q = t % NR;
+            Cx<double> s = dred[tt][q][0];
+#pragma unroll
+            for (int v = 1; v < TPB / 64; v++) { s.re += dred[tt][q][v].re; s.im += dred[tt][q][v].im; }
+            dpart[(i64)blockIdx.x * ntau * NR + t] = s;
+        }
     }
 }
 
@@ -705,18 +748,40 @@ __global__ __launch_bounds__(1024) void k_bcr_tail_lds(i64 N, int l0, int L, con
 }
 
 // border, vector q: t = b2_q - dots_q; y2_q = Sinv t -> y2out + q ntau and the
-// tail of y_q.  One workgroup, NR * ntau threads used.
+// tail of y_q.  One workgroup, NR * ntau threads used.  With dpart the dots
+// are first summed from nblk per-block partials (layout [block][a][q]), wave
+// o handling outputs o, o + waves, ... in a fixed order.
 template <int NR>
 __global__ void k_border_small(int ntau, const Cx<double> *b2_0, const Cx<double> *b2_1, const Cx<double> *dots0,
                                const Cx<double> *dots1, const Cx<double> *__restrict__ Sinv, Cx<double> *y2out,
-                               Cx<double> *ytail0, Cx<double> *ytail1)
+                               Cx<double> *ytail0, Cx<double> *ytail1, const Cx<double> *__restrict__ dpart,
+                               i64 nblk)
 {
     extern __shared__ char sm[];
     Cx<double> *tt = reinterpret_cast<Cx<double> *>(sm);
+    __shared__ Cx<double> dd[2 * 16];
     const int a = threadIdx.x % ntau, q = threadIdx.x / ntau;
     const bool act = q < NR;
+    if (dpart) {
+        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+        for (int o = wv; o < ntau * NR; o += nw) {
+            Cx<double> s = cx0();
+            for (i64 b = lane; b < nblk; b += 64) {
+                const Cx<double> v = dpart[b * ntau * NR + o];
+                s.re += v.re; s.im += v.im;
+            }
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) {
+                s.re += __shfl_xor(s.re, off, 64);
+                s.im += __shfl_xor(s.im, off, 64);
+            }
+            if (lane == 0) dd[o] = s;       // o = a' NR + q'
+        }
+        __syncthreads();
+    }
     if (act) {
-        const Cx<double> b2 = (q == 0 ? b2_0 : b2_1)[a], d = (q == 0 ? dots0 : dots1)[a];
+        const Cx<double> b2 = (q == 0 ? b2_0 : b2_1)[a];
+        const Cx<double> d = dpart ? dd[a * NR + q] : (q == 0 ? dots0 : dots1)[a];
         tt[q * ntau + a] = cx(b2.re - d.re, b2.im - d.im);
     }
     __syncthreads();
@@ -899,20 +964,28 @@ int bbjd_o_bcr_bwd(hipStream_t st, int s, int64_t M, int64_t h, int64_t no, cons
 static inline bool bcr_group_ok(int s, int G) { return (s == 4 || s == 8 || s == 16) && G >= 2 && G <= 4; }
 
 int bbjd_o_bcr_bwd_multi(hipStream_t st, int s, int64_t N, int l, int G, const void *const *dpq,
-                         const void *const *bl, void *x, int nrhs, int64_t vs)
+                         const void *const *bl, void *x, int nrhs, int64_t vs, const void *tauc, int ntau,
+                         int64_t n1, int64_t ldt, void *dpart, int64_t dpart_cap, int64_t *nblk_out)
 {
     if (!bcr_group_ok(s, G) || (nrhs != 1 && nrhs != 2)) return -3;
+    if (nblk_out) *nblk_out = 0;
     BcrLevels f, bb;
     for (int g = 0; g < 4; g++) {
         f.f[g] = (const Cx<double> *)(g < G ? dpq[g] : nullptr);
         bb.f[g] = (const Cx<double> *)(g < G ? bl[g] : nullptr);
     }
     const i64 H = (i64)1 << (l + G), units = (N + H - 1) / H;
+    // the dense rows' partial dots: only from the group that ends at level 0
+    const bool dots = tauc && l == 0 && ntau > 0 && ntau <= 16 && dpart;
 #define BBJD_BWDM(NR, S, GG)                                                                                   \
     {                                                                                                          \
         constexpr int U = TPB / ((1 << (GG - 1)) * S);                                                         \
-        hipLaunchKernelGGL((k_bcr_bwd_multi<NR, S, GG>), dim3((unsigned)((units + U - 1) / U)), dim3(TPB), 0, st, \
-                           (i64)N, l, f, bb, (Cx<double> *)x, (i64)vs);                                        \
+        const i64 nblk = (units + U - 1) / U;                                                                  \
+        const bool dd = dots && nblk * ntau * NR <= dpart_cap;                                                 \
+        hipLaunchKernelGGL((k_bcr_bwd_multi<NR, S, GG>), dim3((unsigned)nblk), dim3(TPB), 0, st, (i64)N, l, f, bb, \
+                           (Cx<double> *)x, (i64)vs, dd ? (const Cx<double> *)tauc : nullptr, ntau, (i64)n1,     \
+                           (i64)ldt, (Cx<double> *)dpart);                                                     \
+        if (dd && nblk_out) *nblk_out = nblk;                                                                  \
     }
     BBJD_BCR_GROUP_DISPATCH(BBJD_BWDM)
 #undef BBJD_BWDM
@@ -989,20 +1062,21 @@ int bbjd_o_bcr_tail(hipStream_t st, int s, int64_t N, int l0, int L, const int64
 }
 
 int bbjd_o_border_small(hipStream_t st, int ntau, const void *b2_0, const void *b2_1, const void *dots0,
-                        const void *dots1, const void *Sinv, void *y2, void *ytail0, void *ytail1, int nrhs)
+                        const void *dots1, const void *Sinv, void *y2, void *ytail0, void *ytail1, int nrhs,
+                        const void *dpart, int64_t nblk)
 {
-    if (ntau <= 0 || ntau * nrhs > 1024) return -2;
-    const int thr = ntau * nrhs <= 64 ? 64 : (ntau * nrhs <= 256 ? 256 : 1024);
+    if (ntau <= 0 || ntau * nrhs > 1024 || (dpart && ntau > 16)) return -2;
+    const int thr = dpart ? 1024 : ntau * nrhs <= 64 ? 64 : (ntau * nrhs <= 256 ? 256 : 1024);
     if (nrhs == 2)
         hipLaunchKernelGGL(k_border_small<2>, dim3(1), dim3(thr), (size_t)2 * ntau * 16, st, ntau,
                            (const Cx<double> *)b2_0, (const Cx<double> *)b2_1, (const Cx<double> *)dots0,
                            (const Cx<double> *)dots1, (const Cx<double> *)Sinv, (Cx<double> *)y2,
-                           (Cx<double> *)ytail0, (Cx<double> *)ytail1);
+                           (Cx<double> *)ytail0, (Cx<double> *)ytail1, (const Cx<double> *)dpart, (i64)nblk);
     else
         hipLaunchKernelGGL(k_border_small<1>, dim3(1), dim3(thr), (size_t)ntau * 16, st, ntau,
                            (const Cx<double> *)b2_0, (const Cx<double> *)b2_0, (const Cx<double> *)dots0,
                            (const Cx<double> *)dots0, (const Cx<double> *)Sinv, (Cx<double> *)y2,
-                           (Cx<double> *)ytail0, (Cx<double> *)ytail0);
+                           (Cx<double> *)ytail0, (Cx<double> *)ytail0, (const Cx<double> *)dpart, (i64)nblk);
     return chk();
 }
 

@@ -33,12 +33,15 @@ int bbjd_o_bcr_bwd(hipStream_t st, int s, int64_t M, int64_t h, int64_t no, cons
                    int nrhs, int64_t vs);
 /* G (2..4) consecutive levels in one launch, for s in {4, 8, 16}; -3 for
  * other shapes.  Backward levels l + G - 1 .. l (dpq[g]: level l + g's odd
- * factor blocks, bl[g]: the buffer holding its right-hand sides); forward
+ * factor blocks, bl[g]: the buffer holding its right-hand sides; with tauc
+ * and l = 0 also per-block partial dots conj(tauc[:, t]) . x over the rows
+ * below n1 into dpart, their block count into *nblk_out, 0 when not made); forward
  * levels l .. l + G - 1 (ag[g]: level l + g's even factor blocks) from the
  * caller's x0 (x1) (l = 0) or from bin into bout (another buffer), writing
  * every row at a multiple of 2^l. */
 int bbjd_o_bcr_bwd_multi(hipStream_t st, int s, int64_t N, int l, int G, const void *const *dpq,
-                         const void *const *bl, void *x, int nrhs, int64_t vs);
+                         const void *const *bl, void *x, int nrhs, int64_t vs, const void *tauc, int ntau,
+                         int64_t n1, int64_t ldt, void *dpart, int64_t dpart_cap, int64_t *nblk_out);
 int bbjd_o_bcr_fwd_multi(hipStream_t st, int s, int64_t N, int64_t n1, int l, int G, const void *const *ag,
                          const void *x0, const void *x1, const void *bin, void *bout, int nrhs, int64_t vs);
 /* first level of the one-workgroup tail for this shape (>= l0, where the
@@ -46,8 +49,11 @@ int bbjd_o_bcr_fwd_multi(hipStream_t st, int s, int64_t N, int64_t n1, int l, in
 int bbjd_o_bcr_tail_level(int s, int64_t N, int l0, int L, int nrhs);
 int bbjd_o_bcr_tail(hipStream_t st, int s, int64_t N, int l0, int L, const int64_t *off, const void *F, void *b,
                     void *x, int nrhs, int64_t vs);
+/* the border's y2 = Sinv (b2 - dots); with dpart (nblk x ntau x nrhs
+ * partials from bbjd_o_bcr_bwd_multi) the dots are summed from it first */
 int bbjd_o_border_small(hipStream_t st, int ntau, const void *b2_0, const void *b2_1, const void *dots0,
-                        const void *dots1, const void *Sinv, void *y2, void *ytail0, void *ytail1, int nrhs);
+                        const void *dots1, const void *Sinv, void *y2, void *ytail0, void *ytail1, int nrhs,
+                        const void *dpart, int64_t nblk);
 int bbjd_o_border_update(hipStream_t st, int64_t n1, int ntau, const void *xb, int64_t vs, const void *Y,
                          const void *y2, void *y0, void *y1, int nrhs);
 #ifdef __cplusplus

@@ -258,6 +258,10 @@ static hipStream_t op_stream(const struct bbjd_op *op)
 /* Levels per fused BCR launch (1: one launch per level). */
 static int g_bcr_group = 4;
 void bbjd_set_bcr_group(int levels) { g_bcr_group = levels < 1 ? 1 : (levels > 4 ? 4 : levels); }
+/* 1: the last backward group also forms the dense rows' partial dots for the
+ * Schur border (no separate projection and finish) */
+static int g_bcr_border_fuse = 1;
+void bbjd_set_bcr_border_fuse(int on) { g_bcr_border_fuse = on != 0; }
 
 /* BCR solve of nrhs right-hand sides into op->xvec: the large levels in
  * launches of up to g_bcr_group levels, the levels from the tail level on,
@@ -272,8 +276,10 @@ void bbjd_set_bcr_group(int levels) { g_bcr_group = levels < 1 ? 1 : (levels > 4
  * and bvec2; the odd rows of level l are final once its group has run and
  * stay in the buffer it wrote (later groups write only multiples of higher
  * powers of two), which the backward level reads (bl[l]). */
-static int bcr_solve(struct bbjd_op *op, hipStream_t st, int nrhs, const void *x0, const void *x1)
+static int bcr_solve(struct bbjd_op *op, hipStream_t st, int nrhs, const void *x0, const void *x1, int want_dots,
+                     int64_t *dots_nblk)
 {
+    if (dots_nblk) *dots_nblk = 0;
     const int s = op->s;
     const size_t ss = (size_t)s * s;
     const int64_t vs = (int64_t)op->N * s;
@@ -334,7 +340,10 @@ static int bcr_solve(struct bbjd_op *op, hipStream_t st, int nrhs, const void *x
                 lv[g] = (const zcx *)op->bcr + (size_t)op->odd_off[lo + g] * ss;
                 lb[g] = buf[bl[lo + g]];
             }
-            const int e = bbjd_o_bcr_bwd_multi(st, s, op->N, lo, G, lv, lb, op->xvec, nrhs, vs);
+            const int dots = want_dots && lo == 0;
+            const int e = bbjd_o_bcr_bwd_multi(st, s, op->N, lo, G, lv, lb, op->xvec, nrhs, vs,
+                                               dots ? op->tauc : NULL, op->ntau, op->n1, op->n, op->rd.part,
+                                               op->rd.part_cap, dots ? dots_nblk : NULL);
             if (e == 0) { l = lo - 1; continue; }
             if (e != -3) { rc |= e; break; }
         }
@@ -353,7 +362,8 @@ static int exact_apply(struct bbjd_op *op, hipStream_t st, int nrhs, const void 
     int rc = 0;
     const bbjd_int n1 = op->n1;
     const int64_t vs = (int64_t)op->N * op->s;
-    rc |= bcr_solve(op, st, nrhs, x0, x1);
+    int64_t dnb = 0;
+    rc |= bcr_solve(op, st, nrhs, x0, x1, op->ntau > 0 && g_bcr_border_fuse, &dnb);
     if (op->ntau == 0) {
         rc |= !HIPOK(hipMemcpyAsync(y0, op->xvec, (size_t)n1 * 16, hipMemcpyDeviceToDevice, st));
         if (nrhs == 2)
@@ -362,10 +372,12 @@ static int exact_apply(struct bbjd_op *op, hipStream_t st, int nrhs, const void 
     }
     const int nt = op->ntau;
     zcx *sm = (zcx *)op->small;   /* dots 2 nt | y2 2 nt */
-    rc |= bbjd_k_gemv_c(st, n1, nt, op->tauc, op->n, op->xvec, nrhs == 2 ? (zcx *)op->xvec + vs : NULL, sm,
-                        sm + nt, &op->rd);
+    if (!dnb)
+        rc |= bbjd_k_gemv_c(st, n1, nt, op->tauc, op->n, op->xvec, nrhs == 2 ? (zcx *)op->xvec + vs : NULL, sm,
+                            sm + nt, &op->rd);
     rc |= bbjd_o_border_small(st, nt, (const zcx *)x0 + n1, nrhs == 2 ? (const zcx *)x1 + n1 : NULL, sm, sm + nt,
-                              op->Sinv, sm + 2 * nt, (zcx *)y0 + n1, nrhs == 2 ? (zcx *)y1 + n1 : NULL, nrhs);
+                              op->Sinv, sm + 2 * nt, (zcx *)y0 + n1, nrhs == 2 ? (zcx *)y1 + n1 : NULL, nrhs,
+                              dnb ? op->rd.part : NULL, dnb);
     rc |= bbjd_o_border_update(st, n1, nt, op->xvec, vs, op->Y, sm + 2 * nt, y0, y1, nrhs);
     return rc;
 }
@@ -577,7 +589,7 @@ static bbjd_op *op_create_impl(const bbjd_pencil_desc *d, bbjd_zomplex sigma, bb
                 memset(col, 0, (size_t)n1 * sizeof(zcx));
                 for (bbjd_int i = lo; i < n1; i++) col[i] = m_at(d, sigma, i, n1 + a);
                 rc |= !HIPOK(hipMemcpy(op->bvec, col, (size_t)n1 * 16, hipMemcpyHostToDevice));
-                rc |= bcr_solve(op, st, 1, NULL, NULL);
+                rc |= bcr_solve(op, st, 1, NULL, NULL, 0, NULL);
                 rc |= !HIPOK(hipMemcpyAsync((zcx *)op->Y + (size_t)a * n1, op->xvec, (size_t)n1 * 16,
                                             hipMemcpyDeviceToDevice, st));
                 /* M21 Y[:, a] */

@@ -103,6 +103,10 @@ void bbjd_set_bcr_group(int levels);
  * with at most this many rows (default 64); applies to operators created
  * afterwards.  Tuning. */
 void bbjd_set_bcr_tail_rows(int rows);
+/* 1 (default): the exact solve's last backward reduction launch also forms
+ * the dense rows' dots of the Schur border; 0: a separate projection.
+ * Tuning and tests. */
+void bbjd_set_bcr_border_fuse(int on);
 
 #ifdef __cplusplus
 }

@@ -242,7 +242,9 @@ def test_bcr_level_groups_match_level_by_level(n, bs, hw, ntau):
     launch (bbjd_set_bcr_group; super-blocks of 8, 16 and 4 rows, the last
     one ragged) equals the one-launch-per-level solve exactly -- the same
     row arithmetic in the same order -- for one and two right-hand sides,
-    and solves M y = x as the oracle's banded LU does"""
+    and solves M y = x as the oracle's banded LU does; with the Schur
+    border's dots formed in the last backward launch
+    (bbjd_set_bcr_border_fuse; another summation order) to 1e-13"""
     lib = _lib.load()
     p = BandedTauPencil.generate(n, bs, hw, ntau, seed=21)
     sigma = 0.3 - 0.2j
@@ -251,6 +253,7 @@ def test_bcr_level_groups_match_level_by_level(n, bs, hw, ntau):
     x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
     x2 = rng.standard_normal(n) - 1j * rng.standard_normal(n)
     try:
+        lib.bbjd_set_bcr_border_fuse(0)
         lib.bbjd_set_bcr_group(1)
         y1 = dev.solve(x)
         z0, z1 = dev.solve2(x, x2)
@@ -260,8 +263,16 @@ def test_bcr_level_groups_match_level_by_level(n, bs, hw, ntau):
             a, b = dev.solve2(x, x2)
             np.testing.assert_array_equal(a, z0)
             np.testing.assert_array_equal(b, z1)
+        lib.bbjd_set_bcr_border_fuse(1)
+        for g in (2, 4):
+            lib.bbjd_set_bcr_group(g)
+            assert np.linalg.norm(dev.solve(x) - y1) <= 1e-13 * np.linalg.norm(y1)
+            a, b = dev.solve2(x, x2)
+            assert np.linalg.norm(a - z0) <= 1e-13 * np.linalg.norm(z0)
+            assert np.linalg.norm(b - z1) <= 1e-13 * np.linalg.norm(z1)
     finally:
         lib.bbjd_set_bcr_group(4)
+        lib.bbjd_set_bcr_border_fuse(1)
         dev.free()
     cpu = orc.CpuPencilOps(p, sigma, 1)
     r1, r2 = cpu.apply("orc_precond", x), cpu.apply("orc_precond", x2)
