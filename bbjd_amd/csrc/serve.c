@@ -114,6 +114,7 @@ static void run_request(server_t *S, worker_t *w, bbjd_request *q)
     p.matvec_B = bbjd_op_matvec_B;
     p.precond = bbjd_op_precond;
     p.op_data = op;
+    p.v0 = q->v0;
     bbjd_result R;
     memset(&R, 0, sizeof(R));
     q->status = bbjd_solve_in(&p, w->ws, &R);

@@ -2,11 +2,11 @@
 
 ``serve(jobs, ...)`` runs a list of independent solves through the C
 serving loop: W worker threads, each with its own stream and resident
-workspace, admitting requests in arrival order.  A job is a dict with
-``op`` (a DeviceOperator already resident on the GPU) or ``pencil`` +
-``pc`` (the server builds the operator at admission), ``sigma``, ``nev``
-and optionally maxsub, minsub, maxiter, inner_maxiter, tol, inner_tol,
-arrival (seconds).
+workspace, admitting requests in arrival order (or largest first).  A job
+is a dict with ``op`` (a DeviceOperator already resident on the GPU) or
+``pencil`` + ``pc`` (the server builds the operator at admission),
+``sigma``, ``nev`` and optionally maxsub, minsub, maxiter, inner_maxiter,
+tol, inner_tol, arrival (seconds) and v0 (a start vector of length n).
 """
 from __future__ import annotations
 
@@ -27,7 +27,7 @@ class Request(C.Structure):
                 ("eigenvectors", C.c_void_p), ("nconv", C.c_int64), ("iters", C.c_int64),
                 ("status", C.c_int), ("worker", C.c_int), ("t_admit", C.c_double),
                 ("t_start", C.c_double), ("t_done", C.c_double), ("qz_seconds", C.c_double),
-                ("stream_bytes", C.c_double)]
+                ("stream_bytes", C.c_double), ("v0", C.c_void_p)]
 
 
 class ServeConfig(C.Structure):
@@ -99,6 +99,12 @@ def serve(jobs, workers=16, precision="fp64", realtime=True, hbm_budget=0, keep_
         q.tol = float(j.get("tol", 1e-10))
         q.inner_tol = float(j.get("inner_tol", 1e-2))
         q.arrival = float(j.get("arrival", 0.0))
+        if j.get("v0") is not None:
+            v0 = np.ascontiguousarray(j["v0"], dtype=np.complex128)
+            if v0.shape != (n,):
+                raise ValueError(f"v0 must have shape ({n},)")
+            keep.append(v0)
+            q.v0 = v0.ctypes.data
         o = {"eigenvalues": np.zeros(nev, np.complex128), "residuals": np.zeros(nev),
              "pair_residuals": np.zeros(nev)}
         if keep_vectors:

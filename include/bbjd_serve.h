@@ -41,6 +41,8 @@ typedef struct {
     double                  qz_seconds;
     double                  stream_bytes;  /* algorithmic HBM bytes of the engine's
                                               n-sized kernels (bbjd_stats) */
+    const bbjd_zomplex     *v0;            /* host start vector (n) or NULL:
+                                              bbjd_params.v0 of the solve    */
 } bbjd_request;
 
 typedef struct {

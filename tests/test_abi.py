@@ -116,12 +116,13 @@ def test_gpu_struct_layouts_match_ctypes(tmp_path):
     from bbjd_amd import serve
     src = tmp_path / "gprobe.c"
     exe = tmp_path / "gprobe"
-    src.write_text(GPU_PROBE)
+    req = "".join(f"  O(bbjd_request, {f});\n" for f, _ in serve.Request._fields_)
+    src.write_text(GPU_PROBE.replace("  return 0;", "  S(bbjd_request);\n" + req + "  return 0;"))
     subprocess.run(["gcc", "-std=c11", f"-I{INC}", str(src), "-o", str(exe)], check=True)
     out = dict(line.rsplit(" ", 1) for line in
                subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.splitlines())
     for cname, cls in (("bbjd_gpu_options", _lib.GpuOptions), ("bbjd_stats", _lib.Stats),
-                       ("bbjd_serve_config", serve.ServeConfig)):
+                       ("bbjd_serve_config", serve.ServeConfig), ("bbjd_request", serve.Request)):
         assert int(out[f"sizeof.{cname}"]) == C.sizeof(cls), cname
         for f, _ in cls._fields_:
             assert int(out[f"{cname}.{f}"]) == getattr(cls, f).offset, (cname, f)
