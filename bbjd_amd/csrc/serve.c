@@ -20,6 +20,7 @@
 #include <math.h>
 #include <omp.h>
 #include <pthread.h>
+#include <sched.h>
 #include <stdlib.h>
 #include <string.h>
 #include <time.h>
@@ -51,6 +52,7 @@ typedef struct server {
     bbjd_params       cap;
     bbjd_gpu_options  opt;
     int               init_failed;
+    int               pin;              /* workers pinned to one CPU each */
 } server_t;
 
 static double now_s(void)
@@ -62,7 +64,8 @@ static double now_s(void)
 
 int bbjd_device_init(int sync_yield)
 {
-    if (sync_yield) hipSetDeviceFlags(hipDeviceScheduleYield);
+    if (sync_yield == 2) hipSetDeviceFlags(hipDeviceScheduleBlockingSync);
+    else if (sync_yield) hipSetDeviceFlags(hipDeviceScheduleYield);
     hipFree(0);
     return 0;
 }
@@ -144,6 +147,21 @@ static void *worker_main(void *arg)
      * the parallelism */
     openblas_set_num_threads_local(1);
     omp_set_num_threads(1);
+    if (S->pin) {
+        /* worker i on the i-th CPU this process may use */
+        cpu_set_t allowed, one;
+        if (sched_getaffinity(0, sizeof(allowed), &allowed) == 0) {
+            const int cnt = CPU_COUNT(&allowed);
+            int want = cnt > 0 ? w->id % cnt : 0, seen = 0;
+            for (int c = 0; c < CPU_SETSIZE && cnt > 0; c++)
+                if (CPU_ISSET(c, &allowed) && seen++ == want) {
+                    CPU_ZERO(&one);
+                    CPU_SET(c, &one);
+                    (void)pthread_setaffinity_np(pthread_self(), sizeof(one), &one);
+                    break;
+                }
+        }
+    }
     w->ws = bbjd_workspace_alloc_gpu(&S->cap, &S->opt);
     pthread_mutex_lock(&S->mu);
     w->ok = w->ws != NULL;
@@ -178,7 +196,9 @@ int bbjd_serve(bbjd_request *reqs, int nreq, const bbjd_serve_config *cfg, bbjd_
 {
     if (!reqs || nreq <= 0 || !cfg || !rep) return -1;
     memset(rep, 0, sizeof(*rep));
-    if (cfg->sync_yield) hipSetDeviceFlags(hipDeviceScheduleYield);   /* ignored once the device is up */
+    /* ignored once the device is up */
+    if (cfg->sync_yield == 2) hipSetDeviceFlags(hipDeviceScheduleBlockingSync);
+    else if (cfg->sync_yield) hipSetDeviceFlags(hipDeviceScheduleYield);
     server_t S;
     memset(&S, 0, sizeof(S));
     pthread_mutex_init(&S.mu, NULL);
@@ -186,6 +206,7 @@ int bbjd_serve(bbjd_request *reqs, int nreq, const bbjd_serve_config *cfg, bbjd_
     pthread_cond_init(&S.cv_free, NULL);
     S.reqs = reqs;
     S.nreq = nreq;
+    S.pin = cfg->pin;
     /* capacity of the resident workspaces: the largest request */
     bbjd_params_init(&S.cap);
     S.cap.n = 1; S.cap.nev = 1; S.cap.maxsub = 2; S.cap.inner_maxiter = 0;

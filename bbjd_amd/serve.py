@@ -32,7 +32,8 @@ class Request(C.Structure):
 
 class ServeConfig(C.Structure):
     _fields_ = [("workers", C.c_int), ("hbm_budget", C.c_size_t), ("precision", C.c_int),
-                ("realtime", C.c_int), ("sync_yield", C.c_int), ("graphs", C.c_int), ("order", C.c_int)]
+                ("realtime", C.c_int), ("sync_yield", C.c_int), ("graphs", C.c_int), ("order", C.c_int),
+                ("pin", C.c_int)]
 
 
 class ServeReport(C.Structure):
@@ -61,11 +62,11 @@ def _setup(lib):
 def device_init(sync_yield=True):
     lib = _lib.load()
     _setup(lib)
-    return lib.bbjd_device_init(int(sync_yield))
+    return lib.bbjd_device_init(2 if sync_yield == "block" else int(bool(sync_yield)))
 
 
 def serve(jobs, workers=16, precision="fp64", realtime=True, hbm_budget=0, keep_vectors=False,
-          sync_yield=True, graphs=False, order="arrival"):
+          sync_yield=True, graphs=False, order="arrival", pin=False):
     """Run jobs through the serving loop; returns (results, report).  graphs:
     capture each request's inner GMRES steps into HIP graphs and replay them.
     order: which arrived request a free worker takes, "arrival" (first come,
@@ -120,9 +121,10 @@ def serve(jobs, workers=16, precision="fp64", realtime=True, hbm_budget=0, keep_
     cfg.hbm_budget = int(hbm_budget)
     cfg.precision = _lib.PREC_MIXED if precision == "mixed" else _lib.PREC_FP64
     cfg.realtime = int(bool(realtime))
-    cfg.sync_yield = int(bool(sync_yield))
+    cfg.sync_yield = 2 if sync_yield == "block" else int(bool(sync_yield))
     cfg.graphs = int(bool(graphs))
     cfg.order = 1 if order == "largest" else 0
+    cfg.pin = int(bool(pin))
     rep = ServeReport()
     rc = lib.bbjd_serve(reqs, nreq, C.byref(cfg), C.byref(rep))
     if rc:

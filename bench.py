@@ -126,6 +126,9 @@ def main():
     ap.add_argument("--no-tile-table", action="store_true", help="engine default tiles instead of the autotuned table")
     ap.add_argument("--order", choices=["arrival", "largest"], default="arrival",
                     help="which arrived request a free worker takes (serving loop)")
+    ap.add_argument("--sync", choices=["yield", "block"], default="yield",
+                    help="how host threads wait on the GPU")
+    ap.add_argument("--pin", action="store_true", help="pin each serving worker to one CPU")
     ap.add_argument("--seed", type=int, default=20261203)
     args = ap.parse_args()
 
@@ -137,7 +140,7 @@ def main():
     from bbjd_amd import BandedTauPencil, DeviceOperator
     from bbjd_amd import serve as srv
 
-    srv.device_init(sync_yield=True)
+    srv.device_init(sync_yield="block" if args.sync == "block" else True)
     # the committed per-shape tile table of this device, when there is one
     from bbjd_amd import autotune
     tt = None if args.no_tile_table else autotune.apply()
@@ -176,7 +179,8 @@ def main():
     srv.ktime_enable(True)
     for s in range(0 if args.plain_only else args.steps):
         t = time.perf_counter()
-        res, rep = srv.serve(jobs_for(all_idx), workers=args.workers, realtime=True, order=args.order)
+        res, rep = srv.serve(jobs_for(all_idx), workers=args.workers, realtime=True, order=args.order,
+                             pin=args.pin, sync_yield="block" if args.sync == "block" else True)
         walls.append(time.perf_counter() - t)
         served.append((res, rep))
         log(f"[bench] step {s}: {rep['pairs']} pairs in {walls[-1]:.1f}s, workers {rep['workers']}, "
@@ -302,7 +306,8 @@ def main():
                                f"{args.rate}/s, n in {{2e4,5e4,1e5,2e5}} (bs 8 block-tridiagonal, "
                                f"4 tau rows), k in {{8,16,32}}, sigma in [-4,4]+[-4,4]i, tol 1e-10, "
                                f"maxsub 4k, minsub 2k, maxiter {args.maxiter}, exact banded LU",
-                   "workers": args.workers, "requests": args.nreq, "admission": args.order},
+                   "workers": args.workers, "requests": args.nreq, "admission": args.order,
+                   "host_wait": args.sync, "workers_pinned": bool(args.pin)},
         "served_eigenpairs_per_s": round(served_rate, 3) if served_rate else None,
         "plain_eigenpairs_per_s": round(plain_rate, 3) if plain_rate else None,
         "served_over_plain": round(served_rate / plain_rate, 2) if served_rate and plain_rate else None,

@@ -145,8 +145,9 @@ int bbjd_workspace_stats(const bbjd_workspace *ws, bbjd_stats *st);
 
 /* First GPU call of a process that serves many solves from many host
  * threads: sync_yield = 1 makes threads waiting on the GPU yield their core
- * (the host QZ of other solves needs it).  Returns 0 (also when the device
- * was already initialised and the flag could not change). */
+ * (the host QZ of other solves needs it), 2 makes them block until the GPU
+ * signals.  Returns 0 (also when the device was already initialised and the
+ * flag could not change). */
 int bbjd_device_init(int sync_yield);
 
 /* Launch timing of the engine's dominant streaming kernel (the one-vector

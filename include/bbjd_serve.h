@@ -51,12 +51,15 @@ typedef struct {
                              (0: 90% of the free device memory at start)     */
     int    precision;     /* bbjd_precision of every request                 */
     int    realtime;      /* 1: admit a request no earlier than its arrival  */
-    int    sync_yield;    /* 1: host threads yield while waiting on the GPU  */
+    int    sync_yield;    /* 1: host threads yield while waiting on the GPU,
+                             2: they block (sleep until an interrupt)        */
     int    graphs;        /* bbjd_gpu_options.graphs of every request        */
     int    order;         /* which arrived request a free worker takes:
                              BBJD_ORDER_ARRIVAL (first come, first served)
                              or BBJD_ORDER_LARGEST (largest estimated work:
                              nev, then n)                                    */
+    int    pin;           /* 1: worker i runs on the i-th CPU the process may
+                             use                                             */
 } bbjd_serve_config;
 
 enum { BBJD_ORDER_ARRIVAL = 0, BBJD_ORDER_LARGEST = 1 };
