@@ -31,8 +31,15 @@ in HBM before the timed region; the timed region is the serving loop.
                else the oracle restatement) on the host cores, on a bounded
                sample of the workload's requests, CPU exact banded LU
 
+--workload small: the batched small solves instead -- 2048 pencils with n
+log-uniform on [4096, 32768] (block size 4, block-tridiagonal, 2 tau rows),
+k = 8, maxsub 32, minsub 16, sigma uniform on the unit disk, GMRES(20) in
+complex64 with the exact banded LU as its preconditioner, all submitted at
+once; plain = a random sample of them solved one at a time.
+
 Usage: python bench.py [--gpus 1] [--steps 1] [--warmup 1] [--nreq 256]
        python bench.py --plain-only ...   (the plain leg alone: for profilers)
+       python bench.py --workload small [--nreq 2048]
 One JSON line on stdout; progress on stderr.
 """
 from __future__ import annotations
@@ -66,6 +73,26 @@ def make_requests(nreq, rate, seed):
                  pseed=int(pseeds[i])) for i in range(nreq)]
 
 
+def make_small_requests(nreq, seed):
+    rng = np.random.default_rng(seed)
+    ns = np.exp(rng.uniform(np.log(4096), np.log(32768), size=nreq))
+    ns = (np.round(ns / 4) * 4).astype(int)
+    rad = np.sqrt(rng.uniform(0, 1, size=nreq))
+    sig = rad * np.exp(1j * rng.uniform(0, 2 * np.pi, size=nreq))
+    pseeds = rng.integers(1, 2**62, size=nreq)
+    return [dict(n=int(ns[i]), k=8, sigma=complex(sig[i]), arrival=0.0, pseed=int(pseeds[i]))
+            for i in range(nreq)]
+
+
+# per workload: pencil (bs, hw, ntau), solver settings, precision, realtime
+WORKLOADS = {
+    "serving": dict(shape=(8, 1, 4), inner=0, inner_tol=1e-2, precision="fp64", realtime=True,
+                    maxsub=lambda k: 4 * k, minsub=lambda k: 2 * k),
+    "small": dict(shape=(4, 1, 2), inner=20, inner_tol=1e-3, precision="mixed", realtime=False,
+                  maxsub=lambda k: 32, minsub=lambda k: 16),
+}
+
+
 def norm_lower_bound(p):
     """max row 2-norm of A: a lower bound of ||A||_2 (stricter residual test)"""
     nb, nd, bs = p.n // p.bs, 2 * p.hw + 1, p.bs
@@ -81,8 +108,9 @@ def pct(a, q):
     return float(np.percentile(np.asarray(a), q)) if len(a) else None
 
 
-def cpu_baseline(reqs, budget_s, threads):
-    """Reference solver on the host (oracle/_ref if built, else the oracle)."""
+def cpu_baseline(reqs, budget_s, threads, wl):
+    """Reference solver on the host (oracle/_ref if built, else the oracle);
+    FP64 throughout (the reference has no complex64 inner solve)."""
     from bbjd_amd import BandedTauPencil
     from oracle import orc
     which = "ref" if orc.have_ref() else "oracle"
@@ -93,11 +121,12 @@ def cpu_baseline(reqs, budget_s, threads):
     for r in sorted(reqs, key=lambda r: (r["n"], r["k"])):
         if secs >= budget_s:
             break
-        p = BandedTauPencil.generate(r["n"], 8, 1, 4, seed=r["pseed"])
+        p = BandedTauPencil.generate(r["n"], *wl["shape"], seed=r["pseed"])
         ops = orc.CpuPencilOps(p, r["sigma"], 1)
         t = time.perf_counter()
-        out = orc.solve(which, p.n, r["k"], r["sigma"], tol=1e-10, maxsub=4 * r["k"], minsub=2 * r["k"],
-                        maxiter=r.get("maxiter", 1000), ops=ops)
+        out = orc.solve(which, p.n, r["k"], r["sigma"], tol=1e-10, maxsub=wl["maxsub"](r["k"]),
+                        minsub=wl["minsub"](r["k"]), maxiter=r.get("maxiter", 1000), inner_maxiter=wl["inner"],
+                        inner_tol=wl["inner_tol"], ops=ops)
         dt = time.perf_counter() - t
         pairs += out["nconv"]
         secs += dt
@@ -114,7 +143,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--nreq", type=int, default=256)
+    ap.add_argument("--workload", choices=["serving", "small"], default="serving")
+    ap.add_argument("--nreq", type=int, default=0, help="requests (default 256 serving, 2048 small)")
     ap.add_argument("--rate", type=float, default=32.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--workers", type=int, default=16)
     ap.add_argument("--maxiter", type=int, default=1000)
@@ -145,7 +175,11 @@ def main():
     from bbjd_amd import autotune
     tt = None if args.no_tile_table else autotune.apply()
     log(f"[bench] tile table: {'autotuned ' + tt['tuned'] if tt else 'engine defaults'}")
-    reqs = make_requests(args.nreq, args.rate, args.seed)
+    wl = WORKLOADS[args.workload]
+    if args.nreq <= 0:
+        args.nreq = 256 if args.workload == "serving" else 2048
+    reqs = (make_requests(args.nreq, args.rate, args.seed) if args.workload == "serving"
+            else make_small_requests(args.nreq, args.seed))
     for r in reqs:
         r["maxiter"] = args.maxiter
 
@@ -153,9 +187,9 @@ def main():
     t0 = time.perf_counter()
     ops, norms = [], []
     for i, r in enumerate(reqs):
-        p = BandedTauPencil.generate(r["n"], 8, 1, 4, seed=r["pseed"])
+        p = BandedTauPencil.generate(r["n"], *wl["shape"], seed=r["pseed"])
         norms.append(norm_lower_bound(p))
-        op = DeviceOperator(p, r["sigma"], pc="exact")
+        op = DeviceOperator(p, r["sigma"], pc="exact", c64=wl["precision"] == "mixed")
         op.pencil = None          # host copy no longer needed
         ops.append(op)
     t_setup = time.perf_counter() - t0
@@ -163,14 +197,16 @@ def main():
     log(f"[bench] {len(ops)} operators resident ({op_bytes / 1e9:.1f} GB) in {t_setup:.1f}s")
 
     def jobs_for(idx, with_arrival=True):
-        return [dict(op=ops[i], sigma=reqs[i]["sigma"], nev=reqs[i]["k"], maxsub=4 * reqs[i]["k"],
-                     minsub=2 * reqs[i]["k"], maxiter=reqs[i]["maxiter"], tol=1e-10,
+        return [dict(op=ops[i], sigma=reqs[i]["sigma"], nev=reqs[i]["k"], maxsub=wl["maxsub"](reqs[i]["k"]),
+                     minsub=wl["minsub"](reqs[i]["k"]), maxiter=reqs[i]["maxiter"], tol=1e-10,
+                     inner_maxiter=wl["inner"], inner_tol=wl["inner_tol"],
                      arrival=reqs[i]["arrival"] if with_arrival else 0.0) for i in idx]
 
     # ---- warmup (untimed): short streams ----
     for w in range(args.warmup):
         idx = list(range(min(16, len(reqs))))
-        srv.serve(jobs_for(idx, False), workers=1 if args.plain_only else args.workers, realtime=False)
+        srv.serve(jobs_for(idx, False), workers=1 if args.plain_only else args.workers, realtime=False,
+                  precision=wl["precision"])
         log(f"[bench] warmup step {w} done")
 
     # ---- timed: served ----
@@ -179,8 +215,9 @@ def main():
     srv.ktime_enable(True)
     for s in range(0 if args.plain_only else args.steps):
         t = time.perf_counter()
-        res, rep = srv.serve(jobs_for(all_idx), workers=args.workers, realtime=True, order=args.order,
-                             pin=args.pin, sync_yield="block" if args.sync == "block" else True)
+        res, rep = srv.serve(jobs_for(all_idx), workers=args.workers, realtime=wl["realtime"], order=args.order,
+                             pin=args.pin, sync_yield="block" if args.sync == "block" else True,
+                             precision=wl["precision"])
         walls.append(time.perf_counter() - t)
         served.append((res, rep))
         log(f"[bench] step {s}: {rep['pairs']} pairs in {walls[-1]:.1f}s, workers {rep['workers']}, "
@@ -211,7 +248,9 @@ def main():
     rng = np.random.default_rng(args.seed + 1)
     classes = {}
     for i, r in enumerate(reqs):
-        classes.setdefault((r["n"], r["k"]), []).append(i)
+        # small solves: n is continuous; eight log-spaced n classes
+        key = (r["n"], r["k"]) if args.workload == "serving" else (int(np.log2(r["n"] / 4096) * 8 / 3), r["k"])
+        classes.setdefault(key, []).append(i)
     sample, cls_of = [], {}
     for key in sorted(classes):
         members = classes[key]
@@ -221,7 +260,7 @@ def main():
             cls_of[members[j]] = key
     srv.ktime_enable(True)
     t = time.perf_counter()
-    pres, prep = srv.serve(jobs_for(sample, False), workers=1, realtime=False)
+    pres, prep = srv.serve(jobs_for(sample, False), workers=1, realtime=False, precision=wl["precision"])
     t_plain = time.perf_counter() - t
     srv.ktime_enable(False)
     kcnt, kms, kbytes = srv.ktime_harvest()
@@ -284,7 +323,7 @@ def main():
     if not args.no_cpu_baseline and not args.plain_only:
         thr = args.cpu_threads or min(16, os.cpu_count() or 1)
         try:
-            cpu = cpu_baseline(reqs, args.cpu_seconds, thr)
+            cpu = cpu_baseline(reqs, args.cpu_seconds, thr, wl)
             log(f"[bench] cpu baseline: {cpu['value']} eigenpairs/s ({cpu['kind']}, {cpu['cores']} threads)")
         except Exception as e:  # noqa: BLE001
             log(f"[bench] cpu baseline unavailable: {e}")
@@ -300,19 +339,25 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c128",
+        "dtype": "c128" if args.workload == "serving" else "c128 (outer loop), c64 (inner solve)",
         "data": "synthetic",
-        "config": {"workload": f"serving loop: {args.nreq} JDQZ requests, Poisson arrivals "
-                               f"{args.rate}/s, n in {{2e4,5e4,1e5,2e5}} (bs 8 block-tridiagonal, "
-                               f"4 tau rows), k in {{8,16,32}}, sigma in [-4,4]+[-4,4]i, tol 1e-10, "
-                               f"maxsub 4k, minsub 2k, maxiter {args.maxiter}, exact banded LU",
+        "config": {"workload": (f"serving loop: {args.nreq} JDQZ requests, Poisson arrivals "
+                                f"{args.rate}/s, n in {{2e4,5e4,1e5,2e5}} (bs 8 block-tridiagonal, "
+                                f"4 tau rows), k in {{8,16,32}}, sigma in [-4,4]+[-4,4]i, tol 1e-10, "
+                                f"maxsub 4k, minsub 2k, maxiter {args.maxiter}, exact banded LU"
+                                if args.workload == "serving" else
+                                f"batched small solves: {args.nreq} JDQZ requests submitted at once, n "
+                                f"log-uniform on [4096, 32768] (bs 4 block-tridiagonal, 2 tau rows), k 8, "
+                                f"sigma on the unit disk, tol 1e-10, maxsub 32, minsub 16, maxiter "
+                                f"{args.maxiter}, GMRES(20) inner solve in complex64 (inner tol 1e-3) with "
+                                f"the exact banded LU as its preconditioner"),
                    "workers": args.workers, "requests": args.nreq, "admission": args.order,
                    "host_wait": args.sync, "workers_pinned": bool(args.pin)},
         "served_eigenpairs_per_s": round(served_rate, 3) if served_rate else None,
         "plain_eigenpairs_per_s": round(plain_rate, 3) if plain_rate else None,
         "served_over_plain": round(served_rate / plain_rate, 2) if served_rate and plain_rate else None,
         "plain_sample_requests": len(sample),
-        "plain_estimate": "per (n, k) class mean of its sampled solves alone, weighted by class counts",
+        "plain_estimate": "per class mean of its sampled solves alone, weighted by class counts",
         "latency_p50_s": round(pct(lat, 50), 3) if lat else None,
         "latency_p99_s": round(pct(lat, 99), 3) if lat else None,
         "pairs": pairs,
