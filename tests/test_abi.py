@@ -7,6 +7,7 @@ import ctypes as C
 import os
 import re
 import subprocess
+import sys
 
 import numpy as np
 import pytest
@@ -175,3 +176,12 @@ def test_autotuned_tile_table_loads_and_is_validated(tmp_path):
         assert autotune.apply(str(tmp_path / "missing.json")) is None
     finally:
         lib.bbjd_set_tile_table(0, None, 0, None, None, None)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus N > 1 exits with a message before
+    touching a device"""
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and r.stdout == ""

@@ -587,3 +587,35 @@ def test_serving_loop_results_alone_equal_mix():
         assert ref["nconv"] == j["nev"]
         assert rel_match(mix[i]["eigenvalues"], ref["evals"]) <= 1e-8
         assert rel_match(ref["evals"], mix[i]["eigenvalues"]) <= 1e-8
+
+
+def test_serving_loop_mixed_precision_alone_equal_mix():
+    """the small-solve form of the serving loop: GMRES(20) in complex64 with
+    the exact LU, FP32 accumulation, four workers; each request's eigenvalues
+    bit-identical alone and in the mix, every FP64 residual below tol, and
+    those the FP64 inner solve also finds within 1e-8 of its values (JDQZ
+    need not return exactly the nev nearest sigma, so the two sets may differ
+    in a pair at the edge of the window)"""
+    from bbjd_amd import serve as srv
+    from bbjd_amd.configs import row_norm_bound
+    rng = np.random.default_rng(3)
+    jobs = []
+    for i, n in enumerate((4096, 6000, 9000, 5004, 12000, 7000)):
+        p = BandedTauPencil.generate(n, 4, 1, 2, seed=200 + i)
+        s = np.sqrt(rng.uniform()) * np.exp(2j * np.pi * rng.uniform())
+        jobs.append(dict(pencil=p, pc="exact", sigma=complex(s), nev=8, maxsub=32, minsub=16, maxiter=1000,
+                         inner_maxiter=20, inner_tol=1e-3, tol=1e-10))
+    mix, rep = srv.serve(jobs, workers=4, realtime=False, precision="mixed")
+    f64, _ = srv.serve(jobs, workers=4, realtime=False, precision="fp64")
+    assert rep["failed"] == 0 and rep["peak_concurrency"] >= 2
+    for i, j in enumerate(jobs):
+        alone, _ = srv.serve([j], workers=1, realtime=False, precision="mixed")
+        a = alone[0]
+        assert a["status"] == 0 and a["nconv"] == 8, i
+        np.testing.assert_array_equal(mix[i]["eigenvalues"], a["eigenvalues"], err_msg=str(i))
+        anorm = row_norm_bound(j["pencil"])
+        assert np.all(a["pair_residuals"] <= 1e-10 * (anorm + np.abs(a["eigenvalues"]))), i
+        assert f64[i]["nconv"] == 8
+        both = [ev for ev in a["eigenvalues"] if np.min(np.abs(f64[i]["eigenvalues"] - ev)) <= 1e-6 * abs(ev)]
+        assert len(both) >= 6, i
+        assert rel_match(np.array(both), f64[i]["eigenvalues"]) <= 1e-8, i
