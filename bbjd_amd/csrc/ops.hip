@@ -17,12 +17,27 @@
 
 typedef long long i64;
 template <typename R> struct alignas(2 * sizeof(R)) Cx { R re, im; };
+// a += q h and a -= q h as four explicit fused multiply-adds into a: the
+// compiler has no choice of which products to fuse, so kernels that apply
+// them in the same order round identically whatever the surrounding code
+__device__ __forceinline__ double ffma(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float ffma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 template <typename R> __device__ __forceinline__ void cfma(Cx<R> &a, Cx<R> q, Cx<R> h)
-{ a.re += q.re * h.re - q.im * h.im; a.im += q.re * h.im + q.im * h.re; }
+{
+    a.re = ffma(q.re, h.re, a.re);
+    a.re = ffma(-q.im, h.im, a.re);
+    a.im = ffma(q.re, h.im, a.im);
+    a.im = ffma(q.im, h.re, a.im);
+}
 __device__ __forceinline__ Cx<double> cx(double a, double b) { Cx<double> c; c.re = a; c.im = b; return c; }
 __device__ __forceinline__ Cx<double> cx0() { return cx(0.0, 0.0); }
 template <typename R> __device__ __forceinline__ void cfms(Cx<R> &a, Cx<R> q, Cx<R> h)
-{ a.re -= q.re * h.re - q.im * h.im; a.im -= q.re * h.im + q.im * h.re; }
+{
+    a.re = ffma(-q.re, h.re, a.re);
+    a.re = ffma(q.im, h.im, a.re);
+    a.im = ffma(-q.re, h.im, a.im);
+    a.im = ffma(-q.im, h.re, a.im);
+}
 
 static constexpr int TPB = 256;
 
@@ -446,8 +461,10 @@ __global__ __launch_bounds__(TPB) void k_bcr_bwd_multi(i64 N, int l, BcrLevels d
 #pragma unroll
                     for (int q = 0; q < NR; q++) {
                         const Cx<double> v = lx[q][slot][r];
-                        p[q].re += tq.re * v.re + tq.im * v.im;
-                        p[q].im += tq.re * v.im - tq.im * v.re;
+                        p[q].re = ffma(tq.re, v.re, p[q].re);
+                        p[q].re = ffma(tq.im, v.im, p[q].re);
+                        p[q].im = ffma(tq.re, v.im, p[q].im);
+                        p[q].im = ffma(-tq.im, v.re, p[q].im);
                     }
                 }
             }
@@ -548,6 +565,267 @@ __global__ __launch_bounds__(TPB) void k_bcr_fwd_multi(i64 N, int l, i64 n1, Bcr
                 for (int q = 0; q < NR; q++) lb[q][sl][r] = acc[q];
             }
         }
+        __syncthreads();
+    }
+    for (int e = t; e < (ROWS - (1 << G)) * S * NR; e += TPB) {
+        const int q = e / ((ROWS - (1 << G)) * S), rem = e % ((ROWS - (1 << G)) * S);
+        const int sl = (1 << G) + rem / S, r = rem % S;
+        const i64 bi = (p0 + sl) << l;
+        if (bi < N) bout[q * vs + bi * S + r] = lb[q][sl][r];
+    }
+}
+
+// The grouped levels with their factor loads issued at launch (s = 4, 8).
+// In k_bcr_bwd_multi and k_bcr_fwd_multi every phase loads its factor blocks
+// after the previous phase's barrier, so each of the G phases waits for a
+// full HBM round trip, while the top phases keep most threads idle.  Here
+// phase 0 runs on all TPB threads and each phase g >= 1 on a thread range of
+// its own, [TPB - 2 TPB/2^g, TPB - TPB/2^g), the same rows in the same
+// per-row order.  A thread loads the factor blocks (and, backward, the
+// right-hand sides) of the first phase it takes part in before the launch's
+// first barrier; backward, a thread of a phase g >= 1 loads its phase-0 row
+// right after solving its phase-g row, forward it holds both rows' factors
+// from the start.  Results are bit-identical to the kernels above.
+template <int G> __device__ __forceinline__ int bcr_phase_of(int t)
+{
+    int gm = 0;
+#pragma unroll
+    for (int g = 1; g < G; g++)
+        if (t >= TPB - 2 * (TPB >> g) && t < TPB - (TPB >> g)) gm = g;
+    return gm;
+}
+__device__ __forceinline__ const Cx<double> *bcr_level_ptr(const BcrLevels &f, int g)
+{
+    return g == 0 ? f.f[0] : g == 1 ? f.f[1] : g == 2 ? f.f[2] : f.f[3];
+}
+
+template <int NR, int S, int G>
+__global__ __launch_bounds__(TPB) void k_bcr_bwd_multi_pf(i64 N, int l, BcrLevels dpq, BcrLevels bl, Cx<double> *x,
+                                                          i64 vs, const Cx<double> *__restrict__ tauc, int ntau,
+                                                          i64 n1, i64 ldt, Cx<double> *__restrict__ dpart)
+{
+    constexpr int U = TPB / ((1 << (G - 1)) * S);
+    constexpr int SLOTS = U * (1 << G) + 1;
+    constexpr i64 ss = (i64)S * S;
+    __shared__ Cx<double> lx[NR][SLOTS][S];
+    const int t = threadIdx.x;
+    const i64 u0 = (i64)blockIdx.x * U;
+    for (int e = t; e < (U + 1) * S * NR; e += TPB) {
+        const int q = e / ((U + 1) * S), rem = e % ((U + 1) * S), u = rem / S, r = rem % S;
+        const i64 i = ((u0 + u) << G) << l;
+        lx[q][u << G][r] = i < N ? x[q * vs + i * S + r] : cx0();
+    }
+    // the row this thread solves in one phase: factor columns, right-hand
+    // sides, its LDS slot and output offset
+    Cx<double> fd[S], fp[S], fq[S], fb[NR][S];
+    int slot = 0, rl = 0;
+    i64 ob = 0;
+    bool ok = false, okq = false;
+    // phase g's thread index tg (tg < TPB >> g = U 2^(G-1-g) S)
+    auto load = [&](int g, int tg) {
+        const int R = 1 << (G - 1 - g);
+        const int rr = tg / S, r = tg % S, u = rr / R, k = rr % R;
+        const i64 hg = (i64)1 << (l + g), M = (N + hg - 1) >> (l + g);
+        const i64 m = ((u0 + u) << (G - g)) + 2 * k + 1;
+        ok = m < M;
+        okq = m + 1 < M;
+        slot = (u << G) + ((2 * k + 1) << g);
+        rl = r;
+        if (ok) {
+            const i64 j = (m - 1) / 2;
+            ob = m * hg * S;
+            const Cx<double> *D = bcr_level_ptr(dpq, g) + j * 3 * ss + r, *P = D + ss, *Qm = P + ss;
+            const Cx<double> *b = bcr_level_ptr(bl, g);
+#pragma unroll
+            for (int c = 0; c < S; c++) {
+                fd[c] = D[(i64)c * S];
+                fp[c] = P[(i64)c * S];
+            }
+            if (okq) {
+#pragma unroll
+                for (int c = 0; c < S; c++) fq[c] = Qm[(i64)c * S];
+            }
+#pragma unroll
+            for (int q = 0; q < NR; q++)
+#pragma unroll
+                for (int c = 0; c < S; c++) fb[q][c] = b[q * vs + ob + c];
+        }
+    };
+    auto solve = [&](int g) {
+        if (!ok) return;
+        Cx<double> acc[NR];
+#pragma unroll
+        for (int q = 0; q < NR; q++) acc[q] = cx0();
+#pragma unroll
+        for (int c = 0; c < S; c++) {
+#pragma unroll
+            for (int q = 0; q < NR; q++) {
+                cfma(acc[q], fd[c], fb[q][c]);
+                cfms(acc[q], fp[c], lx[q][slot - (1 << g)][c]);
+            }
+        }
+        if (okq) {
+#pragma unroll
+            for (int c = 0; c < S; c++) {
+#pragma unroll
+                for (int q = 0; q < NR; q++) cfms(acc[q], fq[c], lx[q][slot + (1 << g)][c]);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < NR; q++) {
+            x[q * vs + ob + rl] = acc[q];
+            lx[q][slot][rl] = acc[q];
+        }
+    };
+    const int gm = bcr_phase_of<G>(t);
+    if (gm > 0) load(gm, t - (TPB - 2 * (TPB >> gm)));
+    else load(0, t);
+    __syncthreads();
+#pragma unroll
+    for (int g = G - 1; g >= 1; g--) {
+        if (gm == g) {
+            solve(g);
+            load(0, t);
+        }
+        __syncthreads();
+    }
+    solve(0);
+    // the dense rows' partial dots, as in k_bcr_bwd_multi
+    if (tauc) {
+        __syncthreads();
+        __shared__ Cx<double> dred[16][NR][TPB / 64];
+        const int lane = t & 63, w = t >> 6;
+        for (int tt = 0; tt < ntau; tt++) {
+            Cx<double> p[NR];
+#pragma unroll
+            for (int q = 0; q < NR; q++) p[q] = cx0();
+            for (int e = t; e < U * (1 << G) * S; e += TPB) {
+                const int sl = e / S, r = e % S;
+                const i64 el = ((u0 << G) + sl) * S + r;
+                if (el < n1 && ((u0 << G) + sl) < N) {
+                    const Cx<double> tq = tauc[el + (i64)tt * ldt];
+#pragma unroll
+                    for (int q = 0; q < NR; q++) {
+                        const Cx<double> v = lx[q][sl][r];
+                        p[q].re = ffma(tq.re, v.re, p[q].re);
+                        p[q].re = ffma(tq.im, v.im, p[q].re);
+                        p[q].im = ffma(tq.re, v.im, p[q].im);
+                        p[q].im = ffma(-tq.im, v.re, p[q].im);
+                    }
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < NR; q++) {
+#pragma unroll
+                for (int o = 32; o >= 1; o >>= 1) {
+                    p[q].re += __shfl_xor(p[q].re, o, 64);
+                    p[q].im += __shfl_xor(p[q].im, o, 64);
+                }
+                if (lane == 0) dred[tt][q][w] = p[q];
+            }
+        }
+        __syncthreads();
+        if (t < ntau * NR) {
+            const int tt = t / NR, q = t % NR;
+            Cx<double> s = dred[tt][q][0];
+#pragma unroll
+            for (int v = 1; v < TPB / 64; v++) { s.re += dred[tt][q][v].re; s.im += dred[tt][q][v].im; }
+            dpart[(i64)blockIdx.x * ntau * NR + t] = s;
+        }
+    }
+}
+
+template <int NR, int S, int G>
+__global__ __launch_bounds__(TPB) void k_bcr_fwd_multi_pf(i64 N, int l, i64 n1, BcrLevels ag,
+                                                          const Cx<double> *__restrict__ x0,
+                                                          const Cx<double> *__restrict__ x1,
+                                                          const Cx<double> *__restrict__ bin, Cx<double> *bout,
+                                                          i64 vs)
+{
+    constexpr int U = TPB / ((1 << (G - 1)) * S) - 1;
+    constexpr int ROWS = (U + 1) << G;
+    constexpr i64 ss = (i64)S * S;
+    __shared__ Cx<double> lb[NR][ROWS][S];
+    const int t = threadIdx.x;
+    const i64 p0 = ((i64)blockIdx.x * U - 1) << G;
+    const Cx<double> *xs[2] = {x0, x1};
+    for (int e = t; e < ROWS * S * NR; e += TPB) {
+        const int q = e / (ROWS * S), rem = e % (ROWS * S), sl = rem / S, r = rem % S;
+        const i64 il = p0 + sl, bi = il << l;
+        Cx<double> v = cx0();
+        if (il >= 0 && bi < N) {
+            if (x0) {
+                const i64 el = bi * S + r;
+                if (el < n1) v = xs[q][el];
+            } else
+                v = bin[q * vs + bi * S + r];
+        }
+        lb[q][sl][r] = v;
+    }
+    // one even row of a phase: its slot, lane, both neighbours' presence and
+    // the factor columns alpha (left) and gamma (right)
+    struct Row {
+        Cx<double> a[S], c[S];
+        int sl, r;
+        bool ok, left, right;
+    };
+    auto load = [&](Row &w, int g, int tg) {
+        const int nrow = (ROWS >> (g + 1)) - 1;
+        w.ok = w.left = w.right = false;
+        if (tg >= nrow * S) return;
+        w.sl = (tg / S + 1) << (g + 1);
+        w.r = tg % S;
+        const i64 il = p0 + w.sl;
+        if (il < 0 || (il << l) >= N) return;
+        const int lv = l + g;
+        const i64 M = (N + ((i64)1 << lv) - 1) >> lv, m = il >> g, j = m / 2;
+        w.ok = true;
+        w.left = m >= 1;
+        w.right = m + 1 < M;
+        const Cx<double> *al = bcr_level_ptr(ag, g) + j * 2 * ss + w.r;
+        if (w.left) {
+#pragma unroll
+            for (int c = 0; c < S; c++) w.a[c] = al[(i64)c * S];
+        }
+        if (w.right) {
+#pragma unroll
+            for (int c = 0; c < S; c++) w.c[c] = al[ss + (i64)c * S];
+        }
+    };
+    auto solve = [&](const Row &w, int g) {
+        if (!w.ok) return;
+        const int h = 1 << g;
+        Cx<double> acc[NR];
+#pragma unroll
+        for (int q = 0; q < NR; q++) acc[q] = lb[q][w.sl][w.r];
+        if (w.left) {
+#pragma unroll
+            for (int c = 0; c < S; c++) {
+#pragma unroll
+                for (int q = 0; q < NR; q++) cfms(acc[q], w.a[c], lb[q][w.sl - h][c]);
+            }
+        }
+        if (w.right) {
+#pragma unroll
+            for (int c = 0; c < S; c++) {
+#pragma unroll
+                for (int q = 0; q < NR; q++) cfms(acc[q], w.c[c], lb[q][w.sl + h][c]);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < NR; q++) lb[q][w.sl][w.r] = acc[q];
+    };
+    const int gm = bcr_phase_of<G>(t);
+    Row w0, w1;
+    load(w0, 0, t);
+    w1.ok = false;
+    if (gm > 0) load(w1, gm, t - (TPB - 2 * (TPB >> gm)));
+    __syncthreads();
+    solve(w0, 0);
+    __syncthreads();
+#pragma unroll
+    for (int g = 1; g < G; g++) {
+        if (gm == g) solve(w1, g);
         __syncthreads();
     }
     for (int e = t; e < (ROWS - (1 << G)) * S * NR; e += TPB) {
@@ -963,6 +1241,11 @@ int bbjd_o_bcr_bwd(hipStream_t st, int s, int64_t M, int64_t h, int64_t no, cons
 
 static inline bool bcr_group_ok(int s, int G) { return (s == 4 || s == 8 || s == 16) && G >= 2 && G <= 4; }
 
+// s = 4, 8: the grouped levels with their factor loads issued at launch
+// (k_bcr_*_multi_pf); 0 selects the phase-by-phase kernels (for tests)
+static int g_bcr_prefetch = 1;
+extern "C" void bbjd_set_bcr_prefetch(int on) { g_bcr_prefetch = on != 0; }
+
 int bbjd_o_bcr_bwd_multi(hipStream_t st, int s, int64_t N, int l, int G, const void *const *dpq,
                          const void *const *bl, void *x, int nrhs, int64_t vs, const void *tauc, int ntau,
                          int64_t n1, int64_t ldt, void *dpart, int64_t dpart_cap, int64_t *nblk_out)
@@ -982,9 +1265,18 @@ int bbjd_o_bcr_bwd_multi(hipStream_t st, int s, int64_t N, int l, int G, const v
         constexpr int U = TPB / ((1 << (GG - 1)) * S);                                                         \
         const i64 nblk = (units + U - 1) / U;                                                                  \
         const bool dd = dots && nblk * ntau * NR <= dpart_cap;                                                 \
+        const Cx<double> *tc = dd ? (const Cx<double> *)tauc : nullptr;                                        \
+        if constexpr (S <= 8) {                                                                                \
+            if (g_bcr_prefetch) {                                                                              \
+                hipLaunchKernelGGL((k_bcr_bwd_multi_pf<NR, S, GG>), dim3((unsigned)nblk), dim3(TPB), 0, st,    \
+                                   (i64)N, l, f, bb, (Cx<double> *)x, (i64)vs, tc, ntau, (i64)n1, (i64)ldt,    \
+                                   (Cx<double> *)dpart);                                                       \
+                if (dd && nblk_out) *nblk_out = nblk;                                                          \
+                break;                                                                                         \
+            }                                                                                                  \
+        }                                                                                                      \
         hipLaunchKernelGGL((k_bcr_bwd_multi<NR, S, GG>), dim3((unsigned)nblk), dim3(TPB), 0, st, (i64)N, l, f, bb, \
-                           (Cx<double> *)x, (i64)vs, dd ? (const Cx<double> *)tauc : nullptr, ntau, (i64)n1,     \
-                           (i64)ldt, (Cx<double> *)dpart);                                                     \
+                           (Cx<double> *)x, (i64)vs, tc, ntau, (i64)n1, (i64)ldt, (Cx<double> *)dpart);         \
         if (dd && nblk_out) *nblk_out = nblk;                                                                  \
     }
     BBJD_BCR_GROUP_DISPATCH(BBJD_BWDM)
@@ -1004,10 +1296,17 @@ int bbjd_o_bcr_fwd_multi(hipStream_t st, int s, int64_t N, int64_t n1, int l, in
 #define BBJD_FWDM(NR, S, GG)                                                                                   \
     {                                                                                                          \
         constexpr int U = TPB / ((1 << (GG - 1)) * S) - 1;                                                     \
-        hipLaunchKernelGGL((k_bcr_fwd_multi<NR, S, GG>), dim3((unsigned)((units + U - 1) / U)), dim3(TPB), 0,   \
-                           st, (i64)N, l, (i64)n1, f, (const Cx<double> *)x0,                                  \
-                           (const Cx<double> *)(x1 ? x1 : x0), (const Cx<double> *)bin, (Cx<double> *)bout,     \
-                           (i64)vs);                                                                           \
+        const unsigned nb = (unsigned)((units + U - 1) / U);                                                   \
+        const Cx<double> *a0 = (const Cx<double> *)x0, *a1 = (const Cx<double> *)(x1 ? x1 : x0);               \
+        if constexpr (S <= 8) {                                                                                \
+            if (g_bcr_prefetch) {                                                                              \
+                hipLaunchKernelGGL((k_bcr_fwd_multi_pf<NR, S, GG>), dim3(nb), dim3(TPB), 0, st, (i64)N, l,     \
+                                   (i64)n1, f, a0, a1, (const Cx<double> *)bin, (Cx<double> *)bout, (i64)vs);  \
+                break;                                                                                         \
+            }                                                                                                  \
+        }                                                                                                      \
+        hipLaunchKernelGGL((k_bcr_fwd_multi<NR, S, GG>), dim3(nb), dim3(TPB), 0, st, (i64)N, l, (i64)n1, f, a0, a1, \
+                           (const Cx<double> *)bin, (Cx<double> *)bout, (i64)vs);                              \
     }
     BBJD_BCR_GROUP_DISPATCH(BBJD_FWDM)
 #undef BBJD_FWDM

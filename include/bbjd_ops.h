@@ -107,6 +107,10 @@ void bbjd_set_bcr_tail_rows(int rows);
  * the dense rows' dots of the Schur border; 0: a separate projection.
  * Tuning and tests. */
 void bbjd_set_bcr_border_fuse(int on);
+/* 1 (default): for s = 4 and 8 the grouped reduction launches issue every
+ * phase's factor loads at launch, each phase on a thread range of its own;
+ * 0: one phase after another.  Bit-identical results.  Tuning and tests. */
+void bbjd_set_bcr_prefetch(int on);
 
 #ifdef __cplusplus
 }

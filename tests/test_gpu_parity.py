@@ -236,7 +236,7 @@ def test_fused_matvec_ab_matches_separate(n, bs, hw, ntau):
         dev.free()
 
 
-@pytest.mark.parametrize("n,bs,hw,ntau", [(60000, 8, 1, 4), (40000, 8, 2, 16), (30004, 4, 1, 2)])
+@pytest.mark.parametrize("n,bs,hw,ntau", [(60000, 8, 1, 4), (40000, 8, 2, 16), (30004, 4, 1, 2), (200000, 8, 1, 4)])
 def test_bcr_level_groups_match_level_by_level(n, bs, hw, ntau):
     """the exact solve with 2, 3 or 4 block-cyclic-reduction levels per
     launch (bbjd_set_bcr_group; super-blocks of 8, 16 and 4 rows, the last
@@ -244,7 +244,10 @@ def test_bcr_level_groups_match_level_by_level(n, bs, hw, ntau):
     row arithmetic in the same order -- for one and two right-hand sides,
     and solves M y = x as the oracle's banded LU does; with the Schur
     border's dots formed in the last backward launch
-    (bbjd_set_bcr_border_fuse; another summation order) to 1e-13"""
+    (bbjd_set_bcr_border_fuse; another summation order) to 1e-13.  The
+    grouped launches that load every phase's factors at launch
+    (bbjd_set_bcr_prefetch, s = 4 and 8) equal the phase-by-phase ones
+    exactly, with and without the fused dots"""
     lib = _lib.load()
     p = BandedTauPencil.generate(n, bs, hw, ntau, seed=21)
     sigma = 0.3 - 0.2j
@@ -259,20 +262,29 @@ def test_bcr_level_groups_match_level_by_level(n, bs, hw, ntau):
         z0, z1 = dev.solve2(x, x2)
         for g in (2, 3, 4):
             lib.bbjd_set_bcr_group(g)
-            np.testing.assert_array_equal(dev.solve(x), y1)
-            a, b = dev.solve2(x, x2)
-            np.testing.assert_array_equal(a, z0)
-            np.testing.assert_array_equal(b, z1)
+            for pf in (0, 1):
+                lib.bbjd_set_bcr_prefetch(pf)
+                np.testing.assert_array_equal(dev.solve(x), y1)
+                a, b = dev.solve2(x, x2)
+                np.testing.assert_array_equal(a, z0)
+                np.testing.assert_array_equal(b, z1)
         lib.bbjd_set_bcr_border_fuse(1)
         for g in (2, 4):
             lib.bbjd_set_bcr_group(g)
-            assert np.linalg.norm(dev.solve(x) - y1) <= 1e-13 * np.linalg.norm(y1)
-            a, b = dev.solve2(x, x2)
-            assert np.linalg.norm(a - z0) <= 1e-13 * np.linalg.norm(z0)
-            assert np.linalg.norm(b - z1) <= 1e-13 * np.linalg.norm(z1)
+            lib.bbjd_set_bcr_prefetch(0)
+            f1 = dev.solve(x)
+            f2 = dev.solve2(x, x2)
+            lib.bbjd_set_bcr_prefetch(1)
+            np.testing.assert_array_equal(dev.solve(x), f1)
+            for u, v in zip(dev.solve2(x, x2), f2):
+                np.testing.assert_array_equal(u, v)
+            assert np.linalg.norm(f1 - y1) <= 1e-13 * np.linalg.norm(y1)
+            assert np.linalg.norm(f2[0] - z0) <= 1e-13 * np.linalg.norm(z0)
+            assert np.linalg.norm(f2[1] - z1) <= 1e-13 * np.linalg.norm(z1)
     finally:
         lib.bbjd_set_bcr_group(4)
         lib.bbjd_set_bcr_border_fuse(1)
+        lib.bbjd_set_bcr_prefetch(1)
         dev.free()
     cpu = orc.CpuPencilOps(p, sigma, 1)
     r1, r2 = cpu.apply("orc_precond", x), cpu.apply("orc_precond", x2)
@@ -593,9 +605,14 @@ def test_serving_loop_mixed_precision_alone_equal_mix():
     """the small-solve form of the serving loop: GMRES(20) in complex64 with
     the exact LU, FP32 accumulation, four workers; each request's eigenvalues
     bit-identical alone and in the mix, every FP64 residual below tol, and
-    those the FP64 inner solve also finds within 1e-8 of its values (JDQZ
-    need not return exactly the nev nearest sigma, so the two sets may differ
-    in a pair at the edge of the window)"""
+    every eigenvalue of the mixed and of the FP64 inner solve within 1e-8 of
+    one of the pencil's 48 eigenvalues nearest sigma, found independently by
+    ARPACK on the shift-and-invert operator (A - sigma B)^-1 B (the oracle's
+    banded LU).  The two sets are not compared with each other: with inexact
+    inner solves JDQZ need not return exactly the nev nearest sigma (one
+    request here locks the 20th nearest), and the two inner precisions may
+    lock different ones"""
+    from scipy.sparse.linalg import LinearOperator, eigs
     from bbjd_amd import serve as srv
     from bbjd_amd.configs import row_norm_bound
     rng = np.random.default_rng(3)
@@ -616,6 +633,10 @@ def test_serving_loop_mixed_precision_alone_equal_mix():
         anorm = row_norm_bound(j["pencil"])
         assert np.all(a["pair_residuals"] <= 1e-10 * (anorm + np.abs(a["eigenvalues"]))), i
         assert f64[i]["nconv"] == 8
-        both = [ev for ev in a["eigenvalues"] if np.min(np.abs(f64[i]["eigenvalues"] - ev)) <= 1e-6 * abs(ev)]
-        assert len(both) >= 6, i
-        assert rel_match(np.array(both), f64[i]["eigenvalues"]) <= 1e-8, i
+        A, B = j["pencil"].to_scipy()
+        lu = orc.CpuPencilOps(j["pencil"], j["sigma"], 1)
+        op = LinearOperator(A.shape, matvec=lambda x: lu.apply("orc_precond", B @ x), dtype=np.complex128)
+        mu = eigs(op, k=48, which="LM", return_eigenvectors=False, tol=1e-12, ncv=128)
+        w = j["sigma"] + 1.0 / mu
+        for ev in np.concatenate([a["eigenvalues"], f64[i]["eigenvalues"]]):
+            assert np.min(np.abs(w - ev)) <= 1e-8 * max(1.0, abs(ev)), (i, ev)
