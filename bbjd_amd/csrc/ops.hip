@@ -108,8 +108,11 @@ __device__ __forceinline__ Cx<R> lin2(Cx<R> a, Cx<R> x, Cx<R> b, Cx<R> y)
 
 static constexpr int AB_MAX_TAU = 16;
 
-template <typename R>
-__global__ __launch_bounds__(TPB) void k_band_ab(i64 n, i64 n1, i64 nb, int bs, int hw,
+// BS, HW > 0: the block size and half-width at compile time, so a row's
+// (2 HW + 1) BS band loads are all in flight at once; 0: runtime bs, hw.
+// The dense rows' values load four at a time.
+template <typename R, int BS, int HW>
+__global__ __launch_bounds__(TPB) void k_band_ab(i64 n, i64 n1, i64 nb, int bs_rt, int hw_rt,
                                                  const Cx<R> *__restrict__ band, const Cx<R> *__restrict__ dg,
                                                  int ntau, const Cx<R> *__restrict__ tauc,
                                                  const Cx<R> *__restrict__ x, Cx<R> *__restrict__ ya,
@@ -127,14 +130,17 @@ __global__ __launch_bounds__(TPB) void k_band_ab(i64 n, i64 n1, i64 nb, int bs, 
         cfma(b, dg[i], xi);
         yb[i] = b;
         if (i < n1) {
+            const int bs = BS > 0 ? BS : bs_rt, hw = BS > 0 ? HW : hw_rt;
             const i64 bi = i / bs;
             const int rr = (int)(i % bs), nd = 2 * hw + 1;
             Cx<R> acc = {0, 0};
+#pragma unroll
             for (int d = 0; d < nd; d++) {
                 const i64 bj = bi + d - hw;
                 if (bj < 0 || bj >= nb) continue;
                 const Cx<R> *blk = band + (bi * nd + d) * bs * bs + rr;
                 const Cx<R> *xs = x + bj * bs;
+#pragma unroll
                 for (int c = 0; c < bs; c++) cfma(acc, blk[(i64)c * bs], xs[c]);
             }
             ya[i] = acc;
@@ -146,16 +152,25 @@ __global__ __launch_bounds__(TPB) void k_band_ab(i64 n, i64 n1, i64 nb, int bs, 
         }
     }
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    for (int c = 0; c < ntau; c++) {
-        Cx<R> p = {0, 0};
-        if (i < n) {
-            const Cx<R> q = tauc[i + (i64)c * n];
-            // conj(q) x
-            p.re = q.re * xi.re + q.im * xi.im;
-            p.im = q.re * xi.im - q.im * xi.re;
+    for (int c0 = 0; c0 < ntau; c0 += 4) {
+        Cx<R> q[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            q[k] = Cx<R>{0, 0};
+            if (i < n && c0 + k < ntau) q[k] = tauc[i + (i64)(c0 + k) * n];
         }
-        p = wave_sum(p);
-        if (lane == 0) red[c][w] = p;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (c0 + k >= ntau) break;
+            Cx<R> p = {0, 0};
+            if (i < n) {
+                // conj(q) x
+                p.re = q[k].re * xi.re + q[k].im * xi.im;
+                p.im = q[k].re * xi.im - q[k].im * xi.re;
+            }
+            p = wave_sum(p);
+            if (lane == 0) red[c0 + k][w] = p;
+        }
     }
     if (r) {
         rs = wave_sum_r(rs);
@@ -1130,17 +1145,23 @@ int bbjd_o_band_abr(hipStream_t s, int64_t n, int64_t n1, int64_t nb, int bs, in
     if (ntau < 0 || ntau > AB_MAX_TAU || (ntau > 0 && nblk * ntau > part_cap)) return -3;
     if (res && (single || ntau >= AB_MAX_TAU || !rpart || !rnorm || nblk > rpart_cap)) return -3;
     const int fin_threads = 64 * (ntau + res);
-#define BBJD_AB(R)                                                                                              \
-    hipLaunchKernelGGL((k_band_ab<R>), dim3(nblk), dim3(TPB), 0, s, (i64)n, (i64)n1, (i64)nb, bs, hw,             \
+#define BBJD_AB_K(R, B, H)                                                                                      \
+    hipLaunchKernelGGL((k_band_ab<R, B, H>), dim3(nblk), dim3(TPB), 0, s, (i64)n, (i64)n1, (i64)nb, bs, hw,       \
                        (const Cx<R> *)band, (const Cx<R> *)dg, ntau, (const Cx<R> *)tauc, (const Cx<R> *)x,       \
                        (Cx<R> *)ya, (Cx<R> *)yb, (Cx<R> *)part, Cx<R>{(R)are, (R)aim}, Cx<R>{(R)bre, (R)bim},     \
-                       (Cx<R> *)r, (R *)rpart);                                                                 \
+                       (Cx<R> *)r, (R *)rpart)
+#define BBJD_AB(R)                                                                                              \
+    if (bs == 8 && hw == 1) BBJD_AB_K(R, 8, 1);                                                                 \
+    else if (bs == 4 && hw == 1) BBJD_AB_K(R, 4, 1);                                                            \
+    else if (bs == 8 && hw == 2) BBJD_AB_K(R, 8, 2);                                                            \
+    else BBJD_AB_K(R, 0, 0);                                                                                    \
     if (fin_threads > 0)                                                                                        \
         hipLaunchKernelGGL((k_tau_finish<R>), dim3(1), dim3(fin_threads), 0, s, (i64)nblk, ntau, (i64)n1,        \
                            (const Cx<R> *)part, (Cx<R> *)ya, (const Cx<R> *)yb, Cx<R>{(R)are, (R)aim},           \
                            Cx<R>{(R)bre, (R)bim}, (Cx<R> *)r, (const R *)rpart, rnorm);
     if (single) { BBJD_AB(float) } else { BBJD_AB(double) }
 #undef BBJD_AB
+#undef BBJD_AB_K
     return chk();
 }
 
