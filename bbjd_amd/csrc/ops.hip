@@ -1040,6 +1040,148 @@ __global__ __launch_bounds__(1024) void k_bcr_tail_lds(i64 N, int l0, int L, con
     }
 }
 
+// k_bcr_tail_lds with every step on threads of its own and its factor
+// blocks loaded at launch: step k < F = L - l0 is forward level l0 + k
+// ((M_l + 1) / 2 even rows), step F the final block (one row), step F + 1 + k
+// backward level L - 1 - k (M_l / 2 odd rows); a step's rows take S
+// consecutive threads each, the steps one range after another.  So the
+// chain is one HBM round trip (the gather and every factor block) and then
+// LDS work between barriers, where k_bcr_tail_lds waits for each level's
+// factors in turn.  Every row's arithmetic is k_bcr_tail_lds's, in its
+// order.  The host launches it with 256 threads when the steps' rows fit
+// (S (sum of M_l + 1) <= 256).
+template <int NR, int S>
+__global__ __launch_bounds__(256) void k_bcr_tail_pf(i64 N, int l0, int L, const i64 *__restrict__ off,
+                                                     const Cx<double> *__restrict__ F,
+                                                     const Cx<double> *__restrict__ b, Cx<double> *x, i64 vs)
+{
+    extern __shared__ char smem_raw[];
+    Cx<double> *lb = reinterpret_cast<Cx<double> *>(smem_raw);
+    constexpr i64 ss = (i64)S * S;
+    const int t = threadIdx.x;
+    const i64 stride0 = (i64)1 << l0;
+    const i64 M0 = (N + stride0 - 1) >> l0;
+    const int nf = L - l0, nsteps = 2 * nf + 1;
+    for (i64 e = t; e < M0 * S * NR; e += 256) {
+        const int q = (int)(e / (M0 * S));
+        const i64 rem = e % (M0 * S), c = rem / S;
+        const int r = (int)(rem % S);
+        lb[e] = b[q * vs + c * stride0 * S + r];
+    }
+    auto L_ = [&](int q, i64 c, int r) -> Cx<double> & { return lb[((i64)q * M0 + c) * S + r]; };
+    // this thread's step, row and lane
+    int step = -1, r = 0;
+    i64 j = 0;
+    {
+        int base = 0;
+        for (int k = 0; k < nsteps && step < 0; k++) {
+            i64 rows;
+            if (k < nf) {
+                const int l = l0 + k;
+                rows = (((N + ((i64)1 << l) - 1) >> l) + 1) / 2;
+            } else if (k == nf)
+                rows = 1;
+            else {
+                const int l = L - 1 - (k - nf - 1);
+                rows = ((N + ((i64)1 << l) - 1) >> l) / 2;
+            }
+            if (t < base + rows * S) {
+                step = k;
+                j = (t - base) / S;
+                r = (t - base) % S;
+            }
+            base += (int)(rows * S);
+        }
+    }
+    // its factor columns: forward alpha, gamma; final D; backward Dinv, P, Q
+    Cx<double> f0[S], f1[S], f2[S];
+    int lv = 0;
+    i64 M = 1, m = 0;
+    if (step >= 0 && step < nf) {
+        lv = l0 + step;
+        M = (N + ((i64)1 << lv) - 1) >> lv;
+        m = 2 * j;
+        const Cx<double> *AG = F + off[2 * lv] * ss + j * 2 * ss + r;
+#pragma unroll
+        for (int c = 0; c < S; c++) { f0[c] = AG[(i64)c * S]; f1[c] = AG[ss + (i64)c * S]; }
+    } else if (step == nf) {
+        const Cx<double> *D = F + off[2 * L] * ss + r;
+#pragma unroll
+        for (int c = 0; c < S; c++) f0[c] = D[(i64)c * S];
+    } else if (step > nf) {
+        lv = L - 1 - (step - nf - 1);
+        M = (N + ((i64)1 << lv) - 1) >> lv;
+        m = 2 * j + 1;
+        const Cx<double> *D = F + off[2 * lv + 1] * ss + j * 3 * ss + r;
+#pragma unroll
+        for (int c = 0; c < S; c++) { f0[c] = D[(i64)c * S]; f1[c] = D[ss + (i64)c * S]; }
+        if (m + 1 < M) {
+#pragma unroll
+            for (int c = 0; c < S; c++) f2[c] = D[2 * ss + (i64)c * S];
+        }
+    }
+    const i64 hc = (i64)1 << (lv - l0);
+    __syncthreads();   /* the gathered rows visible */
+    // barriers stay outside the divergent branches
+    for (int k = 0; k < nsteps; k++) {
+        Cx<double> acc[NR];
+        if (step == k) {
+            if (k < nf) {
+#pragma unroll
+                for (int q = 0; q < NR; q++) acc[q] = L_(q, m * hc, r);
+                if (m >= 1) {
+#pragma unroll
+                    for (int c = 0; c < S; c++)
+#pragma unroll
+                        for (int q = 0; q < NR; q++) cfms(acc[q], f0[c], L_(q, (m - 1) * hc, c));
+                }
+                if (m + 1 < M) {
+#pragma unroll
+                    for (int c = 0; c < S; c++)
+#pragma unroll
+                        for (int q = 0; q < NR; q++) cfms(acc[q], f1[c], L_(q, (m + 1) * hc, c));
+                }
+            } else if (k == nf) {
+#pragma unroll
+                for (int q = 0; q < NR; q++) acc[q] = cx0();
+#pragma unroll
+                for (int c = 0; c < S; c++)
+#pragma unroll
+                    for (int q = 0; q < NR; q++) cfma(acc[q], f0[c], L_(q, 0, c));
+            } else {
+#pragma unroll
+                for (int q = 0; q < NR; q++) acc[q] = cx0();
+#pragma unroll
+                for (int c = 0; c < S; c++)
+#pragma unroll
+                    for (int q = 0; q < NR; q++) {
+                        cfma(acc[q], f0[c], L_(q, m * hc, c));
+                        cfms(acc[q], f1[c], L_(q, (m - 1) * hc, c));
+                    }
+                if (m + 1 < M) {
+#pragma unroll
+                    for (int c = 0; c < S; c++)
+#pragma unroll
+                        for (int q = 0; q < NR; q++) cfms(acc[q], f2[c], L_(q, (m + 1) * hc, c));
+                }
+            }
+        }
+        __syncthreads();
+        if (step == k) {
+            const i64 row = k == nf ? 0 : m * hc;
+#pragma unroll
+            for (int q = 0; q < NR; q++) L_(q, row, r) = acc[q];
+        }
+        __syncthreads();
+    }
+    for (i64 e = t; e < M0 * S * NR; e += 256) {
+        const int q = (int)(e / (M0 * S));
+        const i64 rem = e % (M0 * S), c = rem / S;
+        const int rr = (int)(rem % S);
+        x[q * vs + c * stride0 * S + rr] = lb[e];
+    }
+}
+
 // border, vector q: t = b2_q - dots_q; y2_q = Sinv t -> y2out + q ntau and the
 // tail of y_q.  One workgroup, NR * ntau threads used.  With dpart the dots
 // are first summed from nblk per-block partials (layout [block][a][q]), wave
@@ -1360,6 +1502,23 @@ int bbjd_o_bcr_tail(hipStream_t st, int s, int64_t N, int l0, int L, const int64
     const i64 M0 = (N + ((i64)1 << l0) - 1) >> l0;
     const size_t lds = (size_t)M0 * s * nrhs * sizeof(Cx<double>);
     if (tail_lds_fits(s, N, l0, nrhs)) {
+        // every step on threads of its own when they fit 256
+        i64 rows = 1;
+        for (int l = l0; l < L; l++) rows += (N + ((i64)1 << l) - 1) >> l;
+        if (g_bcr_prefetch && rows * s <= 256) {
+#define BBJD_TAIL_PF(NR, SS)                                                                                   \
+    hipLaunchKernelGGL((k_bcr_tail_pf<NR, SS>), dim3(1), dim3(256), lds, st, (i64)N, l0, L, (const i64 *)off,   \
+                       (const Cx<double> *)F, (const Cx<double> *)b, (Cx<double> *)x, (i64)vs)
+            if (nrhs == 2) {
+                if (s == 4) BBJD_TAIL_PF(2, 4);
+                else BBJD_TAIL_PF(2, 8);
+            } else {
+                if (s == 4) BBJD_TAIL_PF(1, 4);
+                else BBJD_TAIL_PF(1, 8);
+            }
+#undef BBJD_TAIL_PF
+            return chk();
+        }
 #define BBJD_TAIL_LDS(NR, SS)                                                                                  \
     hipLaunchKernelGGL((k_bcr_tail_lds<NR, SS>), dim3(1), dim3(1024), lds, st, (i64)N, l0, L, (const i64 *)off, \
                        (const Cx<double> *)F, (const Cx<double> *)b, (Cx<double> *)x, (i64)vs)
