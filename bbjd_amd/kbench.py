@@ -10,6 +10,13 @@ both passes always: for k <= 128 the fused form, first update and second
 projection in one pass over Q), gram (G = Q^H Q, k x k, on the FP64 matrix
 cores: also prints TFLOP/s for its 8 N K^2 flops).  Prints one line: kernel, shape, average us per
 launch, algorithmic bytes per launch and GB/s.
+
+bcr: the exact preconditioner's solve (block cyclic reduction with the dense
+rows' Schur border) on the serving workload's operator shape (block size 8,
+block-tridiagonal, 4 dense rows, sigma 0.5+0.5i), N rows, K = 1 or 2
+right-hand sides (bbjd_op_precond, bbjd_op_precond2), back to back on the
+null stream; prints the average us per solve.  Its per-kernel figures come
+from a kernel trace or counters around it.
 """
 from __future__ import annotations
 
@@ -32,15 +39,51 @@ def run(kernel, n, k, reps=50, proj_blocks=0, upd_rows=0, blocks_target=0):
     return ms.value, by.value
 
 
+def run_bcr(n, nrhs, reps=50):
+    """average ms per exact solve of nrhs right-hand sides"""
+    import time
+
+    import numpy as np
+
+    from .pencil import BandedTauPencil, DeviceOperator
+    if nrhs not in (1, 2):
+        raise ValueError("bcr takes K = 1 or 2 right-hand sides")
+    lib = _lib.load()
+    op = DeviceOperator(BandedTauPencil.generate(n, 8, 1, 4, seed=3), 0.5 + 0.5j, pc="exact")
+    try:
+        if nrhs == 1:
+            return op.time_apply("bbjd_op_precond", reps)
+        rng = np.random.default_rng(0)
+        xs = [_lib.DeviceBuffer.from_array(rng.standard_normal(n) + 1j * rng.standard_normal(n)) for _ in range(2)]
+        ys = [_lib.DeviceBuffer(16 * n) for _ in range(2)]
+
+        def solve():
+            if lib.bbjd_op_precond2(n, xs[0].ptr, ys[0].ptr, xs[1].ptr, ys[1].ptr, op.handle):
+                raise RuntimeError("bbjd_op_precond2 failed")
+        solve()
+        lib.bbjd_dev_sync()
+        t = time.perf_counter()
+        for _ in range(reps):
+            solve()
+        lib.bbjd_dev_sync()
+        return (time.perf_counter() - t) * 1e3 / reps
+    finally:
+        op.free()
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("kernel", choices=sorted(KERNELS))
+    ap.add_argument("kernel", choices=sorted(KERNELS) + ["bcr"])
     ap.add_argument("n", type=int)
     ap.add_argument("k", type=int)
     ap.add_argument("--reps", type=int, default=50)
     ap.add_argument("--proj-blocks", type=int, default=0, help="blocks per projection launch, 0 = default")
     ap.add_argument("--upd-rows", type=int, default=0, help="rows per block, 0 = chosen from the shape")
     a = ap.parse_args()
+    if a.kernel == "bcr":
+        ms = run_bcr(a.n, a.k, a.reps)
+        print(f"bcr n={a.n} rhs={a.k}: {ms * 1e3:.2f} us/solve", flush=True)
+        return
     ms, by = run(a.kernel, a.n, a.k, a.reps, a.proj_blocks, a.upd_rows)
     extra = ""
     if a.kernel == "gram":
