@@ -134,14 +134,24 @@ __global__ __launch_bounds__(TPB) void k_band_ab(i64 n, i64 n1, i64 nb, int bs_r
             const i64 bi = i / bs;
             const int rr = (int)(i % bs), nd = 2 * hw + 1;
             Cx<R> acc = {0, 0};
+            if constexpr (BS > 0) {
 #pragma unroll
-            for (int d = 0; d < nd; d++) {
-                const i64 bj = bi + d - hw;
-                if (bj < 0 || bj >= nb) continue;
-                const Cx<R> *blk = band + (bi * nd + d) * bs * bs + rr;
-                const Cx<R> *xs = x + bj * bs;
+                for (int d = 0; d < 2 * HW + 1; d++) {
+                    const i64 bj = bi + d - HW;
+                    if (bj < 0 || bj >= nb) continue;
+                    const Cx<R> *blk = band + (bi * nd + d) * BS * BS + rr;
+                    const Cx<R> *xs = x + bj * BS;
 #pragma unroll
-                for (int c = 0; c < bs; c++) cfma(acc, blk[(i64)c * bs], xs[c]);
+                    for (int c = 0; c < BS; c++) cfma(acc, blk[(i64)c * BS], xs[c]);
+                }
+            } else {
+                for (int d = 0; d < nd; d++) {
+                    const i64 bj = bi + d - hw;
+                    if (bj < 0 || bj >= nb) continue;
+                    const Cx<R> *blk = band + (bi * nd + d) * bs * bs + rr;
+                    const Cx<R> *xs = x + bj * bs;
+                    for (int c = 0; c < bs; c++) cfma(acc, blk[(i64)c * bs], xs[c]);
+                }
             }
             ya[i] = acc;
             if (r) {

@@ -78,6 +78,7 @@ void bbjd_serve_config_init(bbjd_serve_config *c)
     c->realtime = 1;
     c->sync_yield = 1;
     c->order = BBJD_ORDER_ARRIVAL;
+    c->window = 8;
 }
 
 static bbjd_int req_n(const bbjd_request *q) { return q->op ? bbjd_op_size(q->op) : (q->pencil ? q->pencil->n : 0); }
@@ -298,8 +299,11 @@ int bbjd_serve(bbjd_request *reqs, int nreq, const bbjd_serve_config *cfg, bbjd_
         while (next < nreq && (!cfg->realtime || reqs[order[next]].arrival <= now || npend == 0))
             pend[npend++] = order[next++];
         int pi = 0;
-        if (cfg->order == BBJD_ORDER_LARGEST)
-            for (int i = 1; i < npend; i++) {
+        int cand = 1;       /* the waiting requests compared, oldest first */
+        if (cfg->order == BBJD_ORDER_LARGEST) cand = npend;
+        else if (cfg->order == BBJD_ORDER_WINDOW) cand = cfg->window < 1 ? 1 : (cfg->window < npend ? cfg->window : npend);
+        if (cand > 1)
+            for (int i = 1; i < cand; i++) {
                 const bbjd_request *a = &reqs[pend[i]], *b = &reqs[pend[pi]];
                 if (a->nev > b->nev || (a->nev == b->nev && req_n(a) > req_n(b))) pi = i;
             }

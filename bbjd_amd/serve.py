@@ -33,7 +33,7 @@ class Request(C.Structure):
 class ServeConfig(C.Structure):
     _fields_ = [("workers", C.c_int), ("hbm_budget", C.c_size_t), ("precision", C.c_int),
                 ("realtime", C.c_int), ("sync_yield", C.c_int), ("graphs", C.c_int), ("order", C.c_int),
-                ("pin", C.c_int)]
+                ("pin", C.c_int), ("window", C.c_int)]
 
 
 class ServeReport(C.Structure):
@@ -66,13 +66,16 @@ def device_init(sync_yield=True):
 
 
 def serve(jobs, workers=16, precision="fp64", realtime=True, hbm_budget=0, keep_vectors=False,
-          sync_yield=True, graphs=False, order="arrival", pin=False):
+          sync_yield=True, graphs=False, order="arrival", pin=False, window=8):
     """Run jobs through the serving loop; returns (results, report).  graphs:
     capture each request's inner GMRES steps into HIP graphs and replay them.
     order: which arrived request a free worker takes, "arrival" (first come,
-    first served) or "largest" (largest estimated work: nev, then n)."""
-    if order not in ("arrival", "largest"):
-        raise ValueError("order must be 'arrival' or 'largest'")
+    first served), "largest" (largest estimated work: nev, then n) or
+    "window" (the largest of the `window` longest-waiting ones)."""
+    if order not in ("arrival", "largest", "window"):
+        raise ValueError("order must be 'arrival', 'largest' or 'window'")
+    if int(window) < 1:
+        raise ValueError("window must be at least 1")
     lib = _lib.load()
     _setup(lib)
     nreq = len(jobs)
@@ -123,7 +126,8 @@ def serve(jobs, workers=16, precision="fp64", realtime=True, hbm_budget=0, keep_
     cfg.realtime = int(bool(realtime))
     cfg.sync_yield = 2 if sync_yield == "block" else int(bool(sync_yield))
     cfg.graphs = int(bool(graphs))
-    cfg.order = 1 if order == "largest" else 0
+    cfg.order = {"arrival": 0, "largest": 1, "window": 2}[order]
+    cfg.window = int(window)
     cfg.pin = int(bool(pin))
     rep = ServeReport()
     rc = lib.bbjd_serve(reqs, nreq, C.byref(cfg), C.byref(rep))

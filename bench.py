@@ -154,8 +154,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--plain-only", action="store_true", help="skip the served steps (profiling the plain leg)")
     ap.add_argument("--no-tile-table", action="store_true", help="engine default tiles instead of the autotuned table")
-    ap.add_argument("--order", choices=["arrival", "largest"], default="arrival",
+    ap.add_argument("--order", choices=["arrival", "largest", "window"], default="arrival",
                     help="which arrived request a free worker takes (serving loop)")
+    ap.add_argument("--window", type=int, default=8,
+                    help="--order window: the largest of this many longest-waiting requests")
     ap.add_argument("--sync", choices=["yield", "block"], default="yield",
                     help="how host threads wait on the GPU")
     ap.add_argument("--pin", action="store_true", help="pin each serving worker to one CPU")
@@ -216,7 +218,7 @@ def main():
     for s in range(0 if args.plain_only else args.steps):
         t = time.perf_counter()
         res, rep = srv.serve(jobs_for(all_idx), workers=args.workers, realtime=wl["realtime"], order=args.order,
-                             pin=args.pin, sync_yield="block" if args.sync == "block" else True,
+                             window=args.window, pin=args.pin, sync_yield="block" if args.sync == "block" else True,
                              precision=wl["precision"])
         walls.append(time.perf_counter() - t)
         served.append((res, rep))
@@ -351,7 +353,8 @@ def main():
                                 f"sigma on the unit disk, tol 1e-10, maxsub 32, minsub 16, maxiter "
                                 f"{args.maxiter}, GMRES(20) inner solve in complex64 (inner tol 1e-3) with "
                                 f"the exact banded LU as its preconditioner"),
-                   "workers": args.workers, "requests": args.nreq, "admission": args.order,
+                   "workers": args.workers, "requests": args.nreq,
+                   "admission": args.order + (f" {args.window}" if args.order == "window" else ""),
                    "host_wait": args.sync, "workers_pinned": bool(args.pin)},
         "served_eigenpairs_per_s": round(served_rate, 3) if served_rate else None,
         "plain_eigenpairs_per_s": round(plain_rate, 3) if plain_rate else None,

@@ -55,14 +55,19 @@ typedef struct {
                              2: they block (sleep until an interrupt)        */
     int    graphs;        /* bbjd_gpu_options.graphs of every request        */
     int    order;         /* which arrived request a free worker takes:
-                             BBJD_ORDER_ARRIVAL (first come, first served)
-                             or BBJD_ORDER_LARGEST (largest estimated work:
-                             nev, then n)                                    */
+                             BBJD_ORDER_ARRIVAL (first come, first served),
+                             BBJD_ORDER_LARGEST (largest estimated work:
+                             nev, then n) or BBJD_ORDER_WINDOW (the largest
+                             of the `window` longest-waiting ones)           */
     int    pin;           /* 1: worker i runs on the i-th CPU the process may
                              use                                             */
+    int    window;        /* BBJD_ORDER_WINDOW: how many of the longest-
+                             waiting requests are compared (default 8); a
+                             request is overtaken by at most window - 1
+                             later arrivals                                  */
 } bbjd_serve_config;
 
-enum { BBJD_ORDER_ARRIVAL = 0, BBJD_ORDER_LARGEST = 1 };
+enum { BBJD_ORDER_ARRIVAL = 0, BBJD_ORDER_LARGEST = 1, BBJD_ORDER_WINDOW = 2 };
 
 typedef struct {
     double   wall_seconds;

@@ -105,7 +105,7 @@ int main(void) {
   O(bbjd_stats, graph_captures); O(bbjd_stats, graph_launches);
   O(bbjd_serve_config, workers); O(bbjd_serve_config, hbm_budget); O(bbjd_serve_config, precision);
   O(bbjd_serve_config, realtime); O(bbjd_serve_config, sync_yield); O(bbjd_serve_config, graphs);
-  O(bbjd_serve_config, order); O(bbjd_serve_config, pin);
+  O(bbjd_serve_config, order); O(bbjd_serve_config, pin); O(bbjd_serve_config, window);
   return 0;
 }
 """

@@ -561,15 +561,28 @@ def test_serving_loop_results_alone_equal_mix():
     """bbjd_serve (continuous batching over resident workspaces, operators
     uploaded and factored on admission): each request's eigenvalues are
     bit-identical alone and in a concurrent mix -- arrivals in real time,
-    largest-first admission, and an HBM budget that throttles admission --
-    every pair's FP64 residual ||A x - lambda B x|| is below
-    tol (||A|| + |lambda| ||B||), and two requests match the oracle"""
+    largest-first admission, windowed admission, and an HBM budget that
+    throttles admission -- every pair's FP64 residual ||A x - lambda B x|| is
+    below tol (||A|| + |lambda| ||B||), and two requests match the oracle.
+    With one worker and every request waiting from the start, the windowed
+    order admits the largest (nev, then n) of the 3 longest-waiting requests
+    each time, the oldest on a tie"""
     from bbjd_amd import serve as srv
     from bbjd_amd.configs import row_norm_bound
     jobs = _serve_jobs()
     mix, rep = srv.serve(jobs, workers=4, realtime=True)
     assert rep["failed"] == 0 and rep["peak_concurrency"] >= 2
     lpt, _ = srv.serve(jobs, workers=4, realtime=False, order="largest")
+    win, _ = srv.serve(jobs, workers=1, realtime=False, order="window", window=3)
+    pend, expect = list(range(len(jobs))), []
+    while pend:
+        best = pend[0]
+        for c in pend[1:3]:
+            if (jobs[c]["nev"], jobs[c]["pencil"].n) > (jobs[best]["nev"], jobs[best]["pencil"].n):
+                best = c
+        expect.append(best)
+        pend.remove(best)
+    assert sorted(range(len(jobs)), key=lambda i: win[i]["t_admit"]) == expect
     budget = rep["peak_bytes"] // 2
     tight, rep_t = srv.serve(jobs, workers=4, realtime=False, hbm_budget=budget)
     # admission keeps within the budget, except that a request runs alone
@@ -587,7 +600,7 @@ def test_serving_loop_results_alone_equal_mix():
         alone, _ = srv.serve([j], workers=1, realtime=False)
         a = alone[0]
         assert a["status"] == 0 and a["nconv"] == j["nev"], i
-        for tag, other in (("mix", mix[i]), ("largest-first", lpt[i]), ("tight", tight[i])):
+        for tag, other in (("mix", mix[i]), ("largest-first", lpt[i]), ("window", win[i]), ("tight", tight[i])):
             np.testing.assert_array_equal(other["eigenvalues"], a["eigenvalues"], err_msg=f"{i} {tag}")
         anorm = row_norm_bound(j["pencil"])           # lower bound of ||A||: a stricter test
         assert np.all(a["pair_residuals"] <= 1e-10 * (anorm + np.abs(a["eigenvalues"]))), i
