@@ -653,3 +653,23 @@ def test_serving_loop_mixed_precision_alone_equal_mix():
         w = j["sigma"] + 1.0 / mu
         for ev in np.concatenate([a["eigenvalues"], f64[i]["eigenvalues"]]):
             assert np.min(np.abs(w - ev)) <= 1e-8 * max(1.0, abs(ev)), (i, ev)
+
+
+def test_serving_loop_isolates_failing_requests():
+    """a request the engine rejects (maxsub 1: BBJD_ERR_INVALID from the
+    host's validation, before any solve kernel) and one that runs out of
+    iterations (maxiter 2: status 1, fewer than nev pairs) are reported as
+    such, the report counts the one failure, and every other request's
+    eigenvalues are bit-identical to those of a run without the two"""
+    from bbjd_amd import serve as srv
+    good = _serve_jobs()[:4]
+    bad = dict(good[1], maxsub=1, arrival=0.01)
+    short = dict(good[2], maxiter=2, arrival=0.03)
+    res, rep = srv.serve(good[:2] + [bad] + good[2:] + [short], workers=3, realtime=False)
+    ref, rep_ref = srv.serve(good, workers=3, realtime=False)
+    assert rep_ref["failed"] == 0 and rep["failed"] == 1
+    assert res[2]["status"] == -2 and res[2]["nconv"] == 0
+    assert res[5]["status"] == 1 and res[5]["nconv"] < short["nev"]
+    for a, b in zip(res[:2] + res[3:5], ref):
+        assert a["status"] == 0 and a["nconv"] == b["nconv"]
+        np.testing.assert_array_equal(a["eigenvalues"], b["eigenvalues"])
