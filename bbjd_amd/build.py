@@ -8,14 +8,16 @@ import subprocess
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def _make(path, jobs=8):
-    subprocess.run(["make", "-C", path, f"-j{jobs}"], check=True)
+def _make(path, jobs=8, force=True):
+    subprocess.run(["make", "-C", path, f"-j{jobs}"] + (["-B"] if force else []), check=True)
 
 
-def build(jobs=8, oracle=True):
-    _make(os.path.join(ROOT, "bbjd_amd", "csrc"), jobs)
+def build(jobs=8, oracle=True, force=True):
+    """force: recompile every source (make -B), also when the built
+    libraries are up to date"""
+    _make(os.path.join(ROOT, "bbjd_amd", "csrc"), jobs, force)
     if oracle:
-        _make(os.path.join(ROOT, "oracle"), jobs)
+        _make(os.path.join(ROOT, "oracle"), jobs, force)
 
 
 if __name__ == "__main__":
