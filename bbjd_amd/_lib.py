@@ -114,6 +114,7 @@ def load():
     _sig(lib, "bbjd_set_bcr_tail_rows", None, C.c_int)
     _sig(lib, "bbjd_set_bcr_border_fuse", None, C.c_int)
     _sig(lib, "bbjd_set_bcr_prefetch", None, C.c_int)
+    _sig(lib, "bbjd_set_c64_direct_io", None, C.c_int)
     _sig(lib, "bbjd_set_cgs_mid_blocks", None, C.c_int)
     _sig(lib, "bbjd_set_tile_table", C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p)
     _sig(lib, "bbjd_block_rotate", C.c_int, I64, I64, I64, C.c_void_p, C.c_void_p)

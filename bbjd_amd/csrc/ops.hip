@@ -760,12 +760,14 @@ __global__ __launch_bounds__(TPB) void k_bcr_bwd_multi_pf(i64 N, int l, BcrLevel
     }
 }
 
+// f0 (one right-hand side, instead of x0): the caller's vector in complex64,
+// widened exactly on load.
 template <int NR, int S, int G>
 __global__ __launch_bounds__(TPB) void k_bcr_fwd_multi_pf(i64 N, int l, i64 n1, BcrLevels ag,
                                                           const Cx<double> *__restrict__ x0,
                                                           const Cx<double> *__restrict__ x1,
                                                           const Cx<double> *__restrict__ bin, Cx<double> *bout,
-                                                          i64 vs)
+                                                          i64 vs, const Cx<float> *__restrict__ f0)
 {
     constexpr int U = TPB / ((1 << (G - 1)) * S) - 1;
     constexpr int ROWS = (U + 1) << G;
@@ -779,7 +781,10 @@ __global__ __launch_bounds__(TPB) void k_bcr_fwd_multi_pf(i64 N, int l, i64 n1, 
         const i64 il = p0 + sl, bi = il << l;
         Cx<double> v = cx0();
         if (il >= 0 && bi < N) {
-            if (x0) {
+            if (f0) {
+                const i64 el = bi * S + r;
+                if (el < n1) v = cx((double)f0[el].re, (double)f0[el].im);
+            } else if (x0) {
                 const i64 el = bi * S + r;
                 if (el < n1) v = xs[q][el];
             } else
@@ -1196,11 +1201,13 @@ __global__ __launch_bounds__(256) void k_bcr_tail_pf(i64 N, int l0, int L, const
 // tail of y_q.  One workgroup, NR * ntau threads used.  With dpart the dots
 // are first summed from nblk per-block partials (layout [block][a][q]), wave
 // o handling outputs o, o + waves, ... in a fixed order.
+// b2f / ytf (one right-hand side): the dense rows' right-hand side read and
+// the tail of y written in complex64 instead of b2_0 / ytail0.
 template <int NR>
 __global__ void k_border_small(int ntau, const Cx<double> *b2_0, const Cx<double> *b2_1, const Cx<double> *dots0,
                                const Cx<double> *dots1, const Cx<double> *__restrict__ Sinv, Cx<double> *y2out,
                                Cx<double> *ytail0, Cx<double> *ytail1, const Cx<double> *__restrict__ dpart,
-                               i64 nblk)
+                               i64 nblk, const Cx<float> *b2f, Cx<float> *ytf)
 {
     extern __shared__ char sm[];
     Cx<double> *tt = reinterpret_cast<Cx<double> *>(sm);
@@ -1225,7 +1232,7 @@ __global__ void k_border_small(int ntau, const Cx<double> *b2_0, const Cx<double
         __syncthreads();
     }
     if (act) {
-        const Cx<double> b2 = (q == 0 ? b2_0 : b2_1)[a];
+        const Cx<double> b2 = b2f ? cx((double)b2f[a].re, (double)b2f[a].im) : (q == 0 ? b2_0 : b2_1)[a];
         const Cx<double> d = dpart ? dd[a * NR + q] : (q == 0 ? dots0 : dots1)[a];
         tt[q * ntau + a] = cx(b2.re - d.re, b2.im - d.im);
     }
@@ -1234,16 +1241,18 @@ __global__ void k_border_small(int ntau, const Cx<double> *b2_0, const Cx<double
         Cx<double> acc = cx0();
         for (int c = 0; c < ntau; c++) cfma(acc, Sinv[(i64)c * ntau + a], tt[q * ntau + c]);
         y2out[q * ntau + a] = acc;
-        (q == 0 ? ytail0 : ytail1)[a] = acc;
+        if (ytf) ytf[a] = Cx<float>{(float)acc.re, (float)acc.im};
+        else (q == 0 ? ytail0 : ytail1)[a] = acc;
     }
 }
 
-// y_q[r] = xb_q[r] - sum_a Y[a n1 + r] y2_q[a], r < n1
+// y_q[r] = xb_q[r] - sum_a Y[a n1 + r] y2_q[a], r < n1; with y0f (one
+// right-hand side) y_0 is written in complex64 there instead.
 template <int NR>
 __global__ __launch_bounds__(TPB) void k_border_update(i64 n1, int ntau, const Cx<double> *__restrict__ xb, i64 vs,
                                                        const Cx<double> *__restrict__ Y,
                                                        const Cx<double> *__restrict__ y2, Cx<double> *y0,
-                                                       Cx<double> *y1)
+                                                       Cx<double> *y1, Cx<float> *y0f)
 {
     for (i64 i = (i64)blockIdx.x * TPB + threadIdx.x; i < n1; i += (i64)gridDim.x * TPB) {
         Cx<double> acc[NR];
@@ -1254,7 +1263,8 @@ __global__ __launch_bounds__(TPB) void k_border_update(i64 n1, int ntau, const C
 #pragma unroll
             for (int q = 0; q < NR; q++) cfms(acc[q], yv, y2[q * ntau + a]);
         }
-        y0[i] = acc[0];
+        if (y0f) y0f[i] = Cx<float>{(float)acc[0].re, (float)acc[0].im};
+        else y0[i] = acc[0];
         if (NR == 2) y1[i] = acc[NR - 1];
     }
 }
@@ -1458,11 +1468,14 @@ int bbjd_o_bcr_bwd_multi(hipStream_t st, int s, int64_t N, int l, int G, const v
 }
 
 int bbjd_o_bcr_fwd_multi(hipStream_t st, int s, int64_t N, int64_t n1, int l, int G, const void *const *ag,
-                         const void *x0, const void *x1, const void *bin, void *bout, int nrhs, int64_t vs)
+                         const void *x0, const void *x1, const void *bin, void *bout, int nrhs, int64_t vs,
+                         int in_c64)
 {
     if (!bcr_group_ok(s, G) || (nrhs != 1 && nrhs != 2) || (x0 && (l != 0 || (nrhs == 2 && !x1))) ||
         (!x0 && (!bin || bin == bout)))
         return -3;
+    // complex64 input: one right-hand side through the prefetching kernels
+    if (in_c64 && (nrhs != 1 || !x0 || s > 8 || !g_bcr_prefetch)) return -3;
     BcrLevels f;
     for (int g = 0; g < 4; g++) f.f[g] = (const Cx<double> *)(g < G ? ag[g] : nullptr);
     const i64 Ml = (N + ((i64)1 << l) - 1) >> l, units = (Ml + ((i64)1 << G) - 1) >> G;
@@ -1473,8 +1486,11 @@ int bbjd_o_bcr_fwd_multi(hipStream_t st, int s, int64_t N, int64_t n1, int l, in
         const Cx<double> *a0 = (const Cx<double> *)x0, *a1 = (const Cx<double> *)(x1 ? x1 : x0);               \
         if constexpr (S <= 8) {                                                                                \
             if (g_bcr_prefetch) {                                                                              \
+                const Cx<float> *f0 = in_c64 ? (const Cx<float> *)x0 : nullptr;                                \
+                if (in_c64) a0 = a1 = nullptr;                                                                 \
                 hipLaunchKernelGGL((k_bcr_fwd_multi_pf<NR, S, GG>), dim3(nb), dim3(TPB), 0, st, (i64)N, l,     \
-                                   (i64)n1, f, a0, a1, (const Cx<double> *)bin, (Cx<double> *)bout, (i64)vs);  \
+                                   (i64)n1, f, a0, a1, (const Cx<double> *)bin, (Cx<double> *)bout, (i64)vs,   \
+                                   f0);                                                                        \
                 break;                                                                                         \
             }                                                                                                  \
         }                                                                                                      \
@@ -1552,34 +1568,39 @@ int bbjd_o_bcr_tail(hipStream_t st, int s, int64_t N, int l0, int L, const int64
 
 int bbjd_o_border_small(hipStream_t st, int ntau, const void *b2_0, const void *b2_1, const void *dots0,
                         const void *dots1, const void *Sinv, void *y2, void *ytail0, void *ytail1, int nrhs,
-                        const void *dpart, int64_t nblk)
+                        const void *dpart, int64_t nblk, int c64)
 {
-    if (ntau <= 0 || ntau * nrhs > 1024 || (dpart && ntau > 16)) return -2;
+    if (ntau <= 0 || ntau * nrhs > 1024 || (dpart && ntau > 16) || (c64 && nrhs != 1)) return -2;
     const int thr = dpart ? 1024 : ntau * nrhs <= 64 ? 64 : (ntau * nrhs <= 256 ? 256 : 1024);
     if (nrhs == 2)
         hipLaunchKernelGGL(k_border_small<2>, dim3(1), dim3(thr), (size_t)2 * ntau * 16, st, ntau,
                            (const Cx<double> *)b2_0, (const Cx<double> *)b2_1, (const Cx<double> *)dots0,
                            (const Cx<double> *)dots1, (const Cx<double> *)Sinv, (Cx<double> *)y2,
-                           (Cx<double> *)ytail0, (Cx<double> *)ytail1, (const Cx<double> *)dpart, (i64)nblk);
+                           (Cx<double> *)ytail0, (Cx<double> *)ytail1, (const Cx<double> *)dpart, (i64)nblk,
+                           (const Cx<float> *)nullptr, (Cx<float> *)nullptr);
     else
         hipLaunchKernelGGL(k_border_small<1>, dim3(1), dim3(thr), (size_t)ntau * 16, st, ntau,
-                           (const Cx<double> *)b2_0, (const Cx<double> *)b2_0, (const Cx<double> *)dots0,
-                           (const Cx<double> *)dots0, (const Cx<double> *)Sinv, (Cx<double> *)y2,
-                           (Cx<double> *)ytail0, (Cx<double> *)ytail0, (const Cx<double> *)dpart, (i64)nblk);
+                           c64 ? nullptr : (const Cx<double> *)b2_0, c64 ? nullptr : (const Cx<double> *)b2_0,
+                           (const Cx<double> *)dots0, (const Cx<double> *)dots0, (const Cx<double> *)Sinv,
+                           (Cx<double> *)y2, c64 ? nullptr : (Cx<double> *)ytail0,
+                           c64 ? nullptr : (Cx<double> *)ytail0, (const Cx<double> *)dpart, (i64)nblk,
+                           c64 ? (const Cx<float> *)b2_0 : nullptr, c64 ? (Cx<float> *)ytail0 : nullptr);
     return chk();
 }
 
 int bbjd_o_border_update(hipStream_t st, int64_t n1, int ntau, const void *xb, int64_t vs, const void *Y,
-                         const void *y2, void *y0, void *y1, int nrhs)
+                         const void *y2, void *y0, void *y1, int nrhs, int c64)
 {
+    if (c64 && nrhs != 1) return -2;
     if (nrhs == 2)
         hipLaunchKernelGGL(k_border_update<2>, dim3(gs_blocks(n1)), dim3(TPB), 0, st, (i64)n1, ntau,
                            (const Cx<double> *)xb, (i64)vs, (const Cx<double> *)Y, (const Cx<double> *)y2,
-                           (Cx<double> *)y0, (Cx<double> *)y1);
+                           (Cx<double> *)y0, (Cx<double> *)y1, (Cx<float> *)nullptr);
     else
         hipLaunchKernelGGL(k_border_update<1>, dim3(gs_blocks(n1)), dim3(TPB), 0, st, (i64)n1, ntau,
                            (const Cx<double> *)xb, (i64)vs, (const Cx<double> *)Y, (const Cx<double> *)y2,
-                           (Cx<double> *)y0, (Cx<double> *)y0);
+                           c64 ? nullptr : (Cx<double> *)y0, c64 ? nullptr : (Cx<double> *)y0,
+                           c64 ? (Cx<float> *)y0 : nullptr);
     return chk();
 }
 

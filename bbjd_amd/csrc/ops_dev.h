@@ -42,20 +42,25 @@ int bbjd_o_bcr_bwd(hipStream_t st, int s, int64_t M, int64_t h, int64_t no, cons
 int bbjd_o_bcr_bwd_multi(hipStream_t st, int s, int64_t N, int l, int G, const void *const *dpq,
                          const void *const *bl, void *x, int nrhs, int64_t vs, const void *tauc, int ntau,
                          int64_t n1, int64_t ldt, void *dpart, int64_t dpart_cap, int64_t *nblk_out);
+/* in_c64: x0 is complex64 (one right-hand side, l = 0, s = 4 or 8 with the
+ * prefetching kernels; else -3 before any launch) */
 int bbjd_o_bcr_fwd_multi(hipStream_t st, int s, int64_t N, int64_t n1, int l, int G, const void *const *ag,
-                         const void *x0, const void *x1, const void *bin, void *bout, int nrhs, int64_t vs);
+                         const void *x0, const void *x1, const void *bin, void *bout, int nrhs, int64_t vs,
+                         int in_c64);
 /* first level of the one-workgroup tail for this shape (>= l0, where the
  * LDS-resident tail fits; l0 when the global tail is used) */
 int bbjd_o_bcr_tail_level(int s, int64_t N, int l0, int L, int nrhs);
 int bbjd_o_bcr_tail(hipStream_t st, int s, int64_t N, int l0, int L, const int64_t *off, const void *F, void *b,
                     void *x, int nrhs, int64_t vs);
 /* the border's y2 = Sinv (b2 - dots); with dpart (nblk x ntau x nrhs
- * partials from bbjd_o_bcr_bwd_multi) the dots are summed from it first */
+ * partials from bbjd_o_bcr_bwd_multi) the dots are summed from it first;
+ * c64 (one right-hand side): b2_0 and ytail0 are complex64 */
 int bbjd_o_border_small(hipStream_t st, int ntau, const void *b2_0, const void *b2_1, const void *dots0,
                         const void *dots1, const void *Sinv, void *y2, void *ytail0, void *ytail1, int nrhs,
-                        const void *dpart, int64_t nblk);
+                        const void *dpart, int64_t nblk, int c64);
+/* c64 (one right-hand side): y0 is complex64 */
 int bbjd_o_border_update(hipStream_t st, int64_t n1, int ntau, const void *xb, int64_t vs, const void *Y,
-                         const void *y2, void *y0, void *y1, int nrhs);
+                         const void *y2, void *y0, void *y1, int nrhs, int c64);
 #ifdef __cplusplus
 }
 #endif

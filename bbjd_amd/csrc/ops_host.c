@@ -262,6 +262,11 @@ void bbjd_set_bcr_group(int levels) { g_bcr_group = levels < 1 ? 1 : (levels > 4
  * Schur border (no separate projection and finish) */
 static int g_bcr_border_fuse = 1;
 void bbjd_set_bcr_border_fuse(int on) { g_bcr_border_fuse = on != 0; }
+/* complex64 applies of the exact preconditioner read and write complex64 in
+ * the solve's own first and last launches (1) or convert in launches of
+ * their own (0) */
+static int g_c64_io = 1;
+void bbjd_set_c64_direct_io(int on) { g_c64_io = on != 0; }
 
 /* BCR solve of nrhs right-hand sides into op->xvec: the large levels in
  * launches of up to g_bcr_group levels, the levels from the tail level on,
@@ -276,8 +281,10 @@ void bbjd_set_bcr_border_fuse(int on) { g_bcr_border_fuse = on != 0; }
  * and bvec2; the odd rows of level l are final once its group has run and
  * stay in the buffer it wrote (later groups write only multiples of higher
  * powers of two), which the backward level reads (bl[l]). */
+/* in_c64: x0 is complex64 (one right-hand side); -3 before any launch when
+ * the first reduction launch cannot read it */
 static int bcr_solve(struct bbjd_op *op, hipStream_t st, int nrhs, const void *x0, const void *x1, int want_dots,
-                     int64_t *dots_nblk)
+                     int64_t *dots_nblk, int in_c64)
 {
     if (dots_nblk) *dots_nblk = 0;
     const int s = op->s;
@@ -285,6 +292,7 @@ static int bcr_solve(struct bbjd_op *op, hipStream_t st, int nrhs, const void *x
     const int64_t vs = (int64_t)op->N * s;
     int rc = 0;
     const int lt = bbjd_o_bcr_tail_level(s, op->N, op->l0, op->L, nrhs);
+    if (in_c64 && (!x0 || s > 64 || (lt < g_bcr_group ? lt : g_bcr_group) < 2)) return -3;
     void *buf[2] = {op->bvec, op->bvec2};
     int cur = 0;
     int bl[64];
@@ -295,8 +303,9 @@ static int bcr_solve(struct bbjd_op *op, hipStream_t st, int nrhs, const void *x
         int e = -3;
         if (G >= 2) {
             for (int g = 0; g < G; g++) lv[g] = (const zcx *)op->bcr + (size_t)op->even_off[g] * ss;
-            e = bbjd_o_bcr_fwd_multi(st, s, op->N, op->n1, 0, G, lv, x0, x1, NULL, buf[0], nrhs, vs);
+            e = bbjd_o_bcr_fwd_multi(st, s, op->N, op->n1, 0, G, lv, x0, x1, NULL, buf[0], nrhs, vs, in_c64);
             if (e == 0) l = G;
+            else if (in_c64) return e;
             else if (e != -3) rc |= e;
         }
         if (e == -3) {
@@ -315,7 +324,7 @@ static int bcr_solve(struct bbjd_op *op, hipStream_t st, int nrhs, const void *x
         if (G >= 2) {
             for (int g = 0; g < G; g++) lv[g] = (const zcx *)op->bcr + (size_t)op->even_off[l + g] * ss;
             const int e = bbjd_o_bcr_fwd_multi(st, s, op->N, op->n1, l, G, lv, NULL, NULL, buf[cur], buf[1 - cur],
-                                               nrhs, vs);
+                                               nrhs, vs, 0);
             if (e == 0) {
                 cur = 1 - cur;
                 for (int g = 0; g < G; g++) bl[l + g] = cur;
@@ -356,14 +365,23 @@ static int bcr_solve(struct bbjd_op *op, hipStream_t st, int nrhs, const void *x
 }
 
 /* y_q = M^-1 x_q for nrhs (1 or 2) vectors */
+/* c64: x0 and y0 are complex64 (one right-hand side), read and written by
+ * the first reduction launch and the border kernels; -3 before any launch
+ * when this operator's shape does not allow it (no dense rows, s > 8, too
+ * few levels before the tail) */
 static int exact_apply(struct bbjd_op *op, hipStream_t st, int nrhs, const void *x0, void *y0, const void *x1,
-                       void *y1)
+                       void *y1, int c64)
 {
     int rc = 0;
     const bbjd_int n1 = op->n1;
     const int64_t vs = (int64_t)op->N * op->s;
     int64_t dnb = 0;
-    rc |= bcr_solve(op, st, nrhs, x0, x1, op->ntau > 0 && g_bcr_border_fuse, &dnb);
+    if (c64 && (nrhs != 1 || op->ntau <= 0)) return -3;
+    {
+        const int e = bcr_solve(op, st, nrhs, x0, x1, op->ntau > 0 && g_bcr_border_fuse, &dnb, c64);
+        if (c64 && e == -3) return -3;
+        rc |= e;
+    }
     if (op->ntau == 0) {
         rc |= !HIPOK(hipMemcpyAsync(y0, op->xvec, (size_t)n1 * 16, hipMemcpyDeviceToDevice, st));
         if (nrhs == 2)
@@ -375,10 +393,13 @@ static int exact_apply(struct bbjd_op *op, hipStream_t st, int nrhs, const void 
     if (!dnb)
         rc |= bbjd_k_gemv_c(st, n1, nt, op->tauc, op->n, op->xvec, nrhs == 2 ? (zcx *)op->xvec + vs : NULL, sm,
                             sm + nt, &op->rd);
-    rc |= bbjd_o_border_small(st, nt, (const zcx *)x0 + n1, nrhs == 2 ? (const zcx *)x1 + n1 : NULL, sm, sm + nt,
-                              op->Sinv, sm + 2 * nt, (zcx *)y0 + n1, nrhs == 2 ? (zcx *)y1 + n1 : NULL, nrhs,
-                              dnb ? op->rd.part : NULL, dnb);
-    rc |= bbjd_o_border_update(st, n1, nt, op->xvec, vs, op->Y, sm + 2 * nt, y0, y1, nrhs);
+    /* the dense rows' part of x and y: complex64 elements are 8 bytes */
+    const void *b2 = c64 ? (const void *)((const char *)x0 + (size_t)n1 * 8) : (const void *)((const zcx *)x0 + n1);
+    void *yt = c64 ? (void *)((char *)y0 + (size_t)n1 * 8) : (void *)((zcx *)y0 + n1);
+    rc |= bbjd_o_border_small(st, nt, b2, nrhs == 2 ? (const zcx *)x1 + n1 : NULL, sm, sm + nt, op->Sinv,
+                              sm + 2 * nt, yt, nrhs == 2 ? (zcx *)y1 + n1 : NULL, nrhs, dnb ? op->rd.part : NULL, dnb,
+                              c64);
+    rc |= bbjd_o_border_update(st, n1, nt, op->xvec, vs, op->Y, sm + 2 * nt, y0, y1, nrhs, c64);
     return rc;
 }
 
@@ -589,7 +610,7 @@ static bbjd_op *op_create_impl(const bbjd_pencil_desc *d, bbjd_zomplex sigma, bb
                 memset(col, 0, (size_t)n1 * sizeof(zcx));
                 for (bbjd_int i = lo; i < n1; i++) col[i] = m_at(d, sigma, i, n1 + a);
                 rc |= !HIPOK(hipMemcpy(op->bvec, col, (size_t)n1 * 16, hipMemcpyHostToDevice));
-                rc |= bcr_solve(op, st, 1, NULL, NULL, 0, NULL);
+                rc |= bcr_solve(op, st, 1, NULL, NULL, 0, NULL, 0);
                 rc |= !HIPOK(hipMemcpyAsync((zcx *)op->Y + (size_t)a * n1, op->xvec, (size_t)n1 * 16,
                                             hipMemcpyDeviceToDevice, st));
                 /* M21 Y[:, a] */
@@ -682,7 +703,7 @@ int bbjd_op_precond(bbjd_int n, const bbjd_zomplex *x, bbjd_zomplex *y, void *da
     if (!op || n != op->n) return -1;
     hipStream_t st = op_stream(op);
     switch (op->pc) {
-    case BBJD_PC_EXACT_LU: return exact_apply(op, st, 1, x, y, NULL, NULL) ? -1 : 0;
+    case BBJD_PC_EXACT_LU: return exact_apply(op, st, 1, x, y, NULL, NULL, 0) ? -1 : 0;
     case BBJD_PC_BLOCK_JACOBI: return bbjd_o_bj_apply(st, n, op->bs, op->bj, x, y, 0) ? -1 : 0;
     default: return HIPOK(hipMemcpyAsync(y, x, (size_t)n * 16, hipMemcpyDeviceToDevice, st)) ? 0 : -1;
     }
@@ -693,7 +714,7 @@ int bbjd_op_precond2(bbjd_int n, const bbjd_zomplex *x0, bbjd_zomplex *y0, const
 {
     struct bbjd_op *op = (struct bbjd_op *)data;
     if (!op || n != op->n) return -1;
-    if (op->pc == BBJD_PC_EXACT_LU) return exact_apply(op, op_stream(op), 2, x0, y0, x1, y1) ? -1 : 0;
+    if (op->pc == BBJD_PC_EXACT_LU) return exact_apply(op, op_stream(op), 2, x0, y0, x1, y1, 0) ? -1 : 0;
     if (bbjd_op_precond(n, x0, y0, data)) return -1;
     return bbjd_op_precond(n, x1, y1, data);
 }
@@ -736,8 +757,14 @@ int bbjd_op_precond_c64(bbjd_int n, const bbjd_cfloat *x, bbjd_cfloat *y, void *
     switch (op->pc) {
     case BBJD_PC_BLOCK_JACOBI: return bbjd_o_bj_apply(st, n, op->bs, op->bj_c, x, y, 1) ? -1 : 0;
     case BBJD_PC_EXACT_LU: {
+        /* the FP64 solve reading and writing complex64 directly: the same
+         * values as widening, solving and rounding in three passes */
+        if (g_c64_io) {
+            const int e = exact_apply(op, st, 1, x, y, NULL, NULL, 1);
+            if (e != -3) return e ? -1 : 0;
+        }
         int rc = bbjd_k_c2z(st, n, x, op->zt1);
-        rc |= exact_apply(op, st, 1, op->zt1, op->zt2, NULL, NULL);
+        rc |= exact_apply(op, st, 1, op->zt1, op->zt2, NULL, NULL, 0);
         rc |= bbjd_k_z2c(st, n, op->zt2, y);
         return rc ? -1 : 0;
     }

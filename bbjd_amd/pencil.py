@@ -159,6 +159,19 @@ class DeviceOperator:
     def solve(self, x):
         return self._apply("bbjd_op_precond", x)
 
+    def solve_c64(self, x):
+        """the complex64 preconditioner apply of the mixed inner solve
+        (bbjd_op_precond_c64; the operator needs c64=True)"""
+        lib = _lib.load()
+        x = np.ascontiguousarray(x, dtype=np.complex64)
+        dx = _lib.DeviceBuffer.from_array(x)
+        dy = _lib.DeviceBuffer(x.nbytes)
+        rc = lib.bbjd_op_precond_c64(self.n, dx.ptr, dy.ptr, self.handle)
+        rc |= lib.bbjd_dev_sync()
+        if rc:
+            raise RuntimeError("bbjd_op_precond_c64 failed")
+        return dy.to_array(np.complex64, self.n)
+
     def matvec_AB(self, x, single=False):
         """(A x, B x) in one pass over x (bbjd_op_matvec_AB; single: the
         complex64 form, bbjd_op_matvec_AB_c64)"""

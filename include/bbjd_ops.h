@@ -111,6 +111,12 @@ void bbjd_set_bcr_border_fuse(int on);
  * phase's factor loads at launch, each phase on a thread range of its own;
  * 0: one phase after another.  Bit-identical results.  Tuning and tests. */
 void bbjd_set_bcr_prefetch(int on);
+/* 1 (default): bbjd_op_precond_c64 with the exact preconditioner reads x and
+ * writes y in complex64 inside the solve's first reduction launch and its
+ * border launches, where the shape allows (dense rows present, s = 4 or 8,
+ * at least two levels before the tail); 0: widening and rounding in launches
+ * of their own.  Bit-identical results.  Tuning and tests. */
+void bbjd_set_c64_direct_io(int on);
 
 #ifdef __cplusplus
 }

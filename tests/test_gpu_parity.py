@@ -292,6 +292,33 @@ def test_bcr_level_groups_match_level_by_level(n, bs, hw, ntau):
     assert np.linalg.norm(z1 - r2) <= 1e-9 * np.linalg.norm(r2)
 
 
+@pytest.mark.parametrize("n,bs,hw,ntau", [(30004, 4, 1, 2), (60000, 8, 1, 4), (4096, 4, 1, 2), (40000, 8, 2, 16),
+                                          (4000, 4, 1, 0)])
+def test_c64_preconditioner_direct_io_matches_conversions(n, bs, hw, ntau):
+    """bbjd_op_precond_c64 with the exact preconditioner reading and writing
+    complex64 inside the solve's own launches (bbjd_set_c64_direct_io; s = 4
+    and 8 with dense rows) equals the widen / FP64 solve / round sequence
+    exactly, and that equals the FP64 solve of the widened vector rounded to
+    complex64; s = 16 and no dense rows take the conversions"""
+    lib = _lib.load()
+    p = BandedTauPencil.generate(n, bs, hw, ntau, seed=31)
+    sigma = 0.4 + 0.3j
+    dev = DeviceOperator(p, sigma, pc="exact", c64=True)
+    rng = np.random.default_rng(n)
+    x = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+    try:
+        lib.bbjd_set_c64_direct_io(0)
+        a = dev.solve_c64(x)
+        lib.bbjd_set_c64_direct_io(1)
+        b = dev.solve_c64(x)
+        y = dev.solve(x.astype(np.complex128))
+    finally:
+        lib.bbjd_set_c64_direct_io(1)
+        dev.free()
+    np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(b, y.astype(np.complex64))
+
+
 def _cfg1(n=2000, k=8, seed=1):
     return BandedTauPencil.generate(n, 4, 1, 2, seed=seed), 0.5 + 0.5j
 
