@@ -116,6 +116,11 @@ int bbjd_k_scale_inv(hipStream_t s, int64_t n, void *x, const double *scal, int 
 /* complex64 form of bbjd_k_scale_inv */
 int bbjd_k_scale_inv_c(hipStream_t s, int64_t n, void *x, const double *scal, int slot, int mode);
 
+/* bbjd_k_scale_inv(_c) that also writes the result to y (x unscaled when
+ * mode 1 skips the scaling) */
+int bbjd_k_scale_inv2(hipStream_t s, int64_t n, void *x, void *y, const double *scal, int slot, int mode);
+int bbjd_k_scale_inv2_c(hipStream_t s, int64_t n, void *x, void *y, const double *scal, int slot, int mode);
+
 /* x *= a (host scalar a = (are, aim)) */
 int bbjd_k_scale(hipStream_t s, int64_t n, void *x, double are, double aim);
 

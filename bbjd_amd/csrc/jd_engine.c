@@ -659,7 +659,8 @@ static int gmres(ctx_t *c, bbjd_int ndefl, const void *QR, const void *QL, zcx t
             rc |= bbjd_k_project_out(c->s, n, (int)ndefl, QL, c->ld, t1, &d->rd);
             if (!rc && apply_pc(c, t1, GCOL(0))) cb = 1;
             rc |= bbjd_k_norm(c->s, n, GCOL(0), &d->rd, SL_G);
-            rc |= bbjd_k_scale_inv(c->s, n, GCOL(0), d->rd.scal, SL_G, 0);
+            /* also into t1 (free again), where step 0 projects it */
+            rc |= bbjd_k_scale_inv2(c->s, n, GCOL(0), t1, d->rd.scal, SL_G, 0);
         } else {
             void *tc0 = d->tc[0];
             rc |= bbjd_k_z2c(c->s, n, r, tc0);
@@ -671,7 +672,8 @@ static int gmres(ctx_t *c, bbjd_int ndefl, const void *QR, const void *QL, zcx t
             } else
                 rc |= !HIPOK(hipMemcpyAsync(GCOL(0), tc0, (size_t)n * 8, hipMemcpyDeviceToDevice, c->s));
             rc |= bbjd_k_norm_c(c->s, n, GCOL(0), &d->rd, SL_G);
-            rc |= bbjd_k_scale_inv_c(c->s, n, GCOL(0), d->rd.scal, SL_G, 0);
+            /* also into tc0 (free again), where step 0 projects it */
+            rc |= bbjd_k_scale_inv2_c(c->s, n, GCOL(0), tc0, d->rd.scal, SL_G, 0);
         }
         rc |= !HIPOK(hipMemcpyAsync(d->hscal, d->rd.scal, SL_COUNT * 8, hipMemcpyDeviceToHost, c->s));
         rc = seg_end(c, 0, rc | cb);
@@ -687,11 +689,12 @@ static int gmres(ctx_t *c, bbjd_int ndefl, const void *QR, const void *QL, zcx t
 
     bbjd_int j;
     for (j = 0; j < ims; j++) {
-        void *vj = GCOL(j), *vj1 = GCOL(j + 1);
+        void *vj1 = GCOL(j + 1);
         if ((seg = seg_begin(c, (int)j + 1)) < 0) return -2;
         if (seg == 0) {
             if (!mixed) {
-                rc |= !HIPOK(hipMemcpyAsync(t1, vj, (size_t)n * 16, hipMemcpyDeviceToDevice, c->s));
+                /* t1 = v_j: written with v_j's normalisation (the previous
+                 * step's last launch, or the prologue's) */
                 rc |= bbjd_k_project_out(c->s, n, (int)ndefl, QR, c->ld, t1, &d->rd);
                 if (!rc && apply_ab(c, t1, t2, vj1)) cb = 1;
                 rc |= bbjd_k_lincomb_dev(c->s, n, d->gco, t2, vj1, t2);
@@ -700,10 +703,10 @@ static int gmres(ctx_t *c, bbjd_int ndefl, const void *QR, const void *QL, zcx t
                 rc |= bbjd_k_cgs2(c->s, n, (int)(j + 1), V, c->ld, vj1, &d->rd, SL_G, d->rd.hsum);
                 rc |= !HIPOK(hipMemcpyAsync(d->hsmall, d->rd.hsum, (size_t)(j + 1) * 16, hipMemcpyDeviceToHost,
                                             c->s));
-                rc |= bbjd_k_scale_inv(c->s, n, vj1, d->rd.scal, SL_G, 0);
+                rc |= bbjd_k_scale_inv2(c->s, n, vj1, t1, d->rd.scal, SL_G, 0);   /* t1: the next step's */
             } else {
                 void *a = d->tc[0], *bq = d->tc[1], *e = d->tc[2];
-                rc |= !HIPOK(hipMemcpyAsync(a, vj, (size_t)n * 8, hipMemcpyDeviceToDevice, c->s));
+                /* a = v_j: written with v_j's normalisation */
                 rc |= bbjd_k_project_out_c(c->s, n, (int)ndefl, qr, c->ld, a, &d->rd);
                 if (d->opt.matvec_AB_c64 && d->opt.callback_space == BBJD_CALLBACK_DEVICE) {
                     hipStream_t prev = tl_cb_stream;
@@ -726,7 +729,7 @@ static int gmres(ctx_t *c, bbjd_int ndefl, const void *QR, const void *QL, zcx t
                 rc |= bbjd_k_cgs2_c(c->s, n, (int)(j + 1), V, c->ld, vj1, &d->rd, SL_G, d->rd.hsum);
                 rc |= !HIPOK(hipMemcpyAsync(d->hsmall, d->rd.hsum, (size_t)(j + 1) * 8, hipMemcpyDeviceToHost,
                                             c->s));
-                rc |= bbjd_k_scale_inv_c(c->s, n, vj1, d->rd.scal, SL_G, 0);
+                rc |= bbjd_k_scale_inv2_c(c->s, n, vj1, a, d->rd.scal, SL_G, 0);   /* a: the next step's */
             }
             /* both norms down (the Hessenberg column came above) */
             rc |= !HIPOK(hipMemcpyAsync(d->hscal, d->rd.scal, SL_COUNT * 8, hipMemcpyDeviceToHost, c->s));

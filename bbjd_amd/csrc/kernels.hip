@@ -812,16 +812,21 @@ __global__ __launch_bounds__(TPB) void k_norm(i64 n, const Cx<R> *__restrict__ x
 
 template <typename R>
 __global__ __launch_bounds__(TPB) void k_scale_inv(i64 n, Cx<R> *x, const double *scal, int slot, int mode,
-                                                   const int *__restrict__ cond)
+                                                   const int *__restrict__ cond, Cx<R> *__restrict__ y)
 {
     if (cond && cond[0] == 0) return;
     const double nrm = scal[slot];
-    if (mode == 1 && !(nrm > 1e-300)) return;
+    if (mode == 1 && !(nrm > 1e-300)) {
+        if (y)
+            for (i64 i = (i64)blockIdx.x * TPB + threadIdx.x; i < n; i += (i64)gridDim.x * TPB) y[i] = x[i];
+        return;
+    }
     const R s = (R)(1.0 / nrm);
     for (i64 i = (i64)blockIdx.x * TPB + threadIdx.x; i < n; i += (i64)gridDim.x * TPB) {
         Cx<R> v = x[i];
         v.re *= s; v.im *= s;
         x[i] = v;
+        if (y) y[i] = v;
     }
 }
 
@@ -1365,7 +1370,7 @@ template <typename R>
 static int scale_inv(hipStream_t s, i64 n, void *x, const double *scal, int slot, const int *cond)
 {
     hipLaunchKernelGGL((k_scale_inv<R>), dim3(grid_stride_blocks(n)), dim3(TPB), 0, s, (i64)n, (Cx<R> *)x, scal,
-                       slot, 1, cond);
+                       slot, 1, cond, (Cx<R> *)nullptr);
     return chk();
 }
 
@@ -1694,15 +1699,25 @@ int bbjd_k_norm_c(hipStream_t s, int64_t n, const void *x, bbjd_red *rd, int slo
 
 int bbjd_k_scale_inv(hipStream_t s, int64_t n, void *x, const double *scal, int slot, int mode)
 {
-    hipLaunchKernelGGL((k_scale_inv<double>), dim3(grid_stride_blocks(n)), dim3(TPB), 0, s, (i64)n,
-                       (Cx<double> *)x, scal, slot, mode, (const int *)nullptr);
-    return chk();
+    return bbjd_k_scale_inv2(s, n, x, nullptr, scal, slot, mode);
 }
 
 int bbjd_k_scale_inv_c(hipStream_t s, int64_t n, void *x, const double *scal, int slot, int mode)
 {
+    return bbjd_k_scale_inv2_c(s, n, x, nullptr, scal, slot, mode);
+}
+
+int bbjd_k_scale_inv2(hipStream_t s, int64_t n, void *x, void *y, const double *scal, int slot, int mode)
+{
+    hipLaunchKernelGGL((k_scale_inv<double>), dim3(grid_stride_blocks(n)), dim3(TPB), 0, s, (i64)n,
+                       (Cx<double> *)x, scal, slot, mode, (const int *)nullptr, (Cx<double> *)y);
+    return chk();
+}
+
+int bbjd_k_scale_inv2_c(hipStream_t s, int64_t n, void *x, void *y, const double *scal, int slot, int mode)
+{
     hipLaunchKernelGGL((k_scale_inv<float>), dim3(grid_stride_blocks(n)), dim3(TPB), 0, s, (i64)n,
-                       (Cx<float> *)x, scal, slot, mode, (const int *)nullptr);
+                       (Cx<float> *)x, scal, slot, mode, (const int *)nullptr, (Cx<float> *)y);
     return chk();
 }
 
