@@ -107,6 +107,7 @@ def load():
     _sig(lib, "bbjd_set_tiles", C.c_int, C.c_int, C.c_int, C.c_int)
     _sig(lib, "bbjd_block_gram", C.c_int, I64, I64, C.c_void_p, I64, C.c_void_p, C.c_void_p)
     _sig(lib, "bbjd_set_fuse_blocks", None, C.c_int)
+    _sig(lib, "bbjd_set_svdv_fused", None, C.c_int)
     _sig(lib, "bbjd_set_cgs_fused", None, C.c_int)
     _sig(lib, "bbjd_set_gram_blocks", None, C.c_int)
     _sig(lib, "bbjd_set_rotate_mfma", None, C.c_int)

@@ -887,6 +887,49 @@ __global__ __launch_bounds__(TPB) void k_axpy_svdv(i64 n, const Cx<double> *d, c
     }
 }
 
+// k_axpy_svdv reducing the two dots itself from the projection's per-tile
+// partials (part[(tile 2 + v)], v = 0: u^H z, 1: u^H s; ntiles tiles), in
+// proj_finish's order (16 lanes, tile = lane mod 16, then a 16-lane xor
+// butterfly), so every block holds the finish's values bit for bit and the
+// finish needs no launch of its own; block 0 also stores them in d.
+__global__ __launch_bounds__(TPB) void k_axpy_svdv_part(i64 n, const Cx<double> *__restrict__ part, int ntiles,
+                                                        Cx<double> *d, const Cx<double> *s, Cx<double> *z)
+{
+    __shared__ Cx<double> dots[2];
+    const int t = threadIdx.x;
+    if (t < 32) {
+        const int vi = t >> 4, l = t & 15;
+        double sr = 0, si = 0;
+        for (int tt = l; tt < ntiles; tt += 16) {
+            const Cx<double> p = part[(i64)tt * 2 + vi];
+            sr += p.re;
+            si += p.im;
+        }
+#pragma unroll
+        for (int m = 8; m >= 1; m >>= 1) {
+            sr += __shfl_xor(sr, m, 64);
+            si += __shfl_xor(si, m, 64);
+        }
+        if (l == 0) {
+            dots[vi] = cx(sr, si);
+            if (blockIdx.x == 0) d[vi] = cx(sr, si);
+        }
+    }
+    __syncthreads();
+    const Cx<double> uz = dots[0], us = dots[1];
+    const double a = hypot(us.re, us.im);
+    Cx<double> c = cx(0.0, 0.0);
+    if (a > 1e-300) {
+        const double den = us.re * us.re + us.im * us.im;
+        c = cx(-(uz.re * us.re + uz.im * us.im) / den, -(uz.im * us.re - uz.re * us.im) / den);
+    }
+    for (i64 i = (i64)blockIdx.x * TPB + t; i < n; i += (i64)gridDim.x * TPB) {
+        Cx<double> v = z[i];
+        cfma(v, s[i], c);
+        z[i] = v;
+    }
+}
+
 // In-place basis rotation X[:, :keep] = X[:, :k] Z (Zr row-major k x keep).
 // A block stages RT rows x k columns of X in LDS, so every input of its rows
 // is read before any output is written; outputs go back through LDS for
@@ -1210,6 +1253,9 @@ extern "C" int64_t bbjd_red_part_entries(int64_t n, int maxk)
 // sits dirty in the L2: on the 16-stream serving workload a threshold of 16
 // served 160 pairs/s, 48 served 153 and fusing every grid 108.  Tunable.
 static int g_fuse_max_blocks = 16;
+// SvdV: the axpy launch reduces the two dots' partials itself (1) or a
+// finish launch does (0)
+static int g_svdv_fused = 1;
 
 static inline NormFin fin_none()
 {
@@ -1238,7 +1284,8 @@ static inline NormFin fin_norm(bbjd_red *rd, int mode, int slot)
 template <typename R, int NV>
 static int proj_pass_nv(hipStream_t s, i64 n, int k, const void *Q, i64 ldq, const void *x0, const void *x1,
                         const void *x2, void *out0, void *out1, void *out2, bbjd_red *rd, int norm_slot,
-                        const int *cond, Cx<R> ca = Cx<R>{0, 0}, Cx<R> cb = Cx<R>{0, 0})
+                        const int *cond, Cx<R> ca = Cx<R>{0, 0}, Cx<R> cb = Cx<R>{0, 0},
+                        int *finish_left = nullptr)
 {
     constexpr int C = NV == 1 ? 16 : 8;
     const int ctiles = k > 0 ? (k + C - 1) / C : 1;
@@ -1281,7 +1328,10 @@ static int proj_pass_nv(hipStream_t s, i64 n, int k, const void *Q, i64 ldq, con
                            (i64)ldq, (const Cx<R> *)x0, (const Cx<R> *)x1, (const Cx<R> *)x2, ca, cb, (Cx<R> *)rd->part, ldp,
                            (R *)rd->pnorm, want_norm, cond, rd->cnt, (Cx<R> *)out0, (Cx<R> *)out1, (Cx<R> *)out2,
                            rd->scal, norm_slot, fused);
-    if (!fused)
+    // finish_left: a caller that reduces the partials itself gets the tile
+    // count instead of a finish launch (0 when the last block finished)
+    if (finish_left) *finish_left = fused ? 0 : nt;
+    if (!fused && !finish_left)
         hipLaunchKernelGGL((k_proj_finish<R, NV, C>), dim3(g.x), dim3(TPB), 0, s, nt, k, (const Cx<R> *)rd->part,
                            ldp, (const R *)rd->pnorm, want_norm, cond, (Cx<R> *)out0, (Cx<R> *)out1,
                            (Cx<R> *)out2, rd->scal, norm_slot);
@@ -1756,11 +1806,19 @@ int bbjd_k_lincomb_norm(hipStream_t s, int64_t n, double are, double aim, const 
 int bbjd_k_svdv(hipStream_t s, int64_t n, const void *u, void *z, const void *sv, bbjd_red *rd)
 {
     Cx<double> *d = (Cx<double> *)rd->coef2;
-    int rc = proj_pass<double>(s, n, 1, u, n, z, sv, d, d + 1, rd, -1, nullptr);
-    hipLaunchKernelGGL(k_axpy_svdv, dim3(grid_stride_blocks(n)), dim3(TPB), 0, s, (i64)n, (const Cx<double> *)d,
-                       (const Cx<double> *)sv, (Cx<double> *)z);
+    int left = 0;
+    int rc = proj_pass_nv<double, 2>(s, n, 1, u, n, z, sv, nullptr, d, d + 1, nullptr, rd, -1, nullptr,
+                                     Cx<double>{0, 0}, Cx<double>{0, 0}, g_svdv_fused ? &left : nullptr);
+    if (left)
+        hipLaunchKernelGGL(k_axpy_svdv_part, dim3(grid_stride_blocks(n)), dim3(TPB), 0, s, (i64)n,
+                           (const Cx<double> *)rd->part, left, d, (const Cx<double> *)sv, (Cx<double> *)z);
+    else
+        hipLaunchKernelGGL(k_axpy_svdv, dim3(grid_stride_blocks(n)), dim3(TPB), 0, s, (i64)n,
+                           (const Cx<double> *)d, (const Cx<double> *)sv, (Cx<double> *)z);
     return rc | chk();
 }
+
+void bbjd_set_svdv_fused(int on) { g_svdv_fused = on != 0; }
 
 void bbjd_set_gram_blocks(int blocks) { if (blocks > 0) g_gram_blocks = blocks; }
 

@@ -221,6 +221,12 @@ int bbjd_kbench(int which, int64_t n, int k, int reps, double *avg_ms, double *b
  * last block of the producing kernel instead of a second launch. */
 void bbjd_set_fuse_blocks(int max_blocks);
 
+/* 1 (default): the Sleijpen-van der Vorst step's axpy launch also reduces
+ * its two dots' partials (in the finishing step's order, so bit-identical),
+ * one launch fewer per outer iteration; 0: a finishing launch of its own.
+ * Tuning and tests. */
+void bbjd_set_svdv_fused(int on);
+
 /* Largest grid of the fused CGS2's middle pass (default 512 blocks). */
 void bbjd_set_cgs_mid_blocks(int max_blocks);
 

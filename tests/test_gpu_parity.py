@@ -700,3 +700,27 @@ def test_serving_loop_isolates_failing_requests():
     for a, b in zip(res[:2] + res[3:5], ref):
         assert a["status"] == 0 and a["nconv"] == b["nconv"]
         np.testing.assert_array_equal(a["eigenvalues"], b["eigenvalues"])
+
+
+def test_svdv_axpy_reduces_dots_bit_identically():
+    """bbjd_set_svdv_fused: the Sleijpen-van der Vorst step's axpy launch
+    reducing its two dots' partials itself gives the same solve, bit for bit
+    (eigenvalues and iteration count), as a finishing launch of its own, at a
+    size whose projection grid is too large for the in-kernel finish"""
+    lib = _lib.load()
+    p = BandedTauPencil.generate(60000, 8, 1, 4, seed=5)
+    sigma = 0.2 + 0.7j
+    dev = DeviceOperator(p, sigma, pc="exact")
+    out = []
+    try:
+        for f in (0, 1):
+            lib.bbjd_set_svdv_fused(f)
+            ev, X, info = jdqz(dev, k=4, sigma=sigma, tol=1e-10, maxsub=20, minsub=10, maxiter=300,
+                               preconditioner=dev, return_info=True)
+            out.append((ev, info["iters"], info["nconv"]))
+    finally:
+        lib.bbjd_set_svdv_fused(1)
+        dev.free()
+    assert out[0][2] == out[1][2] == 4
+    assert out[0][1] == out[1][1]
+    np.testing.assert_array_equal(out[0][0], out[1][0])
