@@ -46,6 +46,52 @@ _STATUS_MSG = {
 }
 
 
+def _identity(x):
+    """B = I (B omitted): a copy, since the engine owns its output buffer"""
+    return np.array(x, dtype=np.complex128)
+
+
+def _from_sparse(M):
+    Mc = sp.csc_matrix(M, dtype=np.complex128)
+    return Mc.dot
+
+
+def _from_method(name):
+    def build(obj):
+        method = getattr(obj, name)
+        return lambda x: np.asarray(method(x), dtype=np.complex128)
+    return build
+
+
+# (test, builder) pairs, first match wins: what a matvec operand may be
+# (reference bbjd.py:183-205) and what a preconditioner may be (:208-219)
+_OPERAND_KINDS = (
+    (sp.issparse, _from_sparse),
+    (lambda o: hasattr(o, "matvec"), _from_method("matvec")),
+    (callable, lambda f: f),
+)
+_PRECOND_KINDS = (
+    (lambda o: hasattr(o, "solve"), _from_method("solve")),
+    (callable, lambda f: f),
+)
+
+
+def _adapt(obj, kinds, error):
+    for test, build in kinds:
+        if test(obj):
+            return build(obj)
+    raise ValueError(error)
+
+
+def _start_vector(v0, n):
+    if v0 is None:
+        return None
+    v0 = np.ascontiguousarray(v0, dtype=np.complex128)
+    if v0.shape != (n,):
+        raise ValueError(f"v0 must have shape ({n},); got {v0.shape}")
+    return v0
+
+
 def jdqz(A, B=None, k: int = 6, sigma: complex = 0.0, which: str = 'nearest', tol: float = 1e-10,
          maxiter: int = 200, maxsub: int = 0, minsub: int = 0, inner_maxiter: int = 0,
          inner_tol: float = 1e-2, preconditioner=None, v0=None, verbose: int = 0,
@@ -72,10 +118,7 @@ def jdqz(A, B=None, k: int = 6, sigma: complex = 0.0, which: str = 'nearest', to
             raise ValueError("with a DeviceOperator A, B comes from the same pencil (pass B=None)")
         if preconditioner is not None and preconditioner is not True and preconditioner is not A:
             raise ValueError("with a DeviceOperator A, preconditioner must be None, True or A")
-        if v0 is not None:
-            v0 = np.ascontiguousarray(v0, dtype=np.complex128)
-            if v0.shape != (n,):
-                raise ValueError(f"v0 must have shape ({n},); got {v0.shape}")
+        v0 = _start_vector(v0, n)
         out = _bbjd_c.jdqz_c(n, k, sigma, target, None, None,
                              None if preconditioner is None else True, tol=tol, maxiter=maxiter,
                              maxsub=maxsub, minsub=minsub, inner_maxiter=inner_maxiter,
@@ -83,61 +126,22 @@ def jdqz(A, B=None, k: int = 6, sigma: complex = 0.0, which: str = 'nearest', to
                              precision=precision, graphs=graphs)
         return _finish(out, k, verbose, return_info)
 
+    # host operands: each becomes a function x -> y (complex128) by the first
+    # matching kind of its table; dense arrays are taken as sparse matrices
     if isinstance(A, np.ndarray):
         A = sp.csr_matrix(A)
     n = A.shape[0]
-    if A.shape != (n, n):
+    if tuple(A.shape) != (n, n):
         raise ValueError(f"A must be square; got shape {A.shape}")
-    if B is None:
-        B_op = sp.eye(n, dtype=np.complex128, format='csr')
-    elif isinstance(B, np.ndarray):
-        B_op = sp.csr_matrix(B)
-    else:
-        B_op = B
-
-    if sp.issparse(A):
-        A_csc = A.astype(np.complex128).tocsc()
-
-        def matvec_A(x):
-            return A_csc @ x
-    elif hasattr(A, 'matvec'):
-        def matvec_A(x):
-            return np.asarray(A.matvec(x), dtype=np.complex128)
-    elif callable(A):
-        matvec_A = A
-    else:
-        raise ValueError("A must be a sparse matrix, LinearOperator, or callable")
-
-    if sp.issparse(B_op):
-        B_csc = B_op.astype(np.complex128).tocsc()
-
-        def matvec_B(x):
-            return B_csc @ x
-    elif hasattr(B_op, 'matvec'):
-        def matvec_B(x):
-            return np.asarray(B_op.matvec(x), dtype=np.complex128)
-    elif callable(B_op):
-        matvec_B = B_op
-    else:
-        raise ValueError("B must be a sparse matrix, LinearOperator, or callable")
-
-    if preconditioner is not None:
-        if hasattr(preconditioner, 'solve'):
-            pc = preconditioner
-
-            def precond_solve(x):
-                return np.asarray(pc.solve(x), dtype=np.complex128)
-        elif callable(preconditioner):
-            precond_solve = preconditioner
-        else:
-            raise ValueError("preconditioner must have a .solve(b) method or be callable")
-    else:
-        precond_solve = None
-
-    if v0 is not None:
-        v0 = np.ascontiguousarray(v0, dtype=np.complex128)
-        if v0.shape != (n,):
-            raise ValueError(f"v0 must have shape ({n},); got {v0.shape}")
+    if isinstance(B, np.ndarray):
+        B = sp.csr_matrix(B)
+    matvec_A = _adapt(A, _OPERAND_KINDS, "A must be a sparse matrix, LinearOperator, or callable")
+    matvec_B = (_identity if B is None else
+                _adapt(B, _OPERAND_KINDS, "B must be a sparse matrix, LinearOperator, or callable"))
+    precond_solve = (None if preconditioner is None else
+                     _adapt(preconditioner, _PRECOND_KINDS,
+                            "preconditioner must have a .solve(b) method or be callable"))
+    v0 = _start_vector(v0, n)
 
     out = _bbjd_c.jdqz_c(n=n, nev=k, sigma=sigma, target=target, matvec_A=matvec_A,
                          matvec_B=matvec_B, precond_solve=precond_solve, tol=tol,

@@ -41,6 +41,7 @@ std::mutex g_kt_mu;
 std::vector<KRec> g_kt;
 std::vector<hipEvent_t> g_kt_pool;
 bool g_kt_on = false;
+constexpr size_t kKtMaxRecords = 1 << 16;   // launches timed between harvests
 hipEvent_t kt_event()
 {
     if (!g_kt_pool.empty()) { hipEvent_t e = g_kt_pool.back(); g_kt_pool.pop_back(); return e; }
@@ -1303,14 +1304,17 @@ static int proj_pass_nv(hipStream_t s, i64 n, int k, const void *Q, i64 ldq, con
         hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
         if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) timed = false;
     }
+    hipEvent_t ea = nullptr, eb = nullptr;
+    if (timed) {
+        // a bounded record: each recorded event keeps its dispatch (and what
+        // that dispatch references) alive until harvested
+        std::lock_guard<std::mutex> lk(g_kt_mu);
+        if (g_kt.size() >= kKtMaxRecords) timed = false;
+        else { ea = kt_event(); eb = kt_event(); }
+    }
     if (timed) {
         // events carried by the dispatch itself: their interval is the
         // kernel's execution, as a kernel trace reports it
-        hipEvent_t ea, eb;
-        {
-            std::lock_guard<std::mutex> lk(g_kt_mu);
-            ea = kt_event(); eb = kt_event();
-        }
         hipExtLaunchKernelGGL((k_proj<R, NV, C, false>), g, dim3(TPB), 0, s, ea, eb, 0, (i64)n, chunk, k,
                               (const Cx<R> *)Q, (i64)ldq, (const Cx<R> *)x0, (const Cx<R> *)x1, (const Cx<R> *)x2,
                               ca, cb, (Cx<R> *)rd->part, ldp, (R *)rd->pnorm, want_norm, cond, rd->cnt, (Cx<R> *)out0,
@@ -2015,3 +2019,4 @@ int bbjd_kbench(int which, int64_t n, int k, int reps, double *avg_ms, double *b
 }
 
 }  // extern "C"
+

@@ -12,6 +12,10 @@
  * admitted while the others are still iterating.  Requests never share
  * device memory, so each result is the same alone and in the mix: the
  * engine's reductions depend only on the request's shape.
+ *
+ * A server (bbjd_server_create) keeps its workers, streams and workspaces
+ * resident across runs (bbjd_server_run), so a stream of request batches
+ * pays the workspace set-up once; bbjd_serve is create + run + destroy.
  */
 #include "../../include/bbjd_serve.h"
 #include "engine.h"
@@ -29,31 +33,31 @@ int openblas_set_num_threads_local(int n);
 
 typedef double complex zcx;
 
-struct server;
 typedef struct {
-    struct server  *S;
+    bbjd_server    *S;
     int             id;
     pthread_t       th;
+    int             started;
     bbjd_workspace *ws;
     int             req;        /* assigned request or -1 */
     size_t          held;       /* bytes charged for its on-demand operator */
     int             ok;
 } worker_t;
 
-typedef struct server {
+struct bbjd_server {
     pthread_mutex_t   mu;
     pthread_cond_t    cv_work, cv_free;
-    bbjd_request     *reqs;
-    int               nreq, stop, active, peak_active;
-    size_t            budget, used, peak;
+    bbjd_request     *reqs;             /* the current run's requests        */
+    int               stop, active, peak_active;
+    size_t            budget, used, peak, wsb;
     double            t0;
     worker_t         *w;
     int               nw;
     bbjd_params       cap;
     bbjd_gpu_options  opt;
     int               init_failed;
-    int               pin;              /* workers pinned to one CPU each */
-} server_t;
+    int               pin;              /* workers pinned to one CPU each    */
+};
 
 static double now_s(void)
 {
@@ -68,6 +72,12 @@ int bbjd_device_init(int sync_yield)
     else if (sync_yield) hipSetDeviceFlags(hipDeviceScheduleYield);
     hipFree(0);
     return 0;
+}
+
+size_t bbjd_device_free_bytes(void)
+{
+    size_t freeb = 0, totb = 0;
+    return hipMemGetInfo(&freeb, &totb) == hipSuccess ? freeb : 0;
 }
 
 void bbjd_serve_config_init(bbjd_serve_config *c)
@@ -91,7 +101,7 @@ static bbjd_int req_maxsub(const bbjd_request *q)
     return q->maxsub > 0 ? q->maxsub : (ms < n ? ms : n);
 }
 
-static void run_request(server_t *S, worker_t *w, bbjd_request *q)
+static void run_request(bbjd_server *S, worker_t *w, bbjd_request *q)
 {
     q->t_start = now_s() - S->t0;
     q->worker = w->id;
@@ -143,7 +153,7 @@ static void run_request(server_t *S, worker_t *w, bbjd_request *q)
 static void *worker_main(void *arg)
 {
     worker_t *w = (worker_t *)arg;
-    server_t *S = w->S;
+    bbjd_server *S = w->S;
     /* one LAPACK thread and one OpenMP thread per worker: the workers are
      * the parallelism */
     openblas_set_num_threads_local(1);
@@ -172,8 +182,9 @@ static void *worker_main(void *arg)
         while (w->req < 0 && !S->stop) pthread_cond_wait(&S->cv_work, &S->mu);
         if (w->req < 0 && S->stop) break;
         const int r = w->req;
+        bbjd_request *q = &S->reqs[r];
         pthread_mutex_unlock(&S->mu);
-        run_request(S, w, &S->reqs[r]);
+        run_request(S, w, q);
         pthread_mutex_lock(&S->mu);
         S->used -= w->held;
         w->held = 0;
@@ -183,6 +194,7 @@ static void *worker_main(void *arg)
     }
     pthread_mutex_unlock(&S->mu);
     if (w->ws) bbjd_workspace_free(w->ws);
+    w->ws = NULL;
     return NULL;
 }
 
@@ -193,109 +205,158 @@ static int cmp_arrival(const void *a, const void *b, void *ctx)
     return x < y ? -1 : x > y ? 1 : (*(const int *)a - *(const int *)b);
 }
 
-int bbjd_serve(bbjd_request *reqs, int nreq, const bbjd_serve_config *cfg, bbjd_serve_report *rep)
+void bbjd_server_destroy(bbjd_server *S)
 {
-    if (!reqs || nreq <= 0 || !cfg || !rep) return -1;
-    memset(rep, 0, sizeof(*rep));
+    if (!S) return;
+    if (S->w) {
+        pthread_mutex_lock(&S->mu);
+        S->stop = 1;
+        pthread_cond_broadcast(&S->cv_work);
+        pthread_mutex_unlock(&S->mu);
+        for (int i = 0; i < S->nw; i++)
+            if (S->w[i].started) pthread_join(S->w[i].th, NULL);
+        free(S->w);
+    }
+    pthread_mutex_destroy(&S->mu);
+    pthread_cond_destroy(&S->cv_work);
+    pthread_cond_destroy(&S->cv_free);
+    free(S);
+}
+
+bbjd_server *bbjd_server_create(const bbjd_request *reqs, int nreq, const bbjd_serve_config *cfg)
+{
+    if (!reqs || nreq <= 0 || !cfg) return NULL;
     /* ignored once the device is up */
     if (cfg->sync_yield == 2) hipSetDeviceFlags(hipDeviceScheduleBlockingSync);
     else if (cfg->sync_yield) hipSetDeviceFlags(hipDeviceScheduleYield);
-    server_t S;
-    memset(&S, 0, sizeof(S));
-    pthread_mutex_init(&S.mu, NULL);
-    pthread_cond_init(&S.cv_work, NULL);
-    pthread_cond_init(&S.cv_free, NULL);
-    S.reqs = reqs;
-    S.nreq = nreq;
-    S.pin = cfg->pin;
-    /* capacity of the resident workspaces: the largest request */
-    bbjd_params_init(&S.cap);
-    S.cap.n = 1; S.cap.nev = 1; S.cap.maxsub = 2; S.cap.inner_maxiter = 0;
+    bbjd_server *S = calloc(1, sizeof(*S));
+    if (!S) return NULL;
+    pthread_mutex_init(&S->mu, NULL);
+    pthread_cond_init(&S->cv_work, NULL);
+    pthread_cond_init(&S->cv_free, NULL);
+    S->pin = cfg->pin;
+    /* capacity of the resident workspaces: the largest of the requests */
+    bbjd_params_init(&S->cap);
+    S->cap.n = 1; S->cap.nev = 1; S->cap.maxsub = 2; S->cap.inner_maxiter = 0;
     for (int i = 0; i < nreq; i++) {
-        bbjd_request *q = &reqs[i];
+        const bbjd_request *q = &reqs[i];
         const bbjd_int n = req_n(q);
-        if (n <= 0 || q->nev <= 0) return -1;
-        if (n > S.cap.n) S.cap.n = n;
-        if (q->nev > S.cap.nev) S.cap.nev = q->nev;
+        if (n <= 0 || q->nev <= 0) { bbjd_server_destroy(S); return NULL; }
+        if (n > S->cap.n) S->cap.n = n;
+        if (q->nev > S->cap.nev) S->cap.nev = q->nev;
         const bbjd_int ms = req_maxsub(q);
-        if (ms > S.cap.maxsub) S.cap.maxsub = ms;
-        if (q->inner_maxiter > S.cap.inner_maxiter) S.cap.inner_maxiter = q->inner_maxiter;
-        q->status = 0; q->nconv = 0; q->iters = 0; q->worker = -1;
+        if (ms > S->cap.maxsub) S->cap.maxsub = ms;
+        if (q->inner_maxiter > S->cap.inner_maxiter) S->cap.inner_maxiter = q->inner_maxiter;
     }
-    bbjd_gpu_options_init(&S.opt);
-    S.opt.callback_space = BBJD_CALLBACK_DEVICE;
-    S.opt.precision = cfg->precision;
-    S.opt.precond2 = bbjd_op_precond2;
-    S.opt.matvec_AB = bbjd_op_matvec_AB;
-    S.opt.matvec_ABr = bbjd_op_matvec_ABr;
-    S.opt.graphs = cfg->graphs;
+    bbjd_gpu_options_init(&S->opt);
+    S->opt.callback_space = BBJD_CALLBACK_DEVICE;
+    S->opt.precision = cfg->precision;
+    S->opt.precond2 = bbjd_op_precond2;
+    S->opt.matvec_AB = bbjd_op_matvec_AB;
+    S->opt.matvec_ABr = bbjd_op_matvec_ABr;
+    S->opt.graphs = cfg->graphs;
     if (cfg->precision == BBJD_PREC_MIXED) {
-        S.opt.matvec_A_c64 = bbjd_op_matvec_A_c64;
-        S.opt.matvec_B_c64 = bbjd_op_matvec_B_c64;
-        S.opt.precond_c64 = bbjd_op_precond_c64;
-        S.opt.matvec_AB_c64 = bbjd_op_matvec_AB_c64;
+        S->opt.matvec_A_c64 = bbjd_op_matvec_A_c64;
+        S->opt.matvec_B_c64 = bbjd_op_matvec_B_c64;
+        S->opt.precond_c64 = bbjd_op_precond_c64;
+        S->opt.matvec_AB_c64 = bbjd_op_matvec_AB_c64;
     }
-    const size_t wsb = bbjd_workspace_bytes(&S.cap, &S.opt);
+    S->wsb = bbjd_workspace_bytes(&S->cap, &S->opt);
     size_t freeb = 0, totb = 0;
-    if (hipMemGetInfo(&freeb, &totb) != hipSuccess) return -1;
-    S.budget = cfg->hbm_budget ? cfg->hbm_budget : (size_t)(0.9 * (double)freeb);
+    if (hipMemGetInfo(&freeb, &totb) != hipSuccess) { bbjd_server_destroy(S); return NULL; }
+    S->budget = cfg->hbm_budget ? cfg->hbm_budget : (size_t)(0.9 * (double)freeb);
     int nw = cfg->workers > 0 ? cfg->workers : 16;
     if (nw > nreq) nw = nreq;
-    while (nw > 1 && (size_t)nw * wsb > S.budget) nw--;
-    if ((size_t)nw * wsb > S.budget) return -1;
-    S.used = (size_t)nw * wsb;
-    S.peak = S.used;
-    rep->workers = nw;
-    rep->workspace_bytes = wsb;
-    S.nw = nw;
-    S.w = calloc((size_t)nw, sizeof(worker_t));
+    while (nw > 1 && (size_t)nw * S->wsb > S->budget) nw--;
+    if ((size_t)nw * S->wsb > S->budget) { bbjd_server_destroy(S); return NULL; }
+    S->w = calloc((size_t)nw, sizeof(worker_t));
+    if (!S->w) { bbjd_server_destroy(S); return NULL; }
+    S->nw = nw;
+    for (int i = 0; i < nw; i++) {
+        S->w[i].S = S; S->w[i].id = i; S->w[i].req = -1;
+        if (pthread_create(&S->w[i].th, NULL, worker_main, &S->w[i]) == 0) S->w[i].started = 1;
+        else {
+            pthread_mutex_lock(&S->mu);
+            S->init_failed++;
+            pthread_mutex_unlock(&S->mu);
+        }
+    }
+    /* wait for the workspaces */
+    pthread_mutex_lock(&S->mu);
+    for (;;) {
+        int up = 0;
+        for (int i = 0; i < nw; i++) up += S->w[i].ok;
+        if (up + S->init_failed >= nw) break;
+        pthread_cond_wait(&S->cv_free, &S->mu);
+    }
+    const int failed_init = S->init_failed;
+    pthread_mutex_unlock(&S->mu);
+    if (failed_init == nw) { bbjd_server_destroy(S); return NULL; }
+    return S;
+}
+
+int bbjd_server_workers(const bbjd_server *S)
+{
+    if (!S) return 0;
+    int up = 0;
+    for (int i = 0; i < S->nw; i++) up += S->w[i].ok;
+    return up;
+}
+
+size_t bbjd_server_workspace_bytes(const bbjd_server *S) { return S ? S->wsb : 0; }
+
+int bbjd_server_run(bbjd_server *S, bbjd_request *reqs, int nreq, const bbjd_serve_config *cfg,
+                    bbjd_serve_report *rep)
+{
+    if (!S || !reqs || nreq <= 0 || !cfg || !rep) return -1;
+    memset(rep, 0, sizeof(*rep));
+    /* every request must fit the resident workspaces */
+    for (int i = 0; i < nreq; i++) {
+        bbjd_request *q = &reqs[i];
+        q->status = 0; q->nconv = 0; q->iters = 0; q->worker = -1;
+        q->t_admit = q->t_start = q->t_done = 0;
+        const bbjd_int n = req_n(q);
+        if (n <= 0 || q->nev <= 0) return -1;
+    }
+    const int nw = S->nw;
     int *order = malloc((size_t)nreq * sizeof(int));
-    if (!S.w || !order) { free(S.w); free(order); return -1; }
+    int *pend = malloc((size_t)nreq * sizeof(int));
+    if (!order || !pend) { free(order); free(pend); return -1; }
     for (int i = 0; i < nreq; i++) order[i] = i;
     qsort_r(order, (size_t)nreq, sizeof(int), cmp_arrival, reqs);
 
-    S.t0 = now_s();
-    for (int i = 0; i < nw; i++) {
-        S.w[i].S = &S; S.w[i].id = i; S.w[i].req = -1;
-        pthread_create(&S.w[i].th, NULL, worker_main, &S.w[i]);
-    }
-    /* wait for the workspaces */
-    pthread_mutex_lock(&S.mu);
-    for (;;) {
-        int up = 0;
-        for (int i = 0; i < nw; i++) up += S.w[i].ok;
-        if (up + S.init_failed >= nw) break;
-        pthread_cond_wait(&S.cv_free, &S.mu);
-    }
-    const int failed_init = S.init_failed;
-    pthread_mutex_unlock(&S.mu);
-    int rc = 0;
-    if (failed_init == nw) rc = -1;
-    S.t0 = now_s();     /* the run starts with every workspace resident */
-    /* requests that have arrived and wait, in arrival order */
-    int *pend = malloc((size_t)nreq * sizeof(int));
-    if (!pend) rc = -1;
+    pthread_mutex_lock(&S->mu);
+    S->reqs = reqs;
+    S->used = (size_t)nw * S->wsb;
+    S->peak = S->used;
+    S->active = 0;
+    S->peak_active = 0;
+    S->t0 = now_s();     /* the run starts with every workspace resident */
+    pthread_mutex_unlock(&S->mu);
+    rep->workers = bbjd_server_workers(S);
+    rep->workspace_bytes = S->wsb;
+
     int npend = 0, next = 0;
-    for (int admitted = 0; admitted < nreq && rc == 0; admitted++) {
+    for (int admitted = 0; admitted < nreq; admitted++) {
         /* a free worker first: the choice among waiting requests is made
          * when one can start */
-        pthread_mutex_lock(&S.mu);
+        pthread_mutex_lock(&S->mu);
         for (;;) {
             int free_slot = 0;
-            for (int i = 0; i < nw && !free_slot; i++) free_slot = S.w[i].ok && S.w[i].req < 0;
+            for (int i = 0; i < nw && !free_slot; i++) free_slot = S->w[i].ok && S->w[i].req < 0;
             if (free_slot) break;
-            pthread_cond_wait(&S.cv_free, &S.mu);
+            pthread_cond_wait(&S->cv_free, &S->mu);
         }
-        pthread_mutex_unlock(&S.mu);
+        pthread_mutex_unlock(&S->mu);
         /* everything that has arrived by now; at least the next arrival */
         if (npend == 0 && cfg->realtime) {
-            const double wait = reqs[order[next]].arrival - (now_s() - S.t0);
+            const double wait = reqs[order[next]].arrival - (now_s() - S->t0);
             if (wait > 0) {
                 struct timespec ts = {(time_t)wait, (long)((wait - (double)(time_t)wait) * 1e9)};
                 nanosleep(&ts, NULL);
             }
         }
-        const double now = now_s() - S.t0;
+        const double now = now_s() - S->t0;
         while (next < nreq && (!cfg->realtime || reqs[order[next]].arrival <= now || npend == 0))
             pend[npend++] = order[next++];
         int pi = 0;
@@ -311,44 +372,49 @@ int bbjd_serve(bbjd_request *reqs, int nreq, const bbjd_serve_config *cfg, bbjd_
         memmove(pend + pi, pend + pi + 1, (size_t)(npend - pi - 1) * sizeof(int));
         npend--;
         bbjd_request *q = &reqs[r];
-        const size_t need = q->op ? 0 : bbjd_op_bytes_for(q->pencil, q->pc, cfg->precision == BBJD_PREC_MIXED);
-        pthread_mutex_lock(&S.mu);
+        const size_t need = q->op ? 0 : bbjd_op_bytes_for(q->pencil, q->pc, S->opt.precision == BBJD_PREC_MIXED);
+        pthread_mutex_lock(&S->mu);
         int slot = -1;
         for (;;) {
             for (int i = 0; i < nw; i++)
-                if (S.w[i].ok && S.w[i].req < 0) { slot = i; break; }
-            if (slot >= 0 && (S.used + need <= S.budget || S.active == 0)) break;
+                if (S->w[i].ok && S->w[i].req < 0) { slot = i; break; }
+            if (slot >= 0 && (S->used + need <= S->budget || S->active == 0)) break;
             slot = -1;
-            pthread_cond_wait(&S.cv_free, &S.mu);
+            pthread_cond_wait(&S->cv_free, &S->mu);
         }
-        S.used += need;
-        if (S.used > S.peak) S.peak = S.used;
-        S.w[slot].held = need;
-        S.w[slot].req = r;
-        S.active++;
-        if (S.active > S.peak_active) S.peak_active = S.active;
-        q->t_admit = now_s() - S.t0;
-        pthread_cond_broadcast(&S.cv_work);
-        pthread_mutex_unlock(&S.mu);
+        S->used += need;
+        if (S->used > S->peak) S->peak = S->used;
+        S->w[slot].held = need;
+        S->w[slot].req = r;
+        S->active++;
+        if (S->active > S->peak_active) S->peak_active = S->active;
+        q->t_admit = now_s() - S->t0;
+        pthread_cond_broadcast(&S->cv_work);
+        pthread_mutex_unlock(&S->mu);
     }
     free(pend);
-    pthread_mutex_lock(&S.mu);
-    while (S.active > 0) pthread_cond_wait(&S.cv_free, &S.mu);
-    S.stop = 1;
-    pthread_cond_broadcast(&S.cv_work);
-    pthread_mutex_unlock(&S.mu);
-    for (int i = 0; i < nw; i++) pthread_join(S.w[i].th, NULL);
-    rep->wall_seconds = now_s() - S.t0;
-    rep->peak_bytes = S.peak;
-    rep->peak_concurrency = S.peak_active;
+    pthread_mutex_lock(&S->mu);
+    while (S->active > 0) pthread_cond_wait(&S->cv_free, &S->mu);
+    rep->wall_seconds = now_s() - S->t0;
+    rep->peak_bytes = S->peak;
+    rep->peak_concurrency = S->peak_active;
+    S->reqs = NULL;
+    pthread_mutex_unlock(&S->mu);
     for (int i = 0; i < nreq; i++) {
         rep->pairs += reqs[i].nconv;
         if (reqs[i].status < 0) rep->failed++;
     }
     free(order);
-    free(S.w);
-    pthread_mutex_destroy(&S.mu);
-    pthread_cond_destroy(&S.cv_work);
-    pthread_cond_destroy(&S.cv_free);
+    return 0;
+}
+
+int bbjd_serve(bbjd_request *reqs, int nreq, const bbjd_serve_config *cfg, bbjd_serve_report *rep)
+{
+    if (!reqs || nreq <= 0 || !cfg || !rep) return -1;
+    memset(rep, 0, sizeof(*rep));
+    bbjd_server *S = bbjd_server_create(reqs, nreq, cfg);
+    if (!S) return -1;
+    const int rc = bbjd_server_run(S, reqs, nreq, cfg, rep);
+    bbjd_server_destroy(S);
     return rc;
 }

@@ -2,7 +2,9 @@
 
 ``serve(jobs, ...)`` runs a list of independent solves through the C
 serving loop: W worker threads, each with its own stream and resident
-workspace, admitting requests in arrival order (or largest first).  A job
+workspace, admitting requests in arrival order (or largest first).
+``Server`` keeps that loop, its threads and workspaces resident across
+batches (``run(jobs)`` per batch, ``close()`` at the end).  A job
 is a dict with ``op`` (a DeviceOperator already resident on the GPU) or
 ``pencil`` + ``pc`` (the server builds the operator at admission),
 ``sigma``, ``nev`` and optionally maxsub, minsub, maxiter, inner_maxiter,
@@ -56,6 +58,18 @@ def _setup(lib):
     lib.bbjd_ktime_harvest.restype = C.c_int
     lib.bbjd_op_bytes_for.argtypes = [C.POINTER(_lib.PencilDesc), C.c_int, C.c_int]
     lib.bbjd_op_bytes_for.restype = C.c_size_t
+    lib.bbjd_server_create.restype = C.c_void_p
+    lib.bbjd_server_create.argtypes = [C.POINTER(Request), C.c_int, C.POINTER(ServeConfig)]
+    lib.bbjd_server_run.restype = C.c_int
+    lib.bbjd_server_run.argtypes = [C.c_void_p, C.POINTER(Request), C.c_int, C.POINTER(ServeConfig),
+                                    C.POINTER(ServeReport)]
+    lib.bbjd_server_destroy.argtypes = [C.c_void_p]
+    lib.bbjd_server_workers.restype = C.c_int
+    lib.bbjd_server_workers.argtypes = [C.c_void_p]
+    lib.bbjd_server_workspace_bytes.restype = C.c_size_t
+    lib.bbjd_server_workspace_bytes.argtypes = [C.c_void_p]
+    lib.bbjd_device_free_bytes.restype = C.c_size_t
+    lib.bbjd_device_free_bytes.argtypes = []
     lib._serve_ready = True
 
 
@@ -65,19 +79,9 @@ def device_init(sync_yield=True):
     return lib.bbjd_device_init(2 if sync_yield == "block" else int(bool(sync_yield)))
 
 
-def serve(jobs, workers=16, precision="fp64", realtime=True, hbm_budget=0, keep_vectors=False,
-          sync_yield=True, graphs=False, order="arrival", pin=False, window=8):
-    """Run jobs through the serving loop; returns (results, report).  graphs:
-    capture each request's inner GMRES steps into HIP graphs and replay them.
-    order: which arrived request a free worker takes, "arrival" (first come,
-    first served), "largest" (largest estimated work: nev, then n) or
-    "window" (the largest of the `window` longest-waiting ones)."""
-    if order not in ("arrival", "largest", "window"):
-        raise ValueError("order must be 'arrival', 'largest' or 'window'")
-    if int(window) < 1:
-        raise ValueError("window must be at least 1")
-    lib = _lib.load()
-    _setup(lib)
+def _requests(jobs, keep_vectors):
+    """bbjd_request array for the jobs, the host arrays it points into, and
+    the output dicts"""
     nreq = len(jobs)
     reqs = (Request * nreq)()
     keep = []
@@ -118,6 +122,15 @@ def serve(jobs, workers=16, precision="fp64", realtime=True, hbm_budget=0, keep_
         q.residuals = o["residuals"].ctypes.data
         q.pair_residuals = o["pair_residuals"].ctypes.data
         outs.append(o)
+    return reqs, keep, outs
+
+
+def _config(lib, workers=16, precision="fp64", realtime=True, hbm_budget=0, sync_yield=True, graphs=False,
+            order="arrival", pin=False, window=8):
+    if order not in ("arrival", "largest", "window"):
+        raise ValueError("order must be 'arrival', 'largest' or 'window'")
+    if int(window) < 1:
+        raise ValueError("window must be at least 1")
     cfg = ServeConfig()
     lib.bbjd_serve_config_init(C.byref(cfg))
     cfg.workers = int(workers)
@@ -129,12 +142,12 @@ def serve(jobs, workers=16, precision="fp64", realtime=True, hbm_budget=0, keep_
     cfg.order = {"arrival": 0, "largest": 1, "window": 2}[order]
     cfg.window = int(window)
     cfg.pin = int(bool(pin))
-    rep = ServeReport()
-    rc = lib.bbjd_serve(reqs, nreq, C.byref(cfg), C.byref(rep))
-    if rc:
-        raise RuntimeError("bbjd_serve failed to set up (device memory?)")
+    return cfg
+
+
+def _results(reqs, outs, keep_vectors):
     results = []
-    for i in range(nreq):
+    for i in range(len(outs)):
         q = reqs[i]
         nc = int(q.nconv)
         o = outs[i]
@@ -147,8 +160,88 @@ def serve(jobs, workers=16, precision="fp64", realtime=True, hbm_budget=0, keep_
         if keep_vectors:
             r["eigenvectors"] = o["eigenvectors"][:nc].T.copy()
         results.append(r)
+    return results
+
+
+def serve(jobs, workers=16, precision="fp64", realtime=True, hbm_budget=0, keep_vectors=False,
+          sync_yield=True, graphs=False, order="arrival", pin=False, window=8):
+    """Run jobs through the serving loop; returns (results, report).  graphs:
+    capture each request's inner GMRES steps into HIP graphs and replay them.
+    order: which arrived request a free worker takes, "arrival" (first come,
+    first served), "largest" (largest estimated work: nev, then n) or
+    "window" (the largest of the `window` longest-waiting ones)."""
+    lib = _lib.load()
+    _setup(lib)
+    cfg = _config(lib, workers, precision, realtime, hbm_budget, sync_yield, graphs, order, pin, window)
+    reqs, keep, outs = _requests(jobs, keep_vectors)
+    rep = ServeReport()
+    rc = lib.bbjd_serve(reqs, len(jobs), C.byref(cfg), C.byref(rep))
+    if rc:
+        raise RuntimeError("bbjd_serve failed to set up (device memory?)")
     report = {f: getattr(rep, f) for f, _ in ServeReport._fields_}
-    return results, report
+    return _results(reqs, outs, keep_vectors), report
+
+
+class Server:
+    """A resident serving loop (bbjd_server_create): worker threads, streams
+    and workspaces sized for the largest of ``capacity_jobs`` stay allocated
+    until close(), and each run() serves a batch of jobs on them."""
+
+    def __init__(self, capacity_jobs, workers=16, precision="fp64", hbm_budget=0, sync_yield=True,
+                 graphs=False, pin=False):
+        lib = _lib.load()
+        _setup(lib)
+        self._lib = lib
+        self.precision = precision
+        self._cfg_kw = dict(workers=workers, precision=precision, hbm_budget=hbm_budget, sync_yield=sync_yield,
+                            graphs=graphs, pin=pin)
+        cfg = _config(lib, **self._cfg_kw)
+        reqs, _keep, _outs = _requests(capacity_jobs, False)
+        self._h = lib.bbjd_server_create(reqs, len(capacity_jobs), C.byref(cfg))
+        if not self._h:
+            raise RuntimeError("bbjd_server_create failed (device memory?)")
+
+    @property
+    def workers(self):
+        return int(self._lib.bbjd_server_workers(self._h))
+
+    @property
+    def workspace_bytes(self):
+        return int(self._lib.bbjd_server_workspace_bytes(self._h))
+
+    def run(self, jobs, realtime=True, order="arrival", window=8, keep_vectors=False):
+        if not self._h:
+            raise RuntimeError("server is closed")
+        cfg = _config(self._lib, realtime=realtime, order=order, window=window, **self._cfg_kw)
+        reqs, keep, outs = _requests(jobs, keep_vectors)
+        rep = ServeReport()
+        if self._lib.bbjd_server_run(self._h, reqs, len(jobs), C.byref(cfg), C.byref(rep)):
+            raise RuntimeError("bbjd_server_run failed")
+        report = {f: getattr(rep, f) for f, _ in ServeReport._fields_}
+        return _results(reqs, outs, keep_vectors), report
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.bbjd_server_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def device_free_bytes():
+    lib = _lib.load()
+    _setup(lib)
+    return int(lib.bbjd_device_free_bytes())
 
 
 def ktime_enable(on=True):

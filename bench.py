@@ -8,10 +8,16 @@ Poisson arrivals at a fixed rate; each request draws n from {2e4, 5e4, 1e5,
 on the diagonal blocks), k from {8, 16, 32}, sigma uniform on [-4,4]+[-4,4]i,
 tol 1e-10, maxsub 4k, minsub 2k, exact banded LU of A - sigma B as the
 preconditioner.  The operators (pencil + factorisation) are built and resident
-in HBM before the timed region; the timed region is the serving loop.
+in HBM before the timed region, and so are the serving loop's worker threads,
+streams and workspaces (one resident server for every step); the timed
+region is the serving loop draining the stream.
 
   value        served eigenpairs/s over the timed steps (all pairs converged
-               at tol, each pair's FP64 residual checked)
+               at tol)
+  verification every pair returned by the first timed step is checked
+               afterwards with a fresh FP64 operator apply on the returned
+               vector: ||A x - lambda B x|| / (||A|| + |lambda| ||B||) <= tol,
+               ||A|| bounded below by its largest row norm (stricter)
   plain        the same solves one at a time (FP64 engine, no overlap): a
                stratified sample -- per (n, k) class of the workload, two of
                its requests (seeded) solved alone -- gives each class's mean
@@ -22,14 +28,16 @@ in HBM before the timed region; the timed region is the serving loop.
   roofline     the projection kernel Q^H x (the dominant streaming kernel):
                durations of every first-pass launch of the plain leg (one
                stream) from start/stop HIP events carried by each dispatch,
-               vs its algorithmic bytes 16 n (k + 1), HBM peak 8 TB/s; the
-               same kernel's rate inside the served region (launches
-               time-share the GPU there) is reported beside it; traffic =
-               average algorithmic bytes x the counter-measured ratio in
-               profiles/k_proj_traffic.json
+               vs its algorithmic bytes 16 n (k + 1), HBM peak 8 TB/s;
+               traffic = average algorithmic bytes x the counter-measured
+               ratio in profiles/k_proj_traffic.json
   cpu_baseline the reference solver compiled from its sources (oracle/_ref;
                else the oracle restatement) on the host cores, on a bounded
-               sample of the workload's requests, CPU exact banded LU
+               stratified sample of the workload's requests (one request from
+               each of the cheapest (n, k) classes), CPU exact banded LU, all
+               cores; also one request on one thread.  Its eigenvalues are
+               compared with the served results of the same requests
+               (max_rel_eig_err_vs_reference).
 
 --workload small: the batched small solves instead -- 2048 pencils with n
 log-uniform on [4096, 32768] (block size 4, block-tridiagonal, 2 tau rows),
@@ -39,6 +47,7 @@ once; plain = a random sample of them solved one at a time.
 
 Usage: python bench.py [--gpus 1] [--steps 1] [--warmup 1] [--nreq 256]
        python bench.py --plain-only ...   (the plain leg alone: for profilers)
+       python bench.py --served-only ...  (the served steps alone: for profilers)
        python bench.py --workload small [--nreq 2048]
 One JSON line on stdout; progress on stderr.
 """
@@ -47,6 +56,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import platform
 import sys
 import time
 
@@ -56,10 +66,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+T_START = time.perf_counter()
 
 
 def log(*a):
-    print(*a, file=sys.stderr, flush=True)
+    print(f"[bench {time.perf_counter() - T_START:6.1f}s]", *a, file=sys.stderr, flush=True)
 
 
 def make_requests(nreq, rate, seed):
@@ -93,6 +104,14 @@ WORKLOADS = {
 }
 
 
+def class_of(r, workload):
+    """(n, k) class of a request; small solves: n is continuous, eight
+    log-spaced n classes"""
+    if workload == "serving":
+        return (r["n"], r["k"])
+    return (int(np.log2(r["n"] / 4096) * 8 / 3), r["k"])
+
+
 def norm_lower_bound(p):
     """max row 2-norm of A: a lower bound of ||A||_2 (stricter residual test)"""
     nb, nd, bs = p.n // p.bs, 2 * p.hw + 1, p.bs
@@ -108,34 +127,121 @@ def pct(a, q):
     return float(np.percentile(np.asarray(a), q)) if len(a) else None
 
 
-def cpu_baseline(reqs, budget_s, threads, wl):
+def host_cpu():
+    model = platform.processor() or ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return model, avail
+
+
+def verify_pairs(results, ops, norms, tol):
+    """FP64 residual of every returned pair from a fresh operator apply on
+    the returned vector (bbjd_op_matvec_ABr: r = A x - lambda B x and ||r||
+    on the device): worst ||r|| / (||A|| + |lambda| ||B||) with ||x|| = 1,
+    ||B|| = 1, the count over tol, and the worst ||r|| / ||x|| alone."""
+    from bbjd_amd import _lib
+    lib = _lib.load()
+    worst, worst_abs, bad, checked = 0.0, 0.0, 0, 0
+    for i, r in enumerate(results):
+        X = r.get("eigenvectors")
+        if X is None or r["nconv"] == 0:
+            continue
+        op = ops[i]
+        n = op.n
+        dx = _lib.DeviceBuffer(n * 16)
+        da, db, dr = _lib.DeviceBuffer(n * 16), _lib.DeviceBuffer(n * 16), _lib.DeviceBuffer(n * 16)
+        dn = _lib.DeviceBuffer(8)
+        for j in range(r["nconv"]):
+            x = np.ascontiguousarray(X[:, j])
+            lam = complex(r["eigenvalues"][j])
+            xn = float(np.linalg.norm(x))
+            if lib.bbjd_memcpy_h2d(dx.ptr, x.ctypes.data, x.nbytes):
+                raise RuntimeError("upload failed")
+            rc = lib.bbjd_op_matvec_ABr(n, dx.ptr, da.ptr, db.ptr, _lib.zc(1.0), _lib.zc(-lam), dr.ptr, dn.ptr,
+                                        op.handle)
+            rc |= lib.bbjd_dev_sync()
+            if rc:
+                raise RuntimeError("verification apply failed")
+            rn = float(dn.to_array(np.float64, 1)[0]) / xn
+            v = rn / (norms[i] + abs(lam))
+            worst = max(worst, v)
+            worst_abs = max(worst_abs, rn)
+            bad += not (v <= tol)
+            checked += 1
+        for b in (dx, da, db, dr, dn):
+            b.free()
+    return worst, bad, checked, worst_abs
+
+
+def cpu_baseline(reqs, served, wl, workload, classes_wanted, threads, budget_s):
     """Reference solver on the host (oracle/_ref if built, else the oracle);
-    FP64 throughout (the reference has no complex64 inner solve)."""
+    FP64 throughout (the reference has no complex64 inner solve).  One
+    request from each wanted class (cheapest first) on `threads` threads,
+    until budget_s is spent; then one request of the cheapest class on one
+    thread.  Returns the baseline dict and per-request eigenvalue errors
+    against the served results of the same requests."""
     from bbjd_amd import BandedTauPencil
     from oracle import orc
     which = "ref" if orc.have_ref() else "oracle"
     lib = orc.load_oracle()
-    lib.orc_set_threads(int(threads))
-    pairs, secs, done = 0, 0.0, []
-    # smallest requests first, so the bounded sample finishes a few solves
-    for r in sorted(reqs, key=lambda r: (r["n"], r["k"])):
-        if secs >= budget_s:
-            break
+
+    def run(i, thr):
+        r = reqs[i]
+        lib.orc_set_threads(int(thr))
         p = BandedTauPencil.generate(r["n"], *wl["shape"], seed=r["pseed"])
         ops = orc.CpuPencilOps(p, r["sigma"], 1)
         t = time.perf_counter()
         out = orc.solve(which, p.n, r["k"], r["sigma"], tol=1e-10, maxsub=wl["maxsub"](r["k"]),
                         minsub=wl["minsub"](r["k"]), maxiter=r.get("maxiter", 1000), inner_maxiter=wl["inner"],
                         inner_tol=wl["inner_tol"], ops=ops)
-        dt = time.perf_counter() - t
+        return out, time.perf_counter() - t
+
+    # the first request (in workload order) of each wanted class
+    first = {}
+    for i, r in enumerate(reqs):
+        first.setdefault(class_of(r, workload), i)
+    pairs, secs, done, errs = 0, 0.0, [], []
+    for key in classes_wanted:
+        if secs >= budget_s or key not in first:
+            break
+        i = first[key]
+        out, dt = run(i, threads)
         pairs += out["nconv"]
         secs += dt
-        done.append(f"n={r['n']} k={r['k']}: {out['nconv']} pairs {dt:.1f}s")
-    return {"value": pairs / secs if secs else None, "unit": "eigenpairs/s", "cores": int(threads),
+        done.append((key, i, out["nconv"], dt))
+        if served is not None:
+            ge = np.asarray(served[i]["eigenvalues"])
+            ce = np.asarray(out["evals"])
+            if ge.size and ce.size:
+                # each served eigenvalue against its nearest reference one
+                e = max(float(np.min(np.abs(ce - g)) / abs(g)) for g in ge)
+                errs.append((i, e, len(ge), len(ce)))
+    one = None
+    if done:
+        i = done[0][1]
+        out1, dt1 = run(i, 1)
+        one = {"value": out1["nconv"] / dt1, "cores": 1, "sample": f"request {i} (n={reqs[i]['n']}, "
+               f"k={reqs[i]['k']}): {out1['nconv']} pairs in {dt1:.2f} s"}
+    model, avail = host_cpu()
+    base = {"value": pairs / secs if secs else None, "unit": "eigenpairs/s", "cores": int(threads),
             "kind": "reference" if which == "ref" else "port",
-            "sample": f"{len(done)} of the workload's requests, smallest first, solved to tol 1e-10 "
-                      f"with a CPU exact banded LU (LAPACK zgbtrf + Schur complement of the tau rows): "
-                      + "; ".join(done)}
+            "sample": (f"one request from each of the {len(done)} cheapest (n, k) classes of the workload "
+                       f"(first in workload order), solved to tol 1e-10 with a CPU exact banded LU (LAPACK "
+                       f"zgbtrf + Schur complement of the tau rows): "
+                       + "; ".join(f"n={k[0]} k={k[1]}: {c} pairs {d:.1f}s" for k, _, c, d in done)),
+            "classes": [list(k) for k, _, _, _ in done],
+            "one_thread": one, "host_cpu": model, "host_cpus_available": avail}
+    return base, errs, done
 
 
 def main():
@@ -149,10 +255,12 @@ def main():
     ap.add_argument("--workers", type=int, default=16)
     ap.add_argument("--maxiter", type=int, default=1000)
     ap.add_argument("--plain-per-class", type=int, default=2, help="plain solves per (n, k) class")
-    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--cpu-seconds", type=float, default=25.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-verify", action="store_true", help="skip the residual check of the returned pairs")
     ap.add_argument("--plain-only", action="store_true", help="skip the served steps (profiling the plain leg)")
+    ap.add_argument("--served-only", action="store_true", help="skip the plain leg (profiling the served steps)")
     ap.add_argument("--no-tile-table", action="store_true", help="engine default tiles instead of the autotuned table")
     ap.add_argument("--order", choices=["arrival", "largest", "window"], default="arrival",
                     help="which arrived request a free worker takes (serving loop)")
@@ -161,6 +269,10 @@ def main():
     ap.add_argument("--sync", choices=["yield", "block"], default="yield",
                     help="how host threads wait on the GPU")
     ap.add_argument("--pin", action="store_true", help="pin each serving worker to one CPU")
+    ap.add_argument("--time-budget", type=float, default=560.0,
+                    help="seconds for the whole run: timed steps that would end past it, less what the "
+                         "plain leg, the verification and the CPU baseline need, are not started (the "
+                         "line then reports the steps timed)")
     ap.add_argument("--seed", type=int, default=20261203)
     args = ap.parse_args()
 
@@ -176,7 +288,7 @@ def main():
     # the committed per-shape tile table of this device, when there is one
     from bbjd_amd import autotune
     tt = None if args.no_tile_table else autotune.apply()
-    log(f"[bench] tile table: {'autotuned ' + tt['tuned'] if tt else 'engine defaults'}")
+    log(f"tile table: {'autotuned ' + tt['tuned'] if tt else 'engine defaults'}")
     wl = WORKLOADS[args.workload]
     if args.nreq <= 0:
         args.nreq = 256 if args.workload == "serving" else 2048
@@ -196,7 +308,8 @@ def main():
         ops.append(op)
     t_setup = time.perf_counter() - t0
     op_bytes = sum(o.device_bytes for o in ops)
-    log(f"[bench] {len(ops)} operators resident ({op_bytes / 1e9:.1f} GB) in {t_setup:.1f}s")
+    log(f"{len(ops)} operators resident ({op_bytes / 1e9:.1f} GB) in {t_setup:.1f}s; "
+        f"free HBM {srv.device_free_bytes() / 1e9:.1f} GB")
 
     def jobs_for(idx, with_arrival=True):
         return [dict(op=ops[i], sigma=reqs[i]["sigma"], nev=reqs[i]["k"], maxsub=wl["maxsub"](reqs[i]["k"]),
@@ -204,101 +317,111 @@ def main():
                      inner_maxiter=wl["inner"], inner_tol=wl["inner_tol"],
                      arrival=reqs[i]["arrival"] if with_arrival else 0.0) for i in idx]
 
-    # ---- warmup (untimed): short streams ----
-    for w in range(args.warmup):
-        idx = list(range(min(16, len(reqs))))
-        srv.serve(jobs_for(idx, False), workers=1 if args.plain_only else args.workers, realtime=False,
-                  precision=wl["precision"])
-        log(f"[bench] warmup step {w} done")
-
-    # ---- timed: served ----
     all_idx = list(range(len(reqs)))
     served, walls = [], []
-    srv.ktime_enable(True)
-    for s in range(0 if args.plain_only else args.steps):
+    steps_timed = 0
+    worst = bad = checked = worst_abs = None
+    if not args.plain_only:
+        # ---- the resident server: workers, streams, workspaces (not timed) ----
         t = time.perf_counter()
-        res, rep = srv.serve(jobs_for(all_idx), workers=args.workers, realtime=wl["realtime"], order=args.order,
-                             window=args.window, pin=args.pin, sync_yield="block" if args.sync == "block" else True,
-                             precision=wl["precision"])
-        walls.append(time.perf_counter() - t)
-        served.append((res, rep))
-        log(f"[bench] step {s}: {rep['pairs']} pairs in {walls[-1]:.1f}s, workers {rep['workers']}, "
-            f"peak concurrency {rep['peak_concurrency']}")
-    srv.ktime_enable(False)
-    s_cnt, s_ms, s_bytes = srv.ktime_harvest()
+        server = srv.Server(jobs_for(all_idx), workers=args.workers, precision=wl["precision"],
+                            sync_yield="block" if args.sync == "block" else True, pin=args.pin)
+        log(f"server: {server.workers} workers, {server.workspace_bytes / 1e9:.2f} GB workspace each, "
+            f"set up in {time.perf_counter() - t:.1f}s; free HBM {srv.device_free_bytes() / 1e9:.1f} GB")
+        # ---- warmup (untimed): the smallest requests ----
+        small_idx = sorted(all_idx, key=lambda i: (reqs[i]["k"], reqs[i]["n"]))[: args.workers]
+        for w in range(args.warmup):
+            server.run(jobs_for(small_idx, False), realtime=False)
+            log(f"warmup step {w} done")
+        # ---- timed: served ----
+        reserve = (0 if args.served_only else 30.0) + (0 if args.no_cpu_baseline else 1.6 * args.cpu_seconds + 10) \
+            + (0 if args.no_verify else 15.0)
+        for s in range(args.steps):
+            elapsed = time.perf_counter() - T_START
+            est = max(walls) if walls else 0.0
+            if s > 0 and elapsed + est + reserve > args.time_budget:
+                log(f"time budget: step {s} of {args.steps} not started ({elapsed:.0f}s elapsed, "
+                    f"{est:.0f}s per step, {reserve:.0f}s reserved)")
+                break
+            free0 = srv.device_free_bytes()
+            t = time.perf_counter()
+            res, rep = server.run(jobs_for(all_idx), realtime=wl["realtime"], order=args.order,
+                                  window=args.window, keep_vectors=(s == 0 and not args.no_verify))
+            walls.append(time.perf_counter() - t)
+            steps_timed += 1
+            if s > 0:
+                for r in res:        # only the first step's pairs are kept for the check
+                    r.pop("eigenvectors", None)
+            served.append((res, rep))
+            log(f"step {s}: {rep['pairs']} pairs in {walls[-1]:.1f}s, workers {rep['workers']}, "
+                f"peak concurrency {rep['peak_concurrency']}, failed {rep['failed']}; free HBM "
+                f"{free0 / 1e9:.1f} -> {srv.device_free_bytes() / 1e9:.1f} GB")
+        server.close()
+        if served and not args.no_verify:
+            t = time.perf_counter()
+            worst, bad, checked, worst_abs = verify_pairs(served[0][0], ops, norms, 1e-10)
+            for r in served[0][0]:
+                r.pop("eigenvectors", None)
+            log(f"verified {checked} pairs of step 0 by fresh operator applies in {time.perf_counter() - t:.1f}s: "
+                f"worst ||Ax - lBx|| / (||A|| + |l| ||B||) {worst:.2e}, worst ||Ax - lBx|| {worst_abs:.2e}, "
+                f"over tol {bad}")
 
     pairs = sum(rep["pairs"] for _, rep in served)
     wall = sum(walls)
     lat = [r["t_done"] - r["arrival"] for res, _ in served for r in res]
-    # FP64 residual of every pair: ||A x - l B x|| / (||A|| + |l| ||B||) <= tol
-    worst = 0.0
-    npairs_checked = 0
-    bad = 0
-    for res, _ in served:
-        for i, r in enumerate(res):
-            for lam, pr in zip(r["eigenvalues"], r["pair_residuals"]):
-                v = pr / (norms[i] + abs(lam) * 1.0)
-                worst = max(worst, v)
-                npairs_checked += 1
-                bad += v > 1e-10
-            bad += sum(r["residuals"] >= 1e-10)
-    statuses = [r["status"] for res, _ in served for r in res]
-    log(f"[bench] served: {pairs} pairs / {wall:.1f}s; worst normalised residual {worst:.2e}; "
-        f"bad {bad}; statuses {sorted(set(statuses))}")
+    statuses = sorted(set(r["status"] for res, _ in served for r in res))
+    conv_over_tol = sum(int(np.sum(r["residuals"] >= 1e-10)) for res, _ in served for r in res)
+    if served:
+        log(f"served: {pairs} pairs / {wall:.1f}s; statuses {statuses}; solver residuals over tol {conv_over_tol}")
 
     # ---- plain: one solve at a time, FP64, stratified by (n, k) class ----
     rng = np.random.default_rng(args.seed + 1)
     classes = {}
     for i, r in enumerate(reqs):
-        # small solves: n is continuous; eight log-spaced n classes
-        key = (r["n"], r["k"]) if args.workload == "serving" else (int(np.log2(r["n"] / 4096) * 8 / 3), r["k"])
-        classes.setdefault(key, []).append(i)
-    sample, cls_of = [], {}
-    for key in sorted(classes):
-        members = classes[key]
-        pick = rng.choice(len(members), size=min(args.plain_per_class, len(members)), replace=False)
-        for j in sorted(pick.tolist()):
-            sample.append(members[j])
-            cls_of[members[j]] = key
-    srv.ktime_enable(True)
-    t = time.perf_counter()
-    pres, prep = srv.serve(jobs_for(sample, False), workers=1, realtime=False, precision=wl["precision"])
-    t_plain = time.perf_counter() - t
-    srv.ktime_enable(False)
-    kcnt, kms, kbytes = srv.ktime_harvest()
-    per_cls = {}
-    for j, i in enumerate(sample):
-        r = pres[j]
-        dt = r["t_done"] - r["t_start"]
-        per_cls.setdefault(cls_of[i], []).append((r["nconv"], dt))
-        log(f"[bench]   plain n={reqs[i]['n']} k={reqs[i]['k']}: {r['nconv']} pairs, {r['iters']} iters, "
-            f"{dt:.2f}s (qz {r['qz_seconds']:.2f}s), status {r['status']}")
-    est_pairs = sum(len(classes[c]) * np.mean([p for p, _ in v]) for c, v in per_cls.items())
-    est_secs = sum(len(classes[c]) * np.mean([d for _, d in v]) for c, v in per_cls.items())
-    plain_rate = est_pairs / est_secs if est_secs else None
-    plain_pairs = prep["pairs"]
-    diff = None
-    if served:
-        # same eigenvalues alone and in the mix
-        mix = served[0][0]
-        diff = 0.0
+        classes.setdefault(class_of(r, args.workload), []).append(i)
+    plain_rate, per_cls, diff, sample, kcnt = None, {}, None, [], 0
+    if not args.served_only:
+        cls_of = {}
+        for key in sorted(classes):
+            members = classes[key]
+            pick = rng.choice(len(members), size=min(args.plain_per_class, len(members)), replace=False)
+            for j in sorted(pick.tolist()):
+                sample.append(members[j])
+                cls_of[members[j]] = key
+        srv.ktime_enable(True)
+        t = time.perf_counter()
+        pres, prep = srv.serve(jobs_for(sample, False), workers=1, realtime=False, precision=wl["precision"])
+        t_plain = time.perf_counter() - t
+        srv.ktime_enable(False)
+        kcnt, kms, kbytes = srv.ktime_harvest()
         for j, i in enumerate(sample):
-            a, b = np.sort_complex(pres[j]["eigenvalues"]), np.sort_complex(mix[i]["eigenvalues"])
-            if a.size and a.size == b.size:
-                diff = max(diff, float(np.max(np.abs(a - b) / np.abs(a))))
-        # where served time goes: the sample's solve and QZ seconds alone vs in
-        # the mix, and the mean number of solves in flight
-        s_alone = sum(pres[j]["t_done"] - pres[j]["t_start"] for j in range(len(sample)))
-        s_mix = sum(mix[i]["t_done"] - mix[i]["t_start"] for i in sample)
-        q_alone = sum(pres[j]["qz_seconds"] for j in range(len(sample)))
-        q_mix = sum(mix[i]["qz_seconds"] for i in sample)
-        busy = sum(r["t_done"] - r["t_start"] for r in mix)
-        log(f"[bench] sample solve seconds alone {s_alone:.1f} / in the mix {s_mix:.1f} "
-            f"(qz {q_alone:.1f} / {q_mix:.1f}); mean solves in flight {busy / walls[0]:.1f}; "
-            f"engine stream {sum(r['stream_bytes'] for r in mix) / walls[0] / 1e9:.0f} GB/s over the step")
-    log(f"[bench] plain: {len(sample)} solves in {len(per_cls)} classes, {plain_pairs} pairs / {t_plain:.1f}s; "
-        f"class-weighted estimate for the workload {plain_rate:.2f} pairs/s; "
-        f"max rel eig diff plain vs served {diff}")
+            r = pres[j]
+            dt = r["t_done"] - r["t_start"]
+            per_cls.setdefault(cls_of[i], []).append((r["nconv"], dt))
+            log(f"  plain n={reqs[i]['n']} k={reqs[i]['k']}: {r['nconv']} pairs, {r['iters']} iters, "
+                f"{dt:.2f}s (qz {r['qz_seconds']:.2f}s), status {r['status']}")
+        est_pairs = sum(len(classes[c]) * np.mean([p for p, _ in v]) for c, v in per_cls.items())
+        est_secs = sum(len(classes[c]) * np.mean([d for _, d in v]) for c, v in per_cls.items())
+        plain_rate = est_pairs / est_secs if est_secs else None
+        if served:
+            # same eigenvalues alone and in the mix
+            mix = served[0][0]
+            diff = 0.0
+            for j, i in enumerate(sample):
+                a, b = np.sort_complex(pres[j]["eigenvalues"]), np.sort_complex(mix[i]["eigenvalues"])
+                if a.size and a.size == b.size:
+                    diff = max(diff, float(np.max(np.abs(a - b) / np.abs(a))))
+            s_alone = sum(pres[j]["t_done"] - pres[j]["t_start"] for j in range(len(sample)))
+            s_mix = sum(mix[i]["t_done"] - mix[i]["t_start"] for i in sample)
+            q_alone = sum(pres[j]["qz_seconds"] for j in range(len(sample)))
+            q_mix = sum(mix[i]["qz_seconds"] for i in sample)
+            busy = sum(r["t_done"] - r["t_start"] for r in mix)
+            log(f"sample solve seconds alone {s_alone:.1f} / in the mix {s_mix:.1f} "
+                f"(qz {q_alone:.1f} / {q_mix:.1f}); mean solves in flight {busy / walls[0]:.1f}; "
+                f"engine stream {sum(r['stream_bytes'] for r in mix) / walls[0] / 1e9:.0f} GB/s over the step")
+        log(f"plain: {len(sample)} solves in {len(per_cls)} classes, {prep['pairs']} pairs / {t_plain:.1f}s; "
+            f"class-weighted estimate for the workload {plain_rate:.2f} pairs/s; "
+            f"max rel eig diff plain vs served {diff}")
 
     served_rate = pairs / wall if wall else None
     roof = None
@@ -317,27 +440,36 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": "k_proj<double,1,16,false> (Q^H x, complex128)", "measured_on": "plain leg",
                 "launches": kcnt, "avg_us": round(kms * 1e3 / kcnt, 2), "avg_bytes": round(kbytes / kcnt)}
-        if s_cnt:
-            roof["served_region_GBs"] = round(s_bytes / (s_ms * 1e-3) / 1e9, 1)
-            roof["served_region_avg_us"] = round(s_ms * 1e3 / s_cnt, 2)
 
-    cpu = None
-    if not args.no_cpu_baseline and not args.plain_only:
-        thr = args.cpu_threads or min(16, os.cpu_count() or 1)
+    cpu, eig_err = None, None
+    if not args.no_cpu_baseline and not args.plain_only and not args.served_only:
+        thr = args.cpu_threads or min(16, host_cpu()[1])
+        # the cheapest classes by the plain GPU leg's time per request
+        order = sorted(per_cls, key=lambda c: np.mean([d for _, d in per_cls[c]]))
         try:
-            cpu = cpu_baseline(reqs, args.cpu_seconds, thr, wl)
-            log(f"[bench] cpu baseline: {cpu['value']} eigenpairs/s ({cpu['kind']}, {cpu['cores']} threads)")
+            cpu, errs, done = cpu_baseline(reqs, served[0][0] if served else None, wl, args.workload, order,
+                                           thr, args.cpu_seconds)
+            if errs:
+                eig_err = max(e for _, e, _, _ in errs)
+            # the plain GPU leg's rate on the same classes, for a like-for-like ratio
+            if done:
+                gp = sum(np.mean([p for p, _ in per_cls[k]]) for k, _, _, _ in done)
+                gs = sum(np.mean([d for _, d in per_cls[k]]) for k, _, _, _ in done)
+                cpu["gpu_plain_on_sample_classes"] = round(gp / gs, 3) if gs else None
+            log(f"cpu baseline: {cpu['value']} eigenpairs/s ({cpu['kind']}, {cpu['cores']} threads, "
+                f"{cpu['host_cpu']}); one thread {cpu['one_thread']}; max rel eig err served vs reference "
+                f"{eig_err} over {len(errs)} requests")
         except Exception as e:  # noqa: BLE001
-            log(f"[bench] cpu baseline unavailable: {e}")
+            log(f"cpu baseline unavailable: {e}")
 
     out = {
         "metric": "eigenpairs/s @tol1e-10 (served vs plain, 1 GPU); p50/p99 solve latency",
         "value": round(served_rate, 3) if served_rate else None,
         "unit": "eigenpairs/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": steps_timed,
         "warmup": args.warmup,
-        "ms_per_step": round(1e3 * wall / args.steps, 1) if served else None,
+        "ms_per_step": round(1e3 * wall / steps_timed, 1) if steps_timed else None,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -356,6 +488,7 @@ def main():
                    "workers": args.workers, "requests": args.nreq,
                    "admission": args.order + (f" {args.window}" if args.order == "window" else ""),
                    "host_wait": args.sync, "workers_pinned": bool(args.pin)},
+        "steps_requested": args.steps,
         "served_eigenpairs_per_s": round(served_rate, 3) if served_rate else None,
         "plain_eigenpairs_per_s": round(plain_rate, 3) if plain_rate else None,
         "served_over_plain": round(served_rate / plain_rate, 2) if served_rate and plain_rate else None,
@@ -364,8 +497,13 @@ def main():
         "latency_p50_s": round(pct(lat, 50), 3) if lat else None,
         "latency_p99_s": round(pct(lat, 99), 3) if lat else None,
         "pairs": pairs,
-        "max_normalised_residual": worst,
-        "pairs_over_tol": int(bad),
+        "statuses": statuses,
+        "solver_residuals_over_tol": conv_over_tol,
+        "verified_pairs": checked,
+        "verified_max_normalised_residual": worst,
+        "verified_max_residual": worst_abs,
+        "verified_pairs_over_tol": bad,
+        "max_rel_eig_err_vs_reference": eig_err,
         "max_rel_eig_diff_alone_vs_mix": diff,
         "setup_seconds": round(t_setup, 1),
         "roofline": roof,

@@ -82,8 +82,32 @@ typedef struct {
 void bbjd_serve_config_init(bbjd_serve_config *c);
 
 /* Serves nreq requests; returns 0 when the run itself worked (individual
- * request failures are in their status), -1 on a set-up error. */
+ * request failures are in their status), -1 on a set-up error.  Same as
+ * bbjd_server_create + bbjd_server_run + bbjd_server_destroy. */
 int bbjd_serve(bbjd_request *reqs, int nreq, const bbjd_serve_config *cfg, bbjd_serve_report *rep);
+
+/* A resident server: its worker threads, HIP streams and workspaces stay
+ * allocated across runs, so batches of requests arriving one after another
+ * do not pay the workspace set-up (and do not allocate or free device
+ * memory) per batch.
+ *
+ * bbjd_server_create sizes the workspaces for the largest of the nreq
+ * requests given (n, nev, maxsub, inner_maxiter; only these fields are
+ * read) and takes workers, hbm_budget, precision, graphs, pin and sync_yield
+ * from cfg; NULL on failure.  bbjd_server_run serves a batch like
+ * bbjd_serve, using cfg's realtime, order and window; a request larger
+ * than the capacity ends with status BBJD_ERR_INVALID.  Arrival times are
+ * relative to the start of each run.  One run at a time per server. */
+typedef struct bbjd_server bbjd_server;
+bbjd_server *bbjd_server_create(const bbjd_request *reqs, int nreq, const bbjd_serve_config *cfg);
+int          bbjd_server_run(bbjd_server *s, bbjd_request *reqs, int nreq, const bbjd_serve_config *cfg,
+                             bbjd_serve_report *rep);
+void         bbjd_server_destroy(bbjd_server *s);
+int          bbjd_server_workers(const bbjd_server *s);           /* workers with a workspace */
+size_t       bbjd_server_workspace_bytes(const bbjd_server *s);   /* per worker               */
+
+/* Free device memory now (hipMemGetInfo), for logs. */
+size_t bbjd_device_free_bytes(void);
 
 #ifdef __cplusplus
 }

@@ -1,6 +1,6 @@
 """σ sweeps (bbjd_amd.sweep): the merge of eigenvalues found by several
 requests on the CPU, and a served sweep with and without warm starts on the
-GPU."""
+GPU, each request checked against the oracle."""
 import numpy as np
 import pytest
 
@@ -65,3 +65,14 @@ def test_served_sweep_merges_overlapping_windows():
     assert len(out["eigenvalues"]) == len(chained["eigenvalues"])
     for ev in chained["eigenvalues"]:
         assert np.min(np.abs(out["eigenvalues"] - ev)) <= 1e-8 * max(1.0, abs(ev))
+    # each request of the plain sweep against the oracle (CPU restatement of
+    # the reference solver) at the same sigma, same settings and start vector:
+    # the same eigenvalues within 1e-8 relative, in both directions
+    from oracle import orc
+    from tests.helpers import rel_match
+    for s, r in zip(sigmas, out["requests"]):
+        ref = orc.solve("oracle", p.n, 5, s, tol=1e-10, maxsub=20, minsub=10, maxiter=1000,
+                        ops=orc.CpuPencilOps(p, s, 1))
+        assert ref["nconv"] == r["nconv"] == 5
+        assert rel_match(r["eigenvalues"], ref["evals"]) <= 1e-8
+        assert rel_match(ref["evals"], r["eigenvalues"]) <= 1e-8
