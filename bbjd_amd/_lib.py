@@ -54,7 +54,7 @@ class Stats(C.Structure):
                 ("inner_steps", C.c_int64), ("dgks_second_passes", C.c_int64),
                 ("sum_k", C.c_int64), ("qz_seconds", C.c_double), ("solve_seconds", C.c_double),
                 ("stream_bytes", C.c_double), ("restart_fallbacks", C.c_int64),
-                ("graph_captures", C.c_int64), ("graph_launches", C.c_int64)]
+                ("graph_captures", C.c_int64), ("graph_launches", C.c_int64), ("qz_schur_ties", C.c_int64)]
 
 
 class PencilDesc(C.Structure):

@@ -1130,11 +1130,24 @@ static bbjd_status solve_impl(const bbjd_params *P, bbjd_workspace *ws, bbjd_res
         /* ---- steps 1-2: projected QZ on the host, Ritz coefficients ---- */
         /* the ordered Schur form where a thick restart may follow (k >= ms),
          * else the eigenvalues and the target Ritz vector only */
-        const int schur = d->opt.qz_mode == BBJD_QZ_SCHUR || k >= ms;
+        int schur = d->opt.qz_mode == BBJD_QZ_SCHUR || k >= ms;
         zcx ta, tb;
         {
             const double tq = now_s();
-            int qrc;
+            int qrc = 0;
+            if (!schur) {
+                qrc = bbjd_qz_target(&d->qz, k, MA, MB, nu0, mu0, al, be, sigma, P->target, eval, R->nconv, tol,
+                                     &p);
+                if (qrc == 2) {             /* ambiguous target: the reference's sequence decides */
+                    schur = 1;
+                    qrc = 0;
+                    d->st.qz_schur_ties++;
+                } else if (!qrc) {
+                    ta = al[p];
+                    tb = be[p];
+                    qrc = bbjd_ritz_vector(&d->qz, k, MA, MB, ta, tb, d->xv);
+                }
+            }
             if (schur) {
                 bbjd_int nkeep = minsub - locked;
                 if (nkeep < 1) nkeep = 1;
@@ -1145,14 +1158,6 @@ static bbjd_status solve_impl(const bbjd_params *P, bbjd_workspace *ws, bbjd_res
                     tb = be[p];
                     bbjd_ritz_coefficients(k, ldm, p, ta, tb, (const zcx *)ws->SA, (const zcx *)ws->SB,
                                            (const zcx *)ws->ZB, d->yv, d->xv);
-                }
-            } else {
-                qrc = bbjd_qz_target(&d->qz, k, MA, MB, nu0, mu0, al, be, sigma, P->target, eval, R->nconv, tol,
-                                     &p);
-                if (!qrc) {
-                    ta = al[p];
-                    tb = be[p];
-                    qrc = bbjd_ritz_vector(&d->qz, k, MA, MB, ta, tb, d->xv);
                 }
             }
             d->st.qz_seconds += now_s() - tq;

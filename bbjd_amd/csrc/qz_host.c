@@ -227,7 +227,8 @@ void bbjd_ritz_coefficients(bbjd_int k, bbjd_int ld, bbjd_int p, zcplx ta, zcplx
 /* Eigenvalues of the projected pencil through the standard eigenproblem of
  * the harmonic form (see the file comment); returns 1 when C is not
  * triangular enough to use it.  alpha / beta = theta. */
-static int eigs_harmonic(bbjd_qz *q, bbjd_int k, const zcplx *MA, const zcplx *MB, zcplx nu0, zcplx mu0)
+static int eigs_harmonic(bbjd_qz *q, bbjd_int k, const zcplx *MA, const zcplx *MB, zcplx nu0, zcplx mu0,
+                         double tol)
 {
     const bbjd_int ld = q->maxsub;
     const int inv = cabs(nu0) > 0;               /* C = nu0 MA + mu0 MB; else C = MB */
@@ -249,13 +250,32 @@ static int eigs_harmonic(bbjd_qz *q, bbjd_int k, const zcplx *MA, const zcplx *M
     blas_int bk = (blas_int)k, ilo = 1, ihi = bk, one = 1, lw = q->lwork, info = 0;
     const zcplx a1 = 1.0;
     scipy_ztrsm_("L", "U", "N", "N", &bk, &bk, &a1, C, &bk, M, &bk);
+    double mfro = 0;
+    for (bbjd_int i = 0; i < k * k; i++) mfro += creal(M[i]) * creal(M[i]) + cimag(M[i]) * cimag(M[i]);
+    mfro = sqrt(mfro);
+    if (!isfinite(mfro)) return 1;
     scipy_zgehrd_(&bk, &ilo, &ihi, M, &bk, q->rhs, q->work, &lw, &info);
     if (info) return 1;
     scipy_zhseqr_("E", "N", &bk, &ilo, &ihi, M, &bk, q->eal, NULL, &one, q->work, &lw, &info);
     if (info) return 1;
+    /* Accuracy of theta from mu.  The Hessenberg QR gives each mu with an
+     * absolute error of about eps ||M||; with nu0 != 0, theta = 1 / (nu0 mu)
+     * - mu0 / nu0 moves by |d mu| / (|nu0| |mu|^2).  When a Ritz value sits
+     * on sigma (C nearly singular: an eigenvalue at the shift), ||M|| is
+     * huge and the values far from sigma lose their digits -- enough that a
+     * converged value is no longer matched within 100 tol and is taken as a
+     * target again.  Any theta that may be off by more than 10 tol sends the
+     * step to the generalized QZ (ZGGEV), whose errors do not grow so. */
+    const double dmu = 4.0 * DBL_EPSILON * mfro;
     for (bbjd_int i = 0; i < k; i++) {
         const zcplx mu = q->eal[i];
         if (!isfinite(creal(mu)) || !isfinite(cimag(mu))) return 1;
+        const double amu = cabs(mu);
+        if (inv) {
+            if (amu * cabs(nu0) > 1e-300 && dmu / (cabs(nu0) * amu * amu) > 10.0 * tol) return 1;
+        } else if (dmu > 10.0 * tol) {
+            return 1;
+        }
         if (inv) {
             /* mu = 1 / (nu0 theta + mu0): theta = (1 - mu0 mu) / (nu0 mu) */
             q->eal[i] = 1.0 - mu0 * mu;
@@ -272,7 +292,9 @@ static int eigs_harmonic(bbjd_qz *q, bbjd_int k, const zcplx *MA, const zcplx *M
  * Schur form, cached while the pencil is unchanged (lock iterations): the
  * harmonic standard eigenproblem, else ZGGEV('N', 'N') on copies of
  * (MA, MB).  The target is the best unconverged value by key over all
- * positions, as in bbjd_qz_solve. */
+ * positions, as in bbjd_qz_solve.  Returns 0, -1 on a LAPACK error, or 2
+ * when the choice is ambiguous (a tie up to rounding, or no unconverged
+ * candidate): then the ordered Schur form decides, as in the reference. */
 int bbjd_qz_target(bbjd_qz *q, bbjd_int k, const zcplx *MA, const zcplx *MB, zcplx nu0, zcplx mu0, zcplx *al,
                    zcplx *be, zcplx sigma, bbjd_target tgt, const zcplx *conv, bbjd_int nconv, double tol,
                    bbjd_int *tpos)
@@ -280,7 +302,7 @@ int bbjd_qz_target(bbjd_qz *q, bbjd_int k, const zcplx *MA, const zcplx *MB, zcp
     const bbjd_int ld = q->maxsub;
     const size_t colb = (size_t)k * sizeof(zcplx);
     if (!(q->ecache_valid && q->ecache_k == k)) {
-        if (eigs_harmonic(q, k, MA, MB, nu0, mu0)) {
+        if (eigs_harmonic(q, k, MA, MB, nu0, mu0, tol)) {
             zcplx *A = q->cSA, *B = q->cSB;   /* scratch: the ZGGES cache is invalid here */
             for (bbjd_int j = 0; j < k; j++) {
                 memcpy(A + j * ld, MA + j * ld, colb);
@@ -299,15 +321,27 @@ int bbjd_qz_target(bbjd_qz *q, bbjd_int k, const zcplx *MA, const zcplx *MB, zcp
     memcpy(al, q->eal, colb);
     memcpy(be, q->ebe, colb);
     bbjd_int p = 0;
-    double bv = DBL_MAX;
+    double bv = DBL_MAX, bv2 = DBL_MAX;
     for (bbjd_int i = 0; i < k; i++) {
         if (cabs(be[i]) < 1e-300) continue;
         zcplx lam = al[i] / be[i];
         if (is_converged(lam, conv, nconv, tol)) continue;
         double v = key_of(tgt, lam, sigma);
-        if (v < bv) { bv = v; p = i; }
+        if (v < bv) { bv2 = bv; bv = v; p = i; }
+        else if (v < bv2) bv2 = v;
     }
     *tpos = p;
+    /* The reference scans the positions of its reordered Schur form and the
+     * first best value wins; this step's order is the Hessenberg QR's.  The
+     * two agree unless the choice is a tie up to rounding or there is no
+     * unconverged candidate (the reference then takes position 0 of its
+     * ordered form): say so, and the caller decides by the reference's
+     * sequence for this iteration. */
+    if (bv == DBL_MAX) return 2;
+    if (bv2 < DBL_MAX) {
+        const double scale = fmax(fmax(fabs(bv), fabs(bv2)), 1e-14);
+        if (bv2 - bv <= 1e-10 * scale) return 2;
+    }
     return 0;
 }
 

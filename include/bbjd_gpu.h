@@ -139,6 +139,10 @@ typedef struct {
                                    space fell back to column-wise CGS2    */
     bbjd_int graph_captures;    /* inner-solve segments captured (graphs)  */
     bbjd_int graph_launches;    /* and launched, captures included         */
+    bbjd_int qz_schur_ties;     /* iterations between restarts whose target
+                                   was a tie up to rounding (or had no
+                                   unconverged candidate), decided by the
+                                   ordered Schur form as the reference does */
 } bbjd_stats;
 
 int bbjd_workspace_stats(const bbjd_workspace *ws, bbjd_stats *st);

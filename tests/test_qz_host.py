@@ -1,0 +1,105 @@
+"""The engine's projected eigenvalue step on the host (qz_host.c,
+bbjd_qz_target: eigenvalues of the projected pencil (MA, MB) through the
+harmonic standard problem C^-1 MB, C = nu0 MA + mu0 MB upper triangular,
+falling back to the generalized QZ) against what the mathematics fixes:
+W^H A V, W^H B V with V spanning the whole space have exactly the
+eigenvalues of (A, B).  Host logic only, no GPU."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from bbjd_amd import _lib
+
+
+def _qz_target(MA, MB, nu0, mu0, sigma, conv=(), tol=1e-10, want_rc=(0,)):
+    lib = _lib.load()
+    k = MA.shape[0]
+    f = lib.bbjd_qz_target
+    f.restype = C.c_int
+    f.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, _lib.zcomplex, _lib.zcomplex, C.c_void_p,
+                  C.c_void_p, _lib.zcomplex, C.c_int, C.c_void_p, C.c_int64, C.c_double, C.POINTER(C.c_int64)]
+    lib.bbjd_qz_init.restype = C.c_int
+    lib.bbjd_qz_init.argtypes = [C.c_void_p, C.c_int64]
+    lib.bbjd_qz_free.argtypes = [C.c_void_p]
+    q = C.create_string_buffer(4096)          # bbjd_qz (a few hundred bytes)
+    assert lib.bbjd_qz_init(q, k) == 0
+    try:
+        MAf, MBf = np.asfortranarray(MA), np.asfortranarray(MB)
+        al, be = np.zeros(k, np.complex128), np.zeros(k, np.complex128)
+        cv = np.ascontiguousarray(np.asarray(conv, np.complex128)) if len(conv) else np.zeros(1, np.complex128)
+        p = C.c_int64(-1)
+        rc = f(q, k, MAf.ctypes.data, MBf.ctypes.data, _lib.zc(nu0), _lib.zc(mu0), al.ctypes.data, be.ctypes.data,
+               _lib.zc(sigma), 0, cv.ctypes.data, len(conv), tol, C.byref(p))
+        assert rc in want_rc, rc
+        return al / be, p.value, rc
+    finally:
+        lib.bbjd_qz_free(q)
+
+
+def _projected(d, sigma, V):
+    A = np.diag(np.asarray(d, np.complex128))
+    B = np.eye(len(d), dtype=np.complex128)
+    nu0 = 1 / np.sqrt(1 + abs(sigma) ** 2)
+    mu0 = -sigma * nu0
+    W, _ = np.linalg.qr((nu0 * A + mu0 * B) @ V)
+    # the engine's W makes C = nu0 MA + mu0 MB upper triangular with a
+    # positive real diagonal (CGS2 column by column); a QR does too up to
+    # column phases, which do not change the eigenvalues
+    return W.conj().T @ A @ V, W.conj().T @ B @ V, nu0, mu0
+
+
+@pytest.mark.parametrize("eps", [0.0, 1e-11, 1e-6])
+def test_eigenvalue_on_sigma_stays_accurate(eps):
+    """sigma exactly on a double eigenvalue and V holding its eigenspace: C
+    is singular to rounding (||C^-1 MB|| ~ 1e20+), where the harmonic
+    route's far eigenvalues lost all their digits (off by 0.3 to 8) and a
+    converged value was no longer matched within 100 tol -- the engine then
+    returned 2 three times on the reference suite's repeated-eigenvalue case
+    (tests/test_bbjd.py:527-543).  Every eigenvalue within 1e-9 now, and the
+    target skips the converged 2."""
+    rng = np.random.default_rng(1)
+    d = [1.0, 2.0, 2.0, 3.0, 4.0]
+    v1 = np.zeros(5, np.complex128)
+    v1[1] = 1
+    v1 += eps * (rng.standard_normal(5) + 1j * rng.standard_normal(5))
+    V, _ = np.linalg.qr(np.column_stack([v1, rng.standard_normal((5, 4)) + 1j * rng.standard_normal((5, 4))]))
+    MA, MB, nu0, mu0 = _projected(d, 2.0, V)
+    # 1 and 3 are equally far from sigma = 2: a tie, reported (rc 2)
+    th, p, rc = _qz_target(MA, MB, nu0, mu0, 2.0, conv=[2.0], want_rc=(2,))
+    for t in th:
+        assert np.min(np.abs(t - np.array(d))) <= 1e-9, th
+    assert abs(th[p] - 2.0) > 0.5            # the converged 2 is not the target again
+    assert min(abs(th[p] - 1.0), abs(th[p] - 3.0)) <= 1e-9
+
+
+def test_ambiguous_target_is_reported():
+    """the eigenvalue-only step scans the Hessenberg QR's order, the
+    reference its reordered Schur form's (src/jd_core.c:430-458), first best
+    value winning: a tie up to rounding, or no unconverged candidate (the
+    reference then takes position 0 of its ordered form), returns 2 so that
+    the engine decides by the reference's sequence; a clear best returns 0"""
+    rng = np.random.default_rng(3)
+    V, _ = np.linalg.qr(rng.standard_normal((6, 6)) + 1j * rng.standard_normal((6, 6)))
+    d = [0.5, 1.0, 1.7, 2.3, 3.0, 4.5]
+    MA, MB, nu0, mu0 = _projected(d, 2.0, V)
+    th, p, _ = _qz_target(MA, MB, nu0, mu0, 2.0, want_rc=(2,))          # 1.7 and 2.3 tie
+    assert min(abs(th[p] - 1.7), abs(th[p] - 2.3)) <= 1e-9
+    th, p, _ = _qz_target(MA, MB, nu0, mu0, 2.05, want_rc=(0,))         # 2.3 is nearer
+    assert abs(th[p] - 2.3) <= 1e-9
+    _qz_target(MA, MB, nu0, mu0, 2.05, conv=d, want_rc=(2,))              # all converged
+
+
+def test_well_separated_sigma_uses_the_standard_problem_accurately():
+    """sigma away from the spectrum: the harmonic route is accurate (the
+    step's cheap path), eigenvalues within 1e-12 relative"""
+    rng = np.random.default_rng(2)
+    n = 40
+    d = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    V, _ = np.linalg.qr(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+    sigma = 0.3 + 0.2j
+    MA, MB, nu0, mu0 = _projected(d, sigma, V)
+    th, p, _ = _qz_target(MA, MB, nu0, mu0, sigma)
+    for t in th:
+        assert np.min(np.abs(t - d)) <= 1e-12 * max(1, abs(t))
+    assert abs(th[p] - sigma) == pytest.approx(np.min(np.abs(d - sigma)), rel=1e-12)
