@@ -2020,3 +2020,95 @@ int bbjd_kbench(int which, int64_t n, int k, int reps, double *avg_ms, double *b
 
 }  // extern "C"
 
+// ---------------------------------------------------------------------------
+// Allocation-free CGS2 on device arrays (bbjd_block_orthogonalize_noalloc):
+// the two-stage reductions' scratch (partial sums, per-block norms, the
+// coefficient vectors, the scalar slots, the decision flags and the
+// last-block-done counters) lives in one file-scope device array instead of
+// a per-call allocation.  Sized for n <= kNaMaxN and k <= kNaMaxK:
+//   partials   kNaPart complex: the projection keeps at most one partial per
+//              row chunk and column (chunks ~ blocks per launch / column
+//              tiles, so <= 16 x blocks per launch, and the launcher refuses
+//              a grid whose partials exceed part_cap), the fused CGS2's
+//              middle pass at most 512 blocks x k (k <= 128);
+//   norms      one double per row tile of >= 64 rows;
+//   h, h2, hsum, coef: k + 8 complex each.
+namespace {
+constexpr int64_t kNaMaxN = (int64_t)1 << 23;
+constexpr int kNaMaxK = 1024;
+constexpr int64_t kNaPart = 65536 + 64;
+constexpr int64_t kNaPnorm = kNaMaxN / BBJD_TILE_MIN_ROWS + 64;
+constexpr int64_t kNaVec = kNaMaxK + 8;
+constexpr size_t na_al(size_t b) { return (b + 255) / 256 * 256; }
+constexpr size_t kNaOffPart = 0;
+constexpr size_t kNaOffPnorm = kNaOffPart + na_al(kNaPart * 16);
+constexpr size_t kNaOffH = kNaOffPnorm + na_al(kNaPnorm * 8);
+constexpr size_t kNaOffH2 = kNaOffH + na_al(kNaVec * 16);
+constexpr size_t kNaOffHsum = kNaOffH2 + na_al(kNaVec * 16);
+constexpr size_t kNaOffCoef = kNaOffHsum + na_al(kNaVec * 16);
+constexpr size_t kNaOffCoef2 = kNaOffCoef + na_al(kNaVec * 16);
+constexpr size_t kNaOffScal = kNaOffCoef2 + na_al(64 * 16);
+constexpr size_t kNaOffInts = kNaOffScal + na_al(SL_COUNT * 8);          // flags, then counters
+constexpr int kNaInts = FL_COUNT + BBJD_CNT_SLOTS;
+constexpr size_t kNaBytes = kNaOffInts + na_al((size_t)kNaInts * 4);
+
+// Per-device address of `scratch`, from hipGetSymbolAddress on the first
+// call on each device (the array's name is not a device address in host
+// code).
+constexpr int kNaMaxDevices = 64;
+std::mutex g_na_mu;
+void *g_na_ptr[kNaMaxDevices];
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNaBytes];
+
+// zeroes the flags and counters (they must start at zero; a previous call
+// may have been cut short)
+__global__ void k_na_clear(int *ints, int cnt)
+{
+    for (int i = threadIdx.x; i < cnt; i += blockDim.x) ints[i] = 0;
+}
+
+__global__ void k_na_copy_norm(const double *src, double *dst) { if (threadIdx.x == 0) *dst = *src; }
+
+static void *na_scratch_ptr()
+{
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kNaMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_na_mu);
+    if (!g_na_ptr[dev]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return nullptr;
+        g_na_ptr[dev] = p;
+    }
+    return g_na_ptr[dev];
+}
+
+extern "C" int bbjd_block_orthogonalize_noalloc(int64_t n, int64_t k, const void *Q, int64_t ldq, void *v,
+                                                void *h_work, double *norm_out, void *stream)
+{
+    if (n <= 0 || n > kNaMaxN || k < 0 || k > kNaMaxK || !v || (k > 0 && (!Q || ldq < n))) return -1;
+    char *base = (char *)na_scratch_ptr();
+    if (!base) return -2;
+    hipStream_t s = (hipStream_t)stream;
+    bbjd_red rd;
+    rd.part = base + kNaOffPart;
+    rd.pnorm = (double *)(base + kNaOffPnorm);
+    rd.h = base + kNaOffH;
+    rd.h2 = base + kNaOffH2;
+    rd.hsum = base + kNaOffHsum;
+    rd.coef = base + kNaOffCoef;
+    rd.coef2 = base + kNaOffCoef2;
+    rd.scal = (double *)(base + kNaOffScal);
+    rd.flag = (int *)(base + kNaOffInts);
+    rd.cnt = rd.flag + FL_COUNT;
+    rd.part_cap = kNaPart;
+    hipLaunchKernelGGL(k_na_clear, dim3(1), dim3(TPB), 0, s, rd.flag, kNaInts);
+    int rc = chk();
+    if (!rc) rc = cgs2<double>(s, n, (int)k, Q, ldq, v, &rd, SL_T0, k > 0 ? h_work : nullptr);
+    if (!rc && norm_out) {
+        hipLaunchKernelGGL(k_na_copy_norm, dim3(1), dim3(64), 0, s, (const double *)rd.scal + SL_T0, norm_out);
+        rc = chk();
+    }
+    return rc ? -2 : 0;
+}

@@ -252,6 +252,25 @@ int bbjd_set_tiles(int proj_blocks, int upd_rows, int blocks_target);
 int bbjd_set_tile_table(int nn, const int64_t *n_upper, int nk, const int *k_upper, const int *proj_blocks,
                         const int *upd_rows);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory -- code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc: bbjd_block_orthogonalize (CGS2 with the DGKS test,
+ * reference src/jd_core.c:207-241) on device arrays, with the reduction
+ * scratch in a file-scope device array of the library (about 2.2 MB on each
+ * GPU it runs on, held until the process ends) instead of a per-call
+ * allocation; it calls no allocating routine.
+ * Q (n x k, column-major, leading dimension ldq >= n), v (n entries,
+ * orthogonalised in place), h_work (k entries: the summed coefficients of
+ * both passes, or NULL) and norm_out (one double: ||v|| after, or NULL) are
+ * device pointers.  Everything is enqueued on stream (hipStream_t; NULL:
+ * the null stream) and nothing waits.  Limits: n <= 2^23, k <= 1024.
+ * Returns 0; -1 for sizes beyond the limits or a NULL array (nothing
+ * enqueued); -2 on a device error (v undefined).  Every call on one GPU
+ * shares the scratch: calls in flight at the same time on one GPU (from
+ * other streams or threads) race; calls ordered on one stream do not. */
+int bbjd_block_orthogonalize_noalloc(bbjd_int n, bbjd_int k, const bbjd_zomplex *Q, bbjd_int ldq, bbjd_zomplex *v,
+                                     bbjd_zomplex *h_work, double *norm_out, void *stream);
+
 /* Device memory helpers for callers without another GPU runtime (tests,
  * tools): allocate / free / synchronise the device. */
 void *bbjd_dev_alloc(size_t bytes);
