@@ -228,7 +228,7 @@ def cpu_baseline(reqs, served, wl, workload, classes_wanted, threads, budget_s):
                 errs.append((i, e, len(ge), len(ce)))
     one = None
     if done:
-        i = done[0][1]
+        i = done[0][1]          # the cheapest class's request
         out1, dt1 = run(i, 1)
         one = {"value": out1["nconv"] / dt1, "cores": 1, "sample": f"request {i} (n={reqs[i]['n']}, "
                f"k={reqs[i]['k']}): {out1['nconv']} pairs in {dt1:.2f} s"}
@@ -255,7 +255,7 @@ def main():
     ap.add_argument("--workers", type=int, default=16)
     ap.add_argument("--maxiter", type=int, default=1000)
     ap.add_argument("--plain-per-class", type=int, default=2, help="plain solves per (n, k) class")
-    ap.add_argument("--cpu-seconds", type=float, default=25.0)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-verify", action="store_true", help="skip the residual check of the returned pairs")
@@ -269,7 +269,7 @@ def main():
     ap.add_argument("--sync", choices=["yield", "block"], default="yield",
                     help="how host threads wait on the GPU")
     ap.add_argument("--pin", action="store_true", help="pin each serving worker to one CPU")
-    ap.add_argument("--time-budget", type=float, default=560.0,
+    ap.add_argument("--time-budget", type=float, default=540.0,
                     help="seconds for the whole run: timed steps that would end past it, less what the "
                          "plain leg, the verification and the CPU baseline need, are not started (the "
                          "line then reports the steps timed)")
@@ -444,8 +444,9 @@ def main():
     cpu, eig_err = None, None
     if not args.no_cpu_baseline and not args.plain_only and not args.served_only:
         thr = args.cpu_threads or min(16, host_cpu()[1])
-        # the cheapest classes by the plain GPU leg's time per request
-        order = sorted(per_cls, key=lambda c: np.mean([d for _, d in per_cls[c]]))
+        # the cheapest classes for the CPU first: its cost grows with n and,
+        # through the projected QZ, with k
+        order = sorted(per_cls, key=lambda c: (c[0] * c[1], c))
         try:
             cpu, errs, done = cpu_baseline(reqs, served[0][0] if served else None, wl, args.workload, order,
                                            thr, args.cpu_seconds)
