@@ -11,5 +11,6 @@ HAVE_C_EXT = _os.path.exists(LIB_PATH)
 
 from .bbjd import jdqz  # noqa: E402
 from .pencil import BandedTauPencil, DeviceOperator  # noqa: E402
+from .sparse import SparseDeviceOperator  # noqa: E402
 
-__all__ = ["jdqz", "HAVE_C_EXT", "BandedTauPencil", "DeviceOperator"]
+__all__ = ["jdqz", "HAVE_C_EXT", "BandedTauPencil", "DeviceOperator", "SparseDeviceOperator"]

@@ -11,9 +11,9 @@ stops later callbacks and is re-raised as itself after the solve
 length raises ValueError instead of being copied blindly (the reference
 copies n elements unchecked, src/_bbjd_c.c:68).
 
-Extension (keyword-only): ``device_op`` -- a ``DeviceOperator`` whose
-device-pointer callbacks replace the three Python callables (no host round
-trip per apply); ``precision`` -- 'fp64' or 'mixed' (complex64 inner GMRES);
+Extension (keyword-only): ``device_op`` -- a ``DeviceOperator`` or a
+``SparseDeviceOperator`` whose device-pointer callbacks replace the three
+Python callables (no host round trip per apply); ``precision`` -- 'fp64' or 'mixed' (complex64 inner GMRES);
 ``qz_mode`` -- 'ritz' (ordered Schur form only where a restart may follow)
 or 'schur' (every iteration, the reference's sequence); ``graphs`` -- with
 ``device_op``, capture the inner GMRES steps into HIP graphs and replay
@@ -81,21 +81,13 @@ def jdqz_c(n, nev, sigma, target, matvec_A, matvec_B, precond_solve, tol=1e-10, 
     errbox: list = []
     keep = []
     if device_op is not None:
+        # the operator names the C callbacks it provides; params takes the
+        # three of the reference's contract, the options the optional ones
         opts.callback_space = _lib.CALLBACK_DEVICE
-        p.matvec_A = _lib.fn_addr("bbjd_op_matvec_A")
-        p.matvec_B = _lib.fn_addr("bbjd_op_matvec_B")
-        p.precond = _lib.fn_addr("bbjd_op_precond") if precond_solve is not None else None
-        if precond_solve is not None:
-            opts.precond2 = _lib.fn_addr("bbjd_op_precond2")
-        opts.matvec_AB = _lib.fn_addr("bbjd_op_matvec_AB")
-        opts.matvec_ABr = _lib.fn_addr("bbjd_op_matvec_ABr")
+        p.precond = None
+        for slot, name in device_op.device_callbacks(precision, precond_solve is not None).items():
+            setattr(p if slot in ("matvec_A", "matvec_B", "precond") else opts, slot, _lib.fn_addr(name))
         p.op_data = device_op.handle
-        if precision == "mixed" and device_op.c64:
-            opts.matvec_A_c64 = _lib.fn_addr("bbjd_op_matvec_A_c64")
-            opts.matvec_B_c64 = _lib.fn_addr("bbjd_op_matvec_B_c64")
-            opts.matvec_AB_c64 = _lib.fn_addr("bbjd_op_matvec_AB_c64")
-            if precond_solve is not None:
-                opts.precond_c64 = _lib.fn_addr("bbjd_op_precond_c64")
     else:
         cbA, cbB = _trampoline(matvec_A, errbox), _trampoline(matvec_B, errbox)
         keep += [cbA, cbB]

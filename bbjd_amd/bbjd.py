@@ -11,10 +11,13 @@ objects with .matvec, or callables (reference bbjd.py:161-205); the
 preconditioner an object with .solve or a callable (bbjd.py:208-219).  These
 run on the host and the engine stages each vector to/from HBM.
 
-Device fast path (extension): pass a ``DeviceOperator`` as A (B None) and
-the same or another DeviceOperator, or True, as ``preconditioner``; then the
-matvecs and the preconditioner run on the GPU as well and no n-vector
-crosses PCIe during the iteration.  ``precision='mixed'`` runs the inner
+Device fast path (extension): pass a ``DeviceOperator`` (banded + dense
+rows) or a ``SparseDeviceOperator`` (any sparsity, sparse-LU
+preconditioner) as A (B None) and the same operator, or True, as
+``preconditioner``; or keep the reference's call with scipy matrices and a
+SuperLU preconditioner and add ``device=True``.  Then the matvecs and the
+preconditioner run on the GPU as well and no n-vector crosses PCIe during
+the iteration.  ``precision='mixed'`` runs the inner
 GMRES (inner_maxiter > 0) in complex64 with FP32 accumulation, and
 ``graphs=True`` replays its steps from captured HIP graphs.
 """
@@ -25,6 +28,7 @@ import scipy.sparse as sp
 
 from . import _bbjd_c
 from .pencil import DeviceOperator
+from .sparse import SparseDeviceOperator
 
 # Target values fixed by reference include/bbjd.h:83-91 / bbjd.py:48-56.
 _TARGET_MAP = {
@@ -95,7 +99,7 @@ def _start_vector(v0, n):
 def jdqz(A, B=None, k: int = 6, sigma: complex = 0.0, which: str = 'nearest', tol: float = 1e-10,
          maxiter: int = 200, maxsub: int = 0, minsub: int = 0, inner_maxiter: int = 0,
          inner_tol: float = 1e-2, preconditioner=None, v0=None, verbose: int = 0,
-         return_info: bool = False, precision: str = 'fp64', graphs: bool = False):
+         return_info: bool = False, precision: str = 'fp64', graphs: bool = False, device: bool = False):
     """Jacobi-Davidson QZ for A x = lambda B x, eigenvalues nearest sigma
     (or by `which`).
 
@@ -106,7 +110,19 @@ def jdqz(A, B=None, k: int = 6, sigma: complex = 0.0, which: str = 'nearest', to
     input and RuntimeError on a negative solver status; status 1 (fewer pairs
     than requested) warns only when verbose >= 1.
     """
-    device = isinstance(A, DeviceOperator)
+    if device and not isinstance(A, (DeviceOperator, SparseDeviceOperator)):
+        # scipy matrices (and a SuperLU or sparse matrix as preconditioner)
+        # moved to the GPU as a general sparse operator
+        if isinstance(A, np.ndarray):
+            A = sp.csr_matrix(A)
+        if not sp.issparse(A):
+            raise ValueError("device=True needs A as a sparse matrix or ndarray")
+        if B is not None and not (sp.issparse(B) or isinstance(B, np.ndarray)):
+            raise ValueError("device=True needs B as a sparse matrix, ndarray or None")
+        A = SparseDeviceOperator(A, B, preconditioner)
+        B = None
+        preconditioner = True if A.has_precond else None
+    device = isinstance(A, (DeviceOperator, SparseDeviceOperator))
     if which not in _TARGET_MAP:
         raise ValueError(f"which must be one of {list(_TARGET_MAP.keys())}; got {which!r}")
     sigma = complex(sigma)
@@ -115,9 +131,9 @@ def jdqz(A, B=None, k: int = 6, sigma: complex = 0.0, which: str = 'nearest', to
     if device:
         n = A.n
         if B is not None and B is not A:
-            raise ValueError("with a DeviceOperator A, B comes from the same pencil (pass B=None)")
+            raise ValueError("with a device operator A, B comes from the same pencil (pass B=None)")
         if preconditioner is not None and preconditioner is not True and preconditioner is not A:
-            raise ValueError("with a DeviceOperator A, preconditioner must be None, True or A")
+            raise ValueError("with a device operator A, preconditioner must be None, True or A")
         v0 = _start_vector(v0, n)
         out = _bbjd_c.jdqz_c(n, k, sigma, target, None, None,
                              None if preconditioner is None else True, tol=tol, maxiter=maxiter,

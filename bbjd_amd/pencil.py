@@ -114,6 +114,22 @@ class DeviceOperator:
     def device_bytes(self):
         return _lib.load().bbjd_op_bytes(self.handle)
 
+    def device_callbacks(self, precision, with_precond):
+        """the engine's callback slots this operator fills (C function names):
+        the three callbacks, the fused A/B and residual passes, both
+        preconditioner solves in one pass, and the complex64 forms for a
+        mixed inner solve when the operator keeps complex64 copies"""
+        cb = {"matvec_A": "bbjd_op_matvec_A", "matvec_B": "bbjd_op_matvec_B",
+              "matvec_AB": "bbjd_op_matvec_AB", "matvec_ABr": "bbjd_op_matvec_ABr"}
+        if with_precond:
+            cb.update(precond="bbjd_op_precond", precond2="bbjd_op_precond2")
+        if precision == "mixed" and self.c64:
+            cb.update(matvec_A_c64="bbjd_op_matvec_A_c64", matvec_B_c64="bbjd_op_matvec_B_c64",
+                      matvec_AB_c64="bbjd_op_matvec_AB_c64")
+            if with_precond:
+                cb["precond_c64"] = "bbjd_op_precond_c64"
+        return cb
+
     def free(self):
         if getattr(self, "handle", None):
             _lib.load().bbjd_op_free(self.handle)

@@ -38,7 +38,7 @@ def scipy_case(A, B, pc_matrix, **kw):
     B = sp.csc_matrix(B)
     lu = splu(sp.csc_matrix(pc_matrix)) if pc_matrix is not None else None
     return dict(matvec_A=lambda x: A @ x, matvec_B=lambda x: B @ x,
-                precond=(lambda x: lu.solve(x)) if lu is not None else None, **kw)
+                precond=(lambda x: lu.solve(x)) if lu is not None else None, _mats=(A, B, lu), **kw)
 
 
 def cases():
@@ -132,6 +132,7 @@ def cases():
 def run_case(builder, which):
     kw = builder()
     kw.pop("_keep", None)
+    kw.pop("_mats", None)
     pen = kw.pop("_pencil", None)
     r = orc.solve(which, **kw)
     rec = {k: kw[k] for k in ("n", "nev", "maxsub", "minsub", "maxiter", "inner_maxiter", "tol",

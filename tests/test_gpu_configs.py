@@ -58,6 +58,7 @@ def _run_golden(name, precision="fp64"):
     kw = CASES[name]()
     keep = kw.pop("_keep", None)  # noqa: F841 -- keeps the CPU operators alive
     pen = kw.pop("_pencil", None)
+    kw.pop("_mats", None)
     tol = kw.get("tol", 1e-10)
     common = dict(tol=tol, maxiter=kw.get("maxiter", 200), maxsub=kw.get("maxsub", 0),
                   minsub=kw.get("minsub", 0), inner_maxiter=kw.get("inner_maxiter", 0),

@@ -29,6 +29,7 @@ def test_oracle_reproduces_reference_fixture(name):
     kw = CASES[name]()
     kw.pop("_keep", None)
     kw.pop("_pencil", None)
+    kw.pop("_mats", None)
     r = orc.solve("oracle", **kw)
     g = GOLDEN[name]
     assert r["status"] == g["status"]
@@ -46,6 +47,7 @@ def test_fixture_is_current_with_the_reference_build(name):
     kw = CASES[name]()
     kw.pop("_keep", None)
     kw.pop("_pencil", None)
+    kw.pop("_mats", None)
     r = orc.solve("ref", **kw)
     g = GOLDEN[name]
     assert (r["nconv"], r["iters"], r["status"]) == (g["nconv"], g["iters"], g["status"])
