@@ -1049,6 +1049,97 @@ __global__ __launch_bounds__(TPB) void k_rotate_mfma(i64 n, int k, int keep, Cx<
     }
 }
 
+// Streaming form of the rotation on the FP64 matrix cores: a block owns 64
+// rows and all keep output columns; wave w owns the output's rows
+// 16w .. 16w+15 against every 16-column block (NT of them), so its A operand
+// is loaded once per 4-column step and reused NT times.  The k input columns
+// (and the matching rows of Z) stream through two LDS buffers in chunks of
+// KC: the next chunk's global loads are in flight while the MFMAs consume
+// the current one.  Outputs stay in registers until every input column of
+// the block's rows has been read, then go straight to X (in place); each
+// 16 x 16 block is stored as four consecutive rows per column (64 bytes).
+// LDS: 2 (64 KC + KC 16 NT) * 16 bytes, independent of k.
+template <int NT, int KC>
+__global__ __launch_bounds__(TPB) void k_rotate_mfma_s(i64 n, int k, int keep, Cx<double> *X, i64 ldx,
+                                                       const Cx<double> *__restrict__ Zr)
+{
+    constexpr int RT = 64, KP = 16 * NT;                 // rows per block, padded output columns
+    constexpr int XE = RT * KC / TPB, ZE = KC * KP / TPB;  // staged elements per thread
+    static_assert(XE >= 1 && ZE >= 1 && RT * KC % TPB == 0 && KC * KP % TPB == 0, "tile shape");
+    __shared__ Cx<double> xs[2][KC * RT];                // column j of the chunk, row r at j RT + r
+    __shared__ Cx<double> zs[2][KC * KP];                // row j of the chunk's Z, column c at j KP + c
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int ml = lane & 15, kq = lane >> 4;
+    const i64 rb = (i64)blockIdx.x * RT;
+    const int nch = (k + KC - 1) / KC;
+    Cx<double> px[XE], pz[ZE];
+    auto load = [&](int c) {
+        const int j0 = c * KC;
+#pragma unroll
+        for (int e = 0; e < XE; e++) {
+            const int idx = t + e * TPB, jj = idx / RT, rr = idx % RT;
+            const int j = j0 + jj;
+            const i64 r = rb + rr;
+            const bool ok = j < k && r < n;
+            Cx<double> v = X[ok ? (i64)j * ldx + r : 0];
+            px[e] = ok ? v : cx(0.0, 0.0);
+        }
+#pragma unroll
+        for (int e = 0; e < ZE; e++) {
+            const int idx = t + e * TPB, jj = idx / KP, cc = idx % KP;
+            const int j = j0 + jj;
+            const bool ok = j < k && cc < keep;
+            Cx<double> v = Zr[ok ? (i64)j * keep + cc : 0];
+            pz[e] = ok ? v : cx(0.0, 0.0);
+        }
+    };
+    auto stage = [&](int b) {
+#pragma unroll
+        for (int e = 0; e < XE; e++) xs[b][t + e * TPB] = px[e];
+#pragma unroll
+        for (int e = 0; e < ZE; e++) zs[b][t + e * TPB] = pz[e];
+    };
+    v4d acr[NT], aci[NT];
+#pragma unroll
+    for (int b = 0; b < NT; b++) {
+        acr[b] = v4d{0, 0, 0, 0};
+        aci[b] = v4d{0, 0, 0, 0};
+    }
+    load(0);
+    stage(0);
+    __syncthreads();
+    for (int c = 0; c < nch; c++) {
+        const int cur = c & 1;
+        if (c + 1 < nch) load(c + 1);                    // in flight during the MFMAs below
+#pragma unroll
+        for (int kk = 0; kk < KC; kk += 4) {
+            const Cx<double> a = xs[cur][(kk + kq) * RT + w * 16 + ml];
+#pragma unroll
+            for (int b = 0; b < NT; b++) {
+                const Cx<double> z = zs[cur][(kk + kq) * KP + b * 16 + ml];
+                acr[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.re, z.re, acr[b], 0, 0, 0);
+                acr[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.im, z.im, acr[b], 0, 0, 0);
+                aci[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.re, z.im, aci[b], 0, 0, 0);
+                aci[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.im, z.re, aci[b], 0, 0, 0);
+            }
+        }
+        // the other buffer's last readers finished before the previous barrier
+        if (c + 1 < nch) stage(cur ^ 1);
+        __syncthreads();
+    }
+    // D(m, n) in lane n + 16 (m % 4), element m / 4: rows 16 w + 4 e + kq
+#pragma unroll
+    for (int b = 0; b < NT; b++) {
+        const int col = b * 16 + ml;
+        if (col < keep)
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const i64 r = rb + w * 16 + 4 * e + kq;
+                if (r < n) X[(i64)col * ldx + r] = cx(acr[b][e], aci[b][e]);
+            }
+    }
+}
+
 // Gram block G = X^H Y of two tall-skinny complex matrices (X n x p, Y n x q)
 // on the FP64 matrix cores.  A block owns a 64 x 64 tile of G and a chunk of
 // rows; the chunk is staged through LDS in slabs of GR_SR rows (coalesced
@@ -1409,6 +1500,11 @@ static int norm_into(hipStream_t s, i64 n, const void *x, bbjd_red *rd, int slot
 static int g_gram_blocks = 512;
 // In-place rotation on the matrix cores (1) or on vector FP64 (0).
 static int g_rotate_mfma = 1;
+// Input columns per LDS chunk of the streaming rotation (0: per output
+// width: 16 up to 32 output columns, else 8, which keeps more blocks
+// resident; measured at n = 2e5, k = 128 -> 64: 287 us at 8, 299 at 16,
+// 334 at 32); tunable.
+static int g_rot_kc = 0;
 
 // Fused CGS2 (k <= 128): on by default, switchable for tests and tuning
 // (2: on, with the second pass's normalisation always by the computed norm).
@@ -1826,7 +1922,8 @@ void bbjd_set_svdv_fused(int on) { g_svdv_fused = on != 0; }
 
 void bbjd_set_gram_blocks(int blocks) { if (blocks > 0) g_gram_blocks = blocks; }
 
-void bbjd_set_rotate_mfma(int on) { g_rotate_mfma = on != 0; }
+void bbjd_set_rotate_mfma(int on) { g_rotate_mfma = on < 0 ? 0 : (on > 2 ? 2 : on); }
+void bbjd_set_rotate_chunk(int kc) { g_rot_kc = (kc == 8 || kc == 16 || kc == 32) ? kc : 0; }
 
 // G (p x q, column-major, ld ldg) = X^H Y; partials in rd->part.
 int bbjd_k_gram(hipStream_t s, int64_t n, int p, const void *X, int64_t ldx, int q, const void *Y, int64_t ldy,
@@ -1856,8 +1953,29 @@ int bbjd_k_rotate(hipStream_t s, int64_t n, int k, int keep, void *X, int64_t ld
 {
     if (keep > k || keep <= 0) return -2;
     const int k4 = (k + 3) & ~3;
-    // matrix cores: 32 rows per block while the staged rows fit 64 KB of LDS
-    // and the output has at most 16 blocks of 16 x 16, else 16 rows
+    // matrix cores, streaming form (any k, keep <= 128): 64-row blocks, the
+    // input columns through double-buffered LDS chunks
+    if (g_rotate_mfma == 1 && keep <= 128) {
+        const dim3 g((unsigned)((n + 63) / 64));
+        const int nt = (keep + 15) / 16;
+#define BBJD_ROT_S(NT, KC)                                                                                       \
+    hipLaunchKernelGGL((k_rotate_mfma_s<NT, KC>), g, dim3(TPB), 0, s, (i64)n, k, keep, (Cx<double> *)X, (i64)ldx, \
+                       (const Cx<double> *)Zr)
+        const int kc = g_rot_kc;
+        if (nt <= 1) BBJD_ROT_S(1, 16);
+        else if (nt <= 2) BBJD_ROT_S(2, 16);
+        else if (nt <= 4) {
+            if (kc == 16) BBJD_ROT_S(4, 16);
+            else if (kc == 32) BBJD_ROT_S(4, 32);
+            else BBJD_ROT_S(4, 8);
+        } else if (nt <= 6) BBJD_ROT_S(6, 8);
+        else if (kc == 16) BBJD_ROT_S(8, 16);
+        else BBJD_ROT_S(8, 8);
+#undef BBJD_ROT_S
+        return chk();
+    }
+    // matrix cores, staged form: 32 rows per block while the staged rows fit
+    // 64 KB of LDS and the output has at most 16 blocks of 16 x 16, else 16
     if (g_rotate_mfma && (size_t)32 * k4 * 16 <= 64 * 1024 && 2 * ((keep + 15) / 16) <= 16) {
         hipLaunchKernelGGL((k_rotate_mfma<32>), dim3((unsigned)((n + 31) / 32)), dim3(TPB), (size_t)32 * k4 * 16, s,
                            (i64)n, k, keep, (Cx<double> *)X, (i64)ldx, (const Cx<double> *)Zr);

@@ -3,7 +3,7 @@
     python -m bbjd_amd.kbench KERNEL N K [--reps R]
 
 KERNEL: proj (Q^H x), proj2 (Q^H [x y]), update (x -= Q h), rotate (in-place
-basis rotation k -> k/2), cgs2 (one CGS2 with the second pass skipped on the
+basis rotation k -> k/2, 8 N K (K/2) flops; --rotate-mode picks the kernel), cgs2 (one CGS2 with the second pass skipped on the
 device), copy (plain copy of N complex values, the reference for
 counter-measured HBM traffic), projout (the two-pass projection v -= Q Q^H v,
 both passes always: for k <= 128 the fused form, first update and second
@@ -79,7 +79,12 @@ def main():
     ap.add_argument("--reps", type=int, default=50)
     ap.add_argument("--proj-blocks", type=int, default=0, help="blocks per projection launch, 0 = default")
     ap.add_argument("--upd-rows", type=int, default=0, help="rows per block, 0 = chosen from the shape")
+    ap.add_argument("--rotate-mode", type=int, default=1,
+                    help="rotate: 1 streaming matrix-core form, 2 staged matrix-core form, 0 vector FP64")
+    ap.add_argument("--rotate-chunk", type=int, default=0, help="rotate: input columns per LDS chunk (8, 16, 32)")
     a = ap.parse_args()
+    _lib.load().bbjd_set_rotate_mfma(a.rotate_mode)
+    _lib.load().bbjd_set_rotate_chunk(a.rotate_chunk)
     if a.kernel == "bcr":
         ms = run_bcr(a.n, a.k, a.reps)
         print(f"bcr n={a.n} rhs={a.k}: {ms * 1e3:.2f} us/solve", flush=True)
@@ -88,6 +93,9 @@ def main():
     extra = ""
     if a.kernel == "gram":
         extra = f", {8.0 * a.n * a.k * a.k / (ms * 1e-3) / 1e12:.2f} TFLOP/s"
+    elif a.kernel == "rotate":
+        keep = max(a.k // 2, 1)
+        extra = f", {8.0 * a.n * a.k * keep / (ms * 1e-3) / 1e12:.2f} TFLOP/s"
     print(f"{a.kernel} n={a.n} k={a.k}: {ms * 1e3:.2f} us/launch, {by / 1e6:.3f} MB/launch, "
           f"{by / (ms * 1e-3) / 1e9:.1f} GB/s{extra}", flush=True)
 

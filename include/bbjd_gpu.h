@@ -196,9 +196,15 @@ int    bbjd_project_out_host(bbjd_int n, bbjd_int k, const bbjd_zomplex *Q, bbjd
  * row chunks x 64 x 64 output tiles, default 512.  For tuning. */
 void bbjd_set_gram_blocks(int blocks);
 
-/* 1 (default): the restart's in-place basis rotation runs on the FP64 matrix
- * cores when its rows fit the LDS tile; 0: on vector FP64.  For tests. */
+/* The restart's in-place basis rotation: 1 (default) on the FP64 matrix
+ * cores in the streaming form (input columns through LDS in chunks, any k,
+ * keep <= 128; the staged form beyond); 2: the staged form (all k input
+ * columns of a row tile in LDS, when they fit); 0: on vector FP64.  For
+ * tests and tuning. */
 void bbjd_set_rotate_mfma(int on);
+/* Input columns per LDS chunk of the streaming rotation: 8, 16 or 32 (0:
+ * chosen from the output width).  For tuning. */
+void bbjd_set_rotate_chunk(int kc);
 
 /* Test entry: X[:, :keep] = X[:, :k] Z in place for host arrays X (n x k,
  * leading dimension n) and Z (k x keep, column-major) through the engine's

@@ -527,18 +527,20 @@ def test_block_gram_matches_numpy(n, p, q):
 
 
 @pytest.mark.parametrize("n,k,keep", [(5000, 64, 32), (70001, 128, 64), (3001, 200, 80), (1000, 9, 3),
-                                       (50, 300, 250)])
+                                       (50, 300, 250), (20011, 256, 128), (4099, 100, 96), (777, 17, 17)])
 def test_block_rotate_matches_numpy(n, k, keep):
-    """the restart's in-place rotation X[:, :keep] = X Z: matrix cores with 32
-    or 16 rows per block (k = 200 exceeds the 32-row tile) and the vector
-    kernel (k = 300, and with the matrix cores switched off), against numpy"""
+    """the restart's in-place rotation X[:, :keep] = X Z against numpy: the
+    streaming matrix-core form (mode 1: input columns through LDS in chunks,
+    every output width up to 128 and ragged row tiles), the staged
+    matrix-core form (mode 2: 32 or 16 rows per block; k = 200 exceeds the
+    32-row tile) and the vector kernel (k = 300, and mode 0)"""
     lib = _lib.load()
     rng = np.random.default_rng(n + k + keep)
     X = np.asfortranarray(rng.standard_normal((n, k)) + 1j * rng.standard_normal((n, k)))
     Z = np.asfortranarray(rng.standard_normal((k, keep)) + 1j * rng.standard_normal((k, keep)))
     ref = X @ Z
     try:
-        for mfma in (1, 0):
+        for mfma in (1, 2, 0):
             lib.bbjd_set_rotate_mfma(mfma)
             Xg = X.copy(order="F")
             assert lib.bbjd_block_rotate(n, k, keep, Xg.ctypes.data, Z.ctypes.data) == 0
