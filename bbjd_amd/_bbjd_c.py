@@ -14,8 +14,9 @@ copies n elements unchecked, src/_bbjd_c.c:68).
 Extension (keyword-only): ``device_op`` -- a ``DeviceOperator`` or a
 ``SparseDeviceOperator`` whose device-pointer callbacks replace the three
 Python callables (no host round trip per apply); ``precision`` -- 'fp64' or 'mixed' (complex64 inner GMRES);
-``qz_mode`` -- 'ritz' (ordered Schur form only where a restart may follow)
-or 'schur' (every iteration, the reference's sequence); ``graphs`` -- with
+``qz_mode`` -- 'ritz' (ordered Schur form only where a restart may follow),
+'schur' (every iteration, the reference's sequence) or 'device' (the
+projected step on the GPU for k <= 64); ``graphs`` -- with
 ``device_op``, capture the inner GMRES steps into HIP graphs and replay
 them (same kernels, same results).
 """
@@ -74,9 +75,9 @@ def jdqz_c(n, nev, sigma, target, matvec_A, matvec_B, precond_solve, tol=1e-10, 
     if precision not in ("fp64", "mixed"):
         raise ValueError("precision must be 'fp64' or 'mixed'")
     opts.precision = _lib.PREC_MIXED if precision == "mixed" else _lib.PREC_FP64
-    if qz_mode not in ("ritz", "schur"):
-        raise ValueError("qz_mode must be 'ritz' or 'schur'")
-    opts.qz_mode = _lib.QZ_SCHUR if qz_mode == "schur" else _lib.QZ_RITZ
+    if qz_mode not in ("ritz", "schur", "device"):
+        raise ValueError("qz_mode must be 'ritz', 'schur' or 'device'")
+    opts.qz_mode = {"ritz": _lib.QZ_RITZ, "schur": _lib.QZ_SCHUR, "device": _lib.QZ_DEVICE}[qz_mode]
     opts.graphs = int(bool(graphs))
     errbox: list = []
     keep = []

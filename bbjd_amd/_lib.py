@@ -38,7 +38,7 @@ class Result(C.Structure):
                 ("residuals", C.c_void_p), ("iters", C.c_int64), ("status", C.c_int)]
 
 
-QZ_RITZ, QZ_SCHUR = 0, 1
+QZ_RITZ, QZ_SCHUR, QZ_DEVICE = 0, 1, 2
 
 
 class GpuOptions(C.Structure):
@@ -54,7 +54,9 @@ class Stats(C.Structure):
                 ("inner_steps", C.c_int64), ("dgks_second_passes", C.c_int64),
                 ("sum_k", C.c_int64), ("qz_seconds", C.c_double), ("solve_seconds", C.c_double),
                 ("stream_bytes", C.c_double), ("restart_fallbacks", C.c_int64),
-                ("graph_captures", C.c_int64), ("graph_launches", C.c_int64), ("qz_schur_ties", C.c_int64)]
+                ("graph_captures", C.c_int64), ("graph_launches", C.c_int64), ("qz_schur_ties", C.c_int64),
+                ("qz_device_steps", C.c_int64), ("qz_device_fallbacks", C.c_int64),
+                ("qz_device_accuracy", C.c_int64)]
 
 
 class PencilDesc(C.Structure):

@@ -64,6 +64,12 @@ struct bbjd_dev {
     struct gkey { bbjd_int ndefl, n, ld; const void *r, *z, *qr, *ql, *op; } gkey;
     double          *gco;              /* device: theta_b, -theta_a         */
     double          *hgco;             /* pinned: their staging             */
+    /* the projected step on the device (qz_mode BBJD_QZ_DEVICE): MA, MB,
+     * the converged values and the outputs in HBM, the problem descriptor
+     * and the outputs' staging pinned */
+    void            *qzd;
+    char            *hqzd;
+    int              zr_ready;          /* d->zr holds the restart's rotation  */
 };
 
 static __thread hipStream_t tl_cb_stream;
@@ -84,7 +90,7 @@ static double now_s(void)
 typedef struct {
     bbjd_int n, ld, maxsub, nev, ims;
     int mixed;
-    size_t off[32];
+    size_t off[48];
     size_t total;
 } layout_t;
 
@@ -92,7 +98,8 @@ typedef struct {
  * come back to the host in one copy */
 enum { O_V, O_W, O_AV, O_BV, O_QR, O_QL, O_GV, O_T1, O_T2, O_T3, O_T4, O_HW, O_PART, O_PNORM, O_H, O_H2,
        O_HSUM, O_PEN, O_SCAL, O_FLAG, O_CNT, O_COEF, O_COEF2, O_ZR, O_GRAM, O_GVC, O_QRC, O_QLC, O_TC0, O_TC1,
-       O_TC2, O_GCO, O_END };
+       O_TC2, O_GCO, O_QZD, O_END };
+_Static_assert(O_END <= 48, "layout_t.off holds every sub-buffer");
 
 static bbjd_int auto_maxsub(const bbjd_params *p)
 {
@@ -100,6 +107,16 @@ static bbjd_int auto_maxsub(const bbjd_params *p)
     if (ms < 40) ms = 40;
     return p->maxsub > 0 ? p->maxsub : (ms < p->n ? ms : p->n);
 }
+
+/* device region of the projected step: MA, MB (maxsub^2 each), the
+ * converged values (nev + 1), alpha, beta, x (maxsub each), 4 ints and the
+ * problem descriptor */
+static size_t qzd_bytes(size_t ms, size_t nev)
+{
+    return (2 * ms * ms + nev + 1 + 3 * ms) * 16 + 512 + (sizeof(bbjd_qz_problem) + 255) / 256 * 256;
+}
+/* pinned staging of the same: descriptor, info, alpha, beta, x */
+static size_t hqzd_bytes(size_t ms) { return 512 + (sizeof(bbjd_qz_problem) + 255) / 256 * 256 + 3 * ms * 16; }
 
 static void make_layout(layout_t *L, const bbjd_params *p, const bbjd_gpu_options *o)
 {
@@ -132,6 +149,7 @@ static void make_layout(layout_t *L, const bbjd_params *p, const bbjd_gpu_option
     sz[O_PEN] = 4 * (ms + 1) * Z;
     sz[O_GRAM] = 2 * ms * ms * Z;
     sz[O_GCO] = 64;
+    sz[O_QZD] = qzd_bytes(ms, (size_t)L->nev);
     if (L->mixed) {
         sz[O_GVC] = ld * (size_t)(L->ims + 1) * C;
         sz[O_QRC] = sz[O_QLC] = ld * (size_t)(L->nev + 1) * C;
@@ -216,6 +234,7 @@ static void workspace_free_impl(bbjd_workspace *ws)
         if (d->stream) (void)hipStreamSynchronize(d->stream);   /* copies into pinned buffers */
         if (d->gx) { graphs_reset(d); free(d->gx); }
         if (d->hgco) hipHostFree(d->hgco);
+        if (d->hqzd) hipHostFree(d->hqzd);
         if (d->dblock) hipFree(d->dblock);
         if (d->hscal) hipHostFree(d->hscal);
         if (d->hsmall) hipHostFree(d->hsmall);
@@ -290,7 +309,7 @@ static bbjd_workspace *workspace_alloc_impl(const bbjd_params *p, const bbjd_gpu
         if (maxk < L.nev + 1) maxk = (int)L.nev + 1;
         d->rd.part_cap = bbjd_red_part_entries(L.n, maxk + 2);
     }
-    d->zr = AT(O_ZR); d->pencil = AT(O_PEN); d->gram = AT(O_GRAM); d->gco = AT(O_GCO);
+    d->zr = AT(O_ZR); d->pencil = AT(O_PEN); d->gram = AT(O_GRAM); d->gco = AT(O_GCO); d->qzd = AT(O_QZD);
     if (L.mixed) {
         d->gvc = AT(O_GVC); d->qrc = AT(O_QRC); d->qlc = AT(O_QLC);
         d->tc[0] = AT(O_TC0); d->tc[1] = AT(O_TC1); d->tc[2] = AT(O_TC2);
@@ -326,6 +345,7 @@ static bbjd_workspace *workspace_alloc_impl(const bbjd_params *p, const bbjd_gpu
     }
     if (!HIPOK(hipHostMalloc((void **)&d->hgram, 2 * ss * sizeof(zcx), 0))) { d->hgram = NULL; goto fail; }
     if (!HIPOK(hipHostMalloc((void **)&d->hgco, 64, 0))) { d->hgco = NULL; goto fail; }
+    if (!HIPOK(hipHostMalloc((void **)&d->hqzd, hqzd_bytes((size_t)L.maxsub), 0))) { d->hqzd = NULL; goto fail; }
     d->gx_cap = (int)L.ims + 1;
     if (!(d->gx = calloc((size_t)d->gx_cap, sizeof(hipGraphExec_t)))) goto fail;
     d->pair_res = calloc((size_t)L.nev + 1, sizeof(double));
@@ -901,9 +921,12 @@ static int thick_restart(ctx_t *c, bbjd_int k, bbjd_int keep, zcx nu0, zcx mu0)
     const bbjd_int ms = c->ldm, n = c->n;
     const zcx *ZB = (const zcx *)ws->ZB;
     int rc = 0;
-    for (bbjd_int j = 0; j < k; j++)
-        for (bbjd_int cc = 0; cc < keep; cc++) d->zrh[(size_t)j * keep + cc] = ZB[j + cc * ms];
-    rc |= !HIPOK(hipMemcpyAsync(d->zr, d->zrh, (size_t)k * keep * 16, hipMemcpyHostToDevice, c->s));
+    if (!d->zr_ready) {        /* else the device step wrote it (Schur vectors of C^-1 MB) */
+        for (bbjd_int j = 0; j < k; j++)
+            for (bbjd_int cc = 0; cc < keep; cc++) d->zrh[(size_t)j * keep + cc] = ZB[j + cc * ms];
+        rc |= !HIPOK(hipMemcpyAsync(d->zr, d->zrh, (size_t)k * keep * 16, hipMemcpyHostToDevice, c->s));
+    }
+    d->zr_ready = 0;
     rc |= bbjd_k_rotate(c->s, n, (int)k, (int)keep, ws->V, c->ld, d->zr);
     rc |= bbjd_k_rotate(c->s, n, (int)k, (int)keep, ws->AV, c->ld, d->zr);
     rc |= bbjd_k_rotate(c->s, n, (int)k, (int)keep, ws->BV, c->ld, d->zr);
@@ -1049,6 +1072,67 @@ static int expand(ctx_t *c, bbjd_int k, const void *z, zcx nu0, zcx mu0)
     return 0;
 }
 
+/* The projected step on the device (qz_mode BBJD_QZ_DEVICE): the ordered
+ * Schur form of C^-1 MB, the reference's selection and target, the Ritz
+ * coefficients in d->xv and, with keep > 0 (a restart follows), the leading
+ * keep Schur vectors as the restart's rotation in d->zr (bbjd_qz_device_batch,
+ * one problem on the solve's stream).  Returns 0 with *p, *ta, *tb set; 1
+ * when the host step is to decide this iteration (k > 64, nu0 = 0, a w from
+ * the Av fallback since the last restart, or a status other than OK from the
+ * device: C not triangular, accuracy, a tie); -1 on a device error. */
+static int qz_device(ctx_t *c, bbjd_int k, const zcx *MA, const zcx *MB, const zcx *conv, bbjd_int nconv,
+                     bbjd_int nkeep, bbjd_int keep, bbjd_int *p, zcx *ta, zcx *tb)
+{
+    struct bbjd_dev *d = c->d;
+    const bbjd_int ldm = c->ldm, msw = c->ws->maxsub, nevw = c->ws->nev;
+    if (k > 64 || cabs(c->nu0) == 0 || c->w_mixed) return 1;
+    char *db = (char *)d->qzd;
+    const size_t mb = (size_t)msw * msw * 16, cb = (size_t)k * ldm * 16;
+    const size_t o_conv = 2 * mb, o_al = o_conv + (size_t)(nevw + 1) * 16, o_be = o_al + (size_t)msw * 16,
+                 o_x = o_be + (size_t)msw * 16, o_info = o_x + (size_t)msw * 16,
+                 o_prob = (o_info + 256 + 255) / 256 * 256;
+    int *hinfo = (int *)d->hqzd;
+    bbjd_qz_problem *hp = (bbjd_qz_problem *)(d->hqzd + 256);
+    zcx *hal = (zcx *)(d->hqzd + 256 + (sizeof(bbjd_qz_problem) + 255) / 256 * 256), *hbe = hal + msw, *hx = hbe + msw;
+    zcx *stg = d->hgram;
+    memcpy(stg, MA, cb);
+    memcpy((char *)stg + cb, MB, cb);
+    if (nconv > 0) memcpy(d->hsmall, conv, (size_t)nconv * 16);
+    memset(hp, 0, sizeof(*hp));
+    hp->k = (int)k; hp->ldm = (int)ldm;
+    hp->MA = (const bbjd_zomplex *)db; hp->MB = (const bbjd_zomplex *)(db + mb);
+    hp->nu0_re = creal(c->nu0); hp->nu0_im = cimag(c->nu0); hp->mu0_re = creal(c->mu0); hp->mu0_im = cimag(c->mu0);
+    hp->sigma_re = creal(c->P->sigma); hp->sigma_im = cimag(c->P->sigma);
+    hp->target = (int)c->P->target; hp->nkeep = (int)nkeep; hp->nconv = (int)nconv; hp->keep = (int)keep;
+    hp->conv = (const bbjd_zomplex *)(db + o_conv);
+    hp->tol = c->P->tol;
+    hp->alpha = (bbjd_zomplex *)(db + o_al); hp->beta = (bbjd_zomplex *)(db + o_be); hp->x = (bbjd_zomplex *)(db + o_x);
+    hp->zr = keep > 0 ? (bbjd_zomplex *)d->zr : NULL;
+    hp->info = (int *)(db + o_info);
+    int rc = 0;
+    rc |= !HIPOK(hipMemcpyAsync(db, stg, cb, hipMemcpyHostToDevice, c->s));
+    rc |= !HIPOK(hipMemcpyAsync(db + mb, (char *)stg + cb, cb, hipMemcpyHostToDevice, c->s));
+    if (nconv > 0) rc |= !HIPOK(hipMemcpyAsync(db + o_conv, d->hsmall, (size_t)nconv * 16, hipMemcpyHostToDevice, c->s));
+    rc |= !HIPOK(hipMemcpyAsync(db + o_prob, hp, sizeof(*hp), hipMemcpyHostToDevice, c->s));
+    if (!rc) rc |= bbjd_qz_device_batch((const bbjd_qz_problem *)(db + o_prob), 1, c->s) ? 1 : 0;
+    rc |= !HIPOK(hipMemcpyAsync(hinfo, db + o_info, 4 * sizeof(int), hipMemcpyDeviceToHost, c->s));
+    rc |= !HIPOK(hipMemcpyAsync(hal, db + o_al, 3 * (size_t)msw * 16, hipMemcpyDeviceToHost, c->s));
+    rc |= !HIPOK(hipStreamSynchronize(c->s));
+    if (rc) return -1;
+    d->st.qz_device_steps++;
+    if (hinfo[0] != BBJD_QZD_OK || hinfo[1] < 0 || hinfo[1] >= k) {
+        d->st.qz_device_fallbacks++;
+        d->st.qz_device_accuracy += hinfo[0] == BBJD_QZD_ACCURACY;
+        return 1;
+    }
+    *p = hinfo[1];
+    *ta = hal[*p];
+    *tb = hbe[*p];
+    memcpy(d->xv, hx, (size_t)k * sizeof(zcx));
+    d->zr_ready = keep > 0;
+    return 0;
+}
+
 static bbjd_status solve_impl(const bbjd_params *P, bbjd_workspace *ws, bbjd_result *R, bbjd_int ms_solve)
 {
     struct bbjd_dev *d = ws->dev;
@@ -1131,11 +1215,25 @@ static bbjd_status solve_impl(const bbjd_params *P, bbjd_workspace *ws, bbjd_res
         /* the ordered Schur form where a thick restart may follow (k >= ms),
          * else the eigenvalues and the target Ritz vector only */
         int schur = d->opt.qz_mode == BBJD_QZ_SCHUR || k >= ms;
-        zcx ta, tb;
+        zcx ta = 0.0, tb = 1.0;
+        d->zr_ready = 0;
         {
             const double tq = now_s();
             int qrc = 0;
-            if (!schur) {
+            int on_device = 0;
+            if (d->opt.qz_mode == BBJD_QZ_DEVICE) {
+                bbjd_int nkeep = minsub - locked, keep = 0;
+                if (nkeep < 1) nkeep = 1;
+                if (k >= ms) {
+                    keep = minsub < k ? minsub : k - 1;
+                    if (keep < locked + 1) keep = locked + 1;
+                }
+                const int q = qz_device(c, k, MA, MB, eval, R->nconv, nkeep, keep, &p, &ta, &tb);
+                if (q < 0) { st = BBJD_ERR_ALLOC; goto done; }
+                on_device = q == 0;
+                if (on_device) schur = 0;
+            }
+            if (!on_device && !schur) {
                 qrc = bbjd_qz_target(&d->qz, k, MA, MB, nu0, mu0, al, be, sigma, P->target, eval, R->nconv, tol,
                                      &p);
                 if (qrc == 2) {             /* ambiguous target: the reference's sequence decides */
@@ -1148,7 +1246,7 @@ static bbjd_status solve_impl(const bbjd_params *P, bbjd_workspace *ws, bbjd_res
                     qrc = bbjd_ritz_vector(&d->qz, k, MA, MB, ta, tb, d->xv);
                 }
             }
-            if (schur) {
+            if (!on_device && schur) {
                 bbjd_int nkeep = minsub - locked;
                 if (nkeep < 1) nkeep = 1;
                 qrc = bbjd_qz_solve(&d->qz, k, MA, MB, (zcx *)ws->SA, (zcx *)ws->SB, (zcx *)ws->ZA,

@@ -93,7 +93,11 @@ typedef struct {
     bbjd_residual_fn    matvec_ABr;
 } bbjd_gpu_options;
 
-enum { BBJD_QZ_RITZ = 0, BBJD_QZ_SCHUR = 1 };
+/* BBJD_QZ_DEVICE: the projected step on the GPU (bbjd_qz_device_batch, one
+ * wavefront) for k <= 64 with the nearest-sigma or smallest-magnitude test
+ * space, the host step (as BBJD_QZ_RITZ) where it does not apply or flags
+ * the iteration. */
+enum { BBJD_QZ_RITZ = 0, BBJD_QZ_SCHUR = 1, BBJD_QZ_DEVICE = 2 };
 
 void bbjd_gpu_options_init(bbjd_gpu_options *o);
 
@@ -143,6 +147,10 @@ typedef struct {
                                    was a tie up to rounding (or had no
                                    unconverged candidate), decided by the
                                    ordered Schur form as the reference does */
+    bbjd_int qz_device_steps;   /* projected steps run on the device         */
+    bbjd_int qz_device_fallbacks; /* of which flagged back to the host      */
+    bbjd_int qz_device_accuracy;  /* of those, flagged for a theta whose
+                                     error bound exceeded 10 tol            */
 } bbjd_stats;
 
 int bbjd_workspace_stats(const bbjd_workspace *ws, bbjd_stats *st);
@@ -276,6 +284,50 @@ int bbjd_set_tile_table(int nn, const int64_t *n_upper, int nk, const int *k_upp
  * other streams or threads) race; calls ordered on one stream do not. */
 int bbjd_block_orthogonalize_noalloc(bbjd_int n, bbjd_int k, const bbjd_zomplex *Q, bbjd_int ldq, bbjd_zomplex *v,
                                      bbjd_zomplex *h_work, double *norm_out, void *stream);
+
+/* The projected step on the device, batched (no counterpart in the
+ * reference, whose solve_projected runs ZGGES/ZTGSEN on the host,
+ * src/jd_core.c:319-462).  One 64-lane wavefront per problem, k <= 64, all
+ * data in LDS: with C = nu0 MA + mu0 MB upper triangular (the engine's
+ * harmonic test space), the ordered Schur form M = Z T Z^H of
+ * M = C^-1 MB, the reference's selection (values within 100 tol of a
+ * converged one, then the nkeep best by the target key) moved to the
+ * leading positions in order, the target (best unconverged value over all
+ * positions), alpha / beta = theta per position with
+ * alpha = 1 - mu0 t_ii, beta = nu0 t_ii, and the Ritz coefficients
+ * x = Z y of the target.  info[0]: BBJD_QZD_* status (anything but OK asks
+ * the caller to take the host step), info[1]: target position, info[2]:
+ * number selected, info[3]: QR iterations. */
+typedef struct {
+    int                 k, ldm;         /* size; leading dimension of MA, MB  */
+    const bbjd_zomplex *MA, *MB;        /* device, column-major               */
+    double              nu0_re, nu0_im, mu0_re, mu0_im, sigma_re, sigma_im;
+    int                 target, nkeep, nconv, keep;
+    const bbjd_zomplex *conv;           /* device, nconv converged values     */
+    double              tol;
+    bbjd_zomplex       *alpha, *beta;   /* device, k each, or NULL            */
+    bbjd_zomplex       *x;              /* device, k: Ritz coefficients       */
+    bbjd_zomplex       *zr;             /* device, k x keep row-major
+                                           (zr[j keep + c] = Z(j, c)) or NULL */
+    bbjd_zomplex       *zfull, *tfull;  /* device, k x k column-major, or NULL */
+    int                *info;           /* device, 4 ints                     */
+} bbjd_qz_problem;
+
+enum { BBJD_QZD_OK = 0, BBJD_QZD_AMBIGUOUS = 1, BBJD_QZD_NOT_TRIANGULAR = 2, BBJD_QZD_ACCURACY = 3,
+       BBJD_QZD_NO_CONVERGENCE = 4, BBJD_QZD_UNSUPPORTED = 5 };
+
+/* nprob problems (probs: a device array) in one launch on stream
+ * (hipStream_t, NULL: the null stream).  Returns 0 or -1. */
+int bbjd_qz_device_batch(const bbjd_qz_problem *probs, int nprob, void *stream);
+
+/* One problem on host arrays through bbjd_qz_device_batch (tests and
+ * tools): MA, MB k x k with leading dimension ldm; outputs host arrays of
+ * k (alpha, beta, x) and k x k column-major (z, t; any may be NULL), info
+ * 4 ints.  Returns 0 or -1 on a device error. */
+int bbjd_qz_device_host(bbjd_int k, const bbjd_zomplex *MA, const bbjd_zomplex *MB, bbjd_int ldm,
+                        bbjd_zomplex nu0, bbjd_zomplex mu0, bbjd_zomplex sigma, int target, bbjd_int nkeep,
+                        const bbjd_zomplex *conv, bbjd_int nconv, double tol, bbjd_zomplex *alpha,
+                        bbjd_zomplex *beta, bbjd_zomplex *x, bbjd_zomplex *z, bbjd_zomplex *t, int *info);
 
 /* Device memory helpers for callers without another GPU runtime (tests,
  * tools): allocate / free / synchronise the device. */
