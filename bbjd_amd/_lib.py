@@ -59,6 +59,17 @@ class Stats(C.Structure):
                 ("qz_device_accuracy", C.c_int64)]
 
 
+class QzProblem(C.Structure):
+    """bbjd_qz_problem: one projected step for bbjd_qz_device_batch (all
+    pointers device addresses)"""
+    _fields_ = [("k", C.c_int), ("ldm", C.c_int), ("MA", C.c_void_p), ("MB", C.c_void_p),
+                ("nu0_re", C.c_double), ("nu0_im", C.c_double), ("mu0_re", C.c_double), ("mu0_im", C.c_double),
+                ("sigma_re", C.c_double), ("sigma_im", C.c_double), ("target", C.c_int), ("nkeep", C.c_int),
+                ("nconv", C.c_int), ("keep", C.c_int), ("conv", C.c_void_p), ("tol", C.c_double),
+                ("alpha", C.c_void_p), ("beta", C.c_void_p), ("x", C.c_void_p), ("zr", C.c_void_p),
+                ("zfull", C.c_void_p), ("tfull", C.c_void_p), ("info", C.c_void_p)]
+
+
 class PencilDesc(C.Structure):
     _fields_ = [("n", C.c_int64), ("bs", C.c_int), ("hw", C.c_int), ("ntau", C.c_int),
                 ("band", C.c_void_p), ("tau", C.c_void_p), ("bdiag", C.c_void_p)]

@@ -146,7 +146,7 @@ __global__ __launch_bounds__(QT) void k_qz_step(const bbjd_qz_problem *__restric
     int *info = P.info;
     const double eps = 2.220446049250313e-16;
     if (k < 1 || k > QK) {
-        if (lane == 0) { info[0] = BBJD_QZD_UNSUPPORTED; info[1] = 0; }
+        if (lane == 0) { info[0] = BBJD_QZD_UNSUPPORTED; info[1] = 0; info[2] = 0; info[3] = 0; }
         return;
     }
     const Z2 nu0 = zmk(P.nu0_re, P.nu0_im), mu0 = zmk(P.mu0_re, P.mu0_im), sigma = zmk(P.sigma_re, P.sigma_im);
@@ -174,7 +174,7 @@ __global__ __launch_bounds__(QT) void k_qz_step(const bbjd_qz_problem *__restric
     }
     __syncthreads();
     if (!(cmax > 0) || lmax > 1e-12 * cmax || !(dmin > 1e-300) || zabs(nu0) == 0.0) {
-        if (lane == 0) { info[0] = BBJD_QZD_NOT_TRIANGULAR; info[1] = 0; }
+        if (lane == 0) { info[0] = BBJD_QZD_NOT_TRIANGULAR; info[1] = 0; info[2] = 0; info[3] = 0; }
         return;
     }
     // M = C^-1 MB: column j (lane = j) by back-substitution over the rows
@@ -204,7 +204,7 @@ __global__ __launch_bounds__(QT) void k_qz_step(const bbjd_qz_problem *__restric
     fro = sqrt(wave_sum(fro));
     __syncthreads();
     if (!isfinite(fro)) {
-        if (lane == 0) { info[0] = BBJD_QZD_ACCURACY; info[1] = 0; }
+        if (lane == 0) { info[0] = BBJD_QZD_ACCURACY; info[1] = 0; info[2] = 0; info[3] = 0; }
         return;
     }
     // ---- 2. Hessenberg reduction, Z accumulated ----
@@ -301,7 +301,7 @@ __global__ __launch_bounds__(QT) void k_qz_step(const bbjd_qz_problem *__restric
         }
     }
     if (!ok) {
-        if (lane == 0) { info[0] = BBJD_QZD_NO_CONVERGENCE; info[1] = 0; }
+        if (lane == 0) { info[0] = BBJD_QZD_NO_CONVERGENCE; info[1] = 0; info[2] = 0; info[3] = 0; }
         return;
     }
     // ---- 4. theta, accuracy, selection, reordering ----
@@ -317,7 +317,7 @@ __global__ __launch_bounds__(QT) void k_qz_step(const bbjd_qz_problem *__restric
         if (am * zabs(nu0) > 1e-300 && dmu / (zabs(nu0) * am * am) > 10.0 * tol) bad = 1;
     }
     if (__ballot(bad)) {
-        if (lane == 0) { info[0] = BBJD_QZD_ACCURACY; info[1] = 0; }
+        if (lane == 0) { info[0] = BBJD_QZD_ACCURACY; info[1] = 0; info[2] = 0; info[3] = 0; }
         return;
     }
     __syncthreads();

@@ -255,6 +255,7 @@ bbjd_server *bbjd_server_create(const bbjd_request *reqs, int nreq, const bbjd_s
     S->opt.matvec_AB = bbjd_op_matvec_AB;
     S->opt.matvec_ABr = bbjd_op_matvec_ABr;
     S->opt.graphs = cfg->graphs;
+    S->opt.qz_mode = cfg->qz_mode;
     if (cfg->precision == BBJD_PREC_MIXED) {
         S->opt.matvec_A_c64 = bbjd_op_matvec_A_c64;
         S->opt.matvec_B_c64 = bbjd_op_matvec_B_c64;

@@ -11,6 +11,13 @@ projection in one pass over Q), gram (G = Q^H Q, k x k, on the FP64 matrix
 cores: also prints TFLOP/s for its 8 N K^2 flops).  Prints one line: kernel, shape, average us per
 launch, algorithmic bytes per launch and GB/s.
 
+qz: the projected step on the device, N problems of size K in one
+bbjd_qz_device_batch launch (projected pencils of a random complex pencil,
+nearest target, nkeep K/3), against the engine's host step for the same
+problems on one thread (bbjd_qz_target: eigenvalues and target, then
+bbjd_ritz_vector); prints us per launch, device problems/s and host ms per
+problem.
+
 bcr: the exact preconditioner's solve (block cyclic reduction with the dense
 rows' Schur border) on the serving workload's operator shape (block size 8,
 block-tridiagonal, 4 dense rows, sigma 0.5+0.5i), N rows, K = 1 or 2
@@ -71,9 +78,77 @@ def run_bcr(n, nrhs, reps=50):
         op.free()
 
 
+def _projected(k, seed, sigma):
+    """W^H A V, W^H B V as the engine forms them (C = nu0 MA + mu0 MB upper
+    triangular) for a random complex A, diagonal B and orthonormal V"""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    n = 3 * k + 4
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    B = np.diag(1.0 + 0.1 * rng.standard_normal(n))
+    V, _ = np.linalg.qr(rng.standard_normal((n, k)) + 1j * rng.standard_normal((n, k)))
+    nu0 = 1 / np.sqrt(1 + abs(sigma) ** 2)
+    W, _ = np.linalg.qr((nu0 * A - sigma * nu0 * B) @ V)
+    return W.conj().T @ A @ V, W.conj().T @ B @ V
+
+
+def run_qz(nprob, k, reps=20):
+    """(device us per launch of nprob problems, host ms per problem)"""
+    import time
+
+    import numpy as np
+
+    from .qz_device import Batch
+    lib = _lib.load()
+    sigma = 0.3 - 0.2j
+    pens = [_projected(k, 100 + i % 16, sigma) for i in range(nprob)]
+    b = Batch([dict(MA=a, MB=m, sigma=sigma, nkeep=max(1, k // 3)) for a, m in pens])
+    try:
+        b.launch()
+        b.results()
+        t = time.perf_counter()
+        for _ in range(reps):
+            b.launch()
+        lib.bbjd_dev_sync()
+        dev_us = (time.perf_counter() - t) * 1e6 / reps
+        st = b.results()[0]["status"]
+    finally:
+        b.free()
+    f = lib.bbjd_qz_target
+    f.restype = C.c_int
+    f.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, _lib.zcomplex, _lib.zcomplex, C.c_void_p,
+                  C.c_void_p, _lib.zcomplex, C.c_int, C.c_void_p, C.c_int64, C.c_double, C.POINTER(C.c_int64)]
+    lib.bbjd_ritz_vector.restype = C.c_int
+    lib.bbjd_ritz_vector.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, _lib.zcomplex, _lib.zcomplex,
+                                     C.c_void_p]
+    lib.bbjd_qz_init.argtypes = [C.c_void_p, C.c_int64]
+    lib.bbjd_qz_free.argtypes = [C.c_void_p]
+    nu0 = 1 / np.sqrt(1 + abs(sigma) ** 2)
+    al, be, x = (np.zeros(k, np.complex128) for _ in range(3))
+    cv = np.zeros(1, np.complex128)
+    p = C.c_int64(-1)
+    hosts = min(nprob, 16)
+    mats = [(np.asfortranarray(a), np.asfortranarray(m)) for a, m in pens[:hosts]]
+    q = C.create_string_buffer(4096)
+    # a fresh step state per problem: the state caches its last eigenvalues
+    # while the pencil is unchanged
+    t = time.perf_counter()
+    for _ in range(reps):
+        for MA, MB in mats:
+            lib.bbjd_qz_init(q, k)
+            rc = f(q, k, MA.ctypes.data, MB.ctypes.data, _lib.zc(nu0), _lib.zc(-sigma * nu0), al.ctypes.data,
+                   be.ctypes.data, _lib.zc(sigma), 0, cv.ctypes.data, 0, 1e-10, C.byref(p))
+            if rc == 0:
+                lib.bbjd_ritz_vector(q, k, MA.ctypes.data, MB.ctypes.data, _lib.zc(al[p.value]),
+                                     _lib.zc(be[p.value]), x.ctypes.data)
+            lib.bbjd_qz_free(q)
+    host_ms = (time.perf_counter() - t) * 1e3 / (reps * hosts)
+    return dev_us, host_ms, st
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("kernel", choices=sorted(KERNELS) + ["bcr"])
+    ap.add_argument("kernel", choices=sorted(KERNELS) + ["bcr", "qz"])
     ap.add_argument("n", type=int)
     ap.add_argument("k", type=int)
     ap.add_argument("--reps", type=int, default=50)
@@ -85,6 +160,12 @@ def main():
     a = ap.parse_args()
     _lib.load().bbjd_set_rotate_mfma(a.rotate_mode)
     _lib.load().bbjd_set_rotate_chunk(a.rotate_chunk)
+    if a.kernel == "qz":
+        dev_us, host_ms, st = run_qz(a.n, a.k, a.reps)
+        print(f"qz problems={a.n} k={a.k}: device {dev_us:.1f} us/launch ({a.n / (dev_us * 1e-6):.0f} problems/s, "
+              f"status {st}); host step {host_ms:.3f} ms/problem on one thread "
+              f"({1e3 / host_ms:.0f} problems/s)", flush=True)
+        return
     if a.kernel == "bcr":
         ms = run_bcr(a.n, a.k, a.reps)
         print(f"bcr n={a.n} rhs={a.k}: {ms * 1e3:.2f} us/solve", flush=True)

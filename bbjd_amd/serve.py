@@ -35,7 +35,7 @@ class Request(C.Structure):
 class ServeConfig(C.Structure):
     _fields_ = [("workers", C.c_int), ("hbm_budget", C.c_size_t), ("precision", C.c_int),
                 ("realtime", C.c_int), ("sync_yield", C.c_int), ("graphs", C.c_int), ("order", C.c_int),
-                ("pin", C.c_int), ("window", C.c_int)]
+                ("pin", C.c_int), ("window", C.c_int), ("qz_mode", C.c_int)]
 
 
 class ServeReport(C.Structure):
@@ -126,7 +126,7 @@ def _requests(jobs, keep_vectors):
 
 
 def _config(lib, workers=16, precision="fp64", realtime=True, hbm_budget=0, sync_yield=True, graphs=False,
-            order="arrival", pin=False, window=8):
+            order="arrival", pin=False, window=8, qz_mode="ritz"):
     if order not in ("arrival", "largest", "window"):
         raise ValueError("order must be 'arrival', 'largest' or 'window'")
     if int(window) < 1:
@@ -142,6 +142,10 @@ def _config(lib, workers=16, precision="fp64", realtime=True, hbm_budget=0, sync
     cfg.order = {"arrival": 0, "largest": 1, "window": 2}[order]
     cfg.window = int(window)
     cfg.pin = int(bool(pin))
+    modes = {"ritz": _lib.QZ_RITZ, "schur": _lib.QZ_SCHUR, "device": _lib.QZ_DEVICE}
+    if qz_mode not in modes:
+        raise ValueError("qz_mode must be 'ritz', 'schur' or 'device'")
+    cfg.qz_mode = modes[qz_mode]
     return cfg
 
 
@@ -164,15 +168,18 @@ def _results(reqs, outs, keep_vectors):
 
 
 def serve(jobs, workers=16, precision="fp64", realtime=True, hbm_budget=0, keep_vectors=False,
-          sync_yield=True, graphs=False, order="arrival", pin=False, window=8):
+          sync_yield=True, graphs=False, order="arrival", pin=False, window=8, qz_mode="ritz"):
     """Run jobs through the serving loop; returns (results, report).  graphs:
     capture each request's inner GMRES steps into HIP graphs and replay them.
     order: which arrived request a free worker takes, "arrival" (first come,
     first served), "largest" (largest estimated work: nev, then n) or
-    "window" (the largest of the `window` longest-waiting ones)."""
+    "window" (the largest of the `window` longest-waiting ones).  qz_mode:
+    where the projected step runs ("ritz": host, eigenvalues first;
+    "schur": host, the reference's ordered Schur form every iteration;
+    "device": on the GPU for subspaces up to 64)."""
     lib = _lib.load()
     _setup(lib)
-    cfg = _config(lib, workers, precision, realtime, hbm_budget, sync_yield, graphs, order, pin, window)
+    cfg = _config(lib, workers, precision, realtime, hbm_budget, sync_yield, graphs, order, pin, window, qz_mode)
     reqs, keep, outs = _requests(jobs, keep_vectors)
     rep = ServeReport()
     rc = lib.bbjd_serve(reqs, len(jobs), C.byref(cfg), C.byref(rep))
@@ -188,13 +195,13 @@ class Server:
     until close(), and each run() serves a batch of jobs on them."""
 
     def __init__(self, capacity_jobs, workers=16, precision="fp64", hbm_budget=0, sync_yield=True,
-                 graphs=False, pin=False):
+                 graphs=False, pin=False, qz_mode="ritz"):
         lib = _lib.load()
         _setup(lib)
         self._lib = lib
         self.precision = precision
         self._cfg_kw = dict(workers=workers, precision=precision, hbm_budget=hbm_budget, sync_yield=sync_yield,
-                            graphs=graphs, pin=pin)
+                            graphs=graphs, pin=pin, qz_mode=qz_mode)
         cfg = _config(lib, **self._cfg_kw)
         reqs, _keep, _outs = _requests(capacity_jobs, False)
         self._h = lib.bbjd_server_create(reqs, len(capacity_jobs), C.byref(cfg))

@@ -269,6 +269,8 @@ def main():
     ap.add_argument("--sync", choices=["yield", "block"], default="yield",
                     help="how host threads wait on the GPU")
     ap.add_argument("--pin", action="store_true", help="pin each serving worker to one CPU")
+    ap.add_argument("--qz-mode", choices=["ritz", "schur", "device"], default="ritz",
+                    help="where the projected step runs (device: on the GPU for subspaces up to 64)")
     ap.add_argument("--time-budget", type=float, default=540.0,
                     help="seconds for the whole run: timed steps that would end past it, less what the "
                          "plain leg, the verification and the CPU baseline need, are not started (the "
@@ -325,7 +327,8 @@ def main():
         # ---- the resident server: workers, streams, workspaces (not timed) ----
         t = time.perf_counter()
         server = srv.Server(jobs_for(all_idx), workers=args.workers, precision=wl["precision"],
-                            sync_yield="block" if args.sync == "block" else True, pin=args.pin)
+                            sync_yield="block" if args.sync == "block" else True, pin=args.pin,
+                            qz_mode=args.qz_mode)
         log(f"server: {server.workers} workers, {server.workspace_bytes / 1e9:.2f} GB workspace each, "
             f"set up in {time.perf_counter() - t:.1f}s; free HBM {srv.device_free_bytes() / 1e9:.1f} GB")
         # ---- warmup (untimed): the smallest requests ----
@@ -390,7 +393,8 @@ def main():
                 cls_of[members[j]] = key
         srv.ktime_enable(True)
         t = time.perf_counter()
-        pres, prep = srv.serve(jobs_for(sample, False), workers=1, realtime=False, precision=wl["precision"])
+        pres, prep = srv.serve(jobs_for(sample, False), workers=1, realtime=False, precision=wl["precision"],
+                               qz_mode=args.qz_mode)
         t_plain = time.perf_counter() - t
         srv.ktime_enable(False)
         kcnt, kms, kbytes = srv.ktime_harvest()
@@ -488,7 +492,8 @@ def main():
                                 f"the exact banded LU as its preconditioner"),
                    "workers": args.workers, "requests": args.nreq,
                    "admission": args.order + (f" {args.window}" if args.order == "window" else ""),
-                   "host_wait": args.sync, "workers_pinned": bool(args.pin)},
+                   "host_wait": args.sync, "workers_pinned": bool(args.pin),
+                   "projected_step": args.qz_mode},
         "steps_requested": args.steps,
         "served_eigenpairs_per_s": round(served_rate, 3) if served_rate else None,
         "plain_eigenpairs_per_s": round(plain_rate, 3) if plain_rate else None,

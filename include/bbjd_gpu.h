@@ -297,7 +297,8 @@ int bbjd_block_orthogonalize_noalloc(bbjd_int n, bbjd_int k, const bbjd_zomplex 
  * alpha = 1 - mu0 t_ii, beta = nu0 t_ii, and the Ritz coefficients
  * x = Z y of the target.  info[0]: BBJD_QZD_* status (anything but OK asks
  * the caller to take the host step), info[1]: target position, info[2]:
- * number selected, info[3]: QR iterations. */
+ * number selected, info[3]: QR iterations (the last three 0 when the step
+ * stopped early: not triangular, accuracy, no convergence, unsupported). */
 typedef struct {
     int                 k, ldm;         /* size; leading dimension of MA, MB  */
     const bbjd_zomplex *MA, *MB;        /* device, column-major               */

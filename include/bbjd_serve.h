@@ -65,6 +65,8 @@ typedef struct {
                              waiting requests are compared (default 8); a
                              request is overtaken by at most window - 1
                              later arrivals                                  */
+    int    qz_mode;       /* bbjd_gpu_options.qz_mode of every request
+                             (BBJD_QZ_RITZ by default)                       */
 } bbjd_serve_config;
 
 enum { BBJD_ORDER_ARRIVAL = 0, BBJD_ORDER_LARGEST = 1, BBJD_ORDER_WINDOW = 2 };

@@ -94,7 +94,7 @@ GPU_PROBE = r"""
 #define S(T) printf("sizeof." #T " %zu\n", sizeof(T))
 #define O(T, f) printf(#T "." #f " %zu\n", offsetof(T, f))
 int main(void) {
-  S(bbjd_gpu_options); S(bbjd_stats); S(bbjd_serve_config);
+  S(bbjd_gpu_options); S(bbjd_stats); S(bbjd_serve_config); S(bbjd_qz_problem);
   O(bbjd_gpu_options, callback_space); O(bbjd_gpu_options, precision); O(bbjd_gpu_options, stream);
   O(bbjd_gpu_options, matvec_A_c64); O(bbjd_gpu_options, matvec_B_c64); O(bbjd_gpu_options, precond_c64);
   O(bbjd_gpu_options, precond2); O(bbjd_gpu_options, qz_mode); O(bbjd_gpu_options, graphs);
@@ -107,6 +107,7 @@ int main(void) {
   O(bbjd_serve_config, workers); O(bbjd_serve_config, hbm_budget); O(bbjd_serve_config, precision);
   O(bbjd_serve_config, realtime); O(bbjd_serve_config, sync_yield); O(bbjd_serve_config, graphs);
   O(bbjd_serve_config, order); O(bbjd_serve_config, pin); O(bbjd_serve_config, window);
+  O(bbjd_serve_config, qz_mode);
   return 0;
 }
 """
@@ -119,12 +120,14 @@ def test_gpu_struct_layouts_match_ctypes(tmp_path):
     src = tmp_path / "gprobe.c"
     exe = tmp_path / "gprobe"
     req = "".join(f"  O(bbjd_request, {f});\n" for f, _ in serve.Request._fields_)
+    req += "".join(f"  O(bbjd_qz_problem, {f});\n" for f, _ in _lib.QzProblem._fields_)
     src.write_text(GPU_PROBE.replace("  return 0;", "  S(bbjd_request);\n" + req + "  return 0;"))
     subprocess.run(["gcc", "-std=c11", f"-I{INC}", str(src), "-o", str(exe)], check=True)
     out = dict(line.rsplit(" ", 1) for line in
                subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.splitlines())
     for cname, cls in (("bbjd_gpu_options", _lib.GpuOptions), ("bbjd_stats", _lib.Stats),
-                       ("bbjd_serve_config", serve.ServeConfig), ("bbjd_request", serve.Request)):
+                       ("bbjd_serve_config", serve.ServeConfig), ("bbjd_request", serve.Request),
+                       ("bbjd_qz_problem", _lib.QzProblem)):
         assert int(out[f"sizeof.{cname}"]) == C.sizeof(cls), cname
         for f, _ in cls._fields_:
             assert int(out[f"{cname}.{f}"]) == getattr(cls, f).offset, (cname, f)

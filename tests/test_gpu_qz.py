@@ -192,3 +192,28 @@ def test_engine_with_device_projected_step_matches_reference(name):
     A, B = p.to_scipy()
     for j in range(nconv):
         assert np.linalg.norm(A @ X[:, j] - ev[j] * (B @ X[:, j])) <= 1e-10 * (1 + abs(ev[j])) * 10
+
+
+def test_batched_launch_equals_one_problem_launches():
+    """many problems of different sizes, targets and flags in one launch:
+    each wavefront's result bit-identical to the same problem launched alone,
+    and the restart rotation the leading keep columns of its Z"""
+    from bbjd_amd.qz_device import step_batch
+    rng = np.random.default_rng(5)
+    probs = []
+    for i, k in enumerate([1, 5, 64, 17, 33, 8, 64, 2, 48, 40, 12, 64]):
+        sigma = complex(rng.standard_normal(), rng.standard_normal())
+        MA, MB = projected(k, 300 + i, sigma)
+        conv = list(sl.eigvals(MA, MB)[:1]) if k > 6 and i % 2 else []
+        probs.append(dict(MA=MA, MB=MB, sigma=sigma, conv=conv, nkeep=max(1, k // 3), target=i % 7,
+                          keep=max(1, k // 2)))
+    probs.insert(3, dict(MA=rng.standard_normal((6, 6)) + 0j, MB=np.eye(6) + 0j, sigma=0.5))   # not triangular
+    batch = step_batch(probs, full=True)
+    assert batch[3]["status"] == NOT_TRIANGULAR
+    for q, got in zip(probs, batch):
+        one = device_step(q["MA"], q["MB"], q["sigma"], q.get("conv", ()), q.get("nkeep", 1), q.get("target", 0))
+        assert got["status"] == one["status"] and got["p"] == one["p"] and got["nsel"] == one["nsel"]
+        if got["status"] != NOT_TRIANGULAR:
+            for key_b, key_o in (("alpha", "al"), ("beta", "be"), ("x", "x"), ("Z", "Z"), ("T", "T")):
+                assert np.array_equal(got[key_b], one[key_o]), key_b
+            assert np.array_equal(got["zr"], got["Z"][:, : q["keep"]])
