@@ -24,11 +24,12 @@
 //      difference of the two after the exchange);
 //   5. the target by the reference's scan of all positions; the Ritz
 //      coefficients x = Z [y; 1; 0] from (T - t_pp I) y = -T(:, p).
-// A problem the host would decide differently is flagged instead (status
-// bits): C not triangular or singular, QR not converged, a theta whose error
-// bound exceeds 10 tol (as in the host's harmonic step), a tie for the
-// target (up to rounding) or no unconverged candidate; the engine then runs
-// the host step for that iteration.
+// A problem the host would decide differently is flagged instead (info[0]):
+// C not triangular or singular, QR not converged, a choice that the thetas'
+// error bounds could change (as in the host's harmonic step: a converged
+// match, the restart's selection, the target, or a target off by more than
+// 10 tol), a tie for the target (up to rounding) or no unconverged
+// candidate; the engine then runs the host step for that iteration.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -113,7 +114,9 @@ struct Qs {                     // LDS of one problem
     Z2 v[QK];                   // reflector / coefficients
     Z2 th[QK];                  // theta per position
     Z2 al[QK], be[QK];
+    double eb[QK];              // error bound of theta per position
     int sel[QK];
+    int flag;
 };
 
 // rows i, i + 1 of T, columns j0 .. k - 1 (lane = column)
@@ -304,32 +307,44 @@ __global__ __launch_bounds__(QT) void k_qz_step(const bbjd_qz_problem *__restric
         if (lane == 0) { info[0] = BBJD_QZD_NO_CONVERGENCE; info[1] = 0; info[2] = 0; info[3] = 0; }
         return;
     }
-    // ---- 4. theta, accuracy, selection, reordering ----
+    // ---- 4. theta, error bounds, selection, reordering ----
+    // The QR gives each mu = t_ii with an absolute error of about
+    // eps ||M||; theta = 1 / (nu0 mu) - mu0 / nu0 then moves by
+    // |d mu| / (|nu0| |mu|^2).  A choice that these bounds could change
+    // (a converged match within 100 tol, the restart's selection, the
+    // target) is flagged for the host, whose generalized QZ decides.
     const double dmu = 4.0 * eps * fro;
     const Z2 *conv = reinterpret_cast<const Z2 *>(P.conv);
-    const double tol = P.tol;
-    int bad = 0;
-    if (lane < k) {
-        const Z2 m = T[lane + lane * QL];
-        const double am = zabs(m);
-        sm.al[lane] = zsub(zmk(1.0, 0.0), zmul(mu0, m));
-        sm.be[lane] = zmul(nu0, m);
-        if (am * zabs(nu0) > 1e-300 && dmu / (zabs(nu0) * am * am) > 10.0 * tol) bad = 1;
-    }
-    if (__ballot(bad)) {
-        if (lane == 0) { info[0] = BBJD_QZD_ACCURACY; info[1] = 0; info[2] = 0; info[3] = 0; }
-        return;
-    }
+    const double tol = P.tol, lim = 100.0 * tol;
+    auto diag_values = [&]() {
+        if (lane < k) {
+            const Z2 m = T[lane + lane * QL];
+            const double am = zabs(m);
+            sm.al[lane] = zsub(zmk(1.0, 0.0), zmul(mu0, m));
+            sm.be[lane] = zmul(nu0, m);
+            sm.eb[lane] = am * zabs(nu0) > 1e-300 ? dmu / (zabs(nu0) * am * am) : INFINITY;
+        }
+    };
+    auto conv_dist = [&](Z2 lam) {
+        double m = INFINITY;
+        for (int c2 = 0; c2 < P.nconv; c2++) m = fmin(m, zabs(zsub(lam, conv[c2])));
+        return m;
+    };
+    diag_values();
     __syncthreads();
     if (lane == 0) {
+        sm.flag = 0;
         for (int i = 0; i < k; i++) {
             sm.sel[i] = 0;
             if (zabs(sm.be[i]) < 1e-300) continue;
             const Z2 lam = zdiv(sm.al[i], sm.be[i]);
             sm.th[i] = lam;
-            for (int c2 = 0; c2 < P.nconv; c2++)
-                if (zabs(zsub(lam, conv[c2])) < tol * 100) { sm.sel[i] = 1; break; }
+            const double m = conv_dist(lam);
+            if (m < lim) sm.sel[i] = 1;
+            // a restart keeps the selected columns: their membership must be certain
+            if (P.keep > 0 && fabs(m - lim) <= sm.eb[i]) sm.flag = 1;
         }
+        double kmax = -INFINITY, emax = 0;
         for (int pick = 0; pick < P.nkeep; pick++) {
             int best = -1;
             double bv = INFINITY;
@@ -340,9 +355,18 @@ __global__ __launch_bounds__(QT) void k_qz_step(const bbjd_qz_problem *__restric
             }
             if (best < 0) break;
             sm.sel[best] = 1;
+            if (bv + sm.eb[best] > kmax + emax) { kmax = bv; emax = sm.eb[best]; }
         }
+        if (P.keep > 0 && kmax > -INFINITY)
+            for (int i = 0; i < k; i++)
+                if (!sm.sel[i] && zabs(sm.be[i]) >= 1e-300 &&
+                    !(key_of(P.target, sm.th[i], sigma) - sm.eb[i] > kmax + emax)) sm.flag = 1;
     }
     __syncthreads();
+    if (sm.flag) {
+        if (lane == 0) { info[0] = BBJD_QZD_ACCURACY; info[1] = 0; info[2] = 0; info[3] = 0; }
+        return;
+    }
     int pos = 0;
     for (int i = 0; i < k; i++) {
         if (!sm.sel[i]) continue;
@@ -367,11 +391,7 @@ __global__ __launch_bounds__(QT) void k_qz_step(const bbjd_qz_problem *__restric
         pos++;
     }
     // ---- 5. target and Ritz coefficients ----
-    if (lane < k) {
-        const Z2 m = T[lane + lane * QL];
-        sm.al[lane] = zsub(zmk(1.0, 0.0), zmul(mu0, m));
-        sm.be[lane] = zmul(nu0, m);
-    }
+    diag_values();
     __syncthreads();
     int p = 0, status = 0;
     {
@@ -379,15 +399,26 @@ __global__ __launch_bounds__(QT) void k_qz_step(const bbjd_qz_problem *__restric
         for (int i = 0; i < k; i++) {
             if (zabs(sm.be[i]) < 1e-300) continue;
             const Z2 lam = zdiv(sm.al[i], sm.be[i]);
-            int isc = 0;
-            for (int c2 = 0; c2 < P.nconv && !isc; c2++) isc = zabs(zsub(lam, conv[c2])) < tol * 100;
-            if (isc) continue;
+            if (conv_dist(lam) < lim) continue;
             const double v = key_of(P.target, lam, sigma);
             if (v < bv) { bv2 = bv; bv = v; p = i; }
             else if (v < bv2) bv2 = v;
         }
+        // a tie or no candidate goes to the host's ordered Schur form
+        // whatever the bounds; otherwise the choice must hold for every
+        // value within its bound
         if (bv == INFINITY) status = BBJD_QZD_AMBIGUOUS;
         else if (bv2 < INFINITY && bv2 - bv <= 1e-10 * fmax(fmax(fabs(bv), fabs(bv2)), 1e-14)) status = BBJD_QZD_AMBIGUOUS;
+        else {
+            const double ep = sm.eb[p];
+            if (!(ep <= 10.0 * tol) || conv_dist(zdiv(sm.al[p], sm.be[p])) - ep < lim) status = BBJD_QZD_ACCURACY;
+            for (int i = 0; i < k && status == 0; i++) {
+                if (i == p || zabs(sm.be[i]) < 1e-300) continue;
+                const Z2 lam = zdiv(sm.al[i], sm.be[i]);
+                if (conv_dist(lam) + sm.eb[i] < lim) continue;
+                if (!(key_of(P.target, lam, sigma) - sm.eb[i] > bv + ep)) status = BBJD_QZD_ACCURACY;
+            }
+        }
     }
     if (lane == 0) {
         // (T(0:p, 0:p) - t_pp I) y = -T(0:p, p), y_p = 1

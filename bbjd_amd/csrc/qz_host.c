@@ -92,13 +92,14 @@ int bbjd_qz_init(bbjd_qz *q, bbjd_int maxsub)
     }
     q->work = malloc((size_t)q->lwork * sizeof(zcplx));
     q->eal = malloc((size_t)maxsub * sizeof(zcplx)); q->ebe = malloc((size_t)maxsub * sizeof(zcplx));
+    q->ebnd = malloc((size_t)maxsub * sizeof(double));
     q->tri = malloc(ss * sizeof(zcplx));
     q->lu = malloc(ss * sizeof(zcplx)); q->rhs = malloc((size_t)maxsub * sizeof(zcplx));
     q->ipiv = malloc((size_t)maxsub * sizeof(blas_int));
     q->iwork = malloc((size_t)q->liwork * sizeof(blas_int));
     free(A); free(B);
     if (!q->cSA || !q->cSB || !q->cZA || !q->cZB || !q->cal || !q->cbe || !q->rwork || !q->bwork ||
-        !q->sel || !q->work || !q->iwork || !q->eal || !q->ebe || !q->lu || !q->rhs || !q->ipiv || !q->tri) {
+        !q->sel || !q->work || !q->iwork || !q->eal || !q->ebe || !q->ebnd || !q->lu || !q->rhs || !q->ipiv || !q->tri) {
         bbjd_qz_free(q);
         return -1;
     }
@@ -109,7 +110,7 @@ void bbjd_qz_free(bbjd_qz *q)
 {
     free(q->cSA); free(q->cSB); free(q->cZA); free(q->cZB); free(q->cal); free(q->cbe);
     free(q->rwork); free(q->bwork); free(q->sel); free(q->work); free(q->iwork);
-    free(q->eal); free(q->ebe); free(q->lu); free(q->rhs); free(q->ipiv); free(q->tri);
+    free(q->eal); free(q->ebe); free(q->ebnd); free(q->lu); free(q->rhs); free(q->ipiv); free(q->tri);
     memset(q, 0, sizeof(*q));
 }
 
@@ -225,10 +226,10 @@ void bbjd_ritz_coefficients(bbjd_int k, bbjd_int ld, bbjd_int p, zcplx ta, zcplx
 }
 
 /* Eigenvalues of the projected pencil through the standard eigenproblem of
- * the harmonic form (see the file comment); returns 1 when C is not
- * triangular enough to use it.  alpha / beta = theta. */
-static int eigs_harmonic(bbjd_qz *q, bbjd_int k, const zcplx *MA, const zcplx *MB, zcplx nu0, zcplx mu0,
-                         double tol)
+ * the harmonic form (see the file comment), with an error bound per value in
+ * q->ebnd; returns 1 when C is not triangular enough to use it.
+ * alpha / beta = theta. */
+static int eigs_harmonic(bbjd_qz *q, bbjd_int k, const zcplx *MA, const zcplx *MB, zcplx nu0, zcplx mu0)
 {
     const bbjd_int ld = q->maxsub;
     const int inv = cabs(nu0) > 0;               /* C = nu0 MA + mu0 MB; else C = MB */
@@ -262,25 +263,21 @@ static int eigs_harmonic(bbjd_qz *q, bbjd_int k, const zcplx *MA, const zcplx *M
      * absolute error of about eps ||M||; with nu0 != 0, theta = 1 / (nu0 mu)
      * - mu0 / nu0 moves by |d mu| / (|nu0| |mu|^2).  When a Ritz value sits
      * on sigma (C nearly singular: an eigenvalue at the shift), ||M|| is
-     * huge and the values far from sigma lose their digits -- enough that a
-     * converged value is no longer matched within 100 tol and is taken as a
-     * target again.  Any theta that may be off by more than 10 tol sends the
-     * step to the generalized QZ (ZGGEV), whose errors do not grow so. */
+     * huge and the values far from sigma lose their digits.  The bound goes
+     * with each value; the caller decides whether the values are accurate
+     * enough for its choice (pick_target). */
     const double dmu = 4.0 * DBL_EPSILON * mfro;
     for (bbjd_int i = 0; i < k; i++) {
         const zcplx mu = q->eal[i];
         if (!isfinite(creal(mu)) || !isfinite(cimag(mu))) return 1;
         const double amu = cabs(mu);
         if (inv) {
-            if (amu * cabs(nu0) > 1e-300 && dmu / (cabs(nu0) * amu * amu) > 10.0 * tol) return 1;
-        } else if (dmu > 10.0 * tol) {
-            return 1;
-        }
-        if (inv) {
+            q->ebnd[i] = amu * cabs(nu0) > 1e-300 ? dmu / (cabs(nu0) * amu * amu) : INFINITY;
             /* mu = 1 / (nu0 theta + mu0): theta = (1 - mu0 mu) / (nu0 mu) */
             q->eal[i] = 1.0 - mu0 * mu;
             q->ebe[i] = nu0 * mu;
         } else {
+            q->ebnd[i] = dmu;
             q->ebe[i] = 1.0;
         }
     }
@@ -288,42 +285,54 @@ static int eigs_harmonic(bbjd_qz *q, bbjd_int k, const zcplx *MA, const zcplx *M
     return 0;
 }
 
-/* Eigenvalues of the projected pencil and the target position without the
- * Schur form, cached while the pencil is unchanged (lock iterations): the
- * harmonic standard eigenproblem, else ZGGEV('N', 'N') on copies of
- * (MA, MB).  The target is the best unconverged value by key over all
- * positions, as in bbjd_qz_solve.  Returns 0, -1 on a LAPACK error, or 2
- * when the choice is ambiguous (a tie up to rounding, or no unconverged
- * candidate): then the ordered Schur form decides, as in the reference. */
-int bbjd_qz_target(bbjd_qz *q, bbjd_int k, const zcplx *MA, const zcplx *MB, zcplx nu0, zcplx mu0, zcplx *al,
-                   zcplx *be, zcplx sigma, bbjd_target tgt, const zcplx *conv, bbjd_int nconv, double tol,
-                   bbjd_int *tpos)
+/* Eigenvalues of (MA, MB) by ZGGEV('N', 'N') on copies, error bounds 0. */
+static int eigs_ggev(bbjd_qz *q, bbjd_int k, const zcplx *MA, const zcplx *MB)
 {
     const bbjd_int ld = q->maxsub;
     const size_t colb = (size_t)k * sizeof(zcplx);
-    if (!(q->ecache_valid && q->ecache_k == k)) {
-        if (eigs_harmonic(q, k, MA, MB, nu0, mu0, tol)) {
-            zcplx *A = q->cSA, *B = q->cSB;   /* scratch: the ZGGES cache is invalid here */
-            for (bbjd_int j = 0; j < k; j++) {
-                memcpy(A + j * ld, MA + j * ld, colb);
-                memcpy(B + j * ld, MB + j * ld, colb);
-            }
-            blas_int bk = (blas_int)k, lda = (blas_int)ld, one = 1, lw = q->lwork, info = 0;
-            scipy_zggev_("N", "N", &bk, A, &lda, B, &lda, q->eal, q->ebe, NULL, &one, NULL, &one, q->work, &lw,
-                         q->rwork, &info);
-            if (info) return -1;
-            q->ggev_calls++;
-        }
-        q->cache_valid = 0;
-        q->ecache_valid = 1;
-        q->ecache_k = k;
+    zcplx *A = q->cSA, *B = q->cSB;   /* scratch: the ZGGES cache is invalid here */
+    for (bbjd_int j = 0; j < k; j++) {
+        memcpy(A + j * ld, MA + j * ld, colb);
+        memcpy(B + j * ld, MB + j * ld, colb);
     }
-    memcpy(al, q->eal, colb);
-    memcpy(be, q->ebe, colb);
+    blas_int bk = (blas_int)k, lda = (blas_int)ld, one = 1, lw = q->lwork, info = 0;
+    scipy_zggev_("N", "N", &bk, A, &lda, B, &lda, q->eal, q->ebe, NULL, &one, NULL, &one, q->work, &lw,
+                 q->rwork, &info);
+    if (info) return -1;
+    for (bbjd_int i = 0; i < k; i++) q->ebnd[i] = 0.0;
+    q->cache_valid = 0;
+    q->ggev_calls++;
+    return 0;
+}
+
+static double conv_distance(zcplx lam, const zcplx *conv, bbjd_int nconv)
+{
+    double m = INFINITY;
+    for (bbjd_int c = 0; c < nconv; c++) {
+        const double d = cabs(lam - conv[c]);
+        if (d < m) m = d;
+    }
+    return m;
+}
+
+/* The target among the cached values (the reference's rule: the best
+ * unconverged value by key over all positions, src/jd_core.c:430-458).
+ * Returns 0, 2 when the choice is ambiguous for exact values (a tie up to
+ * rounding, or no unconverged candidate), or 3 when the values' error
+ * bounds could change the choice: a value whose converged match (within
+ * 100 tol) is uncertain and could compete, a competitor whose key interval
+ * reaches the target's, or a target off by more than 10 tol. */
+static int pick_target(const bbjd_qz *q, bbjd_int k, zcplx sigma, bbjd_target tgt, const zcplx *conv,
+                       bbjd_int nconv, double tol, bbjd_int *tpos)
+{
+    const zcplx *al = q->eal, *be = q->ebe;
+    const double *eb = q->ebnd;
     bbjd_int p = 0;
     double bv = DBL_MAX, bv2 = DBL_MAX;
+    int exact = 1;
     for (bbjd_int i = 0; i < k; i++) {
         if (cabs(be[i]) < 1e-300) continue;
+        if (eb[i] != 0.0) exact = 0;
         zcplx lam = al[i] / be[i];
         if (is_converged(lam, conv, nconv, tol)) continue;
         double v = key_of(tgt, lam, sigma);
@@ -336,13 +345,54 @@ int bbjd_qz_target(bbjd_qz *q, bbjd_int k, const zcplx *MA, const zcplx *MB, zcp
      * two agree unless the choice is a tie up to rounding or there is no
      * unconverged candidate (the reference then takes position 0 of its
      * ordered form): say so, and the caller decides by the reference's
-     * sequence for this iteration. */
+     * sequence for this iteration (whatever the bounds). */
     if (bv == DBL_MAX) return 2;
     if (bv2 < DBL_MAX) {
         const double scale = fmax(fmax(fabs(bv), fabs(bv2)), 1e-14);
         if (bv2 - bv <= 1e-10 * scale) return 2;
     }
+    if (!exact) {
+        const double lim = 100.0 * tol, ep = eb[p];
+        if (!(ep <= 10.0 * tol) || conv_distance(al[p] / be[p], conv, nconv) - ep < lim) return 3;
+        for (bbjd_int i = 0; i < k; i++) {
+            if (i == p || cabs(be[i]) < 1e-300) continue;
+            const zcplx lam = al[i] / be[i];
+            if (conv_distance(lam, conv, nconv) + eb[i] < lim) continue;      /* surely converged */
+            if (!(key_of(tgt, lam, sigma) - eb[i] > bv + ep)) return 3;
+        }
+    }
     return 0;
+}
+
+/* Eigenvalues of the projected pencil and the target position without the
+ * Schur form, cached while the pencil is unchanged (lock iterations): the
+ * harmonic standard eigenproblem, else -- or when its error bounds could
+ * change the choice of target -- ZGGEV('N', 'N') on copies of (MA, MB).  The
+ * target is the best unconverged value by key over all positions, as in
+ * bbjd_qz_solve.  Returns 0, -1 on a LAPACK error, or 2 when the choice is
+ * ambiguous (a tie up to rounding, or no unconverged candidate): then the
+ * ordered Schur form decides, as in the reference. */
+int bbjd_qz_target(bbjd_qz *q, bbjd_int k, const zcplx *MA, const zcplx *MB, zcplx nu0, zcplx mu0, zcplx *al,
+                   zcplx *be, zcplx sigma, bbjd_target tgt, const zcplx *conv, bbjd_int nconv, double tol,
+                   bbjd_int *tpos)
+{
+    const size_t colb = (size_t)k * sizeof(zcplx);
+    if (!(q->ecache_valid && q->ecache_k == k)) {
+        if (eigs_harmonic(q, k, MA, MB, nu0, mu0) && eigs_ggev(q, k, MA, MB)) return -1;
+        q->cache_valid = 0;
+        q->ecache_valid = 1;
+        q->ecache_k = k;
+    }
+    int rc = pick_target(q, k, sigma, tgt, conv, nconv, tol, tpos);
+    if (rc == 3) {
+        /* the harmonic values are not accurate enough for this choice: the
+         * generalized QZ's (cached in their place) decide */
+        if (eigs_ggev(q, k, MA, MB)) return -1;
+        rc = pick_target(q, k, sigma, tgt, conv, nconv, tol, tpos);
+    }
+    memcpy(al, q->eal, colb);
+    memcpy(be, q->ebe, colb);
+    return rc;
 }
 
 /* Ritz vector of (MA, MB) for the eigenvalue ta / tb: two steps of inverse

@@ -20,6 +20,7 @@ typedef struct {
     int       ecache_valid;
     bbjd_int  ecache_k;
     zcplx    *eal, *ebe, *lu, *rhs, *tri;
+    double   *ebnd;             /* error bound of each cached theta (0: ZGGEV) */
     blas_int *ipiv;
     long      ggev_calls, std_calls;
 } bbjd_qz;

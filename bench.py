@@ -269,6 +269,9 @@ def main():
     ap.add_argument("--sync", choices=["yield", "block"], default="yield",
                     help="how host threads wait on the GPU")
     ap.add_argument("--pin", action="store_true", help="pin each serving worker to one CPU")
+    ap.add_argument("--operators", choices=["resident", "on-demand"], default="resident",
+                    help="on-demand: the server builds each request's operator (host factorization, upload) "
+                         "when it admits the request, inside the timed region, charged against free HBM")
     ap.add_argument("--qz-mode", choices=["ritz", "schur", "device"], default="ritz",
                     help="where the projected step runs (device: on the GPU for subspaces up to 64)")
     ap.add_argument("--time-budget", type=float, default=540.0,
@@ -301,20 +304,24 @@ def main():
 
     # ---- resident operators (not timed) ----
     t0 = time.perf_counter()
-    ops, norms = [], []
+    ops, norms, pencils = [], [], []
     for i, r in enumerate(reqs):
         p = BandedTauPencil.generate(r["n"], *wl["shape"], seed=r["pseed"])
         norms.append(norm_lower_bound(p))
         op = DeviceOperator(p, r["sigma"], pc="exact", c64=wl["precision"] == "mixed")
         op.pencil = None          # host copy no longer needed
         ops.append(op)
+        # on-demand operators: the server factors and uploads each request's
+        # operator when it admits the request, inside the timed region
+        pencils.append(p if args.operators == "on-demand" else None)
     t_setup = time.perf_counter() - t0
     op_bytes = sum(o.device_bytes for o in ops)
     log(f"{len(ops)} operators resident ({op_bytes / 1e9:.1f} GB) in {t_setup:.1f}s; "
         f"free HBM {srv.device_free_bytes() / 1e9:.1f} GB")
 
-    def jobs_for(idx, with_arrival=True):
-        return [dict(op=ops[i], sigma=reqs[i]["sigma"], nev=reqs[i]["k"], maxsub=wl["maxsub"](reqs[i]["k"]),
+    def jobs_for(idx, with_arrival=True, on_demand=False):
+        src = (lambda i: dict(pencil=pencils[i], pc="exact")) if on_demand else (lambda i: dict(op=ops[i]))
+        return [dict(**src(i), sigma=reqs[i]["sigma"], nev=reqs[i]["k"], maxsub=wl["maxsub"](reqs[i]["k"]),
                      minsub=wl["minsub"](reqs[i]["k"]), maxiter=reqs[i]["maxiter"], tol=1e-10,
                      inner_maxiter=wl["inner"], inner_tol=wl["inner_tol"],
                      arrival=reqs[i]["arrival"] if with_arrival else 0.0) for i in idx]
@@ -348,7 +355,8 @@ def main():
                 break
             free0 = srv.device_free_bytes()
             t = time.perf_counter()
-            res, rep = server.run(jobs_for(all_idx), realtime=wl["realtime"], order=args.order,
+            res, rep = server.run(jobs_for(all_idx, on_demand=args.operators == "on-demand"),
+                                  realtime=wl["realtime"], order=args.order,
                                   window=args.window, keep_vectors=(s == 0 and not args.no_verify))
             walls.append(time.perf_counter() - t)
             steps_timed += 1
@@ -357,7 +365,8 @@ def main():
                     r.pop("eigenvectors", None)
             served.append((res, rep))
             log(f"step {s}: {rep['pairs']} pairs in {walls[-1]:.1f}s, workers {rep['workers']}, "
-                f"peak concurrency {rep['peak_concurrency']}, failed {rep['failed']}; free HBM "
+                f"peak concurrency {rep['peak_concurrency']}, peak footprint {rep['peak_bytes'] / 1e9:.1f} GB, "
+                f"failed {rep['failed']}; free HBM "
                 f"{free0 / 1e9:.1f} -> {srv.device_free_bytes() / 1e9:.1f} GB")
         server.close()
         if served and not args.no_verify:
@@ -493,7 +502,7 @@ def main():
                    "workers": args.workers, "requests": args.nreq,
                    "admission": args.order + (f" {args.window}" if args.order == "window" else ""),
                    "host_wait": args.sync, "workers_pinned": bool(args.pin),
-                   "projected_step": args.qz_mode},
+                   "projected_step": args.qz_mode, "operators": args.operators},
         "steps_requested": args.steps,
         "served_eigenpairs_per_s": round(served_rate, 3) if served_rate else None,
         "plain_eigenpairs_per_s": round(plain_rate, 3) if plain_rate else None,

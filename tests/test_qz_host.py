@@ -12,7 +12,23 @@ import pytest
 from bbjd_amd import _lib
 
 
-def _qz_target(MA, MB, nu0, mu0, sigma, conv=(), tol=1e-10, want_rc=(0,)):
+def _qz_offsets(tmp_dir):
+    """offsets of the step state's ZGGEV and standard-problem call counters"""
+    import os
+    import subprocess
+    src = os.path.join(tmp_dir, "qzoff.c")
+    exe = os.path.join(tmp_dir, "qzoff")
+    with open(src, "w") as f:
+        f.write('#include <stdio.h>\n#include <stddef.h>\n#include "qz_host.h"\n'
+                'int main(void) { printf("%zu %zu\\n", offsetof(bbjd_qz, ggev_calls), '
+                'offsetof(bbjd_qz, std_calls)); return 0; }\n')
+    csrc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "bbjd_amd", "csrc")
+    subprocess.run(["gcc", "-std=c11", f"-I{csrc}", src, "-o", exe], check=True)
+    a, b = subprocess.run([exe], check=True, capture_output=True, text=True).stdout.split()
+    return int(a), int(b)
+
+
+def _qz_target(MA, MB, nu0, mu0, sigma, conv=(), tol=1e-10, want_rc=(0,), counters=None):
     lib = _lib.load()
     k = MA.shape[0]
     f = lib.bbjd_qz_target
@@ -32,6 +48,8 @@ def _qz_target(MA, MB, nu0, mu0, sigma, conv=(), tol=1e-10, want_rc=(0,)):
         rc = f(q, k, MAf.ctypes.data, MBf.ctypes.data, _lib.zc(nu0), _lib.zc(mu0), al.ctypes.data, be.ctypes.data,
                _lib.zc(sigma), 0, cv.ctypes.data, len(conv), tol, C.byref(p))
         assert rc in want_rc, rc
+        if counters is not None:
+            counters.extend(C.c_long.from_buffer(q, o).value for o in counters.offsets)
         return al / be, p.value, rc
     finally:
         lib.bbjd_qz_free(q)
@@ -103,3 +121,51 @@ def test_well_separated_sigma_uses_the_standard_problem_accurately():
     for t in th:
         assert np.min(np.abs(t - d)) <= 1e-12 * max(1, abs(t))
     assert abs(th[p] - sigma) == pytest.approx(np.min(np.abs(d - sigma)), rel=1e-12)
+
+
+class _Counters(list):
+    def __init__(self, offsets):
+        super().__init__()
+        self.offsets = offsets
+
+
+def _key(lam, sigma):
+    return abs(lam - sigma)
+
+
+def test_error_bounds_decide_only_what_they_can_change(tmp_path):
+    """sigma close to one eigenvalue: ||C^-1 MB|| grows, and the harmonic
+    values far from sigma carry error bounds far above tol.  The step keeps
+    them (no ZGGEV) when no bound can change the choice -- the target is the
+    exact one, the values near sigma accurate -- and takes ZGGEV when a bound
+    could: a converged value near sigma whose match within 100 tol is not
+    certain, or a competitor whose key interval reaches the target's.  Over
+    random cases the chosen target always equals the exact rule's choice on
+    the pencil's true eigenvalues."""
+    offs = _qz_offsets(str(tmp_path))
+    rng = np.random.default_rng(11)
+    kept = ggev = 0
+    for trial in range(60):
+        n = int(rng.integers(6, 40))
+        d = rng.standard_normal(n) * 3 + 1j * rng.standard_normal(n) * 3
+        j = int(rng.integers(n))
+        gap = 10.0 ** rng.uniform(-12, -2)
+        sigma = d[j] + gap * np.exp(2j * np.pi * rng.random())
+        V, _ = np.linalg.qr(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+        MA, MB, nu0, mu0 = _projected(d, sigma, V)
+        conv = [d[j]] if trial % 3 == 0 else []
+        cnt = _Counters(offs)
+        th, p, rc = _qz_target(MA, MB, nu0, mu0, sigma, conv=conv, want_rc=(0, 2), counters=cnt)
+        used_ggev = cnt[0] > 0
+        ggev += used_ggev
+        kept += not used_ggev
+        if rc != 0:
+            continue
+        cand = [x for x in d if not (conv and abs(x - conv[0]) < 1e-8)]
+        best = min(cand, key=lambda x: _key(x, sigma))
+        assert abs(th[p] - best) <= 1e-9 * max(1, abs(best)), (trial, th[p], best)
+        if not used_ggev:
+            # the cheap path kept: its target is accurate to well within tol
+            assert abs(th[p] - best) <= 1e-10
+    # both paths occur over these cases
+    assert kept >= 10 and ggev >= 5, (kept, ggev)
