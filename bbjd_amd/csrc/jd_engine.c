@@ -30,6 +30,11 @@
 typedef double complex zcx;
 typedef struct { float re, im; } c64;
 
+/* Arnoldi steps of the inner solve the host may have in flight at once */
+#define GM_LOOK_MAX 8
+static int g_gmres_look = 4;
+void bbjd_set_gmres_lookahead(int steps) { g_gmres_look = steps < 1 ? 1 : steps > GM_LOOK_MAX ? GM_LOOK_MAX : steps; }
+
 struct bbjd_dev {
     hipStream_t      stream;
     int              own_stream;
@@ -70,6 +75,12 @@ struct bbjd_dev {
     void            *qzd;
     char            *hqzd;
     int              zr_ready;          /* d->zr holds the restart's rotation  */
+    /* inner-solve lookahead: per in-flight Arnoldi step, a pinned slot for
+     * its Hessenberg column and scalar slots, and an event after its copies */
+    char            *hlook;
+    size_t           look_slot;
+    hipEvent_t       lev[GM_LOOK_MAX];
+    int              nlev;
 };
 
 static __thread hipStream_t tl_cb_stream;
@@ -235,6 +246,8 @@ static void workspace_free_impl(bbjd_workspace *ws)
         if (d->gx) { graphs_reset(d); free(d->gx); }
         if (d->hgco) hipHostFree(d->hgco);
         if (d->hqzd) hipHostFree(d->hqzd);
+        if (d->hlook) hipHostFree(d->hlook);
+        for (int i = 0; i < d->nlev; i++) (void)hipEventDestroy(d->lev[i]);
         if (d->dblock) hipFree(d->dblock);
         if (d->hscal) hipHostFree(d->hscal);
         if (d->hsmall) hipHostFree(d->hsmall);
@@ -346,6 +359,12 @@ static bbjd_workspace *workspace_alloc_impl(const bbjd_params *p, const bbjd_gpu
     if (!HIPOK(hipHostMalloc((void **)&d->hgram, 2 * ss * sizeof(zcx), 0))) { d->hgram = NULL; goto fail; }
     if (!HIPOK(hipHostMalloc((void **)&d->hgco, 64, 0))) { d->hgco = NULL; goto fail; }
     if (!HIPOK(hipHostMalloc((void **)&d->hqzd, hqzd_bytes((size_t)L.maxsub), 0))) { d->hqzd = NULL; goto fail; }
+    if (L.ims > 0 && o->callback_space == BBJD_CALLBACK_DEVICE) {
+        d->look_slot = ((size_t)(L.ims + 1) * sizeof(zcx) + 255) / 256 * 256 + 256;
+        if (!HIPOK(hipHostMalloc((void **)&d->hlook, GM_LOOK_MAX * d->look_slot, 0))) { d->hlook = NULL; goto fail; }
+        for (; d->nlev < GM_LOOK_MAX; d->nlev++)
+            if (!HIPOK(hipEventCreateWithFlags(&d->lev[d->nlev], hipEventDisableTiming))) goto fail;
+    }
     d->gx_cap = (int)L.ims + 1;
     if (!(d->gx = calloc((size_t)d->gx_cap, sizeof(hipGraphExec_t)))) goto fail;
     d->pair_res = calloc((size_t)L.nev + 1, sizeof(double));
@@ -621,6 +640,65 @@ static int seg_end(ctx_t *c, int idx, int rc)
     return 0;
 }
 
+/* Enqueues Arnoldi step j of the inner solve: v_{j+1} = M^-1 (I - QL QL^H)
+ * (theta_b A - theta_a B)(I - QR QR^H) v_j, its CGS2 against v_0 .. v_j, the
+ * new vector normalised (also into the next step's input copy), and copies
+ * of the Hessenberg column to hh and of the scalar slots to hs (host).
+ * Only device work: the step reads v_j, theta and the norms from device
+ * memory.  Returns 0, -1 when a callback failed, -2 on an enqueue error. */
+static int arnoldi_enqueue(ctx_t *c, bbjd_int j, bbjd_int ndefl, const void *QR, const void *QL, void *V, void *t1,
+                           void *t2, void *hh, double *hs)
+{
+    struct bbjd_dev *d = c->d;
+    const bbjd_params *P = c->P;
+    const bbjd_int n = c->n;
+    const int mixed = d->opt.precision == BBJD_PREC_MIXED;
+    const size_t esz = mixed ? 8 : 16;
+    const void *qr = mixed ? d->qrc : QR, *ql = mixed ? d->qlc : QL;
+    void *vj1 = (char *)V + (size_t)(j + 1) * c->ld * esz;
+    int rc = 0, cb = 0;
+    if (!mixed) {
+        /* t1 = v_j: written with v_j's normalisation (the previous step's
+         * last launch, or the prologue's) */
+        rc |= bbjd_k_project_out(c->s, n, (int)ndefl, QR, c->ld, t1, &d->rd);
+        if (!rc && apply_ab(c, t1, t2, vj1)) cb = 1;
+        rc |= bbjd_k_lincomb_dev(c->s, n, d->gco, t2, vj1, t2);
+        rc |= bbjd_k_project_out(c->s, n, (int)ndefl, QL, c->ld, t2, &d->rd);
+        if (!rc && !cb && apply_pc(c, t2, vj1)) cb = 1;
+        rc |= bbjd_k_cgs2(c->s, n, (int)(j + 1), V, c->ld, vj1, &d->rd, SL_G, d->rd.hsum);
+        rc |= !HIPOK(hipMemcpyAsync(hh, d->rd.hsum, (size_t)(j + 1) * 16, hipMemcpyDeviceToHost, c->s));
+        rc |= bbjd_k_scale_inv2(c->s, n, vj1, t1, d->rd.scal, SL_G, 0);   /* t1: the next step's */
+    } else {
+        void *a = d->tc[0], *bq = d->tc[1], *e = d->tc[2];
+        /* a = v_j: written with v_j's normalisation */
+        rc |= bbjd_k_project_out_c(c->s, n, (int)ndefl, qr, c->ld, a, &d->rd);
+        if (d->opt.matvec_AB_c64 && d->opt.callback_space == BBJD_CALLBACK_DEVICE) {
+            hipStream_t prev = tl_cb_stream;
+            tl_cb_stream = c->s;
+            if (!rc && d->opt.matvec_AB_c64(n, (const bbjd_cfloat *)a, (bbjd_cfloat *)bq, (bbjd_cfloat *)e,
+                                            P->op_data))
+                cb = 1;
+            tl_cb_stream = prev;
+        } else {
+            if (!rc && apply_c64(c, d->opt.matvec_A_c64, P->matvec_A, a, bq, t1, t2)) cb = 1;
+            if (!rc && !cb && apply_c64(c, d->opt.matvec_B_c64, P->matvec_B, a, e, t1, t2)) cb = 1;
+        }
+        rc |= bbjd_k_lincomb_dev_c(c->s, n, d->gco, bq, e, bq);
+        rc |= bbjd_k_project_out_c(c->s, n, (int)ndefl, ql, c->ld, bq, &d->rd);
+        if (P->precond) {
+            if (!rc && !cb && apply_c64(c, d->opt.precond_c64, (bbjd_matvec_fn)P->precond, bq, vj1, t1, t2))
+                cb = 1;
+        } else
+            rc |= !HIPOK(hipMemcpyAsync(vj1, bq, (size_t)n * 8, hipMemcpyDeviceToDevice, c->s));
+        rc |= bbjd_k_cgs2_c(c->s, n, (int)(j + 1), V, c->ld, vj1, &d->rd, SL_G, d->rd.hsum);
+        rc |= !HIPOK(hipMemcpyAsync(hh, d->rd.hsum, (size_t)(j + 1) * 8, hipMemcpyDeviceToHost, c->s));
+        rc |= bbjd_k_scale_inv2_c(c->s, n, vj1, a, d->rd.scal, SL_G, 0);   /* a: the next step's */
+    }
+    /* both norms down (the Hessenberg column came above) */
+    rc |= !HIPOK(hipMemcpyAsync(hs, d->rd.scal, SL_COUNT * 8, hipMemcpyDeviceToHost, c->s));
+    return cb ? -1 : rc ? -2 : 0;
+}
+
 /* Left-preconditioned GMRES on the projected correction equation
  * (reference src/jd_core.c:568-727).  r, z: device complex128; QR, QL: the
  * ndefl leading columns of the deflation projectors.  Arnoldi is CGS2 with
@@ -653,7 +731,7 @@ static int gmres(ctx_t *c, bbjd_int ndefl, const void *QR, const void *QL, zcx t
     void *V = mixed ? d->gvc : ws->gmres_V;
     const size_t esz = mixed ? 8 : 16;
 #define GCOL(j) ((void *)((char *)V + (size_t)(j) * c->ld * esz))
-    const void *qr = mixed ? d->qrc : QR, *ql = mixed ? d->qlc : QL;
+    const void *ql = mixed ? d->qlc : QL;
 
     c->graphs = d->opt.graphs && d->opt.callback_space == BBJD_CALLBACK_DEVICE;
     if (c->graphs) {
@@ -707,65 +785,50 @@ static int gmres(ctx_t *c, bbjd_int ndefl, const void *QR, const void *QL, zcx t
     memset(g, 0, (size_t)(ims + 1) * sizeof(zcx));
     g[0] = beta0;
 
-    bbjd_int j;
+    /* Lookahead (device callbacks, no graphs): a step needs nothing from the
+     * host, so up to `look` steps are enqueued before the first of them is
+     * read back, and the host tests the reference's stopping rule on each
+     * column as it arrives.  Steps enqueued past the stop are discarded:
+     * they only wrote basis columns past m, the temporaries and reduction
+     * scratch, so the result is the same as stepping one at a time. */
+    const int look = !c->graphs && d->hlook ? g_gmres_look : 1;
+    bbjd_int j, e = 0, fail_at = -1;
     for (j = 0; j < ims; j++) {
-        void *vj1 = GCOL(j + 1);
-        if ((seg = seg_begin(c, (int)j + 1)) < 0) return -2;
-        if (seg == 0) {
-            if (!mixed) {
-                /* t1 = v_j: written with v_j's normalisation (the previous
-                 * step's last launch, or the prologue's) */
-                rc |= bbjd_k_project_out(c->s, n, (int)ndefl, QR, c->ld, t1, &d->rd);
-                if (!rc && apply_ab(c, t1, t2, vj1)) cb = 1;
-                rc |= bbjd_k_lincomb_dev(c->s, n, d->gco, t2, vj1, t2);
-                rc |= bbjd_k_project_out(c->s, n, (int)ndefl, QL, c->ld, t2, &d->rd);
-                if (!rc && !cb && apply_pc(c, t2, vj1)) cb = 1;
-                rc |= bbjd_k_cgs2(c->s, n, (int)(j + 1), V, c->ld, vj1, &d->rd, SL_G, d->rd.hsum);
-                rc |= !HIPOK(hipMemcpyAsync(d->hsmall, d->rd.hsum, (size_t)(j + 1) * 16, hipMemcpyDeviceToHost,
-                                            c->s));
-                rc |= bbjd_k_scale_inv2(c->s, n, vj1, t1, d->rd.scal, SL_G, 0);   /* t1: the next step's */
-            } else {
-                void *a = d->tc[0], *bq = d->tc[1], *e = d->tc[2];
-                /* a = v_j: written with v_j's normalisation */
-                rc |= bbjd_k_project_out_c(c->s, n, (int)ndefl, qr, c->ld, a, &d->rd);
-                if (d->opt.matvec_AB_c64 && d->opt.callback_space == BBJD_CALLBACK_DEVICE) {
-                    hipStream_t prev = tl_cb_stream;
-                    tl_cb_stream = c->s;
-                    if (!rc && d->opt.matvec_AB_c64(n, (const bbjd_cfloat *)a, (bbjd_cfloat *)bq, (bbjd_cfloat *)e,
-                                                    P->op_data))
-                        cb = 1;
-                    tl_cb_stream = prev;
-                } else {
-                    if (!rc && apply_c64(c, d->opt.matvec_A_c64, P->matvec_A, a, bq, t1, t2)) cb = 1;
-                    if (!rc && !cb && apply_c64(c, d->opt.matvec_B_c64, P->matvec_B, a, e, t1, t2)) cb = 1;
-                }
-                rc |= bbjd_k_lincomb_dev_c(c->s, n, d->gco, bq, e, bq);
-                rc |= bbjd_k_project_out_c(c->s, n, (int)ndefl, ql, c->ld, bq, &d->rd);
-                if (P->precond) {
-                    if (!rc && !cb && apply_c64(c, d->opt.precond_c64, (bbjd_matvec_fn)P->precond, bq, vj1, t1, t2))
-                        cb = 1;
-                } else
-                    rc |= !HIPOK(hipMemcpyAsync(vj1, bq, (size_t)n * 8, hipMemcpyDeviceToDevice, c->s));
-                rc |= bbjd_k_cgs2_c(c->s, n, (int)(j + 1), V, c->ld, vj1, &d->rd, SL_G, d->rd.hsum);
-                rc |= !HIPOK(hipMemcpyAsync(d->hsmall, d->rd.hsum, (size_t)(j + 1) * 8, hipMemcpyDeviceToHost,
-                                            c->s));
-                rc |= bbjd_k_scale_inv2_c(c->s, n, vj1, a, d->rd.scal, SL_G, 0);   /* a: the next step's */
+        const void *hh = d->hsmall;
+        const double *hs = d->hscal;
+        if (look > 1) {
+            for (; e < ims && e < j + look && fail_at < 0; e++) {
+                char *slot = d->hlook + (size_t)(e % look) * d->look_slot;
+                const int er = arnoldi_enqueue(c, e, ndefl, QR, QL, V, t1, t2, slot,
+                                               (double *)(slot + d->look_slot - 256));
+                if (er == -2) return -2;
+                if (er == -1) { fail_at = e; break; }
+                if (!HIPOK(hipEventRecord(d->lev[e % look], c->s))) return -2;
             }
-            /* both norms down (the Hessenberg column came above) */
-            rc |= !HIPOK(hipMemcpyAsync(d->hscal, d->rd.scal, SL_COUNT * 8, hipMemcpyDeviceToHost, c->s));
-            rc = seg_end(c, (int)j + 1, rc | cb);
+            if (j == fail_at) return -1;
+            if (!HIPOK(hipEventSynchronize(d->lev[j % look]))) return -2;
+            hh = d->hlook + (size_t)(j % look) * d->look_slot;
+            hs = (const double *)((const char *)hh + d->look_slot - 256);
+        } else {
+            if ((seg = seg_begin(c, (int)j + 1)) < 0) return -2;
+            if (seg == 0) {
+                const int er = arnoldi_enqueue(c, j, ndefl, QR, QL, V, t1, t2, d->hsmall, d->hscal);
+                cb = er == -1;
+                rc = seg_end(c, (int)j + 1, er ? 1 : 0);
+                if (er == -2) rc = -2;
+            }
+            if (cb) return -1;
+            if (rc || !HIPOK(hipStreamSynchronize(c->s))) return -2;
         }
-        if (cb) return -1;
-        if (rc || !HIPOK(hipStreamSynchronize(c->s))) return -2;
         d->st.inner_steps++;
         d->st.stream_bytes += (double)esz * n * (8.0 * ndefl + 4.0 * (j + 1) + 8);
         if (!mixed)
-            for (bbjd_int i = 0; i <= j; i++) H[i + j * ldh] = d->hsmall[i];
+            for (bbjd_int i = 0; i <= j; i++) H[i + j * ldh] = ((const zcx *)hh)[i];
         else {
-            const c64 *hc = (const c64 *)d->hsmall;
+            const c64 *hc = (const c64 *)hh;
             for (bbjd_int i = 0; i <= j; i++) H[i + j * ldh] = hc[i].re + hc[i].im * I;
         }
-        const double wb = d->hscal[SL_NB], wa = d->hscal[SL_G];
+        const double wb = hs[SL_NB], wa = hs[SL_G];
         H[(j + 1) + j * ldh] = wa;
         if (wa < 1e-14 * wb) { j++; break; }                      /* :674 */
         for (bbjd_int i = 0; i < j; i++) {                       /* :680-685 */
@@ -784,6 +847,7 @@ static int gmres(ctx_t *c, bbjd_int ndefl, const void *QR, const void *QL, zcx t
         if (cabs(g[j + 1]) < tol * beta0) { j++; break; }
     }
     const bbjd_int m = j;
+    if (e > m) d->st.inner_steps_discarded += e - m;
     zcx *y = g;                                                    /* :709-716 */
     for (bbjd_int i = m - 1; i >= 0; i--) {
         for (bbjd_int k2 = i + 1; k2 < m; k2++) y[i] -= H[i + k2 * ldh] * y[k2];

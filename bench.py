@@ -269,6 +269,8 @@ def main():
     ap.add_argument("--sync", choices=["yield", "block"], default="yield",
                     help="how host threads wait on the GPU")
     ap.add_argument("--pin", action="store_true", help="pin each serving worker to one CPU")
+    ap.add_argument("--gmres-lookahead", type=int, default=4,
+                    help="Arnoldi steps of the inner GMRES in flight (1: one at a time)")
     ap.add_argument("--operators", choices=["resident", "on-demand"], default="resident",
                     help="on-demand: the server builds each request's operator (host factorization, upload) "
                          "when it admits the request, inside the timed region, charged against free HBM")
@@ -286,13 +288,14 @@ def main():
               "would be placed per GPU without communication; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
-    from bbjd_amd import BandedTauPencil, DeviceOperator
+    from bbjd_amd import BandedTauPencil, DeviceOperator, _lib
     from bbjd_amd import serve as srv
 
     srv.device_init(sync_yield="block" if args.sync == "block" else True)
     # the committed per-shape tile table of this device, when there is one
     from bbjd_amd import autotune
     tt = None if args.no_tile_table else autotune.apply()
+    _lib.load().bbjd_set_gmres_lookahead(args.gmres_lookahead)
     log(f"tile table: {'autotuned ' + tt['tuned'] if tt else 'engine defaults'}")
     wl = WORKLOADS[args.workload]
     if args.nreq <= 0:
@@ -502,7 +505,8 @@ def main():
                    "workers": args.workers, "requests": args.nreq,
                    "admission": args.order + (f" {args.window}" if args.order == "window" else ""),
                    "host_wait": args.sync, "workers_pinned": bool(args.pin),
-                   "projected_step": args.qz_mode, "operators": args.operators},
+                   "projected_step": args.qz_mode, "operators": args.operators,
+                   "gmres_lookahead": args.gmres_lookahead},
         "steps_requested": args.steps,
         "served_eigenpairs_per_s": round(served_rate, 3) if served_rate else None,
         "plain_eigenpairs_per_s": round(plain_rate, 3) if plain_rate else None,

@@ -149,8 +149,10 @@ typedef struct {
                                    ordered Schur form as the reference does */
     bbjd_int qz_device_steps;   /* projected steps run on the device         */
     bbjd_int qz_device_fallbacks; /* of which flagged back to the host      */
-    bbjd_int qz_device_accuracy;  /* of those, flagged for a theta whose
-                                     error bound exceeded 10 tol            */
+    bbjd_int qz_device_accuracy;  /* of those, flagged because the values'
+                                     error bounds could change a choice     */
+    bbjd_int inner_steps_discarded; /* Arnoldi steps run ahead of the inner
+                                       solve's stop and not used            */
 } bbjd_stats;
 
 int bbjd_workspace_stats(const bbjd_workspace *ws, bbjd_stats *st);
@@ -213,6 +215,13 @@ void bbjd_set_rotate_mfma(int on);
 /* Input columns per LDS chunk of the streaming rotation: 8, 16 or 32 (0:
  * chosen from the output width).  For tuning. */
 void bbjd_set_rotate_chunk(int kc);
+
+/* Arnoldi steps of the inner GMRES the host keeps in flight (1..8, default
+ * 4; 1: one step at a time).  With device callbacks and no graphs a step
+ * needs nothing from the host, so the host reads each step's Hessenberg
+ * column while the next ones run; steps past the reference's stop are
+ * discarded, with the same results as one at a time.  Process-wide. */
+void bbjd_set_gmres_lookahead(int steps);
 
 /* Test entry: X[:, :keep] = X[:, :k] Z in place for host arrays X (n x k,
  * leading dimension n) and Z (k x keep, column-major) through the engine's

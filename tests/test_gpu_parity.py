@@ -429,6 +429,40 @@ def test_inner_gmres_graphs_replay_identically(precision, pc):
             assert np.linalg.norm(A @ X[:, j] - graph[0][j] * (B @ X[:, j])) <= 1e-8
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_inner_gmres_lookahead_matches_one_step_at_a_time(precision):
+    """the inner GMRES with 2, 4 and 8 Arnoldi steps in flight
+    (bbjd_set_gmres_lookahead) against one step at a time: identical
+    eigenvalues, vectors, residuals, outer iterations and inner steps used;
+    steps run past the stop are counted and discarded"""
+    lib = _lib.load()
+    p = BandedTauPencil.generate(5000, 4, 1, 2, seed=13)
+    sigma = 0.2 - 0.5j
+    dev = DeviceOperator(p, sigma + 0.4, pc="exact", c64=True)
+    kw = dict(tol=1e-10, maxsub=32, minsub=16, maxiter=300, inner_maxiter=20, inner_tol=1e-3,
+              device_op=dev, precision=precision, return_stats=True)
+    runs = {}
+    try:
+        for look in (1, 2, 4, 8):
+            lib.bbjd_set_gmres_lookahead(look)
+            runs[look] = jdqz_c(p.n, 6, sigma, 0, None, None, True, **kw)
+    finally:
+        lib.bbjd_set_gmres_lookahead(4)
+    one = runs[1]
+    print(f"[lookahead] {precision}: nconv {one[3]} iters {one[4]} inner {one[6]['inner_steps']}; discarded "
+          f"{[runs[k][6]['inner_steps_discarded'] for k in (1, 2, 4, 8)]}", flush=True)
+    assert one[3] == 6 and one[6]["inner_steps"] > 0 and one[6]["inner_steps_discarded"] == 0
+    for look in (2, 4, 8):
+        r = runs[look]
+        assert r[3] == one[3] and r[4] == one[4] and r[5] == one[5]
+        assert r[6]["inner_steps"] == one[6]["inner_steps"]
+        np.testing.assert_array_equal(r[0], one[0])
+        np.testing.assert_array_equal(r[1], one[1])
+        np.testing.assert_array_equal(r[2], one[2])
+    assert 0 < runs[2][6]["inner_steps_discarded"] <= runs[4][6]["inner_steps_discarded"] \
+        <= runs[8][6]["inner_steps_discarded"]
+
+
 def test_correction_gmres_entry_matches_oracle():
     """bbjd_correction_gmres (reference include/bbjd.h:268-282 with the
     argument order of src/jd_core.c:568-582) on host arrays vs the oracle's
