@@ -347,8 +347,11 @@ def main():
             server.run(jobs_for(small_idx, False), realtime=False)
             log(f"warmup step {w} done")
         # ---- timed: served ----
-        reserve = (0 if args.served_only else 30.0) + (0 if args.no_cpu_baseline else 1.6 * args.cpu_seconds + 10) \
-            + (0 if args.no_verify else 15.0)
+        # what the legs after the timed steps take (measured: plain 18 s, the
+        # CPU baseline up to its budget plus one request of ~23 s, the
+        # verification 2 s), with a margin
+        reserve = (0 if args.served_only else 25.0) + (0 if args.no_cpu_baseline else args.cpu_seconds + 25.0) \
+            + (0 if args.no_verify else 6.0)
         for s in range(args.steps):
             elapsed = time.perf_counter() - T_START
             est = max(walls) if walls else 0.0
