@@ -644,8 +644,11 @@ static int seg_end(ctx_t *c, int idx, int rc)
  * (theta_b A - theta_a B)(I - QR QR^H) v_j, its CGS2 against v_0 .. v_j, the
  * new vector normalised (also into the next step's input copy), and copies
  * of the Hessenberg column to hh and of the scalar slots to hs (host).
- * Only device work: the step reads v_j, theta and the norms from device
- * memory.  Returns 0, -1 when a callback failed, -2 on an enqueue error. */
+ * With hs NULL the column is written on the device right before the scalar
+ * slots (the end of the pencil staging, free during the inner solve) and
+ * one copy brings both to hh: the column, then the slots.  Only device
+ * work: the step reads v_j, theta and the norms from device memory.
+ * Returns 0, -1 when a callback failed, -2 on an enqueue error. */
 static int arnoldi_enqueue(ctx_t *c, bbjd_int j, bbjd_int ndefl, const void *QR, const void *QL, void *V, void *t1,
                            void *t2, void *hh, double *hs)
 {
@@ -656,6 +659,8 @@ static int arnoldi_enqueue(ctx_t *c, bbjd_int j, bbjd_int ndefl, const void *QR,
     const size_t esz = mixed ? 8 : 16;
     const void *qr = mixed ? d->qrc : QR, *ql = mixed ? d->qlc : QL;
     void *vj1 = (char *)V + (size_t)(j + 1) * c->ld * esz;
+    const size_t colb = (size_t)(j + 1) * esz;
+    void *hsum = hs ? d->rd.hsum : (void *)((char *)d->rd.scal - colb);
     int rc = 0, cb = 0;
     if (!mixed) {
         /* t1 = v_j: written with v_j's normalisation (the previous step's
@@ -665,8 +670,8 @@ static int arnoldi_enqueue(ctx_t *c, bbjd_int j, bbjd_int ndefl, const void *QR,
         rc |= bbjd_k_lincomb_dev(c->s, n, d->gco, t2, vj1, t2);
         rc |= bbjd_k_project_out(c->s, n, (int)ndefl, QL, c->ld, t2, &d->rd);
         if (!rc && !cb && apply_pc(c, t2, vj1)) cb = 1;
-        rc |= bbjd_k_cgs2(c->s, n, (int)(j + 1), V, c->ld, vj1, &d->rd, SL_G, d->rd.hsum);
-        rc |= !HIPOK(hipMemcpyAsync(hh, d->rd.hsum, (size_t)(j + 1) * 16, hipMemcpyDeviceToHost, c->s));
+        rc |= bbjd_k_cgs2(c->s, n, (int)(j + 1), V, c->ld, vj1, &d->rd, SL_G, hsum);
+        if (hs) rc |= !HIPOK(hipMemcpyAsync(hh, hsum, colb, hipMemcpyDeviceToHost, c->s));
         rc |= bbjd_k_scale_inv2(c->s, n, vj1, t1, d->rd.scal, SL_G, 0);   /* t1: the next step's */
     } else {
         void *a = d->tc[0], *bq = d->tc[1], *e = d->tc[2];
@@ -690,12 +695,13 @@ static int arnoldi_enqueue(ctx_t *c, bbjd_int j, bbjd_int ndefl, const void *QR,
                 cb = 1;
         } else
             rc |= !HIPOK(hipMemcpyAsync(vj1, bq, (size_t)n * 8, hipMemcpyDeviceToDevice, c->s));
-        rc |= bbjd_k_cgs2_c(c->s, n, (int)(j + 1), V, c->ld, vj1, &d->rd, SL_G, d->rd.hsum);
-        rc |= !HIPOK(hipMemcpyAsync(hh, d->rd.hsum, (size_t)(j + 1) * 8, hipMemcpyDeviceToHost, c->s));
+        rc |= bbjd_k_cgs2_c(c->s, n, (int)(j + 1), V, c->ld, vj1, &d->rd, SL_G, hsum);
+        if (hs) rc |= !HIPOK(hipMemcpyAsync(hh, hsum, colb, hipMemcpyDeviceToHost, c->s));
         rc |= bbjd_k_scale_inv2_c(c->s, n, vj1, a, d->rd.scal, SL_G, 0);   /* a: the next step's */
     }
-    /* both norms down (the Hessenberg column came above) */
-    rc |= !HIPOK(hipMemcpyAsync(hs, d->rd.scal, SL_COUNT * 8, hipMemcpyDeviceToHost, c->s));
+    /* both norms down (the Hessenberg column came above, or comes with them) */
+    if (hs) rc |= !HIPOK(hipMemcpyAsync(hs, d->rd.scal, SL_COUNT * 8, hipMemcpyDeviceToHost, c->s));
+    else rc |= !HIPOK(hipMemcpyAsync(hh, hsum, colb + SL_COUNT * 8, hipMemcpyDeviceToHost, c->s));
     return cb ? -1 : rc ? -2 : 0;
 }
 
@@ -792,6 +798,8 @@ static int gmres(ctx_t *c, bbjd_int ndefl, const void *QR, const void *QL, zcx t
      * they only wrote basis columns past m, the temporaries and reduction
      * scratch, so the result is the same as stepping one at a time. */
     const int look = !c->graphs && d->hlook ? g_gmres_look : 1;
+    /* the column fits before the scalar slots: one copy per step */
+    const int joint = (size_t)(ims + 1) * esz <= (size_t)((const char *)d->rd.scal - (const char *)d->pencil);
     bbjd_int j, e = 0, fail_at = -1;
     for (j = 0; j < ims; j++) {
         const void *hh = d->hsmall;
@@ -800,7 +808,7 @@ static int gmres(ctx_t *c, bbjd_int ndefl, const void *QR, const void *QL, zcx t
             for (; e < ims && e < j + look && fail_at < 0; e++) {
                 char *slot = d->hlook + (size_t)(e % look) * d->look_slot;
                 const int er = arnoldi_enqueue(c, e, ndefl, QR, QL, V, t1, t2, slot,
-                                               (double *)(slot + d->look_slot - 256));
+                                               joint ? NULL : (double *)(slot + d->look_slot - 256));
                 if (er == -2) return -2;
                 if (er == -1) { fail_at = e; break; }
                 if (!HIPOK(hipEventRecord(d->lev[e % look], c->s))) return -2;
@@ -808,7 +816,7 @@ static int gmres(ctx_t *c, bbjd_int ndefl, const void *QR, const void *QL, zcx t
             if (j == fail_at) return -1;
             if (!HIPOK(hipEventSynchronize(d->lev[j % look]))) return -2;
             hh = d->hlook + (size_t)(j % look) * d->look_slot;
-            hs = (const double *)((const char *)hh + d->look_slot - 256);
+            hs = (const double *)((const char *)hh + (joint ? (size_t)(j + 1) * esz : d->look_slot - 256));
         } else {
             if ((seg = seg_begin(c, (int)j + 1)) < 0) return -2;
             if (seg == 0) {
