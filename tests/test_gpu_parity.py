@@ -677,6 +677,50 @@ def test_serving_loop_results_alone_equal_mix():
         assert rel_match(ref["evals"], mix[i]["eigenvalues"]) <= 1e-8
 
 
+def test_resident_server_runs_batches_without_allocating():
+    """bbjd_server_create / _run / _destroy (serve.Server): the same batch run
+    twice on one resident server, once with operators built on admission
+    (host pencils) and once with resident device operators, gives
+    eigenvalues bit-identical to the one-shot loop; free device memory does
+    not drop across runs (no workspace allocated per run); the
+    projected step on the device (qz_mode "device") gives the same
+    eigenvalues to 1e-8; a closed server refuses to run"""
+    from bbjd_amd import serve as srv
+    jobs = _serve_jobs(seed=11)[:6]
+    once, _ = srv.serve(jobs, workers=3, realtime=False)
+    ops = [DeviceOperator(j["pencil"], j["sigma"], pc="exact") for j in jobs]
+    resident = [dict(j, op=o) for j, o in zip(jobs, ops)]
+    for r in resident:
+        r.pop("pencil")
+    try:
+        with srv.Server(jobs, workers=3) as server:
+            assert server.workers == 3 and server.workspace_bytes > 0
+            free0 = srv.device_free_bytes()
+            a, rep_a = server.run(jobs, realtime=False)
+            free1 = srv.device_free_bytes()
+            b, rep_b = server.run(resident, realtime=False)
+            free2 = srv.device_free_bytes()
+            print(f"[server] free {free0} -> {free1} -> {free2}; peaks {rep_a['peak_bytes']} {rep_b['peak_bytes']}",
+                  flush=True)
+            # a run allocates no workspace (each is tens of MB here); on-demand
+            # operators are freed when their request ends
+            assert free1 >= free0 - (8 << 20) and free2 >= free1 - (8 << 20)
+            assert rep_a["failed"] == rep_b["failed"] == 0
+            assert rep_b["peak_bytes"] <= rep_a["peak_bytes"]        # nothing to build on admission
+            for i in range(len(jobs)):
+                np.testing.assert_array_equal(a[i]["eigenvalues"], once[i]["eigenvalues"], err_msg=str(i))
+                np.testing.assert_array_equal(b[i]["eigenvalues"], once[i]["eigenvalues"], err_msg=str(i))
+        with pytest.raises(RuntimeError):
+            server.run(jobs)
+        dev, _ = srv.serve(jobs, workers=3, realtime=False, qz_mode="device")
+        for i in range(len(jobs)):
+            assert dev[i]["nconv"] == once[i]["nconv"]
+            assert rel_match(dev[i]["eigenvalues"], once[i]["eigenvalues"]) <= 1e-8
+    finally:
+        for o in ops:
+            o.free()
+
+
 def test_serving_loop_mixed_precision_alone_equal_mix():
     """the small-solve form of the serving loop: GMRES(20) in complex64 with
     the exact LU, FP32 accumulation, four workers; each request's eigenvalues
