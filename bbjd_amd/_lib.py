@@ -56,7 +56,8 @@ class Stats(C.Structure):
                 ("stream_bytes", C.c_double), ("restart_fallbacks", C.c_int64),
                 ("graph_captures", C.c_int64), ("graph_launches", C.c_int64), ("qz_schur_ties", C.c_int64),
                 ("qz_device_steps", C.c_int64), ("qz_device_fallbacks", C.c_int64),
-                ("qz_device_accuracy", C.c_int64), ("inner_steps_discarded", C.c_int64)]
+                ("qz_device_accuracy", C.c_int64), ("inner_steps_discarded", C.c_int64),
+                ("qz_harmonic_restarts", C.c_int64)]
 
 
 class QzProblem(C.Structure):

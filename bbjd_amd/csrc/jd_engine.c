@@ -1321,13 +1321,29 @@ static bbjd_status solve_impl(const bbjd_params *P, bbjd_workspace *ws, bbjd_res
             if (!on_device && schur) {
                 bbjd_int nkeep = minsub - locked;
                 if (nkeep < 1) nkeep = 1;
-                qrc = bbjd_qz_solve(&d->qz, k, MA, MB, (zcx *)ws->SA, (zcx *)ws->SB, (zcx *)ws->ZA,
-                                    (zcx *)ws->ZB, al, be, sigma, P->target, nkeep, eval, R->nconv, tol, &p);
-                if (!qrc) {
-                    ta = al[p];
-                    tb = be[p];
-                    bbjd_ritz_coefficients(k, ldm, p, ta, tb, (const zcx *)ws->SA, (const zcx *)ws->SB,
-                                           (const zcx *)ws->ZB, d->yv, d->xv);
+                /* a restart in RITZ mode: the ordered Schur form of the
+                 * harmonic standard problem (half the generalized QZ's cost),
+                 * unless it defers a choice to the reference's sequence */
+                int harm = 1;
+                if (d->opt.qz_mode == BBJD_QZ_RITZ && k >= ms && !c->w_mixed) {
+                    harm = bbjd_qz_schur_harmonic(&d->qz, k, MA, MB, nu0, mu0, (zcx *)ws->ZB, al, be, sigma,
+                                                  P->target, nkeep, eval, R->nconv, tol, &p, d->xv);
+                    if (harm < 0) qrc = harm;
+                    else if (harm == 0) {
+                        ta = al[p];
+                        tb = be[p];
+                        d->st.qz_harmonic_restarts++;
+                    }
+                }
+                if (harm > 0) {
+                    qrc = bbjd_qz_solve(&d->qz, k, MA, MB, (zcx *)ws->SA, (zcx *)ws->SB, (zcx *)ws->ZA,
+                                        (zcx *)ws->ZB, al, be, sigma, P->target, nkeep, eval, R->nconv, tol, &p);
+                    if (!qrc) {
+                        ta = al[p];
+                        tb = be[p];
+                        bbjd_ritz_coefficients(k, ldm, p, ta, tb, (const zcx *)ws->SA, (const zcx *)ws->SB,
+                                               (const zcx *)ws->ZB, d->yv, d->xv);
+                    }
                 }
             }
             d->st.qz_seconds += now_s() - tq;

@@ -37,6 +37,11 @@ void scipy_zgehrd_(const blas_int *n, const blas_int *ilo, const blas_int *ihi, 
 void scipy_zhseqr_(const char *job, const char *compz, const blas_int *n, const blas_int *ilo, const blas_int *ihi,
                    zcplx *h, const blas_int *ldh, zcplx *w, zcplx *z, const blas_int *ldz, zcplx *work,
                    const blas_int *lwork, blas_int *info);
+void scipy_zunghr_(const blas_int *n, const blas_int *ilo, const blas_int *ihi, zcplx *a, const blas_int *lda,
+                   const zcplx *tau, zcplx *work, const blas_int *lwork, blas_int *info);
+void scipy_ztrsen_(const char *job, const char *compq, const blas_int *sel, const blas_int *n, zcplx *t,
+                   const blas_int *ldt, zcplx *q, const blas_int *ldq, zcplx *w, blas_int *m, double *s,
+                   double *sep, zcplx *work, const blas_int *lwork, blas_int *info);
 void scipy_ztrsm_(const char *side, const char *uplo, const char *transa, const char *diag, const blas_int *m,
                   const blas_int *n, const zcplx *alpha, const zcplx *a, const blas_int *lda, zcplx *b,
                   const blas_int *ldb);

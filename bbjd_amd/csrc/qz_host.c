@@ -89,6 +89,14 @@ int bbjd_qz_init(bbjd_qz *q, bbjd_int maxsub)
         scipy_zhseqr_("E", "N", &bk, &ilo, &ihi, A, &lda, q->cal, NULL, &one, &wq, &lw, &info);
         tl = (blas_int)creal(wq) + 1;
         if (tl > q->lwork) q->lwork = tl;
+        lw = -1;
+        scipy_zhseqr_("S", "V", &bk, &ilo, &ihi, A, &lda, q->cal, B, &lda, &wq, &lw, &info);
+        tl = (blas_int)creal(wq) + 1;
+        if (tl > q->lwork) q->lwork = tl;
+        lw = -1;
+        scipy_zunghr_(&bk, &ilo, &ihi, A, &lda, q->cal, &wq, &lw, &info);
+        tl = (blas_int)creal(wq) + 1;
+        if (tl > q->lwork) q->lwork = tl;
     }
     q->work = malloc((size_t)q->lwork * sizeof(zcplx));
     q->eal = malloc((size_t)maxsub * sizeof(zcplx)); q->ebe = malloc((size_t)maxsub * sizeof(zcplx));
@@ -229,7 +237,11 @@ void bbjd_ritz_coefficients(bbjd_int k, bbjd_int ld, bbjd_int p, zcplx ta, zcplx
  * the harmonic form (see the file comment), with an error bound per value in
  * q->ebnd; returns 1 when C is not triangular enough to use it.
  * alpha / beta = theta. */
-static int eigs_harmonic(bbjd_qz *q, bbjd_int k, const zcplx *MA, const zcplx *MB, zcplx nu0, zcplx mu0)
+/* M = C^-1 MB (ld k, in q->lu) of the harmonic form, or MB^-1 MA when
+ * nu0 = 0, and its Frobenius norm; returns 1 when C is not triangular
+ * enough (or singular) to use it. */
+static int harmonic_matrix(bbjd_qz *q, bbjd_int k, const zcplx *MA, const zcplx *MB, zcplx nu0, zcplx mu0,
+                           double *fro)
 {
     const bbjd_int ld = q->maxsub;
     const int inv = cabs(nu0) > 0;               /* C = nu0 MA + mu0 MB; else C = MB */
@@ -248,13 +260,24 @@ static int eigs_harmonic(bbjd_qz *q, bbjd_int k, const zcplx *MA, const zcplx *M
     /* M = C^-1 MB (or MB^-1 MA), the strictly lower part of C taken as 0 */
     for (bbjd_int j = 0; j < k; j++)
         for (bbjd_int i = 0; i < k; i++) M[i + j * k] = inv ? MB[i + j * ld] : MA[i + j * ld];
-    blas_int bk = (blas_int)k, ilo = 1, ihi = bk, one = 1, lw = q->lwork, info = 0;
+    blas_int bk = (blas_int)k;
     const zcplx a1 = 1.0;
     scipy_ztrsm_("L", "U", "N", "N", &bk, &bk, &a1, C, &bk, M, &bk);
     double mfro = 0;
     for (bbjd_int i = 0; i < k * k; i++) mfro += creal(M[i]) * creal(M[i]) + cimag(M[i]) * cimag(M[i]);
     mfro = sqrt(mfro);
     if (!isfinite(mfro)) return 1;
+    *fro = mfro;
+    return 0;
+}
+
+static int eigs_harmonic(bbjd_qz *q, bbjd_int k, const zcplx *MA, const zcplx *MB, zcplx nu0, zcplx mu0)
+{
+    const int inv = cabs(nu0) > 0;
+    zcplx *M = q->lu;
+    double mfro = 0;
+    if (harmonic_matrix(q, k, MA, MB, nu0, mu0, &mfro)) return 1;
+    blas_int bk = (blas_int)k, ilo = 1, ihi = bk, one = 1, lw = q->lwork, info = 0;
     scipy_zgehrd_(&bk, &ilo, &ihi, M, &bk, q->rhs, q->work, &lw, &info);
     if (info) return 1;
     scipy_zhseqr_("E", "N", &bk, &ilo, &ihi, M, &bk, q->eal, NULL, &one, q->work, &lw, &info);
@@ -361,6 +384,116 @@ static int pick_target(const bbjd_qz *q, bbjd_int k, zcplx sigma, bbjd_target tg
             if (!(key_of(tgt, lam, sigma) - eb[i] > bv + ep)) return 3;
         }
     }
+    return 0;
+}
+
+/* The ordered Schur form for a thick restart through the harmonic standard
+ * problem: M = C^-1 MB = Z T Z^H (ZGEHRD, ZUNGHR, ZHSEQR), the reference's
+ * selection (values within 100 tol of a converged one, then the nkeep best
+ * by key) moved to the leading positions in their order (ZTRSEN), the
+ * target (best unconverged value over all positions) and its Ritz
+ * coefficients x = Z y from T.  M x = mu x is MA x = theta MB x, so the
+ * leading columns of Z span the right deflating subspace that the
+ * reference's ZGGES + ZTGSEN keeps: the restart keeps the same space.  Z
+ * goes to ZB (leading dimension q->maxsub), alpha / beta per position to
+ * al, be.  Returns 0, 1 when the generalized QZ is to decide (C not
+ * triangular, a choice the values' error bounds could change, a tie for the
+ * target or no unconverged candidate), -1 on a LAPACK error. */
+int bbjd_qz_schur_harmonic(bbjd_qz *q, bbjd_int k, const zcplx *MA, const zcplx *MB, zcplx nu0, zcplx mu0,
+                           zcplx *ZB, zcplx *al, zcplx *be, zcplx sigma, bbjd_target tgt, bbjd_int nkeep,
+                           const zcplx *conv, bbjd_int nconv, double tol, bbjd_int *tpos, zcplx *x)
+{
+    const bbjd_int ld = q->maxsub;
+    const int inv = cabs(nu0) > 0;
+    zcplx *T = q->lu;
+    double mfro = 0;
+    q->ecache_valid = 0;                         /* eal, ebe, ebnd are scratch here */
+    if (harmonic_matrix(q, k, MA, MB, nu0, mu0, &mfro)) return 1;
+    blas_int bk = (blas_int)k, ilo = 1, ihi = bk, lw = q->lwork, ldz = (blas_int)ld, info = 0;
+    scipy_zgehrd_(&bk, &ilo, &ihi, T, &bk, q->rhs, q->work, &lw, &info);
+    if (info) return -1;
+    for (bbjd_int j = 0; j < k; j++) memcpy(ZB + j * ld, T + j * k, (size_t)k * sizeof(zcplx));
+    scipy_zunghr_(&bk, &ilo, &ihi, ZB, &ldz, q->rhs, q->work, &lw, &info);
+    if (info) return -1;
+    scipy_zhseqr_("S", "V", &bk, &ilo, &ihi, T, &bk, q->eal, ZB, &ldz, q->work, &lw, &info);
+    if (info) return 1;
+    const double dmu = 4.0 * DBL_EPSILON * mfro, lim = 100.0 * tol;
+    /* theta, its bound and its distance to the converged values per position */
+    double *eb = q->ebnd;
+    for (bbjd_int i = 0; i < k; i++) {
+        const zcplx mu = T[i + i * k];
+        if (!isfinite(creal(mu)) || !isfinite(cimag(mu))) return 1;
+        const double amu = cabs(mu);
+        if (inv) {
+            eb[i] = amu * cabs(nu0) > 1e-300 ? dmu / (cabs(nu0) * amu * amu) : INFINITY;
+            al[i] = 1.0 - mu0 * mu;
+            be[i] = nu0 * mu;
+        } else {
+            eb[i] = dmu;
+            al[i] = mu;
+            be[i] = 1.0;
+        }
+    }
+    /* selection, certain for every value within its bound */
+    blas_int *sel = q->sel;
+    double kmax = -INFINITY, emax = 0;
+    for (bbjd_int i = 0; i < k; i++) {
+        sel[i] = 0;
+        if (cabs(be[i]) < 1e-300) continue;
+        const double m = conv_distance(al[i] / be[i], conv, nconv);
+        if (m < lim) sel[i] = 1;
+        if (fabs(m - lim) <= eb[i]) return 1;
+    }
+    for (bbjd_int pick = 0; pick < nkeep; pick++) {
+        bbjd_int best = -1;
+        double bv = INFINITY;
+        for (bbjd_int i = 0; i < k; i++) {
+            if (sel[i] || cabs(be[i]) < 1e-300) continue;
+            const double v = key_of(tgt, al[i] / be[i], sigma);
+            if (v < bv) { bv = v; best = i; }
+        }
+        if (best < 0) break;
+        sel[best] = 1;
+        if (bv + eb[best] > kmax + emax) { kmax = bv; emax = eb[best]; }
+    }
+    if (kmax > -INFINITY)
+        for (bbjd_int i = 0; i < k; i++)
+            if (!sel[i] && cabs(be[i]) >= 1e-300 && !(key_of(tgt, al[i] / be[i], sigma) - eb[i] > kmax + emax))
+                return 1;
+    blas_int m = 0;
+    double sdum = 0, sepdum = 0;
+    scipy_ztrsen_("N", "V", sel, &bk, T, &bk, ZB, &ldz, q->eal, &m, &sdum, &sepdum, q->work, &lw, &info);
+    if (info) return -1;
+    for (bbjd_int i = 0; i < k; i++) {
+        const zcplx mu = T[i + i * k];
+        const double amu = cabs(mu);
+        if (inv) {
+            eb[i] = amu * cabs(nu0) > 1e-300 ? dmu / (cabs(nu0) * amu * amu) : INFINITY;
+            al[i] = 1.0 - mu0 * mu;
+            be[i] = nu0 * mu;
+        } else {
+            al[i] = mu;
+        }
+    }
+    /* the target by the reference's scan of the ordered form's positions */
+    for (bbjd_int i = 0; i < k; i++) q->ebe[i] = be[i], q->eal[i] = al[i];
+    const int rc = pick_target(q, k, sigma, tgt, conv, nconv, tol, tpos);
+    if (rc) return 1;
+    /* Ritz coefficients: (T(0:p, 0:p) - t_pp I) y = -T(0:p, p), y_p = 1 */
+    const bbjd_int p = *tpos;
+    zcplx *y = q->rhs;
+    const zcplx tpp = T[p + p * k];
+    y[p] = 1.0;
+    for (bbjd_int i = p - 1; i >= 0; i--) {
+        zcplx acc = T[i + p * k];
+        for (bbjd_int j = i + 1; j < p; j++) acc += T[i + j * k] * y[j];
+        const zcplx d = T[i + i * k] - tpp;
+        y[i] = cabs(d) > 1e-300 ? -acc / d : 0.0;
+    }
+    blas_int bp1 = (blas_int)(p + 1), one = 1;
+    const zcplx a1 = 1.0, b0 = 0.0;
+    scipy_zgemv_("N", &bk, &bp1, &a1, ZB, &ldz, y, &one, &b0, x, &one);
+    q->harm_schur_calls++;
     return 0;
 }
 

@@ -22,7 +22,7 @@ typedef struct {
     zcplx    *eal, *ebe, *lu, *rhs, *tri;
     double   *ebnd;             /* error bound of each cached theta (0: ZGGEV) */
     blas_int *ipiv;
-    long      ggev_calls, std_calls;
+    long      ggev_calls, std_calls, harm_schur_calls;
 } bbjd_qz;
 
 int  bbjd_qz_init(bbjd_qz *q, bbjd_int maxsub);
@@ -34,6 +34,9 @@ int  bbjd_qz_solve(bbjd_qz *q, bbjd_int k, const zcplx *MA, const zcplx *MB, zcp
 int  bbjd_qz_target(bbjd_qz *q, bbjd_int k, const zcplx *MA, const zcplx *MB, zcplx nu0, zcplx mu0, zcplx *al,
                     zcplx *be, zcplx sigma, bbjd_target tgt, const zcplx *conv, bbjd_int nconv, double tol,
                     bbjd_int *tpos);
+int  bbjd_qz_schur_harmonic(bbjd_qz *q, bbjd_int k, const zcplx *MA, const zcplx *MB, zcplx nu0, zcplx mu0,
+                            zcplx *ZB, zcplx *al, zcplx *be, zcplx sigma, bbjd_target tgt, bbjd_int nkeep,
+                            const zcplx *conv, bbjd_int nconv, double tol, bbjd_int *tpos, zcplx *x);
 int  bbjd_ritz_vector(bbjd_qz *q, bbjd_int k, const zcplx *MA, const zcplx *MB, zcplx ta, zcplx tb, zcplx *x);
 void bbjd_ritz_coefficients(bbjd_int k, bbjd_int ld, bbjd_int p, zcplx ta, zcplx tb, const zcplx *SA,
                             const zcplx *SB, const zcplx *ZB, zcplx *y, zcplx *x);

@@ -153,6 +153,8 @@ typedef struct {
                                      error bounds could change a choice     */
     bbjd_int inner_steps_discarded; /* Arnoldi steps run ahead of the inner
                                        solve's stop and not used            */
+    bbjd_int qz_harmonic_restarts;  /* restarts whose ordered Schur form came
+                                       from the harmonic standard problem   */
 } bbjd_stats;
 
 int bbjd_workspace_stats(const bbjd_workspace *ws, bbjd_stats *st);

@@ -104,7 +104,7 @@ int main(void) {
   O(bbjd_stats, solve_seconds); O(bbjd_stats, stream_bytes); O(bbjd_stats, restart_fallbacks);
   O(bbjd_stats, graph_captures); O(bbjd_stats, graph_launches); O(bbjd_stats, qz_schur_ties);
   O(bbjd_stats, qz_device_steps); O(bbjd_stats, qz_device_fallbacks); O(bbjd_stats, qz_device_accuracy);
-  O(bbjd_stats, inner_steps_discarded);
+  O(bbjd_stats, inner_steps_discarded); O(bbjd_stats, qz_harmonic_restarts);
   O(bbjd_serve_config, workers); O(bbjd_serve_config, hbm_budget); O(bbjd_serve_config, precision);
   O(bbjd_serve_config, realtime); O(bbjd_serve_config, sync_yield); O(bbjd_serve_config, graphs);
   O(bbjd_serve_config, order); O(bbjd_serve_config, pin); O(bbjd_serve_config, window);

@@ -7,6 +7,7 @@ eigenvalues of (A, B).  Host logic only, no GPU."""
 import ctypes as C
 
 import numpy as np
+import scipy.linalg as sl
 import pytest
 
 from bbjd_amd import _lib
@@ -169,3 +170,90 @@ def test_error_bounds_decide_only_what_they_can_change(tmp_path):
             assert abs(th[p] - best) <= 1e-10
     # both paths occur over these cases
     assert kept >= 10 and ggev >= 5, (kept, ggev)
+
+
+def _solve_ref(MA, MB, sigma, nkeep, conv, tol=1e-10):
+    """bbjd_qz_solve: the reference's ZGGES + selection + ZTGSEN sequence"""
+    lib = _lib.load()
+    k = MA.shape[0]
+    f = lib.bbjd_qz_solve
+    f.restype = C.c_int
+    f.argtypes = [C.c_void_p, C.c_int64] + [C.c_void_p] * 8 + [_lib.zcomplex, C.c_int, C.c_int64, C.c_void_p,
+                                                               C.c_int64, C.c_double, C.POINTER(C.c_int64)]
+    lib.bbjd_qz_init.argtypes = [C.c_void_p, C.c_int64]
+    lib.bbjd_qz_free.argtypes = [C.c_void_p]
+    q = C.create_string_buffer(4096)
+    assert lib.bbjd_qz_init(q, k) == 0
+    try:
+        MAf, MBf = np.asfortranarray(MA), np.asfortranarray(MB)
+        SA, SB, ZA, ZB = (np.zeros((k, k), np.complex128, order="F") for _ in range(4))
+        al, be = np.zeros(k, np.complex128), np.zeros(k, np.complex128)
+        cv = np.ascontiguousarray(np.asarray(conv, np.complex128)) if len(conv) else np.zeros(1, np.complex128)
+        p = C.c_int64(-1)
+        rc = f(q, k, MAf.ctypes.data, MBf.ctypes.data, SA.ctypes.data, SB.ctypes.data, ZA.ctypes.data,
+               ZB.ctypes.data, al.ctypes.data, be.ctypes.data, _lib.zc(sigma), 0, nkeep, cv.ctypes.data, len(conv),
+               tol, C.byref(p))
+        assert rc == 0
+        return al / be, p.value, ZB
+    finally:
+        lib.bbjd_qz_free(q)
+
+
+def _schur_harmonic(MA, MB, nu0, mu0, sigma, nkeep, conv, tol=1e-10):
+    lib = _lib.load()
+    k = MA.shape[0]
+    f = lib.bbjd_qz_schur_harmonic
+    f.restype = C.c_int
+    f.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, _lib.zcomplex, _lib.zcomplex, C.c_void_p,
+                  C.c_void_p, C.c_void_p, _lib.zcomplex, C.c_int, C.c_int64, C.c_void_p, C.c_int64, C.c_double,
+                  C.POINTER(C.c_int64), C.c_void_p]
+    q = C.create_string_buffer(4096)
+    assert lib.bbjd_qz_init(q, k) == 0
+    try:
+        MAf, MBf = np.asfortranarray(MA), np.asfortranarray(MB)
+        Z = np.zeros((k, k), np.complex128, order="F")
+        al, be, x = (np.zeros(k, np.complex128) for _ in range(3))
+        cv = np.ascontiguousarray(np.asarray(conv, np.complex128)) if len(conv) else np.zeros(1, np.complex128)
+        p = C.c_int64(-1)
+        rc = f(q, k, MAf.ctypes.data, MBf.ctypes.data, _lib.zc(nu0), _lib.zc(mu0), Z.ctypes.data, al.ctypes.data,
+               be.ctypes.data, _lib.zc(sigma), 0, nkeep, cv.ctypes.data, len(conv), tol, C.byref(p), x.ctypes.data)
+        return rc, al / be, p.value, Z, x
+    finally:
+        lib.bbjd_qz_free(q)
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_harmonic_restart_keeps_the_reference_subspace(seed):
+    """the restart's ordered Schur form from the harmonic standard problem
+    (bbjd_qz_schur_harmonic) against the reference's ZGGES + ZTGSEN sequence
+    (bbjd_qz_solve) on projected pencils of random complex ones, with and
+    without converged values: the same selected values in the leading
+    positions (1e-10), the same kept subspace (every principal angle between
+    the leading columns below 1e-9), the same target value, Ritz coefficients
+    with MA x = theta MB x (1e-10 relative), Z unitary"""
+    rng = np.random.default_rng(40 + seed)
+    n, k = 60, int(rng.integers(8, 40))
+    d = rng.standard_normal(n) * 2 + 1j * rng.standard_normal(n) * 2
+    sigma = complex(rng.standard_normal(), rng.standard_normal())
+    V, _ = np.linalg.qr(rng.standard_normal((n, k)) + 1j * rng.standard_normal((n, k)))
+    A = np.diag(d) + 0.3 * (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+    nu0 = 1 / np.sqrt(1 + abs(sigma) ** 2)
+    mu0 = -sigma * nu0
+    W, _ = np.linalg.qr((nu0 * A + mu0 * np.eye(n)) @ V)
+    MA, MB = W.conj().T @ A @ V, W.conj().T @ V
+    th = sl.eigvals(MA, MB)
+    conv = [th[np.argsort(np.abs(th - sigma))[1]]] if seed % 2 else []
+    nkeep = max(1, k // 3)
+    rc, hth, hp, Z, x = _schur_harmonic(MA, MB, nu0, mu0, sigma, nkeep, conv)
+    assert rc == 0
+    rth, rp, ZB = _solve_ref(MA, MB, sigma, nkeep, conv)
+    m = nkeep + len(conv)
+    assert np.sort_complex(np.round(hth[:m], 8)).tolist() == np.sort_complex(np.round(rth[:m], 8)).tolist()
+    for t in hth[:m]:
+        assert np.min(np.abs(rth[:m] - t)) <= 1e-10 * max(1, abs(t))
+    assert np.abs(Z.conj().T @ Z - np.eye(k)).max() <= 1e-13
+    s = np.linalg.svd(Z[:, :m].conj().T @ ZB[:, :m], compute_uv=False)
+    assert np.all(s >= 1 - 1e-9)                         # same subspace
+    assert abs(hth[hp] - rth[rp]) <= 1e-10 * max(1, abs(rth[rp]))
+    r = MA @ x - hth[hp] * (MB @ x)
+    assert np.linalg.norm(r) <= 1e-10 * np.linalg.norm(MA) * np.linalg.norm(x)
