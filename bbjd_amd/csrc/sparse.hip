@@ -5,6 +5,11 @@
 // 64-lane wavefront, from the average row length) per row, partial sums
 // reduced across the group by shuffles in a fixed order.
 //
+// With dense rows pinned (bbjd_sparse_border) the factors are those of
+// M11 = M[R', R'] and the pinned rows enter through the Schur complement:
+// one block per pinned row sums M21 z in a fixed tree, and one pass forms
+// y_R = Sinv (x_R - M21 z) in every block and y_R' = z - Y y_R.
+//
 // M^-1 b = Pc U^-1 L^-1 Pr b from host-computed LU factors: a gather applies
 // Pr into a scratch vector, the two triangular solves run in place on it,
 // a gather applies Pc into y.  A triangular solve is ordered by level sets:
@@ -124,6 +129,55 @@ __global__ void k_copy(i64 n, const Z *__restrict__ src, Z *__restrict__ dst)
         dst[i] = src[i];
 }
 
+// Pinned dense rows: t[r] = sum_j M21[r, j] z[j], one block per row, the
+// lanes' partial sums reduced in a fixed tree.
+__global__ __launch_bounds__(kTpb) void k_border_dots(i64 m, const Z *__restrict__ M21, const Z *__restrict__ z,
+                                                      Z *__restrict__ t)
+{
+    __shared__ double sr[kTpb], si[kTpb];
+    const Z *row = M21 + (i64)blockIdx.x * m;
+    Z acc = {0.0, 0.0};
+    for (i64 j = threadIdx.x; j < m; j += kTpb) acc = zfma(acc, row[j], z[j]);
+    sr[threadIdx.x] = acc.re;
+    si[threadIdx.x] = acc.im;
+    __syncthreads();
+    for (int h = kTpb / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) {
+            sr[threadIdx.x] += sr[threadIdx.x + h];
+            si[threadIdx.x] += si[threadIdx.x + h];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) t[blockIdx.x] = Z{sr[0], si[0]};
+}
+
+// y_R = Sinv (x_R - t), computed by every block (d <= 64), then
+// y[rest[i]] = z[i] - sum_j Y[i + j m] y_R[j] and, from block 0, y[R_j] = y_R[j].
+__global__ __launch_bounds__(kTpb) void k_border_combine(i64 m, int d, const int *__restrict__ rows,
+                                                         const int *__restrict__ rest, const Z *__restrict__ Sinv,
+                                                         const Z *__restrict__ Y, const Z *__restrict__ t,
+                                                         const Z *__restrict__ xr, const Z *__restrict__ z,
+                                                         Z *__restrict__ y)
+{
+    __shared__ Z rhs[64], y2[64];
+    const int tid = threadIdx.x;
+    if (tid < d) rhs[tid] = Z{xr[tid].re - t[tid].re, xr[tid].im - t[tid].im};
+    __syncthreads();
+    if (tid < d) {
+        Z a = {0.0, 0.0};
+        for (int c = 0; c < d; c++) a = zfma(a, Sinv[tid + (i64)c * d], rhs[c]);
+        y2[tid] = a;
+        if (blockIdx.x == 0) y[rows[tid]] = a;
+    }
+    __syncthreads();
+    for (i64 i = (i64)blockIdx.x * kTpb + tid; i < m; i += (i64)gridDim.x * kTpb) {
+        Z a = {0.0, 0.0};
+        for (int j = 0; j < d; j++) a = zfma(a, Y[i + (i64)j * m], y2[j]);
+        const Z zi = z[i];
+        y[rest[i]] = Z{zi.re - a.re, zi.im - a.im};
+    }
+}
+
 inline int grid_for(i64 work, int per_block)
 {
     i64 b = (work + per_block - 1) / per_block;
@@ -175,6 +229,11 @@ struct bbjd_sparse_op {
     DTri L, U;
     int *iperm_r = nullptr, *perm_c = nullptr;
     Z *s = nullptr;
+    // pinned dense rows (bbjd_sparse_border): the LU is M11's, of size m
+    int has_border = 0, d = 0;
+    i64 m = 0;
+    int *rows = nullptr, *rest = nullptr;
+    Z *M21 = nullptr, *Y = nullptr, *Sinv = nullptr, *t = nullptr, *xr = nullptr, *z = nullptr;
 };
 
 static hipStream_t sp_stream(const bbjd_sparse_op *op)
@@ -366,20 +425,38 @@ static void sparse_free_impl(bbjd_sparse_op *op)
 
 extern "C" {
 
-bbjd_sparse_op *bbjd_sparse_op_create(const bbjd_csr *A, const bbjd_csr *B, const bbjd_sparse_lu *lu, void *stream)
+static bbjd_sparse_op *create_impl(const bbjd_csr *A, const bbjd_csr *B, const bbjd_sparse_lu *lu,
+                                   const bbjd_sparse_border *bd, void *stream)
 {
     if (!A || A->n <= 0 || A->n > 0x7fffffff) return nullptr;
     const i64 n = A->n;
     if (!csr_ok(A, n, n) || (B && !csr_ok(B, n, n))) return nullptr;
+    // pinned rows: R increasing in [0, n), the rest R' in order; the LU is
+    // M11's (size m = n - d) and its row permutation is composed with R'
+    const int d = bd ? (int)bd->d : 0;
+    std::vector<int> rest;
+    if (bd) {
+        if (!lu || d < 1 || d > 64 || d >= n || !bd->rows || !bd->M21 || !bd->Y || !bd->Sinv) return nullptr;
+        for (int j = 0; j < d; j++)
+            if (bd->rows[j] < 0 || bd->rows[j] >= n || (j > 0 && bd->rows[j] <= bd->rows[j - 1])) return nullptr;
+        rest.reserve((size_t)(n - d));
+        for (i64 i = 0, j = 0; i < n; i++) {
+            if (j < d && bd->rows[j] == i) { j++; continue; }
+            rest.push_back((int)i);
+        }
+    }
+    const i64 m = n - d;
     HTri hl, hu;
     std::vector<int> iperm_r, perm_c;
     if (lu) {
-        if (!csr_ok(&lu->L, n, n) || !csr_ok(&lu->U, n, n) || !perm_ok(lu->perm_r, n) || !perm_ok(lu->perm_c, n))
+        if (!csr_ok(&lu->L, m, m) || !csr_ok(&lu->U, m, m) || !perm_ok(lu->perm_r, m) || !perm_ok(lu->perm_c, m))
             return nullptr;
-        if (!plan_tri(&lu->L, n, 1, hl) || !plan_tri(&lu->U, n, 0, hu)) return nullptr;
-        iperm_r.assign((size_t)n, 0);
-        perm_c.assign(lu->perm_c, lu->perm_c + n);
-        for (i64 i = 0; i < n; i++) iperm_r[(size_t)lu->perm_r[i]] = (int)i;
+        if (!plan_tri(&lu->L, m, 1, hl) || !plan_tri(&lu->U, m, 0, hu)) return nullptr;
+        iperm_r.assign((size_t)m, 0);
+        perm_c.assign(lu->perm_c, lu->perm_c + m);
+        for (i64 i = 0; i < m; i++) iperm_r[(size_t)lu->perm_r[i]] = (int)i;
+        if (bd)
+            for (i64 i = 0; i < m; i++) iperm_r[(size_t)i] = rest[(size_t)iperm_r[(size_t)i]];
     }
     const i64 nnzA = A->ptr[n], nnzB = B ? B->ptr[n] : 0;
     Layout L;
@@ -394,12 +471,18 @@ bbjd_sparse_op *bbjd_sparse_op_create(const bbjd_csr *A, const bbjd_csr *B, cons
             oT[t][0] = L.take(ht[t]->ptr.size() * 8);
             oT[t][1] = L.take(ht[t]->col.size() * 4);
             oT[t][2] = L.take(ht[t]->val.size() * 16);
-            oT[t][3] = L.take((size_t)n * 16);
-            oT[t][4] = L.take((size_t)n * 4);
+            oT[t][3] = L.take((size_t)m * 16);
+            oT[t][4] = L.take((size_t)m * 4);
             oT[t][5] = L.take(ht[t]->lvl.size() * 8);
         }
-    const size_t oIr = lu ? L.take((size_t)n * 4) : 0, oPc = lu ? L.take((size_t)n * 4) : 0;
+    const size_t oIr = lu ? L.take((size_t)m * 4) : 0, oPc = lu ? L.take((size_t)m * 4) : 0;
     const size_t oS = L.take((size_t)n * 16);
+    size_t oRows = 0, oRest = 0, oM21 = 0, oY = 0, oSinv = 0, oT2 = 0, oXr = 0, oZ = 0;
+    if (bd) {
+        oRows = L.take((size_t)d * 4); oRest = L.take((size_t)m * 4);
+        oM21 = L.take((size_t)d * m * 16); oY = L.take((size_t)m * d * 16); oSinv = L.take((size_t)d * d * 16);
+        oT2 = L.take((size_t)d * 16); oXr = L.take((size_t)d * 16); oZ = L.take((size_t)m * 16);
+    }
 
     bbjd_sparse_op *op = new (std::nothrow) bbjd_sparse_op();
     if (!op) return nullptr;
@@ -442,20 +525,41 @@ bbjd_sparse_op *bbjd_sparse_op_create(const bbjd_csr *A, const bbjd_csr *B, cons
             up(oT[t][0], h.ptr.data(), h.ptr.size() * 8);
             up(oT[t][1], h.col.data(), h.col.size() * 4);
             up(oT[t][2], h.val.data(), h.val.size() * 16);
-            up(oT[t][3], h.dinv.data(), (size_t)n * 16);
-            up(oT[t][4], h.order.data(), (size_t)n * 4);
+            up(oT[t][3], h.dinv.data(), (size_t)m * 16);
+            up(oT[t][4], h.order.data(), (size_t)m * 4);
             up(oT[t][5], h.lvl.data(), h.lvl.size() * 8);
         }
         op->iperm_r = (int *)(b + oIr);
         op->perm_c = (int *)(b + oPc);
-        up(oIr, iperm_r.data(), (size_t)n * 4);
-        up(oPc, perm_c.data(), (size_t)n * 4);
+        up(oIr, iperm_r.data(), (size_t)m * 4);
+        up(oPc, perm_c.data(), (size_t)m * 4);
     }
     op->s = (Z *)(b + oS);
+    if (bd) {
+        op->has_border = 1; op->d = d; op->m = m;
+        op->rows = (int *)(b + oRows); op->rest = (int *)(b + oRest);
+        op->M21 = (Z *)(b + oM21); op->Y = (Z *)(b + oY); op->Sinv = (Z *)(b + oSinv);
+        op->t = (Z *)(b + oT2); op->xr = (Z *)(b + oXr); op->z = (Z *)(b + oZ);
+        up(oRows, bd->rows, (size_t)d * 4); up(oRest, rest.data(), (size_t)m * 4);
+        up(oM21, bd->M21, (size_t)d * m * 16); up(oY, bd->Y, (size_t)m * d * 16);
+        up(oSinv, bd->Sinv, (size_t)d * d * 16);
+    }
     // the host arrays go out of scope: the copies must have completed
     rc |= hipStreamSynchronize(st) != hipSuccess;
     if (rc) { sparse_free_impl(op); return nullptr; }
     return op;
+}
+
+bbjd_sparse_op *bbjd_sparse_op_create(const bbjd_csr *A, const bbjd_csr *B, const bbjd_sparse_lu *lu, void *stream)
+{
+    return create_impl(A, B, lu, nullptr, stream);
+}
+
+bbjd_sparse_op *bbjd_sparse_op_create_bordered(const bbjd_csr *A, const bbjd_csr *B, const bbjd_sparse_lu *lu,
+                                               const bbjd_sparse_border *border, void *stream)
+{
+    if (!border) return nullptr;
+    return create_impl(A, B, lu, border, stream);
 }
 
 void bbjd_sparse_op_free(bbjd_sparse_op *op)
@@ -472,7 +576,8 @@ int bbjd_sparse_op_levels(const bbjd_sparse_op *op, int64_t *levels_L, int64_t *
     if (!op) return -1;
     if (levels_L) *levels_L = op->has_lu ? op->L.nlev : 0;
     if (levels_U) *levels_U = op->has_lu ? op->U.nlev : 0;
-    if (launches) *launches = op->has_lu ? (int64_t)(op->L.segs.size() + op->U.segs.size() + 2) : 1;
+    if (launches)
+        *launches = op->has_lu ? (int64_t)(op->L.segs.size() + op->U.segs.size() + (op->has_border ? 5 : 2)) : 1;
     return 0;
 }
 
@@ -501,6 +606,23 @@ int bbjd_sparse_precond(bbjd_int n, const bbjd_zomplex *x, bbjd_zomplex *y, void
     const int g = grid_for(n, kTpb);
     if (!op->has_lu) {
         hipLaunchKernelGGL(k_copy, dim3(g), dim3(kTpb), 0, st, (i64)n, (const Z *)x, (Z *)y);
+        return hipGetLastError() == hipSuccess ? 0 : -1;
+    }
+    if (op->has_border) {
+        // s = Pr x[R'] and x_R read first (y may be x), s <- U^-1 L^-1 s,
+        // z = Pc s = M11^-1 x[R'], t = M21 z, then the Schur complement
+        const i64 m = op->m;
+        const int gm = grid_for(m, kTpb);
+        hipLaunchKernelGGL(k_gather, dim3(gm), dim3(kTpb), 0, st, m, (const int *)op->iperm_r, (const Z *)x, op->s);
+        hipLaunchKernelGGL(k_gather, dim3(1), dim3(64), 0, st, (i64)op->d, (const int *)op->rows, (const Z *)x, op->xr);
+        if (hipGetLastError() != hipSuccess) return -1;
+        if (launch_tri(st, op->L, op->s) || launch_tri(st, op->U, op->s)) return -1;
+        hipLaunchKernelGGL(k_gather, dim3(gm), dim3(kTpb), 0, st, m, (const int *)op->perm_c, (const Z *)op->s, op->z);
+        hipLaunchKernelGGL(k_border_dots, dim3(op->d), dim3(kTpb), 0, st, m, (const Z *)op->M21, (const Z *)op->z,
+                           op->t);
+        hipLaunchKernelGGL(k_border_combine, dim3(gm), dim3(kTpb), 0, st, m, op->d, (const int *)op->rows,
+                           (const int *)op->rest, (const Z *)op->Sinv, (const Z *)op->Y, (const Z *)op->t,
+                           (const Z *)op->xr, (const Z *)op->z, (Z *)y);
         return hipGetLastError() == hipSuccess ? 0 : -1;
     }
     // s = Pr x, s <- L^-1 s, s <- U^-1 s, y = Pc s

@@ -39,6 +39,23 @@ typedef struct {
     const int32_t *perm_r, *perm_c;
 } bbjd_sparse_lu;
 
+/* Dense rows pinned out of the sparse LU, as FPLUS (the reference's
+ * recommended preconditioner, README.md:39-44) does: a row with far more
+ * nonzeros than the rest makes a pivoting sparse LU fill densely (SuperLU on
+ * a banded pencil with 4 dense rows: nnz(U) 32 M at n = 8000 against 0.12 M
+ * for the band part).  With R the d pinned indices (those rows and the same
+ * columns) and R' the rest, M11 = M[R', R'] is factored on its own and
+ *     y_R  = S^-1 (b_R - M21 M11^-1 b_R'),   S = M22 - M21 Y,
+ *     y_R' = M11^-1 b_R' - Y y_R,            Y = M11^-1 M12,
+ * with M21 = M[R, R'], M12 = M[R', R], M22 = M[R, R]. */
+typedef struct {
+    bbjd_int            d;       /* pinned rows, 1 <= d <= 64                */
+    const int32_t      *rows;    /* R, increasing, in [0, n)                 */
+    const bbjd_zomplex *M21;     /* d x (n - d), row-major                   */
+    const bbjd_zomplex *Y;       /* (n - d) x d, column-major                */
+    const bbjd_zomplex *Sinv;    /* d x d, column-major                      */
+} bbjd_sparse_border;
+
 typedef struct bbjd_sparse_op bbjd_sparse_op;
 
 /* Uploads A, B (NULL: the identity) and the LU factors (NULL: no
@@ -49,6 +66,10 @@ typedef struct bbjd_sparse_op bbjd_sparse_op;
  * device error. */
 bbjd_sparse_op *bbjd_sparse_op_create(const bbjd_csr *A, const bbjd_csr *B, const bbjd_sparse_lu *lu,
                                       void *stream);
+/* The same with the preconditioner's dense rows pinned: lu factors M11 (of
+ * size n - d, in the order of R', the indices not in border->rows). */
+bbjd_sparse_op *bbjd_sparse_op_create_bordered(const bbjd_csr *A, const bbjd_csr *B, const bbjd_sparse_lu *lu,
+                                               const bbjd_sparse_border *border, void *stream);
 void     bbjd_sparse_op_free(bbjd_sparse_op *op);
 size_t   bbjd_sparse_op_bytes(const bbjd_sparse_op *op);   /* device bytes held */
 /* level sets of the two triangular solves and the launches each takes */
