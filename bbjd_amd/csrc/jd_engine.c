@@ -1321,11 +1321,12 @@ static bbjd_status solve_impl(const bbjd_params *P, bbjd_workspace *ws, bbjd_res
             if (!on_device && schur) {
                 bbjd_int nkeep = minsub - locked;
                 if (nkeep < 1) nkeep = 1;
-                /* a restart in RITZ mode: the ordered Schur form of the
-                 * harmonic standard problem (half the generalized QZ's cost),
-                 * unless it defers a choice to the reference's sequence */
+                /* a restart (RITZ mode, or DEVICE mode where the device step
+                 * did not apply): the ordered Schur form of the harmonic
+                 * standard problem (half the generalized QZ's cost), unless it
+                 * defers a choice to the reference's sequence */
                 int harm = 1;
-                if (d->opt.qz_mode == BBJD_QZ_RITZ && k >= ms && !c->w_mixed) {
+                if (d->opt.qz_mode != BBJD_QZ_SCHUR && k >= ms && !c->w_mixed) {
                     harm = bbjd_qz_schur_harmonic(&d->qz, k, MA, MB, nu0, mu0, (zcx *)ws->ZB, al, be, sigma,
                                                   P->target, nkeep, eval, R->nconv, tol, &p, d->xv);
                     if (harm < 0) qrc = harm;
