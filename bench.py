@@ -271,6 +271,8 @@ def main():
     ap.add_argument("--pin", action="store_true", help="pin each serving worker to one CPU")
     ap.add_argument("--gmres-lookahead", type=int, default=4,
                     help="Arnoldi steps of the inner GMRES in flight (1: one at a time)")
+    ap.add_argument("--proj-blocks", type=int, default=0,
+                    help="blocks per projection launch for every shape (0: the tile table or the engine default)")
     ap.add_argument("--fuse-blocks", type=int, default=0,
                     help="finish reductions in the last block for grids up to this many blocks (0: engine default)")
     ap.add_argument("--operators", choices=["resident", "on-demand"], default="resident",
@@ -300,6 +302,9 @@ def main():
     _lib.load().bbjd_set_gmres_lookahead(args.gmres_lookahead)
     if args.fuse_blocks > 0:
         _lib.load().bbjd_set_fuse_blocks(args.fuse_blocks)
+    if args.proj_blocks > 0:
+        _lib.load().bbjd_set_tile_table(0, None, 0, None, None, None)
+        _lib.load().bbjd_set_tiles(args.proj_blocks, 0, 0)
     log(f"tile table: {'autotuned ' + tt['tuned'] if tt else 'engine defaults'}")
     wl = WORKLOADS[args.workload]
     if args.nreq <= 0:
