@@ -18,7 +18,10 @@ Python callables (no host round trip per apply); ``precision`` -- 'fp64' or 'mix
 'schur' (every iteration, the reference's sequence) or 'device' (the
 projected step on the GPU for k <= 64); ``graphs`` -- with
 ``device_op``, capture the inner GMRES steps into HIP graphs and replay
-them (same kernels, same results).
+them (same kernels, same results); ``inner_form`` -- with precision
+'mixed', 'reference' (the reference's GMRES, two-sided projections) or
+'projected' (the projected preconditioner: one oblique projection per
+step, bbjd_gpu_options.inner_form).
 """
 from __future__ import annotations
 
@@ -56,7 +59,8 @@ def _trampoline(fn, errbox):
 
 def jdqz_c(n, nev, sigma, target, matvec_A, matvec_B, precond_solve, tol=1e-10, maxiter=200,
            maxsub=0, minsub=0, inner_maxiter=0, inner_tol=1e-2, verbose=0, v0=None, *,
-           device_op=None, precision="fp64", return_stats=False, qz_mode="ritz", graphs=False):
+           device_op=None, precision="fp64", return_stats=False, qz_mode="ritz", graphs=False,
+           inner_form="reference"):
     lib = _lib.load()
     n, nev = int(n), int(nev)
     p = _lib.Params()
@@ -79,6 +83,9 @@ def jdqz_c(n, nev, sigma, target, matvec_A, matvec_B, precond_solve, tol=1e-10, 
         raise ValueError("qz_mode must be 'ritz', 'schur' or 'device'")
     opts.qz_mode = {"ritz": _lib.QZ_RITZ, "schur": _lib.QZ_SCHUR, "device": _lib.QZ_DEVICE}[qz_mode]
     opts.graphs = int(bool(graphs))
+    if inner_form not in ("reference", "projected"):
+        raise ValueError("inner_form must be 'reference' or 'projected'")
+    opts.inner_form = _lib.INNER_PROJECTED if inner_form == "projected" else _lib.INNER_REFERENCE
     errbox: list = []
     keep = []
     if device_op is not None:

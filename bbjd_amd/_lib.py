@@ -39,6 +39,7 @@ class Result(C.Structure):
 
 
 QZ_RITZ, QZ_SCHUR, QZ_DEVICE = 0, 1, 2
+INNER_REFERENCE, INNER_PROJECTED = 0, 1
 
 
 class GpuOptions(C.Structure):
@@ -46,7 +47,7 @@ class GpuOptions(C.Structure):
                 ("matvec_A_c64", C.c_void_p), ("matvec_B_c64", C.c_void_p),
                 ("precond_c64", C.c_void_p), ("precond2", C.c_void_p), ("qz_mode", C.c_int),
                 ("graphs", C.c_int), ("matvec_AB", C.c_void_p), ("matvec_AB_c64", C.c_void_p),
-                ("matvec_ABr", C.c_void_p)]
+                ("matvec_ABr", C.c_void_p), ("inner_form", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -57,7 +58,8 @@ class Stats(C.Structure):
                 ("graph_captures", C.c_int64), ("graph_launches", C.c_int64), ("qz_schur_ties", C.c_int64),
                 ("qz_device_steps", C.c_int64), ("qz_device_fallbacks", C.c_int64),
                 ("qz_device_accuracy", C.c_int64), ("inner_steps_discarded", C.c_int64),
-                ("qz_harmonic_restarts", C.c_int64)]
+                ("qz_harmonic_restarts", C.c_int64), ("inner_projected", C.c_int64),
+                ("inner_projected_fallbacks", C.c_int64)]
 
 
 class QzProblem(C.Structure):

@@ -151,6 +151,11 @@ int bbjd_k_rotate(hipStream_t s, int64_t n, int k, int keep, void *X, int64_t ld
 int bbjd_k_gram(hipStream_t s, int64_t n, int p, const void *X, int64_t ldx, int q, const void *Y, int64_t ldy,
                 void *G, int64_t ldg, bbjd_red *rd);
 
+/* x -= Y[:, :k] (M c) in complex64 with FP32 accumulation: M is k x k,
+ * column-major, c has k entries (device arrays) */
+int bbjd_k_update_pre_c(hipStream_t s, int64_t n, int k, const void *Y, int64_t ldy, const void *M, const void *c,
+                        void *x);
+
 /* precision conversion */
 int bbjd_k_z2c(hipStream_t s, int64_t n, const void *z, void *c);
 int bbjd_k_c2z(hipStream_t s, int64_t n, const void *c, void *z);

@@ -50,6 +50,15 @@ struct bbjd_dev {
     void            *gvc;              /* complex64 basis ld x (ims + 1)   */
     void            *qrc, *qlc;        /* complex64 projectors             */
     void            *tc[3];            /* complex64 n-vectors               */
+    /* projected-preconditioner inner solve (BBJD_INNER_PROJECTED): Yh =
+     * M^-1 Q_left (complex64, nev + 1 columns), H^-1 (complex64, ndefl^2),
+     * staging of H's new entries; host: H (complex128, ld nev + 1), pinned
+     * copies; pp_valid = leading columns of Yh and H that hold locked
+     * columns (the last one of a call is the current Ritz pair's, redone) */
+    void            *yc, *hinv, *hnew;
+    zcx             *hh;
+    c64             *hpp;
+    bbjd_int         pp_ld, pp_valid;
     /* host side */
     double          *hscal;            /* pinned: SL_COUNT doubles + flags  */
     zcx             *hsmall;           /* pinned: small vector staging      */
@@ -66,7 +75,7 @@ struct bbjd_dev {
      * GMRES prologue, [1 + j] Arnoldi step j, valid for gkey */
     hipGraphExec_t  *gx;
     int              gx_cap;
-    struct gkey { bbjd_int ndefl, n, ld; const void *r, *z, *qr, *ql, *op; } gkey;
+    struct gkey { bbjd_int ndefl, n, ld; const void *r, *z, *qr, *ql, *op; int pp; } gkey;
     double          *gco;              /* device: theta_b, -theta_a         */
     double          *hgco;             /* pinned: their staging             */
     /* the projected step on the device (qz_mode BBJD_QZ_DEVICE): MA, MB,
@@ -100,7 +109,7 @@ static double now_s(void)
 /* ------------------------------------------------------------------ */
 typedef struct {
     bbjd_int n, ld, maxsub, nev, ims;
-    int mixed;
+    int mixed, proj;
     size_t off[48];
     size_t total;
 } layout_t;
@@ -109,7 +118,7 @@ typedef struct {
  * come back to the host in one copy */
 enum { O_V, O_W, O_AV, O_BV, O_QR, O_QL, O_GV, O_T1, O_T2, O_T3, O_T4, O_HW, O_PART, O_PNORM, O_H, O_H2,
        O_HSUM, O_PEN, O_SCAL, O_FLAG, O_CNT, O_COEF, O_COEF2, O_ZR, O_GRAM, O_GVC, O_QRC, O_QLC, O_TC0, O_TC1,
-       O_TC2, O_GCO, O_QZD, O_END };
+       O_TC2, O_GCO, O_QZD, O_YC, O_HINV, O_HNEW, O_END };
 _Static_assert(O_END <= 48, "layout_t.off holds every sub-buffer");
 
 static bbjd_int auto_maxsub(const bbjd_params *p)
@@ -138,6 +147,7 @@ static void make_layout(layout_t *L, const bbjd_params *p, const bbjd_gpu_option
     L->nev = p->nev;
     L->ims = p->inner_maxiter > 0 ? p->inner_maxiter : 1;
     L->mixed = o && o->precision == BBJD_PREC_MIXED;
+    L->proj = L->mixed && o->inner_form == BBJD_INNER_PROJECTED && p->inner_maxiter > 0;
     const size_t Z = 16, C = 8, ld = (size_t)L->ld, ms = (size_t)L->maxsub;
     size_t sz[O_END];
     memset(sz, 0, sizeof(sz));
@@ -165,6 +175,14 @@ static void make_layout(layout_t *L, const bbjd_params *p, const bbjd_gpu_option
         sz[O_GVC] = ld * (size_t)(L->ims + 1) * C;
         sz[O_QRC] = sz[O_QLC] = ld * (size_t)(L->nev + 1) * C;
         sz[O_TC0] = sz[O_TC1] = sz[O_TC2] = ld * C;
+    }
+    if (L->proj) {
+        /* projected preconditioner: M^-1 Q_left (complex64), H^-1 and the
+         * new entries of H */
+        const size_t K = (size_t)L->nev + 1;
+        sz[O_YC] = ld * K * C;
+        sz[O_HINV] = K * K * C;
+        sz[O_HNEW] = 2 * K * K * C;
     }
     size_t off = 0;
     for (int i = 0; i < O_END; i++) {
@@ -257,6 +275,8 @@ static void workspace_free_impl(bbjd_workspace *ws)
         if (d->zrh) hipHostFree(d->zrh);
         if (d->hcol) hipHostFree(d->hcol);
         if (d->hgram) hipHostFree(d->hgram);
+        if (d->hpp) hipHostFree(d->hpp);
+        free(d->hh);
         free(d->al); free(d->be); free(d->yv); free(d->xv); free(d->pair_res);
         bbjd_qz_free(&d->qz);
         free(d);
@@ -327,6 +347,10 @@ static bbjd_workspace *workspace_alloc_impl(const bbjd_params *p, const bbjd_gpu
         d->gvc = AT(O_GVC); d->qrc = AT(O_QRC); d->qlc = AT(O_QLC);
         d->tc[0] = AT(O_TC0); d->tc[1] = AT(O_TC1); d->tc[2] = AT(O_TC2);
     }
+    if (L.proj) {
+        d->yc = AT(O_YC); d->hinv = AT(O_HINV); d->hnew = AT(O_HNEW);
+        d->pp_ld = L.nev + 1;
+    }
 #undef AT
     if (!HIPOK(hipMemsetAsync(d->rd.scal, 0, SL_COUNT * 8, d->stream))) goto fail;
     if (!HIPOK(hipMemsetAsync(d->rd.flag, 0, FL_COUNT * 4, d->stream))) goto fail;
@@ -368,6 +392,12 @@ static bbjd_workspace *workspace_alloc_impl(const bbjd_params *p, const bbjd_gpu
     d->gx_cap = (int)L.ims + 1;
     if (!(d->gx = calloc((size_t)d->gx_cap, sizeof(hipGraphExec_t)))) goto fail;
     d->pair_res = calloc((size_t)L.nev + 1, sizeof(double));
+    if (L.proj) {
+        const size_t K = (size_t)L.nev + 1;
+        /* H, H^-1 and the elimination's K x 2K work array */
+        if (!(d->hh = calloc(4 * K * K, sizeof(zcx)))) goto fail;
+        if (!HIPOK(hipHostMalloc((void **)&d->hpp, 3 * K * K * sizeof(c64), 0))) { d->hpp = NULL; goto fail; }
+    }
     if (!ws->MA || !ws->MB || !ws->SA || !ws->SB || !ws->ZA || !ws->ZB || !ws->gmres_H || !ws->gmres_cs ||
         !ws->gmres_sn || !ws->gmres_g || !d->al || !d->be || !d->yv || !d->xv || !d->pair_res)
         goto fail;
@@ -459,6 +489,8 @@ typedef struct {
     int               w_mixed;   /* a W column came from the Av fallback since
                                     the last restart: no triangular relation */
     int               graphs;    /* capture/replay the inner-solve segments  */
+    int               pp;        /* this inner solve uses the projected
+                                    preconditioner (BBJD_INNER_PROJECTED)   */
 } ctx_t;
 
 #define COLZ(M, j) ((void *)((zcx *)(M) + (size_t)(j) * c->ld))
@@ -640,6 +672,106 @@ static int seg_end(ctx_t *c, int idx, int rc)
     return 0;
 }
 
+/* Inverse of the K x K complex matrix A (column-major, leading dimension
+ * lda) into X (column-major, leading dimension K) by Gauss-Jordan elimination
+ * with partial pivoting; returns 1 when a pivot falls below 1e-10 of the
+ * largest entry (numerically singular for this use), 0 otherwise. */
+static int small_inverse(bbjd_int K, const zcx *A, bbjd_int lda, zcx *X, zcx *work)
+{
+    /* work: K x 2K, [A | I] row-reduced in place */
+    const bbjd_int ldw = K;
+    double amax = 0;
+    for (bbjd_int j = 0; j < K; j++)
+        for (bbjd_int i = 0; i < K; i++) {
+            work[i + j * ldw] = A[i + j * lda];
+            work[i + (K + j) * ldw] = i == j ? 1.0 : 0.0;
+            if (cabs(A[i + j * lda]) > amax) amax = cabs(A[i + j * lda]);
+        }
+    if (!(amax > 0)) return 1;
+    for (bbjd_int c = 0; c < K; c++) {
+        bbjd_int p = c;
+        for (bbjd_int i = c + 1; i < K; i++)
+            if (cabs(work[i + c * ldw]) > cabs(work[p + c * ldw])) p = i;
+        if (cabs(work[p + c * ldw]) < 1e-10 * amax) return 1;
+        if (p != c)
+            for (bbjd_int j = c; j < 2 * K; j++) {
+                const zcx t = work[c + j * ldw];
+                work[c + j * ldw] = work[p + j * ldw];
+                work[p + j * ldw] = t;
+            }
+        const zcx ip = 1.0 / work[c + c * ldw];
+        for (bbjd_int j = c; j < 2 * K; j++) work[c + j * ldw] *= ip;
+        for (bbjd_int i = 0; i < K; i++) {
+            if (i == c) continue;
+            const zcx f = work[i + c * ldw];
+            if (f == 0) continue;
+            for (bbjd_int j = c; j < 2 * K; j++) work[i + j * ldw] -= f * work[c + j * ldw];
+        }
+    }
+    for (bbjd_int j = 0; j < K; j++)
+        for (bbjd_int i = 0; i < K; i++) X[i + j * K] = work[i + (K + j) * ldw];
+    return 0;
+}
+
+/* Set-up of the projected-preconditioner inner solve (BBJD_INNER_PROJECTED)
+ * for ndefl = K deflation columns.  Columns of Yh = M^-1 Q_left and the
+ * rows and columns of H = Q_right^H Yh from pp_valid on are (re)computed in
+ * complex64 (column K - 1 holds the current Ritz pair's projector columns,
+ * which change every outer iteration; the earlier ones are locked pairs'),
+ * H comes to the host, and H^-1 goes to the device.  One host sync.
+ * Returns 0, 1 when H is numerically singular (the call takes the
+ * reference form), -1 when a callback failed, -2 on an enqueue error. */
+static int pp_prepare(ctx_t *c, bbjd_int K)
+{
+    struct bbjd_dev *d = c->d;
+    const bbjd_params *P = c->P;
+    const bbjd_int n = c->n, ldH = d->pp_ld;
+    bbjd_int v0 = d->pp_valid < K ? d->pp_valid : K - 1;
+    if (v0 < 0) v0 = 0;
+    const bbjd_int nnew = K - v0;
+    c64 *hcol = (c64 *)d->hnew, *hrow = (c64 *)d->hnew + (size_t)nnew * K;
+    int rc = 0;
+    for (bbjd_int j = v0; j < K; j++)
+        if (apply_c64(c, d->opt.precond_c64, (bbjd_matvec_fn)P->precond, COLC(d->qlc, j), COLC(d->yc, j),
+                      c->ws->tmp3, c->ws->tmp4))
+            return -1;
+    for (bbjd_int j = v0; j < K; j++) {
+        /* H[:, j] = Q^H Yh[:, j] */
+        rc |= bbjd_k_gemv_c_c(c->s, n, (int)K, d->qrc, c->ld, COLC(d->yc, j), NULL, hcol + (size_t)(j - v0) * K,
+                              NULL, &d->rd);
+        /* H[j, :v0] = conj(Yh[:, :v0]^H Q[:, j]) */
+        if (v0 > 0)
+            rc |= bbjd_k_gemv_c_c(c->s, n, (int)v0, d->yc, c->ld, COLC(d->qrc, j), NULL,
+                                  hrow + (size_t)(j - v0) * v0, NULL, &d->rd);
+    }
+    const size_t cnt = (size_t)nnew * K + (size_t)nnew * v0;
+    rc |= !HIPOK(hipMemcpyAsync(d->hpp, d->hnew, cnt * sizeof(c64), hipMemcpyDeviceToHost, c->s));
+    if (rc || !HIPOK(hipStreamSynchronize(c->s))) return -2;
+    zcx *H = d->hh;
+    const c64 *hc = d->hpp, *hr = d->hpp + (size_t)nnew * K;
+    for (bbjd_int j = v0; j < K; j++) {
+        for (bbjd_int i = 0; i < K; i++) {
+            const c64 e = hc[(size_t)(j - v0) * K + i];
+            H[i + j * ldH] = e.re + e.im * I;
+        }
+        for (bbjd_int i = 0; i < v0; i++) {
+            const c64 e = hr[(size_t)(j - v0) * v0 + i];
+            H[j + i * ldH] = e.re - e.im * I;
+        }
+    }
+    d->pp_valid = K - 1;
+    /* H^-1 (complex128 on the host, then complex64 to the device) */
+    zcx *X = d->hh + (size_t)ldH * ldH;
+    const int sing = small_inverse(K, H, ldH, X, X + (size_t)ldH * ldH);
+    if (!sing) {
+        c64 *up = d->hpp + (size_t)2 * K * K;
+        for (size_t i = 0; i < (size_t)K * K; i++) { up[i].re = (float)creal(X[i]); up[i].im = (float)cimag(X[i]); }
+        rc |= !HIPOK(hipMemcpyAsync(d->hinv, up, (size_t)K * K * sizeof(c64), hipMemcpyHostToDevice, c->s));
+    }
+    if (rc) return -2;
+    return sing ? 1 : 0;
+}
+
 /* Enqueues Arnoldi step j of the inner solve: v_{j+1} = M^-1 (I - QL QL^H)
  * (theta_b A - theta_a B)(I - QR QR^H) v_j, its CGS2 against v_0 .. v_j, the
  * new vector normalised (also into the next step's input copy), and copies
@@ -673,6 +805,29 @@ static int arnoldi_enqueue(ctx_t *c, bbjd_int j, bbjd_int ndefl, const void *QR,
         rc |= bbjd_k_cgs2(c->s, n, (int)(j + 1), V, c->ld, vj1, &d->rd, SL_G, hsum);
         if (hs) rc |= !HIPOK(hipMemcpyAsync(hh, hsum, colb, hipMemcpyDeviceToHost, c->s));
         rc |= bbjd_k_scale_inv2(c->s, n, vj1, t1, d->rd.scal, SL_G, 0);   /* t1: the next step's */
+    } else if (c->pp) {
+        /* projected preconditioner: v_j is in Q_right-perp already, so the
+         * operator reads it in place; after M^-1 one oblique projection
+         * (I - Yh H^-1 Q^H) replaces both sides' projections */
+        const void *vj = (const char *)V + (size_t)j * c->ld * esz;
+        void *bq = d->tc[1], *e = d->tc[2];
+        if (d->opt.matvec_AB_c64 && d->opt.callback_space == BBJD_CALLBACK_DEVICE) {
+            hipStream_t prev = tl_cb_stream;
+            tl_cb_stream = c->s;
+            if (d->opt.matvec_AB_c64(n, (const bbjd_cfloat *)vj, (bbjd_cfloat *)bq, (bbjd_cfloat *)e, P->op_data))
+                cb = 1;
+            tl_cb_stream = prev;
+        } else {
+            if (apply_c64(c, d->opt.matvec_A_c64, P->matvec_A, vj, bq, t1, t2)) cb = 1;
+            if (!cb && apply_c64(c, d->opt.matvec_B_c64, P->matvec_B, vj, e, t1, t2)) cb = 1;
+        }
+        rc |= bbjd_k_lincomb_dev_c(c->s, n, d->gco, bq, e, bq);
+        if (!rc && !cb && apply_c64(c, d->opt.precond_c64, (bbjd_matvec_fn)P->precond, bq, vj1, t1, t2)) cb = 1;
+        rc |= bbjd_k_gemv_c_c(c->s, n, (int)ndefl, qr, c->ld, vj1, NULL, d->rd.coef, NULL, &d->rd);
+        rc |= bbjd_k_update_pre_c(c->s, n, (int)ndefl, d->yc, c->ld, d->hinv, d->rd.coef, vj1);
+        rc |= bbjd_k_cgs2_c(c->s, n, (int)(j + 1), V, c->ld, vj1, &d->rd, SL_G, hsum);
+        if (hs) rc |= !HIPOK(hipMemcpyAsync(hh, hsum, colb, hipMemcpyDeviceToHost, c->s));
+        rc |= bbjd_k_scale_inv_c(c->s, n, vj1, d->rd.scal, SL_G, 0);
     } else {
         void *a = d->tc[0], *bq = d->tc[1], *e = d->tc[2];
         /* a = v_j: written with v_j's normalisation */
@@ -720,7 +875,7 @@ static int arnoldi_enqueue(ctx_t *c, bbjd_int j, bbjd_int ndefl, const void *QR,
  * graphs a segment is captured once per deflation count and replayed on
  * later outer iterations. */
 static int gmres(ctx_t *c, bbjd_int ndefl, const void *QR, const void *QL, zcx ta, zcx tb, const void *r, void *z,
-                 bbjd_int ims, double tol)
+                 bbjd_int ims, double tol, int allow_pp)
 {
     bbjd_workspace *ws = c->ws;
     struct bbjd_dev *d = c->d;
@@ -740,11 +895,21 @@ static int gmres(ctx_t *c, bbjd_int ndefl, const void *QR, const void *QL, zcx t
     const void *ql = mixed ? d->qlc : QL;
 
     c->graphs = d->opt.graphs && d->opt.callback_space == BBJD_CALLBACK_DEVICE;
+    c->pp = 0;
+    if (allow_pp && mixed && d->yc && P->precond && ndefl <= d->pp_ld) {
+        const int pr = pp_prepare(c, ndefl);
+        if (pr == -1) return -1;
+        if (pr < 0) return -2;
+        c->pp = pr == 0;
+        d->st.inner_projected++;
+        if (pr == 1) d->st.inner_projected_fallbacks++;
+    }
     if (c->graphs) {
         struct gkey key;
         memset(&key, 0, sizeof(key));
         key.ndefl = ndefl; key.n = n; key.ld = c->ld; key.r = r; key.z = z; key.qr = QR; key.ql = QL;
         key.op = P->op_data;
+        key.pp = c->pp;
         if (memcmp(&key, &d->gkey, sizeof(key))) {
             rc |= !HIPOK(hipStreamSynchronize(c->s));
             graphs_drop(d);
@@ -765,6 +930,16 @@ static int gmres(ctx_t *c, bbjd_int ndefl, const void *QR, const void *QL, zcx t
             rc |= bbjd_k_norm(c->s, n, GCOL(0), &d->rd, SL_G);
             /* also into t1 (free again), where step 0 projects it */
             rc |= bbjd_k_scale_inv2(c->s, n, GCOL(0), t1, d->rd.scal, SL_G, 0);
+        } else if (c->pp) {
+            /* v0 = (I - Yh H^-1 Q^H) M^-1 (-r), normalised */
+            void *tc0 = d->tc[0];
+            rc |= bbjd_k_z2c(c->s, n, r, tc0);
+            rc |= bbjd_k_scale_c(c->s, n, tc0, -1.0, 0.0);
+            if (!rc && apply_c64(c, d->opt.precond_c64, (bbjd_matvec_fn)P->precond, tc0, GCOL(0), t1, t2)) cb = 1;
+            rc |= bbjd_k_gemv_c_c(c->s, n, (int)ndefl, d->qrc, c->ld, GCOL(0), NULL, d->rd.coef, NULL, &d->rd);
+            rc |= bbjd_k_update_pre_c(c->s, n, (int)ndefl, d->yc, c->ld, d->hinv, d->rd.coef, GCOL(0));
+            rc |= bbjd_k_norm_c(c->s, n, GCOL(0), &d->rd, SL_G);
+            rc |= bbjd_k_scale_inv_c(c->s, n, GCOL(0), d->rd.scal, SL_G, 0);
         } else {
             void *tc0 = d->tc[0];
             rc |= bbjd_k_z2c(c->s, n, r, tc0);
@@ -829,7 +1004,7 @@ static int gmres(ctx_t *c, bbjd_int ndefl, const void *QR, const void *QL, zcx t
             if (rc || !HIPOK(hipStreamSynchronize(c->s))) return -2;
         }
         d->st.inner_steps++;
-        d->st.stream_bytes += (double)esz * n * (8.0 * ndefl + 4.0 * (j + 1) + 8);
+        d->st.stream_bytes += (double)esz * n * ((c->pp ? 3.0 : 8.0) * ndefl + 4.0 * (j + 1) + 8);
         if (!mixed)
             for (bbjd_int i = 0; i <= j; i++) H[i + j * ldh] = ((const zcx *)hh)[i];
         else {
@@ -1247,6 +1422,7 @@ static bbjd_status solve_impl(const bbjd_params *P, bbjd_workspace *ws, bbjd_res
     cx.nu0 = nu0;
     cx.mu0 = mu0;
     ws->kdim = 0; ws->nlock = 0;
+    d->pp_valid = 0;
     bbjd_qz_invalidate(&d->qz);
     rng_seed(d, 42);
     hipMemsetAsync(d->rd.flag, 0, FL_COUNT * 4, c->s);
@@ -1416,7 +1592,7 @@ static bbjd_status solve_impl(const bbjd_params *P, bbjd_workspace *ws, bbjd_res
         if (P->inner_maxiter > 0) {
             if (deflation_append(c, locked, u, Au, Bu, nu0, mu0)) { st = BBJD_ERR_ALLOC; goto done; }
             int g = gmres(c, ws->nlock + 1, ws->Q_right, ws->Q_left, ta, tb, r, zc, P->inner_maxiter,
-                          P->inner_tol);
+                          P->inner_tol, 1);
             if (g == -1) { st = BBJD_ERR_CALLBACK; goto done; }
             if (g) { st = BBJD_ERR_ALLOC; goto done; }
         } else {
@@ -1748,7 +1924,7 @@ int bbjd_correction_gmres(bbjd_int n, const bbjd_zomplex *Q_right, const bbjd_zo
             rc |= bbjd_k_z2c(cx.s, n, (const zcx *)ql + j * ws->ld, (c64 *)d->qlc + j * ws->ld);
         }
     if (rc) return -1;
-    int g = gmres(&cx, ndefl, qr, ql, theta_a, theta_b, dr, dz, maxiter, tol);
+    int g = gmres(&cx, ndefl, qr, ql, theta_a, theta_b, dr, dz, maxiter, tol, 0);
     if (g) return -1;
     if (!devspace) {
         rc |= !HIPOK(hipMemcpyAsync(z, dz, (size_t)n * 16, hipMemcpyDeviceToHost, cx.s));

@@ -406,6 +406,60 @@ __global__ __launch_bounds__(TPB) void k_update(i64 n, int k, const Cx<R> *__res
     if (t == 0) *cnt = 0;
 }
 
+// x -= Y[:, :k] (M c) with a k x k matrix M (column-major) applied to the
+// coefficients first: every block forms M c in LDS (k^2 products, M read from
+// cache), then streams its rows of Y.  The inner solve's oblique projector
+// (I - Y H^-1 Q^H) uses it with M = H^-1.  Dynamic LDS: 2 k complex.
+template <typename R, int E>
+__global__ __launch_bounds__(TPB) void k_update_pre(i64 n, int k, const Cx<R> *__restrict__ Y, i64 ldy,
+                                                    const Cx<R> *__restrict__ M, const Cx<R> *__restrict__ c,
+                                                    Cx<R> *__restrict__ x)
+{
+    extern __shared__ char smem_raw[];
+    Cx<R> *sc = reinterpret_cast<Cx<R> *>(smem_raw);
+    Cx<R> *sh = sc + k;
+    const int t = threadIdx.x;
+    for (int j = t; j < k; j += TPB) sc[j] = c[j];
+    __syncthreads();
+    for (int j = t; j < k; j += TPB) {
+        Cx<R> a = cx<R>(0, 0);
+        for (int i = 0; i < k; i++) cfma(a, M[(i64)i * k + j], sc[i]);
+        sh[j] = cx(-a.re, -a.im);
+    }
+    __syncthreads();
+    const i64 rb = (i64)blockIdx.x * (TPB * E);
+    Cx<R> a[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        const i64 r = rb + t + (i64)e * TPB;
+        a[e] = r < n ? x[r] : cx<R>(0, 0);
+    }
+    if (rb + TPB * E <= n) {
+#pragma unroll 4
+        for (int j = 0; j < k; j++) {
+            const Cx<R> hj = sh[j];
+            const Cx<R> *q = Y + (i64)j * ldy + rb + t;
+#pragma unroll
+            for (int e = 0; e < E; e++) cfma(a[e], q[(i64)e * TPB], hj);
+        }
+    } else {
+        for (int j = 0; j < k; j++) {
+            const Cx<R> hj = sh[j];
+            const Cx<R> *q = Y + (i64)j * ldy;
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                const i64 r = rb + t + (i64)e * TPB;
+                if (r < n) cfma(a[e], q[r], hj);
+            }
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        const i64 r = rb + t + (i64)e * TPB;
+        if (r < n) x[r] = a[e];
+    }
+}
+
 // The same update with the columns split over the block's four waves: a
 // block owns 64 * RW rows, wave w accumulates columns w, w + 4, ... of its
 // rows, and the four partial sums are added in a fixed order through LDS.
@@ -1822,6 +1876,29 @@ int bbjd_k_gemv_c(hipStream_t s, int64_t n, int k, const void *Q, int64_t ldq, c
 int bbjd_k_gemv_c_c(hipStream_t s, int64_t n, int k, const void *Q, int64_t ldq, const void *x, const void *x2,
                     void *out, void *out2, bbjd_red *rd)
 { return proj_pass<float>(s, n, k, Q, ldq, x, x2, out, out2, rd, -1, nullptr); }
+
+// x -= Y[:, :k] (M c), complex64 (M k x k column-major, c k entries; device)
+int bbjd_k_update_pre_c(hipStream_t s, int64_t n, int k, const void *Y, int64_t ldy, const void *M, const void *c,
+                        void *x)
+{
+    if (k <= 0) return 0;
+    if (k > 2048 || n <= 0) return -2;          // LDS: 2 k complex64
+    const size_t lds = 2 * (size_t)k * sizeof(Cx<float>);
+    // enough blocks to spread the rows over the device: four rows per
+    // thread only on the largest vectors
+    const int e = n >= (i64)512 * 4 * TPB ? 4 : n >= (i64)256 * 2 * TPB ? 2 : 1;
+    const int nb = (int)((n + (i64)TPB * e - 1) / ((i64)TPB * e));
+#define BBJD_LAUNCH_PRE(E)                                                                                         \
+    hipLaunchKernelGGL((k_update_pre<float, E>), dim3(nb), dim3(TPB), lds, s, (i64)n, k, (const Cx<float> *)Y,     \
+                       (i64)ldy, (const Cx<float> *)M, (const Cx<float> *)c, (Cx<float> *)x)
+    switch (e) {
+    case 4: BBJD_LAUNCH_PRE(4); break;
+    case 2: BBJD_LAUNCH_PRE(2); break;
+    default: BBJD_LAUNCH_PRE(1); break;
+    }
+#undef BBJD_LAUNCH_PRE
+    return chk();
+}
 
 int bbjd_k_gemv_n(hipStream_t s, int64_t n, int k, const void *Q, int64_t ldq, const void *c, void *y,
                   bbjd_red *rd, int slot)

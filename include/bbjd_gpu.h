@@ -91,7 +91,22 @@ typedef struct {
     /* optional, device callbacks only: the residual step's A u, B u and
      * r = theta_b A u - theta_a B u with ||r|| in one call (same A, B). */
     bbjd_residual_fn    matvec_ABr;
+    /* inner solve of BBJD_PREC_MIXED with a preconditioner:
+     * BBJD_INNER_REFERENCE (default) is the reference's GMRES on
+     * M^-1 P_L (theta_b A - theta_a B) P_R, two fixed projections on each
+     * side per step; BBJD_INNER_PROJECTED uses the projected preconditioner
+     * of Fokkema, Sleijpen and van der Vorst (SIAM J. Sci. Comput. 20, 1998):
+     * with Yh = M^-1 Q_left and H = Q_right^H Yh (both kept on the device
+     * across outer iterations, H^-1 formed on the host), GMRES runs on
+     * (I - Yh H^-1 Q_right^H) M^-1 (theta_b A - theta_a B) over Q_right-perp,
+     * one projection and one update per step.  Same exact solution of the
+     * correction equation; the outer FP64 loop and its convergence test are
+     * unchanged.  Falls back to the reference form for a call whose H is
+     * numerically singular. */
+    int                 inner_form;
 } bbjd_gpu_options;
+
+enum { BBJD_INNER_REFERENCE = 0, BBJD_INNER_PROJECTED = 1 };
 
 /* BBJD_QZ_DEVICE: the projected step on the GPU (bbjd_qz_device_batch, one
  * wavefront) for k <= 64 with the nearest-sigma or smallest-magnitude test
@@ -155,6 +170,9 @@ typedef struct {
                                        solve's stop and not used            */
     bbjd_int qz_harmonic_restarts;  /* restarts whose ordered Schur form came
                                        from the harmonic standard problem   */
+    bbjd_int inner_projected;       /* inner solves in BBJD_INNER_PROJECTED  */
+    bbjd_int inner_projected_fallbacks; /* of which took the reference form
+                                       (H numerically singular)             */
 } bbjd_stats;
 
 int bbjd_workspace_stats(const bbjd_workspace *ws, bbjd_stats *st);

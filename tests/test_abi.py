@@ -99,16 +99,18 @@ int main(void) {
   O(bbjd_gpu_options, matvec_A_c64); O(bbjd_gpu_options, matvec_B_c64); O(bbjd_gpu_options, precond_c64);
   O(bbjd_gpu_options, precond2); O(bbjd_gpu_options, qz_mode); O(bbjd_gpu_options, graphs);
   O(bbjd_gpu_options, matvec_AB); O(bbjd_gpu_options, matvec_AB_c64); O(bbjd_gpu_options, matvec_ABr);
+  O(bbjd_gpu_options, inner_form);
   O(bbjd_stats, outer_iters); O(bbjd_stats, lock_iters); O(bbjd_stats, restarts); O(bbjd_stats, inner_steps);
   O(bbjd_stats, dgks_second_passes); O(bbjd_stats, sum_k); O(bbjd_stats, qz_seconds);
   O(bbjd_stats, solve_seconds); O(bbjd_stats, stream_bytes); O(bbjd_stats, restart_fallbacks);
   O(bbjd_stats, graph_captures); O(bbjd_stats, graph_launches); O(bbjd_stats, qz_schur_ties);
   O(bbjd_stats, qz_device_steps); O(bbjd_stats, qz_device_fallbacks); O(bbjd_stats, qz_device_accuracy);
   O(bbjd_stats, inner_steps_discarded); O(bbjd_stats, qz_harmonic_restarts);
+  O(bbjd_stats, inner_projected); O(bbjd_stats, inner_projected_fallbacks);
   O(bbjd_serve_config, workers); O(bbjd_serve_config, hbm_budget); O(bbjd_serve_config, precision);
   O(bbjd_serve_config, realtime); O(bbjd_serve_config, sync_yield); O(bbjd_serve_config, graphs);
   O(bbjd_serve_config, order); O(bbjd_serve_config, pin); O(bbjd_serve_config, window);
-  O(bbjd_serve_config, qz_mode);
+  O(bbjd_serve_config, qz_mode); O(bbjd_serve_config, inner_form);
   return 0;
 }
 """

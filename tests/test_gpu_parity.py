@@ -394,6 +394,74 @@ def test_inner_gmres_approximate_preconditioner(precision):
     assert info["iters"] <= info0["iters"]
 
 
+def test_inner_projected_preconditioner_matches_oracle():
+    """complex64 inner GMRES(5) in the projected-preconditioner form
+    (inner_form 'projected': one oblique projection (I - Yh H^-1 Q^H) per
+    Arnoldi step instead of two projections on each side) with the exact LU,
+    on the serving workload's shape (block size 8, block-tridiagonal, 4 tau
+    rows): the oracle's eigenvalue set (FP64, one-step correction) within
+    1e-8 both ways, every pair's FP64 residual from a fresh matvec below tol,
+    the form taken by every inner solve, and about as many outer iterations
+    as the reference form's inner solve with the same budget"""
+    from bbjd_amd.configs import row_norm_bound
+    p = BandedTauPencil.generate(20000, 8, 1, 4, seed=31)
+    sigma = 1.5 - 2.2j
+    ref = orc.solve("oracle", p.n, 8, sigma, tol=1e-10, maxsub=32, minsub=16, maxiter=1000,
+                    ops=orc.CpuPencilOps(p, sigma, 1))
+    assert ref["nconv"] == 8
+    dev = DeviceOperator(p, sigma, pc="exact", c64=True)
+    kw = dict(tol=1e-10, maxsub=32, minsub=16, maxiter=1000, inner_maxiter=5, inner_tol=1e-2, device_op=dev,
+              precision="mixed", return_stats=True)
+    ev, X, res, nconv, iters, status, st = jdqz_c(p.n, 8, sigma, 0, None, None, True, inner_form="projected", **kw)
+    rf = jdqz_c(p.n, 8, sigma, 0, None, None, True, inner_form="reference", **kw)
+    print(f"[projected] iters {iters} (reference form {rf[4]}, oracle plain {ref['iters']}), inner steps "
+          f"{st['inner_steps']} (reference form {rf[6]['inner_steps']}), projected calls {st['inner_projected']}",
+          flush=True)
+    assert status == 0 and nconv == 8
+    assert st["inner_projected"] == st["outer_iters"] - st["lock_iters"] > 0
+    assert st["inner_projected_fallbacks"] == 0
+    assert rf[6]["inner_projected"] == 0
+    assert rel_match(ev, ref["evals"]) <= 1e-8 and rel_match(ref["evals"], ev) <= 1e-8
+    A, B = p.to_scipy()
+    anorm = row_norm_bound(p)
+    for j in range(8):
+        x = X[:, j]
+        assert np.linalg.norm(A @ x - ev[j] * (B @ x)) / ((anorm + abs(ev[j])) * np.linalg.norm(x)) <= 1e-10
+    # the same budget of inner steps converges alike in both forms (64 and
+    # 64 outer iterations here); the projected form's steps are cheaper
+    assert iters <= 1.2 * rf[4]
+
+
+def test_serving_loop_projected_inner_alone_equal_mix():
+    """the served form of the serving workload: complex64 inner GMRES(5) in
+    the projected-preconditioner form with the exact LU, block size 8,
+    4 tau rows; each request's eigenvalues bit-identical alone and in a mix
+    of four workers, and each request's set within 1e-8 of the oracle's
+    plain FP64 solve both ways"""
+    from bbjd_amd import serve as srv
+    rng = np.random.default_rng(17)
+    jobs, pens = [], []
+    for i, (n, k) in enumerate(((20000, 8), (20000, 16), (24000, 8), (20000, 8), (16000, 16))):
+        p = BandedTauPencil.generate(n, 8, 1, 4, seed=300 + i)
+        s = complex(rng.uniform(-4, 4), rng.uniform(-4, 4))
+        pens.append(p)
+        jobs.append(dict(pencil=p, pc="exact", sigma=s, nev=k, maxsub=4 * k, minsub=2 * k, maxiter=1000,
+                         inner_maxiter=5, inner_tol=1e-2, tol=1e-10))
+    kw = dict(realtime=False, precision="mixed", inner_form="projected")
+    mix, rep = srv.serve(jobs, workers=4, **kw)
+    assert rep["failed"] == 0 and rep["peak_concurrency"] >= 2
+    for i, j in enumerate(jobs):
+        alone, _ = srv.serve([j], workers=1, **kw)
+        a = alone[0]
+        assert a["status"] == 0 and a["nconv"] == j["nev"], i
+        np.testing.assert_array_equal(mix[i]["eigenvalues"], a["eigenvalues"], err_msg=str(i))
+        ref = orc.solve("oracle", pens[i].n, j["nev"], j["sigma"], tol=1e-10, maxsub=j["maxsub"],
+                        minsub=j["minsub"], maxiter=1000, ops=orc.CpuPencilOps(pens[i], j["sigma"], 1))
+        assert ref["nconv"] == j["nev"], i
+        assert rel_match(a["eigenvalues"], ref["evals"]) <= 1e-8, i
+        assert rel_match(ref["evals"], a["eigenvalues"]) <= 1e-8, i
+
+
 @pytest.mark.parametrize("precision,pc", [("mixed", "exact"), ("fp64", "exact"), ("mixed", "block_jacobi")])
 def test_inner_gmres_graphs_replay_identically(precision, pc):
     """the inner GMRES prologue and Arnoldi steps captured into HIP graphs
