@@ -201,6 +201,8 @@ static void *worker_main(void *arg)
 static int cmp_arrival(const void *a, const void *b, void *ctx)
 {
     const bbjd_request *R = (const bbjd_request *)ctx;
+    const int wa = R[*(const int *)a].wave, wb = R[*(const int *)b].wave;
+    if (wa != wb) return wa < wb ? -1 : 1;
     const double x = R[*(const int *)a].arrival, y = R[*(const int *)b].arrival;
     return x < y ? -1 : x > y ? 1 : (*(const int *)a - *(const int *)b);
 }
@@ -256,6 +258,7 @@ bbjd_server *bbjd_server_create(const bbjd_request *reqs, int nreq, const bbjd_s
     S->opt.matvec_ABr = bbjd_op_matvec_ABr;
     S->opt.graphs = cfg->graphs;
     S->opt.qz_mode = cfg->qz_mode;
+    S->opt.inner_form = cfg->inner_form;
     if (cfg->precision == BBJD_PREC_MIXED) {
         S->opt.matvec_A_c64 = bbjd_op_matvec_A_c64;
         S->opt.matvec_B_c64 = bbjd_op_matvec_B_c64;
@@ -315,7 +318,7 @@ int bbjd_server_run(bbjd_server *S, bbjd_request *reqs, int nreq, const bbjd_ser
     for (int i = 0; i < nreq; i++) {
         bbjd_request *q = &reqs[i];
         q->status = 0; q->nconv = 0; q->iters = 0; q->worker = -1;
-        q->t_admit = q->t_start = q->t_done = 0;
+        q->t_admit = q->t_start = q->t_done = q->t_arrive = 0;
         const bbjd_int n = req_n(q);
         if (n <= 0 || q->nev <= 0) return -1;
     }
@@ -338,6 +341,9 @@ int bbjd_server_run(bbjd_server *S, bbjd_request *reqs, int nreq, const bbjd_ser
     rep->workspace_bytes = S->wsb;
 
     int npend = 0, next = 0;
+    /* the wave being admitted and the time its arrivals count from */
+    int wave = reqs[order[0]].wave;
+    double base = 0;
     for (int admitted = 0; admitted < nreq; admitted++) {
         /* a free worker first: the choice among waiting requests is made
          * when one can start */
@@ -351,15 +357,18 @@ int bbjd_server_run(bbjd_server *S, bbjd_request *reqs, int nreq, const bbjd_ser
         pthread_mutex_unlock(&S->mu);
         /* everything that has arrived by now; at least the next arrival */
         if (npend == 0 && cfg->realtime) {
-            const double wait = reqs[order[next]].arrival - (now_s() - S->t0);
+            const double wait = base + reqs[order[next]].arrival - (now_s() - S->t0);
             if (wait > 0) {
                 struct timespec ts = {(time_t)wait, (long)((wait - (double)(time_t)wait) * 1e9)};
                 nanosleep(&ts, NULL);
             }
         }
         const double now = now_s() - S->t0;
-        while (next < nreq && (!cfg->realtime || reqs[order[next]].arrival <= now || npend == 0))
+        while (next < nreq && reqs[order[next]].wave == wave &&
+               (!cfg->realtime || base + reqs[order[next]].arrival <= now || npend == 0)) {
+            reqs[order[next]].t_arrive = base + reqs[order[next]].arrival;
             pend[npend++] = order[next++];
+        }
         int pi = 0;
         int cand = 1;       /* the waiting requests compared, oldest first */
         if (cfg->order == BBJD_ORDER_LARGEST) cand = npend;
@@ -392,6 +401,11 @@ int bbjd_server_run(bbjd_server *S, bbjd_request *reqs, int nreq, const bbjd_ser
         q->t_admit = now_s() - S->t0;
         pthread_cond_broadcast(&S->cv_work);
         pthread_mutex_unlock(&S->mu);
+        /* the wave's last request is admitted: the next wave's clock starts */
+        if (npend == 0 && next < nreq && reqs[order[next]].wave != wave) {
+            wave = reqs[order[next]].wave;
+            base = q->t_admit;
+        }
     }
     free(pend);
     pthread_mutex_lock(&S->mu);

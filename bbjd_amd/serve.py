@@ -8,7 +8,9 @@ batches (``run(jobs)`` per batch, ``close()`` at the end).  A job
 is a dict with ``op`` (a DeviceOperator already resident on the GPU) or
 ``pencil`` + ``pc`` (the server builds the operator at admission),
 ``sigma``, ``nev`` and optionally maxsub, minsub, maxiter, inner_maxiter,
-tol, inner_tol, arrival (seconds) and v0 (a start vector of length n).
+tol, inner_tol, arrival (seconds), wave (arrival wave: its arrivals count
+from the admission of the previous wave's last request), keep_vectors and v0
+(a start vector of length n).
 """
 from __future__ import annotations
 
@@ -29,13 +31,13 @@ class Request(C.Structure):
                 ("eigenvectors", C.c_void_p), ("nconv", C.c_int64), ("iters", C.c_int64),
                 ("status", C.c_int), ("worker", C.c_int), ("t_admit", C.c_double),
                 ("t_start", C.c_double), ("t_done", C.c_double), ("qz_seconds", C.c_double),
-                ("stream_bytes", C.c_double), ("v0", C.c_void_p)]
+                ("stream_bytes", C.c_double), ("v0", C.c_void_p), ("wave", C.c_int), ("t_arrive", C.c_double)]
 
 
 class ServeConfig(C.Structure):
     _fields_ = [("workers", C.c_int), ("hbm_budget", C.c_size_t), ("precision", C.c_int),
                 ("realtime", C.c_int), ("sync_yield", C.c_int), ("graphs", C.c_int), ("order", C.c_int),
-                ("pin", C.c_int), ("window", C.c_int), ("qz_mode", C.c_int)]
+                ("pin", C.c_int), ("window", C.c_int), ("qz_mode", C.c_int), ("inner_form", C.c_int)]
 
 
 class ServeReport(C.Structure):
@@ -107,6 +109,7 @@ def _requests(jobs, keep_vectors):
         q.tol = float(j.get("tol", 1e-10))
         q.inner_tol = float(j.get("inner_tol", 1e-2))
         q.arrival = float(j.get("arrival", 0.0))
+        q.wave = int(j.get("wave", 0))
         if j.get("v0") is not None:
             v0 = np.ascontiguousarray(j["v0"], dtype=np.complex128)
             if v0.shape != (n,):
@@ -115,7 +118,7 @@ def _requests(jobs, keep_vectors):
             q.v0 = v0.ctypes.data
         o = {"eigenvalues": np.zeros(nev, np.complex128), "residuals": np.zeros(nev),
              "pair_residuals": np.zeros(nev)}
-        if keep_vectors:
+        if j.get("keep_vectors", keep_vectors):
             o["eigenvectors"] = np.zeros((nev, n), np.complex128)
             q.eigenvectors = o["eigenvectors"].ctypes.data
         q.eigenvalues = o["eigenvalues"].ctypes.data
@@ -126,7 +129,7 @@ def _requests(jobs, keep_vectors):
 
 
 def _config(lib, workers=16, precision="fp64", realtime=True, hbm_budget=0, sync_yield=True, graphs=False,
-            order="arrival", pin=False, window=8, qz_mode="ritz"):
+            order="arrival", pin=False, window=8, qz_mode="ritz", inner_form="reference"):
     if order not in ("arrival", "largest", "window"):
         raise ValueError("order must be 'arrival', 'largest' or 'window'")
     if int(window) < 1:
@@ -146,6 +149,10 @@ def _config(lib, workers=16, precision="fp64", realtime=True, hbm_budget=0, sync
     if qz_mode not in modes:
         raise ValueError("qz_mode must be 'ritz', 'schur' or 'device'")
     cfg.qz_mode = modes[qz_mode]
+    forms = {"reference": _lib.INNER_REFERENCE, "projected": _lib.INNER_PROJECTED}
+    if inner_form not in forms:
+        raise ValueError("inner_form must be 'reference' or 'projected'")
+    cfg.inner_form = forms[inner_form]
     return cfg
 
 
@@ -156,19 +163,20 @@ def _results(reqs, outs, keep_vectors):
         nc = int(q.nconv)
         o = outs[i]
         r = {"nconv": nc, "iters": int(q.iters), "status": int(q.status), "worker": int(q.worker),
-             "arrival": q.arrival, "t_admit": q.t_admit, "t_start": q.t_start, "t_done": q.t_done,
+             "arrival": q.arrival, "wave": q.wave, "t_arrive": q.t_arrive, "t_admit": q.t_admit, "t_start": q.t_start, "t_done": q.t_done,
              "qz_seconds": q.qz_seconds, "stream_bytes": q.stream_bytes,
              "eigenvalues": o["eigenvalues"][:nc].copy(),
              "residuals": o["residuals"][:nc].copy(),
              "pair_residuals": o["pair_residuals"][:nc].copy()}
-        if keep_vectors:
+        if "eigenvectors" in o:
             r["eigenvectors"] = o["eigenvectors"][:nc].T.copy()
         results.append(r)
     return results
 
 
 def serve(jobs, workers=16, precision="fp64", realtime=True, hbm_budget=0, keep_vectors=False,
-          sync_yield=True, graphs=False, order="arrival", pin=False, window=8, qz_mode="ritz"):
+          sync_yield=True, graphs=False, order="arrival", pin=False, window=8, qz_mode="ritz",
+          inner_form="reference"):
     """Run jobs through the serving loop; returns (results, report).  graphs:
     capture each request's inner GMRES steps into HIP graphs and replay them.
     order: which arrived request a free worker takes, "arrival" (first come,
@@ -176,10 +184,14 @@ def serve(jobs, workers=16, precision="fp64", realtime=True, hbm_budget=0, keep_
     "window" (the largest of the `window` longest-waiting ones).  qz_mode:
     where the projected step runs ("ritz": host, eigenvalues first;
     "schur": host, the reference's ordered Schur form every iteration;
-    "device": on the GPU for subspaces up to 64)."""
+    "device": on the GPU for subspaces up to 64).  inner_form: the mixed
+    inner solve's form ("reference": the reference's GMRES with two-sided
+    projections; "projected": the projected preconditioner, one oblique
+    projection per step)."""
     lib = _lib.load()
     _setup(lib)
-    cfg = _config(lib, workers, precision, realtime, hbm_budget, sync_yield, graphs, order, pin, window, qz_mode)
+    cfg = _config(lib, workers, precision, realtime, hbm_budget, sync_yield, graphs, order, pin, window, qz_mode,
+                  inner_form)
     reqs, keep, outs = _requests(jobs, keep_vectors)
     rep = ServeReport()
     rc = lib.bbjd_serve(reqs, len(jobs), C.byref(cfg), C.byref(rep))
@@ -195,13 +207,13 @@ class Server:
     until close(), and each run() serves a batch of jobs on them."""
 
     def __init__(self, capacity_jobs, workers=16, precision="fp64", hbm_budget=0, sync_yield=True,
-                 graphs=False, pin=False, qz_mode="ritz"):
+                 graphs=False, pin=False, qz_mode="ritz", inner_form="reference"):
         lib = _lib.load()
         _setup(lib)
         self._lib = lib
         self.precision = precision
         self._cfg_kw = dict(workers=workers, precision=precision, hbm_budget=hbm_budget, sync_yield=sync_yield,
-                            graphs=graphs, pin=pin, qz_mode=qz_mode)
+                            graphs=graphs, pin=pin, qz_mode=qz_mode, inner_form=inner_form)
         cfg = _config(lib, **self._cfg_kw)
         reqs, _keep, _outs = _requests(capacity_jobs, False)
         self._h = lib.bbjd_server_create(reqs, len(capacity_jobs), C.byref(cfg))

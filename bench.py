@@ -10,10 +10,17 @@ tol 1e-10, maxsub 4k, minsub 2k, exact banded LU of A - sigma B as the
 preconditioner.  The operators (pencil + factorisation) are built and resident
 in HBM before the timed region, and so are the serving loop's worker threads,
 streams and workspaces (one resident server for every step); the timed
-region is the serving loop draining the stream.
+region is the serving loop draining the stream.  Timed steps run as arrival
+waves of one server run (--pipeline steps per run): a step's requests start
+arriving when the previous step's last request has been admitted, so workers
+freed in one step's tail take the next step's requests.
 
   value        served eigenpairs/s over the timed steps (all pairs converged
-               at tol)
+               at tol).  The served form is the FP64 engine with the
+               reference's one-step correction, so each request's results are
+               those of the plain path (and of the reference's trajectory);
+               --served-precision mixed --served-inner m serves the
+               complex64 inner GMRES(m) form instead
   verification every pair returned by the first timed step is checked
                afterwards with a fresh FP64 operator apply on the returned
                vector: ||A x - lambda B x|| / (||A|| + |lambda| ||B||) <= tol,
@@ -23,7 +30,8 @@ region is the serving loop draining the stream.
                its requests (seeded) solved alone -- gives each class's mean
                pairs and seconds per request; weighted by the class counts of
                the 256 requests they estimate the plain path's eigenpairs/s
-               on the whole workload
+               on the whole workload; the sampled requests' eigenvalue sets
+               are compared with the served ones
   latency      p50/p99 of completion - arrival over the timed requests
   roofline     the projection kernel Q^H x (the dominant streaming kernel):
                durations of every first-pass launch of the plain leg (one
@@ -42,8 +50,9 @@ region is the serving loop draining the stream.
 --workload small: the batched small solves instead -- 2048 pencils with n
 log-uniform on [4096, 32768] (block size 4, block-tridiagonal, 2 tau rows),
 k = 8, maxsub 32, minsub 16, sigma uniform on the unit disk, GMRES(20) in
-complex64 with the exact banded LU as its preconditioner, all submitted at
-once; plain = a random sample of them solved one at a time.
+complex64 (projected-preconditioner form) with the exact banded LU as its
+preconditioner, all submitted at once; plain = a random sample of them solved
+one at a time in the same form.
 
 Usage: python bench.py [--gpus 1] [--steps 1] [--warmup 1] [--nreq 256]
        python bench.py --plain-only ...   (the plain leg alone: for profilers)
@@ -96,11 +105,15 @@ def make_small_requests(nreq, seed):
 
 
 # per workload: pencil (bs, hw, ntau), solver settings, precision, realtime
+# served: the served form's inner solve; plain: the plain leg's (the serving
+# workload's plain path is the FP64 engine with the one-step correction)
 WORKLOADS = {
-    "serving": dict(shape=(8, 1, 4), inner=0, inner_tol=1e-2, precision="fp64", realtime=True,
-                    maxsub=lambda k: 4 * k, minsub=lambda k: 2 * k),
-    "small": dict(shape=(4, 1, 2), inner=20, inner_tol=1e-3, precision="mixed", realtime=False,
-                  maxsub=lambda k: 32, minsub=lambda k: 16),
+    "serving": dict(shape=(8, 1, 4), realtime=True, maxsub=lambda k: 4 * k, minsub=lambda k: 2 * k,
+                    served=dict(inner=0, inner_tol=1e-2, precision="fp64", inner_form="projected"),
+                    plain=dict(inner=0, inner_tol=1e-2, precision="fp64", inner_form="reference")),
+    "small": dict(shape=(4, 1, 2), realtime=False, maxsub=lambda k: 32, minsub=lambda k: 16,
+                  served=dict(inner=20, inner_tol=1e-3, precision="mixed", inner_form="projected"),
+                  plain=dict(inner=20, inner_tol=1e-3, precision="mixed", inner_form="reference")),
 }
 
 
@@ -202,8 +215,8 @@ def cpu_baseline(reqs, served, wl, workload, classes_wanted, threads, budget_s):
         ops = orc.CpuPencilOps(p, r["sigma"], 1)
         t = time.perf_counter()
         out = orc.solve(which, p.n, r["k"], r["sigma"], tol=1e-10, maxsub=wl["maxsub"](r["k"]),
-                        minsub=wl["minsub"](r["k"]), maxiter=r.get("maxiter", 1000), inner_maxiter=wl["inner"],
-                        inner_tol=wl["inner_tol"], ops=ops)
+                        minsub=wl["minsub"](r["k"]), maxiter=r.get("maxiter", 1000),
+                        inner_maxiter=wl["served"]["inner"], inner_tol=wl["served"]["inner_tol"], ops=ops)
         return out, time.perf_counter() - t
 
     # the first request (in workload order) of each wanted class
@@ -237,7 +250,10 @@ def cpu_baseline(reqs, served, wl, workload, classes_wanted, threads, budget_s):
             "kind": "reference" if which == "ref" else "port",
             "sample": (f"one request from each of the {len(done)} cheapest (n, k) classes of the workload "
                        f"(first in workload order), solved to tol 1e-10 with a CPU exact banded LU (LAPACK "
-                       f"zgbtrf + Schur complement of the tau rows): "
+                       f"zgbtrf + Schur complement of the tau rows)"
+                       + (f" as the preconditioner of an FP64 inner GMRES({wl['served']['inner']}) (the served "
+                          f"form's inner solve; the reference has no complex64 one)" if wl["served"]["inner"] else "")
+                       + ": "
                        + "; ".join(f"n={k[0]} k={k[1]}: {c} pairs {d:.1f}s" for k, _, c, d in done)),
             "classes": [list(k) for k, _, _, _ in done],
             "one_thread": one, "host_cpu": model, "host_cpus_available": avail}
@@ -284,6 +300,17 @@ def main():
                     help="seconds for the whole run: timed steps that would end past it, less what the "
                          "plain leg, the verification and the CPU baseline need, are not started (the "
                          "line then reports the steps timed)")
+    ap.add_argument("--served-inner", type=int, default=-1,
+                    help="inner GMRES steps of the served form (-1: the workload's default)")
+    ap.add_argument("--served-inner-tol", type=float, default=0.0,
+                    help="inner GMRES relative tolerance of the served form (0: the workload's default)")
+    ap.add_argument("--served-precision", choices=["fp64", "mixed"], default=None,
+                    help="inner-solve precision of the served form (default: the workload's)")
+    ap.add_argument("--served-inner-form", choices=["reference", "projected"], default=None,
+                    help="form of the served form's mixed inner solve (default: the workload's)")
+    ap.add_argument("--pipeline", type=int, default=5,
+                    help="timed steps per server run, each one arrival wave (1: every step drains before "
+                         "the next starts arriving)")
     ap.add_argument("--seed", type=int, default=20261203)
     args = ap.parse_args()
 
@@ -306,7 +333,19 @@ def main():
         _lib.load().bbjd_set_tile_table(0, None, 0, None, None, None)
         _lib.load().bbjd_set_tiles(args.proj_blocks, 0, 0)
     log(f"tile table: {'autotuned ' + tt['tuned'] if tt else 'engine defaults'}")
-    wl = WORKLOADS[args.workload]
+    wl = dict(WORKLOADS[args.workload])
+    wl["served"] = dict(wl["served"])
+    if args.served_inner >= 0:
+        wl["served"]["inner"] = args.served_inner
+    if args.served_inner_tol > 0:
+        wl["served"]["inner_tol"] = args.served_inner_tol
+    if args.served_precision:
+        wl["served"]["precision"] = args.served_precision
+    if args.served_inner_form:
+        wl["served"]["inner_form"] = args.served_inner_form
+    if args.workload == "small":
+        # the small solves' plain leg is the served form solved one at a time
+        wl["plain"] = dict(wl["served"])
     if args.nreq <= 0:
         args.nreq = 256 if args.workload == "serving" else 2048
     reqs = (make_requests(args.nreq, args.rate, args.seed) if args.workload == "serving"
@@ -320,7 +359,8 @@ def main():
     for i, r in enumerate(reqs):
         p = BandedTauPencil.generate(r["n"], *wl["shape"], seed=r["pseed"])
         norms.append(norm_lower_bound(p))
-        op = DeviceOperator(p, r["sigma"], pc="exact", c64=wl["precision"] == "mixed")
+        op = DeviceOperator(p, r["sigma"], pc="exact", c64="mixed" in (wl["served"]["precision"],
+                                                                      wl["plain"]["precision"]))
         op.pencil = None          # host copy no longer needed
         ops.append(op)
         # on-demand operators: the server factors and uploads each request's
@@ -331,21 +371,24 @@ def main():
     log(f"{len(ops)} operators resident ({op_bytes / 1e9:.1f} GB) in {t_setup:.1f}s; "
         f"free HBM {srv.device_free_bytes() / 1e9:.1f} GB")
 
-    def jobs_for(idx, with_arrival=True, on_demand=False):
+    def jobs_for(idx, with_arrival=True, on_demand=False, form="served", wave=0, keep=False):
         src = (lambda i: dict(pencil=pencils[i], pc="exact")) if on_demand else (lambda i: dict(op=ops[i]))
+        f = wl[form]
         return [dict(**src(i), sigma=reqs[i]["sigma"], nev=reqs[i]["k"], maxsub=wl["maxsub"](reqs[i]["k"]),
                      minsub=wl["minsub"](reqs[i]["k"]), maxiter=reqs[i]["maxiter"], tol=1e-10,
-                     inner_maxiter=wl["inner"], inner_tol=wl["inner_tol"],
-                     arrival=reqs[i]["arrival"] if with_arrival else 0.0) for i in idx]
+                     inner_maxiter=f["inner"], inner_tol=f["inner_tol"],
+                     arrival=reqs[i]["arrival"] if with_arrival else 0.0, wave=wave, keep_vectors=keep)
+                for i in idx]
 
     all_idx = list(range(len(reqs)))
-    served, walls = [], []
+    served, walls, run_steps = [], [], []
     steps_timed = 0
     worst = bad = checked = worst_abs = None
     if not args.plain_only:
         # ---- the resident server: workers, streams, workspaces (not timed) ----
         t = time.perf_counter()
-        server = srv.Server(jobs_for(all_idx), workers=args.workers, precision=wl["precision"],
+        server = srv.Server(jobs_for(all_idx), workers=args.workers, precision=wl["served"]["precision"],
+                            inner_form=wl["served"]["inner_form"],
                             sync_yield="block" if args.sync == "block" else True, pin=args.pin,
                             qz_mode=args.qz_mode)
         log(f"server: {server.workers} workers, {server.workspace_bytes / 1e9:.2f} GB workspace each, "
@@ -361,28 +404,46 @@ def main():
         # verification 2 s), with a margin
         reserve = (0 if args.served_only else 25.0) + (0 if args.no_cpu_baseline else args.cpu_seconds + 25.0) \
             + (0 if args.no_verify else 6.0)
-        for s in range(args.steps):
+        # Steps run in runs of up to --pipeline steps, each step one arrival
+        # wave of the run: a step's requests start arriving when the last
+        # request of the step before has been admitted, so the workers that
+        # free up in one step's tail take the next step's requests (the
+        # continuous stream of a server under load) instead of idling until
+        # the whole step has finished.  Every request of every timed step is
+        # solved inside the timed region.
+        nsteps_req = len(all_idx)
+        s = 0
+        per_step = None
+        while s < args.steps:
             elapsed = time.perf_counter() - T_START
-            est = max(walls) if walls else 0.0
-            if s > 0 and elapsed + est + reserve > args.time_budget:
-                log(f"time budget: step {s} of {args.steps} not started ({elapsed:.0f}s elapsed, "
-                    f"{est:.0f}s per step, {reserve:.0f}s reserved)")
-                break
+            c = min(args.pipeline, args.steps - s)
+            if per_step is not None:
+                fit = int((args.time_budget - elapsed - reserve) // per_step)
+                if fit < c:
+                    log(f"time budget: {c - max(fit, 0)} of the remaining {args.steps - s} steps not started "
+                        f"({elapsed:.0f}s elapsed, {per_step:.0f}s per step, {reserve:.0f}s reserved)")
+                    c = fit
+                if c <= 0:
+                    break
+            jobs = []
+            for w in range(c):
+                jobs += jobs_for(all_idx, on_demand=args.operators == "on-demand", wave=w,
+                                 keep=(s + w == 0 and not args.no_verify))
             free0 = srv.device_free_bytes()
             t = time.perf_counter()
-            res, rep = server.run(jobs_for(all_idx, on_demand=args.operators == "on-demand"),
-                                  realtime=wl["realtime"], order=args.order,
-                                  window=args.window, keep_vectors=(s == 0 and not args.no_verify))
+            res, rep = server.run(jobs, realtime=wl["realtime"], order=args.order, window=args.window)
             walls.append(time.perf_counter() - t)
-            steps_timed += 1
-            if s > 0:
-                for r in res:        # only the first step's pairs are kept for the check
-                    r.pop("eigenvectors", None)
-            served.append((res, rep))
-            log(f"step {s}: {rep['pairs']} pairs in {walls[-1]:.1f}s, workers {rep['workers']}, "
-                f"peak concurrency {rep['peak_concurrency']}, peak footprint {rep['peak_bytes'] / 1e9:.1f} GB, "
-                f"failed {rep['failed']}; free HBM "
+            run_steps.append(c)
+            per_step = walls[-1] / c
+            for w in range(c):
+                rw = res[w * nsteps_req:(w + 1) * nsteps_req]
+                served.append((rw, {"pairs": sum(r["nconv"] for r in rw)}))
+            steps_timed += c
+            log(f"steps {s}-{s + c - 1}: {rep['pairs']} pairs in {walls[-1]:.1f}s ({per_step:.1f}s per step), "
+                f"workers {rep['workers']}, peak concurrency {rep['peak_concurrency']}, peak footprint "
+                f"{rep['peak_bytes'] / 1e9:.1f} GB, failed {rep['failed']}; free HBM "
                 f"{free0 / 1e9:.1f} -> {srv.device_free_bytes() / 1e9:.1f} GB")
+            s += c
         server.close()
         if served and not args.no_verify:
             t = time.perf_counter()
@@ -395,7 +456,7 @@ def main():
 
     pairs = sum(rep["pairs"] for _, rep in served)
     wall = sum(walls)
-    lat = [r["t_done"] - r["arrival"] for res, _ in served for r in res]
+    lat = [r["t_done"] - r["t_arrive"] for res, _ in served for r in res]
     statuses = sorted(set(r["status"] for res, _ in served for r in res))
     conv_over_tol = sum(int(np.sum(r["residuals"] >= 1e-10)) for res, _ in served for r in res)
     if served:
@@ -406,7 +467,7 @@ def main():
     classes = {}
     for i, r in enumerate(reqs):
         classes.setdefault(class_of(r, args.workload), []).append(i)
-    plain_rate, per_cls, diff, sample, kcnt = None, {}, None, [], 0
+    plain_rate, per_cls, diff, sample, kcnt, nconv_mismatch = None, {}, None, [], 0, None
     if not args.served_only:
         cls_of = {}
         for key in sorted(classes):
@@ -417,7 +478,8 @@ def main():
                 cls_of[members[j]] = key
         srv.ktime_enable(True)
         t = time.perf_counter()
-        pres, prep = srv.serve(jobs_for(sample, False), workers=1, realtime=False, precision=wl["precision"],
+        pres, prep = srv.serve(jobs_for(sample, False, form="plain"), workers=1, realtime=False,
+                               precision=wl["plain"]["precision"], inner_form=wl["plain"]["inner_form"],
                                qz_mode=args.qz_mode)
         t_plain = time.perf_counter() - t
         srv.ktime_enable(False)
@@ -432,24 +494,32 @@ def main():
         est_secs = sum(len(classes[c]) * np.mean([d for _, d in v]) for c, v in per_cls.items())
         plain_rate = est_pairs / est_secs if est_secs else None
         if served:
-            # same eigenvalues alone and in the mix
+            # the same eigenvalue sets from the plain leg (alone) and the
+            # served form (in the mix): each value against its nearest in the
+            # other set, both ways; a count mismatch is reported as such
             mix = served[0][0]
-            diff = 0.0
+            diff, nconv_mismatch = 0.0, 0
             for j, i in enumerate(sample):
-                a, b = np.sort_complex(pres[j]["eigenvalues"]), np.sort_complex(mix[i]["eigenvalues"])
-                if a.size and a.size == b.size:
-                    diff = max(diff, float(np.max(np.abs(a - b) / np.abs(a))))
+                a, b = np.asarray(pres[j]["eigenvalues"]), np.asarray(mix[i]["eigenvalues"])
+                if a.size != b.size:
+                    nconv_mismatch += 1
+                if a.size and b.size:
+                    d_ab = max(float(np.min(np.abs(b - x)) / abs(x)) for x in a)
+                    d_ba = max(float(np.min(np.abs(a - x)) / abs(x)) for x in b)
+                    diff = max(diff, d_ab, d_ba)
             s_alone = sum(pres[j]["t_done"] - pres[j]["t_start"] for j in range(len(sample)))
             s_mix = sum(mix[i]["t_done"] - mix[i]["t_start"] for i in sample)
             q_alone = sum(pres[j]["qz_seconds"] for j in range(len(sample)))
             q_mix = sum(mix[i]["qz_seconds"] for i in sample)
-            busy = sum(r["t_done"] - r["t_start"] for r in mix)
+            first_run = [r for res_s, _ in served[:run_steps[0]] for r in res_s]
+            busy = sum(r["t_done"] - r["t_start"] for r in first_run)
             log(f"sample solve seconds alone {s_alone:.1f} / in the mix {s_mix:.1f} "
                 f"(qz {q_alone:.1f} / {q_mix:.1f}); mean solves in flight {busy / walls[0]:.1f}; "
-                f"engine stream {sum(r['stream_bytes'] for r in mix) / walls[0] / 1e9:.0f} GB/s over the step")
+                f"engine stream {sum(r['stream_bytes'] for r in first_run) / walls[0] / 1e9:.0f} GB/s over the "
+                f"first run of {run_steps[0]} steps")
         log(f"plain: {len(sample)} solves in {len(per_cls)} classes, {prep['pairs']} pairs / {t_plain:.1f}s; "
             f"class-weighted estimate for the workload {plain_rate:.2f} pairs/s; "
-            f"max rel eig diff plain vs served {diff}")
+            f"max rel eig diff plain vs served {diff}, pair-count mismatches {nconv_mismatch}")
 
     served_rate = pairs / wall if wall else None
     roof = None
@@ -502,7 +572,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c128" if args.workload == "serving" else "c128 (outer loop), c64 (inner solve)",
+        "dtype": ("c128 (outer loop), c64 (inner solve)" if wl["served"]["precision"] == "mixed"
+                  and wl["served"]["inner"] else "c128"),
         "data": "synthetic",
         "config": {"workload": (f"serving loop: {args.nreq} JDQZ requests, Poisson arrivals "
                                 f"{args.rate}/s, n in {{2e4,5e4,1e5,2e5}} (bs 8 block-tridiagonal, "
@@ -518,7 +589,7 @@ def main():
                    "admission": args.order + (f" {args.window}" if args.order == "window" else ""),
                    "host_wait": args.sync, "workers_pinned": bool(args.pin),
                    "projected_step": args.qz_mode, "operators": args.operators,
-                   "gmres_lookahead": args.gmres_lookahead},
+                   "gmres_lookahead": args.gmres_lookahead, "steps_per_run": args.pipeline},
         "steps_requested": args.steps,
         "served_eigenpairs_per_s": round(served_rate, 3) if served_rate else None,
         "plain_eigenpairs_per_s": round(plain_rate, 3) if plain_rate else None,
@@ -535,7 +606,14 @@ def main():
         "verified_max_residual": worst_abs,
         "verified_pairs_over_tol": bad,
         "max_rel_eig_err_vs_reference": eig_err,
-        "max_rel_eig_diff_alone_vs_mix": diff,
+        "max_rel_eig_diff_plain_vs_served": diff,
+        "plain_vs_served_pair_count_mismatches": nconv_mismatch,
+        "served_form": (f"{wl['served']['precision']} inner GMRES({wl['served']['inner']}), inner tol "
+                        f"{wl['served']['inner_tol']:g}, {wl['served']['inner_form']} form" if wl["served"]["inner"] else
+                        f"{wl['served']['precision']}, one-step correction"),
+        "plain_form": (f"{wl['plain']['precision']} inner GMRES({wl['plain']['inner']}), inner tol "
+                       f"{wl['plain']['inner_tol']:g}" if wl["plain"]["inner"] else
+                       f"{wl['plain']['precision']}, one-step correction"),
         "setup_seconds": round(t_setup, 1),
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -43,6 +43,14 @@ typedef struct {
                                               n-sized kernels (bbjd_stats) */
     const bbjd_zomplex     *v0;            /* host start vector (n) or NULL:
                                               bbjd_params.v0 of the solve    */
+    /* arrival waves: a request's arrival counts from the moment the last
+     * request of every earlier wave was admitted (wave 0: from the start of
+     * the run), so successive batches of a stream arrive as the backlog of
+     * the one before drains instead of after it has finished.  Waves are
+     * admitted in increasing order. */
+    int                     wave;
+    double                  t_arrive;      /* output: effective arrival, seconds
+                                              after the start of the run     */
 } bbjd_request;
 
 typedef struct {
@@ -67,6 +75,8 @@ typedef struct {
                              later arrivals                                  */
     int    qz_mode;       /* bbjd_gpu_options.qz_mode of every request
                              (BBJD_QZ_RITZ by default)                       */
+    int    inner_form;    /* bbjd_gpu_options.inner_form of every request
+                             (BBJD_INNER_REFERENCE by default)               */
 } bbjd_serve_config;
 
 enum { BBJD_ORDER_ARRIVAL = 0, BBJD_ORDER_LARGEST = 1, BBJD_ORDER_WINDOW = 2 };

@@ -789,6 +789,43 @@ def test_resident_server_runs_batches_without_allocating():
             o.free()
 
 
+def test_serving_loop_arrival_waves():
+    """arrival waves (bbjd_request.wave): the same six requests submitted as
+    three waves of one run on a resident server with two workers; a wave's
+    arrivals count from the admission of the previous wave's last request
+    (t_arrive = that admission + arrival), no request is admitted before it
+    arrives or before every request of an earlier wave, and each copy's
+    eigenvalues are bit-identical to the single-wave run"""
+    from bbjd_amd import serve as srv
+    jobs = _serve_jobs(seed=13)[:6]
+    ops = [DeviceOperator(j["pencil"], j["sigma"], pc="exact") for j in jobs]
+    resident = [dict(j, op=o) for j, o in zip(jobs, ops)]
+    for r in resident:
+        r.pop("pencil")
+    try:
+        with srv.Server(resident, workers=2) as server:
+            once, _ = server.run(resident, realtime=True)
+            waves = [dict(r, wave=w) for w in range(3) for r in resident]
+            res, rep = server.run(waves, realtime=True)
+        assert rep["failed"] == 0
+        m = len(resident)
+        for w in range(3):
+            part = res[w * m:(w + 1) * m]
+            base = 0.0 if w == 0 else max(r["t_admit"] for r in res[(w - 1) * m:w * m])
+            for i, r in enumerate(part):
+                assert r["wave"] == w
+                assert abs(r["t_arrive"] - (base + resident[i]["arrival"])) <= 1e-9, (w, i)
+                assert r["t_admit"] >= r["t_arrive"] - 1e-6, (w, i)
+                if w > 0:
+                    assert r["t_admit"] >= base - 1e-6, (w, i)
+                np.testing.assert_array_equal(r["eigenvalues"], once[i]["eigenvalues"], err_msg=f"{w} {i}")
+        # with two workers the next wave starts while the previous one still runs
+        assert min(r["t_start"] for r in res[m:2 * m]) < max(r["t_done"] for r in res[:m])
+    finally:
+        for o in ops:
+            o.free()
+
+
 def test_serving_loop_mixed_precision_alone_equal_mix():
     """the small-solve form of the serving loop: GMRES(20) in complex64 with
     the exact LU, FP32 accumulation, four workers; each request's eigenvalues
