@@ -25,6 +25,7 @@
 #include <omp.h>
 #include <pthread.h>
 #include <sched.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #include <time.h>
@@ -57,6 +58,11 @@ struct bbjd_server {
     bbjd_gpu_options  opt;
     int               init_failed;
     int               pin;              /* workers pinned to one CPU each    */
+    /* the current run's arrival waves: requests and pairs left per wave
+     * (index wave - wave_min), reported as each wave completes */
+    int              *wave_left;
+    int64_t          *wave_pairs;
+    int               wave_min, nwaves, progress;
 };
 
 static double now_s(void)
@@ -186,6 +192,15 @@ static void *worker_main(void *arg)
         pthread_mutex_unlock(&S->mu);
         run_request(S, w, q);
         pthread_mutex_lock(&S->mu);
+        if (S->wave_left) {
+            const int wi = q->wave - S->wave_min;
+            S->wave_pairs[wi] += q->nconv;
+            if (--S->wave_left[wi] == 0 && S->progress) {
+                fprintf(stderr, "[serve] wave %d done at %.1f s: %lld pairs\n", q->wave, q->t_done,
+                        (long long)S->wave_pairs[wi]);
+                fflush(stderr);
+            }
+        }
         S->used -= w->held;
         w->held = 0;
         w->req = -1;
@@ -329,7 +344,27 @@ int bbjd_server_run(bbjd_server *S, bbjd_request *reqs, int nreq, const bbjd_ser
     for (int i = 0; i < nreq; i++) order[i] = i;
     qsort_r(order, (size_t)nreq, sizeof(int), cmp_arrival, reqs);
 
+    /* per-wave counts for the completion reports */
+    int wmin = reqs[0].wave, wmax = reqs[0].wave;
+    for (int i = 1; i < nreq; i++) {
+        if (reqs[i].wave < wmin) wmin = reqs[i].wave;
+        if (reqs[i].wave > wmax) wmax = reqs[i].wave;
+    }
+    int *wave_left = NULL;
+    int64_t *wave_pairs = NULL;
+    const long nwv = (long)wmax - (long)wmin + 1;
+    if (nwv <= 65536) {
+        wave_left = calloc((size_t)nwv, sizeof(int));
+        wave_pairs = calloc((size_t)nwv, sizeof(int64_t));
+        if (!wave_left || !wave_pairs) { free(wave_left); free(wave_pairs); wave_left = NULL; wave_pairs = NULL; }
+        else for (int i = 0; i < nreq; i++) wave_left[reqs[i].wave - wmin]++;
+    }
     pthread_mutex_lock(&S->mu);
+    S->wave_left = wave_left;
+    S->wave_pairs = wave_pairs;
+    S->wave_min = wmin;
+    S->nwaves = (int)nwv;
+    S->progress = cfg->progress;
     S->reqs = reqs;
     S->used = (size_t)nw * S->wsb;
     S->peak = S->used;
@@ -414,7 +449,11 @@ int bbjd_server_run(bbjd_server *S, bbjd_request *reqs, int nreq, const bbjd_ser
     rep->peak_bytes = S->peak;
     rep->peak_concurrency = S->peak_active;
     S->reqs = NULL;
+    S->wave_left = NULL;
+    S->wave_pairs = NULL;
     pthread_mutex_unlock(&S->mu);
+    free(wave_left);
+    free(wave_pairs);
     for (int i = 0; i < nreq; i++) {
         rep->pairs += reqs[i].nconv;
         if (reqs[i].status < 0) rep->failed++;

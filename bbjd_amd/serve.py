@@ -37,7 +37,8 @@ class Request(C.Structure):
 class ServeConfig(C.Structure):
     _fields_ = [("workers", C.c_int), ("hbm_budget", C.c_size_t), ("precision", C.c_int),
                 ("realtime", C.c_int), ("sync_yield", C.c_int), ("graphs", C.c_int), ("order", C.c_int),
-                ("pin", C.c_int), ("window", C.c_int), ("qz_mode", C.c_int), ("inner_form", C.c_int)]
+                ("pin", C.c_int), ("window", C.c_int), ("qz_mode", C.c_int), ("inner_form", C.c_int),
+                ("progress", C.c_int)]
 
 
 class ServeReport(C.Structure):
@@ -129,7 +130,7 @@ def _requests(jobs, keep_vectors):
 
 
 def _config(lib, workers=16, precision="fp64", realtime=True, hbm_budget=0, sync_yield=True, graphs=False,
-            order="arrival", pin=False, window=8, qz_mode="ritz", inner_form="reference"):
+            order="arrival", pin=False, window=8, qz_mode="ritz", inner_form="reference", progress=False):
     if order not in ("arrival", "largest", "window"):
         raise ValueError("order must be 'arrival', 'largest' or 'window'")
     if int(window) < 1:
@@ -153,6 +154,7 @@ def _config(lib, workers=16, precision="fp64", realtime=True, hbm_budget=0, sync
     if inner_form not in forms:
         raise ValueError("inner_form must be 'reference' or 'projected'")
     cfg.inner_form = forms[inner_form]
+    cfg.progress = int(bool(progress))
     return cfg
 
 
@@ -228,10 +230,12 @@ class Server:
     def workspace_bytes(self):
         return int(self._lib.bbjd_server_workspace_bytes(self._h))
 
-    def run(self, jobs, realtime=True, order="arrival", window=8, keep_vectors=False):
+    def run(self, jobs, realtime=True, order="arrival", window=8, keep_vectors=False, progress=False):
+        """serve a batch; progress: a line on stderr as each arrival wave's
+        last request completes"""
         if not self._h:
             raise RuntimeError("server is closed")
-        cfg = _config(self._lib, realtime=realtime, order=order, window=window, **self._cfg_kw)
+        cfg = _config(self._lib, realtime=realtime, order=order, window=window, progress=progress, **self._cfg_kw)
         reqs, keep, outs = _requests(jobs, keep_vectors)
         rep = ServeReport()
         if self._lib.bbjd_server_run(self._h, reqs, len(jobs), C.byref(cfg), C.byref(rep)):

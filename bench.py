@@ -308,7 +308,7 @@ def main():
                     help="inner-solve precision of the served form (default: the workload's)")
     ap.add_argument("--served-inner-form", choices=["reference", "projected"], default=None,
                     help="form of the served form's mixed inner solve (default: the workload's)")
-    ap.add_argument("--pipeline", type=int, default=5,
+    ap.add_argument("--pipeline", type=int, default=10,
                     help="timed steps per server run, each one arrival wave (1: every step drains before "
                          "the next starts arriving)")
     ap.add_argument("--seed", type=int, default=20261203)
@@ -431,7 +431,8 @@ def main():
                                  keep=(s + w == 0 and not args.no_verify))
             free0 = srv.device_free_bytes()
             t = time.perf_counter()
-            res, rep = server.run(jobs, realtime=wl["realtime"], order=args.order, window=args.window)
+            res, rep = server.run(jobs, realtime=wl["realtime"], order=args.order, window=args.window,
+                                  progress=True)
             walls.append(time.perf_counter() - t)
             run_steps.append(c)
             per_step = walls[-1] / c

@@ -77,6 +77,8 @@ typedef struct {
                              (BBJD_QZ_RITZ by default)                       */
     int    inner_form;    /* bbjd_gpu_options.inner_form of every request
                              (BBJD_INNER_REFERENCE by default)               */
+    int    progress;      /* 1: a line on stderr as each arrival wave's last
+                             request completes                               */
 } bbjd_serve_config;
 
 enum { BBJD_ORDER_ARRIVAL = 0, BBJD_ORDER_LARGEST = 1, BBJD_ORDER_WINDOW = 2 };

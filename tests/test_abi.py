@@ -110,7 +110,7 @@ int main(void) {
   O(bbjd_serve_config, workers); O(bbjd_serve_config, hbm_budget); O(bbjd_serve_config, precision);
   O(bbjd_serve_config, realtime); O(bbjd_serve_config, sync_yield); O(bbjd_serve_config, graphs);
   O(bbjd_serve_config, order); O(bbjd_serve_config, pin); O(bbjd_serve_config, window);
-  O(bbjd_serve_config, qz_mode); O(bbjd_serve_config, inner_form);
+  O(bbjd_serve_config, qz_mode); O(bbjd_serve_config, inner_form); O(bbjd_serve_config, progress);
   return 0;
 }
 """
