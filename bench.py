@@ -289,6 +289,8 @@ def main():
                     help="Arnoldi steps of the inner GMRES in flight (1: one at a time)")
     ap.add_argument("--proj-blocks", type=int, default=0,
                     help="blocks per projection launch for every shape (0: the tile table or the engine default)")
+    ap.add_argument("--mid-blocks", type=int, default=0,
+                    help="largest grid of the CGS2 middle pass (0: engine default)")
     ap.add_argument("--fuse-blocks", type=int, default=0,
                     help="finish reductions in the last block for grids up to this many blocks (0: engine default)")
     ap.add_argument("--operators", choices=["resident", "on-demand"], default="resident",
@@ -329,6 +331,8 @@ def main():
     _lib.load().bbjd_set_gmres_lookahead(args.gmres_lookahead)
     if args.fuse_blocks > 0:
         _lib.load().bbjd_set_fuse_blocks(args.fuse_blocks)
+    if args.mid_blocks > 0:
+        _lib.load().bbjd_set_cgs_mid_blocks(args.mid_blocks)
     if args.proj_blocks > 0:
         _lib.load().bbjd_set_tile_table(0, None, 0, None, None, None)
         _lib.load().bbjd_set_tiles(args.proj_blocks, 0, 0)
