@@ -18,8 +18,10 @@ preconditioner) as A (B None) and the same operator, or True, as
 SuperLU preconditioner and add ``device=True``.  Then the matvecs and the
 preconditioner run on the GPU as well and no n-vector crosses PCIe during
 the iteration.  ``precision='mixed'`` runs the inner
-GMRES (inner_maxiter > 0) in complex64 with FP32 accumulation, and
-``graphs=True`` replays its steps from captured HIP graphs.
+GMRES (inner_maxiter > 0) in complex64 with FP32 accumulation
+(``inner_form='projected'``: on the projected preconditioner, one oblique
+projection per step), and ``graphs=True`` replays its steps from captured
+HIP graphs.
 """
 from __future__ import annotations
 
@@ -99,7 +101,8 @@ def _start_vector(v0, n):
 def jdqz(A, B=None, k: int = 6, sigma: complex = 0.0, which: str = 'nearest', tol: float = 1e-10,
          maxiter: int = 200, maxsub: int = 0, minsub: int = 0, inner_maxiter: int = 0,
          inner_tol: float = 1e-2, preconditioner=None, v0=None, verbose: int = 0,
-         return_info: bool = False, precision: str = 'fp64', graphs: bool = False, device: bool = False):
+         return_info: bool = False, precision: str = 'fp64', graphs: bool = False, device: bool = False,
+         inner_form: str = 'reference'):
     """Jacobi-Davidson QZ for A x = lambda B x, eigenvalues nearest sigma
     (or by `which`).
 
@@ -139,7 +142,7 @@ def jdqz(A, B=None, k: int = 6, sigma: complex = 0.0, which: str = 'nearest', to
                              None if preconditioner is None else True, tol=tol, maxiter=maxiter,
                              maxsub=maxsub, minsub=minsub, inner_maxiter=inner_maxiter,
                              inner_tol=inner_tol, verbose=verbose, v0=v0, device_op=A,
-                             precision=precision, graphs=graphs)
+                             precision=precision, graphs=graphs, inner_form=inner_form)
         return _finish(out, k, verbose, return_info)
 
     # host operands: each becomes a function x -> y (complex128) by the first
@@ -163,7 +166,7 @@ def jdqz(A, B=None, k: int = 6, sigma: complex = 0.0, which: str = 'nearest', to
                          matvec_B=matvec_B, precond_solve=precond_solve, tol=tol,
                          maxiter=maxiter, maxsub=maxsub, minsub=minsub,
                          inner_maxiter=inner_maxiter, inner_tol=inner_tol, verbose=verbose,
-                         v0=v0, precision=precision)
+                         v0=v0, precision=precision, inner_form=inner_form)
     return _finish(out, k, verbose, return_info)
 
 

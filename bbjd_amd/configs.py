@@ -72,7 +72,7 @@ def rel_diff(a, b):
 
 
 def plain(cfg, n, bs, hw, ntau, k, sigma, maxsub, minsub, pc, seed, inner=0, inner_tol=1e-2,
-          precision="fp64", maxiter=2000, verbose=1, pc_shift=0.0, graphs=False):
+          precision="fp64", maxiter=2000, verbose=1, pc_shift=0.0, graphs=False, inner_form="reference"):
     t0 = time.perf_counter()
     p = BandedTauPencil.generate(n, bs, hw, ntau, seed=seed)
     op = DeviceOperator(p, sigma + pc_shift, pc=pc, c64=precision == "mixed")
@@ -80,12 +80,14 @@ def plain(cfg, n, bs, hw, ntau, k, sigma, maxsub, minsub, pc, seed, inner=0, inn
     t = time.perf_counter()
     ev, X, info = jdqz(op, k=k, sigma=sigma, tol=1e-10, maxsub=maxsub, minsub=minsub, maxiter=maxiter,
                        inner_maxiter=inner, inner_tol=inner_tol, preconditioner=op, return_info=True,
-                       precision=precision, verbose=verbose, graphs=graphs)
+                       precision=precision, verbose=verbose, graphs=graphs,
+                       inner_form=inner_form if precision == "mixed" else "reference")
     dt = time.perf_counter() - t
     res = pair_residuals(op, p, ev, X) if len(ev) else np.zeros(0)
     return {"config": cfg, "n": n, "k": k, "sigma": [sigma.real, sigma.imag], "maxsub": maxsub,
             "minsub": minsub, "preconditioner": pc + (f" at sigma + {pc_shift}" if pc_shift else ""),
-            "inner_maxiter": inner, "precision": precision, "graphs": bool(graphs), "_ev": ev,
+            "inner_maxiter": inner, "precision": precision, "graphs": bool(graphs),
+            "inner_form": inner_form if precision == "mixed" else "reference", "_ev": ev,
             "nconv": int(info["nconv"]), "iters": int(info["iters"]), "status": int(info["status"]),
             "solve_seconds": round(dt, 2), "eigenpairs_per_s": round(info["nconv"] / dt, 3),
             "setup_seconds": round(t_setup, 2), "device_bytes": op.device_bytes,
@@ -152,6 +154,9 @@ def main():
     ap.add_argument("--pc", choices=["block_jacobi", "shifted", "exact"], default="block_jacobi",
                     help="preconditioner: config 3 block_jacobi or shifted, config 5 block_jacobi or exact")
     ap.add_argument("--pc-shift", type=float, default=0.02, help="config 3 shifted: delta")
+    ap.add_argument("--inner-form", choices=["reference", "projected"], default="reference",
+                    help="config 3: the complex64 inner solve's form")
+    ap.add_argument("--n", type=int, default=0, help="config 3: a smaller n (0: 250000)")
     ap.add_argument("--graphs", action="store_true",
                     help="configs 3 and 5: replay the inner GMRES steps from captured HIP graphs")
     a = ap.parse_args()
@@ -165,10 +170,12 @@ def main():
     elif a.config == 3:
         shift = a.pc_shift if a.pc == "shifted" else 0.0
         pc = "exact" if a.pc == "shifted" else "block_jacobi"
-        out = plain(3, 250_000, 8, 2, 8, 32, 1 + 2j, 120, 48, pc, a.seed, inner=50, inner_tol=1e-3,
-                    precision="mixed", verbose=a.verbose, pc_shift=shift, graphs=a.graphs)
+        n3 = a.n or 250_000
+        out = plain(3, n3, 8, 2, 8, 32, 1 + 2j, 120, 48, pc, a.seed, inner=50, inner_tol=1e-3,
+                    precision="mixed", verbose=a.verbose, pc_shift=shift, graphs=a.graphs,
+                    inner_form=a.inner_form)
         if out["nconv"]:
-            ref = plain(3, 250_000, 8, 2, 8, 32, 1 + 2j, 120, 48, pc, a.seed, inner=50, inner_tol=1e-3,
+            ref = plain(3, n3, 8, 2, 8, 32, 1 + 2j, 120, 48, pc, a.seed, inner=50, inner_tol=1e-3,
                         precision="fp64", verbose=a.verbose, pc_shift=shift)
             out["fp64_nconv"] = ref["nconv"]
             out["fp64_solve_seconds"] = ref["solve_seconds"]
