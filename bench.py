@@ -461,6 +461,16 @@ def main():
 
     pairs = sum(rep["pairs"] for _, rep in served)
     wall = sum(walls)
+    # the engine's algorithmic HBM bytes (its n-sized kernels; the operators'
+    # applies are not counted) over the served region, and the solves in
+    # flight on average
+    served_region = None
+    if served and wall:
+        sb = sum(r["stream_bytes"] for res, _ in served for r in res)
+        busy_all = sum(r["t_done"] - r["t_start"] for res, _ in served for r in res)
+        served_region = {"engine_stream_GBs": round(sb / wall / 1e9, 1),
+                         "frac_of_hbm_peak": round(sb / wall / 1e9 / HBM_PEAK_GBS, 4),
+                         "mean_solves_in_flight": round(busy_all / wall, 2)}
     lat = [r["t_done"] - r["t_arrive"] for res, _ in served for r in res]
     statuses = sorted(set(r["status"] for res, _ in served for r in res))
     conv_over_tol = sum(int(np.sum(r["residuals"] >= 1e-10)) for res, _ in served for r in res)
@@ -621,6 +631,7 @@ def main():
                        f"{wl['plain']['precision']}, one-step correction"),
         "setup_seconds": round(t_setup, 1),
         "roofline": roof,
+        "served_region": served_region,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
