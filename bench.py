@@ -30,8 +30,14 @@ freed in one step's tail take the next step's requests.
                of its requests (seeded) solved alone -- gives each class's mean
                pairs and seconds per request; weighted by the class counts of
                the 256 requests they estimate the plain path's eigenpairs/s
-               on the whole workload; the sampled requests' eigenvalue sets
-               are compared with the served ones
+               on the whole workload.  The line's estimate weights each
+               request by its outer iterations, known exactly from the first
+               served step (the served form runs the plain arithmetic, so the
+               trajectories are the same; checked on the sample): plain time
+               = sum of iterations x the class's sampled seconds per
+               iteration.  The class-mean estimate is reported beside it.
+               The sampled requests' eigenvalue sets are compared with the
+               served ones
   latency      p50/p99 of completion - arrival over the timed requests
   roofline     the projection kernel Q^H x (the dominant streaming kernel):
                durations of every first-pass launch of the plain leg (one
@@ -483,6 +489,7 @@ def main():
     for i, r in enumerate(reqs):
         classes.setdefault(class_of(r, args.workload), []).append(i)
     plain_rate, per_cls, diff, sample, kcnt, nconv_mismatch = None, {}, None, [], 0, None
+    plain_class_mean, plain_estimator = None, None
     if not args.served_only:
         cls_of = {}
         for key in sorted(classes):
@@ -502,12 +509,36 @@ def main():
         for j, i in enumerate(sample):
             r = pres[j]
             dt = r["t_done"] - r["t_start"]
-            per_cls.setdefault(cls_of[i], []).append((r["nconv"], dt))
+            per_cls.setdefault(cls_of[i], []).append((r["nconv"], dt, r["iters"]))
             log(f"  plain n={reqs[i]['n']} k={reqs[i]['k']}: {r['nconv']} pairs, {r['iters']} iters, "
                 f"{dt:.2f}s (qz {r['qz_seconds']:.2f}s), status {r['status']}")
-        est_pairs = sum(len(classes[c]) * np.mean([p for p, _ in v]) for c, v in per_cls.items())
-        est_secs = sum(len(classes[c]) * np.mean([d for _, d in v]) for c, v in per_cls.items())
-        plain_rate = est_pairs / est_secs if est_secs else None
+        # class means of the sampled solves, weighted by the class counts
+        est_pairs = sum(len(classes[c]) * np.mean([v[0] for v in vs]) for c, vs in per_cls.items())
+        est_secs = sum(len(classes[c]) * np.mean([v[1] for v in vs]) for c, vs in per_cls.items())
+        plain_class_mean = est_pairs / est_secs if est_secs else None
+        plain_rate, plain_estimator = plain_class_mean, "class means of the sampled solves, weighted by class counts"
+        if served:
+            # Iteration-weighted: a request's plain time is its outer
+            # iterations times its class's seconds per iteration (from the
+            # sampled solves).  The served form runs the plain form's
+            # arithmetic, so every request's iterations and pairs are known
+            # exactly from the first served step (checked on the sample).
+            first = served[0][0]
+            iters_match = all(pres[j]["iters"] == first[i]["iters"] and pres[j]["nconv"] == first[i]["nconv"]
+                              for j, i in enumerate(sample))
+            if iters_match:
+                spi = {c: sum(v[1] for v in vs) / max(1, sum(v[2] for v in vs)) for c, vs in per_cls.items()}
+                pred_secs = sum(first[i]["iters"] * spi[class_of(reqs[i], args.workload)]
+                                for i in range(len(first)) if class_of(reqs[i], args.workload) in spi)
+                pred_pairs = sum(first[i]["nconv"] for i in range(len(first))
+                                 if class_of(reqs[i], args.workload) in spi)
+                if pred_secs > 0:
+                    plain_rate = pred_pairs / pred_secs
+                    plain_estimator = ("every request's outer iterations (from the first served step, the plain "
+                                       "form's trajectory) times its class's plain seconds per iteration from "
+                                       "the sampled solves")
+            log(f"plain estimators: iteration-weighted {plain_rate:.2f}, class means {plain_class_mean:.2f} "
+                f"pairs/s; sampled iterations equal the served ones: {iters_match}")
         if served:
             # the same eigenvalue sets from the plain leg (alone) and the
             # served form (in the mix): each value against its nearest in the
@@ -533,7 +564,7 @@ def main():
                 f"engine stream {sum(r['stream_bytes'] for r in first_run) / walls[0] / 1e9:.0f} GB/s over the "
                 f"first run of {run_steps[0]} steps")
         log(f"plain: {len(sample)} solves in {len(per_cls)} classes, {prep['pairs']} pairs / {t_plain:.1f}s; "
-            f"class-weighted estimate for the workload {plain_rate:.2f} pairs/s; "
+            f"estimate for the workload {plain_rate:.2f} pairs/s; "
             f"max rel eig diff plain vs served {diff}, pair-count mismatches {nconv_mismatch}")
 
     served_rate = pairs / wall if wall else None
@@ -567,8 +598,8 @@ def main():
                 eig_err = max(e for _, e, _, _ in errs)
             # the plain GPU leg's rate on the same classes, for a like-for-like ratio
             if done:
-                gp = sum(np.mean([p for p, _ in per_cls[k]]) for k, _, _, _ in done)
-                gs = sum(np.mean([d for _, d in per_cls[k]]) for k, _, _, _ in done)
+                gp = sum(np.mean([v[0] for v in per_cls[k]]) for k, _, _, _ in done)
+                gs = sum(np.mean([v[1] for v in per_cls[k]]) for k, _, _, _ in done)
                 cpu["gpu_plain_on_sample_classes"] = round(gp / gs, 3) if gs else None
             log(f"cpu baseline: {cpu['value']} eigenpairs/s ({cpu['kind']}, {cpu['cores']} threads, "
                 f"{cpu['host_cpu']}); one thread {cpu['one_thread']}; max rel eig err served vs reference "
@@ -610,7 +641,8 @@ def main():
         "plain_eigenpairs_per_s": round(plain_rate, 3) if plain_rate else None,
         "served_over_plain": round(served_rate / plain_rate, 2) if served_rate and plain_rate else None,
         "plain_sample_requests": len(sample),
-        "plain_estimate": "per class mean of its sampled solves alone, weighted by class counts",
+        "plain_estimate": plain_estimator if not args.served_only else None,
+        "plain_class_mean_eigenpairs_per_s": round(plain_class_mean, 3) if plain_class_mean else None,
         "latency_p50_s": round(pct(lat, 50), 3) if lat else None,
         "latency_p99_s": round(pct(lat, 99), 3) if lat else None,
         "pairs": pairs,
