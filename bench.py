@@ -56,9 +56,10 @@ freed in one step's tail take the next step's requests.
 --workload small: the batched small solves instead -- 2048 pencils with n
 log-uniform on [4096, 32768] (block size 4, block-tridiagonal, 2 tau rows),
 k = 8, maxsub 32, minsub 16, sigma uniform on the unit disk, GMRES(20) in
-complex64 (projected-preconditioner form) with the exact banded LU as its
-preconditioner, all submitted at once; plain = a random sample of them solved
-one at a time in the same form.
+complex64 (the reference's GMRES; --served-inner-form projected for the
+projected preconditioner) with the exact banded LU as its preconditioner,
+all submitted at once; plain = a random sample of them solved one at a time
+in the same form.
 
 Usage: python bench.py [--gpus 1] [--steps 1] [--warmup 1] [--nreq 256]
        python bench.py --plain-only ...   (the plain leg alone: for profilers)
@@ -118,7 +119,7 @@ WORKLOADS = {
                     served=dict(inner=0, inner_tol=1e-2, precision="fp64", inner_form="projected"),
                     plain=dict(inner=0, inner_tol=1e-2, precision="fp64", inner_form="reference")),
     "small": dict(shape=(4, 1, 2), realtime=False, maxsub=lambda k: 32, minsub=lambda k: 16,
-                  served=dict(inner=20, inner_tol=1e-3, precision="mixed", inner_form="projected"),
+                  served=dict(inner=20, inner_tol=1e-3, precision="mixed", inner_form="reference"),
                   plain=dict(inner=20, inner_tol=1e-3, precision="mixed", inner_form="reference")),
 }
 
@@ -223,7 +224,23 @@ def cpu_baseline(reqs, served, wl, workload, classes_wanted, threads, budget_s):
         out = orc.solve(which, p.n, r["k"], r["sigma"], tol=1e-10, maxsub=wl["maxsub"](r["k"]),
                         minsub=wl["minsub"](r["k"]), maxiter=r.get("maxiter", 1000),
                         inner_maxiter=wl["served"]["inner"], inner_tol=wl["served"]["inner_tol"], ops=ops)
-        return out, time.perf_counter() - t
+        return out, time.perf_counter() - t, p, ops
+
+    def ranks(p, ops, sigma, k, sets):
+        """each set's eigenvalues by their rank in the distance to sigma among
+        the pencil's k + 24 eigenvalues nearest sigma (ARPACK on the shift-
+        and-invert operator (A - sigma B)^-1 B with the CPU banded LU): rank
+        list 0 .. k-1 means the k nearest; None when ARPACK fails"""
+        from scipy.sparse.linalg import LinearOperator, eigs
+        try:
+            A, Bm = p.to_scipy()
+            op = LinearOperator(A.shape, matvec=lambda x: ops.apply("orc_precond", Bm @ x), dtype=np.complex128)
+            mu = eigs(op, k=k + 24, which="LM", return_eigenvectors=False, tol=1e-12, ncv=max(3 * (k + 24), 96))
+        except Exception:  # noqa: BLE001
+            return [None] * len(sets)
+        w = sigma + 1.0 / mu
+        w = w[np.argsort(np.abs(w - sigma))]
+        return [sorted(int(np.argmin(np.abs(w - e))) for e in np.asarray(ev)) for ev in sets]
 
     # the first request (in workload order) of each wanted class
     first = {}
@@ -234,7 +251,7 @@ def cpu_baseline(reqs, served, wl, workload, classes_wanted, threads, budget_s):
         if secs >= budget_s or key not in first:
             break
         i = first[key]
-        out, dt = run(i, threads)
+        out, dt, p, ops = run(i, threads)
         pairs += out["nconv"]
         secs += dt
         done.append((key, i, out["nconv"], dt))
@@ -244,11 +261,14 @@ def cpu_baseline(reqs, served, wl, workload, classes_wanted, threads, budget_s):
             if ge.size and ce.size:
                 # each served eigenvalue against its nearest reference one
                 e = max(float(np.min(np.abs(ce - g)) / abs(g)) for g in ge)
-                errs.append((i, e, len(ge), len(ce)))
+                # which eigenvalues each returned, by distance rank to sigma
+                # (outside the timed CPU solve)
+                rs, rr = ranks(p, ops, reqs[i]["sigma"], reqs[i]["k"], [ge, ce]) if p.n <= 60000 else (None, None)
+                errs.append((i, e, len(ge), len(ce), rs, rr))
     one = None
     if done:
         i = done[0][1]          # the cheapest class's request
-        out1, dt1 = run(i, 1)
+        out1, dt1, _, _ = run(i, 1)
         one = {"value": out1["nconv"] / dt1, "cores": 1, "sample": f"request {i} (n={reqs[i]['n']}, "
                f"k={reqs[i]['k']}): {out1['nconv']} pairs in {dt1:.2f} s"}
     model, avail = host_cpu()
@@ -585,7 +605,7 @@ def main():
                 "kernel": "k_proj<double,1,16,false> (Q^H x, complex128)", "measured_on": "plain leg",
                 "launches": kcnt, "avg_us": round(kms * 1e3 / kcnt, 2), "avg_bytes": round(kbytes / kcnt)}
 
-    cpu, eig_err = None, None
+    cpu, eig_err, sets = None, None, None
     if not args.no_cpu_baseline and not args.plain_only and not args.served_only:
         thr = args.cpu_threads or min(16, host_cpu()[1])
         # the cheapest classes for the CPU first: its cost grows with n and,
@@ -595,7 +615,16 @@ def main():
             cpu, errs, done = cpu_baseline(reqs, served[0][0] if served else None, wl, args.workload, order,
                                            thr, args.cpu_seconds)
             if errs:
-                eig_err = max(e for _, e, _, _ in errs)
+                eig_err = max(e[1] for e in errs)
+                for i, e, ng, nc, rs, rr in errs:
+                    log(f"  request {i}: served {ng} / reference {nc} eigenvalues, max rel diff {e:.2e}; "
+                        f"distance ranks served {rs}, reference {rr}")
+                knear = lambda rk, k: rk is not None and rk == list(range(k))
+                sets = {"requests": len(errs),
+                        "equal_within_1e-8": sum(e[1] <= 1e-8 for e in errs),
+                        "served_are_the_k_nearest": sum(knear(e[4], reqs[e[0]]["k"]) for e in errs),
+                        "reference_are_the_k_nearest": sum(knear(e[5], reqs[e[0]]["k"]) for e in errs),
+                        "ranks": [{"request": e[0], "served": e[4], "reference": e[5]} for e in errs]}
             # the plain GPU leg's rate on the same classes, for a like-for-like ratio
             if done:
                 gp = sum(np.mean([v[0] for v in per_cls[k]]) for k, _, _, _ in done)
@@ -653,6 +682,7 @@ def main():
         "verified_max_residual": worst_abs,
         "verified_pairs_over_tol": bad,
         "max_rel_eig_err_vs_reference": eig_err,
+        "eigenvalue_sets_vs_reference": sets,
         "max_rel_eig_diff_plain_vs_served": diff,
         "plain_vs_served_pair_count_mismatches": nconv_mismatch,
         "served_form": (f"{wl['served']['precision']} inner GMRES({wl['served']['inner']}), inner tol "
