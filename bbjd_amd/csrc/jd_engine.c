@@ -90,9 +90,24 @@ struct bbjd_dev {
     size_t           look_slot;
     hipEvent_t       lev[GM_LOOK_MAX];
     int              nlev;
+    /* bbjd_gpu_options.blocking_sync: host waits on the workspace's stream
+     * sleep on this event (created with hipEventBlockingSync) instead of
+     * the process's default wait */
+    hipEvent_t       bev;
 };
 
 static __thread hipStream_t tl_cb_stream;
+
+/* Waits for the work enqueued on s (a workspace's stream): through the
+ * workspace's blocking event when it has one, else hipStreamSynchronize. */
+static hipError_t dsync(const struct bbjd_dev *d, hipStream_t s)
+{
+    if (d->bev) {
+        const hipError_t e = hipEventRecord(d->bev, s);
+        return e != hipSuccess ? e : hipEventSynchronize(d->bev);
+    }
+    return hipStreamSynchronize(s);
+}
 void *bbjd_callback_stream(void) { return (void *)tl_cb_stream; }
 
 static double now_s(void)
@@ -266,6 +281,7 @@ static void workspace_free_impl(bbjd_workspace *ws)
         if (d->hqzd) hipHostFree(d->hqzd);
         if (d->hlook) hipHostFree(d->hlook);
         for (int i = 0; i < d->nlev; i++) (void)hipEventDestroy(d->lev[i]);
+        if (d->bev) (void)hipEventDestroy(d->bev);
         if (d->dblock) hipFree(d->dblock);
         if (d->hscal) hipHostFree(d->hscal);
         if (d->hsmall) hipHostFree(d->hsmall);
@@ -387,7 +403,9 @@ static bbjd_workspace *workspace_alloc_impl(const bbjd_params *p, const bbjd_gpu
         d->look_slot = ((size_t)(L.ims + 1) * sizeof(zcx) + 255) / 256 * 256 + 256;
         if (!HIPOK(hipHostMalloc((void **)&d->hlook, GM_LOOK_MAX * d->look_slot, 0))) { d->hlook = NULL; goto fail; }
         for (; d->nlev < GM_LOOK_MAX; d->nlev++)
-            if (!HIPOK(hipEventCreateWithFlags(&d->lev[d->nlev], hipEventDisableTiming))) goto fail;
+            if (!HIPOK(hipEventCreateWithFlags(&d->lev[d->nlev], hipEventDisableTiming |
+                                                                      (o->blocking_sync ? hipEventBlockingSync : 0))))
+                goto fail;
     }
     d->gx_cap = (int)L.ims + 1;
     if (!(d->gx = calloc((size_t)d->gx_cap, sizeof(hipGraphExec_t)))) goto fail;
@@ -402,6 +420,11 @@ static bbjd_workspace *workspace_alloc_impl(const bbjd_params *p, const bbjd_gpu
         !ws->gmres_sn || !ws->gmres_g || !d->al || !d->be || !d->yv || !d->xv || !d->pair_res)
         goto fail;
     if (bbjd_qz_init(&d->qz, L.maxsub)) goto fail;
+    if (o->blocking_sync &&
+        !HIPOK(hipEventCreateWithFlags(&d->bev, hipEventDisableTiming | hipEventBlockingSync))) {
+        d->bev = NULL;
+        goto fail;
+    }
     if (!HIPOK(hipStreamSynchronize(d->stream))) goto fail;
     return ws;
 fail:
@@ -516,7 +539,7 @@ static int sync_scal(ctx_t *c)
 {
     struct bbjd_dev *d = c->d;
     if (!HIPOK(hipMemcpyAsync(d->hscal, d->rd.scal, SL_COUNT * 8, hipMemcpyDeviceToHost, c->s))) return -1;
-    if (!HIPOK(hipStreamSynchronize(c->s))) return -1;
+    if (!HIPOK(dsync(c->d, c->s))) return -1;
     return 0;
 }
 
@@ -533,7 +556,7 @@ static int apply_op(ctx_t *c, bbjd_matvec_fn f, const void *x, void *y)
     }
     const size_t b = (size_t)c->n * sizeof(zcx);
     if (!HIPOK(hipMemcpyAsync(d->hx, x, b, hipMemcpyDeviceToHost, c->s))) return -1;
-    if (!HIPOK(hipStreamSynchronize(c->s))) return -1;
+    if (!HIPOK(dsync(c->d, c->s))) return -1;
     int rc = f(c->n, (const bbjd_zomplex *)d->hx, (bbjd_zomplex *)d->hy, c->P->op_data);
     if (rc) return rc;
     if (!HIPOK(hipMemcpyAsync(y, d->hy, b, hipMemcpyHostToDevice, c->s))) return -1;
@@ -746,7 +769,7 @@ static int pp_prepare(ctx_t *c, bbjd_int K)
     }
     const size_t cnt = (size_t)nnew * K + (size_t)nnew * v0;
     rc |= !HIPOK(hipMemcpyAsync(d->hpp, d->hnew, cnt * sizeof(c64), hipMemcpyDeviceToHost, c->s));
-    if (rc || !HIPOK(hipStreamSynchronize(c->s))) return -2;
+    if (rc || !HIPOK(dsync(c->d, c->s))) return -2;
     zcx *H = d->hh;
     const c64 *hc = d->hpp, *hr = d->hpp + (size_t)nnew * K;
     for (bbjd_int j = v0; j < K; j++) {
@@ -911,7 +934,7 @@ static int gmres(ctx_t *c, bbjd_int ndefl, const void *QR, const void *QL, zcx t
         key.op = P->op_data;
         key.pp = c->pp;
         if (memcmp(&key, &d->gkey, sizeof(key))) {
-            rc |= !HIPOK(hipStreamSynchronize(c->s));
+            rc |= !HIPOK(dsync(c->d, c->s));
             graphs_drop(d);
             d->gkey = key;
         }
@@ -958,7 +981,7 @@ static int gmres(ctx_t *c, bbjd_int ndefl, const void *QR, const void *QL, zcx t
         rc = seg_end(c, 0, rc | cb);
     }
     if (cb) return -1;
-    if (rc || !HIPOK(hipStreamSynchronize(c->s))) return -2;
+    if (rc || !HIPOK(dsync(c->d, c->s))) return -2;
     const double beta0 = d->hscal[SL_G];
     if (beta0 < 1e-300) {
         return HIPOK(hipMemsetAsync(z, 0, (size_t)n * 16, c->s)) ? 0 : -2;
@@ -1001,7 +1024,7 @@ static int gmres(ctx_t *c, bbjd_int ndefl, const void *QR, const void *QL, zcx t
                 if (er == -2) rc = -2;
             }
             if (cb) return -1;
-            if (rc || !HIPOK(hipStreamSynchronize(c->s))) return -2;
+            if (rc || !HIPOK(dsync(c->d, c->s))) return -2;
         }
         d->st.inner_steps++;
         d->st.stream_bytes += (double)esz * n * ((c->pp ? 3.0 : 8.0) * ndefl + 4.0 * (j + 1) + 8);
@@ -1091,7 +1114,7 @@ static int pencil_update(ctx_t *c, bbjd_int k, int rowrel, int col_done)
      * slots (the norms of the expansion, for the breakdown checks) */
     const size_t gap = (size_t)((const char *)d->rd.scal - (const char *)pen);
     rc |= !HIPOK(hipMemcpyAsync(d->hcol, pen, gap + SL_COUNT * 8, hipMemcpyDeviceToHost, c->s));
-    rc |= !HIPOK(hipStreamSynchronize(c->s));
+    rc |= !HIPOK(dsync(c->d, c->s));
     if (rc) return -1;
     memcpy(d->hscal, (const char *)d->hcol + gap, SL_COUNT * 8);
     zcx *MA = (zcx *)ws->MA, *MB = (zcx *)ws->MB;
@@ -1124,7 +1147,7 @@ static int cholqr_pass(ctx_t *c, bbjd_int keep, int check)
     zcx *G = d->hgram;
     int rc = bbjd_k_gram(c->s, n, (int)keep, ws->W, c->ld, (int)keep, ws->W, c->ld, d->gram, keep, &d->rd);
     rc |= !HIPOK(hipMemcpyAsync(G, d->gram, (size_t)keep * keep * 16, hipMemcpyDeviceToHost, c->s));
-    rc |= !HIPOK(hipStreamSynchronize(c->s));
+    rc |= !HIPOK(dsync(c->d, c->s));
     if (rc) return -1;
     if (check)
         for (bbjd_int j = 0; j < keep; j++)
@@ -1198,7 +1221,7 @@ static int thick_restart(ctx_t *c, bbjd_int k, bbjd_int keep, zcx nu0, zcx mu0)
     rc |= bbjd_k_gram(c->s, n, (int)keep, ws->W, c->ld, (int)keep, ws->AV, c->ld, G, keep, &d->rd);
     rc |= bbjd_k_gram(c->s, n, (int)keep, ws->W, c->ld, (int)keep, ws->BV, c->ld, G + keep * keep, keep, &d->rd);
     rc |= !HIPOK(hipMemcpyAsync(hg, G, (size_t)2 * keep * keep * 16, hipMemcpyDeviceToHost, c->s));
-    rc |= !HIPOK(hipStreamSynchronize(c->s));
+    rc |= !HIPOK(dsync(c->d, c->s));
     zcx *MA = (zcx *)ws->MA, *MB = (zcx *)ws->MB;
     for (bbjd_int j = 0; j < keep; j++)
         for (bbjd_int i = 0; i < keep; i++) {
@@ -1229,7 +1252,7 @@ static int random_vector(ctx_t *c, void *v)
     }
     int rc = !HIPOK(hipMemcpyAsync(v, tmpv, (size_t)n * 16, hipMemcpyHostToDevice, c->s));
     if (!d->hx) {
-        rc |= !HIPOK(hipStreamSynchronize(c->s));
+        rc |= !HIPOK(dsync(c->d, c->s));
         free(tmpv);
     }
     return rc ? -2 : 0;
@@ -1364,7 +1387,7 @@ static int qz_device(ctx_t *c, bbjd_int k, const zcx *MA, const zcx *MB, const z
     if (!rc) rc |= bbjd_qz_device_batch((const bbjd_qz_problem *)(db + o_prob), 1, c->s) ? 1 : 0;
     rc |= !HIPOK(hipMemcpyAsync(hinfo, db + o_info, 4 * sizeof(int), hipMemcpyDeviceToHost, c->s));
     rc |= !HIPOK(hipMemcpyAsync(hal, db + o_al, 3 * (size_t)msw * 16, hipMemcpyDeviceToHost, c->s));
-    rc |= !HIPOK(hipStreamSynchronize(c->s));
+    rc |= !HIPOK(dsync(c->d, c->s));
     if (rc) return -1;
     d->st.qz_device_steps++;
     if (hinfo[0] != BBJD_QZD_OK || hinfo[1] < 0 || hinfo[1] >= k) {
@@ -1567,7 +1590,7 @@ static bbjd_status solve_impl(const bbjd_params *P, bbjd_workspace *ws, bbjd_res
             eval[R->nconv] = lam;
             rc |= !HIPOK(hipMemcpyAsync((zcx *)R->eigenvectors + (size_t)R->nconv * n, u, (size_t)n * 16,
                                         hipMemcpyDeviceToHost, c->s));
-            rc |= !HIPOK(hipStreamSynchronize(c->s));
+            rc |= !HIPOK(dsync(c->d, c->s));
             R->residuals[R->nconv] = rel;
             /* ||A u - lambda B u|| with ||u|| = 1: the FP64 residual of the
              * returned pair from the operator applies, r / theta_b */
@@ -1641,7 +1664,7 @@ done:
     {
         int flags[FL_COUNT];
         if (HIPOK(hipMemcpyAsync(flags, d->rd.flag, sizeof(flags), hipMemcpyDeviceToHost, c->s)) &&
-            HIPOK(hipStreamSynchronize(c->s)))
+            HIPOK(dsync(c->d, c->s)))
             d->st.dgks_second_passes = flags[FL_DGKS_COUNT];
         /* second CGS passes at the mean subspace size: one more read of the
          * basis in the fused form (k <= 128), two otherwise */
@@ -1654,7 +1677,7 @@ done:
     R->status = st;
     return st;
 cb_err:
-    hipStreamSynchronize(c->s);
+    dsync(c->d, c->s);
     d->st.solve_seconds = now_s() - t_start;
     R->status = BBJD_ERR_CALLBACK;
     return BBJD_ERR_CALLBACK;

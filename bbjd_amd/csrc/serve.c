@@ -243,9 +243,10 @@ void bbjd_server_destroy(bbjd_server *S)
 bbjd_server *bbjd_server_create(const bbjd_request *reqs, int nreq, const bbjd_serve_config *cfg)
 {
     if (!reqs || nreq <= 0 || !cfg) return NULL;
-    /* ignored once the device is up */
-    if (cfg->sync_yield == 2) hipSetDeviceFlags(hipDeviceScheduleBlockingSync);
-    else if (cfg->sync_yield) hipSetDeviceFlags(hipDeviceScheduleYield);
+    /* the process's wait (ignored once the device is up); blocking waits are
+     * the workspaces' own (bbjd_gpu_options.blocking_sync), so other solves
+     * of the process keep theirs */
+    if (cfg->sync_yield) hipSetDeviceFlags(hipDeviceScheduleYield);
     bbjd_server *S = calloc(1, sizeof(*S));
     if (!S) return NULL;
     pthread_mutex_init(&S->mu, NULL);
@@ -274,6 +275,7 @@ bbjd_server *bbjd_server_create(const bbjd_request *reqs, int nreq, const bbjd_s
     S->opt.graphs = cfg->graphs;
     S->opt.qz_mode = cfg->qz_mode;
     S->opt.inner_form = cfg->inner_form;
+    S->opt.blocking_sync = cfg->sync_yield == 2;
     if (cfg->precision == BBJD_PREC_MIXED) {
         S->opt.matvec_A_c64 = bbjd_op_matvec_A_c64;
         S->opt.matvec_B_c64 = bbjd_op_matvec_B_c64;

@@ -308,8 +308,9 @@ def main():
                     help="which arrived request a free worker takes (serving loop)")
     ap.add_argument("--window", type=int, default=8,
                     help="--order window: the largest of this many longest-waiting requests")
-    ap.add_argument("--sync", choices=["yield", "block"], default="yield",
-                    help="how host threads wait on the GPU")
+    ap.add_argument("--sync", choices=["yield", "block"], default="block",
+                    help="how the served leg's workers wait on the GPU (block: each workspace's waits sleep "
+                         "until the GPU signals; the plain leg always yields)")
     ap.add_argument("--pin", action="store_true", help="pin each serving worker to one CPU")
     ap.add_argument("--gmres-lookahead", type=int, default=4,
                     help="Arnoldi steps of the inner GMRES in flight (1: one at a time)")
@@ -350,7 +351,7 @@ def main():
     from bbjd_amd import BandedTauPencil, DeviceOperator, _lib
     from bbjd_amd import serve as srv
 
-    srv.device_init(sync_yield="block" if args.sync == "block" else True)
+    srv.device_init(sync_yield=True)
     # the committed per-shape tile table of this device, when there is one
     from bbjd_amd import autotune
     tt = None if args.no_tile_table else autotune.apply()

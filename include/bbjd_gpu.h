@@ -104,6 +104,11 @@ typedef struct {
      * unchanged.  Falls back to the reference form for a call whose H is
      * numerically singular. */
     int                 inner_form;
+    /* 1: the engine's host waits on its stream sleep until the GPU signals
+     * (events created with hipEventBlockingSync) instead of spinning, so a
+     * thread waiting on the GPU leaves its core to others (many workspaces
+     * solving at once on a busy host); 0: the process's default wait. */
+    int                 blocking_sync;
 } bbjd_gpu_options;
 
 enum { BBJD_INNER_REFERENCE = 0, BBJD_INNER_PROJECTED = 1 };

@@ -60,7 +60,9 @@ typedef struct {
     int    precision;     /* bbjd_precision of every request                 */
     int    realtime;      /* 1: admit a request no earlier than its arrival  */
     int    sync_yield;    /* 1: host threads yield while waiting on the GPU,
-                             2: they block (sleep until an interrupt)        */
+                             2: the workers block (sleep until the GPU
+                             signals: bbjd_gpu_options.blocking_sync of
+                             every workspace)                                 */
     int    graphs;        /* bbjd_gpu_options.graphs of every request        */
     int    order;         /* which arrived request a free worker takes:
                              BBJD_ORDER_ARRIVAL (first come, first served),

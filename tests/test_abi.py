@@ -99,7 +99,7 @@ int main(void) {
   O(bbjd_gpu_options, matvec_A_c64); O(bbjd_gpu_options, matvec_B_c64); O(bbjd_gpu_options, precond_c64);
   O(bbjd_gpu_options, precond2); O(bbjd_gpu_options, qz_mode); O(bbjd_gpu_options, graphs);
   O(bbjd_gpu_options, matvec_AB); O(bbjd_gpu_options, matvec_AB_c64); O(bbjd_gpu_options, matvec_ABr);
-  O(bbjd_gpu_options, inner_form);
+  O(bbjd_gpu_options, inner_form); O(bbjd_gpu_options, blocking_sync);
   O(bbjd_stats, outer_iters); O(bbjd_stats, lock_iters); O(bbjd_stats, restarts); O(bbjd_stats, inner_steps);
   O(bbjd_stats, dgks_second_passes); O(bbjd_stats, sum_k); O(bbjd_stats, qz_seconds);
   O(bbjd_stats, solve_seconds); O(bbjd_stats, stream_bytes); O(bbjd_stats, restart_fallbacks);

@@ -826,6 +826,23 @@ def test_serving_loop_arrival_waves():
             o.free()
 
 
+def test_serving_loop_blocking_waits_same_results():
+    """workers whose waits sleep until the GPU signals (sync_yield "block":
+    bbjd_gpu_options.blocking_sync, events created with
+    hipEventBlockingSync) give the same eigenvalues as yielding ones, FP64
+    and complex64 inner solves"""
+    from bbjd_amd import serve as srv
+    jobs = _serve_jobs(seed=17)[:5]
+    for prec, inner in (("fp64", 0), ("mixed", 5)):
+        js = [dict(j, inner_maxiter=inner) for j in jobs]
+        a, _ = srv.serve(js, workers=3, realtime=False, precision=prec)
+        b, rep = srv.serve(js, workers=3, realtime=False, precision=prec, sync_yield="block")
+        assert rep["failed"] == 0
+        for i in range(len(js)):
+            np.testing.assert_array_equal(a[i]["eigenvalues"], b[i]["eigenvalues"], err_msg=f"{prec} {i}")
+            assert a[i]["iters"] == b[i]["iters"]
+
+
 def test_serving_loop_mixed_precision_alone_equal_mix():
     """the small-solve form of the serving loop: GMRES(20) in complex64 with
     the exact LU, FP32 accumulation, four workers; each request's eigenvalues
