@@ -47,7 +47,8 @@ class GpuOptions(C.Structure):
                 ("matvec_A_c64", C.c_void_p), ("matvec_B_c64", C.c_void_p),
                 ("precond_c64", C.c_void_p), ("precond2", C.c_void_p), ("qz_mode", C.c_int),
                 ("graphs", C.c_int), ("matvec_AB", C.c_void_p), ("matvec_AB_c64", C.c_void_p),
-                ("matvec_ABr", C.c_void_p), ("inner_form", C.c_int), ("blocking_sync", C.c_int)]
+                ("matvec_ABr", C.c_void_p), ("inner_form", C.c_int), ("blocking_sync", C.c_int),
+                ("inner_batcher", C.c_void_p)]
 
 
 class Stats(C.Structure):

@@ -184,6 +184,26 @@ void bbjd_capture_unlock(void);
 int  bbjd_devwide_lock(void);
 void bbjd_devwide_unlock(int held);
 
+/* Launch recording for grouped execution of independent solves (launch.h,
+ * kernels.hip).  Between bbjd_rec_begin and bbjd_rec_end, launches made by
+ * the calling thread through the engine's launchers (and the copies below)
+ * are appended to the job's list instead of being launched; bbjd_rec_flush
+ * launches every job's list on s, merging launches of the same kernel from
+ * different jobs into one grouped launch (each job's blocks see the grid
+ * and arguments of its own launch).  *launches: launches made. */
+void *bbjd_rec_create(void);
+void  bbjd_rec_destroy(void *rec);
+void  bbjd_rec_begin(void *rec, int job);
+void  bbjd_rec_end(void);
+int   bbjd_rec_flush(void *rec, hipStream_t s, long *launches);
+/* launches recorded and launches made by flushes, since creation */
+void  bbjd_rec_counts(const void *rec, long *recorded, long *launches);
+/* hipMemcpyAsync / hipMemsetAsync, recorded in order while recording */
+int   bbjd_copy_async(void *dst, const void *src, size_t bytes, int kind, hipStream_t s);
+int   bbjd_memset_async(void *dst, int value, size_t bytes, hipStream_t s);
+/* device copy of bytes (a multiple of 8) by a kernel (groupable) */
+int   bbjd_k_copy(hipStream_t s, void *dst, const void *src, size_t bytes);
+
 /* bytes of reduction scratch for the given shape */
 int64_t bbjd_red_part_entries(int64_t n, int maxk);
 

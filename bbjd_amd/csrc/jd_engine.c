@@ -15,6 +15,7 @@
  */
 #include "../../include/bbjd.h"
 #include "../../include/bbjd_gpu.h"
+#include "../../include/bbjd_ops.h"
 #include "engine.h"
 #include "lapack_host.h"
 #include "qz_host.h"
@@ -514,6 +515,10 @@ typedef struct {
     int               graphs;    /* capture/replay the inner-solve segments  */
     int               pp;        /* this inner solve uses the projected
                                     preconditioner (BBJD_INNER_PROJECTED)   */
+    char             *stage;     /* batched inner solve: the device slot that
+                                    receives the Hessenberg columns and the
+                                    scalar slots (NULL: copied to the host)  */
+    size_t            stage_sc;  /* offset of the scalar slots in it         */
 } ctx_t;
 
 #define COLZ(M, j) ((void *)((zcx *)(M) + (size_t)(j) * c->ld))
@@ -795,6 +800,27 @@ static int pp_prepare(ctx_t *c, bbjd_int K)
     return sing ? 1 : 0;
 }
 
+/* Copies of an Arnoldi step's results (and the prologue's scalars): to the
+ * host (hh, hs), or in a batched inner solve to the staging slot, where the
+ * column ends right before the scalar slots (at stage_sc).  hs non-NULL: the
+ * column is not contiguous with the slots on the device and goes first. */
+static int copy_column(ctx_t *c, void *hh, const void *hsum, size_t colb, const double *hs)
+{
+    if (!hs) return 0;
+    if (c->stage) return bbjd_k_copy(c->s, c->stage + c->stage_sc - colb, hsum, colb) != 0;
+    return !HIPOK(hipMemcpyAsync(hh, hsum, colb, hipMemcpyDeviceToHost, c->s));
+}
+static int copy_scalars(ctx_t *c, void *hh, const void *hsum, size_t colb, double *hs)
+{
+    struct bbjd_dev *d = c->d;
+    if (c->stage) {
+        if (hs) return bbjd_k_copy(c->s, c->stage + c->stage_sc, d->rd.scal, SL_COUNT * 8) != 0;
+        return bbjd_k_copy(c->s, c->stage + c->stage_sc - colb, hsum, colb + SL_COUNT * 8) != 0;
+    }
+    if (hs) return !HIPOK(hipMemcpyAsync(hs, d->rd.scal, SL_COUNT * 8, hipMemcpyDeviceToHost, c->s));
+    return !HIPOK(hipMemcpyAsync(hh, hsum, colb + SL_COUNT * 8, hipMemcpyDeviceToHost, c->s));
+}
+
 /* Enqueues Arnoldi step j of the inner solve: v_{j+1} = M^-1 (I - QL QL^H)
  * (theta_b A - theta_a B)(I - QR QR^H) v_j, its CGS2 against v_0 .. v_j, the
  * new vector normalised (also into the next step's input copy), and copies
@@ -826,7 +852,7 @@ static int arnoldi_enqueue(ctx_t *c, bbjd_int j, bbjd_int ndefl, const void *QR,
         rc |= bbjd_k_project_out(c->s, n, (int)ndefl, QL, c->ld, t2, &d->rd);
         if (!rc && !cb && apply_pc(c, t2, vj1)) cb = 1;
         rc |= bbjd_k_cgs2(c->s, n, (int)(j + 1), V, c->ld, vj1, &d->rd, SL_G, hsum);
-        if (hs) rc |= !HIPOK(hipMemcpyAsync(hh, hsum, colb, hipMemcpyDeviceToHost, c->s));
+        rc |= copy_column(c, hh, hsum, colb, hs);
         rc |= bbjd_k_scale_inv2(c->s, n, vj1, t1, d->rd.scal, SL_G, 0);   /* t1: the next step's */
     } else if (c->pp) {
         /* projected preconditioner: v_j is in Q_right-perp already, so the
@@ -849,7 +875,7 @@ static int arnoldi_enqueue(ctx_t *c, bbjd_int j, bbjd_int ndefl, const void *QR,
         rc |= bbjd_k_gemv_c_c(c->s, n, (int)ndefl, qr, c->ld, vj1, NULL, d->rd.coef, NULL, &d->rd);
         rc |= bbjd_k_update_pre_c(c->s, n, (int)ndefl, d->yc, c->ld, d->hinv, d->rd.coef, vj1);
         rc |= bbjd_k_cgs2_c(c->s, n, (int)(j + 1), V, c->ld, vj1, &d->rd, SL_G, hsum);
-        if (hs) rc |= !HIPOK(hipMemcpyAsync(hh, hsum, colb, hipMemcpyDeviceToHost, c->s));
+        rc |= copy_column(c, hh, hsum, colb, hs);
         rc |= bbjd_k_scale_inv_c(c->s, n, vj1, d->rd.scal, SL_G, 0);
     } else {
         void *a = d->tc[0], *bq = d->tc[1], *e = d->tc[2];
@@ -872,16 +898,183 @@ static int arnoldi_enqueue(ctx_t *c, bbjd_int j, bbjd_int ndefl, const void *QR,
             if (!rc && !cb && apply_c64(c, d->opt.precond_c64, (bbjd_matvec_fn)P->precond, bq, vj1, t1, t2))
                 cb = 1;
         } else
-            rc |= !HIPOK(hipMemcpyAsync(vj1, bq, (size_t)n * 8, hipMemcpyDeviceToDevice, c->s));
+            rc |= bbjd_copy_async(vj1, bq, (size_t)n * 8, hipMemcpyDeviceToDevice, c->s) != 0;
         rc |= bbjd_k_cgs2_c(c->s, n, (int)(j + 1), V, c->ld, vj1, &d->rd, SL_G, hsum);
-        if (hs) rc |= !HIPOK(hipMemcpyAsync(hh, hsum, colb, hipMemcpyDeviceToHost, c->s));
+        rc |= copy_column(c, hh, hsum, colb, hs);
         rc |= bbjd_k_scale_inv2_c(c->s, n, vj1, a, d->rd.scal, SL_G, 0);   /* a: the next step's */
     }
     /* both norms down (the Hessenberg column came above, or comes with them) */
-    if (hs) rc |= !HIPOK(hipMemcpyAsync(hs, d->rd.scal, SL_COUNT * 8, hipMemcpyDeviceToHost, c->s));
-    else rc |= !HIPOK(hipMemcpyAsync(hh, hsum, colb + SL_COUNT * 8, hipMemcpyDeviceToHost, c->s));
+    rc |= copy_scalars(c, hh, hsum, colb, hs);
     return cb ? -1 : rc ? -2 : 0;
 }
+
+/* Host side of one inner GMRES call: Hessenberg matrix, Givens rotations
+ * and the rotated right-hand side (workspace arrays). */
+typedef struct {
+    zcx    *H, *sn, *g;
+    double *cs;
+    bbjd_int ldh, ims;
+    double  tol, beta0;
+} gm_t;
+
+static void gm_begin(ctx_t *c, gm_t *G, bbjd_int ims, double tol)
+{
+    bbjd_workspace *ws = c->ws;
+    G->H = (zcx *)ws->gmres_H; G->sn = (zcx *)ws->gmres_sn; G->g = (zcx *)ws->gmres_g;
+    G->cs = (double *)ws->gmres_cs;
+    G->ldh = ws->inner_maxiter + 1;
+    G->ims = ims;
+    G->tol = tol;
+    G->beta0 = 0;
+    memset(G->H, 0, (size_t)G->ldh * ws->inner_maxiter * sizeof(zcx));
+}
+
+static void gm_start(gm_t *G, double beta0)
+{
+    memset(G->g, 0, (size_t)(G->ims + 1) * sizeof(zcx));
+    G->g[0] = beta0;
+    G->beta0 = beta0;
+}
+
+/* Device work of the GMRES prologue: v_0 = M^-1 (I - QL QL^H)(-r) (FP64 and
+ * the complex64 reference form) or (I - Yh H^-1 Q^H) M^-1 (-r) (projected
+ * form), normalised, ||v_0|| in scal[SL_G]; the scalar slots then go to the
+ * host (hscal) or the staging slot.  *cb: a callback failed. */
+static int gm_prologue(ctx_t *c, bbjd_int ndefl, const void *QL, const void *r, void *V, void *t1, void *t2, int *cb)
+{
+    struct bbjd_dev *d = c->d;
+    const bbjd_params *P = c->P;
+    const bbjd_int n = c->n;
+    const int mixed = d->opt.precision == BBJD_PREC_MIXED;
+    const size_t esz = mixed ? 8 : 16;
+    void *v0 = V;
+    const void *ql = mixed ? d->qlc : QL;
+    int rc = 0;
+    (void)esz;
+    if (!mixed) {
+        rc |= bbjd_k_lincomb(c->s, n, -1.0, 0.0, r, 0.0, 0.0, r, t1);
+        rc |= bbjd_k_project_out(c->s, n, (int)ndefl, QL, c->ld, t1, &d->rd);
+        if (!rc && apply_pc(c, t1, v0)) *cb = 1;
+        rc |= bbjd_k_norm(c->s, n, v0, &d->rd, SL_G);
+        /* also into t1 (free again), where step 0 projects it */
+        rc |= bbjd_k_scale_inv2(c->s, n, v0, t1, d->rd.scal, SL_G, 0);
+    } else if (c->pp) {
+        /* v0 = (I - Yh H^-1 Q^H) M^-1 (-r), normalised */
+        void *tc0 = d->tc[0];
+        rc |= bbjd_k_z2c(c->s, n, r, tc0);
+        rc |= bbjd_k_scale_c(c->s, n, tc0, -1.0, 0.0);
+        if (!rc && apply_c64(c, d->opt.precond_c64, (bbjd_matvec_fn)P->precond, tc0, v0, t1, t2)) *cb = 1;
+        rc |= bbjd_k_gemv_c_c(c->s, n, (int)ndefl, d->qrc, c->ld, v0, NULL, d->rd.coef, NULL, &d->rd);
+        rc |= bbjd_k_update_pre_c(c->s, n, (int)ndefl, d->yc, c->ld, d->hinv, d->rd.coef, v0);
+        rc |= bbjd_k_norm_c(c->s, n, v0, &d->rd, SL_G);
+        rc |= bbjd_k_scale_inv_c(c->s, n, v0, d->rd.scal, SL_G, 0);
+    } else {
+        void *tc0 = d->tc[0];
+        rc |= bbjd_k_z2c(c->s, n, r, tc0);
+        rc |= bbjd_k_scale_c(c->s, n, tc0, -1.0, 0.0);
+        rc |= bbjd_k_project_out_c(c->s, n, (int)ndefl, ql, c->ld, tc0, &d->rd);
+        if (P->precond) {
+            if (!rc && apply_c64(c, d->opt.precond_c64, (bbjd_matvec_fn)P->precond, tc0, v0, t1, t2)) *cb = 1;
+        } else
+            rc |= bbjd_copy_async(v0, tc0, (size_t)n * 8, hipMemcpyDeviceToDevice, c->s) != 0;
+        rc |= bbjd_k_norm_c(c->s, n, v0, &d->rd, SL_G);
+        /* also into tc0 (free again), where step 0 projects it */
+        rc |= bbjd_k_scale_inv2_c(c->s, n, v0, tc0, d->rd.scal, SL_G, 0);
+    }
+    if (c->stage) rc |= bbjd_k_copy(c->s, c->stage + c->stage_sc, d->rd.scal, SL_COUNT * 8) != 0;
+    else rc |= !HIPOK(hipMemcpyAsync(d->hscal, d->rd.scal, SL_COUNT * 8, hipMemcpyDeviceToHost, c->s));
+    return rc;
+}
+
+/* Column j of the Arnoldi process arrived (hh: the Hessenberg column in the
+ * basis precision, hs: the scalar slots): the reference's breakdown test,
+ * Givens update and stopping rule (src/jd_core.c:674-706).  Returns 1 when
+ * the solve stops after this column. */
+static int gm_column(ctx_t *c, gm_t *G, bbjd_int j, bbjd_int ndefl, const void *hh, const double *hs)
+{
+    struct bbjd_dev *d = c->d;
+    const int mixed = d->opt.precision == BBJD_PREC_MIXED;
+    const size_t esz = mixed ? 8 : 16;
+    const bbjd_int ldh = G->ldh;
+    zcx *H = G->H, *sn = G->sn, *g = G->g;
+    double *cs = G->cs;
+    d->st.inner_steps++;
+    d->st.stream_bytes += (double)esz * c->n * ((c->pp ? 3.0 : 8.0) * ndefl + 4.0 * (j + 1) + 8);
+    if (!mixed)
+        for (bbjd_int i = 0; i <= j; i++) H[i + j * ldh] = ((const zcx *)hh)[i];
+    else {
+        const c64 *hc = (const c64 *)hh;
+        for (bbjd_int i = 0; i <= j; i++) H[i + j * ldh] = hc[i].re + hc[i].im * I;
+    }
+    const double wb = hs[SL_NB], wa = hs[SL_G];
+    H[(j + 1) + j * ldh] = wa;
+    if (wa < 1e-14 * wb) return 1;                           /* :674 */
+    for (bbjd_int i = 0; i < j; i++) {                       /* :680-685 */
+        zcx h1 = H[i + j * ldh], h2 = H[i + 1 + j * ldh];
+        H[i + j * ldh] = cs[i] * h1 + sn[i] * h2;
+        H[i + 1 + j * ldh] = -conj(sn[i]) * h1 + cs[i] * h2;
+    }
+    double cj; zcx sj, rj;
+    givens(H[j + j * ldh], H[j + 1 + j * ldh], &cj, &sj, &rj);
+    cs[j] = cj; sn[j] = sj;
+    H[j + j * ldh] = rj;
+    H[j + 1 + j * ldh] = 0.0;
+    zcx g1 = g[j], g2 = g[j + 1];
+    g[j] = cs[j] * g1 + sn[j] * g2;
+    g[j + 1] = -conj(sn[j]) * g1 + cs[j] * g2;
+    return cabs(g[j + 1]) < G->tol * G->beta0;
+}
+
+/* z = V y from the first m columns (back substitution on the host,
+ * reference src/jd_core.c:709-716), projected with I - QR QR^H in FP64
+ * (:724).  Only device work, through recordable launches and copies. */
+static int gm_finish(ctx_t *c, gm_t *G, bbjd_int m, bbjd_int ndefl, const void *QR, const void *V, void *z)
+{
+    struct bbjd_dev *d = c->d;
+    const bbjd_int n = c->n, ldh = G->ldh;
+    const int mixed = d->opt.precision == BBJD_PREC_MIXED;
+    zcx *H = G->H, *y = G->g;
+    int rc = 0;
+    for (bbjd_int i = m - 1; i >= 0; i--) {
+        for (bbjd_int k2 = i + 1; k2 < m; k2++) y[i] -= H[i + k2 * ldh] * y[k2];
+        if (cabs(H[i + i * ldh]) > 1e-300) y[i] /= H[i + i * ldh];
+    }
+    if (m == 0) {
+        rc |= bbjd_memset_async(z, 0, (size_t)n * 16, c->s) != 0;
+    } else if (!mixed) {
+        memcpy(d->hsmall, y, (size_t)m * sizeof(zcx));
+        rc |= bbjd_copy_async(d->rd.coef, d->hsmall, (size_t)m * sizeof(zcx), hipMemcpyHostToDevice, c->s) != 0;
+        rc |= bbjd_k_gemv_n(c->s, n, (int)m, V, c->ld, d->rd.coef, z, &d->rd, SL_T1);
+    } else {
+        c64 *yc = (c64 *)d->hsmall;
+        for (bbjd_int i = 0; i < m; i++) { yc[i].re = (float)creal(y[i]); yc[i].im = (float)cimag(y[i]); }
+        rc |= bbjd_copy_async(d->rd.coef, yc, (size_t)m * 8, hipMemcpyHostToDevice, c->s) != 0;
+        rc |= bbjd_k_gemv_n_c(c->s, n, (int)m, V, c->ld, d->rd.coef, d->tc[0], &d->rd, SL_T1);
+        rc |= bbjd_k_c2z(c->s, n, d->tc[0], z);
+    }
+    rc |= bbjd_k_project_out(c->s, n, (int)ndefl, QR, c->ld, z, &d->rd);  /* :724, FP64 */
+    return rc;
+}
+
+/* One inner solve submitted to a batcher (the workspace's thread waits). */
+struct gm_job {
+    ctx_t          *c;
+    bbjd_int        ndefl, ims;
+    const void     *QR, *QL, *r;
+    void           *z;
+    double          tol;
+    double          t_queued;
+    int             rc, done;
+    struct gm_job  *next;
+    /* batch state */
+    gm_t            G;
+    bbjd_int        m;
+    int             active, cb;
+    hipStream_t     own;
+};
+
+static int batcher_run(bbjd_batcher *B, struct gm_job *job);
+static int batch_applies(const ctx_t *c);
 
 /* Left-preconditioned GMRES on the projected correction equation
  * (reference src/jd_core.c:568-727).  r, z: device complex128; QR, QL: the
@@ -896,18 +1089,15 @@ static int arnoldi_enqueue(ctx_t *c, bbjd_int j, bbjd_int ndefl, const void *QR,
  * theta_b and -theta_a are read from device memory (gco) and the new basis
  * vector is scaled by the reciprocal of its device-side norm, so with
  * graphs a segment is captured once per deflation count and replayed on
- * later outer iterations. */
+ * later outer iterations.  With a batcher the call is handed to it and runs
+ * with other workspaces' calls (batcher_run). */
 static int gmres(ctx_t *c, bbjd_int ndefl, const void *QR, const void *QL, zcx ta, zcx tb, const void *r, void *z,
                  bbjd_int ims, double tol, int allow_pp)
 {
     bbjd_workspace *ws = c->ws;
     struct bbjd_dev *d = c->d;
-    const bbjd_params *P = c->P;
-    const bbjd_int n = c->n, ldh = ws->inner_maxiter + 1;
+    const bbjd_int n = c->n;
     const int mixed = d->opt.precision == BBJD_PREC_MIXED;
-    zcx *H = (zcx *)ws->gmres_H, *sn = (zcx *)ws->gmres_sn, *g = (zcx *)ws->gmres_g;
-    double *cs = (double *)ws->gmres_cs;
-    memset(H, 0, (size_t)ldh * ws->inner_maxiter * sizeof(zcx));
     int rc = 0, seg;
     /* FP64 temporaries: t1 = tmp3, t2 = tmp4 as in the reference (tmp4 holds
      * r, read only by the first step) */
@@ -915,11 +1105,10 @@ static int gmres(ctx_t *c, bbjd_int ndefl, const void *QR, const void *QL, zcx t
     void *V = mixed ? d->gvc : ws->gmres_V;
     const size_t esz = mixed ? 8 : 16;
 #define GCOL(j) ((void *)((char *)V + (size_t)(j) * c->ld * esz))
-    const void *ql = mixed ? d->qlc : QL;
 
     c->graphs = d->opt.graphs && d->opt.callback_space == BBJD_CALLBACK_DEVICE;
     c->pp = 0;
-    if (allow_pp && mixed && d->yc && P->precond && ndefl <= d->pp_ld) {
+    if (allow_pp && mixed && d->yc && c->P->precond && ndefl <= d->pp_ld) {
         const int pr = pp_prepare(c, ndefl);
         if (pr == -1) return -1;
         if (pr < 0) return -2;
@@ -931,7 +1120,7 @@ static int gmres(ctx_t *c, bbjd_int ndefl, const void *QR, const void *QL, zcx t
         struct gkey key;
         memset(&key, 0, sizeof(key));
         key.ndefl = ndefl; key.n = n; key.ld = c->ld; key.r = r; key.z = z; key.qr = QR; key.ql = QL;
-        key.op = P->op_data;
+        key.op = c->P->op_data;
         key.pp = c->pp;
         if (memcmp(&key, &d->gkey, sizeof(key))) {
             rc |= !HIPOK(dsync(c->d, c->s));
@@ -943,41 +1132,22 @@ static int gmres(ctx_t *c, bbjd_int ndefl, const void *QR, const void *QL, zcx t
     rc |= !HIPOK(hipMemcpyAsync(d->gco, d->hgco, 4 * sizeof(double), hipMemcpyHostToDevice, c->s));
     if (rc) return -2;
 
+    if (batch_applies(c)) {
+        /* the batch runs on the batcher's stream: everything enqueued here
+         * before (the projectors, r, theta) has finished first */
+        if (!HIPOK(dsync(c->d, c->s))) return -2;
+        struct gm_job job;
+        memset(&job, 0, sizeof(job));
+        job.c = c; job.ndefl = ndefl; job.ims = ims; job.QR = QR; job.QL = QL; job.r = r; job.z = z; job.tol = tol;
+        return batcher_run((bbjd_batcher *)d->opt.inner_batcher, &job);
+    }
+
+    gm_t G;
+    gm_begin(c, &G, ims, tol);
     int cb = 0;              /* a callback failed */
     if ((seg = seg_begin(c, 0)) < 0) return -2;
     if (seg == 0) {
-        if (!mixed) {
-            rc |= bbjd_k_lincomb(c->s, n, -1.0, 0.0, r, 0.0, 0.0, r, t1);
-            rc |= bbjd_k_project_out(c->s, n, (int)ndefl, QL, c->ld, t1, &d->rd);
-            if (!rc && apply_pc(c, t1, GCOL(0))) cb = 1;
-            rc |= bbjd_k_norm(c->s, n, GCOL(0), &d->rd, SL_G);
-            /* also into t1 (free again), where step 0 projects it */
-            rc |= bbjd_k_scale_inv2(c->s, n, GCOL(0), t1, d->rd.scal, SL_G, 0);
-        } else if (c->pp) {
-            /* v0 = (I - Yh H^-1 Q^H) M^-1 (-r), normalised */
-            void *tc0 = d->tc[0];
-            rc |= bbjd_k_z2c(c->s, n, r, tc0);
-            rc |= bbjd_k_scale_c(c->s, n, tc0, -1.0, 0.0);
-            if (!rc && apply_c64(c, d->opt.precond_c64, (bbjd_matvec_fn)P->precond, tc0, GCOL(0), t1, t2)) cb = 1;
-            rc |= bbjd_k_gemv_c_c(c->s, n, (int)ndefl, d->qrc, c->ld, GCOL(0), NULL, d->rd.coef, NULL, &d->rd);
-            rc |= bbjd_k_update_pre_c(c->s, n, (int)ndefl, d->yc, c->ld, d->hinv, d->rd.coef, GCOL(0));
-            rc |= bbjd_k_norm_c(c->s, n, GCOL(0), &d->rd, SL_G);
-            rc |= bbjd_k_scale_inv_c(c->s, n, GCOL(0), d->rd.scal, SL_G, 0);
-        } else {
-            void *tc0 = d->tc[0];
-            rc |= bbjd_k_z2c(c->s, n, r, tc0);
-            rc |= bbjd_k_scale_c(c->s, n, tc0, -1.0, 0.0);
-            rc |= bbjd_k_project_out_c(c->s, n, (int)ndefl, ql, c->ld, tc0, &d->rd);
-            if (P->precond) {
-                if (!rc && apply_c64(c, d->opt.precond_c64, (bbjd_matvec_fn)P->precond, tc0, GCOL(0), t1, t2))
-                    cb = 1;
-            } else
-                rc |= !HIPOK(hipMemcpyAsync(GCOL(0), tc0, (size_t)n * 8, hipMemcpyDeviceToDevice, c->s));
-            rc |= bbjd_k_norm_c(c->s, n, GCOL(0), &d->rd, SL_G);
-            /* also into tc0 (free again), where step 0 projects it */
-            rc |= bbjd_k_scale_inv2_c(c->s, n, GCOL(0), tc0, d->rd.scal, SL_G, 0);
-        }
-        rc |= !HIPOK(hipMemcpyAsync(d->hscal, d->rd.scal, SL_COUNT * 8, hipMemcpyDeviceToHost, c->s));
+        rc |= gm_prologue(c, ndefl, QL, r, GCOL(0), t1, t2, &cb);
         rc = seg_end(c, 0, rc | cb);
     }
     if (cb) return -1;
@@ -986,8 +1156,7 @@ static int gmres(ctx_t *c, bbjd_int ndefl, const void *QR, const void *QL, zcx t
     if (beta0 < 1e-300) {
         return HIPOK(hipMemsetAsync(z, 0, (size_t)n * 16, c->s)) ? 0 : -2;
     }
-    memset(g, 0, (size_t)(ims + 1) * sizeof(zcx));
-    g[0] = beta0;
+    gm_start(&G, beta0);
 
     /* Lookahead (device callbacks, no graphs): a step needs nothing from the
      * host, so up to `look` steps are enqueued before the first of them is
@@ -1026,54 +1195,327 @@ static int gmres(ctx_t *c, bbjd_int ndefl, const void *QR, const void *QL, zcx t
             if (cb) return -1;
             if (rc || !HIPOK(dsync(c->d, c->s))) return -2;
         }
-        d->st.inner_steps++;
-        d->st.stream_bytes += (double)esz * n * ((c->pp ? 3.0 : 8.0) * ndefl + 4.0 * (j + 1) + 8);
-        if (!mixed)
-            for (bbjd_int i = 0; i <= j; i++) H[i + j * ldh] = ((const zcx *)hh)[i];
-        else {
-            const c64 *hc = (const c64 *)hh;
-            for (bbjd_int i = 0; i <= j; i++) H[i + j * ldh] = hc[i].re + hc[i].im * I;
-        }
-        const double wb = hs[SL_NB], wa = hs[SL_G];
-        H[(j + 1) + j * ldh] = wa;
-        if (wa < 1e-14 * wb) { j++; break; }                      /* :674 */
-        for (bbjd_int i = 0; i < j; i++) {                       /* :680-685 */
-            zcx h1 = H[i + j * ldh], h2 = H[i + 1 + j * ldh];
-            H[i + j * ldh] = cs[i] * h1 + sn[i] * h2;
-            H[i + 1 + j * ldh] = -conj(sn[i]) * h1 + cs[i] * h2;
-        }
-        double cj; zcx sj, rj;
-        givens(H[j + j * ldh], H[j + 1 + j * ldh], &cj, &sj, &rj);
-        cs[j] = cj; sn[j] = sj;
-        H[j + j * ldh] = rj;
-        H[j + 1 + j * ldh] = 0.0;
-        zcx g1 = g[j], g2 = g[j + 1];
-        g[j] = cs[j] * g1 + sn[j] * g2;
-        g[j + 1] = -conj(sn[j]) * g1 + cs[j] * g2;
-        if (cabs(g[j + 1]) < tol * beta0) { j++; break; }
+        if (gm_column(c, &G, j, ndefl, hh, hs)) { j++; break; }
     }
     const bbjd_int m = j;
     if (e > m) d->st.inner_steps_discarded += e - m;
-    zcx *y = g;                                                    /* :709-716 */
-    for (bbjd_int i = m - 1; i >= 0; i--) {
-        for (bbjd_int k2 = i + 1; k2 < m; k2++) y[i] -= H[i + k2 * ldh] * y[k2];
-        if (cabs(H[i + i * ldh]) > 1e-300) y[i] /= H[i + i * ldh];
-    }
-    if (m == 0) {
-        rc |= !HIPOK(hipMemsetAsync(z, 0, (size_t)n * 16, c->s));
-    } else if (!mixed) {
-        rc |= upload_small(c, d->rd.coef, y, m);
-        rc |= bbjd_k_gemv_n(c->s, n, (int)m, V, c->ld, d->rd.coef, z, &d->rd, SL_T1);
-    } else {
-        c64 *yc = (c64 *)d->hsmall;
-        for (bbjd_int i = 0; i < m; i++) { yc[i].re = (float)creal(y[i]); yc[i].im = (float)cimag(y[i]); }
-        rc |= !HIPOK(hipMemcpyAsync(d->rd.coef, yc, (size_t)m * 8, hipMemcpyHostToDevice, c->s));
-        rc |= bbjd_k_gemv_n_c(c->s, n, (int)m, V, c->ld, d->rd.coef, d->tc[0], &d->rd, SL_T1);
-        rc |= bbjd_k_c2z(c->s, n, d->tc[0], z);
-    }
-    rc |= bbjd_k_project_out(c->s, n, (int)ndefl, QR, c->ld, z, &d->rd);  /* :724, FP64 */
 #undef GCOL
-    return rc ? -2 : 0;
+    return gm_finish(c, &G, m, ndefl, QR, V, z) ? -2 : 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* inner-solve batcher                                                 */
+/* ------------------------------------------------------------------ */
+/* Threads with a HIP stream each run batches of inner solves handed over by
+ * workspaces solving at once.  A batch records every solve's device work of
+ * one phase (the prologue, Arnoldi step j, the final combination) through
+ * the recording launchers, then launches all of them on the thread's stream,
+ * kernels of the same instantiation from different solves merged into one
+ * launch (bbjd_rec_flush); each solve's kernels run with its own grid,
+ * arguments and order, so its results are those of running alone.  Each
+ * step's Hessenberg columns and scalar slots come back in one copy of the
+ * staging slots, and the host then runs each solve's Givens update and
+ * stopping rule. */
+#define BATCH_THREADS_MAX 8
+#define BATCH_JOBS_MAX 16
+
+struct bthread {
+    bbjd_batcher *B;
+    pthread_t     th;
+    hipStream_t   s;
+    hipEvent_t    ev;
+    void         *rec;
+    char         *dstage, *hstage;
+    int           started;
+};
+
+struct bbjd_batcher {
+    pthread_mutex_t    mu;
+    pthread_cond_t     cv_job, cv_done;
+    struct gm_job     *head, *tail;
+    int                queued;     /* jobs waiting                         */
+    int                running;    /* jobs in batches                      */
+    int                active;     /* solves of the batcher's workspaces   */
+    int                stop;
+    int                nth, max_jobs;
+    double             window;
+    bbjd_int           ims_max;
+    size_t             slot, stage_sc;
+    const void        *busy[BATCH_THREADS_MAX * BATCH_JOBS_MAX];   /* operators in running batches */
+    int                nbusy;
+    struct bthread     t[BATCH_THREADS_MAX];
+    bbjd_batcher_stats st;
+};
+
+static int batch_applies(const ctx_t *c)
+{
+    const struct bbjd_dev *d = c->d;
+    const bbjd_batcher *B = (const bbjd_batcher *)d->opt.inner_batcher;
+    return B && d->opt.precision == BBJD_PREC_MIXED && !c->pp && !c->graphs &&
+           d->opt.callback_space == BBJD_CALLBACK_DEVICE && d->opt.matvec_AB_c64 == bbjd_op_matvec_AB_c64 &&
+           d->opt.precond_c64 == bbjd_op_precond_c64 && c->P->op_data && c->ws->inner_maxiter <= B->ims_max;
+}
+
+static int batcher_run(bbjd_batcher *B, struct gm_job *job)
+{
+    pthread_mutex_lock(&B->mu);
+    job->t_queued = now_s();
+    job->next = NULL;
+    if (B->tail) B->tail->next = job;
+    else B->head = job;
+    B->tail = job;
+    B->queued++;
+    pthread_cond_broadcast(&B->cv_job);
+    while (!job->done) pthread_cond_wait(&B->cv_done, &B->mu);
+    pthread_mutex_unlock(&B->mu);
+    return job->rc;
+}
+
+static void batcher_enter(bbjd_batcher *B)
+{
+    pthread_mutex_lock(&B->mu);
+    B->active++;
+    pthread_mutex_unlock(&B->mu);
+}
+
+static void batcher_leave(bbjd_batcher *B)
+{
+    pthread_mutex_lock(&B->mu);
+    B->active--;
+    pthread_cond_broadcast(&B->cv_job);
+    pthread_mutex_unlock(&B->mu);
+}
+
+/* launches the recorded phase, brings the staging slots of nj jobs to the
+ * host and waits for both */
+static int batch_phase_end(struct bthread *T, int nj, int copy)
+{
+    bbjd_batcher *B = T->B;
+    long nl = 0;
+    int rc = bbjd_rec_flush(T->rec, T->s, &nl);
+    if (copy) rc |= !HIPOK(hipMemcpyAsync(T->hstage, T->dstage, (size_t)nj * B->slot, hipMemcpyDeviceToHost, T->s));
+    rc |= !HIPOK(hipEventRecord(T->ev, T->s));
+    rc |= !HIPOK(hipEventSynchronize(T->ev));
+    return rc;
+}
+
+static void batch_run(struct bthread *T, struct gm_job **jobs, int nj)
+{
+    bbjd_batcher *B = T->B;
+    int err = 0;
+    for (int i = 0; i < nj; i++) {
+        struct gm_job *J = jobs[i];
+        ctx_t *c = J->c;
+        J->own = c->s;
+        c->s = T->s;
+        c->stage = T->dstage + (size_t)i * B->slot;
+        c->stage_sc = B->stage_sc;
+        J->rc = 0;
+        J->cb = 0;
+        J->m = 0;
+        J->active = 1;
+        gm_begin(c, &J->G, J->ims, J->tol);
+        bbjd_rec_begin(T->rec, i);
+        const int r = gm_prologue(c, J->ndefl, J->QL, J->r, c->d->gvc, c->ws->tmp3, c->ws->tmp4, &J->cb);
+        bbjd_rec_end();
+        if (J->cb) { J->rc = -1; J->active = 0; }
+        else if (r) { J->rc = -2; J->active = 0; }
+    }
+    err |= batch_phase_end(T, nj, 1);
+    for (int i = 0; i < nj && !err; i++) {
+        struct gm_job *J = jobs[i];
+        if (!J->active) continue;
+        const double beta0 = ((const double *)(T->hstage + (size_t)i * B->slot + B->stage_sc))[SL_G];
+        if (beta0 < 1e-300) { J->active = 0; J->m = -1; continue; }   /* z = 0 */
+        gm_start(&J->G, beta0);
+    }
+    for (bbjd_int j = 0; !err; j++) {
+        int na = 0;
+        for (int i = 0; i < nj; i++) {
+            struct gm_job *J = jobs[i];
+            if (!J->active) continue;
+            ctx_t *c = J->c;
+            struct bbjd_dev *d = c->d;
+            const int joint = (size_t)(J->ims + 1) * 8 <= (size_t)((const char *)d->rd.scal - (const char *)d->pencil);
+            bbjd_rec_begin(T->rec, i);
+            const int er = arnoldi_enqueue(c, j, J->ndefl, J->QR, J->QL, d->gvc, c->ws->tmp3, c->ws->tmp4, NULL,
+                                           joint ? NULL : d->hscal);
+            bbjd_rec_end();
+            if (er) { J->rc = er == -1 ? -1 : -2; J->active = 0; continue; }
+            na++;
+        }
+        if (!na) break;
+        err |= batch_phase_end(T, nj, 1);
+        if (err) break;
+        for (int i = 0; i < nj; i++) {
+            struct gm_job *J = jobs[i];
+            if (!J->active) continue;
+            const char *hb = T->hstage + (size_t)i * B->slot;
+            const double *hs = (const double *)(hb + B->stage_sc);
+            const void *hh = hb + B->stage_sc - (size_t)(j + 1) * 8;
+            B->st.steps++;
+            if (gm_column(J->c, &J->G, j, J->ndefl, hh, hs)) { J->m = j + 1; J->active = 0; }
+            else if (j + 1 == J->ims) { J->m = J->ims; J->active = 0; }
+        }
+    }
+    if (!err) {
+        for (int i = 0; i < nj; i++) {
+            struct gm_job *J = jobs[i];
+            if (J->rc) continue;
+            ctx_t *c = J->c;
+            bbjd_rec_begin(T->rec, i);
+            int r;
+            if (J->m < 0) r = bbjd_memset_async(J->z, 0, (size_t)c->n * 16, c->s);
+            else r = gm_finish(c, &J->G, J->m, J->ndefl, J->QR, c->d->gvc, J->z);
+            bbjd_rec_end();
+            if (r) J->rc = -2;
+        }
+        err |= batch_phase_end(T, nj, 0);
+    }
+    for (int i = 0; i < nj; i++) {
+        struct gm_job *J = jobs[i];
+        if (err && !J->rc) J->rc = -2;
+        J->c->s = J->own;
+        J->c->stage = NULL;
+    }
+}
+
+static void *batcher_main(void *arg)
+{
+    struct bthread *T = (struct bthread *)arg;
+    bbjd_batcher *B = T->B;
+    struct gm_job *jobs[BATCH_JOBS_MAX];
+    pthread_mutex_lock(&B->mu);
+    for (;;) {
+        while (!B->head && !B->stop) pthread_cond_wait(&B->cv_job, &B->mu);
+        if (!B->head) break;
+        /* more may come while solves of the batcher's workspaces are neither
+         * waiting here nor in a batch: wait for them up to the window */
+        while (B->head && B->queued < B->max_jobs && B->queued + B->running < B->active && !B->stop) {
+            const double left = B->head->t_queued + B->window - now_s();
+            if (left <= 0) break;
+            struct timespec ts;
+            clock_gettime(CLOCK_REALTIME, &ts);
+            const double t = ts.tv_sec + ts.tv_nsec * 1e-9 + left;
+            ts.tv_sec = (time_t)t;
+            ts.tv_nsec = (long)((t - (double)ts.tv_sec) * 1e9);
+            pthread_cond_timedwait(&B->cv_job, &B->mu, &ts);
+        }
+        if (!B->head) continue;
+        /* take up to max_jobs, one per operator and none whose operator is in
+         * a running batch (an operator's scratch serves one apply at a time) */
+        int nj = 0;
+        struct gm_job *prev = NULL, *J = B->head;
+        while (J && nj < B->max_jobs) {
+            struct gm_job *next = J->next;
+            const void *op = J->c->P->op_data;
+            int dup = 0;
+            for (int i = 0; i < B->nbusy; i++) dup |= B->busy[i] == op;
+            if (!dup) {
+                B->busy[B->nbusy++] = op;
+                if (prev) prev->next = next;
+                else B->head = next;
+                if (B->tail == J) B->tail = prev;
+                jobs[nj++] = J;
+            } else
+                prev = J;
+            J = next;
+        }
+        if (!nj) {
+            /* every waiting solve's operator is busy: wait for a batch to end */
+            pthread_cond_wait(&B->cv_job, &B->mu);
+            continue;
+        }
+        B->queued -= nj;
+        B->running += nj;
+        pthread_mutex_unlock(&B->mu);
+        batch_run(T, jobs, nj);
+        pthread_mutex_lock(&B->mu);
+        for (int i = 0; i < nj; i++) {
+            jobs[i]->done = 1;
+            for (int b = 0; b < B->nbusy; b++)
+                if (B->busy[b] == jobs[i]->c->P->op_data) { B->busy[b] = B->busy[--B->nbusy]; break; }
+        }
+        B->running -= nj;
+        B->st.batches++;
+        B->st.jobs += nj;
+        pthread_cond_broadcast(&B->cv_done);
+        pthread_cond_broadcast(&B->cv_job);
+    }
+    pthread_mutex_unlock(&B->mu);
+    return NULL;
+}
+
+bbjd_batcher *bbjd_batcher_create(int threads, int max_jobs, double window_us, bbjd_int inner_maxiter)
+{
+    if (threads < 1 || threads > BATCH_THREADS_MAX || max_jobs < 1 || max_jobs > BATCH_JOBS_MAX ||
+        inner_maxiter < 1 || window_us < 0)
+        return NULL;
+    bbjd_batcher *B = (bbjd_batcher *)calloc(1, sizeof(*B));
+    if (!B) return NULL;
+    pthread_mutex_init(&B->mu, NULL);
+    pthread_cond_init(&B->cv_job, NULL);
+    pthread_cond_init(&B->cv_done, NULL);
+    B->nth = threads;
+    B->max_jobs = max_jobs;
+    B->window = window_us * 1e-6;
+    B->ims_max = inner_maxiter;
+    /* a slot: the Hessenberg column (complex64, ending at stage_sc), then
+     * the scalar slots */
+    B->stage_sc = ((size_t)(inner_maxiter + 1) * 8 + 255) / 256 * 256;
+    B->slot = B->stage_sc + 256;
+    int ok = 1;
+    for (int i = 0; i < threads && ok; i++) {
+        struct bthread *T = &B->t[i];
+        T->B = B;
+        ok &= HIPOK(hipStreamCreateWithFlags(&T->s, hipStreamNonBlocking));
+        ok &= HIPOK(hipEventCreateWithFlags(&T->ev, hipEventDisableTiming | hipEventBlockingSync));
+        ok &= HIPOK(hipMalloc((void **)&T->dstage, BATCH_JOBS_MAX * B->slot));
+        ok &= HIPOK(hipHostMalloc((void **)&T->hstage, BATCH_JOBS_MAX * B->slot, hipHostMallocDefault));
+        T->rec = bbjd_rec_create();
+        ok &= T->rec != NULL;
+        if (ok) T->started = pthread_create(&T->th, NULL, batcher_main, T) == 0;
+        ok &= T->started;
+    }
+    if (!ok) { bbjd_batcher_destroy(B); return NULL; }
+    return B;
+}
+
+void bbjd_batcher_destroy(bbjd_batcher *B)
+{
+    if (!B) return;
+    pthread_mutex_lock(&B->mu);
+    B->stop = 1;
+    pthread_cond_broadcast(&B->cv_job);
+    pthread_mutex_unlock(&B->mu);
+    for (int i = 0; i < B->nth; i++) {
+        struct bthread *T = &B->t[i];
+        if (T->started) pthread_join(T->th, NULL);
+        if (T->rec) bbjd_rec_destroy(T->rec);
+        if (T->hstage) (void)hipHostFree(T->hstage);
+        if (T->dstage) (void)hipFree(T->dstage);
+        if (T->ev) (void)hipEventDestroy(T->ev);
+        if (T->s) (void)hipStreamDestroy(T->s);
+    }
+    pthread_cond_destroy(&B->cv_job);
+    pthread_cond_destroy(&B->cv_done);
+    pthread_mutex_destroy(&B->mu);
+    free(B);
+}
+
+void bbjd_batcher_get_stats(const bbjd_batcher *B, bbjd_batcher_stats *st)
+{
+    if (!B || !st) return;
+    pthread_mutex_lock((pthread_mutex_t *)&B->mu);
+    *st = B->st;
+    st->recorded = st->launches = 0;
+    for (int i = 0; i < B->nth; i++) {
+        long r = 0, l = 0;
+        if (B->t[i].rec) bbjd_rec_counts(B->t[i].rec, &r, &l);
+        st->recorded += r;
+        st->launches += l;
+    }
+    pthread_mutex_unlock((pthread_mutex_t *)&B->mu);
 }
 
 /* Pencil column/row k after expansion (reference src/jd_core.c:1085-1104):
@@ -1403,7 +1845,7 @@ static int qz_device(ctx_t *c, bbjd_int k, const zcx *MA, const zcx *MB, const z
     return 0;
 }
 
-static bbjd_status solve_impl(const bbjd_params *P, bbjd_workspace *ws, bbjd_result *R, bbjd_int ms_solve)
+static bbjd_status solve_body(const bbjd_params *P, bbjd_workspace *ws, bbjd_result *R, bbjd_int ms_solve)
 {
     struct bbjd_dev *d = ws->dev;
     const double t_start = now_s();
@@ -1681,6 +2123,17 @@ cb_err:
     d->st.solve_seconds = now_s() - t_start;
     R->status = BBJD_ERR_CALLBACK;
     return BBJD_ERR_CALLBACK;
+}
+
+/* a solve counts as active for its workspace's batcher, which then knows
+ * whether more inner solves may still come */
+static bbjd_status solve_impl(const bbjd_params *P, bbjd_workspace *ws, bbjd_result *R, bbjd_int ms_solve)
+{
+    bbjd_batcher *B = ws->dev ? (bbjd_batcher *)ws->dev->opt.inner_batcher : NULL;
+    if (B) batcher_enter(B);
+    const bbjd_status st = solve_body(P, ws, R, ms_solve);
+    if (B) batcher_leave(B);
+    return st;
 }
 
 bbjd_status bbjd_solve(const bbjd_params *P, bbjd_workspace *ws, bbjd_result *R)

@@ -26,6 +26,7 @@
 #include <mutex>
 #include <vector>
 #include "engine.h"
+#include "launch.h"
 
 typedef long long i64;
 
@@ -244,7 +245,7 @@ __device__ __forceinline__ void proj_finish(int c0, int ntiles, int k, const Cx<
 // instantiation, so profiles tell the passes that ran from those that
 // returned at once).
 template <typename R, int NV, int C, bool COND>
-__global__ __launch_bounds__(TPB) void k_proj(i64 n, i64 chunk, int k, const Cx<R> *__restrict__ Q, i64 ldq,
+__device__ __forceinline__ void k_proj_b(const VB vb, i64 n, i64 chunk, int k, const Cx<R> *__restrict__ Q, i64 ldq,
                                               const Cx<R> *__restrict__ x0, const Cx<R> *__restrict__ x1,
                                               const Cx<R> *__restrict__ x2, Cx<R> ca, Cx<R> cb, Cx<R> *part,
                                               int ldp, R *pnorm, int want_norm, const int *__restrict__ cond,
@@ -259,10 +260,10 @@ __global__ __launch_bounds__(TPB) void k_proj(i64 n, i64 chunk, int k, const Cx<
     __shared__ R lds4[4];
     __shared__ int lflag;
     const int t = threadIdx.x;
-    const int tile = blockIdx.y;
-    const int c0 = blockIdx.x * C;
+    const int tile = vb.by;
+    const int c0 = vb.bx * C;
     const i64 r1 = (i64)(tile + 1) * chunk < n ? (i64)(tile + 1) * chunk : n;
-    const bool do_norm = want_norm && blockIdx.x == 0;
+    const bool do_norm = want_norm && vb.bx == 0;
     const Cx<R> *q0 = Q + (i64)c0 * ldq;
 
     // A block streams its chunk of rows; a thread accumulates C columns
@@ -331,28 +332,45 @@ __global__ __launch_bounds__(TPB) void k_proj(i64 n, i64 chunk, int k, const Cx<
         if (t == 0) pnorm[tile] = s;
     }
     if (!fused) return;
-    if (!last_block_of(cnt + blockIdx.x, gridDim.y, &lflag)) return;
-    proj_finish<R, NV, C>(c0, gridDim.y, k, part, ldp, pnorm, want_norm && blockIdx.x == 0, out0, out1, out2,
+    if (!last_block_of(cnt + vb.bx, vb.gy, &lflag)) return;
+    proj_finish<R, NV, C>(c0, vb.gy, k, part, ldp, pnorm, want_norm && vb.bx == 0, out0, out1, out2,
                           scal, norm_slot, lds4);
-    if (t == 0) cnt[blockIdx.x] = 0;
+    if (t == 0) cnt[vb.bx] = 0;
+}
+template <typename R, int NV, int C, bool COND>
+__global__ __launch_bounds__(TPB) void k_proj(i64 n, i64 chunk, int k, const Cx<R> *__restrict__ Q, i64 ldq,
+                                              const Cx<R> *__restrict__ x0, const Cx<R> *__restrict__ x1,
+                                              const Cx<R> *__restrict__ x2, Cx<R> ca, Cx<R> cb, Cx<R> *part,
+                                              int ldp, R *pnorm, int want_norm, const int *__restrict__ cond,
+                                              int *cnt, Cx<R> *out0, Cx<R> *out1, Cx<R> *out2, double *scal,
+                                              int norm_slot, int fused)
+{
+    k_proj_b<R, NV, C, COND>(VB_HW, n, chunk, k, Q, ldq, x0, x1, x2, ca, cb, part, ldp, pnorm, want_norm, cond, cnt, out0, out1, out2, scal, norm_slot, fused);
 }
 
 template <typename R, int NV, int C>
-__global__ __launch_bounds__(TPB) void k_proj_finish(int ntiles, int k, const Cx<R> *part, int ldp, const R *pnorm,
+__device__ __forceinline__ void k_proj_finish_b(const VB vb, int ntiles, int k, const Cx<R> *part, int ldp, const R *pnorm,
                                                      int want_norm, const int *__restrict__ cond, Cx<R> *out0,
                                                      Cx<R> *out1, Cx<R> *out2, double *scal, int norm_slot)
 {
     if (cond && cond[0] == 0) return;
     __shared__ R lds4[4];
-    proj_finish<R, NV, C>(blockIdx.x * C, ntiles, k, part, ldp, pnorm, want_norm && blockIdx.x == 0, out0, out1,
+    proj_finish<R, NV, C>(vb.bx * C, ntiles, k, part, ldp, pnorm, want_norm && vb.bx == 0, out0, out1,
                           out2, scal, norm_slot, lds4);
+}
+template <typename R, int NV, int C>
+__global__ __launch_bounds__(TPB) void k_proj_finish(int ntiles, int k, const Cx<R> *part, int ldp, const R *pnorm,
+                                                     int want_norm, const int *__restrict__ cond, Cx<R> *out0,
+                                                     Cx<R> *out1, Cx<R> *out2, double *scal, int norm_slot)
+{
+    k_proj_finish_b<R, NV, C>(VB_HW, ntiles, k, part, ldp, pnorm, want_norm, cond, out0, out1, out2, scal, norm_slot);
 }
 
 // x = beta*x + alpha * Q[:, :k] h; with fin.mode >= 0 the squared norm of
 // the result is reduced (last block) and finished per NormFin.
 // Dynamic LDS: k complex.
 template <typename R, int E>
-__global__ __launch_bounds__(TPB) void k_update(i64 n, int k, const Cx<R> *__restrict__ Q, i64 ldq,
+__device__ __forceinline__ void k_update_b(const VB vb, i64 n, int k, const Cx<R> *__restrict__ Q, i64 ldq,
                                                 const Cx<R> *__restrict__ h, R alpha, int beta_one,
                                                 Cx<R> *__restrict__ x, R *pnorm, const int *__restrict__ cond,
                                                 int *cnt, NormFin fin, int fused)
@@ -365,7 +383,7 @@ __global__ __launch_bounds__(TPB) void k_update(i64 n, int k, const Cx<R> *__res
     const int t = threadIdx.x;
     for (int j = t; j < k; j += TPB) { const Cx<R> hj = h[j]; sh[j] = cx(alpha * hj.re, alpha * hj.im); }
     __syncthreads();
-    const i64 rb = (i64)blockIdx.x * (TPB * E);
+    const i64 rb = (i64)vb.bx * (TPB * E);
     Cx<R> a[E];
 #pragma unroll
     for (int e = 0; e < E; e++) {
@@ -400,10 +418,18 @@ __global__ __launch_bounds__(TPB) void k_update(i64 n, int k, const Cx<R> *__res
     }
     if (fin.mode < 0) return;
     s = block_sum(s, lds4);
-    if (t == 0) pnorm[blockIdx.x] = s;
-    if (!fused || !last_block_of(cnt, gridDim.x, &lflag)) return;
-    finish_norm<R>(pnorm, gridDim.x, fin, lds4);
+    if (t == 0) pnorm[vb.bx] = s;
+    if (!fused || !last_block_of(cnt, vb.gx, &lflag)) return;
+    finish_norm<R>(pnorm, vb.gx, fin, lds4);
     if (t == 0) *cnt = 0;
+}
+template <typename R, int E>
+__global__ __launch_bounds__(TPB) void k_update(i64 n, int k, const Cx<R> *__restrict__ Q, i64 ldq,
+                                                const Cx<R> *__restrict__ h, R alpha, int beta_one,
+                                                Cx<R> *__restrict__ x, R *pnorm, const int *__restrict__ cond,
+                                                int *cnt, NormFin fin, int fused)
+{
+    k_update_b<R, E>(VB_HW, n, k, Q, ldq, h, alpha, beta_one, x, pnorm, cond, cnt, fin, fused);
 }
 
 // x -= Y[:, :k] (M c) with a k x k matrix M (column-major) applied to the
@@ -466,7 +492,7 @@ __global__ __launch_bounds__(TPB) void k_update_pre(i64 n, int k, const Cx<R> *_
 // Four times the blocks of k_update for the same rows, so more loads are in
 // flight on shapes whose rows alone do not fill the GPU.
 template <typename R, int RW, bool COND>
-__global__ __launch_bounds__(TPB) void k_update_split(i64 n, int k, const Cx<R> *__restrict__ Q, i64 ldq,
+__device__ __forceinline__ void k_update_split_b(const VB vb, i64 n, int k, const Cx<R> *__restrict__ Q, i64 ldq,
                                                       const Cx<R> *__restrict__ h, R alpha, int beta_one,
                                                       Cx<R> *__restrict__ x, R *pnorm,
                                                       const int *__restrict__ cond, int *cnt, NormFin fin,
@@ -481,7 +507,7 @@ __global__ __launch_bounds__(TPB) void k_update_split(i64 n, int k, const Cx<R> 
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     for (int j = t; j < k; j += TPB) { const Cx<R> hj = h[j]; sh[j] = cx(alpha * hj.re, alpha * hj.im); }
     __syncthreads();
-    const i64 rb = (i64)blockIdx.x * (64 * RW);
+    const i64 rb = (i64)vb.bx * (64 * RW);
     Cx<R> a[RW];
 #pragma unroll
     for (int e = 0; e < RW; e++) a[e] = cx<R>(0, 0);
@@ -519,10 +545,19 @@ __global__ __launch_bounds__(TPB) void k_update_split(i64 n, int k, const Cx<R> 
     }
     if (fin.mode < 0) return;
     s = block_sum(s, lds4);
-    if (t == 0) pnorm[blockIdx.x] = s;
-    if (!fused || !last_block_of(cnt, gridDim.x, &lflag)) return;
-    finish_norm<R>(pnorm, gridDim.x, fin, lds4);
+    if (t == 0) pnorm[vb.bx] = s;
+    if (!fused || !last_block_of(cnt, vb.gx, &lflag)) return;
+    finish_norm<R>(pnorm, vb.gx, fin, lds4);
     if (t == 0) *cnt = 0;
+}
+template <typename R, int RW, bool COND>
+__global__ __launch_bounds__(TPB) void k_update_split(i64 n, int k, const Cx<R> *__restrict__ Q, i64 ldq,
+                                                      const Cx<R> *__restrict__ h, R alpha, int beta_one,
+                                                      Cx<R> *__restrict__ x, R *pnorm,
+                                                      const int *__restrict__ cond, int *cnt, NormFin fin,
+                                                      int fused)
+{
+    k_update_split_b<R, RW, COND>(VB_HW, n, k, Q, ldq, h, alpha, beta_one, x, pnorm, cond, cnt, fin, fused);
 }
 
 // ---------------------------------------------------------------------------
@@ -604,7 +639,7 @@ __device__ __forceinline__ void cgs_mid_finish(int col0, int col_step, bool writ
 // fin.decide, of ||v1||^2.  KW >= ceil(k / 4) columns per wave; a block walks
 // `tiles` consecutive 64-row tiles.
 template <typename R, int KW>
-__global__ __launch_bounds__(TPB) void k_cgs_mid(i64 n, int k, const Cx<R> *__restrict__ Q, i64 ldq,
+__device__ __forceinline__ void k_cgs_mid_b(const VB vb, i64 n, int k, const Cx<R> *__restrict__ Q, i64 ldq,
                                                  const Cx<R> *__restrict__ h, Cx<R> *x, const Cx<R> *__restrict__ xa,
                                                  const Cx<R> *__restrict__ xb, Cx<R> ca, Cx<R> cb, int tiles,
                                                  Cx<R> *part, R *pnorm, int *cnt, MidFin fin, int fused)
@@ -626,7 +661,7 @@ __global__ __launch_bounds__(TPB) void k_cgs_mid(i64 n, int k, const Cx<R> *__re
 #pragma unroll
     for (int g = 0; g < NG; g++) acc[g] = 0;
     R nrm = 0;
-    const i64 tb0 = (i64)blockIdx.x * tiles;
+    const i64 tb0 = (i64)vb.bx * tiles;
     for (int tt = 0; tt < tiles; tt++) {
         const i64 rb = (tb0 + tt) * 64;
         if (rb >= n) break;                       // uniform over the block
@@ -670,25 +705,39 @@ __global__ __launch_bounds__(TPB) void k_cgs_mid(i64 n, int k, const Cx<R> *__re
 #pragma unroll
         for (int g = 0; g < NG; g++) {
             const int j = w + 4 * (g * (G / 2) + (vi >> 1));
-            if (j < k) reinterpret_cast<R *>(part + (i64)blockIdx.x * k + j)[vi & 1] = acc[g];
+            if (j < k) reinterpret_cast<R *>(part + (i64)vb.bx * k + j)[vi & 1] = acc[g];
         }
     }
     if (fin.decide && w == 0) {
         const R s = wave_sum(nrm);
-        if (lane == 0) pnorm[blockIdx.x] = s;
+        if (lane == 0) pnorm[vb.bx] = s;
     }
-    if (!fused || !last_block_of(cnt, gridDim.x, &lflag)) return;
-    cgs_mid_finish<R>(0, 4, true, gridDim.x, k, part, pnorm, fin, lds4, &lp2);
+    if (!fused || !last_block_of(cnt, vb.gx, &lflag)) return;
+    cgs_mid_finish<R>(0, 4, true, vb.gx, k, part, pnorm, fin, lds4, &lp2);
     if (t == 0) *cnt = 0;
+}
+template <typename R, int KW>
+__global__ __launch_bounds__(TPB) void k_cgs_mid(i64 n, int k, const Cx<R> *__restrict__ Q, i64 ldq,
+                                                 const Cx<R> *__restrict__ h, Cx<R> *x, const Cx<R> *__restrict__ xa,
+                                                 const Cx<R> *__restrict__ xb, Cx<R> ca, Cx<R> cb, int tiles,
+                                                 Cx<R> *part, R *pnorm, int *cnt, MidFin fin, int fused)
+{
+    k_cgs_mid_b<R, KW>(VB_HW, n, k, Q, ldq, h, x, xa, xb, ca, cb, tiles, part, pnorm, cnt, fin, fused);
 }
 
 template <typename R>
-__global__ __launch_bounds__(TPB) void k_cgs_mid_fin(int nblk, int k, const Cx<R> *part, const R *pnorm,
+__device__ __forceinline__ void k_cgs_mid_fin_b(const VB vb, int nblk, int k, const Cx<R> *part, const R *pnorm,
                                                      MidFin fin)
 {
     __shared__ R lds4[4];
     __shared__ int lp2;
-    cgs_mid_finish<R>(blockIdx.x * 4, gridDim.x * 4, blockIdx.x == 0, nblk, k, part, pnorm, fin, lds4, &lp2);
+    cgs_mid_finish<R>(vb.bx * 4, vb.gx * 4, vb.bx == 0, nblk, k, part, pnorm, fin, lds4, &lp2);
+}
+template <typename R>
+__global__ __launch_bounds__(TPB) void k_cgs_mid_fin(int nblk, int k, const Cx<R> *part, const R *pnorm,
+                                                     MidFin fin)
+{
+    k_cgs_mid_fin_b<R>(VB_HW, nblk, k, part, pnorm, fin);
 }
 
 // Last launch of the fused CGS2 (k <= 128).  With the second pass (decided,
@@ -699,7 +748,7 @@ __global__ __launch_bounds__(TPB) void k_cgs_mid_fin(int nblk, int k, const Cx<R
 // output (when ||v1|| > 1e-300), else nothing.  Columns split over the four
 // waves as in k_update_split.
 template <typename R, int RW>
-__global__ __launch_bounds__(TPB) void k_cgs_final(i64 n, int k, const Cx<R> *__restrict__ Q, i64 ldq,
+__device__ __forceinline__ void k_cgs_final_b(const VB vb, i64 n, int k, const Cx<R> *__restrict__ Q, i64 ldq,
                                                    const Cx<R> *__restrict__ h2, Cx<R> *__restrict__ x, int scale,
                                                    int force, double trust, const double *scal, int *flag,
                                                    R *pnorm, int *cnt, NormFin fin, int fused)
@@ -710,7 +759,7 @@ __global__ __launch_bounds__(TPB) void k_cgs_final(i64 n, int k, const Cx<R> *__
     __shared__ int lflag;
     __shared__ double lsc;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    const i64 rb = (i64)blockIdx.x * (64 * RW);
+    const i64 rb = (i64)vb.bx * (64 * RW);
     if (!force && flag[FL_PASS2] == 0) {
         if (!scale) return;
         const double nrm = scal[SL_NA];
@@ -744,7 +793,7 @@ __global__ __launch_bounds__(TPB) void k_cgs_final(i64 n, int k, const Cx<R> *__
             else resc = 1;
         }
         lsc = sc;
-        if (blockIdx.x == 0) flag[FL_RESCALE] = resc;
+        if (vb.bx == 0) flag[FL_RESCALE] = resc;
     }
     __syncthreads();
     const R sc = (R)lsc;
@@ -786,10 +835,18 @@ __global__ __launch_bounds__(TPB) void k_cgs_final(i64 n, int k, const Cx<R> *__
     }
     if (fin.mode < 0) return;
     s = block_sum(s, lds4);
-    if (t == 0) pnorm[blockIdx.x] = s;
-    if (!fused || !last_block_of(cnt, gridDim.x, &lflag)) return;
-    finish_norm<R>(pnorm, gridDim.x, fin, lds4);
+    if (t == 0) pnorm[vb.bx] = s;
+    if (!fused || !last_block_of(cnt, vb.gx, &lflag)) return;
+    finish_norm<R>(pnorm, vb.gx, fin, lds4);
     if (t == 0) *cnt = 0;
+}
+template <typename R, int RW>
+__global__ __launch_bounds__(TPB) void k_cgs_final(i64 n, int k, const Cx<R> *__restrict__ Q, i64 ldq,
+                                                   const Cx<R> *__restrict__ h2, Cx<R> *__restrict__ x, int scale,
+                                                   int force, double trust, const double *scal, int *flag,
+                                                   R *pnorm, int *cnt, NormFin fin, int fused)
+{
+    k_cgs_final_b<R, RW>(VB_HW, n, k, Q, ldq, h2, x, scale, force, trust, scal, flag, pnorm, cnt, fin, fused);
 }
 
 // y = Q[:, :k] c with the k <= 128 coefficients passed by value in the
@@ -837,21 +894,26 @@ __global__ __launch_bounds__(TPB) void k_gemv_n_karg(i64 n, int k, const Cx<doub
 
 // separate finish of a norm (one workgroup)
 template <typename R>
-__global__ __launch_bounds__(TPB) void k_norm_fin(const R *pnorm, int nblk, NormFin fin, const int *__restrict__ cond)
+__device__ __forceinline__ void k_norm_fin_b(const VB vb, const R *pnorm, int nblk, NormFin fin, const int *__restrict__ cond)
 {
     if (cond && cond[0] == 0) return;
     __shared__ R lds4[4];
     finish_norm<R>(pnorm, nblk, fin, lds4);
 }
+template <typename R>
+__global__ __launch_bounds__(TPB) void k_norm_fin(const R *pnorm, int nblk, NormFin fin, const int *__restrict__ cond)
+{
+    k_norm_fin_b<R>(VB_HW, pnorm, nblk, fin, cond);
+}
 
 // ||x|| -> scal[slot] in one launch.
 template <typename R, int E>
-__global__ __launch_bounds__(TPB) void k_norm(i64 n, const Cx<R> *__restrict__ x, R *pnorm, int *cnt, NormFin fin,
+__device__ __forceinline__ void k_norm_b(const VB vb, i64 n, const Cx<R> *__restrict__ x, R *pnorm, int *cnt, NormFin fin,
                                               int fused)
 {
     __shared__ R lds4[4];
     __shared__ int lflag;
-    const i64 rb = (i64)blockIdx.x * (TPB * E);
+    const i64 rb = (i64)vb.bx * (TPB * E);
     R s = 0;
 #pragma unroll
     for (int e = 0; e < E; e++) {
@@ -859,44 +921,61 @@ __global__ __launch_bounds__(TPB) void k_norm(i64 n, const Cx<R> *__restrict__ x
         if (r < n) { const Cx<R> v = x[r]; s += v.re * v.re + v.im * v.im; }
     }
     s = block_sum(s, lds4);
-    if (threadIdx.x == 0) pnorm[blockIdx.x] = s;
-    if (!fused || !last_block_of(cnt, gridDim.x, &lflag)) return;
-    finish_norm<R>(pnorm, gridDim.x, fin, lds4);
+    if (threadIdx.x == 0) pnorm[vb.bx] = s;
+    if (!fused || !last_block_of(cnt, vb.gx, &lflag)) return;
+    finish_norm<R>(pnorm, vb.gx, fin, lds4);
     if (threadIdx.x == 0) *cnt = 0;
+}
+template <typename R, int E>
+__global__ __launch_bounds__(TPB) void k_norm(i64 n, const Cx<R> *__restrict__ x, R *pnorm, int *cnt, NormFin fin,
+                                              int fused)
+{
+    k_norm_b<R, E>(VB_HW, n, x, pnorm, cnt, fin, fused);
 }
 
 template <typename R>
-__global__ __launch_bounds__(TPB) void k_scale_inv(i64 n, Cx<R> *x, const double *scal, int slot, int mode,
+__device__ __forceinline__ void k_scale_inv_b(const VB vb, i64 n, Cx<R> *x, const double *scal, int slot, int mode,
                                                    const int *__restrict__ cond, Cx<R> *__restrict__ y)
 {
     if (cond && cond[0] == 0) return;
     const double nrm = scal[slot];
     if (mode == 1 && !(nrm > 1e-300)) {
         if (y)
-            for (i64 i = (i64)blockIdx.x * TPB + threadIdx.x; i < n; i += (i64)gridDim.x * TPB) y[i] = x[i];
+            for (i64 i = (i64)vb.bx * TPB + threadIdx.x; i < n; i += (i64)vb.gx * TPB) y[i] = x[i];
         return;
     }
     const R s = (R)(1.0 / nrm);
-    for (i64 i = (i64)blockIdx.x * TPB + threadIdx.x; i < n; i += (i64)gridDim.x * TPB) {
+    for (i64 i = (i64)vb.bx * TPB + threadIdx.x; i < n; i += (i64)vb.gx * TPB) {
         Cx<R> v = x[i];
         v.re *= s; v.im *= s;
         x[i] = v;
         if (y) y[i] = v;
     }
 }
+template <typename R>
+__global__ __launch_bounds__(TPB) void k_scale_inv(i64 n, Cx<R> *x, const double *scal, int slot, int mode,
+                                                   const int *__restrict__ cond, Cx<R> *__restrict__ y)
+{
+    k_scale_inv_b<R>(VB_HW, n, x, scal, slot, mode, cond, y);
+}
 
+template <typename R>
+__device__ __forceinline__ void k_scale_b(const VB vb, i64 n, Cx<R> *x, R are, R aim)
+{
+    const Cx<R> a = cx(are, aim);
+    for (i64 i = (i64)vb.bx * TPB + threadIdx.x; i < n; i += (i64)vb.gx * TPB) x[i] = a * x[i];
+}
 template <typename R>
 __global__ __launch_bounds__(TPB) void k_scale(i64 n, Cx<R> *x, R are, R aim)
 {
-    const Cx<R> a = cx(are, aim);
-    for (i64 i = (i64)blockIdx.x * TPB + threadIdx.x; i < n; i += (i64)gridDim.x * TPB) x[i] = a * x[i];
+    k_scale_b<R>(VB_HW, n, x, are, aim);
 }
 
 // out = a x + b y; with co non-NULL the coefficients are read from device
 // memory instead, a = (co[0], co[1]), b = (co[2], co[3]) (doubles, rounded
 // to R as host arguments are), so a captured graph replays with new ones.
 template <typename R, int E>
-__global__ __launch_bounds__(TPB) void k_lincomb(i64 n, R are, R aim, const Cx<R> *x, R bre, R bim,
+__device__ __forceinline__ void k_lincomb_b(const VB vb, i64 n, R are, R aim, const Cx<R> *x, R bre, R bim,
                                                  const Cx<R> *y, Cx<R> *out, R *pnorm, int *cnt, NormFin fin,
                                                  int fused, const double *__restrict__ co)
 {
@@ -904,7 +983,7 @@ __global__ __launch_bounds__(TPB) void k_lincomb(i64 n, R are, R aim, const Cx<R
     __shared__ int lflag;
     if (co) { are = (R)co[0]; aim = (R)co[1]; bre = (R)co[2]; bim = (R)co[3]; }
     const Cx<R> a = cx(are, aim), b = cx(bre, bim);
-    const i64 rb = (i64)blockIdx.x * (TPB * E);
+    const i64 rb = (i64)vb.bx * (TPB * E);
     R s = 0;
 #pragma unroll
     for (int e = 0; e < E; e++) {
@@ -917,10 +996,17 @@ __global__ __launch_bounds__(TPB) void k_lincomb(i64 n, R are, R aim, const Cx<R
     }
     if (fin.mode < 0) return;
     s = block_sum(s, lds4);
-    if (threadIdx.x == 0) pnorm[blockIdx.x] = s;
-    if (!fused || !last_block_of(cnt, gridDim.x, &lflag)) return;
-    finish_norm<R>(pnorm, gridDim.x, fin, lds4);
+    if (threadIdx.x == 0) pnorm[vb.bx] = s;
+    if (!fused || !last_block_of(cnt, vb.gx, &lflag)) return;
+    finish_norm<R>(pnorm, vb.gx, fin, lds4);
     if (threadIdx.x == 0) *cnt = 0;
+}
+template <typename R, int E>
+__global__ __launch_bounds__(TPB) void k_lincomb(i64 n, R are, R aim, const Cx<R> *x, R bre, R bim,
+                                                 const Cx<R> *y, Cx<R> *out, R *pnorm, int *cnt, NormFin fin,
+                                                 int fused, const double *__restrict__ co)
+{
+    k_lincomb_b<R, E>(VB_HW, n, are, aim, x, bre, bim, y, out, pnorm, cnt, fin, fused, co);
 }
 
 // Sleijpen-van der Vorst step z += c s, with c = -(u^H z)/(u^H s) when
@@ -1301,15 +1387,24 @@ __global__ __launch_bounds__(TPB) void k_gram_finish(int nch, int p, int q, cons
     }
 }
 
+__device__ __forceinline__ void k_z2c_b(const VB vb, i64 n, const Cx<double> *z, Cx<float> *c)
+{
+    for (i64 i = (i64)vb.bx * TPB + threadIdx.x; i < n; i += (i64)vb.gx * TPB)
+        c[i] = cx((float)z[i].re, (float)z[i].im);
+}
 __global__ __launch_bounds__(TPB) void k_z2c(i64 n, const Cx<double> *z, Cx<float> *c)
 {
-    for (i64 i = (i64)blockIdx.x * TPB + threadIdx.x; i < n; i += (i64)gridDim.x * TPB)
-        c[i] = cx((float)z[i].re, (float)z[i].im);
+    k_z2c_b(VB_HW, n, z, c);
+}
+
+__device__ __forceinline__ void k_c2z_b(const VB vb, i64 n, const Cx<float> *c, Cx<double> *z)
+{
+    for (i64 i = (i64)vb.bx * TPB + threadIdx.x; i < n; i += (i64)vb.gx * TPB)
+        z[i] = cx((double)c[i].re, (double)c[i].im);
 }
 __global__ __launch_bounds__(TPB) void k_c2z(i64 n, const Cx<float> *c, Cx<double> *z)
 {
-    for (i64 i = (i64)blockIdx.x * TPB + threadIdx.x; i < n; i += (i64)gridDim.x * TPB)
-        z[i] = cx((double)c[i].re, (double)c[i].im);
+    k_c2z_b(VB_HW, n, c, z);
 }
 
 // ---------------------------------------------------------------------------
@@ -1468,12 +1563,12 @@ static int proj_pass_nv(hipStream_t s, i64 n, int k, const void *Q, i64 ldq, con
         std::lock_guard<std::mutex> lk(g_kt_mu);
         g_kt.push_back({ea, eb, bytes});
     } else if (cond)
-        hipLaunchKernelGGL((k_proj<R, NV, C, true>), g, dim3(TPB), 0, s, (i64)n, chunk, k, (const Cx<R> *)Q,
+        bbjd::gl<&k_proj_b<R, NV, C, true>, TPB>(k_proj<R, NV, C, true>, g, 0, s, (i64)n, chunk, k, (const Cx<R> *)Q,
                            (i64)ldq, (const Cx<R> *)x0, (const Cx<R> *)x1, (const Cx<R> *)x2, ca, cb, (Cx<R> *)rd->part, ldp,
                            (R *)rd->pnorm, want_norm, cond, rd->cnt, (Cx<R> *)out0, (Cx<R> *)out1, (Cx<R> *)out2,
                            rd->scal, norm_slot, fused);
     else
-        hipLaunchKernelGGL((k_proj<R, NV, C, false>), g, dim3(TPB), 0, s, (i64)n, chunk, k, (const Cx<R> *)Q,
+        bbjd::gl<&k_proj_b<R, NV, C, false>, TPB>(k_proj<R, NV, C, false>, g, 0, s, (i64)n, chunk, k, (const Cx<R> *)Q,
                            (i64)ldq, (const Cx<R> *)x0, (const Cx<R> *)x1, (const Cx<R> *)x2, ca, cb, (Cx<R> *)rd->part, ldp,
                            (R *)rd->pnorm, want_norm, cond, rd->cnt, (Cx<R> *)out0, (Cx<R> *)out1, (Cx<R> *)out2,
                            rd->scal, norm_slot, fused);
@@ -1481,7 +1576,7 @@ static int proj_pass_nv(hipStream_t s, i64 n, int k, const void *Q, i64 ldq, con
     // count instead of a finish launch (0 when the last block finished)
     if (finish_left) *finish_left = fused ? 0 : nt;
     if (!fused && !finish_left)
-        hipLaunchKernelGGL((k_proj_finish<R, NV, C>), dim3(g.x), dim3(TPB), 0, s, nt, k, (const Cx<R> *)rd->part,
+        bbjd::gl<&k_proj_finish_b<R, NV, C>, TPB>(k_proj_finish<R, NV, C>, dim3(g.x), 0, s, nt, k, (const Cx<R> *)rd->part,
                            ldp, (const R *)rd->pnorm, want_norm, cond, (Cx<R> *)out0, (Cx<R> *)out1,
                            (Cx<R> *)out2, rd->scal, norm_slot);
     return chk();
@@ -1507,7 +1602,7 @@ static int update_pass(hipStream_t s, i64 n, int k, const void *Q, i64 ldq, cons
     const int fused = nt <= g_fuse_max_blocks;
     if (rows < TPB) {
 #define BBJD_LAUNCH_SPLIT(RW, CO)                                                                                  \
-    hipLaunchKernelGGL((k_update_split<R, RW, CO>), dim3(nt), dim3(TPB), lds, s, (i64)n, k, (const Cx<R> *)Q,        \
+    bbjd::gl<&k_update_split_b<R, RW, CO>, TPB>(k_update_split<R, RW, CO>, dim3(nt), lds, s, (i64)n, k, (const Cx<R> *)Q,        \
                        (i64)ldq, (const Cx<R> *)h, alpha, beta_one, (Cx<R> *)x, (R *)rd->pnorm, cond,                \
                        rd->cnt + BBJD_CNT_UPD, fin, fused)
         if (rows == 128) {
@@ -1519,17 +1614,17 @@ static int update_pass(hipStream_t s, i64 n, int k, const void *Q, i64 ldq, cons
         }
 #undef BBJD_LAUNCH_SPLIT
         if (!fused && fin.mode >= 0)
-            hipLaunchKernelGGL((k_norm_fin<R>), dim3(1), dim3(TPB), 0, s, (const R *)rd->pnorm, nt, fin, cond);
+            bbjd::gl<&k_norm_fin_b<R>, TPB>(k_norm_fin<R>, dim3(1), 0, s, (const R *)rd->pnorm, nt, fin, cond);
         return chk();
     }
 #define BBJD_LAUNCH_UPD(E)                                                                                         \
-    hipLaunchKernelGGL((k_update<R, E>), dim3(nt), dim3(TPB), lds, s, (i64)n, k, (const Cx<R> *)Q, (i64)ldq,         \
+    bbjd::gl<&k_update_b<R, E>, TPB>(k_update<R, E>, dim3(nt), lds, s, (i64)n, k, (const Cx<R> *)Q, (i64)ldq,         \
                        (const Cx<R> *)h, alpha, beta_one, (Cx<R> *)x, (R *)rd->pnorm, cond, rd->cnt + BBJD_CNT_UPD, \
                        fin, fused)
     BBJD_E_SWITCH(rows, BBJD_LAUNCH_UPD)
 #undef BBJD_LAUNCH_UPD
     if (!fused && fin.mode >= 0)
-        hipLaunchKernelGGL((k_norm_fin<R>), dim3(1), dim3(TPB), 0, s, (const R *)rd->pnorm, nt, fin, cond);
+        bbjd::gl<&k_norm_fin_b<R>, TPB>(k_norm_fin<R>, dim3(1), 0, s, (const R *)rd->pnorm, nt, fin, cond);
     return chk();
 }
 
@@ -1540,12 +1635,12 @@ static int norm_into(hipStream_t s, i64 n, const void *x, bbjd_red *rd, int slot
     const int nt = ntiles(n, rows);
     const int fused = nt <= g_fuse_max_blocks;
 #define BBJD_LAUNCH_NORM(E)                                                                                        \
-    hipLaunchKernelGGL((k_norm<R, E>), dim3(nt), dim3(TPB), 0, s, (i64)n, (const Cx<R> *)x, (R *)rd->pnorm,         \
+    bbjd::gl<&k_norm_b<R, E>, TPB>(k_norm<R, E>, dim3(nt), 0, s, (i64)n, (const Cx<R> *)x, (R *)rd->pnorm,         \
                        rd->cnt + BBJD_CNT_VEC, fin_norm(rd, 0, slot), fused)
     BBJD_E_SWITCH(rows, BBJD_LAUNCH_NORM)
 #undef BBJD_LAUNCH_NORM
     if (!fused)
-        hipLaunchKernelGGL((k_norm_fin<R>), dim3(1), dim3(TPB), 0, s, (const R *)rd->pnorm, nt,
+        bbjd::gl<&k_norm_fin_b<R>, TPB>(k_norm_fin<R>, dim3(1), 0, s, (const R *)rd->pnorm, nt,
                            fin_norm(rd, 0, slot), (const int *)nullptr);
     return chk();
 }
@@ -1573,7 +1668,7 @@ static inline int cgs_kw(int k)
 template <typename R>
 static int scale_inv(hipStream_t s, i64 n, void *x, const double *scal, int slot, const int *cond)
 {
-    hipLaunchKernelGGL((k_scale_inv<R>), dim3(grid_stride_blocks(n)), dim3(TPB), 0, s, (i64)n, (Cx<R> *)x, scal,
+    bbjd::gl<&k_scale_inv_b<R>, TPB>(k_scale_inv<R>, dim3(grid_stride_blocks(n)), 0, s, (i64)n, (Cx<R> *)x, scal,
                        slot, 1, cond, (Cx<R> *)nullptr);
     return chk();
 }
@@ -1603,7 +1698,7 @@ static int cgs_mid(hipStream_t s, i64 n, int k, const void *Q, i64 ldq, void *v,
     const int fused = nb <= g_fuse_max_blocks;
     int *cnt = rd->cnt + BBJD_CNT_MID;
 #define BBJD_LAUNCH_MID(KW)                                                                                         \
-    hipLaunchKernelGGL((k_cgs_mid<R, KW>), dim3((unsigned)nb), dim3(TPB), 0, s, (i64)n, k, (const Cx<R> *)Q,         \
+    bbjd::gl<&k_cgs_mid_b<R, KW>, TPB>(k_cgs_mid<R, KW>, dim3((unsigned)nb), 0, s, (i64)n, k, (const Cx<R> *)Q,         \
                        (i64)ldq, (const Cx<R> *)f.h1, (Cx<R> *)v, (const Cx<R> *)xa, (const Cx<R> *)xb, ca, cb,      \
                        (int)per, (Cx<R> *)rd->part, (R *)rd->pnorm, cnt, f, fused)
     switch (cgs_kw(k)) {
@@ -1616,7 +1711,7 @@ static int cgs_mid(hipStream_t s, i64 n, int k, const void *Q, i64 ldq, void *v,
     }
 #undef BBJD_LAUNCH_MID
     if (!fused)
-        hipLaunchKernelGGL((k_cgs_mid_fin<R>), dim3((unsigned)((k + 3) / 4)), dim3(TPB), 0, s, (int)nb, k,
+        bbjd::gl<&k_cgs_mid_fin_b<R>, TPB>(k_cgs_mid_fin<R>, dim3((unsigned)((k + 3) / 4)), 0, s, (int)nb, k,
                            (const Cx<R> *)rd->part, (const R *)rd->pnorm, f);
     return chk();
 }
@@ -1631,11 +1726,11 @@ static int cgs_final(hipStream_t s, i64 n, int k, const void *Q, i64 ldq, void *
     const int fused = nt <= g_fuse_max_blocks;
     // mode 2 never trusts the estimate (e <= ||v1||^2 < 2 ||v1||^2)
     const double trust = g_cgs_fused == 2 ? 2.0 : 0.25;
-    hipLaunchKernelGGL((k_cgs_final<R, RW>), dim3(nt), dim3(TPB), 0, s, (i64)n, k, (const Cx<R> *)Q, (i64)ldq,
+    bbjd::gl<&k_cgs_final_b<R, RW>, TPB>(k_cgs_final<R, RW>, dim3(nt), 0, s, (i64)n, k, (const Cx<R> *)Q, (i64)ldq,
                        (const Cx<R> *)rd->h2, (Cx<R> *)v, scale, force, trust, (const double *)rd->scal,
                        rd->flag, (R *)rd->pnorm, rd->cnt + BBJD_CNT_UPD, fin, fused);
     if (!fused && fin.mode >= 0)
-        hipLaunchKernelGGL((k_norm_fin<R>), dim3(1), dim3(TPB), 0, s, (const R *)rd->pnorm, nt, fin,
+        bbjd::gl<&k_norm_fin_b<R>, TPB>(k_norm_fin<R>, dim3(1), 0, s, (const R *)rd->pnorm, nt, fin,
                            force ? (const int *)nullptr : (const int *)(rd->flag + FL_PASS2));
     return chk();
 }
@@ -1758,17 +1853,38 @@ static int lincomb(hipStream_t s, i64 n, double are, double aim, const void *x, 
     const int nt = ntiles(n, rows);
     const int fused = nt <= g_fuse_max_blocks;
 #define BBJD_LAUNCH_LC(E)                                                                                          \
-    hipLaunchKernelGGL((k_lincomb<R, E>), dim3(nt), dim3(TPB), 0, s, (i64)n, (R)are, (R)aim, (const Cx<R> *)x,      \
+    bbjd::gl<&k_lincomb_b<R, E>, TPB>(k_lincomb<R, E>, dim3(nt), 0, s, (i64)n, (R)are, (R)aim, (const Cx<R> *)x,      \
                        (R)bre, (R)bim, (const Cx<R> *)y, (Cx<R> *)out, rd ? (R *)rd->pnorm : (R *)nullptr,          \
                        rd ? rd->cnt + BBJD_CNT_VEC : (int *)nullptr, rd ? fin_norm(rd, 0, slot) : fin_none(), fused, \
                        co)
     BBJD_E_SWITCH(rows, BBJD_LAUNCH_LC)
 #undef BBJD_LAUNCH_LC
     if (rd && !fused)
-        hipLaunchKernelGGL((k_norm_fin<R>), dim3(1), dim3(TPB), 0, s, (const R *)rd->pnorm, nt,
+        bbjd::gl<&k_norm_fin_b<R>, TPB>(k_norm_fin<R>, dim3(1), 0, s, (const R *)rd->pnorm, nt,
                            fin_norm(rd, 0, slot), (const int *)nullptr);
     return chk();
 }
+
+// Copy of nw 8-byte words (small device-to-device staging copies that a
+// grouped launch can merge).
+__device__ __forceinline__ void k_copy_words_b(const VB vb, i64 nw, const unsigned long long *__restrict__ src,
+                                               unsigned long long *__restrict__ dst)
+{
+    for (i64 i = (i64)vb.bx * TPB + threadIdx.x; i < nw; i += (i64)vb.gx * TPB) dst[i] = src[i];
+}
+__global__ __launch_bounds__(TPB) void k_copy_words(i64 nw, const unsigned long long *__restrict__ src,
+                                                    unsigned long long *__restrict__ dst)
+{
+    k_copy_words_b(VB_HW, nw, src, dst);
+}
+
+namespace bbjd {
+Recorder *&tl_rec()
+{
+    static thread_local Recorder *r = nullptr;
+    return r;
+}
+}  // namespace bbjd
 
 extern "C" {
 
@@ -1865,7 +1981,6 @@ int bbjd_k_cgs2_c(hipStream_t s, int64_t n, int k, const void *Q, int64_t ldq, v
 int bbjd_k_project_out(hipStream_t s, int64_t n, int k, const void *Q, int64_t ldq, void *v, bbjd_red *rd)
 { return project_out<double>(s, n, k, Q, ldq, v, rd); }
 
-
 int bbjd_k_project_out_c(hipStream_t s, int64_t n, int k, const void *Q, int64_t ldq, void *v, bbjd_red *rd)
 { return project_out<float>(s, n, k, Q, ldq, v, rd); }
 
@@ -1889,7 +2004,7 @@ int bbjd_k_update_pre_c(hipStream_t s, int64_t n, int k, const void *Y, int64_t 
     const int e = n >= (i64)512 * 4 * TPB ? 4 : n >= (i64)256 * 2 * TPB ? 2 : 1;
     const int nb = (int)((n + (i64)TPB * e - 1) / ((i64)TPB * e));
 #define BBJD_LAUNCH_PRE(E)                                                                                         \
-    hipLaunchKernelGGL((k_update_pre<float, E>), dim3(nb), dim3(TPB), lds, s, (i64)n, k, (const Cx<float> *)Y,     \
+    BBJD_SL(s, (k_update_pre<float, E>), dim3(nb), dim3(TPB), lds, (i64)n, k, (const Cx<float> *)Y,     \
                        (i64)ldy, (const Cx<float> *)M, (const Cx<float> *)c, (Cx<float> *)x)
     switch (e) {
     case 4: BBJD_LAUNCH_PRE(4); break;
@@ -1909,7 +2024,7 @@ int bbjd_k_gemv_n_host(hipStream_t s, int64_t n, int k, const void *Q, int64_t l
     if (k < 1 || k > 128) return -2;
     Coef128 c;
     memcpy(c.v, hc, (size_t)k * sizeof(Cx<double>));
-    hipLaunchKernelGGL(k_gemv_n_karg<2>, dim3(ntiles(n, 128)), dim3(TPB), 0, s, (i64)n, k, (const Cx<double> *)Q,
+    BBJD_SL(s, k_gemv_n_karg<2>, dim3(ntiles(n, 128)), dim3(TPB), 0, (i64)n, k, (const Cx<double> *)Q,
                        (i64)ldq, c, (Cx<double> *)y);
     return chk();
 }
@@ -1936,28 +2051,28 @@ int bbjd_k_scale_inv_c(hipStream_t s, int64_t n, void *x, const double *scal, in
 
 int bbjd_k_scale_inv2(hipStream_t s, int64_t n, void *x, void *y, const double *scal, int slot, int mode)
 {
-    hipLaunchKernelGGL((k_scale_inv<double>), dim3(grid_stride_blocks(n)), dim3(TPB), 0, s, (i64)n,
+    bbjd::gl<&k_scale_inv_b<double>, TPB>(k_scale_inv<double>, dim3(grid_stride_blocks(n)), 0, s, (i64)n,
                        (Cx<double> *)x, scal, slot, mode, (const int *)nullptr, (Cx<double> *)y);
     return chk();
 }
 
 int bbjd_k_scale_inv2_c(hipStream_t s, int64_t n, void *x, void *y, const double *scal, int slot, int mode)
 {
-    hipLaunchKernelGGL((k_scale_inv<float>), dim3(grid_stride_blocks(n)), dim3(TPB), 0, s, (i64)n,
+    bbjd::gl<&k_scale_inv_b<float>, TPB>(k_scale_inv<float>, dim3(grid_stride_blocks(n)), 0, s, (i64)n,
                        (Cx<float> *)x, scal, slot, mode, (const int *)nullptr, (Cx<float> *)y);
     return chk();
 }
 
 int bbjd_k_scale(hipStream_t s, int64_t n, void *x, double are, double aim)
 {
-    hipLaunchKernelGGL((k_scale<double>), dim3(grid_stride_blocks(n)), dim3(TPB), 0, s, (i64)n, (Cx<double> *)x,
+    bbjd::gl<&k_scale_b<double>, TPB>(k_scale<double>, dim3(grid_stride_blocks(n)), 0, s, (i64)n, (Cx<double> *)x,
                        are, aim);
     return chk();
 }
 
 int bbjd_k_scale_c(hipStream_t s, int64_t n, void *x, double are, double aim)
 {
-    hipLaunchKernelGGL((k_scale<float>), dim3(grid_stride_blocks(n)), dim3(TPB), 0, s, (i64)n, (Cx<float> *)x,
+    bbjd::gl<&k_scale_b<float>, TPB>(k_scale<float>, dim3(grid_stride_blocks(n)), 0, s, (i64)n, (Cx<float> *)x,
                        (float)are, (float)aim);
     return chk();
 }
@@ -1987,10 +2102,10 @@ int bbjd_k_svdv(hipStream_t s, int64_t n, const void *u, void *z, const void *sv
     int rc = proj_pass_nv<double, 2>(s, n, 1, u, n, z, sv, nullptr, d, d + 1, nullptr, rd, -1, nullptr,
                                      Cx<double>{0, 0}, Cx<double>{0, 0}, g_svdv_fused ? &left : nullptr);
     if (left)
-        hipLaunchKernelGGL(k_axpy_svdv_part, dim3(grid_stride_blocks(n)), dim3(TPB), 0, s, (i64)n,
+        BBJD_SL(s, k_axpy_svdv_part, dim3(grid_stride_blocks(n)), dim3(TPB), 0, (i64)n,
                            (const Cx<double> *)rd->part, left, d, (const Cx<double> *)sv, (Cx<double> *)z);
     else
-        hipLaunchKernelGGL(k_axpy_svdv, dim3(grid_stride_blocks(n)), dim3(TPB), 0, s, (i64)n,
+        BBJD_SL(s, k_axpy_svdv, dim3(grid_stride_blocks(n)), dim3(TPB), 0, (i64)n,
                            (const Cx<double> *)d, (const Cx<double> *)sv, (Cx<double> *)z);
     return rc | chk();
 }
@@ -2082,15 +2197,108 @@ int bbjd_k_rotate(hipStream_t s, int64_t n, int k, int keep, void *X, int64_t ld
 
 int bbjd_k_z2c(hipStream_t s, int64_t n, const void *z, void *c)
 {
-    hipLaunchKernelGGL(k_z2c, dim3(grid_stride_blocks(n)), dim3(TPB), 0, s, (i64)n, (const Cx<double> *)z,
+    bbjd::gl<&k_z2c_b, TPB>(k_z2c, dim3(grid_stride_blocks(n)), 0, s, (i64)n, (const Cx<double> *)z,
                        (Cx<float> *)c);
     return chk();
 }
 
 int bbjd_k_c2z(hipStream_t s, int64_t n, const void *c, void *z)
 {
-    hipLaunchKernelGGL(k_c2z, dim3(grid_stride_blocks(n)), dim3(TPB), 0, s, (i64)n, (const Cx<float> *)c,
+    bbjd::gl<&k_c2z_b, TPB>(k_c2z, dim3(grid_stride_blocks(n)), 0, s, (i64)n, (const Cx<float> *)c,
                        (Cx<double> *)z);
+    return chk();
+}
+
+// ---------------------------------------------------------------------------
+// Recording and grouped launch (launch.h).  bbjd_rec_flush launches the jobs'
+// lists: closures as soon as they reach the head of their job's list, then
+// the groupable kernel that heads the most lists, for all of those jobs in
+// one launch, until every list is empty.  Each job's launches keep their
+// order (one stream), and jobs share no writable memory, so the interleaving
+// does not change any job's results.
+void *bbjd_rec_create(void) { return new bbjd::Recorder; }
+void bbjd_rec_destroy(void *rec) { delete static_cast<bbjd::Recorder *>(rec); }
+void bbjd_rec_begin(void *rec, int job)
+{
+    bbjd::Recorder *r = static_cast<bbjd::Recorder *>(rec);
+    if ((int)r->jobs.size() <= job) r->jobs.resize(job + 1);
+    r->cur = job;
+    bbjd::tl_rec() = r;
+}
+void bbjd_rec_end(void) { bbjd::tl_rec() = nullptr; }
+void bbjd_rec_counts(const void *rec, long *recorded, long *launches)
+{
+    const bbjd::Recorder *r = static_cast<const bbjd::Recorder *>(rec);
+    *recorded = r->recorded;
+    *launches = r->launches;
+}
+
+int bbjd_rec_flush(void *rec, hipStream_t s, long *launches)
+{
+    bbjd::Recorder *r = static_cast<bbjd::Recorder *>(rec);
+    const int nj = (int)r->jobs.size();
+    std::vector<size_t> h(nj, 0);
+    std::vector<bbjd::Op *> sel;
+    sel.reserve(bbjd::kGroupMax);
+    long nl = 0;
+    int rc = 0;
+    for (;;) {
+        bool any = false, ran = false;
+        for (int j = 0; j < nj; j++)
+            while (h[j] < r->jobs[j].size() && !r->jobs[j][h[j]].fn) {
+                r->jobs[j][h[j]].solo(s);
+                h[j]++;
+                nl++;
+                ran = true;
+            }
+        bbjd::GroupFn best = nullptr;
+        int bestc = 0;
+        for (int j = 0; j < nj; j++) {
+            if (h[j] >= r->jobs[j].size()) continue;
+            any = true;
+            const bbjd::GroupFn f = r->jobs[j][h[j]].fn;
+            int c = 0;
+            for (int i = 0; i < nj; i++) c += h[i] < r->jobs[i].size() && r->jobs[i][h[i]].fn == f;
+            if (c > bestc) { bestc = c; best = f; }
+        }
+        if (!any) break;
+        if (!best) { if (!ran) { rc = -1; break; } continue; }
+        sel.clear();
+        for (int j = 0; j < nj && (int)sel.size() < bbjd::kGroupMax; j++)
+            if (h[j] < r->jobs[j].size() && r->jobs[j][h[j]].fn == best) {
+                sel.push_back(&r->jobs[j][h[j]]);
+                h[j]++;
+            }
+        best(sel.data(), (int)sel.size(), s);
+        nl++;
+    }
+    for (auto &l : r->jobs) l.clear();
+    r->launches += nl;
+    if (launches) *launches = nl;
+    if (hipGetLastError() != hipSuccess) rc = -1;
+    return rc;
+}
+
+int bbjd_copy_async(void *dst, const void *src, size_t bytes, int kind, hipStream_t s)
+{
+    if (!bbjd::tl_rec()) return hipMemcpyAsync(dst, src, bytes, (hipMemcpyKind)kind, s) == hipSuccess ? 0 : -1;
+    bbjd::sl(s, [=](hipStream_t s_) { (void)hipMemcpyAsync(dst, src, bytes, (hipMemcpyKind)kind, s_); });
+    return 0;
+}
+
+int bbjd_memset_async(void *dst, int value, size_t bytes, hipStream_t s)
+{
+    if (!bbjd::tl_rec()) return hipMemsetAsync(dst, value, bytes, s) == hipSuccess ? 0 : -1;
+    bbjd::sl(s, [=](hipStream_t s_) { (void)hipMemsetAsync(dst, value, bytes, s_); });
+    return 0;
+}
+
+int bbjd_k_copy(hipStream_t s, void *dst, const void *src, size_t bytes)
+{
+    if (bytes % 8) return -2;
+    const i64 nw = (i64)(bytes / 8);
+    bbjd::gl<&k_copy_words_b, TPB>(k_copy_words, dim3((unsigned)((nw + TPB - 1) / TPB)), 0, s, nw,
+                                   (const unsigned long long *)src, (unsigned long long *)dst);
     return chk();
 }
 

@@ -14,6 +14,7 @@
 // Templated on the real type for the complex64 copies of the mixed path.
 #include <hip/hip_runtime.h>
 #include "ops_dev.h"
+#include "launch.h"
 
 typedef long long i64;
 template <typename R> struct alignas(2 * sizeof(R)) Cx { R re, im; };
@@ -42,10 +43,10 @@ template <typename R> __device__ __forceinline__ void cfms(Cx<R> &a, Cx<R> q, Cx
 static constexpr int TPB = 256;
 
 template <typename R>
-__global__ __launch_bounds__(TPB) void k_band_mv(i64 n1, i64 nb, int bs, int hw, const Cx<R> *__restrict__ band,
+__device__ __forceinline__ void k_band_mv_b(const VB vb, i64 n1, i64 nb, int bs, int hw, const Cx<R> *__restrict__ band,
                                                  const Cx<R> *__restrict__ x, Cx<R> *__restrict__ y)
 {
-    const i64 i = (i64)blockIdx.x * TPB + threadIdx.x;
+    const i64 i = (i64)vb.bx * TPB + threadIdx.x;
     if (i >= n1) return;
     const i64 bi = i / bs;
     const int r = (int)(i % bs), nd = 2 * hw + 1;
@@ -59,16 +60,28 @@ __global__ __launch_bounds__(TPB) void k_band_mv(i64 n1, i64 nb, int bs, int hw,
     }
     y[i] = acc;
 }
+template <typename R>
+__global__ __launch_bounds__(TPB) void k_band_mv(i64 n1, i64 nb, int bs, int hw, const Cx<R> *__restrict__ band,
+                                                 const Cx<R> *__restrict__ x, Cx<R> *__restrict__ y)
+{
+    k_band_mv_b<R>(VB_HW, n1, nb, bs, hw, band, x, y);
+}
 
 template <typename R>
-__global__ __launch_bounds__(TPB) void k_diag_mv(i64 n, const Cx<R> *__restrict__ dg, const Cx<R> *__restrict__ x,
+__device__ __forceinline__ void k_diag_mv_b(const VB vb, i64 n, const Cx<R> *__restrict__ dg, const Cx<R> *__restrict__ x,
                                                  Cx<R> *__restrict__ y)
 {
-    for (i64 i = (i64)blockIdx.x * TPB + threadIdx.x; i < n; i += (i64)gridDim.x * TPB) {
+    for (i64 i = (i64)vb.bx * TPB + threadIdx.x; i < n; i += (i64)vb.gx * TPB) {
         Cx<R> a = {0, 0};
         cfma(a, dg[i], x[i]);
         y[i] = a;
     }
+}
+template <typename R>
+__global__ __launch_bounds__(TPB) void k_diag_mv(i64 n, const Cx<R> *__restrict__ dg, const Cx<R> *__restrict__ x,
+                                                 Cx<R> *__restrict__ y)
+{
+    k_diag_mv_b<R>(VB_HW, n, dg, x, y);
 }
 
 // y_A = A x and y_B = B x in one pass over x (the block-banded rows i < n1,
@@ -112,7 +125,7 @@ static constexpr int AB_MAX_TAU = 16;
 // (2 HW + 1) BS band loads are all in flight at once; 0: runtime bs, hw.
 // The dense rows' values load four at a time.
 template <typename R, int BS, int HW>
-__global__ __launch_bounds__(TPB) void k_band_ab(i64 n, i64 n1, i64 nb, int bs_rt, int hw_rt,
+__device__ __forceinline__ void k_band_ab_b(const VB vb, i64 n, i64 n1, i64 nb, int bs_rt, int hw_rt,
                                                  const Cx<R> *__restrict__ band, const Cx<R> *__restrict__ dg,
                                                  int ntau, const Cx<R> *__restrict__ tauc,
                                                  const Cx<R> *__restrict__ x, Cx<R> *__restrict__ ya,
@@ -121,7 +134,7 @@ __global__ __launch_bounds__(TPB) void k_band_ab(i64 n, i64 n1, i64 nb, int bs_r
 {
     __shared__ Cx<R> red[AB_MAX_TAU][TPB / 64];
     __shared__ R rred[TPB / 64];
-    const i64 i = (i64)blockIdx.x * TPB + threadIdx.x;
+    const i64 i = (i64)vb.bx * TPB + threadIdx.x;
     Cx<R> xi = {0, 0};
     R rs = 0;
     if (i < n) {
@@ -191,21 +204,31 @@ __global__ __launch_bounds__(TPB) void k_band_ab(i64 n, i64 n1, i64 nb, int bs_r
         Cx<R> s = red[threadIdx.x][0];
 #pragma unroll
         for (int v = 1; v < TPB / 64; v++) { s.re += red[threadIdx.x][v].re; s.im += red[threadIdx.x][v].im; }
-        part[(i64)blockIdx.x * ntau + threadIdx.x] = s;
+        part[(i64)vb.bx * ntau + threadIdx.x] = s;
     }
     if (r && threadIdx.x == 0) {
         R s = rred[0];
 #pragma unroll
         for (int v = 1; v < TPB / 64; v++) s += rred[v];
-        rpart[blockIdx.x] = s;
+        rpart[vb.bx] = s;
     }
+}
+template <typename R, int BS, int HW>
+__global__ __launch_bounds__(TPB) void k_band_ab(i64 n, i64 n1, i64 nb, int bs_rt, int hw_rt,
+                                                 const Cx<R> *__restrict__ band, const Cx<R> *__restrict__ dg,
+                                                 int ntau, const Cx<R> *__restrict__ tauc,
+                                                 const Cx<R> *__restrict__ x, Cx<R> *__restrict__ ya,
+                                                 Cx<R> *__restrict__ yb, Cx<R> *__restrict__ part, Cx<R> ca,
+                                                 Cx<R> cb, Cx<R> *__restrict__ r, R *__restrict__ rpart)
+{
+    k_band_ab_b<R, BS, HW>(VB_HW, n, n1, nb, bs_rt, hw_rt, band, dg, ntau, tauc, x, ya, yb, part, ca, cb, r, rpart);
 }
 
 // y_A's dense rows: wave c sums column c of the nblk block partials in a
 // fixed order.  With r: wave ntau sums the band rows' |r|^2 partials, then
 // forms the dense rows' r = a y_A + b y_B and writes ||r|| to *rnorm.
 template <typename R>
-__global__ __launch_bounds__(64 * AB_MAX_TAU) void k_tau_finish(i64 nblk, int ntau, i64 n1,
+__device__ __forceinline__ void k_tau_finish_b(const VB vb, i64 nblk, int ntau, i64 n1,
                                                                       const Cx<R> *__restrict__ part,
                                                                       Cx<R> *__restrict__ ya,
                                                                       const Cx<R> *__restrict__ yb, Cx<R> ca,
@@ -238,12 +261,23 @@ __global__ __launch_bounds__(64 * AB_MAX_TAU) void k_tau_finish(i64 nblk, int nt
         *rnorm = sqrt((double)rs);
     }
 }
+template <typename R>
+__global__ __launch_bounds__(64 * AB_MAX_TAU) void k_tau_finish(i64 nblk, int ntau, i64 n1,
+                                                                      const Cx<R> *__restrict__ part,
+                                                                      Cx<R> *__restrict__ ya,
+                                                                      const Cx<R> *__restrict__ yb, Cx<R> ca,
+                                                                      Cx<R> cb, Cx<R> *__restrict__ r,
+                                                                      const R *__restrict__ rpart,
+                                                                      double *__restrict__ rnorm)
+{
+    k_tau_finish_b<R>(VB_HW, nblk, ntau, n1, part, ya, yb, ca, cb, r, rpart, rnorm);
+}
 
 template <typename R>
-__global__ __launch_bounds__(TPB) void k_bj_apply(i64 n, int bs, const Cx<R> *__restrict__ inv,
+__device__ __forceinline__ void k_bj_apply_b(const VB vb, i64 n, int bs, const Cx<R> *__restrict__ inv,
                                                   const Cx<R> *__restrict__ x, Cx<R> *__restrict__ y)
 {
-    const i64 i = (i64)blockIdx.x * TPB + threadIdx.x;
+    const i64 i = (i64)vb.bx * TPB + threadIdx.x;
     if (i >= n) return;
     const i64 bi = i / bs;
     const int r = (int)(i % bs);
@@ -252,6 +286,12 @@ __global__ __launch_bounds__(TPB) void k_bj_apply(i64 n, int bs, const Cx<R> *__
     Cx<R> a = {0, 0};
     for (int c = 0; c < bs; c++) cfma(a, D[(i64)c * bs], xs[c]);
     y[i] = a;
+}
+template <typename R>
+__global__ __launch_bounds__(TPB) void k_bj_apply(i64 n, int bs, const Cx<R> *__restrict__ inv,
+                                                  const Cx<R> *__restrict__ x, Cx<R> *__restrict__ y)
+{
+    k_bj_apply_b<R>(VB_HW, n, bs, inv, x, y);
 }
 
 // BCR on NR right-hand sides at once (vector q at b + q vs): the factor
@@ -625,7 +665,7 @@ __device__ __forceinline__ const Cx<double> *bcr_level_ptr(const BcrLevels &f, i
 }
 
 template <int NR, int S, int G>
-__global__ __launch_bounds__(TPB) void k_bcr_bwd_multi_pf(i64 N, int l, BcrLevels dpq, BcrLevels bl, Cx<double> *x,
+__device__ __forceinline__ void k_bcr_bwd_multi_pf_b(const VB vb, i64 N, int l, BcrLevels dpq, BcrLevels bl, Cx<double> *x,
                                                           i64 vs, const Cx<double> *__restrict__ tauc, int ntau,
                                                           i64 n1, i64 ldt, Cx<double> *__restrict__ dpart)
 {
@@ -634,7 +674,7 @@ __global__ __launch_bounds__(TPB) void k_bcr_bwd_multi_pf(i64 N, int l, BcrLevel
     constexpr i64 ss = (i64)S * S;
     __shared__ Cx<double> lx[NR][SLOTS][S];
     const int t = threadIdx.x;
-    const i64 u0 = (i64)blockIdx.x * U;
+    const i64 u0 = (i64)vb.bx * U;
     for (int e = t; e < (U + 1) * S * NR; e += TPB) {
         const int q = e / ((U + 1) * S), rem = e % ((U + 1) * S), u = rem / S, r = rem % S;
         const i64 i = ((u0 + u) << G) << l;
@@ -755,15 +795,22 @@ __global__ __launch_bounds__(TPB) void k_bcr_bwd_multi_pf(i64 N, int l, BcrLevel
             Cx<double> s = dred[tt][q][0];
 #pragma unroll
             for (int v = 1; v < TPB / 64; v++) { s.re += dred[tt][q][v].re; s.im += dred[tt][q][v].im; }
-            dpart[(i64)blockIdx.x * ntau * NR + t] = s;
+            dpart[(i64)vb.bx * ntau * NR + t] = s;
         }
     }
+}
+template <int NR, int S, int G>
+__global__ __launch_bounds__(TPB) void k_bcr_bwd_multi_pf(i64 N, int l, BcrLevels dpq, BcrLevels bl, Cx<double> *x,
+                                                          i64 vs, const Cx<double> *__restrict__ tauc, int ntau,
+                                                          i64 n1, i64 ldt, Cx<double> *__restrict__ dpart)
+{
+    k_bcr_bwd_multi_pf_b<NR, S, G>(VB_HW, N, l, dpq, bl, x, vs, tauc, ntau, n1, ldt, dpart);
 }
 
 // f0 (one right-hand side, instead of x0): the caller's vector in complex64,
 // widened exactly on load.
 template <int NR, int S, int G>
-__global__ __launch_bounds__(TPB) void k_bcr_fwd_multi_pf(i64 N, int l, i64 n1, BcrLevels ag,
+__device__ __forceinline__ void k_bcr_fwd_multi_pf_b(const VB vb, i64 N, int l, i64 n1, BcrLevels ag,
                                                           const Cx<double> *__restrict__ x0,
                                                           const Cx<double> *__restrict__ x1,
                                                           const Cx<double> *__restrict__ bin, Cx<double> *bout,
@@ -774,7 +821,7 @@ __global__ __launch_bounds__(TPB) void k_bcr_fwd_multi_pf(i64 N, int l, i64 n1, 
     constexpr i64 ss = (i64)S * S;
     __shared__ Cx<double> lb[NR][ROWS][S];
     const int t = threadIdx.x;
-    const i64 p0 = ((i64)blockIdx.x * U - 1) << G;
+    const i64 p0 = ((i64)vb.bx * U - 1) << G;
     const Cx<double> *xs[2] = {x0, x1};
     for (int e = t; e < ROWS * S * NR; e += TPB) {
         const int q = e / (ROWS * S), rem = e % (ROWS * S), sl = rem / S, r = rem % S;
@@ -864,6 +911,15 @@ __global__ __launch_bounds__(TPB) void k_bcr_fwd_multi_pf(i64 N, int l, i64 n1, 
         const i64 bi = (p0 + sl) << l;
         if (bi < N) bout[q * vs + bi * S + r] = lb[q][sl][r];
     }
+}
+template <int NR, int S, int G>
+__global__ __launch_bounds__(TPB) void k_bcr_fwd_multi_pf(i64 N, int l, i64 n1, BcrLevels ag,
+                                                          const Cx<double> *__restrict__ x0,
+                                                          const Cx<double> *__restrict__ x1,
+                                                          const Cx<double> *__restrict__ bin, Cx<double> *bout,
+                                                          i64 vs, const Cx<float> *__restrict__ f0)
+{
+    k_bcr_fwd_multi_pf_b<NR, S, G>(VB_HW, N, l, n1, ag, x0, x1, bin, bout, vs, f0);
 }
 
 // The small levels l0..L-1, the final s x s solve and the backward levels
@@ -1066,7 +1122,7 @@ __global__ __launch_bounds__(1024) void k_bcr_tail_lds(i64 N, int l0, int L, con
 // order.  The host launches it with 256 threads when the steps' rows fit
 // (S (sum of M_l + 1) <= 256).
 template <int NR, int S>
-__global__ __launch_bounds__(256) void k_bcr_tail_pf(i64 N, int l0, int L, const i64 *__restrict__ off,
+__device__ __forceinline__ void k_bcr_tail_pf_b(const VB vb, i64 N, int l0, int L, const i64 *__restrict__ off,
                                                      const Cx<double> *__restrict__ F,
                                                      const Cx<double> *__restrict__ b, Cx<double> *x, i64 vs)
 {
@@ -1196,6 +1252,13 @@ __global__ __launch_bounds__(256) void k_bcr_tail_pf(i64 N, int l0, int L, const
         x[q * vs + c * stride0 * S + rr] = lb[e];
     }
 }
+template <int NR, int S>
+__global__ __launch_bounds__(256) void k_bcr_tail_pf(i64 N, int l0, int L, const i64 *__restrict__ off,
+                                                     const Cx<double> *__restrict__ F,
+                                                     const Cx<double> *__restrict__ b, Cx<double> *x, i64 vs)
+{
+    k_bcr_tail_pf_b<NR, S>(VB_HW, N, l0, L, off, F, b, x, vs);
+}
 
 // border, vector q: t = b2_q - dots_q; y2_q = Sinv t -> y2out + q ntau and the
 // tail of y_q.  One workgroup, NR * ntau threads used.  With dpart the dots
@@ -1204,7 +1267,7 @@ __global__ __launch_bounds__(256) void k_bcr_tail_pf(i64 N, int l0, int L, const
 // b2f / ytf (one right-hand side): the dense rows' right-hand side read and
 // the tail of y written in complex64 instead of b2_0 / ytail0.
 template <int NR>
-__global__ void k_border_small(int ntau, const Cx<double> *b2_0, const Cx<double> *b2_1, const Cx<double> *dots0,
+__device__ __forceinline__ void k_border_small_b(const VB vb, int ntau, const Cx<double> *b2_0, const Cx<double> *b2_1, const Cx<double> *dots0,
                                const Cx<double> *dots1, const Cx<double> *__restrict__ Sinv, Cx<double> *y2out,
                                Cx<double> *ytail0, Cx<double> *ytail1, const Cx<double> *__restrict__ dpart,
                                i64 nblk, const Cx<float> *b2f, Cx<float> *ytf)
@@ -1245,16 +1308,24 @@ __global__ void k_border_small(int ntau, const Cx<double> *b2_0, const Cx<double
         else (q == 0 ? ytail0 : ytail1)[a] = acc;
     }
 }
+template <int NR>
+__global__ void k_border_small(int ntau, const Cx<double> *b2_0, const Cx<double> *b2_1, const Cx<double> *dots0,
+                               const Cx<double> *dots1, const Cx<double> *__restrict__ Sinv, Cx<double> *y2out,
+                               Cx<double> *ytail0, Cx<double> *ytail1, const Cx<double> *__restrict__ dpart,
+                               i64 nblk, const Cx<float> *b2f, Cx<float> *ytf)
+{
+    k_border_small_b<NR>(VB_HW, ntau, b2_0, b2_1, dots0, dots1, Sinv, y2out, ytail0, ytail1, dpart, nblk, b2f, ytf);
+}
 
 // y_q[r] = xb_q[r] - sum_a Y[a n1 + r] y2_q[a], r < n1; with y0f (one
 // right-hand side) y_0 is written in complex64 there instead.
 template <int NR>
-__global__ __launch_bounds__(TPB) void k_border_update(i64 n1, int ntau, const Cx<double> *__restrict__ xb, i64 vs,
+__device__ __forceinline__ void k_border_update_b(const VB vb, i64 n1, int ntau, const Cx<double> *__restrict__ xb, i64 vs,
                                                        const Cx<double> *__restrict__ Y,
                                                        const Cx<double> *__restrict__ y2, Cx<double> *y0,
                                                        Cx<double> *y1, Cx<float> *y0f)
 {
-    for (i64 i = (i64)blockIdx.x * TPB + threadIdx.x; i < n1; i += (i64)gridDim.x * TPB) {
+    for (i64 i = (i64)vb.bx * TPB + threadIdx.x; i < n1; i += (i64)vb.gx * TPB) {
         Cx<double> acc[NR];
 #pragma unroll
         for (int q = 0; q < NR; q++) acc[q] = xb[q * vs + i];
@@ -1268,6 +1339,14 @@ __global__ __launch_bounds__(TPB) void k_border_update(i64 n1, int ntau, const C
         if (NR == 2) y1[i] = acc[NR - 1];
     }
 }
+template <int NR>
+__global__ __launch_bounds__(TPB) void k_border_update(i64 n1, int ntau, const Cx<double> *__restrict__ xb, i64 vs,
+                                                       const Cx<double> *__restrict__ Y,
+                                                       const Cx<double> *__restrict__ y2, Cx<double> *y0,
+                                                       Cx<double> *y1, Cx<float> *y0f)
+{
+    k_border_update_b<NR>(VB_HW, n1, ntau, xb, vs, Y, y2, y0, y1, y0f);
+}
 
 static inline int blocks_for(i64 n) { i64 b = (n + TPB - 1) / TPB; return (int)(b < 1 ? 1 : b); }
 static inline int gs_blocks(i64 n) { i64 b = (n + TPB - 1) / TPB; if (b > 8192) b = 8192; return (int)(b < 1 ? 1 : b); }
@@ -1280,10 +1359,10 @@ int bbjd_o_band_mv(hipStream_t s, int64_t n1, int64_t nb, int bs, int hw, const 
 {
     if (n1 <= 0) return 0;
     if (single)
-        hipLaunchKernelGGL((k_band_mv<float>), dim3(blocks_for(n1)), dim3(TPB), 0, s, (i64)n1, (i64)nb, bs, hw,
+        bbjd::gl<&k_band_mv_b<float>, TPB>(k_band_mv<float>, dim3(blocks_for(n1)), 0, s, (i64)n1, (i64)nb, bs, hw,
                            (const Cx<float> *)band, (const Cx<float> *)x, (Cx<float> *)y);
     else
-        hipLaunchKernelGGL((k_band_mv<double>), dim3(blocks_for(n1)), dim3(TPB), 0, s, (i64)n1, (i64)nb, bs, hw,
+        bbjd::gl<&k_band_mv_b<double>, TPB>(k_band_mv<double>, dim3(blocks_for(n1)), 0, s, (i64)n1, (i64)nb, bs, hw,
                            (const Cx<double> *)band, (const Cx<double> *)x, (Cx<double> *)y);
     return chk();
 }
@@ -1308,7 +1387,7 @@ int bbjd_o_band_abr(hipStream_t s, int64_t n, int64_t n1, int64_t nb, int bs, in
     if (res && (single || ntau >= AB_MAX_TAU || !rpart || !rnorm || nblk > rpart_cap)) return -3;
     const int fin_threads = 64 * (ntau + res);
 #define BBJD_AB_K(R, B, H)                                                                                      \
-    hipLaunchKernelGGL((k_band_ab<R, B, H>), dim3(nblk), dim3(TPB), 0, s, (i64)n, (i64)n1, (i64)nb, bs, hw,       \
+    bbjd::gl<&k_band_ab_b<R, B, H>, TPB>(k_band_ab<R, B, H>, dim3(nblk), 0, s, (i64)n, (i64)n1, (i64)nb, bs, hw,       \
                        (const Cx<R> *)band, (const Cx<R> *)dg, ntau, (const Cx<R> *)tauc, (const Cx<R> *)x,       \
                        (Cx<R> *)ya, (Cx<R> *)yb, (Cx<R> *)part, Cx<R>{(R)are, (R)aim}, Cx<R>{(R)bre, (R)bim},     \
                        (Cx<R> *)r, (R *)rpart)
@@ -1318,7 +1397,7 @@ int bbjd_o_band_abr(hipStream_t s, int64_t n, int64_t n1, int64_t nb, int bs, in
     else if (bs == 8 && hw == 2) BBJD_AB_K(R, 8, 2);                                                            \
     else BBJD_AB_K(R, 0, 0);                                                                                    \
     if (fin_threads > 0)                                                                                        \
-        hipLaunchKernelGGL((k_tau_finish<R>), dim3(1), dim3(fin_threads), 0, s, (i64)nblk, ntau, (i64)n1,        \
+        BBJD_SL(s, (k_tau_finish<R>), dim3(1), dim3(fin_threads), 0, (i64)nblk, ntau, (i64)n1,        \
                            (const Cx<R> *)part, (Cx<R> *)ya, (const Cx<R> *)yb, Cx<R>{(R)are, (R)aim},           \
                            Cx<R>{(R)bre, (R)bim}, (Cx<R> *)r, (const R *)rpart, rnorm);
     if (single) { BBJD_AB(float) } else { BBJD_AB(double) }
@@ -1330,10 +1409,10 @@ int bbjd_o_band_abr(hipStream_t s, int64_t n, int64_t n1, int64_t nb, int bs, in
 int bbjd_o_diag_mv(hipStream_t s, int64_t n, const void *dg, const void *x, void *y, int single)
 {
     if (single)
-        hipLaunchKernelGGL((k_diag_mv<float>), dim3(gs_blocks(n)), dim3(TPB), 0, s, (i64)n, (const Cx<float> *)dg,
+        bbjd::gl<&k_diag_mv_b<float>, TPB>(k_diag_mv<float>, dim3(gs_blocks(n)), 0, s, (i64)n, (const Cx<float> *)dg,
                            (const Cx<float> *)x, (Cx<float> *)y);
     else
-        hipLaunchKernelGGL((k_diag_mv<double>), dim3(gs_blocks(n)), dim3(TPB), 0, s, (i64)n, (const Cx<double> *)dg,
+        bbjd::gl<&k_diag_mv_b<double>, TPB>(k_diag_mv<double>, dim3(gs_blocks(n)), 0, s, (i64)n, (const Cx<double> *)dg,
                            (const Cx<double> *)x, (Cx<double> *)y);
     return chk();
 }
@@ -1341,10 +1420,10 @@ int bbjd_o_diag_mv(hipStream_t s, int64_t n, const void *dg, const void *x, void
 int bbjd_o_bj_apply(hipStream_t s, int64_t n, int bs, const void *inv, const void *x, void *y, int single)
 {
     if (single)
-        hipLaunchKernelGGL((k_bj_apply<float>), dim3(blocks_for(n)), dim3(TPB), 0, s, (i64)n, bs,
+        bbjd::gl<&k_bj_apply_b<float>, TPB>(k_bj_apply<float>, dim3(blocks_for(n)), 0, s, (i64)n, bs,
                            (const Cx<float> *)inv, (const Cx<float> *)x, (Cx<float> *)y);
     else
-        hipLaunchKernelGGL((k_bj_apply<double>), dim3(blocks_for(n)), dim3(TPB), 0, s, (i64)n, bs,
+        bbjd::gl<&k_bj_apply_b<double>, TPB>(k_bj_apply<double>, dim3(blocks_for(n)), 0, s, (i64)n, bs,
                            (const Cx<double> *)inv, (const Cx<double> *)x, (Cx<double> *)y);
     return chk();
 }
@@ -1373,7 +1452,7 @@ int bbjd_o_bcr_fwd(hipStream_t st, int s, int64_t M, int64_t h, int64_t ne, cons
 {
     if (ne <= 0) return 0;
 #define BBJD_FWD(NR, S)                                                                                         \
-    hipLaunchKernelGGL((k_bcr_fwd<NR, S>), dim3(blocks_for(ne * s)), dim3(TPB), 0, st, s, (i64)M, (i64)h, (i64)ne, \
+    BBJD_SL(st, (k_bcr_fwd<NR, S>), dim3(blocks_for(ne * s)), dim3(TPB), 0, s, (i64)M, (i64)h, (i64)ne, \
                        (const Cx<double> *)AG, (Cx<double> *)b, (i64)vs)
     BBJD_BCR_DISPATCH(BBJD_FWD)
 #undef BBJD_FWD
@@ -1386,7 +1465,7 @@ int bbjd_o_bcr_fwd0(hipStream_t st, int s, int64_t N, int64_t n1, const void *AG
     if (N <= 0) return 0;
     if (s > 64 || (nrhs == 2 && !x1)) return -2;
 #define BBJD_FWD0(NR, S)                                                                                        \
-    hipLaunchKernelGGL((k_bcr_fwd0<NR, S>), dim3(blocks_for(N * s)), dim3(TPB), 0, st, s, (i64)N, (i64)n1,        \
+    BBJD_SL(st, (k_bcr_fwd0<NR, S>), dim3(blocks_for(N * s)), dim3(TPB), 0, s, (i64)N, (i64)n1,        \
                        (const Cx<double> *)AG, (const Cx<double> *)x0, (const Cx<double> *)(x1 ? x1 : x0),       \
                        (Cx<double> *)b, (i64)vs)
     BBJD_BCR_DISPATCH(BBJD_FWD0)
@@ -1399,7 +1478,7 @@ int bbjd_o_bcr_bwd(hipStream_t st, int s, int64_t M, int64_t h, int64_t no, cons
 {
     if (no <= 0) return 0;
 #define BBJD_BWD(NR, S)                                                                                         \
-    hipLaunchKernelGGL((k_bcr_bwd<NR, S>), dim3(blocks_for(no * s)), dim3(TPB), 0, st, s, (i64)M, (i64)h, (i64)no, \
+    BBJD_SL(st, (k_bcr_bwd<NR, S>), dim3(blocks_for(no * s)), dim3(TPB), 0, s, (i64)M, (i64)h, (i64)no, \
                        (const Cx<double> *)DPQ, (const Cx<double> *)b, (Cx<double> *)x, (i64)vs)
     BBJD_BCR_DISPATCH(BBJD_BWD)
 #undef BBJD_BWD
@@ -1451,14 +1530,14 @@ int bbjd_o_bcr_bwd_multi(hipStream_t st, int s, int64_t N, int l, int G, const v
         const Cx<double> *tc = dd ? (const Cx<double> *)tauc : nullptr;                                        \
         if constexpr (S <= 8) {                                                                                \
             if (g_bcr_prefetch) {                                                                              \
-                hipLaunchKernelGGL((k_bcr_bwd_multi_pf<NR, S, GG>), dim3((unsigned)nblk), dim3(TPB), 0, st,    \
+                bbjd::gl<&k_bcr_bwd_multi_pf_b<NR, S, GG>, TPB>(k_bcr_bwd_multi_pf<NR, S, GG>, dim3((unsigned)nblk), 0, st,    \
                                    (i64)N, l, f, bb, (Cx<double> *)x, (i64)vs, tc, ntau, (i64)n1, (i64)ldt,    \
                                    (Cx<double> *)dpart);                                                       \
                 if (dd && nblk_out) *nblk_out = nblk;                                                          \
                 break;                                                                                         \
             }                                                                                                  \
         }                                                                                                      \
-        hipLaunchKernelGGL((k_bcr_bwd_multi<NR, S, GG>), dim3((unsigned)nblk), dim3(TPB), 0, st, (i64)N, l, f, bb, \
+        BBJD_SL(st, (k_bcr_bwd_multi<NR, S, GG>), dim3((unsigned)nblk), dim3(TPB), 0, (i64)N, l, f, bb, \
                            (Cx<double> *)x, (i64)vs, tc, ntau, (i64)n1, (i64)ldt, (Cx<double> *)dpart);         \
         if (dd && nblk_out) *nblk_out = nblk;                                                                  \
     }
@@ -1488,13 +1567,13 @@ int bbjd_o_bcr_fwd_multi(hipStream_t st, int s, int64_t N, int64_t n1, int l, in
             if (g_bcr_prefetch) {                                                                              \
                 const Cx<float> *f0 = in_c64 ? (const Cx<float> *)x0 : nullptr;                                \
                 if (in_c64) a0 = a1 = nullptr;                                                                 \
-                hipLaunchKernelGGL((k_bcr_fwd_multi_pf<NR, S, GG>), dim3(nb), dim3(TPB), 0, st, (i64)N, l,     \
+                bbjd::gl<&k_bcr_fwd_multi_pf_b<NR, S, GG>, TPB>(k_bcr_fwd_multi_pf<NR, S, GG>, dim3(nb), 0, st, (i64)N, l,     \
                                    (i64)n1, f, a0, a1, (const Cx<double> *)bin, (Cx<double> *)bout, (i64)vs,   \
                                    f0);                                                                        \
                 break;                                                                                         \
             }                                                                                                  \
         }                                                                                                      \
-        hipLaunchKernelGGL((k_bcr_fwd_multi<NR, S, GG>), dim3(nb), dim3(TPB), 0, st, (i64)N, l, (i64)n1, f, a0, a1, \
+        BBJD_SL(st, (k_bcr_fwd_multi<NR, S, GG>), dim3(nb), dim3(TPB), 0, (i64)N, l, (i64)n1, f, a0, a1, \
                            (const Cx<double> *)bin, (Cx<double> *)bout, (i64)vs);                              \
     }
     BBJD_BCR_GROUP_DISPATCH(BBJD_FWDM)
@@ -1533,7 +1612,7 @@ int bbjd_o_bcr_tail(hipStream_t st, int s, int64_t N, int l0, int L, const int64
         for (int l = l0; l < L; l++) rows += (N + ((i64)1 << l) - 1) >> l;
         if (g_bcr_prefetch && rows * s <= 256) {
 #define BBJD_TAIL_PF(NR, SS)                                                                                   \
-    hipLaunchKernelGGL((k_bcr_tail_pf<NR, SS>), dim3(1), dim3(256), lds, st, (i64)N, l0, L, (const i64 *)off,   \
+    bbjd::gl<&k_bcr_tail_pf_b<NR, SS>, 256>(k_bcr_tail_pf<NR, SS>, dim3(1), lds, st, (i64)N, l0, L, (const i64 *)off,   \
                        (const Cx<double> *)F, (const Cx<double> *)b, (Cx<double> *)x, (i64)vs)
             if (nrhs == 2) {
                 if (s == 4) BBJD_TAIL_PF(2, 4);
@@ -1546,7 +1625,7 @@ int bbjd_o_bcr_tail(hipStream_t st, int s, int64_t N, int l0, int L, const int64
             return chk();
         }
 #define BBJD_TAIL_LDS(NR, SS)                                                                                  \
-    hipLaunchKernelGGL((k_bcr_tail_lds<NR, SS>), dim3(1), dim3(1024), lds, st, (i64)N, l0, L, (const i64 *)off, \
+    BBJD_SL(st, (k_bcr_tail_lds<NR, SS>), dim3(1), dim3(1024), lds, (i64)N, l0, L, (const i64 *)off, \
                        (const Cx<double> *)F, (const Cx<double> *)b, (Cx<double> *)x, (i64)vs)
         if (nrhs == 2) {
             if (s == 4) BBJD_TAIL_LDS(2, 4);
@@ -1559,7 +1638,7 @@ int bbjd_o_bcr_tail(hipStream_t st, int s, int64_t N, int l0, int L, const int64
         return chk();
     }
 #define BBJD_TAIL(NR, S)                                                                                        \
-    hipLaunchKernelGGL((k_bcr_tail<NR, S>), dim3(1), dim3(1024), 0, st, s, (i64)N, l0, L, (const i64 *)off,       \
+    BBJD_SL(st, (k_bcr_tail<NR, S>), dim3(1), dim3(1024), 0, s, (i64)N, l0, L, (const i64 *)off,       \
                        (const Cx<double> *)F, (Cx<double> *)b, (Cx<double> *)x, (i64)vs)
     BBJD_BCR_DISPATCH(BBJD_TAIL)
 #undef BBJD_TAIL
@@ -1573,13 +1652,13 @@ int bbjd_o_border_small(hipStream_t st, int ntau, const void *b2_0, const void *
     if (ntau <= 0 || ntau * nrhs > 1024 || (dpart && ntau > 16) || (c64 && nrhs != 1)) return -2;
     const int thr = dpart ? 1024 : ntau * nrhs <= 64 ? 64 : (ntau * nrhs <= 256 ? 256 : 1024);
     if (nrhs == 2)
-        hipLaunchKernelGGL(k_border_small<2>, dim3(1), dim3(thr), (size_t)2 * ntau * 16, st, ntau,
+        BBJD_SL(st, k_border_small<2>, dim3(1), dim3(thr), (size_t)2 * ntau * 16, ntau,
                            (const Cx<double> *)b2_0, (const Cx<double> *)b2_1, (const Cx<double> *)dots0,
                            (const Cx<double> *)dots1, (const Cx<double> *)Sinv, (Cx<double> *)y2,
                            (Cx<double> *)ytail0, (Cx<double> *)ytail1, (const Cx<double> *)dpart, (i64)nblk,
                            (const Cx<float> *)nullptr, (Cx<float> *)nullptr);
     else
-        hipLaunchKernelGGL(k_border_small<1>, dim3(1), dim3(thr), (size_t)ntau * 16, st, ntau,
+        BBJD_SL(st, k_border_small<1>, dim3(1), dim3(thr), (size_t)ntau * 16, ntau,
                            c64 ? nullptr : (const Cx<double> *)b2_0, c64 ? nullptr : (const Cx<double> *)b2_0,
                            (const Cx<double> *)dots0, (const Cx<double> *)dots0, (const Cx<double> *)Sinv,
                            (Cx<double> *)y2, c64 ? nullptr : (Cx<double> *)ytail0,
@@ -1593,11 +1672,11 @@ int bbjd_o_border_update(hipStream_t st, int64_t n1, int ntau, const void *xb, i
 {
     if (c64 && nrhs != 1) return -2;
     if (nrhs == 2)
-        hipLaunchKernelGGL(k_border_update<2>, dim3(gs_blocks(n1)), dim3(TPB), 0, st, (i64)n1, ntau,
+        bbjd::gl<&k_border_update_b<2>, TPB>(k_border_update<2>, dim3(gs_blocks(n1)), 0, st, (i64)n1, ntau,
                            (const Cx<double> *)xb, (i64)vs, (const Cx<double> *)Y, (const Cx<double> *)y2,
                            (Cx<double> *)y0, (Cx<double> *)y1, (Cx<float> *)nullptr);
     else
-        hipLaunchKernelGGL(k_border_update<1>, dim3(gs_blocks(n1)), dim3(TPB), 0, st, (i64)n1, ntau,
+        bbjd::gl<&k_border_update_b<1>, TPB>(k_border_update<1>, dim3(gs_blocks(n1)), 0, st, (i64)n1, ntau,
                            (const Cx<double> *)xb, (i64)vs, (const Cx<double> *)Y, (const Cx<double> *)y2,
                            c64 ? nullptr : (Cx<double> *)y0, c64 ? nullptr : (Cx<double> *)y0,
                            c64 ? (Cx<float> *)y0 : nullptr);

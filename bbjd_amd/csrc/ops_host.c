@@ -315,9 +315,9 @@ static int bcr_solve(struct bbjd_op *op, hipStream_t st, int nrhs, const void *x
         }
         for (int k = 0; k < l; k++) bl[k] = 0;
     } else if (x0) {
-        rc |= !HIPOK(hipMemcpyAsync(op->bvec, x0, (size_t)op->n1 * 16, hipMemcpyDeviceToDevice, st));
+        rc |= bbjd_copy_async(op->bvec, x0, (size_t)op->n1 * 16, hipMemcpyDeviceToDevice, st) != 0;
         if (nrhs == 2)
-            rc |= !HIPOK(hipMemcpyAsync((zcx *)op->bvec + vs, x1, (size_t)op->n1 * 16, hipMemcpyDeviceToDevice, st));
+            rc |= bbjd_copy_async((zcx *)op->bvec + vs, x1, (size_t)op->n1 * 16, hipMemcpyDeviceToDevice, st) != 0;
     }
     while (l < lt) {
         const int G = lt - l < g_bcr_group ? lt - l : g_bcr_group;
@@ -383,9 +383,9 @@ static int exact_apply(struct bbjd_op *op, hipStream_t st, int nrhs, const void 
         rc |= e;
     }
     if (op->ntau == 0) {
-        rc |= !HIPOK(hipMemcpyAsync(y0, op->xvec, (size_t)n1 * 16, hipMemcpyDeviceToDevice, st));
+        rc |= bbjd_copy_async(y0, op->xvec, (size_t)n1 * 16, hipMemcpyDeviceToDevice, st) != 0;
         if (nrhs == 2)
-            rc |= !HIPOK(hipMemcpyAsync(y1, (zcx *)op->xvec + vs, (size_t)n1 * 16, hipMemcpyDeviceToDevice, st));
+            rc |= bbjd_copy_async(y1, (zcx *)op->xvec + vs, (size_t)n1 * 16, hipMemcpyDeviceToDevice, st) != 0;
         return rc;
     }
     const int nt = op->ntau;
@@ -705,7 +705,7 @@ int bbjd_op_precond(bbjd_int n, const bbjd_zomplex *x, bbjd_zomplex *y, void *da
     switch (op->pc) {
     case BBJD_PC_EXACT_LU: return exact_apply(op, st, 1, x, y, NULL, NULL, 0) ? -1 : 0;
     case BBJD_PC_BLOCK_JACOBI: return bbjd_o_bj_apply(st, n, op->bs, op->bj, x, y, 0) ? -1 : 0;
-    default: return HIPOK(hipMemcpyAsync(y, x, (size_t)n * 16, hipMemcpyDeviceToDevice, st)) ? 0 : -1;
+    default: return bbjd_copy_async(y, x, (size_t)n * 16, hipMemcpyDeviceToDevice, st) ? -1 : 0;
     }
 }
 
@@ -768,6 +768,6 @@ int bbjd_op_precond_c64(bbjd_int n, const bbjd_cfloat *x, bbjd_cfloat *y, void *
         rc |= bbjd_k_z2c(st, n, op->zt2, y);
         return rc ? -1 : 0;
     }
-    default: return HIPOK(hipMemcpyAsync(y, x, (size_t)n * 8, hipMemcpyDeviceToDevice, st)) ? 0 : -1;
+    default: return bbjd_copy_async(y, x, (size_t)n * 8, hipMemcpyDeviceToDevice, st) ? -1 : 0;
     }
 }

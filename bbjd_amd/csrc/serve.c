@@ -63,6 +63,7 @@ struct bbjd_server {
     int              *wave_left;
     int64_t          *wave_pairs;
     int               wave_min, nwaves, progress;
+    bbjd_batcher     *batcher;          /* grouped inner solves, or NULL     */
 };
 
 static double now_s(void)
@@ -95,6 +96,7 @@ void bbjd_serve_config_init(bbjd_serve_config *c)
     c->sync_yield = 1;
     c->order = BBJD_ORDER_ARRIVAL;
     c->window = 8;
+    c->inner_window_us = 100;
 }
 
 static bbjd_int req_n(const bbjd_request *q) { return q->op ? bbjd_op_size(q->op) : (q->pencil ? q->pencil->n : 0); }
@@ -234,6 +236,7 @@ void bbjd_server_destroy(bbjd_server *S)
             if (S->w[i].started) pthread_join(S->w[i].th, NULL);
         free(S->w);
     }
+    bbjd_batcher_destroy(S->batcher);
     pthread_mutex_destroy(&S->mu);
     pthread_cond_destroy(&S->cv_work);
     pthread_cond_destroy(&S->cv_free);
@@ -281,6 +284,12 @@ bbjd_server *bbjd_server_create(const bbjd_request *reqs, int nreq, const bbjd_s
         S->opt.matvec_B_c64 = bbjd_op_matvec_B_c64;
         S->opt.precond_c64 = bbjd_op_precond_c64;
         S->opt.matvec_AB_c64 = bbjd_op_matvec_AB_c64;
+        if (cfg->inner_batch > 0 && S->cap.inner_maxiter > 0) {
+            S->batcher = bbjd_batcher_create(cfg->inner_batchers > 0 ? cfg->inner_batchers : 2, cfg->inner_batch,
+                                             cfg->inner_window_us, S->cap.inner_maxiter);
+            if (!S->batcher) { bbjd_server_destroy(S); return NULL; }
+            S->opt.inner_batcher = S->batcher;
+        }
     }
     S->wsb = bbjd_workspace_bytes(&S->cap, &S->opt);
     size_t freeb = 0, totb = 0;
@@ -314,6 +323,12 @@ bbjd_server *bbjd_server_create(const bbjd_request *reqs, int nreq, const bbjd_s
     pthread_mutex_unlock(&S->mu);
     if (failed_init == nw) { bbjd_server_destroy(S); return NULL; }
     return S;
+}
+
+void bbjd_server_batch_stats(const bbjd_server *S, bbjd_batcher_stats *st)
+{
+    memset(st, 0, sizeof(*st));
+    if (S && S->batcher) bbjd_batcher_get_stats(S->batcher, st);
 }
 
 int bbjd_server_workers(const bbjd_server *S)

@@ -38,7 +38,13 @@ class ServeConfig(C.Structure):
     _fields_ = [("workers", C.c_int), ("hbm_budget", C.c_size_t), ("precision", C.c_int),
                 ("realtime", C.c_int), ("sync_yield", C.c_int), ("graphs", C.c_int), ("order", C.c_int),
                 ("pin", C.c_int), ("window", C.c_int), ("qz_mode", C.c_int), ("inner_form", C.c_int),
-                ("progress", C.c_int)]
+                ("progress", C.c_int), ("inner_batch", C.c_int), ("inner_batchers", C.c_int),
+                ("inner_window_us", C.c_double)]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("batches", C.c_long), ("jobs", C.c_long), ("steps", C.c_long), ("recorded", C.c_long),
+                ("launches", C.c_long)]
 
 
 class ServeReport(C.Structure):
@@ -71,6 +77,7 @@ def _setup(lib):
     lib.bbjd_server_workers.argtypes = [C.c_void_p]
     lib.bbjd_server_workspace_bytes.restype = C.c_size_t
     lib.bbjd_server_workspace_bytes.argtypes = [C.c_void_p]
+    lib.bbjd_server_batch_stats.argtypes = [C.c_void_p, C.POINTER(BatchStats)]
     lib.bbjd_device_free_bytes.restype = C.c_size_t
     lib.bbjd_device_free_bytes.argtypes = []
     lib._serve_ready = True
@@ -130,7 +137,8 @@ def _requests(jobs, keep_vectors):
 
 
 def _config(lib, workers=16, precision="fp64", realtime=True, hbm_budget=0, sync_yield=True, graphs=False,
-            order="arrival", pin=False, window=8, qz_mode="ritz", inner_form="reference", progress=False):
+            order="arrival", pin=False, window=8, qz_mode="ritz", inner_form="reference", progress=False,
+            inner_batch=0, inner_batchers=2, inner_window_us=100.0):
     if order not in ("arrival", "largest", "window"):
         raise ValueError("order must be 'arrival', 'largest' or 'window'")
     if int(window) < 1:
@@ -155,6 +163,11 @@ def _config(lib, workers=16, precision="fp64", realtime=True, hbm_budget=0, sync
         raise ValueError("inner_form must be 'reference' or 'projected'")
     cfg.inner_form = forms[inner_form]
     cfg.progress = int(bool(progress))
+    if not 0 <= int(inner_batch) <= 16 or not 1 <= int(inner_batchers) <= 8 or inner_window_us < 0:
+        raise ValueError("inner_batch must be in [0, 16], inner_batchers in [1, 8], inner_window_us >= 0")
+    cfg.inner_batch = int(inner_batch)
+    cfg.inner_batchers = int(inner_batchers)
+    cfg.inner_window_us = float(inner_window_us)
     return cfg
 
 
@@ -178,7 +191,7 @@ def _results(reqs, outs, keep_vectors):
 
 def serve(jobs, workers=16, precision="fp64", realtime=True, hbm_budget=0, keep_vectors=False,
           sync_yield=True, graphs=False, order="arrival", pin=False, window=8, qz_mode="ritz",
-          inner_form="reference"):
+          inner_form="reference", inner_batch=0, inner_batchers=2, inner_window_us=100.0):
     """Run jobs through the serving loop; returns (results, report).  graphs:
     capture each request's inner GMRES steps into HIP graphs and replay them.
     order: which arrived request a free worker takes, "arrival" (first come,
@@ -189,11 +202,15 @@ def serve(jobs, workers=16, precision="fp64", realtime=True, hbm_budget=0, keep_
     "device": on the GPU for subspaces up to 64).  inner_form: the mixed
     inner solve's form ("reference": the reference's GMRES with two-sided
     projections; "projected": the projected preconditioner, one oblique
-    projection per step)."""
+    projection per step).  inner_batch > 0 (mixed precision): the workers'
+    inner solves run in batches of up to inner_batch on inner_batchers
+    threads, each Arnoldi step's kernels of a batch merged into shared
+    launches; results are the same as without."""
     lib = _lib.load()
     _setup(lib)
     cfg = _config(lib, workers, precision, realtime, hbm_budget, sync_yield, graphs, order, pin, window, qz_mode,
-                  inner_form)
+                  inner_form, inner_batch=inner_batch, inner_batchers=inner_batchers,
+                  inner_window_us=inner_window_us)
     reqs, keep, outs = _requests(jobs, keep_vectors)
     rep = ServeReport()
     rc = lib.bbjd_serve(reqs, len(jobs), C.byref(cfg), C.byref(rep))
@@ -209,13 +226,16 @@ class Server:
     until close(), and each run() serves a batch of jobs on them."""
 
     def __init__(self, capacity_jobs, workers=16, precision="fp64", hbm_budget=0, sync_yield=True,
-                 graphs=False, pin=False, qz_mode="ritz", inner_form="reference"):
+                 graphs=False, pin=False, qz_mode="ritz", inner_form="reference", inner_batch=0,
+                 inner_batchers=2, inner_window_us=100.0):
         lib = _lib.load()
         _setup(lib)
         self._lib = lib
         self.precision = precision
         self._cfg_kw = dict(workers=workers, precision=precision, hbm_budget=hbm_budget, sync_yield=sync_yield,
-                            graphs=graphs, pin=pin, qz_mode=qz_mode, inner_form=inner_form)
+                            graphs=graphs, pin=pin, qz_mode=qz_mode, inner_form=inner_form,
+                            inner_batch=inner_batch, inner_batchers=inner_batchers,
+                            inner_window_us=inner_window_us)
         cfg = _config(lib, **self._cfg_kw)
         reqs, _keep, _outs = _requests(capacity_jobs, False)
         self._h = lib.bbjd_server_create(reqs, len(capacity_jobs), C.byref(cfg))
@@ -229,6 +249,13 @@ class Server:
     @property
     def workspace_bytes(self):
         return int(self._lib.bbjd_server_workspace_bytes(self._h))
+
+    def batch_stats(self):
+        """counters of the grouped inner solves since creation: batches,
+        jobs (inner solves), steps, recorded launches and launches made"""
+        st = BatchStats()
+        self._lib.bbjd_server_batch_stats(self._h, C.byref(st))
+        return {f: int(getattr(st, f)) for f, _ in BatchStats._fields_}
 
     def run(self, jobs, realtime=True, order="arrival", window=8, keep_vectors=False, progress=False):
         """serve a batch; progress: a line on stderr as each arrival wave's

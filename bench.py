@@ -340,6 +340,11 @@ def main():
     ap.add_argument("--pipeline", type=int, default=10,
                     help="timed steps per server run, each one arrival wave (1: every step drains before "
                          "the next starts arriving)")
+    ap.add_argument("--inner-batch", type=int, default=0,
+                    help="served leg, mixed inner solves: up to this many per grouped batch (0: off)")
+    ap.add_argument("--inner-batchers", type=int, default=2, help="batcher threads (streams) with --inner-batch")
+    ap.add_argument("--inner-window-us", type=float, default=100.0,
+                    help="how long a batcher waits for more inner solves (microseconds)")
     ap.add_argument("--seed", type=int, default=20261203)
     args = ap.parse_args()
 
@@ -413,6 +418,7 @@ def main():
 
     all_idx = list(range(len(reqs)))
     served, walls, run_steps = [], [], []
+    batch = None
     steps_timed = 0
     worst = bad = checked = worst_abs = None
     if not args.plain_only:
@@ -421,7 +427,8 @@ def main():
         server = srv.Server(jobs_for(all_idx), workers=args.workers, precision=wl["served"]["precision"],
                             inner_form=wl["served"]["inner_form"],
                             sync_yield="block" if args.sync == "block" else True, pin=args.pin,
-                            qz_mode=args.qz_mode)
+                            qz_mode=args.qz_mode, inner_batch=args.inner_batch,
+                            inner_batchers=args.inner_batchers, inner_window_us=args.inner_window_us)
         log(f"server: {server.workers} workers, {server.workspace_bytes / 1e9:.2f} GB workspace each, "
             f"set up in {time.perf_counter() - t:.1f}s; free HBM {srv.device_free_bytes() / 1e9:.1f} GB")
         # ---- warmup (untimed): the smallest requests ----
@@ -476,6 +483,10 @@ def main():
                 f"{rep['peak_bytes'] / 1e9:.1f} GB, failed {rep['failed']}; free HBM "
                 f"{free0 / 1e9:.1f} -> {srv.device_free_bytes() / 1e9:.1f} GB")
             s += c
+        batch = server.batch_stats() if args.inner_batch else None
+        if batch:
+            log(f"grouped inner solves: {batch['jobs']} in {batch['batches']} batches, {batch['steps']} steps, "
+                f"{batch['recorded']} launches recorded, {batch['launches']} made")
         server.close()
         if served and not args.no_verify:
             t = time.perf_counter()
@@ -692,6 +703,8 @@ def main():
         "plain_form": (f"{wl['plain']['precision']} inner GMRES({wl['plain']['inner']}), inner tol "
                        f"{wl['plain']['inner_tol']:g}" if wl["plain"]["inner"] else
                        f"{wl['plain']['precision']}, one-step correction"),
+        "inner_batching": (dict(max_jobs=args.inner_batch, batchers=args.inner_batchers,
+                                window_us=args.inner_window_us, **batch) if args.inner_batch and batch else None),
         "setup_seconds": round(t_setup, 1),
         "roofline": roof,
         "served_region": served_region,

@@ -109,6 +109,14 @@ typedef struct {
      * thread waiting on the GPU leaves its core to others (many workspaces
      * solving at once on a busy host); 0: the process's default wait. */
     int                 blocking_sync;
+    /* a bbjd_batcher shared by workspaces that solve at once (serving):
+     * their inner solves of BBJD_PREC_MIXED in the reference form, with the
+     * library's banded operators (bbjd_op_*_c64 as device callbacks), run
+     * together on the batcher's threads, each Arnoldi step's kernels of all
+     * the solves in the batch merged into shared launches; every solve's
+     * arithmetic is the one it does alone (same kernels, grids and order),
+     * so are its results.  Other inner solves, and NULL, run on their own. */
+    void               *inner_batcher;
 } bbjd_gpu_options;
 
 enum { BBJD_INNER_REFERENCE = 0, BBJD_INNER_PROJECTED = 1 };
@@ -120,6 +128,25 @@ enum { BBJD_INNER_REFERENCE = 0, BBJD_INNER_PROJECTED = 1 };
 enum { BBJD_QZ_RITZ = 0, BBJD_QZ_SCHUR = 1, BBJD_QZ_DEVICE = 2 };
 
 void bbjd_gpu_options_init(bbjd_gpu_options *o);
+
+/* Inner-solve batcher: `threads` threads, each with its own HIP stream, run
+ * batches of up to max_jobs (<= 16) inner solves submitted by workspaces
+ * that name it in bbjd_gpu_options.inner_batcher.  A thread takes the solves
+ * waiting when it is free, and when fewer than max_jobs are waiting while
+ * other solves of the batcher's workspaces are still running, waits up to
+ * window_us for more.  inner_maxiter: the largest of the workspaces'.  No
+ * counterpart in the reference (one solve per process there). */
+typedef struct bbjd_batcher bbjd_batcher;
+bbjd_batcher *bbjd_batcher_create(int threads, int max_jobs, double window_us, bbjd_int inner_maxiter);
+void          bbjd_batcher_destroy(bbjd_batcher *b);
+typedef struct {
+    long batches;       /* batches run                                   */
+    long jobs;          /* inner solves run in them                      */
+    long steps;         /* Arnoldi steps (one per solve per step)        */
+    long recorded;      /* kernel launches and copies the solves made    */
+    long launches;      /* launches that ran them                        */
+} bbjd_batcher_stats;
+void bbjd_batcher_get_stats(const bbjd_batcher *b, bbjd_batcher_stats *st);
 
 /* Exact device bytes bbjd_workspace_alloc_gpu() will take for (p, o);
  * the serving loop admits requests against this figure. */

@@ -81,6 +81,14 @@ typedef struct {
                              (BBJD_INNER_REFERENCE by default)               */
     int    progress;      /* 1: a line on stderr as each arrival wave's last
                              request completes                               */
+    /* grouped inner solves (BBJD_PREC_MIXED): > 0 runs the workers' inner
+     * solves through a bbjd_batcher of inner_batchers threads (0: 2) taking
+     * up to inner_batch solves per batch and waiting up to inner_window_us
+     * for more (bbjd_gpu_options.inner_batcher); 0: every worker launches
+     * its own kernels                                                     */
+    int    inner_batch;
+    int    inner_batchers;
+    double inner_window_us;
 } bbjd_serve_config;
 
 enum { BBJD_ORDER_ARRIVAL = 0, BBJD_ORDER_LARGEST = 1, BBJD_ORDER_WINDOW = 2 };
@@ -120,6 +128,8 @@ int          bbjd_server_run(bbjd_server *s, bbjd_request *reqs, int nreq, const
                              bbjd_serve_report *rep);
 void         bbjd_server_destroy(bbjd_server *s);
 int          bbjd_server_workers(const bbjd_server *s);           /* workers with a workspace */
+/* the server's inner-solve batcher's counters (zeros without one) */
+void         bbjd_server_batch_stats(const bbjd_server *s, bbjd_batcher_stats *st);
 size_t       bbjd_server_workspace_bytes(const bbjd_server *s);   /* per worker               */
 
 /* Free device memory now (hipMemGetInfo), for logs. */

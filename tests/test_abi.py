@@ -100,6 +100,7 @@ int main(void) {
   O(bbjd_gpu_options, precond2); O(bbjd_gpu_options, qz_mode); O(bbjd_gpu_options, graphs);
   O(bbjd_gpu_options, matvec_AB); O(bbjd_gpu_options, matvec_AB_c64); O(bbjd_gpu_options, matvec_ABr);
   O(bbjd_gpu_options, inner_form); O(bbjd_gpu_options, blocking_sync);
+  O(bbjd_gpu_options, inner_batcher);
   O(bbjd_stats, outer_iters); O(bbjd_stats, lock_iters); O(bbjd_stats, restarts); O(bbjd_stats, inner_steps);
   O(bbjd_stats, dgks_second_passes); O(bbjd_stats, sum_k); O(bbjd_stats, qz_seconds);
   O(bbjd_stats, solve_seconds); O(bbjd_stats, stream_bytes); O(bbjd_stats, restart_fallbacks);
@@ -111,6 +112,8 @@ int main(void) {
   O(bbjd_serve_config, realtime); O(bbjd_serve_config, sync_yield); O(bbjd_serve_config, graphs);
   O(bbjd_serve_config, order); O(bbjd_serve_config, pin); O(bbjd_serve_config, window);
   O(bbjd_serve_config, qz_mode); O(bbjd_serve_config, inner_form); O(bbjd_serve_config, progress);
+  O(bbjd_serve_config, inner_batch); O(bbjd_serve_config, inner_batchers);
+  O(bbjd_serve_config, inner_window_us); S(bbjd_batcher_stats);
   return 0;
 }
 """
@@ -124,13 +127,14 @@ def test_gpu_struct_layouts_match_ctypes(tmp_path):
     exe = tmp_path / "gprobe"
     req = "".join(f"  O(bbjd_request, {f});\n" for f, _ in serve.Request._fields_)
     req += "".join(f"  O(bbjd_qz_problem, {f});\n" for f, _ in _lib.QzProblem._fields_)
+    req += "".join(f"  O(bbjd_batcher_stats, {f});\n" for f, _ in serve.BatchStats._fields_)
     src.write_text(GPU_PROBE.replace("  return 0;", "  S(bbjd_request);\n" + req + "  return 0;"))
     subprocess.run(["gcc", "-std=c11", f"-I{INC}", str(src), "-o", str(exe)], check=True)
     out = dict(line.rsplit(" ", 1) for line in
                subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.splitlines())
     for cname, cls in (("bbjd_gpu_options", _lib.GpuOptions), ("bbjd_stats", _lib.Stats),
                        ("bbjd_serve_config", serve.ServeConfig), ("bbjd_request", serve.Request),
-                       ("bbjd_qz_problem", _lib.QzProblem)):
+                       ("bbjd_qz_problem", _lib.QzProblem), ("bbjd_batcher_stats", serve.BatchStats)):
         assert int(out[f"sizeof.{cname}"]) == C.sizeof(cls), cname
         for f, _ in cls._fields_:
             assert int(out[f"{cname}.{f}"]) == getattr(cls, f).offset, (cname, f)

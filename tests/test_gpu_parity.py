@@ -884,6 +884,39 @@ def test_serving_loop_mixed_precision_alone_equal_mix():
             assert np.min(np.abs(w - ev)) <= 1e-8 * max(1.0, abs(ev)), (i, ev)
 
 
+@pytest.mark.parametrize("pc", ["exact", "block_jacobi"])
+def test_serving_loop_grouped_inner_solves_identical(pc):
+    """inner solves run in batches on the batcher's threads (inner_batch):
+    each Arnoldi step's kernels of the solves in a batch are merged into
+    shared launches, each solve's blocks running with its own grid and
+    arguments, so every request's eigenvalues, pair residuals and iteration
+    count are bit-identical to the workers launching their own kernels.
+    Requests of different n (different preconditioner level counts, so
+    different launch sequences to merge); the batcher's counters show the
+    merging (fewer launches than recorded ones)"""
+    from bbjd_amd import serve as srv
+    rng = np.random.default_rng(29)
+    jobs = []
+    for i, n in enumerate((4096, 6000, 9000, 5004, 12000, 7000, 16384, 4500)):
+        p = BandedTauPencil.generate(n, 4, 1, 2, seed=500 + i)
+        s = np.sqrt(rng.uniform()) * np.exp(2j * np.pi * rng.uniform())
+        jobs.append(dict(pencil=p, pc=pc, sigma=complex(s), nev=8 if pc == "exact" else 4, maxsub=32, minsub=16,
+                         maxiter=300, inner_maxiter=20, inner_tol=1e-3, tol=1e-10))
+    base, rep0 = srv.serve(jobs, workers=4, realtime=False, precision="mixed")
+    with srv.Server(jobs, workers=4, precision="mixed", inner_batch=4, inner_batchers=2) as S:
+        grp, rep = S.run(jobs, realtime=False)
+        st = S.batch_stats()
+    assert rep["failed"] == 0 and rep0["failed"] == 0
+    assert st["jobs"] > 0 and st["batches"] < st["jobs"] and 0 < st["launches"] < st["recorded"], st
+    for i in range(len(jobs)):
+        assert grp[i]["status"] == base[i]["status"] and grp[i]["nconv"] == base[i]["nconv"], i
+        assert grp[i]["iters"] == base[i]["iters"], i
+        np.testing.assert_array_equal(grp[i]["eigenvalues"], base[i]["eigenvalues"], err_msg=str(i))
+        np.testing.assert_array_equal(grp[i]["pair_residuals"], base[i]["pair_residuals"], err_msg=str(i))
+    if pc == "exact":
+        assert all(r["nconv"] == 8 for r in grp)
+
+
 def test_serving_loop_isolates_failing_requests():
     """a request the engine rejects (maxsub 1: BBJD_ERR_INVALID from the
     host's validation, before any solve kernel) and one that runs out of
