@@ -1068,7 +1068,7 @@ struct gm_job {
     struct gm_job  *next;
     /* batch state */
     gm_t            G;
-    bbjd_int        m;
+    bbjd_int        m, enq;        /* steps consumed / enqueued          */
     int             active, cb;
     hipStream_t     own;
 };
@@ -1224,6 +1224,7 @@ struct bthread {
     pthread_t     th;
     hipStream_t   s;
     hipEvent_t    ev;
+    hipEvent_t    lev[GM_LOOK_MAX];   /* after each step in flight's copy */
     void         *rec;
     char         *dstage, *hstage;
     int           started;
@@ -1313,6 +1314,7 @@ static void batch_run(struct bthread *T, struct gm_job **jobs, int nj)
         J->rc = 0;
         J->cb = 0;
         J->m = 0;
+        J->enq = 0;
         J->active = 1;
         gm_begin(c, &J->G, J->ims, J->tol);
         bbjd_rec_begin(T->rec, i);
@@ -1329,33 +1331,57 @@ static void batch_run(struct bthread *T, struct gm_job **jobs, int nj)
         if (beta0 < 1e-300) { J->active = 0; J->m = -1; continue; }   /* z = 0 */
         gm_start(&J->G, beta0);
     }
+    /* Arnoldi steps with lookahead, as in the single solve: up to L steps
+     * of every running solve are in flight, each step one flush (all
+     * solves' kernels of that step) and one copy of the staging slots; a
+     * solve's steps past its stop are discarded (they write only basis
+     * columns past m, temporaries and reduction scratch) */
+    const int L = g_gmres_look;
+    bbjd_int e = 0;
     for (bbjd_int j = 0; !err; j++) {
-        int na = 0;
-        for (int i = 0; i < nj; i++) {
-            struct gm_job *J = jobs[i];
-            if (!J->active) continue;
-            ctx_t *c = J->c;
-            struct bbjd_dev *d = c->d;
-            const int joint = (size_t)(J->ims + 1) * 8 <= (size_t)((const char *)d->rd.scal - (const char *)d->pencil);
-            bbjd_rec_begin(T->rec, i);
-            const int er = arnoldi_enqueue(c, j, J->ndefl, J->QR, J->QL, d->gvc, c->ws->tmp3, c->ws->tmp4, NULL,
-                                           joint ? NULL : d->hscal);
-            bbjd_rec_end();
-            if (er) { J->rc = er == -1 ? -1 : -2; J->active = 0; continue; }
-            na++;
+        while (e < j + L && !err) {
+            int na = 0;
+            for (int i = 0; i < nj; i++) {
+                struct gm_job *J = jobs[i];
+                if (!J->active || e >= J->ims) continue;
+                ctx_t *c = J->c;
+                struct bbjd_dev *d = c->d;
+                const int joint =
+                    (size_t)(J->ims + 1) * 8 <= (size_t)((const char *)d->rd.scal - (const char *)d->pencil);
+                c->stage = T->dstage + ((size_t)(e % L) * BATCH_JOBS_MAX + i) * B->slot;
+                bbjd_rec_begin(T->rec, i);
+                const int er = arnoldi_enqueue(c, e, J->ndefl, J->QR, J->QL, d->gvc, c->ws->tmp3, c->ws->tmp4,
+                                               NULL, joint ? NULL : d->hscal);
+                bbjd_rec_end();
+                if (er) { J->rc = er == -1 ? -1 : -2; J->active = 0; continue; }
+                J->enq = e + 1;
+                na++;
+            }
+            if (!na) break;
+            const size_t off = (size_t)(e % L) * BATCH_JOBS_MAX * B->slot;
+            long nl = 0;
+            err |= bbjd_rec_flush(T->rec, T->s, &nl);
+            err |= !HIPOK(hipMemcpyAsync(T->hstage + off, T->dstage + off, (size_t)nj * B->slot,
+                                         hipMemcpyDeviceToHost, T->s));
+            err |= !HIPOK(hipEventRecord(T->lev[e % L], T->s));
+            e++;
         }
-        if (!na) break;
-        err |= batch_phase_end(T, nj, 1);
+        if (err || j >= e) break;
+        err |= !HIPOK(hipEventSynchronize(T->lev[j % L]));
         if (err) break;
         for (int i = 0; i < nj; i++) {
             struct gm_job *J = jobs[i];
-            if (!J->active) continue;
-            const char *hb = T->hstage + (size_t)i * B->slot;
+            if (!J->active || J->enq <= j) continue;
+            const char *hb = T->hstage + ((size_t)(j % L) * BATCH_JOBS_MAX + i) * B->slot;
             const double *hs = (const double *)(hb + B->stage_sc);
             const void *hh = hb + B->stage_sc - (size_t)(j + 1) * 8;
             B->st.steps++;
-            if (gm_column(J->c, &J->G, j, J->ndefl, hh, hs)) { J->m = j + 1; J->active = 0; }
-            else if (j + 1 == J->ims) { J->m = J->ims; J->active = 0; }
+            int stop = gm_column(J->c, &J->G, j, J->ndefl, hh, hs);
+            if (stop || j + 1 == J->ims) {
+                J->m = j + 1;
+                J->active = 0;
+                J->c->d->st.inner_steps_discarded += J->enq - J->m;
+            }
         }
     }
     if (!err) {
@@ -1470,8 +1496,11 @@ bbjd_batcher *bbjd_batcher_create(int threads, int max_jobs, double window_us, b
         T->B = B;
         ok &= HIPOK(hipStreamCreateWithFlags(&T->s, hipStreamNonBlocking));
         ok &= HIPOK(hipEventCreateWithFlags(&T->ev, hipEventDisableTiming | hipEventBlockingSync));
-        ok &= HIPOK(hipMalloc((void **)&T->dstage, BATCH_JOBS_MAX * B->slot));
-        ok &= HIPOK(hipHostMalloc((void **)&T->hstage, BATCH_JOBS_MAX * B->slot, hipHostMallocDefault));
+        for (int l = 0; l < GM_LOOK_MAX; l++)
+            ok &= HIPOK(hipEventCreateWithFlags(&T->lev[l], hipEventDisableTiming | hipEventBlockingSync));
+        ok &= HIPOK(hipMalloc((void **)&T->dstage, (size_t)GM_LOOK_MAX * BATCH_JOBS_MAX * B->slot));
+        ok &= HIPOK(hipHostMalloc((void **)&T->hstage, (size_t)GM_LOOK_MAX * BATCH_JOBS_MAX * B->slot,
+                                  hipHostMallocDefault));
         T->rec = bbjd_rec_create();
         ok &= T->rec != NULL;
         if (ok) T->started = pthread_create(&T->th, NULL, batcher_main, T) == 0;
@@ -1495,6 +1524,8 @@ void bbjd_batcher_destroy(bbjd_batcher *B)
         if (T->hstage) (void)hipHostFree(T->hstage);
         if (T->dstage) (void)hipFree(T->dstage);
         if (T->ev) (void)hipEventDestroy(T->ev);
+        for (int l = 0; l < GM_LOOK_MAX; l++)
+            if (T->lev[l]) (void)hipEventDestroy(T->lev[l]);
         if (T->s) (void)hipStreamDestroy(T->s);
     }
     pthread_cond_destroy(&B->cv_job);

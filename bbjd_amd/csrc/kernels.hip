@@ -59,12 +59,26 @@ template <typename R> __device__ __forceinline__ Cx<R> operator+(Cx<R> a, Cx<R> 
 template <typename R> __device__ __forceinline__ Cx<R> operator-(Cx<R> a, Cx<R> b) { return cx(a.re - b.re, a.im - b.im); }
 template <typename R> __device__ __forceinline__ Cx<R> operator*(Cx<R> a, Cx<R> b)
 { return cx(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re); }
+// Accumulations as four explicit fused multiply-adds into the accumulator
+// (the fusion a loop body gets does not depend on the compiler's choices).
+__device__ __forceinline__ double ffma(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float ffma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 // a += conj(q) * x
 template <typename R> __device__ __forceinline__ void cfma_conj(R &are, R &aim, Cx<R> q, Cx<R> x)
-{ are += q.re * x.re + q.im * x.im; aim += q.re * x.im - q.im * x.re; }
+{
+    are = ffma(q.re, x.re, are);
+    are = ffma(q.im, x.im, are);
+    aim = ffma(q.re, x.im, aim);
+    aim = ffma(-q.im, x.re, aim);
+}
 // a += q * h
 template <typename R> __device__ __forceinline__ void cfma(Cx<R> &a, Cx<R> q, Cx<R> h)
-{ a.re += q.re * h.re - q.im * h.im; a.im += q.re * h.im + q.im * h.re; }
+{
+    a.re = ffma(q.re, h.re, a.re);
+    a.re = ffma(-q.im, h.im, a.re);
+    a.im = ffma(q.re, h.im, a.im);
+    a.im = ffma(q.im, h.re, a.im);
+}
 
 static constexpr int TPB = BBJD_TPB;
 typedef double v4d __attribute__((ext_vector_type(4)));   // f64 MFMA accumulator
@@ -2269,10 +2283,11 @@ int bbjd_rec_flush(void *rec, hipStream_t s, long *launches)
                 sel.push_back(&r->jobs[j][h[j]]);
                 h[j]++;
             }
-        best(sel.data(), (int)sel.size(), s);
+        best(sel.data(), (int)sel.size(), s, r->arena.data());
         nl++;
     }
     for (auto &l : r->jobs) l.clear();
+    r->arena.clear();
     r->launches += nl;
     if (launches) *launches = nl;
     if (hipGetLastError() != hipSuccess) rc = -1;

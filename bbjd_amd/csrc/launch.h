@@ -22,13 +22,16 @@
 
 #include <hip/hip_runtime.h>
 #include <functional>
-#include <memory>
+#include <new>
+#include <type_traits>
 #include <tuple>
 #include <utility>
 #include <vector>
 
-struct VB { int bx, by, gx, gy; };
-#define VB_HW VB{(int)blockIdx.x, (int)blockIdx.y, (int)gridDim.x, (int)gridDim.y}
+// the block index and grid size a kernel body sees; unsigned like the
+// hardware's, so a body called with VB_HW compiles to the kernel it was
+struct VB { unsigned bx, by, gx, gy; };
+#define VB_HW VB{blockIdx.x, blockIdx.y, gridDim.x, gridDim.y}
 
 namespace bbjd {
 
@@ -50,13 +53,13 @@ __global__ __launch_bounds__(BS) void k_group(Group<P...> g)
 {
     int j = 0;
     while (j + 1 < g.nj && (int)blockIdx.x >= g.start[j + 1]) j++;
-    const int l = (int)blockIdx.x - g.start[j];
-    const VB vb{l % g.gx[j], l / g.gx[j], g.gx[j], g.gy[j]};
+    const unsigned l = blockIdx.x - (unsigned)g.start[j], gx = (unsigned)g.gx[j];
+    const VB vb{l % gx, l / gx, gx, (unsigned)g.gy[j]};
     std::apply([&](P... p) { BODY(vb, p...); }, g.a[j]);
 }
 
 struct Op;
-typedef void (*GroupFn)(Op *const *ops, int n, hipStream_t s);
+typedef void (*GroupFn)(Op *const *ops, int n, hipStream_t s, const unsigned char *arena);
 
 // One recorded launch: a groupable kernel (fn != nullptr; ops with the same
 // fn are merged) or a closure run in order on its own.
@@ -65,29 +68,40 @@ struct Op {
     void *kern = nullptr;          // the job's own kernel (one-job launch)
     dim3 grid;
     size_t lds = 0;
-    std::shared_ptr<void> args;    // std::tuple<P...>
+    size_t args = 0;               // offset of its std::tuple<P...> in the arena
     std::function<void(hipStream_t)> solo;
 };
 
 struct Recorder {
     std::vector<std::vector<Op>> jobs;
+    std::vector<unsigned char> arena;   // argument tuples of the recorded launches
     int cur = -1;
     long launches = 0;             // launches made by flushes so far
     long recorded = 0;             // launches recorded so far
+
+    // room for an object of the given size and alignment (<= 16); arena
+    // offsets stay valid when it grows
+    size_t alloc(size_t size, size_t align)
+    {
+        const size_t off = (arena.size() + align - 1) / align * align;
+        arena.resize(off + size);
+        return off;
+    }
 };
 
 // the recorder of the calling thread while it records, else null
 Recorder *&tl_rec();
 
 template <auto BODY, int BS, typename... P>
-void group_launch(Op *const *ops, int n, hipStream_t s)
+void group_launch(Op *const *ops, int n, hipStream_t s, const unsigned char *arena)
 {
     typedef void (*K)(P...);
+    typedef std::tuple<P...> T;
     if (n == 1) {
         const Op &o = *ops[0];
         const K kern = reinterpret_cast<K>(o.kern);
         std::apply([&](P... p) { hipLaunchKernelGGL(kern, o.grid, dim3(BS), o.lds, s, p...); },
-                   *static_cast<std::tuple<P...> *>(o.args.get()));
+                   *reinterpret_cast<const T *>(arena + o.args));
         return;
     }
     static_assert(sizeof(Group<P...>) <= 4096, "grouped kernel arguments exceed 4 KiB");
@@ -100,7 +114,7 @@ void group_launch(Op *const *ops, int n, hipStream_t s)
         g.start[j] = tot;
         g.gx[j] = (int)o.grid.x;
         g.gy[j] = (int)o.grid.y;
-        g.a[j] = *static_cast<std::tuple<P...> *>(o.args.get());
+        g.a[j] = *reinterpret_cast<const T *>(arena + o.args);
         tot += (int)(o.grid.x * o.grid.y);
         if (o.lds > lds) lds = o.lds;
     }
@@ -117,12 +131,15 @@ inline void gl(void (*kern)(P...), dim3 grid, size_t lds, hipStream_t s, A... a)
         hipLaunchKernelGGL(kern, grid, dim3(BS), lds, s, static_cast<P>(a)...);
         return;
     }
+    typedef std::tuple<P...> T;
+    static_assert(std::is_trivially_destructible<T>::value && alignof(T) <= 16, "kernel arguments");
     Op o;
     o.fn = &group_launch<BODY, BS, P...>;
     o.kern = reinterpret_cast<void *>(kern);
     o.grid = grid;
     o.lds = lds;
-    o.args = std::make_shared<std::tuple<P...>>(static_cast<P>(a)...);
+    o.args = r->alloc(sizeof(T), alignof(T));
+    new (r->arena.data() + o.args) T(static_cast<P>(a)...);
     r->jobs[r->cur].push_back(std::move(o));
     r->recorded++;
 }

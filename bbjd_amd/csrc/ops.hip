@@ -1391,18 +1391,33 @@ int bbjd_o_band_abr(hipStream_t s, int64_t n, int64_t n1, int64_t nb, int bs, in
                        (const Cx<R> *)band, (const Cx<R> *)dg, ntau, (const Cx<R> *)tauc, (const Cx<R> *)x,       \
                        (Cx<R> *)ya, (Cx<R> *)yb, (Cx<R> *)part, Cx<R>{(R)are, (R)aim}, Cx<R>{(R)bre, (R)bim},     \
                        (Cx<R> *)r, (R *)rpart)
+/* the dense rows' finish, groupable for the block sizes of up to 3 dense
+ * rows with the residual (one wave per row) */
+#define BBJD_TAU_FIN(R, BSZ)                                                                                    \
+    bbjd::gl<&k_tau_finish_b<R>, BSZ>(k_tau_finish<R>, dim3(1), 0, s, (i64)nblk, ntau, (i64)n1,                 \
+                                      (const Cx<R> *)part, (Cx<R> *)ya, (const Cx<R> *)yb,                      \
+                                      Cx<R>{(R)are, (R)aim}, Cx<R>{(R)bre, (R)bim}, (Cx<R> *)r,                 \
+                                      (const R *)rpart, rnorm)
 #define BBJD_AB(R)                                                                                              \
     if (bs == 8 && hw == 1) BBJD_AB_K(R, 8, 1);                                                                 \
     else if (bs == 4 && hw == 1) BBJD_AB_K(R, 4, 1);                                                            \
     else if (bs == 8 && hw == 2) BBJD_AB_K(R, 8, 2);                                                            \
     else BBJD_AB_K(R, 0, 0);                                                                                    \
     if (fin_threads > 0)                                                                                        \
-        BBJD_SL(s, (k_tau_finish<R>), dim3(1), dim3(fin_threads), 0, (i64)nblk, ntau, (i64)n1,        \
-                           (const Cx<R> *)part, (Cx<R> *)ya, (const Cx<R> *)yb, Cx<R>{(R)are, (R)aim},           \
-                           Cx<R>{(R)bre, (R)bim}, (Cx<R> *)r, (const R *)rpart, rnorm);
+        switch (fin_threads) {                                                                                  \
+        case 64: BBJD_TAU_FIN(R, 64); break;                                                                    \
+        case 128: BBJD_TAU_FIN(R, 128); break;                                                                  \
+        case 192: BBJD_TAU_FIN(R, 192); break;                                                                  \
+        case 256: BBJD_TAU_FIN(R, 256); break;                                                                  \
+        default:                                                                                                \
+            BBJD_SL(s, (k_tau_finish<R>), dim3(1), dim3(fin_threads), 0, (i64)nblk, ntau, (i64)n1,              \
+                    (const Cx<R> *)part, (Cx<R> *)ya, (const Cx<R> *)yb, Cx<R>{(R)are, (R)aim},                 \
+                    Cx<R>{(R)bre, (R)bim}, (Cx<R> *)r, (const R *)rpart, rnorm);                                \
+        }
     if (single) { BBJD_AB(float) } else { BBJD_AB(double) }
 #undef BBJD_AB
 #undef BBJD_AB_K
+#undef BBJD_TAU_FIN
     return chk();
 }
 
@@ -1651,7 +1666,22 @@ int bbjd_o_border_small(hipStream_t st, int ntau, const void *b2_0, const void *
 {
     if (ntau <= 0 || ntau * nrhs > 1024 || (dpart && ntau > 16) || (c64 && nrhs != 1)) return -2;
     const int thr = dpart ? 1024 : ntau * nrhs <= 64 ? 64 : (ntau * nrhs <= 256 ? 256 : 1024);
-    if (nrhs == 2)
+    if (nrhs == 1 && (thr == 64 || thr == 256 || thr == 1024)) {
+        /* groupable */
+        const Cx<double> *b20 = c64 ? nullptr : (const Cx<double> *)b2_0;
+        Cx<double> *yt0 = c64 ? nullptr : (Cx<double> *)ytail0;
+        const Cx<float> *b2f = c64 ? (const Cx<float> *)b2_0 : nullptr;
+        Cx<float> *ytf = c64 ? (Cx<float> *)ytail0 : nullptr;
+#define BBJD_BORDER1(BSZ)                                                                                       \
+    bbjd::gl<&k_border_small_b<1>, BSZ>(k_border_small<1>, dim3(1), (size_t)ntau * 16, st, ntau, b20, b20,     \
+                                        (const Cx<double> *)dots0, (const Cx<double> *)dots0,                   \
+                                        (const Cx<double> *)Sinv, (Cx<double> *)y2, yt0, yt0,                   \
+                                        (const Cx<double> *)dpart, (i64)nblk, b2f, ytf)
+        if (thr == 64) BBJD_BORDER1(64);
+        else if (thr == 256) BBJD_BORDER1(256);
+        else BBJD_BORDER1(1024);
+#undef BBJD_BORDER1
+    } else if (nrhs == 2)
         BBJD_SL(st, k_border_small<2>, dim3(1), dim3(thr), (size_t)2 * ntau * 16, ntau,
                            (const Cx<double> *)b2_0, (const Cx<double> *)b2_1, (const Cx<double> *)dots0,
                            (const Cx<double> *)dots1, (const Cx<double> *)Sinv, (Cx<double> *)y2,

@@ -116,9 +116,11 @@ def make_small_requests(nreq, seed):
 # workload's plain path is the FP64 engine with the one-step correction)
 WORKLOADS = {
     "serving": dict(shape=(8, 1, 4), realtime=True, maxsub=lambda k: 4 * k, minsub=lambda k: 2 * k,
+                    workers=16, inner_batch=0,
                     served=dict(inner=0, inner_tol=1e-2, precision="fp64", inner_form="projected"),
                     plain=dict(inner=0, inner_tol=1e-2, precision="fp64", inner_form="reference")),
     "small": dict(shape=(4, 1, 2), realtime=False, maxsub=lambda k: 32, minsub=lambda k: 16,
+                  workers=96, inner_batch=16,
                   served=dict(inner=20, inner_tol=1e-3, precision="mixed", inner_form="reference"),
                   plain=dict(inner=20, inner_tol=1e-3, precision="mixed", inner_form="reference")),
 }
@@ -294,7 +296,8 @@ def main():
     ap.add_argument("--workload", choices=["serving", "small"], default="serving")
     ap.add_argument("--nreq", type=int, default=0, help="requests (default 256 serving, 2048 small)")
     ap.add_argument("--rate", type=float, default=32.0, help="Poisson arrival rate, requests/s")
-    ap.add_argument("--workers", type=int, default=16)
+    ap.add_argument("--workers", type=int, default=0,
+                    help="serving workers (default: 16 serving, 96 small, whose inner solves run grouped)")
     ap.add_argument("--maxiter", type=int, default=1000)
     ap.add_argument("--plain-per-class", type=int, default=3, help="plain solves per (n, k) class")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
@@ -340,10 +343,11 @@ def main():
     ap.add_argument("--pipeline", type=int, default=10,
                     help="timed steps per server run, each one arrival wave (1: every step drains before "
                          "the next starts arriving)")
-    ap.add_argument("--inner-batch", type=int, default=0,
-                    help="served leg, mixed inner solves: up to this many per grouped batch (0: off)")
-    ap.add_argument("--inner-batchers", type=int, default=2, help="batcher threads (streams) with --inner-batch")
-    ap.add_argument("--inner-window-us", type=float, default=100.0,
+    ap.add_argument("--inner-batch", type=int, default=-1,
+                    help="served leg, mixed inner solves: up to this many per grouped batch (0: off; default: "
+                         "16 for the small workload, off for the serving workload)")
+    ap.add_argument("--inner-batchers", type=int, default=4, help="batcher threads (streams) with --inner-batch")
+    ap.add_argument("--inner-window-us", type=float, default=300.0,
                     help="how long a batcher waits for more inner solves (microseconds)")
     ap.add_argument("--seed", type=int, default=20261203)
     args = ap.parse_args()
@@ -382,6 +386,12 @@ def main():
     if args.workload == "small":
         # the small solves' plain leg is the served form solved one at a time
         wl["plain"] = dict(wl["served"])
+    if args.workers <= 0:
+        args.workers = wl["workers"]
+    if args.inner_batch < 0:
+        args.inner_batch = wl["inner_batch"]
+    if wl["served"]["precision"] != "mixed" or not wl["served"]["inner"]:
+        args.inner_batch = 0            # only mixed inner solves are grouped
     if args.nreq <= 0:
         args.nreq = 256 if args.workload == "serving" else 2048
     reqs = (make_requests(args.nreq, args.rate, args.seed) if args.workload == "serving"
