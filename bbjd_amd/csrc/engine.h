@@ -204,6 +204,10 @@ int   bbjd_memset_async(void *dst, int value, size_t bytes, hipStream_t s);
 /* device copy of bytes (a multiple of 8) by a kernel (groupable) */
 int   bbjd_k_copy(hipStream_t s, void *dst, const void *src, size_t bytes);
 
+/* batcher job choice: 1 (default) the longest-waiting inner solve and those
+ * nearest to it in n; 0 arrival order */
+void  bbjd_set_batch_pick(int nearest_n);
+
 /* bytes of reduction scratch for the given shape */
 int64_t bbjd_red_part_entries(int64_t n, int maxk);
 

@@ -1218,6 +1218,9 @@ static int gmres(ctx_t *c, bbjd_int ndefl, const void *QR, const void *QL, zcx t
  * stopping rule. */
 #define BATCH_THREADS_MAX 8
 #define BATCH_JOBS_MAX 16
+#define BATCH_CAND_MAX 256
+static int g_batch_pick = 1;
+void bbjd_set_batch_pick(int nearest_n) { g_batch_pick = nearest_n != 0; }
 
 struct bthread {
     bbjd_batcher *B;
@@ -1429,23 +1432,48 @@ static void *batcher_main(void *arg)
         }
         if (!B->head) continue;
         /* take up to max_jobs, one per operator and none whose operator is in
-         * a running batch (an operator's scratch serves one apply at a time) */
-        int nj = 0;
-        struct gm_job *prev = NULL, *J = B->head;
-        while (J && nj < B->max_jobs) {
-            struct gm_job *next = J->next;
+         * a running batch (an operator's scratch serves one apply at a time):
+         * the longest-waiting one, then (g_batch_pick 1) those nearest to it
+         * in n, whose launch sequences match best (the preconditioner's level
+         * structure follows n), or (0) the next ones in arrival order */
+        int nj = 0, ncand = 0;
+        struct gm_job *cand[BATCH_CAND_MAX];
+        for (struct gm_job *J = B->head; J && ncand < BATCH_CAND_MAX; J = J->next) {
             const void *op = J->c->P->op_data;
             int dup = 0;
             for (int i = 0; i < B->nbusy; i++) dup |= B->busy[i] == op;
-            if (!dup) {
-                B->busy[B->nbusy++] = op;
-                if (prev) prev->next = next;
-                else B->head = next;
-                if (B->tail == J) B->tail = prev;
-                jobs[nj++] = J;
-            } else
-                prev = J;
-            J = next;
+            for (int i = 0; i < ncand; i++) dup |= cand[i]->c->P->op_data == op;
+            if (!dup) cand[ncand++] = J;
+        }
+        if (g_batch_pick && ncand > B->max_jobs) {
+            /* insertion sort of cand[1..] by distance in log n from cand[0] */
+            const double l0 = log((double)cand[0]->c->n);
+            for (int i = 2; i < ncand; i++) {
+                struct gm_job *x = cand[i];
+                const double dx = fabs(log((double)x->c->n) - l0);
+                int k = i - 1;
+                while (k >= 1 && fabs(log((double)cand[k]->c->n) - l0) > dx) { cand[k + 1] = cand[k]; k--; }
+                cand[k + 1] = x;
+            }
+        }
+        for (int i = 0; i < ncand && nj < B->max_jobs; i++) {
+            cand[i]->done = 2;        /* taken (done is 0 while queued) */
+            B->busy[B->nbusy++] = cand[i]->c->P->op_data;
+            jobs[nj++] = cand[i];
+        }
+        {
+            struct gm_job *prev = NULL;
+            for (struct gm_job *J = B->head; J;) {
+                struct gm_job *next = J->next;
+                if (J->done == 2) {
+                    J->done = 0;
+                    if (prev) prev->next = next;
+                    else B->head = next;
+                    if (B->tail == J) B->tail = prev;
+                } else
+                    prev = J;
+                J = next;
+            }
         }
         if (!nj) {
             /* every waiting solve's operator is busy: wait for a batch to end */

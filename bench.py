@@ -349,6 +349,8 @@ def main():
     ap.add_argument("--inner-batchers", type=int, default=4, help="batcher threads (streams) with --inner-batch")
     ap.add_argument("--inner-window-us", type=float, default=300.0,
                     help="how long a batcher waits for more inner solves (microseconds)")
+    ap.add_argument("--batch-pick", choices=["nearest", "arrival"], default="nearest",
+                    help="which waiting inner solves a batch takes with the oldest: nearest in n, or arrival order")
     ap.add_argument("--seed", type=int, default=20261203)
     args = ap.parse_args()
 
@@ -367,6 +369,7 @@ def main():
     _lib.load().bbjd_set_gmres_lookahead(args.gmres_lookahead)
     if args.fuse_blocks > 0:
         _lib.load().bbjd_set_fuse_blocks(args.fuse_blocks)
+    _lib.load().bbjd_set_batch_pick(1 if args.batch_pick == "nearest" else 0)
     if args.mid_blocks > 0:
         _lib.load().bbjd_set_cgs_mid_blocks(args.mid_blocks)
     if args.proj_blocks > 0:
@@ -714,7 +717,8 @@ def main():
                        f"{wl['plain']['inner_tol']:g}" if wl["plain"]["inner"] else
                        f"{wl['plain']['precision']}, one-step correction"),
         "inner_batching": (dict(max_jobs=args.inner_batch, batchers=args.inner_batchers,
-                                window_us=args.inner_window_us, **batch) if args.inner_batch and batch else None),
+                                window_us=args.inner_window_us, pick=args.batch_pick, **batch)
+                           if args.inner_batch and batch else None),
         "setup_seconds": round(t_setup, 1),
         "roofline": roof,
         "served_region": served_region,
