@@ -196,6 +196,11 @@ void  bbjd_rec_destroy(void *rec);
 void  bbjd_rec_begin(void *rec, int job);
 void  bbjd_rec_end(void);
 int   bbjd_rec_flush(void *rec, hipStream_t s, long *launches);
+/* the launch order of a flush for nj lists of lens[j] launch keys (keys of
+ * list j at keys[sum of lens[0..j-1]], 0: a launch that is never merged):
+ * groups receives each launch as its list count followed by the lists; returns
+ * the number of launches, -1 when cap ints do not hold them (tests) */
+int   bbjd_rec_plan(int nj, const int *lens, const int64_t *keys, int *groups, int cap);
 /* launches recorded and launches made by flushes, since creation */
 void  bbjd_rec_counts(const void *rec, long *recorded, long *launches);
 /* hipMemcpyAsync / hipMemsetAsync, recorded in order while recording */

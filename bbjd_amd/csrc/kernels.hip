@@ -1892,6 +1892,59 @@ __global__ __launch_bounds__(TPB) void k_copy_words(i64 nw, const unsigned long 
 }
 
 namespace bbjd {
+// The order of a flush.  nj lists of launches, launch i of list j with key
+// key(j, i) (0: a closure, launched on its own): each step launches, as one
+// group, every list whose next launch has the same key as the next launch
+// of the list with the most launches left (ties: the key that heads the
+// most lists).  Advancing the longest list first lines the lists' ends up,
+// so solves whose launch sequences differ in the middle (preconditioner
+// levels of different n) merge again in the common part that follows;
+// closures run as soon as they head their list.  emit(sel, n, at) gets the
+// n lists of a launch and their positions.  Returns 0.
+template <class Len, class Key, class Emit>
+int plan_merge(int nj, Len len, Key key, Emit emit)
+{
+    std::vector<size_t> h(nj, 0);
+    int sel[kGroupMax];
+    size_t at[kGroupMax];
+    for (;;) {
+        for (int j = 0; j < nj; j++)
+            while (h[j] < len(j) && key(j, h[j]) == 0) {
+                sel[0] = j;
+                at[0] = h[j];
+                emit(sel, 1, at);
+                h[j]++;
+            }
+        int lead = -1;
+        size_t lead_rem = 0;
+        int lead_cnt = 0;
+        for (int j = 0; j < nj; j++) {
+            if (h[j] >= len(j)) continue;
+            const size_t rem = len(j) - h[j];
+            const uintptr_t k = key(j, h[j]);
+            int cnt = 0;
+            for (int i = 0; i < nj; i++) cnt += h[i] < len(i) && key(i, h[i]) == k;
+            if (lead < 0 || rem > lead_rem || (rem == lead_rem && cnt > lead_cnt)) {
+                lead = j;
+                lead_rem = rem;
+                lead_cnt = cnt;
+            }
+        }
+        if (lead < 0) return 0;
+        const uintptr_t k = key(lead, h[lead]);
+        int n = 0;
+        sel[n] = lead;
+        at[n++] = h[lead];
+        for (int j = 0; j < nj && n < kGroupMax; j++)
+            if (j != lead && h[j] < len(j) && key(j, h[j]) == k) {
+                sel[n] = j;
+                at[n++] = h[j];
+            }
+        emit(sel, n, at);
+        for (int g = 0; g < n; g++) h[sel[g]]++;
+    }
+}
+
 Recorder *&tl_rec()
 {
     static thread_local Recorder *r = nullptr;
@@ -2224,9 +2277,7 @@ int bbjd_k_c2z(hipStream_t s, int64_t n, const void *c, void *z)
 
 // ---------------------------------------------------------------------------
 // Recording and grouped launch (launch.h).  bbjd_rec_flush launches the jobs'
-// lists: closures as soon as they reach the head of their job's list, then
-// the groupable kernel that heads the most lists, for all of those jobs in
-// one launch, until every list is empty.  Each job's launches keep their
+// lists in the order bbjd::plan_merge gives.  Each job's launches keep their
 // order (one stream), and jobs share no writable memory, so the interleaving
 // does not change any job's results.
 void *bbjd_rec_create(void) { return new bbjd::Recorder; }
@@ -2250,47 +2301,44 @@ int bbjd_rec_flush(void *rec, hipStream_t s, long *launches)
 {
     bbjd::Recorder *r = static_cast<bbjd::Recorder *>(rec);
     const int nj = (int)r->jobs.size();
-    std::vector<size_t> h(nj, 0);
-    std::vector<bbjd::Op *> sel;
-    sel.reserve(bbjd::kGroupMax);
     long nl = 0;
-    int rc = 0;
-    for (;;) {
-        bool any = false, ran = false;
-        for (int j = 0; j < nj; j++)
-            while (h[j] < r->jobs[j].size() && !r->jobs[j][h[j]].fn) {
-                r->jobs[j][h[j]].solo(s);
-                h[j]++;
-                nl++;
-                ran = true;
+    const int rc = bbjd::plan_merge(
+        nj, [&](int j) { return r->jobs[j].size(); },
+        [&](int j, size_t i) { return (uintptr_t)r->jobs[j][i].fn; },
+        [&](const int *sel, int n, const size_t *at) {
+            const bbjd::Op &o0 = r->jobs[sel[0]][at[0]];
+            if (!o0.fn) {
+                o0.solo(s);
+            } else {
+                bbjd::Op *ops[bbjd::kGroupMax];
+                for (int g = 0; g < n; g++) ops[g] = &r->jobs[sel[g]][at[g]];
+                o0.fn(ops, n, s, r->arena.data());
             }
-        bbjd::GroupFn best = nullptr;
-        int bestc = 0;
-        for (int j = 0; j < nj; j++) {
-            if (h[j] >= r->jobs[j].size()) continue;
-            any = true;
-            const bbjd::GroupFn f = r->jobs[j][h[j]].fn;
-            int c = 0;
-            for (int i = 0; i < nj; i++) c += h[i] < r->jobs[i].size() && r->jobs[i][h[i]].fn == f;
-            if (c > bestc) { bestc = c; best = f; }
-        }
-        if (!any) break;
-        if (!best) { if (!ran) { rc = -1; break; } continue; }
-        sel.clear();
-        for (int j = 0; j < nj && (int)sel.size() < bbjd::kGroupMax; j++)
-            if (h[j] < r->jobs[j].size() && r->jobs[j][h[j]].fn == best) {
-                sel.push_back(&r->jobs[j][h[j]]);
-                h[j]++;
-            }
-        best(sel.data(), (int)sel.size(), s, r->arena.data());
-        nl++;
-    }
+            nl++;
+        });
     for (auto &l : r->jobs) l.clear();
     r->arena.clear();
     r->launches += nl;
     if (launches) *launches = nl;
-    if (hipGetLastError() != hipSuccess) rc = -1;
-    return rc;
+    if (rc || hipGetLastError() != hipSuccess) return -1;
+    return 0;
+}
+
+int bbjd_rec_plan(int nj, const int *lens, const int64_t *keys, int *groups, int cap)
+{
+    if (nj < 0 || (nj && (!lens || !keys || !groups))) return -1;
+    std::vector<size_t> off(nj + 1, 0);
+    for (int j = 0; j < nj; j++) off[j + 1] = off[j] + (size_t)lens[j];
+    int used = 0, nl = 0, over = 0;
+    const int rc = bbjd::plan_merge(
+        nj, [&](int j) { return (size_t)lens[j]; }, [&](int j, size_t i) { return (uintptr_t)keys[off[j] + i]; },
+        [&](const int *sel, int n, const size_t *) {
+            if (used + 1 + n > cap) { over = 1; return; }
+            groups[used++] = n;
+            for (int g = 0; g < n; g++) groups[used++] = sel[g];
+            nl++;
+        });
+    return rc || over ? -1 : nl;
 }
 
 int bbjd_copy_async(void *dst, const void *src, size_t bytes, int kind, hipStream_t s)
