@@ -120,7 +120,7 @@ WORKLOADS = {
                     served=dict(inner=0, inner_tol=1e-2, precision="fp64", inner_form="projected"),
                     plain=dict(inner=0, inner_tol=1e-2, precision="fp64", inner_form="reference")),
     "small": dict(shape=(4, 1, 2), realtime=False, maxsub=lambda k: 32, minsub=lambda k: 16,
-                  workers=96, inner_batch=16,
+                  workers=128, inner_batch=16,
                   served=dict(inner=20, inner_tol=1e-3, precision="mixed", inner_form="reference"),
                   plain=dict(inner=20, inner_tol=1e-3, precision="mixed", inner_form="reference")),
 }
@@ -297,7 +297,7 @@ def main():
     ap.add_argument("--nreq", type=int, default=0, help="requests (default 256 serving, 2048 small)")
     ap.add_argument("--rate", type=float, default=32.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--workers", type=int, default=0,
-                    help="serving workers (default: 16 serving, 96 small, whose inner solves run grouped)")
+                    help="serving workers (default: 16 serving, 128 small, whose inner solves run grouped)")
     ap.add_argument("--maxiter", type=int, default=1000)
     ap.add_argument("--plain-per-class", type=int, default=3, help="plain solves per (n, k) class")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
