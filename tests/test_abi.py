@@ -196,3 +196,23 @@ def test_bench_refuses_more_than_one_gpu():
                        text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in r.stderr and r.stdout == ""
+
+
+def test_batcher_and_serving_options_are_validated():
+    """the inner-solve batcher refuses out-of-range settings before touching
+    a device (threads 1..8, jobs per batch 1..16, a window >= 0, inner
+    iterations >= 1), and the Python serving config checks them too"""
+    from bbjd_amd import serve
+    lib = _lib.load()
+    lib.bbjd_batcher_create.restype = C.c_void_p
+    lib.bbjd_batcher_create.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int64]
+    for args in ((0, 4, 100.0, 20), (9, 4, 100.0, 20), (2, 0, 100.0, 20), (2, 17, 100.0, 20),
+                 (2, 4, -1.0, 20), (2, 4, 100.0, 0)):
+        assert not lib.bbjd_batcher_create(*args), args
+    serve._setup(lib)
+    for kw in (dict(inner_batch=17), dict(inner_batch=-1), dict(inner_batchers=0), dict(inner_batchers=9),
+               dict(inner_window_us=-1.0)):
+        with pytest.raises(ValueError):
+            serve._config(lib, **kw)
+    cfg = serve._config(lib, inner_batch=16, inner_batchers=4, inner_window_us=300.0)
+    assert (cfg.inner_batch, cfg.inner_batchers, cfg.inner_window_us) == (16, 4, 300.0)
