@@ -1883,7 +1883,8 @@ static int qz_device(ctx_t *c, bbjd_int k, const zcx *MA, const zcx *MB, const z
     int rc = 0;
     rc |= !HIPOK(hipMemcpyAsync(db, stg, cb, hipMemcpyHostToDevice, c->s));
     rc |= !HIPOK(hipMemcpyAsync(db + mb, (char *)stg + cb, cb, hipMemcpyHostToDevice, c->s));
-    if (nconv > 0) rc |= !HIPOK(hipMemcpyAsync(db + o_conv, d->hsmall, (size_t)nconv * 16, hipMemcpyHostToDevice, c->s));
+    if (nconv > 0) rc |= !HIPOK(hipMemcpyAsync(db + o_conv, d->hsmall, (size_t)nconv * 16, hipMemcpyHostToDevice,
+        c->s));
     rc |= !HIPOK(hipMemcpyAsync(db + o_prob, hp, sizeof(*hp), hipMemcpyHostToDevice, c->s));
     if (!rc) rc |= bbjd_qz_device_batch((const bbjd_qz_problem *)(db + o_prob), 1, c->s) ? 1 : 0;
     rc |= !HIPOK(hipMemcpyAsync(hinfo, db + o_info, 4 * sizeof(int), hipMemcpyDeviceToHost, c->s));
@@ -2284,7 +2285,8 @@ double bbjd_block_orthogonalize(bbjd_int n, bbjd_int k, const bbjd_zomplex *Q, b
         rc |= bbjd_k_cgs2(h.s, n, (int)k, dq, n, dv, &h.rd, SL_T0, h.rd.hsum);
         rc |= !HIPOK(hipMemcpyAsync(v, dv, vb, hipMemcpyDeviceToHost, h.s));
         rc |= !HIPOK(hipMemcpyAsync(h.hscal, h.rd.scal, SL_COUNT * 8, hipMemcpyDeviceToHost, h.s));
-        if (h_work && k > 0) rc |= !HIPOK(hipMemcpyAsync(h_work, h.rd.hsum, (size_t)k * 16, hipMemcpyDeviceToHost, h.s));
+        if (h_work && k > 0) rc |= !HIPOK(hipMemcpyAsync(h_work, h.rd.hsum, (size_t)k * 16, hipMemcpyDeviceToHost,
+            h.s));
         rc |= !HIPOK(hipStreamSynchronize(h.s));
         if (!rc) res = h.hscal[SL_T0];
     }

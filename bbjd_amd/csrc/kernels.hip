@@ -55,8 +55,10 @@ hipEvent_t kt_event()
 template <typename R> struct alignas(2 * sizeof(R)) Cx { R re, im; };
 
 template <typename R> __device__ __forceinline__ Cx<R> cx(R a, R b) { Cx<R> c; c.re = a; c.im = b; return c; }
-template <typename R> __device__ __forceinline__ Cx<R> operator+(Cx<R> a, Cx<R> b) { return cx(a.re + b.re, a.im + b.im); }
-template <typename R> __device__ __forceinline__ Cx<R> operator-(Cx<R> a, Cx<R> b) { return cx(a.re - b.re, a.im - b.im); }
+template <typename R> __device__ __forceinline__ Cx<R> operator+(Cx<R> a, Cx<R> b) { return cx(a.re + b.re,
+                                                                 a.im + b.im); }
+template <typename R> __device__ __forceinline__ Cx<R> operator-(Cx<R> a, Cx<R> b) { return cx(a.re - b.re,
+                                                                 a.im - b.im); }
 template <typename R> __device__ __forceinline__ Cx<R> operator*(Cx<R> a, Cx<R> b)
 { return cx(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re); }
 // Accumulations with an explicit fused multiply-add of one product and the
@@ -358,11 +360,13 @@ __global__ __launch_bounds__(TPB) void k_proj(i64 n, i64 chunk, int k, const Cx<
                                               int *cnt, Cx<R> *out0, Cx<R> *out1, Cx<R> *out2, double *scal,
                                               int norm_slot, int fused)
 {
-    k_proj_b<R, NV, C, COND>(VB_HW, n, chunk, k, Q, ldq, x0, x1, x2, ca, cb, part, ldp, pnorm, want_norm, cond, cnt, out0, out1, out2, scal, norm_slot, fused);
+    k_proj_b<R, NV, C, COND>(VB_HW, n, chunk, k, Q, ldq, x0, x1, x2, ca, cb, part, ldp, pnorm, want_norm, cond, cnt,
+                             out0, out1, out2, scal, norm_slot, fused);
 }
 
 template <typename R, int NV, int C>
-__device__ __forceinline__ void k_proj_finish_b(const VB vb, int ntiles, int k, const Cx<R> *part, int ldp, const R *pnorm,
+__device__ __forceinline__ void k_proj_finish_b(const VB vb, int ntiles, int k, const Cx<R> *part, int ldp,
+                                                const R *pnorm,
                                                      int want_norm, const int *__restrict__ cond, Cx<R> *out0,
                                                      Cx<R> *out1, Cx<R> *out2, double *scal, int norm_slot)
 {
@@ -907,7 +911,8 @@ __global__ __launch_bounds__(TPB) void k_gemv_n_karg(i64 n, int k, const Cx<doub
 
 // separate finish of a norm (one workgroup)
 template <typename R>
-__device__ __forceinline__ void k_norm_fin_b(const VB vb, const R *pnorm, int nblk, NormFin fin, const int *__restrict__ cond)
+__device__ __forceinline__ void k_norm_fin_b(const VB vb, const R *pnorm, int nblk, NormFin fin,
+                                             const int *__restrict__ cond)
 {
     if (cond && cond[0] == 0) return;
     __shared__ R lds4[4];
@@ -921,7 +926,8 @@ __global__ __launch_bounds__(TPB) void k_norm_fin(const R *pnorm, int nblk, Norm
 
 // ||x|| -> scal[slot] in one launch.
 template <typename R, int E>
-__device__ __forceinline__ void k_norm_b(const VB vb, i64 n, const Cx<R> *__restrict__ x, R *pnorm, int *cnt, NormFin fin,
+__device__ __forceinline__ void k_norm_b(const VB vb, i64 n, const Cx<R> *__restrict__ x, R *pnorm, int *cnt,
+                                         NormFin fin,
                                               int fused)
 {
     __shared__ R lds4[4];
@@ -1577,19 +1583,22 @@ static int proj_pass_nv(hipStream_t s, i64 n, int k, const void *Q, i64 ldq, con
         g_kt.push_back({ea, eb, bytes});
     } else if (cond)
         bbjd::gl<&k_proj_b<R, NV, C, true>, TPB>(k_proj<R, NV, C, true>, g, 0, s, (i64)n, chunk, k, (const Cx<R> *)Q,
-                           (i64)ldq, (const Cx<R> *)x0, (const Cx<R> *)x1, (const Cx<R> *)x2, ca, cb, (Cx<R> *)rd->part, ldp,
+                           (i64)ldq, (const Cx<R> *)x0, (const Cx<R> *)x1, (const Cx<R> *)x2, ca, cb,
+                            (Cx<R> *)rd->part, ldp,
                            (R *)rd->pnorm, want_norm, cond, rd->cnt, (Cx<R> *)out0, (Cx<R> *)out1, (Cx<R> *)out2,
                            rd->scal, norm_slot, fused);
     else
         bbjd::gl<&k_proj_b<R, NV, C, false>, TPB>(k_proj<R, NV, C, false>, g, 0, s, (i64)n, chunk, k, (const Cx<R> *)Q,
-                           (i64)ldq, (const Cx<R> *)x0, (const Cx<R> *)x1, (const Cx<R> *)x2, ca, cb, (Cx<R> *)rd->part, ldp,
+                           (i64)ldq, (const Cx<R> *)x0, (const Cx<R> *)x1, (const Cx<R> *)x2, ca, cb,
+                            (Cx<R> *)rd->part, ldp,
                            (R *)rd->pnorm, want_norm, cond, rd->cnt, (Cx<R> *)out0, (Cx<R> *)out1, (Cx<R> *)out2,
                            rd->scal, norm_slot, fused);
     // finish_left: a caller that reduces the partials itself gets the tile
     // count instead of a finish launch (0 when the last block finished)
     if (finish_left) *finish_left = fused ? 0 : nt;
     if (!fused && !finish_left)
-        bbjd::gl<&k_proj_finish_b<R, NV, C>, TPB>(k_proj_finish<R, NV, C>, dim3(g.x), 0, s, nt, k, (const Cx<R> *)rd->part,
+        bbjd::gl<&k_proj_finish_b<R, NV, C>, TPB>(k_proj_finish<R, NV, C>, dim3(g.x), 0, s, nt, k,
+                                                  (const Cx<R> *)rd->part,
                            ldp, (const R *)rd->pnorm, want_norm, cond, (Cx<R> *)out0, (Cx<R> *)out1,
                            (Cx<R> *)out2, rd->scal, norm_slot);
     return chk();
@@ -1615,7 +1624,8 @@ static int update_pass(hipStream_t s, i64 n, int k, const void *Q, i64 ldq, cons
     const int fused = nt <= g_fuse_max_blocks;
     if (rows < TPB) {
 #define BBJD_LAUNCH_SPLIT(RW, CO)                                                                                  \
-    bbjd::gl<&k_update_split_b<R, RW, CO>, TPB>(k_update_split<R, RW, CO>, dim3(nt), lds, s, (i64)n, k, (const Cx<R> *)Q,        \
+    bbjd::gl<&k_update_split_b<R, RW, CO>, TPB>(k_update_split<R, RW, CO>, dim3(nt), lds, s, (i64)n, k,              \
+                                                (const Cx<R> *)Q,                                                    \
                        (i64)ldq, (const Cx<R> *)h, alpha, beta_one, (Cx<R> *)x, (R *)rd->pnorm, cond,                \
                        rd->cnt + BBJD_CNT_UPD, fin, fused)
         if (rows == 128) {
@@ -1711,7 +1721,8 @@ static int cgs_mid(hipStream_t s, i64 n, int k, const void *Q, i64 ldq, void *v,
     const int fused = nb <= g_fuse_max_blocks;
     int *cnt = rd->cnt + BBJD_CNT_MID;
 #define BBJD_LAUNCH_MID(KW)                                                                                         \
-    bbjd::gl<&k_cgs_mid_b<R, KW>, TPB>(k_cgs_mid<R, KW>, dim3((unsigned)nb), 0, s, (i64)n, k, (const Cx<R> *)Q,         \
+    bbjd::gl<&k_cgs_mid_b<R, KW>, TPB>(k_cgs_mid<R, KW>, dim3((unsigned)nb), 0, s, (i64)n, k,                        \
+                                       (const Cx<R> *)Q,                                                             \
                        (i64)ldq, (const Cx<R> *)f.h1, (Cx<R> *)v, (const Cx<R> *)xa, (const Cx<R> *)xb, ca, cb,      \
                        (int)per, (Cx<R> *)rd->part, (R *)rd->pnorm, cnt, f, fused)
     switch (cgs_kw(k)) {
@@ -2375,7 +2386,8 @@ int bbjd_k_copy(hipStream_t s, void *dst, const void *src, size_t bytes)
 __global__ void k_fill(i64 cnt, Cx<double> *a, double scale)
 {
     for (i64 i = (i64)blockIdx.x * TPB + threadIdx.x; i < cnt; i += (i64)gridDim.x * TPB)
-        a[i] = cx(scale * (double)((i * 2654435761LL) % 1000 - 500) * 1e-3, scale * (double)((i * 40503LL) % 997 - 498) * 1e-3);
+        a[i] = cx(scale * (double)((i * 2654435761LL) % 1000 - 500) * 1e-3,
+                  scale * (double)((i * 40503LL) % 997 - 498) * 1e-3);
 }
 
 // Plain streaming copy: the reference point for counter-based HBM traffic.

@@ -68,7 +68,8 @@ __global__ __launch_bounds__(TPB) void k_band_mv(i64 n1, i64 nb, int bs, int hw,
 }
 
 template <typename R>
-__device__ __forceinline__ void k_diag_mv_b(const VB vb, i64 n, const Cx<R> *__restrict__ dg, const Cx<R> *__restrict__ x,
+__device__ __forceinline__ void k_diag_mv_b(const VB vb, i64 n, const Cx<R> *__restrict__ dg,
+                                            const Cx<R> *__restrict__ x,
                                                  Cx<R> *__restrict__ y)
 {
     for (i64 i = (i64)vb.bx * TPB + threadIdx.x; i < n; i += (i64)vb.gx * TPB) {
@@ -665,7 +666,8 @@ __device__ __forceinline__ const Cx<double> *bcr_level_ptr(const BcrLevels &f, i
 }
 
 template <int NR, int S, int G>
-__device__ __forceinline__ void k_bcr_bwd_multi_pf_b(const VB vb, i64 N, int l, BcrLevels dpq, BcrLevels bl, Cx<double> *x,
+__device__ __forceinline__ void k_bcr_bwd_multi_pf_b(const VB vb, i64 N, int l, BcrLevels dpq, BcrLevels bl,
+                                                     Cx<double> *x,
                                                           i64 vs, const Cx<double> *__restrict__ tauc, int ntau,
                                                           i64 n1, i64 ldt, Cx<double> *__restrict__ dpart)
 {
@@ -1267,7 +1269,8 @@ __global__ __launch_bounds__(256) void k_bcr_tail_pf(i64 N, int l0, int L, const
 // b2f / ytf (one right-hand side): the dense rows' right-hand side read and
 // the tail of y written in complex64 instead of b2_0 / ytail0.
 template <int NR>
-__device__ __forceinline__ void k_border_small_b(const VB vb, int ntau, const Cx<double> *b2_0, const Cx<double> *b2_1, const Cx<double> *dots0,
+__device__ __forceinline__ void k_border_small_b(const VB vb, int ntau, const Cx<double> *b2_0,
+                                                 const Cx<double> *b2_1, const Cx<double> *dots0,
                                const Cx<double> *dots1, const Cx<double> *__restrict__ Sinv, Cx<double> *y2out,
                                Cx<double> *ytail0, Cx<double> *ytail1, const Cx<double> *__restrict__ dpart,
                                i64 nblk, const Cx<float> *b2f, Cx<float> *ytf)
@@ -1320,7 +1323,8 @@ __global__ void k_border_small(int ntau, const Cx<double> *b2_0, const Cx<double
 // y_q[r] = xb_q[r] - sum_a Y[a n1 + r] y2_q[a], r < n1; with y0f (one
 // right-hand side) y_0 is written in complex64 there instead.
 template <int NR>
-__device__ __forceinline__ void k_border_update_b(const VB vb, i64 n1, int ntau, const Cx<double> *__restrict__ xb, i64 vs,
+__device__ __forceinline__ void k_border_update_b(const VB vb, i64 n1, int ntau, const Cx<double> *__restrict__ xb,
+                                                  i64 vs,
                                                        const Cx<double> *__restrict__ Y,
                                                        const Cx<double> *__restrict__ y2, Cx<double> *y0,
                                                        Cx<double> *y1, Cx<float> *y0f)
@@ -1545,7 +1549,8 @@ int bbjd_o_bcr_bwd_multi(hipStream_t st, int s, int64_t N, int l, int G, const v
         const Cx<double> *tc = dd ? (const Cx<double> *)tauc : nullptr;                                        \
         if constexpr (S <= 8) {                                                                                \
             if (g_bcr_prefetch) {                                                                              \
-                bbjd::gl<&k_bcr_bwd_multi_pf_b<NR, S, GG>, TPB>(k_bcr_bwd_multi_pf<NR, S, GG>, dim3((unsigned)nblk), 0, st,    \
+                bbjd::gl<&k_bcr_bwd_multi_pf_b<NR, S, GG>, TPB>(k_bcr_bwd_multi_pf<NR, S, GG>,                       \
+                                                                dim3((unsigned)nblk), 0, st,                         \
                                    (i64)N, l, f, bb, (Cx<double> *)x, (i64)vs, tc, ntau, (i64)n1, (i64)ldt,    \
                                    (Cx<double> *)dpart);                                                       \
                 if (dd && nblk_out) *nblk_out = nblk;                                                          \
@@ -1582,7 +1587,8 @@ int bbjd_o_bcr_fwd_multi(hipStream_t st, int s, int64_t N, int64_t n1, int l, in
             if (g_bcr_prefetch) {                                                                              \
                 const Cx<float> *f0 = in_c64 ? (const Cx<float> *)x0 : nullptr;                                \
                 if (in_c64) a0 = a1 = nullptr;                                                                 \
-                bbjd::gl<&k_bcr_fwd_multi_pf_b<NR, S, GG>, TPB>(k_bcr_fwd_multi_pf<NR, S, GG>, dim3(nb), 0, st, (i64)N, l,     \
+                bbjd::gl<&k_bcr_fwd_multi_pf_b<NR, S, GG>, TPB>(k_bcr_fwd_multi_pf<NR, S, GG>, dim3(nb), 0, st,      \
+                                                                (i64)N, l,                                           \
                                    (i64)n1, f, a0, a1, (const Cx<double> *)bin, (Cx<double> *)bout, (i64)vs,   \
                                    f0);                                                                        \
                 break;                                                                                         \
@@ -1627,7 +1633,8 @@ int bbjd_o_bcr_tail(hipStream_t st, int s, int64_t N, int l0, int L, const int64
         for (int l = l0; l < L; l++) rows += (N + ((i64)1 << l) - 1) >> l;
         if (g_bcr_prefetch && rows * s <= 256) {
 #define BBJD_TAIL_PF(NR, SS)                                                                                   \
-    bbjd::gl<&k_bcr_tail_pf_b<NR, SS>, 256>(k_bcr_tail_pf<NR, SS>, dim3(1), lds, st, (i64)N, l0, L, (const i64 *)off,   \
+    bbjd::gl<&k_bcr_tail_pf_b<NR, SS>, 256>(k_bcr_tail_pf<NR, SS>, dim3(1), lds, st, (i64)N, l0, L,                  \
+                                            (const i64 *)off,                                                        \
                        (const Cx<double> *)F, (const Cx<double> *)b, (Cx<double> *)x, (i64)vs)
             if (nrhs == 2) {
                 if (s == 4) BBJD_TAIL_PF(2, 4);
