@@ -1303,10 +1303,13 @@ static int batch_phase_end(struct bthread *T, int nj, int copy)
     return rc;
 }
 
-static void batch_run(struct bthread *T, struct gm_job **jobs, int nj)
+/* runs a batch; returns the Arnoldi steps consumed (counted under the
+ * batcher's lock by the caller) */
+static long batch_run(struct bthread *T, struct gm_job **jobs, int nj)
 {
     bbjd_batcher *B = T->B;
     int err = 0;
+    long steps = 0;
     for (int i = 0; i < nj; i++) {
         struct gm_job *J = jobs[i];
         ctx_t *c = J->c;
@@ -1378,7 +1381,7 @@ static void batch_run(struct bthread *T, struct gm_job **jobs, int nj)
             const char *hb = T->hstage + ((size_t)(j % L) * BATCH_JOBS_MAX + i) * B->slot;
             const double *hs = (const double *)(hb + B->stage_sc);
             const void *hh = hb + B->stage_sc - (size_t)(j + 1) * 8;
-            B->st.steps++;
+            steps++;
             int stop = gm_column(J->c, &J->G, j, J->ndefl, hh, hs);
             if (stop || j + 1 == J->ims) {
                 J->m = j + 1;
@@ -1407,6 +1410,7 @@ static void batch_run(struct bthread *T, struct gm_job **jobs, int nj)
         J->c->s = J->own;
         J->c->stage = NULL;
     }
+    return steps;
 }
 
 static void *batcher_main(void *arg)
@@ -1483,8 +1487,9 @@ static void *batcher_main(void *arg)
         B->queued -= nj;
         B->running += nj;
         pthread_mutex_unlock(&B->mu);
-        batch_run(T, jobs, nj);
+        const long steps = batch_run(T, jobs, nj);
         pthread_mutex_lock(&B->mu);
+        B->st.steps += steps;
         for (int i = 0; i < nj; i++) {
             jobs[i]->done = 1;
             for (int b = 0; b < B->nbusy; b++)

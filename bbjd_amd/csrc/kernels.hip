@@ -2304,8 +2304,8 @@ void bbjd_rec_end(void) { bbjd::tl_rec() = nullptr; }
 void bbjd_rec_counts(const void *rec, long *recorded, long *launches)
 {
     const bbjd::Recorder *r = static_cast<const bbjd::Recorder *>(rec);
-    *recorded = r->recorded;
-    *launches = r->launches;
+    *recorded = r->recorded.load(std::memory_order_relaxed);
+    *launches = r->launches.load(std::memory_order_relaxed);
 }
 
 int bbjd_rec_flush(void *rec, hipStream_t s, long *launches)
@@ -2329,7 +2329,7 @@ int bbjd_rec_flush(void *rec, hipStream_t s, long *launches)
         });
     for (auto &l : r->jobs) l.clear();
     r->arena.clear();
-    r->launches += nl;
+    r->launches.fetch_add(nl, std::memory_order_relaxed);
     if (launches) *launches = nl;
     if (rc || hipGetLastError() != hipSuccess) return -1;
     return 0;

@@ -21,6 +21,7 @@
 #define BBJD_LAUNCH_H
 
 #include <hip/hip_runtime.h>
+#include <atomic>
 #include <functional>
 #include <new>
 #include <type_traits>
@@ -76,8 +77,9 @@ struct Recorder {
     std::vector<std::vector<Op>> jobs;
     std::vector<unsigned char> arena;   // argument tuples of the recorded launches
     int cur = -1;
-    long launches = 0;             // launches made by flushes so far
-    long recorded = 0;             // launches recorded so far
+    // counters read by other threads (bbjd_rec_counts) while this one runs
+    std::atomic<long> launches{0};   // launches made by flushes so far
+    std::atomic<long> recorded{0};   // launches recorded so far
 
     // room for an object of the given size and alignment (<= 16); arena
     // offsets stay valid when it grows
@@ -141,7 +143,7 @@ inline void gl(void (*kern)(P...), dim3 grid, size_t lds, hipStream_t s, A... a)
     o.args = r->alloc(sizeof(T), alignof(T));
     new (r->arena.data() + o.args) T(static_cast<P>(a)...);
     r->jobs[r->cur].push_back(std::move(o));
-    r->recorded++;
+    r->recorded.fetch_add(1, std::memory_order_relaxed);
 }
 
 // any other launch or copy: run now, or recorded as a closure
@@ -152,7 +154,7 @@ template <typename F> inline void sl(hipStream_t s, F &&f)
     Op o;
     o.solo = std::function<void(hipStream_t)>(std::forward<F>(f));
     r->jobs[r->cur].push_back(std::move(o));
-    r->recorded++;
+    r->recorded.fetch_add(1, std::memory_order_relaxed);
 }
 
 }  // namespace bbjd
