@@ -2495,6 +2495,77 @@ int bbjd_kbench(int which, int64_t n, int k, int reps, double *avg_ms, double *b
     return rc ? -1 : 0;
 }
 
+// The inner solve's projection Q^H x (complex64, k columns: k_proj plus
+// its finish) of `jobs` independent solves, each with its own Q, x and
+// reduction scratch, recorded and flushed as grouped launches (jobs > 1) or
+// launched one after another (jobs = 1 with the same harness), reps times.
+// *avg_ms: wall time per rep on the stream (host recording included);
+// *bytes: algorithmic bytes per rep, jobs x 8 n (k + 1).  The kernel trace of
+// a run gives the grouped kernel's own duration.
+int bbjd_kbench_grouped(int jobs, int64_t n, int k, int reps, double *avg_ms, double *bytes)
+{
+    if (jobs < 1 || jobs > bbjd::kGroupMax || n <= 0 || k <= 0 || k > 128 || reps <= 0) return -2;
+    hipStream_t s;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return -1;
+    const i64 ld = (n + 63) / 64 * 64;
+    const i64 pe = bbjd_red_part_entries(n, k + 2);
+    const size_t npn = (size_t)ntiles(n, BBJD_TILE_MIN_ROWS) + 64;
+    const size_t per_q = (size_t)ld * k * 8, per_x = (size_t)ld * 8;
+    const size_t per_blk = (size_t)pe * 8 + npn * 8 + (size_t)(k + 8) * 8 * 5 + 8192;
+    void *mem = nullptr;
+    if (hipMalloc(&mem, (per_q + per_x + per_blk) * jobs) != hipSuccess) { (void)hipStreamDestroy(s); return -1; }
+    std::vector<bbjd_red> rd(jobs);
+    std::vector<char *> Q(jobs), x(jobs);
+    char *b = (char *)mem;
+    for (int j = 0; j < jobs; j++) {
+        Q[j] = b; b += per_q;
+        x[j] = b; b += per_x;
+        bbjd_red &r = rd[j];
+        r.part = b; b += (size_t)pe * 8;
+        r.pnorm = (double *)b; b += npn * 8;
+        r.h = b; b += (size_t)(k + 8) * 8;
+        r.h2 = b; b += (size_t)(k + 8) * 8;
+        r.hsum = b; b += (size_t)(k + 8) * 8;
+        r.coef = b; b += (size_t)(k + 8) * 8;
+        r.coef2 = b; b += (size_t)(k + 8) * 8;
+        r.scal = (double *)b; r.flag = (int *)(b + 256); r.cnt = (int *)(b + 512);
+        b += 8192;
+        r.part_cap = pe;
+        hipLaunchKernelGGL(k_fill, dim3(1024), dim3(TPB), 0, s, (i64)ld * k / 2, (Cx<double> *)Q[j], 0.01 * (j + 1));
+        hipLaunchKernelGGL(k_fill, dim3(256), dim3(TPB), 0, s, (i64)ld / 2, (Cx<double> *)x[j], 0.7);
+        (void)hipMemsetAsync(r.flag, 0, 256 + BBJD_CNT_SLOTS * 4, s);
+    }
+    void *rec = jobs > 1 ? bbjd_rec_create() : nullptr;
+    auto launch = [&]() {
+        for (int j = 0; j < jobs; j++) {
+            if (rec) bbjd_rec_begin(rec, j);
+            (void)proj_pass<float>(s, n, k, Q[j], ld, x[j], nullptr, rd[j].h2, nullptr, &rd[j], -1, nullptr);
+            if (rec) bbjd_rec_end();
+        }
+        if (rec) (void)bbjd_rec_flush(rec, s, nullptr);
+    };
+    launch();
+    launch();
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    (void)hipEventRecord(e0, s);
+    for (int r = 0; r < reps; r++) launch();
+    (void)hipEventRecord(e1, s);
+    float ms = 0;
+    int rc = hipEventSynchronize(e1) != hipSuccess;
+    rc |= hipEventElapsedTime(&ms, e0, e1) != hipSuccess;
+    rc |= hipGetLastError() != hipSuccess;
+    if (avg_ms) *avg_ms = ms / reps;
+    if (bytes) *bytes = 8.0 * (double)n * (k + 1.0) * jobs;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rec) bbjd_rec_destroy(rec);
+    (void)hipFree(mem);
+    (void)hipStreamDestroy(s);
+    return rc ? -1 : 0;
+}
+
 }  // extern "C"
 
 // ---------------------------------------------------------------------------

@@ -18,6 +18,12 @@ problems on one thread (bbjd_qz_target: eigenvalues and target, then
 bbjd_ritz_vector); prints us per launch, device problems/s and host ms per
 problem.
 
+gproj: the inner solve's complex64 projection Q^H x (k_proj plus its
+finish) of --jobs independent solves (each its own Q, x and scratch),
+recorded and flushed as grouped launches (launch.h) when --jobs > 1;
+prints us per rep (host recording included), bytes per rep and GB/s.  Its
+kernel trace gives the grouped kernel's own duration.
+
 bcr: the exact preconditioner's solve (block cyclic reduction with the dense
 rows' Schur border) on the serving workload's operator shape (block size 8,
 block-tridiagonal, 4 dense rows, sigma 0.5+0.5i), N rows, K = 1 or 2
@@ -43,6 +49,19 @@ def run(kernel, n, k, reps=50, proj_blocks=0, upd_rows=0, blocks_target=0):
     rc = lib.bbjd_kbench(KERNELS[kernel], int(n), int(k), int(reps), C.byref(ms), C.byref(by))
     if rc:
         raise RuntimeError(f"bbjd_kbench({kernel}, {n}, {k}) failed: {rc}")
+    return ms.value, by.value
+
+
+def run_grouped(n, k, jobs, reps=50):
+    """average ms per rep and bytes per rep of the grouped projection"""
+    lib = _lib.load()
+    lib.bbjd_kbench_grouped.restype = C.c_int
+    lib.bbjd_kbench_grouped.argtypes = [C.c_int, C.c_int64, C.c_int, C.c_int, C.POINTER(C.c_double),
+                                        C.POINTER(C.c_double)]
+    ms, by = C.c_double(), C.c_double()
+    rc = lib.bbjd_kbench_grouped(int(jobs), int(n), int(k), int(reps), C.byref(ms), C.byref(by))
+    if rc:
+        raise RuntimeError(f"bbjd_kbench_grouped({jobs}, {n}, {k}) failed: {rc}")
     return ms.value, by.value
 
 
@@ -148,7 +167,7 @@ def run_qz(nprob, k, reps=20):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("kernel", choices=sorted(KERNELS) + ["bcr", "qz"])
+    ap.add_argument("kernel", choices=sorted(KERNELS) + ["bcr", "qz", "gproj"])
     ap.add_argument("n", type=int)
     ap.add_argument("k", type=int)
     ap.add_argument("--reps", type=int, default=50)
@@ -157,6 +176,7 @@ def main():
     ap.add_argument("--rotate-mode", type=int, default=1,
                     help="rotate: 1 streaming matrix-core form, 2 staged matrix-core form, 0 vector FP64")
     ap.add_argument("--rotate-chunk", type=int, default=0, help="rotate: input columns per LDS chunk (8, 16, 32)")
+    ap.add_argument("--jobs", type=int, default=16, help="gproj: independent solves per grouped launch (1..16)")
     a = ap.parse_args()
     _lib.load().bbjd_set_rotate_mfma(a.rotate_mode)
     _lib.load().bbjd_set_rotate_chunk(a.rotate_chunk)
@@ -165,6 +185,11 @@ def main():
         print(f"qz problems={a.n} k={a.k}: device {dev_us:.1f} us/launch ({a.n / (dev_us * 1e-6):.0f} problems/s, "
               f"status {st}); host step {host_ms:.3f} ms/problem on one thread "
               f"({1e3 / host_ms:.0f} problems/s)", flush=True)
+        return
+    if a.kernel == "gproj":
+        ms, by = run_grouped(a.n, a.k, a.jobs, a.reps)
+        print(f"gproj n={a.n} k={a.k} jobs={a.jobs}: {ms * 1e3:.2f} us/rep, {by / 1e6:.3f} MB/rep, "
+              f"{by / (ms * 1e-3) / 1e9:.1f} GB/s", flush=True)
         return
     if a.kernel == "bcr":
         ms = run_bcr(a.n, a.k, a.reps)
